@@ -1,0 +1,140 @@
+/* ngbem_amd — C-ABI of the MI355X operator assembly + H-matrix apply library.
+ *
+ * One library, libngbem_amd.so (ngbem_amd/_lib/), built with hipcc for gfx950.
+ * Plain C types only; every call returns 0 on success or a negative status, and
+ * bem_last_error() holds the message of the last failure on the calling thread.
+ * No C++ exception crosses this boundary.
+ *
+ * The reference (Weggler/ngbem) is C++ behind pybind11; these entry points
+ * replace, one for one, what its Python factories and NGSolve's BaseMatrix
+ * interface reach into:
+ *   bem_op_create + bem_op_assemble  <- the ten factories, src/python_bem.cpp:27-173,
+ *        which run GenericIntegralOperator's constructor (src/ngbem.cpp:314-330):
+ *        ClusterTree (src/hmat.cpp:409-453), HMatrix partition (src/hmat.cpp:614-621)
+ *        and CalcHMatrix (src/ngbem.cpp:118-157: CalcBlockMatrix :373-738,
+ *        CalcFarFieldBlock :743-1163, dense fallback :143-153)
+ *   bem_op_apply  <- HMatrix::Mult/MultAdd/MultTrans/MultTransAdd (src/hmat.cpp:632-666),
+ *        BEMBlock::MultAdd/MultTransAdd (src/hmat.cpp:477-500), LowRankMatrix
+ *        (src/hmat.cpp:527-565); the adjoint flag has no reference counterpart
+ *        (the reference transpose is plain, src/hmat.cpp:547-548)
+ *   bem_op_stats  <- HMatrix::NZE (src/hmat.cpp:623-630), VHeight/VWidth/IsComplex
+ *        (src/hmat.hpp:186-197)
+ *   bem_op_block_* / bem_op_tree  <- BEMBlock accessors (src/hmat.hpp:77-107) and
+ *        ClusterTree::ClusterDofs (src/hmat.hpp:62-66), for parity checks
+ */
+#ifndef NGBEM_AMD_H
+#define NGBEM_AMD_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct bem_mesh bem_mesh;
+typedef struct bem_space bem_space;
+typedef struct bem_op bem_op;
+
+/* space kinds (src/python_bem.cpp callers pass NGSolve spaces of these types) */
+enum {
+  BEM_SPACE_P0 = 0, /* SurfaceL2, order 0: one dof per triangle, shape 1 */
+  BEM_SPACE_P1 = 1  /* H1, order 1: one dof per vertex, barycentric hats */
+};
+
+/* kernel ids (src/ngbem.hpp:230-431); evaluators follow the factories:
+ * LAPLACE_HS uses n x grad (DiffOpBoundaryRot), HELMHOLTZ_HS uses
+ * [n x grad, u] (DiffOpHelmholtz), all others the plain trace. */
+enum {
+  BEM_LAPLACE_SL = 0,
+  BEM_LAPLACE_DL = 1,
+  BEM_LAPLACE_HS = 2,
+  BEM_HELMHOLTZ_SL = 3,
+  BEM_HELMHOLTZ_DL = 4,
+  BEM_HELMHOLTZ_CF = 5,
+  BEM_HELMHOLTZ_HS = 6
+};
+
+/* apply flags */
+enum {
+  BEM_APPLY_TRANS = 1,     /* y = alpha A^T x + beta y (plain transpose, MultTransAdd) */
+  BEM_APPLY_CONJ = 2,      /* with TRANS: A^H (adjoint) */
+  BEM_APPLY_MIXED = 4,     /* use the FP32 / complex64 copy of the operator */
+  BEM_APPLY_ON_DEVICE = 8  /* x and y are device pointers (else host pointers) */
+};
+
+/* vector dtypes */
+enum { BEM_F64 = 0, BEM_C128 = 1 };
+
+typedef struct {
+  int intorder;  /* quadrature order (BEMParameters.intorder) */
+  int leafsize;  /* cluster leaf size */
+  double eta;    /* admissibility parameter */
+  double eps;    /* ACA tolerance */
+  int dense;     /* 1: method="dense" (leafsize=INT_MAX, eta=0 as src/python_bem.cpp:30-33) */
+  int mixed;     /* 1: also build the FP32/complex64 copy for BEM_APPLY_MIXED */
+} bem_params;
+
+typedef struct {
+  int64_t nze;            /* sum m*n (dense) + R*(m+n) (low rank), as HMatrix::NZE */
+  int64_t near_entries;   /* stored dense entries */
+  int64_t far_factor_entries;
+  int32_t n_blocks, n_near, n_far, n_fallback;
+  int32_t max_rank;
+  int32_t height, width;  /* VHeight (test ndof), VWidth (trial ndof) */
+  int32_t is_complex;
+  int32_t n_singular_pairs[3]; /* identical, common edge, common vertex */
+  double t_setup, t_near, t_far, t_total; /* seconds (host clock, device synchronised) */
+  double near_kernel_evals, far_kernel_evals;
+} bem_stats;
+
+typedef struct {
+  int32_t near;      /* 1 dense, 0 low rank */
+  int32_t fallback;  /* admissible block stored dense (shared vertices) */
+  int32_t row_cluster, col_cluster;
+  int32_t row0, m, col0, n; /* ranges in cluster order */
+  int32_t rank;      /* low rank: R; dense: -1 */
+} bem_block_info;
+
+const char* bem_last_error(void);
+int bem_device_count(void);
+
+int bem_mesh_create(const double* xyz, int nv, const int32_t* tri, int nt, bem_mesh** out);
+int bem_mesh_destroy(bem_mesh* m);
+
+int bem_space_create(bem_mesh* m, int kind, int order, bem_space** out);
+int bem_space_ndof(const bem_space* s);
+int bem_space_destroy(bem_space* s);
+
+/* tree + partition + device setup; no quadrature yet */
+int bem_op_create(int kernel, double kappa, bem_space* trial, bem_space* test, const bem_params* p, bem_op** out);
+/* near-field quadrature + ACA on the device; stream is a hipStream_t (NULL = default) */
+int bem_op_assemble(bem_op* op, void* stream);
+/* y = alpha op(A) x + beta y for nrhs right-hand sides (columns of length ld).
+ * alpha/beta point to 2 doubles (re, im); dtype BEM_F64 only for real operators. */
+int bem_op_apply(bem_op* op, int flags, const double* alpha, const void* x, const double* beta, void* y,
+                 int nrhs, int ld, int dtype, void* stream);
+int bem_op_stats(const bem_op* op, bem_stats* st);
+/* measurement: run `iters` applies on device buffers of ones and return the
+ * average milliseconds of [gather, low-rank B^T x, row/column kernel, scatter,
+ * whole apply] from HIP events recorded on `stream` between the launches.
+ * No reference counterpart (the reference times with NGSolve Timers,
+ * src/hmat.cpp:650-651). */
+int bem_op_profile_apply(bem_op* op, int flags, int iters, double* ms, void* stream);
+int bem_op_destroy(bem_op* op);
+
+/* introspection for parity tests */
+int bem_op_num_blocks(const bem_op* op);
+int bem_op_block_info(const bem_op* op, int b, bem_block_info* info);
+/* global dofs of block b (rows: test, cols: trial) */
+int bem_op_block_dofs(const bem_op* op, int b, int32_t* rows, int32_t* cols);
+/* copy block b to host: dense -> m*n row-major; low rank -> U (m*R) then V (R*n).
+ * Values are double (real operators) or interleaved complex. */
+int bem_op_block_data(const bem_op* op, int b, void* out);
+/* cluster tree: which 0 trial, 1 test; perm[ndof]; cl[7*ncl] = level,parent,child1,child2,
+ * number,permupos,radius; cen[3*ncl].  Returns the cluster count. */
+int bem_op_tree(const bem_op* op, int which, int32_t* perm, double* cl, double* cen, int cap);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,25 @@
+"""ngbem_amd: MI355X-native boundary element operator assembly and H-matrix apply.
+
+Drop-in for the operator hot path of Weggler/ngbem: the same factory names and
+keyword arguments (src/python_bem.cpp:27-173), an H-matrix ``.mat`` with the
+BaseMatrix calls NGSolve's solvers use, and Krylov solvers with
+ngsolve.krylovspace's argument names.  Assembly (Sauter-Schwab near field, ACA
+far field) and apply run in hand-written HIP kernels for gfx950 behind the
+C-ABI in include/ngbem_amd.h.
+"""
+from .mesh import Mesh, icosphere, ellipsoid
+from .spaces import SurfaceL2, H1
+from .operators import (BaseMatrix, HMatrix, IntegralOperator, IntegralOperatorC, MassMatrix,
+                        SingleLayerPotentialOperator, DoubleLayerPotentialOperator, HypersingularOperator,
+                        HelmholtzSingleLayerPotentialOperator, HelmholtzDoubleLayerPotentialOperator,
+                        HelmholtzCombinedFieldOperator, HelmholtzHypersingularOperator,
+                        MaxwellSingleLayerPotentialOperator, MaxwellSingleLayerPotentialOperatorCurl,
+                        MaxwellDoubleLayerPotentialOperator)
+from .krylov import CG, GMRes, RefinedSolve
+
+__all__ = ["Mesh", "icosphere", "ellipsoid", "SurfaceL2", "H1", "BaseMatrix", "HMatrix", "IntegralOperator",
+           "IntegralOperatorC", "MassMatrix", "SingleLayerPotentialOperator", "DoubleLayerPotentialOperator",
+           "HypersingularOperator", "HelmholtzSingleLayerPotentialOperator", "HelmholtzDoubleLayerPotentialOperator",
+           "HelmholtzCombinedFieldOperator", "HelmholtzHypersingularOperator", "MaxwellSingleLayerPotentialOperator",
+           "MaxwellSingleLayerPotentialOperatorCurl", "MaxwellDoubleLayerPotentialOperator", "CG", "GMRes",
+           "RefinedSolve"]

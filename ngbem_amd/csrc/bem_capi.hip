@@ -1,0 +1,367 @@
+// C-ABI of the MI355X BEM library (declared in include/ngbem_amd.h).
+// Status codes: 0 ok, -1 invalid argument, -2 HIP/runtime failure.
+#include <hip/hip_runtime.h>
+
+#include <chrono>
+#include <climits>
+#include <cstring>
+#include <memory>
+#include <string>
+
+#include "../../include/ngbem_amd.h"
+#include "bem_op.hpp"
+
+struct bem_mesh {
+  std::shared_ptr<bem::Mesh> m;
+};
+struct bem_space {
+  std::shared_ptr<bem::Space> s;
+  std::shared_ptr<bem::Mesh> m;
+};
+struct bem_op {
+  bem::Operator op;
+};
+
+namespace {
+thread_local std::string g_err;
+
+int fail(int code, const std::string& msg) {
+  g_err = msg;
+  return code;
+}
+
+template <class F>
+int guarded(F&& f) {
+  try {
+    f();
+    return 0;
+  } catch (const bem::HipError& e) {
+    return fail(-2, e.what());
+  } catch (const std::exception& e) {
+    return fail(-1, e.what());
+  } catch (...) {
+    return fail(-2, "unknown error");
+  }
+}
+
+double now() {
+  return std::chrono::duration<double>(std::chrono::steady_clock::now().time_since_epoch()).count();
+}
+
+// admissible blocks whose patches share a vertex are stored dense, like the
+// reference's exception path (src/ngbem.cpp:839-842, 143-153)
+void mark_fallbacks(bem::Operator& op) {
+  const bem::Mesh& m = *op.mesh;
+  std::vector<int> stamp(m.nv, -1), pI, pJ;
+  for (size_t b = 0; b < op.blocks.size(); b++) {
+    bem::Block& bl = op.blocks[b];
+    if (bl.near) continue;
+    bem::patch(*op.test, &op.test_tree->perm[bl.r0], bl.m, pI);
+    bem::patch(*op.trial, &op.trial_tree.perm[bl.c0], bl.n, pJ);
+    for (int e : pI)
+      for (int k = 0; k < 3; k++) stamp[m.tri[3 * e + k]] = (int)b;
+    bool shared = false;
+    for (int e : pJ)
+      for (int k = 0; k < 3; k++)
+        if (stamp[m.tri[3 * e + k]] == (int)b) shared = true;
+    if (shared) {
+      bl.fallback = 1;
+      op.n_fallback++;
+    }
+  }
+}
+}  // namespace
+
+extern "C" {
+
+const char* bem_last_error(void) { return g_err.c_str(); }
+
+int bem_device_count(void) {
+  int n = 0;
+  if (hipGetDeviceCount(&n) != hipSuccess) return 0;
+  return n;
+}
+
+int bem_mesh_create(const double* xyz, int nv, const int32_t* tri, int nt, bem_mesh** out) {
+  if (!xyz || !tri || !out || nv <= 0 || nt <= 0) return fail(-1, "bem_mesh_create: bad arguments");
+  return guarded([&] {
+    auto m = std::make_shared<bem::Mesh>();
+    m->nv = nv;
+    m->nt = nt;
+    m->xyz.assign(xyz, xyz + 3 * (size_t)nv);
+    m->tri.assign(tri, tri + 3 * (size_t)nt);
+    for (int v : m->tri)
+      if (v < 0 || v >= nv) throw std::runtime_error("bem_mesh_create: vertex index out of range");
+    m->setup();
+    for (int e = 0; e < nt; e++)
+      if (!(m->J[e] > 0.0)) throw std::runtime_error("bem_mesh_create: degenerate triangle " + std::to_string(e));
+    *out = new bem_mesh{m};
+  });
+}
+
+int bem_mesh_destroy(bem_mesh* m) {
+  delete m;
+  return 0;
+}
+
+int bem_space_create(bem_mesh* m, int kind, int order, bem_space** out) {
+  if (!m || !out) return fail(-1, "bem_space_create: bad arguments");
+  if (!((kind == BEM_SPACE_P0 && order == 0) || (kind == BEM_SPACE_P1 && order == 1)))
+    return fail(-1, "bem_space_create: supported spaces are SurfaceL2 order 0 and H1 order 1");
+  return guarded([&] {
+    auto s = std::make_shared<bem::Space>();
+    s->setup(m->m.get(), kind == BEM_SPACE_P0 ? bem::SP_P0 : bem::SP_P1);
+    *out = new bem_space{s, m->m};
+  });
+}
+
+int bem_space_ndof(const bem_space* s) { return s ? s->s->ndof : -1; }
+
+int bem_space_destroy(bem_space* s) {
+  delete s;
+  return 0;
+}
+
+int bem_op_create(int kernel, double kappa, bem_space* trial, bem_space* test, const bem_params* p, bem_op** out) {
+  if (!trial || !out || !p) return fail(-1, "bem_op_create: bad arguments");
+  if (kernel < 0 || kernel >= bem::K_COUNT) return fail(-1, "bem_op_create: unknown kernel id");
+  if (!test) test = trial;
+  if (test->m != trial->m) return fail(-1, "bem_op_create: trial and test spaces must live on the same mesh");
+  if (p->intorder < 0 || p->intorder > 40) return fail(-1, "bem_op_create: intorder out of range [0, 40]");
+  if (p->leafsize < 1) return fail(-1, "bem_op_create: leafsize must be positive");
+  if (!(p->eps > 0.0)) return fail(-1, "bem_op_create: eps must be positive");
+  if ((kernel == bem::K_LHS || kernel == bem::K_HHS) &&
+      (trial->s->kind != bem::SP_P1 || test->s->kind != bem::SP_P1))
+    return fail(-1, "bem_op_create: hypersingular operators need H1 order 1 spaces");
+  if (kernel >= bem::K_HSL && !(kappa > 0.0)) return fail(-1, "bem_op_create: kappa must be positive");
+  return guarded([&] {
+    const double t0 = now();
+    auto h = std::make_unique<bem_op>();
+    bem::Operator& op = h->op;
+    op.kernel = kernel;
+    op.kappa = kappa;
+    op.kinfo = bem::KernelInfo::make(kernel, kappa);
+    op.complex_valued = op.kinfo.complex_valued;
+    op.intorder = p->intorder;
+    op.leafsize = p->dense ? INT_MAX : p->leafsize;
+    op.eta = p->dense ? 0.0 : p->eta;
+    op.eps = p->eps;
+    op.dense = p->dense != 0;
+    op.mixed = p->mixed != 0;
+    op.mesh = trial->m;
+    op.trial = trial->s;
+    op.test = test->s;
+    op.same_space = trial->s == test->s || (trial->s->kind == test->s->kind);
+    op.trial_tree.build(*op.trial, op.leafsize);
+    if (op.same_space) op.test_tree = &op.trial_tree;
+    else {
+      op.test_tree_own.build(*op.test, op.leafsize);
+      op.test_tree = &op.test_tree_own;
+    }
+    bem::partition(*op.test_tree, op.trial_tree, op.eta, op.blocks);
+    mark_fallbacks(op);
+    op.blk_slot.resize(op.blocks.size());
+    for (size_t b = 0; b < op.blocks.size(); b++) {
+      const bem::Block& bl = op.blocks[b];
+      if (bl.near || bl.fallback) {
+        op.blk_slot[b] = (int)op.near.size();
+        bem::NearBlk nb{};
+        nb.r0 = bl.r0; nb.m = bl.m; nb.c0 = bl.c0; nb.n = bl.n;
+        op.near.push_back(nb);
+      } else {
+        op.blk_slot[b] = (int)op.far.size();
+        bem::FarBlk fb{};
+        fb.r0 = bl.r0; fb.m = bl.m; fb.c0 = bl.c0; fb.n = bl.n;
+        op.far.push_back(fb);
+      }
+    }
+    op.trig = bem::trig_rule(op.intorder);
+    for (int k = 0; k < 3; k++) op.ss[k] = bem::sauter_schwab(k, op.intorder);
+    op.t_setup = now() - t0;
+    *out = h.release();
+  });
+}
+
+int bem_op_assemble(bem_op* h, void* stream) {
+  if (!h) return fail(-1, "bem_op_assemble: null operator");
+  return guarded([&] {
+    bem::Operator& op = h->op;
+    hipStream_t s = (hipStream_t)stream;
+    const double t0 = now();
+    bem::setup_device(op, s);
+    bem::assemble_near(op, s);
+    bem::assemble_far(op, s);
+    bem::build_matvec_plan(op, s);
+    if (op.mixed) bem::make_mixed_copy(op, s);
+    op.t_setup += now() - t0 - op.t_near - op.t_far;
+    op.assembled = true;
+  });
+}
+
+int bem_op_apply(bem_op* h, int flags, const double* alpha, const void* x, const double* beta, void* y, int nrhs,
+                 int ld, int dtype, void* stream) {
+  if (!h || !alpha || !beta || !x || !y || nrhs < 1) return fail(-1, "bem_op_apply: bad arguments");
+  bem::Operator& op = h->op;
+  const bool trans = flags & BEM_APPLY_TRANS;
+  const int nin = trans ? op.ndof_test() : op.ndof_trial();
+  const int nout = trans ? op.ndof_trial() : op.ndof_test();
+  if (nrhs > 1 && ld < std::max(nin, nout)) return fail(-1, "bem_op_apply: ld smaller than the vector length");
+  if (op.complex_valued && dtype != BEM_C128) return fail(-1, "bem_op_apply: complex operator needs complex128 vectors");
+  if (!op.complex_valued && dtype != BEM_F64) return fail(-1, "bem_op_apply: real operator needs float64 vectors");
+  return guarded([&] {
+    hipStream_t s = (hipStream_t)stream;
+    const size_t esz = op.complex_valued ? 16 : 8;
+    const int dflags = flags & (BEM_APPLY_TRANS | BEM_APPLY_CONJ | BEM_APPLY_MIXED);
+    if (flags & BEM_APPLY_ON_DEVICE) {
+      for (int r = 0; r < nrhs; r++)
+        bem::apply_device(op, dflags, alpha, (const char*)x + (size_t)r * ld * esz, beta,
+                          (char*)y + (size_t)r * ld * esz, s);
+    } else {
+      bem::DBuf dx, dy;
+      dx.alloc((size_t)nin * esz);
+      dy.alloc((size_t)nout * esz);
+      for (int r = 0; r < nrhs; r++) {
+        const char* xr = (const char*)x + (size_t)r * (nrhs > 1 ? ld : nin) * esz;
+        char* yr = (char*)y + (size_t)r * (nrhs > 1 ? ld : nout) * esz;
+        BEM_HIP(hipMemcpyAsync(dx.p, xr, (size_t)nin * esz, hipMemcpyHostToDevice, s));
+        if (!(beta[0] == 0.0 && beta[1] == 0.0))
+          BEM_HIP(hipMemcpyAsync(dy.p, yr, (size_t)nout * esz, hipMemcpyHostToDevice, s));
+        bem::apply_device(op, dflags, alpha, dx.p, beta, dy.p, s);
+        BEM_HIP(hipMemcpyAsync(yr, dy.p, (size_t)nout * esz, hipMemcpyDeviceToHost, s));
+        BEM_HIP(hipStreamSynchronize(s));
+      }
+    }
+  });
+}
+
+int bem_op_profile_apply(bem_op* h, int flags, int iters, double* ms, void* stream) {
+  if (!h || !ms || iters < 1) return fail(-1, "bem_op_profile_apply: bad arguments");
+  return guarded([&] {
+    bem::Operator& op = h->op;
+    hipStream_t s = (hipStream_t)stream;
+    const size_t esz = op.complex_valued ? 16 : 8;
+    const int n = std::max(op.ndof_trial(), op.ndof_test());
+    bem::DBuf dx, dy;
+    dx.alloc((size_t)n * esz);
+    dy.alloc((size_t)n * esz);
+    std::vector<double> ones((size_t)n * esz / 8, 1.0);
+    BEM_HIP(hipMemcpyAsync(dx.p, ones.data(), (size_t)n * esz, hipMemcpyHostToDevice, s));
+    hipEvent_t ev[5];
+    for (auto& e : ev) BEM_HIP(hipEventCreate(&e));
+    const double one[2] = {1.0, 0.0}, zero[2] = {0.0, 0.0};
+    for (int k = 0; k < 5; k++) ms[k] = 0.0;
+    const int dflags = flags & (BEM_APPLY_TRANS | BEM_APPLY_CONJ | BEM_APPLY_MIXED);
+    for (int it = 0; it < iters; it++) {
+      bem::apply_device(op, dflags, one, dx.p, zero, dy.p, s, ev);
+      BEM_HIP(hipEventSynchronize(ev[4]));
+      float t;
+      for (int k = 0; k < 4; k++) {
+        BEM_HIP(hipEventElapsedTime(&t, ev[k], ev[k + 1]));
+        ms[k] += t / iters;
+      }
+      BEM_HIP(hipEventElapsedTime(&t, ev[0], ev[4]));
+      ms[4] += t / iters;
+    }
+    for (auto& e : ev) (void)hipEventDestroy(e);
+  });
+}
+
+int bem_op_stats(const bem_op* h, bem_stats* st) {
+  if (!h || !st) return fail(-1, "bem_op_stats: bad arguments");
+  const bem::Operator& op = h->op;
+  std::memset(st, 0, sizeof(*st));
+  st->near_entries = op.near_entries;
+  st->far_factor_entries = op.far_factor_entries;
+  st->nze = op.near_entries + op.far_factor_entries;
+  st->n_blocks = (int)op.blocks.size();
+  st->n_near = (int)op.near.size();
+  st->n_far = (int)op.far.size();
+  st->n_fallback = op.n_fallback;
+  int mr = 0;
+  for (int r : op.far_rank) mr = std::max(mr, r);
+  st->max_rank = mr;
+  st->height = op.ndof_test();
+  st->width = op.ndof_trial();
+  st->is_complex = op.complex_valued;
+  for (int c = 0; c < 3; c++) st->n_singular_pairs[c] = op.n_sing[c];
+  st->t_setup = op.t_setup;
+  st->t_near = op.t_near;
+  st->t_far = op.t_far;
+  st->t_total = op.t_setup + op.t_near + op.t_far;
+  st->near_kernel_evals = op.near_evals;
+  st->far_kernel_evals = op.far_evals;
+  return 0;
+}
+
+int bem_op_destroy(bem_op* h) {
+  delete h;
+  return 0;
+}
+
+int bem_op_num_blocks(const bem_op* h) { return h ? (int)h->op.blocks.size() : -1; }
+
+int bem_op_block_info(const bem_op* h, int b, bem_block_info* info) {
+  if (!h || !info || b < 0 || b >= (int)h->op.blocks.size()) return fail(-1, "bem_op_block_info: bad arguments");
+  const bem::Operator& op = h->op;
+  const bem::Block& bl = op.blocks[b];
+  info->near = bl.near || bl.fallback;
+  info->fallback = bl.fallback;
+  info->row_cluster = bl.row;
+  info->col_cluster = bl.col;
+  info->row0 = bl.r0;
+  info->m = bl.m;
+  info->col0 = bl.c0;
+  info->n = bl.n;
+  info->rank = info->near ? -1 : (op.assembled ? op.far[op.blk_slot[b]].R : -2);
+  return 0;
+}
+
+int bem_op_block_dofs(const bem_op* h, int b, int32_t* rows, int32_t* cols) {
+  if (!h || b < 0 || b >= (int)h->op.blocks.size()) return fail(-1, "bem_op_block_dofs: bad arguments");
+  const bem::Operator& op = h->op;
+  const bem::Block& bl = op.blocks[b];
+  if (rows) std::memcpy(rows, &op.test_tree->perm[bl.r0], sizeof(int) * bl.m);
+  if (cols) std::memcpy(cols, &op.trial_tree.perm[bl.c0], sizeof(int) * bl.n);
+  return 0;
+}
+
+int bem_op_block_data(const bem_op* h, int b, void* out) {
+  if (!h || !out || b < 0 || b >= (int)h->op.blocks.size()) return fail(-1, "bem_op_block_data: bad arguments");
+  const bem::Operator& op = h->op;
+  if (!op.assembled) return fail(-1, "bem_op_block_data: operator not assembled");
+  return guarded([&] {
+    const size_t esz = op.complex_valued ? 16 : 8;
+    const bem::Block& bl = op.blocks[b];
+    const int slot = op.blk_slot[b];
+    if (bl.near || bl.fallback) {
+      const bem::NearBlk& nb = op.near[slot];
+      BEM_HIP(hipMemcpy(out, (const char*)op.d_near_val.p + nb.off * esz, (size_t)nb.m * nb.n * esz,
+                        hipMemcpyDeviceToHost));
+    } else {
+      const bem::FarBlk& fb = op.far[slot];
+      const size_t nu = (size_t)fb.m * fb.R, nv = (size_t)fb.R * fb.n;
+      if (nu) BEM_HIP(hipMemcpy(out, (const char*)op.d_fU.p + fb.uoff * esz, nu * esz, hipMemcpyDeviceToHost));
+      if (nv)
+        BEM_HIP(hipMemcpy((char*)out + nu * esz, (const char*)op.d_fV.p + fb.voff * esz, nv * esz,
+                          hipMemcpyDeviceToHost));
+    }
+  });
+}
+
+int bem_op_tree(const bem_op* h, int which, int32_t* perm, double* cl, double* cen, int cap) {
+  if (!h) return fail(-1, "bem_op_tree: null operator");
+  const bem::Tree& t = which ? *h->op.test_tree : h->op.trial_tree;
+  const int n = (int)t.cl.size();
+  if (perm) std::memcpy(perm, t.perm.data(), sizeof(int) * t.perm.size());
+  if (cl && cen && cap >= n)
+    for (int i = 0; i < n; i++) {
+      const bem::Cluster& c = t.cl[i];
+      const double v[7] = {(double)c.level,  (double)c.parent,   (double)c.child1, (double)c.child2,
+                           (double)c.number, (double)c.permupos, c.radius};
+      std::memcpy(cl + 7 * i, v, sizeof(v));
+      std::memcpy(cen + 3 * i, c.centre, sizeof(c.centre));
+    }
+  return n;
+}
+
+}  // extern "C"

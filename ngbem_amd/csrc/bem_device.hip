@@ -1,0 +1,1530 @@
+// Device side of the MI355X BEM library (gfx950 / CDNA4, 64-wide wavefronts).
+//
+// Hot path, restated from Weggler/ngbem and re-laid-out for the GPU:
+//   near field  src/ngbem.cpp:373-738 (CalcBlockMatrix)
+//     - regular pairs: one lane per element pair, tensor triangle rule, both
+//       elements' quadrature points precomputed once per element in HBM
+//     - singular pairs: binned by class (identical / common edge / common
+//       vertex), one lane per pair; every lane walks the same Sauter-Schwab
+//       point at the same time, so the rule is read through wave-uniform
+//       (scalar) loads and the per-lane state is just the pair geometry
+//   far field   src/ngbem.cpp:743-1163 (CalcFarFieldBlock): partially pivoted
+//     ACA, one workgroup per (admissible block, kernel component), pivot search
+//     by wavefront shuffles + LDS, rows/columns of the kernel matrix evaluated
+//     on the fly; then a projection kernel forms the dof-level factors
+//   matvec      src/hmat.cpp:473-565, 632-666: vectors are permuted to cluster
+//     order so every block is a contiguous row/column range; low-rank blocks
+//     first form B^T x (one workgroup per block), then one workgroup per leaf
+//     row cluster sums every dense and low-rank contribution of its rows in a
+//     fixed order (deterministic, no atomics) and writes y once.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <chrono>
+#include <cmath>
+#include <cstring>
+#include <numeric>
+
+#include "bem_op.hpp"
+
+namespace bem {
+
+// ------------------------------------------------------------ arithmetic --
+
+template <class R>
+struct cx {
+  R x, y;
+};
+using zc = cx<double>;
+using fc = cx<float>;
+
+template <class R>
+__host__ __device__ __forceinline__ cx<R> operator+(cx<R> a, cx<R> b) { return {a.x + b.x, a.y + b.y}; }
+template <class R>
+__host__ __device__ __forceinline__ cx<R> operator-(cx<R> a, cx<R> b) { return {a.x - b.x, a.y - b.y}; }
+template <class R>
+__host__ __device__ __forceinline__ cx<R> operator*(cx<R> a, cx<R> b) {
+  return {a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x};
+}
+template <class R>
+__host__ __device__ __forceinline__ cx<R> operator*(cx<R> a, R s) { return {a.x * s, a.y * s}; }
+template <class R>
+__host__ __device__ __forceinline__ cx<R>& operator+=(cx<R>& a, cx<R> b) { a.x += b.x; a.y += b.y; return a; }
+template <class R>
+__host__ __device__ __forceinline__ cx<R>& operator-=(cx<R>& a, cx<R> b) { a.x -= b.x; a.y -= b.y; return a; }
+
+__host__ __device__ __forceinline__ double cconj(double a) { return a; }
+__host__ __device__ __forceinline__ float cconj(float a) { return a; }
+template <class R>
+__host__ __device__ __forceinline__ cx<R> cconj(cx<R> a) { return {a.x, -a.y}; }
+__host__ __device__ __forceinline__ double abs2(double a) { return a * a; }
+template <class R>
+__host__ __device__ __forceinline__ R abs2(cx<R> a) { return a.x * a.x + a.y * a.y; }
+__host__ __device__ __forceinline__ double real_(double a) { return a; }
+template <class R>
+__host__ __device__ __forceinline__ R real_(cx<R> a) { return a.x; }
+template <class T> __host__ __device__ __forceinline__ T zero_();
+template <> __host__ __device__ __forceinline__ double zero_<double>() { return 0.0; }
+template <> __host__ __device__ __forceinline__ float zero_<float>() { return 0.0f; }
+template <> __host__ __device__ __forceinline__ zc zero_<zc>() { return {0.0, 0.0}; }
+template <> __host__ __device__ __forceinline__ fc zero_<fc>() { return {0.0f, 0.0f}; }
+// scalar-times-value with a real factor
+__host__ __device__ __forceinline__ double rmul(double a, double s) { return a * s; }
+template <class R>
+__host__ __device__ __forceinline__ cx<R> rmul(cx<R> a, R s) { return {a.x * s, a.y * s}; }
+// conversions between storage/compute types
+__host__ __device__ __forceinline__ void cvt(double& o, double a) { o = a; }
+__host__ __device__ __forceinline__ void cvt(float& o, double a) { o = (float)a; }
+__host__ __device__ __forceinline__ void cvt(double& o, float a) { o = a; }
+__host__ __device__ __forceinline__ void cvt(float& o, float a) { o = a; }
+__host__ __device__ __forceinline__ void cvt(zc& o, zc a) { o = a; }
+__host__ __device__ __forceinline__ void cvt(fc& o, zc a) { o = {(float)a.x, (float)a.y}; }
+__host__ __device__ __forceinline__ void cvt(zc& o, fc a) { o = {a.x, a.y}; }
+__host__ __device__ __forceinline__ void cvt(fc& o, fc a) { o = a; }
+
+__device__ __forceinline__ void atomic_add(double* p, double v) { unsafeAtomicAdd(p, v); }
+__device__ __forceinline__ void atomic_add(zc* p, zc v) {
+  unsafeAtomicAdd(&p->x, v.x);
+  unsafeAtomicAdd(&p->y, v.y);
+}
+
+template <class T>
+__device__ __forceinline__ T shfl_xor(T v, int m) { return __shfl_xor(v, m, 64); }
+template <class R>
+__device__ __forceinline__ cx<R> shfl_xor(cx<R> v, int m) {
+  return {__shfl_xor(v.x, m, 64), __shfl_xor(v.y, m, 64)};
+}
+template <class T>
+__device__ __forceinline__ T wave_sum(T v) {
+#pragma unroll
+  for (int m = 32; m >= 1; m >>= 1) v += shfl_xor(v, m);
+  return v;
+}
+
+// ---------------------------------------------------------------- kernels --
+// src/ngbem.hpp:230-431.  eval() returns all components at (x, y, n_x, n_y).
+
+struct P3 {
+  double x, y, z;
+};
+__device__ __forceinline__ double dot3(P3 a, P3 b) { return a.x * b.x + a.y * b.y + a.z * b.z; }
+
+struct TermC {
+  double fac;
+  int kc, trc, tec;
+};
+
+constexpr double kInv4Pi = 0.07957747154594767;  // 1/(4 pi)
+
+struct KLapSL {
+  typedef double T;
+  static constexpr int NC = 1, NTERM = 1;
+  double kappa;
+  __device__ static constexpr TermC term(int) { return {1.0, 0, 0, 0}; }
+  __device__ __forceinline__ void eval(P3 x, P3 y, P3, P3, T* o) const {
+    double dx = x.x - y.x, dy = x.y - y.y, dz = x.z - y.z;
+    double r = sqrt(dx * dx + dy * dy + dz * dz);
+    o[0] = kInv4Pi / r;
+  }
+};
+struct KLapHS : KLapSL {  // src/ngbem.hpp:279-300: 1/(4 pi r), curl . curl terms
+  static constexpr int NTERM = 3;
+  __device__ static constexpr TermC term(int t) { return {1.0, 0, t, t}; }
+};
+struct KLapDL {
+  typedef double T;
+  static constexpr int NC = 1, NTERM = 1;
+  double kappa;
+  __device__ static constexpr TermC term(int) { return {1.0, 0, 0, 0}; }
+  __device__ __forceinline__ void eval(P3 x, P3 y, P3, P3 ny, T* o) const {
+    double dx = x.x - y.x, dy = x.y - y.y, dz = x.z - y.z;
+    double r = sqrt(dx * dx + dy * dy + dz * dz);
+    double nyd = ny.x * dx + ny.y * dy + ny.z * dz;
+    o[0] = nyd * kInv4Pi / (r * r * r);
+  }
+};
+struct KHelSL {
+  typedef zc T;
+  static constexpr int NC = 1, NTERM = 1;
+  double kappa;
+  __device__ static constexpr TermC term(int) { return {1.0, 0, 0, 0}; }
+  __device__ __forceinline__ void eval(P3 x, P3 y, P3, P3, T* o) const {
+    double dx = x.x - y.x, dy = x.y - y.y, dz = x.z - y.z;
+    double r = sqrt(dx * dx + dy * dy + dz * dz);
+    double s, c;
+    sincos(kappa * r, &s, &c);
+    double g = kInv4Pi / r;
+    o[0] = {c * g, s * g};
+  }
+};
+struct KHelDL {  // e^{ikr}/(4 pi r^3) <n_y, x-y> (1 - i k r)
+  typedef zc T;
+  static constexpr int NC = 1, NTERM = 1;
+  double kappa;
+  __device__ static constexpr TermC term(int) { return {1.0, 0, 0, 0}; }
+  __device__ __forceinline__ void eval(P3 x, P3 y, P3, P3 ny, T* o) const {
+    double dx = x.x - y.x, dy = x.y - y.y, dz = x.z - y.z;
+    double r = sqrt(dx * dx + dy * dy + dz * dz);
+    double nyd = ny.x * dx + ny.y * dy + ny.z * dz;
+    double kr = kappa * r, s, c;
+    sincos(kr, &s, &c);
+    double a = nyd * kInv4Pi / (r * r * r);
+    o[0] = {a * (c + kr * s), a * (s - kr * c)};
+  }
+};
+struct KHelCF {  // e^{ikr}/(4 pi r^3) (<n_y, x-y>(1 - i k r) - i k r^2)
+  typedef zc T;
+  static constexpr int NC = 1, NTERM = 1;
+  double kappa;
+  __device__ static constexpr TermC term(int) { return {1.0, 0, 0, 0}; }
+  __device__ __forceinline__ void eval(P3 x, P3 y, P3, P3 ny, T* o) const {
+    double dx = x.x - y.x, dy = x.y - y.y, dz = x.z - y.z;
+    double r2 = dx * dx + dy * dy + dz * dz;
+    double r = sqrt(r2);
+    double nyd = ny.x * dx + ny.y * dy + ny.z * dz;
+    double kr = kappa * r, s, c;
+    sincos(kr, &s, &c);
+    double g = kInv4Pi / (r2 * r);
+    double B = kr * nyd + kappa * r2;  // imaginary part of -(nyd(1-ikr) - i k r^2) ... times -1
+    o[0] = {g * (c * nyd + s * B), g * (s * nyd - c * B)};
+  }
+};
+struct KHelHS {  // {G, -k^2 G <n_x,n_y>}, src/ngbem.hpp:371-397
+  typedef zc T;
+  static constexpr int NC = 2, NTERM = 4;
+  double kappa;
+  __device__ static constexpr TermC term(int t) { return t < 3 ? TermC{1.0, 0, t, t} : TermC{1.0, 1, 3, 3}; }
+  __device__ __forceinline__ void eval(P3 x, P3 y, P3 nx, P3 ny, T* o) const {
+    double dx = x.x - y.x, dy = x.y - y.y, dz = x.z - y.z;
+    double r = sqrt(dx * dx + dy * dy + dz * dz);
+    double s, c;
+    sincos(kappa * r, &s, &c);
+    double g = kInv4Pi / r;
+    double f = -kappa * kappa * dot3(nx, ny) * g;
+    o[0] = {c * g, s * g};
+    o[1] = {c * f, s * f};
+  }
+};
+
+// ------------------------------------------------------------- evaluators --
+// shape rows at one point, layout dof*DIM + comp (src/diffops.hpp:12-165)
+
+template <int SP, int EV>
+struct Ev;
+template <>
+struct Ev<SP_P0, EV_ID> {
+  static constexpr int NL = 1, DIM = 1;
+  static constexpr bool P1 = false;
+  __device__ __forceinline__ static void shape(const double*, const double*, double* o) { o[0] = 1.0; }
+};
+template <>
+struct Ev<SP_P1, EV_ID> {
+  static constexpr int NL = 3, DIM = 1;
+  static constexpr bool P1 = true;
+  __device__ __forceinline__ static void shape(const double* lam, const double*, double* o) {
+    o[0] = lam[0]; o[1] = lam[1]; o[2] = lam[2];
+  }
+};
+template <>
+struct Ev<SP_P1, EV_ROT> {
+  static constexpr int NL = 3, DIM = 3;
+  static constexpr bool P1 = true;
+  __device__ __forceinline__ static void shape(const double*, const double* curl, double* o) {
+#pragma unroll
+    for (int i = 0; i < 9; i++) o[i] = curl[i];
+  }
+};
+template <>
+struct Ev<SP_P1, EV_HELM> {
+  static constexpr int NL = 3, DIM = 4;
+  static constexpr bool P1 = true;
+  __device__ __forceinline__ static void shape(const double* lam, const double* curl, double* o) {
+#pragma unroll
+    for (int k = 0; k < 3; k++) {
+      o[4 * k + 0] = curl[3 * k + 0];
+      o[4 * k + 1] = curl[3 * k + 1];
+      o[4 * k + 2] = curl[3 * k + 2];
+      o[4 * k + 3] = lam[k];
+    }
+  }
+};
+
+// elmat[ii][jj] += sum_t fac_t * f * K_{kc_t} * phi_i[ii,tec_t] * phi_j[jj,trc_t]
+template <class KER, class EI, class EJ>
+__device__ __forceinline__ void accum(typename KER::T (&el)[EI::NL][EJ::NL], const typename KER::T* kv, double f,
+                                      const double* si, const double* sj) {
+  typedef typename KER::T T;
+#pragma unroll
+  for (int t = 0; t < KER::NTERM; t++) {
+    const TermC tm = KER::term(t);
+    T k = rmul(kv[tm.kc], tm.fac * f);
+#pragma unroll
+    for (int ii = 0; ii < EI::NL; ii++) {
+      T ks = rmul(k, si[ii * EI::DIM + tm.tec]);
+#pragma unroll
+      for (int jj = 0; jj < EJ::NL; jj++) el[ii][jj] += rmul(ks, sj[jj * EJ::DIM + tm.trc]);
+    }
+  }
+}
+
+// element data layout
+constexpr int GEO = 16;  // p0 p1 p2 n J pad
+__device__ __forceinline__ P3 ld3(const double* p) { return {p[0], p[1], p[2]}; }
+
+// ------------------------------------------------------ regular near field --
+
+template <class KER, class EI, class EJ, bool DIRECT>
+__global__ __launch_bounds__(256) void k_near_regular(const NearBlk* __restrict__ blks, const int2* __restrict__ chunks,
+                                                      const int* __restrict__ pIb, const int* __restrict__ pJb,
+                                                      const double* __restrict__ geo, const int4* __restrict__ vid,
+                                                      const double* __restrict__ quad, const double* __restrict__ curl,
+                                                      const double* __restrict__ trig, int ntq,
+                                                      const int* __restrict__ ipt, const int* __restrict__ ipr, KER ker,
+                                                      typename KER::T* __restrict__ val) {
+  typedef typename KER::T T;
+  const int2 ch = chunks[blockIdx.x];
+  const NearBlk nb = blks[ch.x];
+  const int p = ch.y + (int)threadIdx.x;
+  if (p >= nb.pm * nb.pn) return;
+  const int ia = p / nb.pn, ja = p - ia * nb.pn;
+  const int e = pIb[nb.pI + ia], f = pJb[nb.pJ + ja];
+  const int4 ve = vid[e], vf = vid[f];
+  const bool touch = ve.x == vf.x || ve.x == vf.y || ve.x == vf.z || ve.y == vf.x || ve.y == vf.y || ve.y == vf.z ||
+                     ve.z == vf.x || ve.z == vf.y || ve.z == vf.z;
+  if (touch) return;  // singular pair: written by k_near_singular
+  const P3 ne = ld3(geo + (size_t)e * GEO + 9), nf = ld3(geo + (size_t)f * GEO + 9);
+  double ce[9], cf[9];
+  if (EI::DIM > 1)
+    for (int i = 0; i < 9; i++) ce[i] = curl[(size_t)e * 9 + i];
+  if (EJ::DIM > 1)
+    for (int i = 0; i < 9; i++) cf[i] = curl[(size_t)f * 9 + i];
+  T el[EI::NL][EJ::NL];
+#pragma unroll
+  for (int a = 0; a < EI::NL; a++)
+#pragma unroll
+    for (int b = 0; b < EJ::NL; b++) el[a][b] = zero_<T>();
+  const double* qe = quad + (size_t)e * ntq * 4;
+  const double* qf = quad + (size_t)f * ntq * 4;
+  for (int ix = 0; ix < ntq; ix++) {
+    const P3 x = ld3(qe + 4 * ix);
+    const double wx = qe[4 * ix + 3];
+    double lx[3] = {trig[3 * ix], trig[3 * ix + 1], 1.0 - trig[3 * ix] - trig[3 * ix + 1]};
+    double si[EI::NL * EI::DIM];
+    EI::shape(lx, ce, si);
+    for (int iy = 0; iy < ntq; iy++) {
+      const P3 y = ld3(qf + 4 * iy);
+      const double wy = qf[4 * iy + 3];
+      double ly[3] = {trig[3 * iy], trig[3 * iy + 1], 1.0 - trig[3 * iy] - trig[3 * iy + 1]};
+      double sj[EJ::NL * EJ::DIM];
+      EJ::shape(ly, cf, sj);
+      T kv[KER::NC];
+      ker.eval(x, y, ne, nf, kv);
+      accum<KER, EI, EJ>(el, kv, wx * wy, si, sj);
+    }
+  }
+  if (DIRECT) {
+    val[nb.off + (int64_t)ia * nb.n + ja] = el[0][0];
+  } else {
+    const int vi[3] = {ve.x, ve.y, ve.z}, vj[3] = {vf.x, vf.y, vf.z};
+#pragma unroll
+    for (int ii = 0; ii < EI::NL; ii++) {
+      const int row = EI::P1 ? ipt[vi[ii]] - nb.r0 : ia;
+      if (row < 0 || row >= nb.m) continue;
+#pragma unroll
+      for (int jj = 0; jj < EJ::NL; jj++) {
+        const int col = EJ::P1 ? ipr[vj[jj]] - nb.c0 : ja;
+        if (col < 0 || col >= nb.n) continue;
+        atomic_add(&val[nb.off + (int64_t)row * nb.n + col], el[ii][jj]);
+      }
+    }
+  }
+}
+
+// ----------------------------------------------------- singular near field --
+// pair = (near block, test elem, trial elem, code); code packs the vertex
+// permutations px (bits 0-5) and py (bits 6-11): Sauter-Schwab vertex k of the
+// test element is its local vertex px[k] (src/ngbem.cpp:569-642).
+
+template <class KER, class EI, class EJ, bool DIRECT>
+__global__ __launch_bounds__(256) void k_near_singular(const int4* __restrict__ pairs, int npairs,
+                                                       const NearBlk* __restrict__ blks, const double* __restrict__ rule,
+                                                       int Q, const double* __restrict__ geo,
+                                                       const int* __restrict__ vidf, const double* __restrict__ curl,
+                                                       const int* __restrict__ ipt, const int* __restrict__ ipr, KER ker,
+                                                       typename KER::T* __restrict__ val) {
+  typedef typename KER::T T;
+  const int p = blockIdx.x * 256 + threadIdx.x;
+  if (p >= npairs) return;
+  const int4 pr = pairs[p];
+  const NearBlk nb = blks[pr.x];
+  const int e = pr.y, f = pr.z;
+  int px[3], py[3];
+#pragma unroll
+  for (int k = 0; k < 3; k++) {
+    px[k] = (pr.w >> (2 * k)) & 3;
+    py[k] = (pr.w >> (6 + 2 * k)) & 3;
+  }
+  const double* ge = geo + (size_t)e * GEO;
+  const double* gf = geo + (size_t)f * GEO;
+  const P3 E0 = ld3(ge + 3 * px[0]), E1 = ld3(ge + 3 * px[1]), E2 = ld3(ge + 3 * px[2]);
+  const P3 F0 = ld3(gf + 3 * py[0]), F1 = ld3(gf + 3 * py[1]), F2 = ld3(gf + 3 * py[2]);
+  const P3 ne = ld3(ge + 9), nf = ld3(gf + 9);
+  const double JJ = ge[12] * gf[12];
+  // curls in Sauter-Schwab vertex order
+  double ce[9], cf[9];
+  if (EI::DIM > 1)
+    for (int k = 0; k < 3; k++)
+      for (int c = 0; c < 3; c++) ce[3 * k + c] = curl[(size_t)e * 9 + 3 * px[k] + c];
+  if (EJ::DIM > 1)
+    for (int k = 0; k < 3; k++)
+      for (int c = 0; c < 3; c++) cf[3 * k + c] = curl[(size_t)f * 9 + 3 * py[k] + c];
+  T el[EI::NL][EJ::NL];
+#pragma unroll
+  for (int a = 0; a < EI::NL; a++)
+#pragma unroll
+    for (int b = 0; b < EJ::NL; b++) el[a][b] = zero_<T>();
+  for (int q = 0; q < Q; q++) {
+    const double* r = rule + 5 * q;  // wave-uniform
+    const double x0 = r[0], x1 = r[1], y0 = r[2], y1 = r[3], w = r[4];
+    const double lx[3] = {1.0 - x0 - x1, x0, x1};
+    const double ly[3] = {1.0 - y0 - y1, y0, y1};
+    const P3 X = {lx[0] * E0.x + lx[1] * E1.x + lx[2] * E2.x, lx[0] * E0.y + lx[1] * E1.y + lx[2] * E2.y,
+                  lx[0] * E0.z + lx[1] * E1.z + lx[2] * E2.z};
+    const P3 Y = {ly[0] * F0.x + ly[1] * F1.x + ly[2] * F2.x, ly[0] * F0.y + ly[1] * F1.y + ly[2] * F2.y,
+                  ly[0] * F0.z + ly[1] * F1.z + ly[2] * F2.z};
+    double si[EI::NL * EI::DIM], sj[EJ::NL * EJ::DIM];
+    EI::shape(lx, ce, si);
+    EJ::shape(ly, cf, sj);
+    T kv[KER::NC];
+    ker.eval(X, Y, ne, nf, kv);
+    accum<KER, EI, EJ>(el, kv, JJ * w, si, sj);
+  }
+  if (DIRECT) {
+    const int row = ipt[e] - nb.r0, col = ipr[f] - nb.c0;
+    val[nb.off + (int64_t)row * nb.n + col] = el[0][0];
+  } else {
+#pragma unroll
+    for (int kx = 0; kx < EI::NL; kx++) {
+      const int row = EI::P1 ? ipt[vidf[4 * e + px[kx]]] - nb.r0 : ipt[e] - nb.r0;
+      if (row < 0 || row >= nb.m) continue;
+#pragma unroll
+      for (int ky = 0; ky < EJ::NL; ky++) {
+        const int col = EJ::P1 ? ipr[vidf[4 * f + py[ky]]] - nb.c0 : ipr[f] - nb.c0;
+        if (col < 0 || col >= nb.n) continue;
+        atomic_add(&val[nb.off + (int64_t)row * nb.n + col], el[kx][ky]);
+      }
+    }
+  }
+}
+
+// ------------------------------------------------------------------- ACA --
+// src/ngbem.cpp:902-1049 for one (far block, component) per workgroup.
+// Workspace: U column l at U[l*np + i] (point rows), V row l at V[l*nq + j].
+
+constexpr int ACA_THREADS = 256;
+constexpr int ACA_WAVES = ACA_THREADS / 64;
+constexpr int IP_CHUNK = 8;  // inner products reduced together
+
+struct ArgMax {
+  double a2;
+  int idx;
+  double vr, vi;
+};
+__device__ __forceinline__ bool better(const ArgMax& a, const ArgMax& b) {
+  return a.a2 > b.a2 || (a.a2 == b.a2 && a.idx < b.idx);
+}
+__device__ __forceinline__ ArgMax wave_argmax(ArgMax v) {
+#pragma unroll
+  for (int m = 32; m >= 1; m >>= 1) {
+    ArgMax o{__shfl_xor(v.a2, m, 64), __shfl_xor(v.idx, m, 64), __shfl_xor(v.vr, m, 64), __shfl_xor(v.vi, m, 64)};
+    if (better(o, v)) v = o;
+  }
+  return v;
+}
+__device__ __forceinline__ void setv(ArgMax& a, double v) { a.vr = v; a.vi = 0.0; }
+__device__ __forceinline__ void setv(ArgMax& a, zc v) { a.vr = v.x; a.vi = v.y; }
+__device__ __forceinline__ void getv(const ArgMax& a, double& v) { v = a.vr; }
+__device__ __forceinline__ void getv(const ArgMax& a, zc& v) { v = {a.vr, a.vi}; }
+__device__ __forceinline__ double div_(double a, double b) { return a / b; }
+__device__ __forceinline__ zc div_(double a, zc b) {  // a / b, a real
+  double d = b.x * b.x + b.y * b.y;
+  return {a * b.x / d, -a * b.y / d};
+}
+
+template <class T>
+__device__ ArgMax block_argmax(ArgMax v, ArgMax* sh) {
+  v = wave_argmax(v);
+  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+  if (lane == 0) sh[wv] = v;
+  __syncthreads();
+  ArgMax r = sh[0];
+  for (int w = 1; w < ACA_WAVES; w++)
+    if (better(sh[w], r)) r = sh[w];
+  __syncthreads();
+  return r;
+}
+
+template <class T>
+__device__ __forceinline__ T block_sum(T v, T* sh) {
+  v = wave_sum(v);
+  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+  if (lane == 0) sh[wv] = v;
+  __syncthreads();
+  T r = sh[0];
+  for (int w = 1; w < ACA_WAVES; w++) r += sh[w];
+  __syncthreads();
+  return r;
+}
+
+struct PointSet {
+  const int* patch;  // sorted elements
+  const double* quad;
+  const double* geo;
+  int ntq;
+  __device__ __forceinline__ void get(int i, P3& x, P3& n) const {
+    const int a = i / ntq, q = i - a * ntq;
+    const int e = patch[a];
+    x = ld3(quad + ((size_t)e * ntq + q) * 4);
+    n = ld3(geo + (size_t)e * GEO + 9);
+  }
+};
+
+template <class KER>
+__device__ __forceinline__ typename KER::T kcomp(const KER& ker, P3 x, P3 y, P3 nx, P3 ny, int comp) {
+  typename KER::T kv[KER::NC];
+  ker.eval(x, y, nx, ny, kv);
+  typename KER::T r = kv[0];
+#pragma unroll
+  for (int c = 1; c < KER::NC; c++)
+    if (comp == c) r = kv[c];
+  return r;
+}
+
+template <class KER>
+__global__ __launch_bounds__(ACA_THREADS) void k_aca(const AcaTask* __restrict__ tasks, const FarBlk* __restrict__ fb,
+                                                     const int* __restrict__ fpatch, const double* __restrict__ quad,
+                                                     const double* __restrict__ geo, int ntq, KER ker, double eps,
+                                                     typename KER::T* __restrict__ wsU, typename KER::T* __restrict__ wsV,
+                                                     int2* __restrict__ result) {
+  typedef typename KER::T T;
+  extern __shared__ unsigned char smem_raw[];
+  T* s_coef = reinterpret_cast<T*>(smem_raw);  // cap entries
+  __shared__ ArgMax s_am[ACA_WAVES];
+  __shared__ T s_red[ACA_WAVES * 2 * IP_CHUNK];
+  __shared__ double s_dred[ACA_WAVES * 2];
+  __shared__ int s_flag;
+
+  const AcaTask tk = tasks[blockIdx.x];
+  const FarBlk b = fb[tk.blk];
+  const PointSet X{fpatch + b.pI, quad, geo, ntq}, Y{fpatch + b.pJ, quad, geo, ntq};
+  const int np = tk.np, nq = tk.nq, comp = tk.comp;
+  T* U = wsU + tk.uoff;
+  T* V = wsV + tk.voff;
+  const int pmax = min(np, nq);
+  const int kmax = min(pmax, tk.cap);
+  const int tid = threadIdx.x;
+
+  int ik = 0, ikm1 = 0, jkm1 = nq + 1, rank = -1;
+  double norm2 = 0.0;
+  for (int k = 0; k < kmax; k++) {
+    for (int l = tid; l < k; l += ACA_THREADS) s_coef[l] = U[(size_t)l * np + ik];
+    __syncthreads();
+    // row ik of the residual: v_j = K(x_ik, y_j) - sum_l V[l][j] U[ik][l]
+    P3 xi, nxi;
+    X.get(ik, xi, nxi);
+    ArgMax best{-1.0, 0x7fffffff, 0.0, 0.0};
+    T* vk = V + (size_t)k * nq;
+    for (int j = tid; j < nq; j += ACA_THREADS) {
+      P3 yj, nyj;
+      Y.get(j, yj, nyj);
+      T v = kcomp(ker, xi, yj, nxi, nyj, comp);
+      for (int l = 0; l < k; l++) v -= V[(size_t)l * nq + j] * s_coef[l];
+      vk[j] = v;
+      ArgMax c{abs2(v), j, 0.0, 0.0};
+      setv(c, v);
+      if (j != jkm1 && better(c, best)) best = c;
+    }
+    const ArgMax am = block_argmax<T>(best, s_am);
+    if (!(am.a2 > 0.0)) {  // zero pivot: rank k (src/ngbem.cpp:977-980)
+      rank = k;
+      break;
+    }
+    const int jk = am.idx;
+    const double vkj = sqrt(am.a2);
+    T piv;
+    getv(am, piv);
+    const T sc = div_(sqrt(vkj), piv);
+    const double rs = 1.0 / sqrt(vkj);
+    for (int l = tid; l < k; l += ACA_THREADS) s_coef[l] = V[(size_t)l * nq + jk];
+    __syncthreads();
+    // column jk: u_i = (K(x_i, y_jk) - sum_l U[i][l] V[l][jk]) / sqrt(|v_jk|)
+    P3 yk, nyk;
+    Y.get(jk, yk, nyk);
+    ArgMax bu{-1.0, 0x7fffffff, 0.0, 0.0};
+    T* uk = U + (size_t)k * np;
+    double nu2 = 0.0;
+    for (int i = tid; i < np; i += ACA_THREADS) {
+      P3 x, nx;
+      X.get(i, x, nx);
+      T u = kcomp(ker, x, yk, nx, nyk, comp);
+      for (int l = 0; l < k; l++) u -= U[(size_t)l * np + i] * s_coef[l];
+      u = rmul(u, rs);
+      uk[i] = u;
+      nu2 += abs2(u);
+      ArgMax c{abs2(u), i, 0.0, 0.0};
+      if (i != ikm1 && better(c, bu)) bu = c;
+    }
+    // scale row k by sqrt(|v_jk|)/v_jk and take its norm
+    double nv2 = 0.0;
+    for (int j = tid; j < nq; j += ACA_THREADS) {
+      T v = vk[j] * sc;
+      vk[j] = v;
+      nv2 += abs2(v);
+    }
+    const ArgMax amu = block_argmax<T>(bu, s_am);
+    nv2 = wave_sum(nv2);
+    nu2 = wave_sum(nu2);
+    if ((tid & 63) == 0) {
+      s_dred[2 * (tid >> 6)] = nv2;
+      s_dred[2 * (tid >> 6) + 1] = nu2;
+    }
+    // cross terms sum_l Re(conj(<v_k, v_l>) <u_k, u_l>), src/ngbem.cpp:1012-1015
+    double cross = 0.0;
+    for (int l0 = 0; l0 < k; l0 += IP_CHUNK) {
+      const int nl = min(IP_CHUNK, k - l0);
+      T iv[IP_CHUNK], iu[IP_CHUNK];
+#pragma unroll
+      for (int c = 0; c < IP_CHUNK; c++) { iv[c] = zero_<T>(); iu[c] = zero_<T>(); }
+      for (int j = tid; j < nq; j += ACA_THREADS) {
+        const T v = vk[j];
+#pragma unroll
+        for (int c = 0; c < IP_CHUNK; c++)
+          if (c < nl) iv[c] += v * cconj(V[(size_t)(l0 + c) * nq + j]);
+      }
+      for (int i = tid; i < np; i += ACA_THREADS) {
+        const T u = uk[i];
+#pragma unroll
+        for (int c = 0; c < IP_CHUNK; c++)
+          if (c < nl) iu[c] += u * cconj(U[(size_t)(l0 + c) * np + i]);
+      }
+#pragma unroll
+      for (int c = 0; c < IP_CHUNK; c++) {
+        iv[c] = wave_sum(iv[c]);
+        iu[c] = wave_sum(iu[c]);
+      }
+      if ((tid & 63) == 0)
+#pragma unroll
+        for (int c = 0; c < IP_CHUNK; c++) {
+          s_red[(tid >> 6) * 2 * IP_CHUNK + c] = iv[c];
+          s_red[(tid >> 6) * 2 * IP_CHUNK + IP_CHUNK + c] = iu[c];
+        }
+      __syncthreads();
+      if (tid == 0)
+        for (int c = 0; c < nl; c++) {
+          T sv = s_red[c], su = s_red[IP_CHUNK + c];
+          for (int w = 1; w < ACA_WAVES; w++) {
+            sv += s_red[w * 2 * IP_CHUNK + c];
+            su += s_red[w * 2 * IP_CHUNK + IP_CHUNK + c];
+          }
+          cross += real_(cconj(sv) * su);
+        }
+      __syncthreads();
+    }
+    __syncthreads();
+    if (tid == 0) {
+      double v2 = 0.0, u2 = 0.0;
+      for (int w = 0; w < ACA_WAVES; w++) {
+        v2 += s_dred[2 * w];
+        u2 += s_dred[2 * w + 1];
+      }
+      const double norm_k = sqrt(v2) * sqrt(u2);
+      norm2 += norm_k * norm_k + 2.0 * cross;
+      s_flag = norm_k < eps * sqrt(norm2);
+    }
+    __syncthreads();
+    if (amu.a2 > 0.0) ik = amu.idx;  // keep the old row pivot if the column is zero
+    ikm1 = ik;
+    jkm1 = jk;
+    const int stop = s_flag;
+    __syncthreads();
+    if (stop) {
+      rank = k + 1;
+      break;
+    }
+  }
+  if (tid == 0) {
+    int status = 0;
+    if (rank < 0) {
+      if (kmax == pmax) rank = pmax;
+      else { rank = kmax; status = 1; }  // capacity reached: rerun with a larger cap
+    }
+    result[blockIdx.x] = make_int2(rank, status);
+  }
+}
+
+// dof-level factors from the point-level ACA factors (src/ngbem.cpp:1045-1160):
+// U2[ii][r0_t + l] = fac_t sum_{(a,k) in rowmap(ii)} sum_q phi_k,tec(q) w_aq U_c[l][a*ntq+q]
+// V2[r0_t + l][jj] =       sum_{(a,k) in colmap(jj)} sum_q phi_k,trc(q) w_aq V_c[l][a*ntq+q]
+struct ProjTask {
+  int blk;
+  int ncomp;
+  int64_t uoff[2], voff[2];  // workspace offsets per component
+  int rank[2];
+  int np, nq;
+  int64_t ou, ov;            // output offsets (batch buffer)
+};
+
+template <class KER, class EI, class EJ>
+__global__ __launch_bounds__(256) void k_project(const ProjTask* __restrict__ tasks, const FarBlk* __restrict__ fb,
+                                                 const int* __restrict__ fpatch, const int* __restrict__ maps,
+                                                 const double* __restrict__ quad, const double* __restrict__ curl,
+                                                 const double* __restrict__ trig, int ntq,
+                                                 const typename KER::T* __restrict__ wsU,
+                                                 const typename KER::T* __restrict__ wsV, typename KER::T* __restrict__ oU,
+                                                 typename KER::T* __restrict__ oV) {
+  typedef typename KER::T T;
+  const ProjTask tk = tasks[blockIdx.x];
+  const FarBlk b = fb[tk.blk];
+  int r0[KER::NTERM], rk[KER::NTERM], R = 0;
+#pragma unroll
+  for (int t = 0; t < KER::NTERM; t++) {
+    r0[t] = R;
+    rk[t] = tk.rank[KER::term(t).kc];
+    R += rk[t];
+  }
+  const int* PI = fpatch + b.pI;
+  const int* PJ = fpatch + b.pJ;
+  const int* rmo = maps + b.rmap;  // m+1 offsets then entries a*4+k
+  const int* cmo = maps + b.cmap;
+  // U2: m x R
+  for (int idx = threadIdx.x; idx < b.m * R; idx += 256) {
+    const int ii = idx / R, col = idx - ii * R;
+    int t = 0;
+    while (t + 1 < KER::NTERM && col >= r0[t + 1]) t++;
+    const TermC tm = KER::term(t);
+    const int l = col - r0[t];
+    const T* Uc = wsU + tk.uoff[tm.kc] + (size_t)l * tk.np;
+    T s = zero_<T>();
+    for (int k2 = rmo[ii]; k2 < rmo[ii + 1]; k2++) {
+      const int ent = rmo[b.m + 1 + k2];
+      const int a = ent >> 2, kl = ent & 3, e = PI[a];
+      double c9[9];
+      if (EI::DIM > 1)
+        for (int c = 0; c < 9; c++) c9[c] = curl[(size_t)e * 9 + c];
+      for (int q = 0; q < ntq; q++) {
+        const double lam[3] = {trig[3 * q], trig[3 * q + 1], 1.0 - trig[3 * q] - trig[3 * q + 1]};
+        double sh[EI::NL * EI::DIM];
+        EI::shape(lam, c9, sh);
+        const double w = quad[((size_t)e * ntq + q) * 4 + 3];
+        s += rmul(Uc[a * ntq + q], sh[kl * EI::DIM + tm.tec] * w);
+      }
+    }
+    oU[tk.ou + (size_t)ii * R + col] = rmul(s, tm.fac);
+  }
+  // V2: R x n
+  for (int idx = threadIdx.x; idx < b.n * R; idx += 256) {
+    const int col = idx / b.n, jj = idx - col * b.n;
+    int t = 0;
+    while (t + 1 < KER::NTERM && col >= r0[t + 1]) t++;
+    const TermC tm = KER::term(t);
+    const int l = col - r0[t];
+    const T* Vc = wsV + tk.voff[tm.kc] + (size_t)l * tk.nq;
+    T s = zero_<T>();
+    for (int k2 = cmo[jj]; k2 < cmo[jj + 1]; k2++) {
+      const int ent = cmo[b.n + 1 + k2];
+      const int a = ent >> 2, kl = ent & 3, e = PJ[a];
+      double c9[9];
+      if (EJ::DIM > 1)
+        for (int c = 0; c < 9; c++) c9[c] = curl[(size_t)e * 9 + c];
+      for (int q = 0; q < ntq; q++) {
+        const double lam[3] = {trig[3 * q], trig[3 * q + 1], 1.0 - trig[3 * q] - trig[3 * q + 1]};
+        double sh[EJ::NL * EJ::DIM];
+        EJ::shape(lam, c9, sh);
+        const double w = quad[((size_t)e * ntq + q) * 4 + 3];
+        s += rmul(Vc[a * ntq + q], sh[kl * EJ::DIM + tm.trc] * w);
+      }
+    }
+    oV[tk.ov + (size_t)col * b.n + jj] = s;
+  }
+}
+
+// ---------------------------------------------------------------- matvec --
+
+template <class VT, class XT>
+__global__ void k_gather(const int* __restrict__ perm, int n, const XT* __restrict__ x, VT* __restrict__ xp, int conj) {
+  const int i = blockIdx.x * 256 + threadIdx.x;
+  if (i >= n) return;
+  XT v = x[perm[i]];
+  if (conj) v = cconj(v);
+  cvt(xp[i], v);
+}
+
+template <class YT, class VT>
+__global__ void k_scatter(const int* __restrict__ perm, int n, const VT* __restrict__ yp, YT* __restrict__ y,
+                          YT alpha, YT beta, int beta_zero, int conj) {
+  const int i = blockIdx.x * 256 + threadIdx.x;
+  if (i >= n) return;
+  YT v;
+  cvt(v, yp[i]);
+  if (conj) v = cconj(v);
+  const int d = perm[i];
+  YT r = alpha * v;
+  if (!beta_zero) r += beta * y[d];
+  y[d] = r;
+}
+
+// t_b = V_b x[c0 : c0+n]  (forward)  or  t_b = U_b^T x[r0 : r0+m] (transpose)
+template <class ST, class VT, bool TRANS, bool CONJ>
+__global__ __launch_bounds__(256) void k_mv_far_t(const FarBlk* __restrict__ fb, const ST* __restrict__ fU,
+                                                  const ST* __restrict__ fV, const VT* __restrict__ xp,
+                                                  VT* __restrict__ tbuf) {
+  const FarBlk b = fb[blockIdx.x];
+  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+  if (!TRANS) {
+    const ST* V = fV + b.voff;
+    for (int l = wv; l < b.R; l += 4) {
+      VT acc = zero_<VT>();
+      const ST* row = V + (size_t)l * b.n;
+      for (int j = lane; j < b.n; j += 64) {
+        VT a;
+        cvt(a, row[j]);
+        acc += a * xp[b.c0 + j];
+      }
+      acc = wave_sum(acc);
+      if (lane == 0) tbuf[b.toff + l] = acc;
+    }
+  } else {
+    // lanes over rank columns, waves over row ranges; U is m x R row-major
+    __shared__ VT sh[4][64];
+    const ST* U = fU + b.uoff;
+    for (int l0 = 0; l0 < b.R; l0 += 64) {
+      const int l = l0 + lane;
+      VT acc = zero_<VT>();
+      if (l < b.R)
+        for (int i = wv; i < b.m; i += 4) {
+          VT a;
+          cvt(a, U[(size_t)i * b.R + l]);
+          if (CONJ) a = cconj(a);
+          acc += a * xp[b.r0 + i];
+        }
+      sh[wv][lane] = acc;
+      __syncthreads();
+      if (wv == 0 && l < b.R) tbuf[b.toff + l] = sh[0][lane] + sh[1][lane] + sh[2][lane] + sh[3][lane];
+      __syncthreads();
+    }
+  }
+}
+
+struct Tile {
+  int p0, len;       // positions
+  int s0, s1;        // segment range
+};
+
+// forward: one workgroup per leaf row cluster; each wave owns rows, lanes run
+// along the row of every covering block
+template <class ST, class VT>
+__global__ __launch_bounds__(256) void k_mv_rows(const Tile* __restrict__ tiles, const Seg* __restrict__ segs,
+                                                 const NearBlk* __restrict__ nbk, const FarBlk* __restrict__ fbk,
+                                                 const ST* __restrict__ nval, const ST* __restrict__ fU,
+                                                 const VT* __restrict__ xp, const VT* __restrict__ tbuf,
+                                                 VT* __restrict__ yp) {
+  const Tile tl = tiles[blockIdx.x];
+  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+  for (int r = wv; r < tl.len; r += 4) {
+    VT acc = zero_<VT>();
+    for (int s = tl.s0; s < tl.s1; s++) {
+      const Seg sg = segs[s];
+      if (!sg.far) {
+        const NearBlk b = nbk[sg.blk];
+        const ST* row = nval + b.off + (int64_t)(sg.off + r) * b.n;
+        const VT* xx = xp + b.c0;
+        int j = lane;
+        for (; j + 192 < b.n; j += 256) {
+          VT a0, a1, a2, a3;
+          cvt(a0, row[j]);
+          cvt(a1, row[j + 64]);
+          cvt(a2, row[j + 128]);
+          cvt(a3, row[j + 192]);
+          acc += a0 * xx[j];
+          acc += a1 * xx[j + 64];
+          acc += a2 * xx[j + 128];
+          acc += a3 * xx[j + 192];
+        }
+        for (; j < b.n; j += 64) {
+          VT a;
+          cvt(a, row[j]);
+          acc += a * xx[j];
+        }
+      } else {
+        const FarBlk b = fbk[sg.blk];
+        const ST* row = fU + b.uoff + (int64_t)(sg.off + r) * b.R;
+        for (int l = lane; l < b.R; l += 64) {
+          VT a;
+          cvt(a, row[l]);
+          acc += a * tbuf[b.toff + l];
+        }
+      }
+    }
+    acc = wave_sum(acc);
+    if (lane == 0) yp[tl.p0 + r] = acc;
+  }
+}
+
+// transpose: one workgroup per leaf column cluster; lanes own columns, waves
+// split the rows of each covering block and combine through LDS
+template <class ST, class VT, bool CONJ>
+__global__ __launch_bounds__(256) void k_mv_cols(const Tile* __restrict__ tiles, const Seg* __restrict__ segs,
+                                                 const NearBlk* __restrict__ nbk, const FarBlk* __restrict__ fbk,
+                                                 const ST* __restrict__ nval, const ST* __restrict__ fV,
+                                                 const VT* __restrict__ xp, const VT* __restrict__ tbuf,
+                                                 VT* __restrict__ yp) {
+  __shared__ VT sh[4][64];
+  const Tile tl = tiles[blockIdx.x];
+  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+  for (int c0 = 0; c0 < tl.len; c0 += 64) {
+    const int c = c0 + lane;
+    VT acc = zero_<VT>();
+    if (c < tl.len)
+      for (int s = tl.s0; s < tl.s1; s++) {
+        const Seg sg = segs[s];
+        if (!sg.far) {
+          const NearBlk b = nbk[sg.blk];
+          const ST* col = nval + b.off + sg.off + c;
+          for (int i = wv; i < b.m; i += 4) {
+            VT a;
+            cvt(a, col[(int64_t)i * b.n]);
+            if (CONJ) a = cconj(a);
+            acc += a * xp[b.r0 + i];
+          }
+        } else {
+          const FarBlk b = fbk[sg.blk];
+          const ST* col = fV + b.voff + sg.off + c;
+          for (int l = wv; l < b.R; l += 4) {
+            VT a;
+            cvt(a, col[(int64_t)l * b.n]);
+            if (CONJ) a = cconj(a);
+            acc += a * tbuf[b.toff + l];
+          }
+        }
+      }
+    sh[wv][lane] = acc;
+    __syncthreads();
+    if (wv == 0 && c < tl.len) yp[tl.p0 + c] = sh[0][lane] + sh[1][lane] + sh[2][lane] + sh[3][lane];
+    __syncthreads();
+  }
+}
+
+template <class S, class D>
+__global__ void k_convert(const S* __restrict__ a, D* __restrict__ b, int64_t n) {
+  for (int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x; i < n; i += (int64_t)gridDim.x * 256) cvt(b[i], a[i]);
+}
+
+// --------------------------------------------------------------- dispatch --
+
+static double now() {
+  return std::chrono::duration<double>(std::chrono::steady_clock::now().time_since_epoch()).count();
+}
+
+template <class F>
+static void dispatch_kernel(int id, double kappa, F&& f) {
+  switch (id) {
+    case K_LSL: f(KLapSL{kappa}); break;
+    case K_LDL: f(KLapDL{kappa}); break;
+    case K_LHS: f(KLapHS{{kappa}}); break;
+    case K_HSL: f(KHelSL{kappa}); break;
+    case K_HDL: f(KHelDL{kappa}); break;
+    case K_HCF: f(KHelCF{kappa}); break;
+    case K_HHS: f(KHelHS{kappa}); break;
+    default: throw std::runtime_error("unknown kernel id");
+  }
+}
+
+// call f(KER, EI, EJ) for the operator's kernel and evaluator pair
+template <class F>
+static void dispatch_full(const Operator& op, F&& f) {
+  const int st = op.test->kind, sr = op.trial->kind;
+  dispatch_kernel(op.kernel, op.kappa, [&](auto ker) {
+    using KER = decltype(ker);
+    if constexpr (std::is_same<KER, KLapHS>::value) {
+      f(ker, Ev<SP_P1, EV_ROT>{}, Ev<SP_P1, EV_ROT>{});
+    } else if constexpr (std::is_same<KER, KHelHS>::value) {
+      f(ker, Ev<SP_P1, EV_HELM>{}, Ev<SP_P1, EV_HELM>{});
+    } else {
+      if (st == SP_P0 && sr == SP_P0) f(ker, Ev<SP_P0, EV_ID>{}, Ev<SP_P0, EV_ID>{});
+      else if (st == SP_P0 && sr == SP_P1) f(ker, Ev<SP_P0, EV_ID>{}, Ev<SP_P1, EV_ID>{});
+      else if (st == SP_P1 && sr == SP_P0) f(ker, Ev<SP_P1, EV_ID>{}, Ev<SP_P0, EV_ID>{});
+      else f(ker, Ev<SP_P1, EV_ID>{}, Ev<SP_P1, EV_ID>{});
+    }
+  });
+}
+
+// ------------------------------------------------------------------ setup --
+
+void setup_device(Operator& op, hipStream_t s) {
+  const Mesh& m = *op.mesh;
+  const int nt = m.nt, ntq = op.trig.size();
+  std::vector<double> geo((size_t)nt * GEO, 0.0), quad((size_t)nt * ntq * 4), curl((size_t)nt * 9);
+  std::vector<int> vid((size_t)nt * 4, 0);
+  for (int e = 0; e < nt; e++) {
+    for (int k = 0; k < 3; k++) {
+      const int v = m.tri[3 * e + k];
+      vid[4 * e + k] = v;
+      for (int c = 0; c < 3; c++) geo[(size_t)e * GEO + 3 * k + c] = m.xyz[3 * v + c];
+    }
+    for (int c = 0; c < 3; c++) geo[(size_t)e * GEO + 9 + c] = m.nrm[3 * e + c];
+    geo[(size_t)e * GEO + 12] = m.J[e];
+    for (int q = 0; q < ntq; q++) {
+      double x[3];
+      m.point(e, op.trig.xi[q], op.trig.eta[q], x);
+      double* o = &quad[((size_t)e * ntq + q) * 4];
+      o[0] = x[0]; o[1] = x[1]; o[2] = x[2];
+      o[3] = m.J[e] * op.trig.w[q];
+    }
+    double c[3][3];
+    m.curls(e, c);
+    for (int k = 0; k < 3; k++)
+      for (int d = 0; d < 3; d++) curl[(size_t)e * 9 + 3 * k + d] = c[k][d];
+  }
+  std::vector<double> trig((size_t)ntq * 3);
+  for (int q = 0; q < ntq; q++) {
+    trig[3 * q] = op.trig.xi[q];
+    trig[3 * q + 1] = op.trig.eta[q];
+    trig[3 * q + 2] = op.trig.w[q];
+  }
+  op.d_geo.upload(geo, s);
+  op.d_vid.upload(vid, s);
+  op.d_curl.upload(curl, s);
+  op.d_quad.upload(quad, s);
+  op.d_trig.upload(trig, s);
+  for (int k = 0; k < 3; k++) {
+    const Rule4& r = op.ss[k];
+    std::vector<double> rr((size_t)r.size() * 5);
+    for (int q = 0; q < r.size(); q++) {
+      rr[5 * q] = r.x0[q]; rr[5 * q + 1] = r.x1[q]; rr[5 * q + 2] = r.y0[q]; rr[5 * q + 3] = r.y1[q];
+      rr[5 * q + 4] = r.w[q];
+    }
+    op.d_ss[k].upload(rr, s);
+  }
+  op.d_perm_trial.upload(op.trial_tree.perm, s);
+  op.d_iperm_trial.upload(op.trial_tree.iperm, s);
+  op.d_perm_test.upload(op.test_tree->perm, s);
+  op.d_iperm_test.upload(op.test_tree->iperm, s);
+  BEM_HIP(hipStreamSynchronize(s));
+}
+
+// ------------------------------------------------------------- near field --
+
+static int perm_code(const int px[3], const int py[3]) {
+  return px[0] | (px[1] << 2) | (px[2] << 4) | (py[0] << 6) | (py[1] << 8) | (py[2] << 10);
+}
+
+void assemble_near(Operator& op, hipStream_t s) {
+  const double t0 = now();
+  const Mesh& m = *op.mesh;
+  const Space& te = *op.test;
+  const Space& tr = *op.trial;
+  const Tree& tt = *op.test_tree;
+  const Tree& rt = op.trial_tree;
+  const bool p1i = te.kind == SP_P1, p1j = tr.kind == SP_P1;
+  const bool direct = !p1i && !p1j;
+  // near tables + patches
+  std::vector<int> patch_all;  // P1 patches; P0 sides index the permutation
+  std::vector<int2> chunks;
+  std::vector<int4> pairs[3];
+  int64_t off = 0;
+  std::vector<int> pI, pJ, neigh;
+  std::vector<char> mark(m.nt, 0);
+  for (size_t k = 0; k < op.near.size(); k++) {
+    NearBlk& nb = op.near[k];
+    nb.off = off;
+    off += (int64_t)nb.m * nb.n;
+    if (p1i) {
+      patch(te, &tt.perm[nb.r0], nb.m, pI);
+      nb.pI = (int)patch_all.size();
+      nb.pm = (int)pI.size();
+      patch_all.insert(patch_all.end(), pI.begin(), pI.end());
+    } else {
+      pI.assign(tt.perm.begin() + nb.r0, tt.perm.begin() + nb.r0 + nb.m);
+      nb.pI = nb.r0;
+      nb.pm = nb.m;
+    }
+    if (p1j) {
+      patch(tr, &rt.perm[nb.c0], nb.n, pJ);
+      nb.pJ = (int)patch_all.size();
+      nb.pn = (int)pJ.size();
+      patch_all.insert(patch_all.end(), pJ.begin(), pJ.end());
+    } else {
+      nb.pJ = nb.c0;
+      nb.pn = nb.n;
+    }
+    const int64_t np = (int64_t)nb.pm * nb.pn;
+    if (np >= (int64_t)1 << 31) throw std::runtime_error("near block too large for one launch grid");
+    for (int64_t st = 0; st < np; st += 256) chunks.push_back(make_int2((int)k, (int)st));
+    // touching pairs among the patches
+    auto in_trial = [&](int f) {
+      if (!p1j) {
+        const int pos = rt.iperm[f];
+        return pos >= nb.c0 && pos < nb.c0 + nb.n;
+      }
+      for (int c = 0; c < 3; c++) {
+        const int pos = rt.iperm[m.tri[3 * f + c]];
+        if (pos >= nb.c0 && pos < nb.c0 + nb.n) return true;
+      }
+      return false;
+    };
+    for (int e : pI) {
+      neigh.clear();
+      for (int c = 0; c < 3; c++) {
+        const int v = m.tri[3 * e + c];
+        for (int q = m.ve_off[v]; q < m.ve_off[v + 1]; q++) {
+          const int f = m.ve_list[q];
+          if (!mark[f]) { mark[f] = 1; neigh.push_back(f); }
+        }
+      }
+      for (int f : neigh) {
+        mark[f] = 0;
+        if (!in_trial(f)) continue;
+        int px[3], py[3];
+        const int nc = classify_pair(m, e, f, px, py);
+        int cls;
+        if (nc == 3) {  // identical: reference point (x0,x1) is (xi,eta): SS vertex 0 -> local 2
+          px[0] = 2; px[1] = 0; px[2] = 1;
+          py[0] = 2; py[1] = 0; py[2] = 1;
+          cls = 0;
+        } else if (nc == 2) cls = 1;
+        else if (nc == 1) cls = 2;
+        else continue;
+        pairs[cls].push_back(make_int4((int)k, e, f, perm_code(px, py)));
+      }
+    }
+  }
+  op.near_entries = off;
+  for (int c = 0; c < 3; c++) op.n_sing[c] = (int)pairs[c].size();
+  const int ntq = op.trig.size();
+  op.near_evals = 0;
+  for (const NearBlk& nb : op.near) op.near_evals += (double)nb.pm * nb.pn * ntq * ntq;
+  for (int c = 0; c < 3; c++) op.near_evals += (double)pairs[c].size() * (op.ss[c].size() - (double)ntq * ntq);
+
+  op.d_near_blk.upload(op.near, s);
+  op.d_near_chunks.upload(chunks, s);
+  op.d_patch.upload(patch_all, s);
+  for (int c = 0; c < 3; c++) op.d_spairs[c].upload(pairs[c], s);
+  const size_t esz = op.complex_valued ? 16 : 8;
+  op.d_near_val.alloc((size_t)off * esz);
+  if (!direct && off) BEM_HIP(hipMemsetAsync(op.d_near_val.p, 0, (size_t)off * esz, s));
+  const int* pIb = p1i ? op.d_patch.as<int>() : op.d_perm_test.as<int>();
+  const int* pJb = p1j ? op.d_patch.as<int>() : op.d_perm_trial.as<int>();
+  dispatch_full(op, [&](auto ker, auto ei, auto ej) {
+    using KER = decltype(ker);
+    using EI = decltype(ei);
+    using EJ = decltype(ej);
+    using T = typename KER::T;
+    T* val = op.d_near_val.as<T>();
+    auto launch = [&](auto dflag) {
+      constexpr bool D = decltype(dflag)::value;
+      if (!chunks.empty())
+        hipLaunchKernelGGL((k_near_regular<KER, EI, EJ, D>), dim3((unsigned)chunks.size()), dim3(256), 0, s,
+                           op.d_near_blk.as<NearBlk>(), op.d_near_chunks.as<int2>(), pIb, pJb, op.d_geo.as<double>(),
+                           op.d_vid.as<int4>(), op.d_quad.as<double>(), op.d_curl.as<double>(),
+                           op.d_trig.as<double>(), ntq, op.d_iperm_test.as<int>(), op.d_iperm_trial.as<int>(), ker,
+                           val);
+      for (int c = 0; c < 3; c++) {
+        const int np = (int)pairs[c].size();
+        if (!np) continue;
+        hipLaunchKernelGGL((k_near_singular<KER, EI, EJ, D>), dim3((np + 255) / 256), dim3(256), 0, s,
+                           op.d_spairs[c].as<int4>(), np, op.d_near_blk.as<NearBlk>(), op.d_ss[c].as<double>(),
+                           op.ss[c].size(), op.d_geo.as<double>(), op.d_vid.as<int>(), op.d_curl.as<double>(),
+                           op.d_iperm_test.as<int>(), op.d_iperm_trial.as<int>(), ker, val);
+      }
+    };
+    if constexpr (!EI::P1 && !EJ::P1) launch(std::true_type{});
+    else launch(std::false_type{});
+  });
+  BEM_HIP(hipGetLastError());
+  BEM_HIP(hipStreamSynchronize(s));
+  op.t_near = now() - t0;
+}
+
+// -------------------------------------------------------------- far field --
+
+void assemble_far(Operator& op, hipStream_t s) {
+  const double t0 = now();
+  op.far_factor_entries = 0;
+  op.far_evals = 0;
+  const int nfar = (int)op.far.size();
+  if (nfar == 0) {
+    op.t_far = now() - t0;
+    return;
+  }
+  const Space& te = *op.test;
+  const Space& tr = *op.trial;
+  const Tree& tt = *op.test_tree;
+  const Tree& rt = op.trial_tree;
+  const int ntq = op.trig.size();
+  const int ncomp = op.kinfo.ncomp;
+  // sorted patches (ACA point order) and projection maps
+  std::vector<int> fpatch, maps, pI, pJ, dofs;
+  auto build_map = [&](const Space& sp, const Tree& tree, int p0, int len, const std::vector<int>& P) {
+    const int base = (int)maps.size();
+    maps.resize(base + len + 1);
+    std::vector<int> ents;
+    for (int ii = 0; ii < len; ii++) {
+      maps[base + ii] = (int)ents.size();
+      const int d = tree.perm[p0 + ii];
+      for (int q = sp.ed_off[d]; q < sp.ed_off[d + 1]; q++) {
+        const int e = sp.ed_list[q];
+        const int a = (int)(std::lower_bound(P.begin(), P.end(), e) - P.begin());
+        int kl = 0;
+        for (int l = 0; l < sp.nloc; l++)
+          if (sp.dofs[e * sp.nloc + l] == d) kl = l;
+        ents.push_back(a * 4 + kl);
+      }
+    }
+    maps[base + len] = (int)ents.size();
+    maps.insert(maps.end(), ents.begin(), ents.end());
+    return base;
+  };
+  for (int k = 0; k < nfar; k++) {
+    FarBlk& fb = op.far[k];
+    patch(te, &tt.perm[fb.r0], fb.m, pI);
+    patch(tr, &rt.perm[fb.c0], fb.n, pJ);
+    fb.pI = (int)fpatch.size();
+    fb.pm = (int)pI.size();
+    fpatch.insert(fpatch.end(), pI.begin(), pI.end());
+    fb.pJ = (int)fpatch.size();
+    fb.pn = (int)pJ.size();
+    fpatch.insert(fpatch.end(), pJ.begin(), pJ.end());
+    fb.rmap = build_map(te, tt, fb.r0, fb.m, pI);
+    fb.cmap = build_map(tr, rt, fb.c0, fb.n, pJ);
+  }
+  op.d_far_patch.upload(fpatch, s);
+  op.d_far_maps.upload(maps, s);
+  op.d_far_blk.upload(op.far, s);
+
+  size_t free_b = 0, total_b = 0;
+  BEM_HIP(hipMemGetInfo(&free_b, &total_b));
+  const size_t esz = op.complex_valued ? 16 : 8;
+  const size_t budget = std::max<size_t>(std::min<size_t>(free_b / 4, (size_t)24 << 30), (size_t)256 << 20);
+
+  struct Pending {
+    int blk, cap;
+  };
+  std::vector<Pending> todo;
+  const int cap0 = 32;
+  for (int k = 0; k < nfar; k++) todo.push_back({k, cap0});
+  struct Piece {
+    DBuf U, V;
+    size_t nu = 0, nv = 0;
+  };
+  std::vector<std::unique_ptr<Piece>> pieces;
+  std::vector<int> piece_of(nfar, -1);
+  std::vector<int64_t> loc_u(nfar), loc_v(nfar);
+  op.far_rank.assign(nfar, 0);
+  DBuf wsU, wsV, d_tasks, d_res, d_ptasks;
+  while (!todo.empty()) {
+    // take a batch within the workspace budget
+    std::vector<AcaTask> tasks;
+    std::vector<Pending> batch;
+    size_t usedU = 0, usedV = 0;
+    size_t i = 0;
+    for (; i < todo.size(); i++) {
+      const FarBlk& fb = op.far[todo[i].blk];
+      const int np = fb.pm * ntq, nq = fb.pn * ntq;
+      const int cap = std::min(todo[i].cap, std::min(np, nq));
+      if (cap > 4096) throw std::runtime_error("ACA rank above 4096 requested; raise eps or lower leafsize");
+      const size_t need = (size_t)cap * (np + nq) * esz * ncomp;
+      if (!batch.empty() && (usedU + usedV) * esz + need > budget) break;
+      batch.push_back({todo[i].blk, cap});
+      for (int c = 0; c < ncomp; c++) {
+        AcaTask t{};
+        t.uoff = (int64_t)usedU;
+        t.voff = (int64_t)usedV;
+        t.blk = todo[i].blk;
+        t.comp = c;
+        t.np = np;
+        t.nq = nq;
+        t.cap = cap;
+        tasks.push_back(t);
+        usedU += (size_t)cap * np;
+        usedV += (size_t)cap * nq;
+      }
+    }
+    todo.erase(todo.begin(), todo.begin() + i);
+    wsU.alloc(std::max(usedU * esz, wsU.bytes));
+    wsV.alloc(std::max(usedV * esz, wsV.bytes));
+    d_tasks.upload(tasks, s);
+    d_res.alloc(tasks.size() * sizeof(int2));
+    int maxcap = 0;
+    for (auto& t : tasks) maxcap = std::max(maxcap, t.cap);
+    dispatch_kernel(op.kernel, op.kappa, [&](auto ker) {
+      using KER = decltype(ker);
+      using T = typename KER::T;
+      hipLaunchKernelGGL((k_aca<KER>), dim3((unsigned)tasks.size()), dim3(ACA_THREADS), maxcap * sizeof(T), s,
+                         d_tasks.as<AcaTask>(), op.d_far_blk.as<FarBlk>(), op.d_far_patch.as<int>(),
+                         op.d_quad.as<double>(), op.d_geo.as<double>(), ntq, ker, op.eps, wsU.as<T>(), wsV.as<T>(),
+                         d_res.as<int2>());
+    });
+    BEM_HIP(hipGetLastError());
+    std::vector<int2> res(tasks.size());
+    BEM_HIP(hipMemcpyAsync(res.data(), d_res.p, res.size() * sizeof(int2), hipMemcpyDeviceToHost, s));
+    BEM_HIP(hipStreamSynchronize(s));
+    // finished blocks -> projection into a batch piece; capped blocks -> retry
+    std::vector<ProjTask> ptasks;
+    auto piece = std::make_unique<Piece>();
+    for (size_t bi = 0; bi < batch.size(); bi++) {
+      const int blk = batch[bi].blk;
+      bool capped = false;
+      int rk[2] = {0, 0};
+      for (int c = 0; c < ncomp; c++) {
+        const int2 r = res[bi * ncomp + c];
+        rk[c] = r.x;
+        capped |= r.y != 0;
+      }
+      const FarBlk& fb = op.far[blk];
+      const int np = fb.pm * ntq, nq = fb.pn * ntq;
+      for (int c = 0; c < ncomp; c++) op.far_evals += (double)rk[c] * (np + nq);
+      if (capped) {
+        todo.push_back({blk, batch[bi].cap * 4});
+        continue;
+      }
+      int R = 0;
+      for (auto& t : op.kinfo.terms) R += rk[t.kc];
+      ProjTask pt{};
+      pt.blk = blk;
+      pt.ncomp = ncomp;
+      for (int c = 0; c < ncomp; c++) {
+        pt.uoff[c] = tasks[bi * ncomp + c].uoff;
+        pt.voff[c] = tasks[bi * ncomp + c].voff;
+        pt.rank[c] = rk[c];
+      }
+      pt.np = np;
+      pt.nq = nq;
+      pt.ou = (int64_t)piece->nu;
+      pt.ov = (int64_t)piece->nv;
+      piece->nu += (size_t)fb.m * R;
+      piece->nv += (size_t)R * fb.n;
+      op.far_rank[blk] = R;
+      piece_of[blk] = (int)pieces.size();
+      loc_u[blk] = pt.ou;
+      loc_v[blk] = pt.ov;
+      ptasks.push_back(pt);
+    }
+    if (!ptasks.empty()) {
+      piece->U.alloc(std::max<size_t>(piece->nu, 1) * esz);
+      piece->V.alloc(std::max<size_t>(piece->nv, 1) * esz);
+      d_ptasks.upload(ptasks, s);
+      dispatch_full(op, [&](auto ker, auto ei, auto ej) {
+        using KER = decltype(ker);
+        using T = typename KER::T;
+        hipLaunchKernelGGL((k_project<KER, decltype(ei), decltype(ej)>), dim3((unsigned)ptasks.size()), dim3(256), 0,
+                           s, d_ptasks.as<ProjTask>(), op.d_far_blk.as<FarBlk>(), op.d_far_patch.as<int>(),
+                           op.d_far_maps.as<int>(), op.d_quad.as<double>(), op.d_curl.as<double>(),
+                           op.d_trig.as<double>(), ntq, wsU.as<T>(), wsV.as<T>(), piece->U.as<T>(), piece->V.as<T>());
+      });
+      BEM_HIP(hipGetLastError());
+      BEM_HIP(hipStreamSynchronize(s));
+      pieces.push_back(std::move(piece));
+    }
+  }
+  wsU.release();
+  wsV.release();
+  // concatenate pieces into the final factor arrays
+  size_t tu = 0, tv = 0;
+  std::vector<size_t> bu(pieces.size()), bv(pieces.size());
+  for (size_t p = 0; p < pieces.size(); p++) {
+    bu[p] = tu;
+    bv[p] = tv;
+    tu += pieces[p]->nu;
+    tv += pieces[p]->nv;
+  }
+  op.d_fU.alloc(std::max<size_t>(tu, 1) * esz);
+  op.d_fV.alloc(std::max<size_t>(tv, 1) * esz);
+  for (size_t p = 0; p < pieces.size(); p++) {
+    if (pieces[p]->nu)
+      BEM_HIP(hipMemcpyAsync((char*)op.d_fU.p + bu[p] * esz, pieces[p]->U.p, pieces[p]->nu * esz,
+                             hipMemcpyDeviceToDevice, s));
+    if (pieces[p]->nv)
+      BEM_HIP(hipMemcpyAsync((char*)op.d_fV.p + bv[p] * esz, pieces[p]->V.p, pieces[p]->nv * esz,
+                             hipMemcpyDeviceToDevice, s));
+  }
+  BEM_HIP(hipStreamSynchronize(s));
+  pieces.clear();
+  int toff = 0;
+  for (int k = 0; k < nfar; k++) {
+    FarBlk& fb = op.far[k];
+    fb.R = op.far_rank[k];
+    fb.uoff = (int64_t)bu[piece_of[k]] + loc_u[k];
+    fb.voff = (int64_t)bv[piece_of[k]] + loc_v[k];
+    fb.toff = toff;
+    toff += fb.R;
+    op.far_factor_entries += (int64_t)fb.R * (fb.m + fb.n);
+  }
+  op.t_total = toff;
+  op.d_far_blk.upload(op.far, s);
+  BEM_HIP(hipStreamSynchronize(s));
+  op.t_far = now() - t0;
+}
+
+// ---------------------------------------------------------- matvec plans --
+
+static void plan(const Operator& op, bool trans, std::vector<Tile>& tiles, std::vector<Seg>& segs) {
+  const Tree& tree = trans ? op.trial_tree : *op.test_tree;
+  const int nleaf = (int)tree.leaves.size();
+  std::vector<int> lp0(nleaf);
+  for (int i = 0; i < nleaf; i++) lp0[i] = (int)tree.cl[tree.leaves[i]].permupos;
+  std::vector<std::vector<Seg>> per(nleaf);
+  auto add = [&](int slot, int far, int p0, int len) {
+    int li = (int)(std::upper_bound(lp0.begin(), lp0.end(), p0) - lp0.begin()) - 1;
+    while (li < nleaf && lp0[li] < p0 + len) {
+      per[li].push_back(Seg{slot, far, lp0[li] - p0, 0});
+      li++;
+    }
+  };
+  for (size_t b = 0; b < op.blocks.size(); b++) {
+    const int slot = op.blk_slot[b];
+    if (op.blocks[b].near || op.blocks[b].fallback) {
+      const NearBlk& nb = op.near[slot];
+      if (trans) add(slot, 0, nb.c0, nb.n);
+      else add(slot, 0, nb.r0, nb.m);
+    } else {
+      const FarBlk& fb = op.far[slot];
+      if (trans) add(slot, 1, fb.c0, fb.n);
+      else add(slot, 1, fb.r0, fb.m);
+    }
+  }
+  tiles.clear();
+  segs.clear();
+  for (int i = 0; i < nleaf; i++) {
+    const Cluster& c = tree.cl[tree.leaves[i]];
+    Tile t{(int)c.permupos, (int)c.number, (int)segs.size(), 0};
+    segs.insert(segs.end(), per[i].begin(), per[i].end());
+    t.s1 = (int)segs.size();
+    tiles.push_back(t);
+  }
+}
+
+void build_matvec_plan(Operator& op, hipStream_t s) {
+  std::vector<Tile> tiles;
+  std::vector<Seg> segs;
+  plan(op, false, tiles, segs);
+  op.d_tile_fwd.upload(tiles, s);
+  op.d_seg_fwd.upload(segs, s);
+  op.n_tiles_fwd = (int)tiles.size();
+  plan(op, true, tiles, segs);
+  op.d_tile_trn.upload(tiles, s);
+  op.d_seg_trn.upload(segs, s);
+  op.n_tiles_trn = (int)tiles.size();
+  const size_t esz = op.complex_valued ? 16 : 8;
+  const int nmax = std::max(op.ndof_trial(), op.ndof_test());
+  op.d_xp.alloc((size_t)nmax * esz);
+  op.d_yp.alloc((size_t)nmax * esz);
+  op.d_t.alloc((size_t)std::max(op.t_total, 1) * esz);
+  BEM_HIP(hipStreamSynchronize(s));
+}
+
+void make_mixed_copy(Operator& op, hipStream_t s) {
+  const size_t n = (size_t)op.near_entries, nu = 0;
+  (void)nu;
+  auto conv = [&](DBuf& src, DBuf& dst, size_t count) {
+    if (op.complex_valued) {
+      dst.alloc(std::max<size_t>(count, 1) * sizeof(fc));
+      if (count)
+        hipLaunchKernelGGL((k_convert<zc, fc>), dim3(4096), dim3(256), 0, s, src.as<zc>(), dst.as<fc>(),
+                           (int64_t)count);
+    } else {
+      dst.alloc(std::max<size_t>(count, 1) * sizeof(float));
+      if (count)
+        hipLaunchKernelGGL((k_convert<double, float>), dim3(4096), dim3(256), 0, s, src.as<double>(), dst.as<float>(),
+                           (int64_t)count);
+    }
+  };
+  conv(op.d_near_val, op.d_near_val32, n);
+  size_t tu = 0, tv = 0;
+  for (const FarBlk& fb : op.far) {
+    tu = std::max(tu, (size_t)(fb.uoff + (int64_t)fb.m * fb.R));
+    tv = std::max(tv, (size_t)(fb.voff + (int64_t)fb.R * fb.n));
+  }
+  conv(op.d_fU, op.d_fU32, tu);
+  conv(op.d_fV, op.d_fV32, tv);
+  BEM_HIP(hipGetLastError());
+  BEM_HIP(hipStreamSynchronize(s));
+}
+
+// ----------------------------------------------------------------- apply --
+
+template <class ST, class VT, class XT>
+static void apply_t(Operator& op, int flags, const XT* x, XT* y, XT alpha, XT beta, bool beta_zero, const ST* nval,
+                    const ST* fU, const ST* fV, hipStream_t s, hipEvent_t* ev) {
+  auto mark = [&](int i) {
+    if (ev) BEM_HIP(hipEventRecord(ev[i], s));
+  };
+  const bool trans = flags & 1, conj = (flags & 2) != 0;
+  VT* xp = op.d_xp.as<VT>();
+  VT* yp = op.d_yp.as<VT>();
+  VT* tb = op.d_t.as<VT>();
+  const int nin = trans ? op.ndof_test() : op.ndof_trial();
+  const int nout = trans ? op.ndof_trial() : op.ndof_test();
+  const int* pin = trans ? op.d_perm_test.as<int>() : op.d_perm_trial.as<int>();
+  const int* pout = trans ? op.d_perm_trial.as<int>() : op.d_perm_test.as<int>();
+  const int nfar = (int)op.far.size();
+  mark(0);
+  hipLaunchKernelGGL((k_gather<VT, XT>), dim3((nin + 255) / 256), dim3(256), 0, s, pin, nin, x, xp, 0);
+  mark(1);
+  if (!trans) {
+    if (nfar)
+      hipLaunchKernelGGL((k_mv_far_t<ST, VT, false, false>), dim3(nfar), dim3(256), 0, s, op.d_far_blk.as<FarBlk>(),
+                         fU, fV, xp, tb);
+    mark(2);
+    hipLaunchKernelGGL((k_mv_rows<ST, VT>), dim3(op.n_tiles_fwd), dim3(256), 0, s, op.d_tile_fwd.as<Tile>(),
+                       op.d_seg_fwd.as<Seg>(), op.d_near_blk.as<NearBlk>(), op.d_far_blk.as<FarBlk>(), nval, fU, xp,
+                       tb, yp);
+  } else if (!conj) {
+    if (nfar)
+      hipLaunchKernelGGL((k_mv_far_t<ST, VT, true, false>), dim3(nfar), dim3(256), 0, s, op.d_far_blk.as<FarBlk>(),
+                         fU, fV, xp, tb);
+    mark(2);
+    hipLaunchKernelGGL((k_mv_cols<ST, VT, false>), dim3(op.n_tiles_trn), dim3(256), 0, s, op.d_tile_trn.as<Tile>(),
+                       op.d_seg_trn.as<Seg>(), op.d_near_blk.as<NearBlk>(), op.d_far_blk.as<FarBlk>(), nval, fV, xp,
+                       tb, yp);
+  } else {
+    if (nfar)
+      hipLaunchKernelGGL((k_mv_far_t<ST, VT, true, true>), dim3(nfar), dim3(256), 0, s, op.d_far_blk.as<FarBlk>(),
+                         fU, fV, xp, tb);
+    mark(2);
+    hipLaunchKernelGGL((k_mv_cols<ST, VT, true>), dim3(op.n_tiles_trn), dim3(256), 0, s, op.d_tile_trn.as<Tile>(),
+                       op.d_seg_trn.as<Seg>(), op.d_near_blk.as<NearBlk>(), op.d_far_blk.as<FarBlk>(), nval, fV, xp,
+                       tb, yp);
+  }
+  mark(3);
+  hipLaunchKernelGGL((k_scatter<XT, VT>), dim3((nout + 255) / 256), dim3(256), 0, s, pout, nout, yp, y, alpha, beta,
+                     (int)beta_zero, 0);
+  mark(4);
+}
+
+void apply_device(Operator& op, int flags, const double* alpha, const void* x, const double* beta, void* y,
+                  hipStream_t s, hipEvent_t* ev) {
+  if (!op.assembled) throw std::runtime_error("operator not assembled");
+  const bool mixed = (flags & 4) != 0;
+  if (mixed && !op.mixed) throw std::runtime_error("operator was created without the mixed-precision copy");
+  const bool beta_zero = beta[0] == 0.0 && beta[1] == 0.0;
+  if (op.complex_valued) {
+    const zc a{alpha[0], alpha[1]}, b{beta[0], beta[1]};
+    if (!mixed)
+      apply_t<zc, zc, zc>(op, flags, (const zc*)x, (zc*)y, a, b, beta_zero, op.d_near_val.as<zc>(), op.d_fU.as<zc>(),
+                          op.d_fV.as<zc>(), s, ev);
+    else
+      apply_t<fc, fc, zc>(op, flags, (const zc*)x, (zc*)y, a, b, beta_zero, op.d_near_val32.as<fc>(),
+                          op.d_fU32.as<fc>(), op.d_fV32.as<fc>(), s, ev);
+  } else {
+    if (alpha[1] != 0.0 || beta[1] != 0.0) throw std::runtime_error("complex scalars on a real operator");
+    if (!mixed)
+      apply_t<double, double, double>(op, flags & 1, (const double*)x, (double*)y, alpha[0], beta[0], beta_zero,
+                                      op.d_near_val.as<double>(), op.d_fU.as<double>(), op.d_fV.as<double>(), s, ev);
+    else
+      apply_t<float, float, double>(op, flags & 1, (const double*)x, (double*)y, alpha[0], beta[0], beta_zero,
+                                    op.d_near_val32.as<float>(), op.d_fU32.as<float>(), op.d_fV32.as<float>(), s, ev);
+  }
+  BEM_HIP(hipGetLastError());
+}
+
+}  // namespace bem

@@ -1,0 +1,512 @@
+// Host side of the MI355X BEM library: geometry, rules, cluster tree, block
+// partition.  Reference behaviour restated from Weggler/ngbem:
+//   rules      src/intrules.cpp:13-136 (Sauter-Schwab), NGSolve ET_SEGM/ET_HEX/ET_TRIG
+//              replaced by Gauss-Legendre / tensor / named triangle rules (below)
+//   tree       src/hmat.cpp:13-150 (cluster data), 155-374 (PCA bisection), 379-453
+//   partition  src/hmat.cpp:457-471 (admissibility), 587-621 (block recursion)
+//   pairs      src/ngbem.cpp:527-531, 569-586, 627-642 (classification, permutations)
+#include "bem_host.hpp"
+
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+
+namespace bem {
+
+// ------------------------------------------------------------------ mesh --
+
+void Mesh::setup() {
+  nrm.assign(3 * (size_t)nt, 0.0);
+  J.assign(nt, 0.0);
+  std::vector<int> cnt(nv + 1, 0);
+  for (int e = 0; e < nt; e++) {
+    const double* p0 = &xyz[3 * tri[3 * e + 0]];
+    const double* p1 = &xyz[3 * tri[3 * e + 1]];
+    const double* p2 = &xyz[3 * tri[3 * e + 2]];
+    double a[3] = {p0[0] - p2[0], p0[1] - p2[1], p0[2] - p2[2]};
+    double b[3] = {p1[0] - p2[0], p1[1] - p2[1], p1[2] - p2[2]};
+    double c[3] = {a[1] * b[2] - a[2] * b[1], a[2] * b[0] - a[0] * b[2], a[0] * b[1] - a[1] * b[0]};
+    double len = std::sqrt(c[0] * c[0] + c[1] * c[1] + c[2] * c[2]);
+    J[e] = len;
+    for (int k = 0; k < 3; k++) nrm[3 * e + k] = c[k] / len;
+    for (int k = 0; k < 3; k++) cnt[tri[3 * e + k] + 1]++;
+  }
+  ve_off.assign(nv + 1, 0);
+  for (int v = 0; v < nv; v++) ve_off[v + 1] = ve_off[v] + cnt[v + 1];
+  ve_list.assign(ve_off[nv], 0);
+  std::vector<int> fill(ve_off.begin(), ve_off.end() - 1);
+  for (int e = 0; e < nt; e++)
+    for (int k = 0; k < 3; k++) ve_list[fill[tri[3 * e + k]]++] = e;
+}
+
+void Mesh::point(int e, double xi, double eta, double* x) const {
+  const double* p0 = &xyz[3 * tri[3 * e + 0]];
+  const double* p1 = &xyz[3 * tri[3 * e + 1]];
+  const double* p2 = &xyz[3 * tri[3 * e + 2]];
+  double l2 = 1.0 - xi - eta;
+  for (int k = 0; k < 3; k++) x[k] = xi * p0[k] + eta * p1[k] + l2 * p2[k];
+}
+
+// n x grad_Gamma(lambda_k) for the flat element (DiffOpBoundaryRot,
+// src/diffops.hpp:12-57): grad = [a b] G^{-1} grad_ref, a = p0-p2, b = p1-p2
+void Mesh::curls(int e, double c[3][3]) const {
+  const double* p0 = &xyz[3 * tri[3 * e + 0]];
+  const double* p1 = &xyz[3 * tri[3 * e + 1]];
+  const double* p2 = &xyz[3 * tri[3 * e + 2]];
+  double a[3], b[3];
+  for (int k = 0; k < 3; k++) { a[k] = p0[k] - p2[k]; b[k] = p1[k] - p2[k]; }
+  double aa = a[0] * a[0] + a[1] * a[1] + a[2] * a[2];
+  double ab = a[0] * b[0] + a[1] * b[1] + a[2] * b[2];
+  double bb = b[0] * b[0] + b[1] * b[1] + b[2] * b[2];
+  double det = aa * bb - ab * ab;
+  double i00 = bb / det, i01 = -ab / det, i11 = aa / det;
+  double g[3][3];
+  for (int k = 0; k < 3; k++) {
+    g[0][k] = i00 * a[k] + i01 * b[k];
+    g[1][k] = i01 * a[k] + i11 * b[k];
+    g[2][k] = -g[0][k] - g[1][k];
+  }
+  const double* n = &nrm[3 * e];
+  for (int k = 0; k < 3; k++) {
+    c[k][0] = n[1] * g[k][2] - n[2] * g[k][1];
+    c[k][1] = n[2] * g[k][0] - n[0] * g[k][2];
+    c[k][2] = n[0] * g[k][1] - n[1] * g[k][0];
+  }
+}
+
+void Space::setup(const Mesh* m, int k) {
+  mesh = m;
+  kind = k;
+  if (k == SP_P0) {
+    ndof = m->nt;
+    nloc = 1;
+    dofs.resize(m->nt);
+    for (int e = 0; e < m->nt; e++) dofs[e] = e;
+  } else {
+    ndof = m->nv;
+    nloc = 3;
+    dofs.assign(m->tri.begin(), m->tri.end());
+  }
+  std::vector<int> cnt(ndof + 1, 0);
+  for (int e = 0; e < m->nt; e++)
+    for (int l = 0; l < nloc; l++) cnt[dofs[e * nloc + l] + 1]++;
+  ed_off.assign(ndof + 1, 0);
+  for (int d = 0; d < ndof; d++) ed_off[d + 1] = ed_off[d] + cnt[d + 1];
+  ed_list.assign(ed_off[ndof], 0);
+  std::vector<int> fill(ed_off.begin(), ed_off.end() - 1);
+  for (int e = 0; e < m->nt; e++)
+    for (int l = 0; l < nloc; l++) ed_list[fill[dofs[e * nloc + l]]++] = e;
+}
+
+// ----------------------------------------------------------------- rules --
+
+void gauss01(int n, std::vector<double>& x, std::vector<double>& w) {
+  x.assign(n, 0.0);
+  w.assign(n, 0.0);
+  auto legendre = [n](double z, double& pn, double& pn1) {
+    double p0 = 1.0, p1 = z;
+    for (int k = 2; k <= n; k++) {
+      double p2 = ((2.0 * k - 1.0) * z * p1 - (k - 1.0) * p0) / k;
+      p0 = p1;
+      p1 = p2;
+    }
+    pn = p1;
+    pn1 = p0;
+  };
+  for (int i = 0; i < n; i++) {
+    double z = std::cos(kPi * (i + 0.75) / (n + 0.5));
+    for (int it = 0; it < 100; it++) {
+      double pn, pn1;
+      legendre(z, pn, pn1);
+      double dp = n * (z * pn - pn1) / (z * z - 1.0);
+      double dz = pn / dp;
+      z -= dz;
+      if (std::fabs(dz) < 1e-17 && it > 2) break;
+    }
+    double pn, pn1;
+    legendre(z, pn, pn1);
+    double dp = n * (z * pn - pn1) / (z * z - 1.0);
+    x[i] = 0.5 * (1.0 - z);
+    w[i] = 1.0 / ((1.0 - z * z) * dp * dp);
+  }
+}
+
+// src/intrules.cpp:13-136; the ET_SEGM rule of order p is taken as the
+// (p/2+1)-point Gauss-Legendre rule, ET_HEX as its tensor cube.
+Rule4 sauter_schwab(int kind, int order) {
+  std::vector<double> s, sw;
+  gauss01(order / 2 + 1, s, sw);
+  int n = (int)s.size();
+  Rule4 r;
+  auto add = [&r](double a0, double a1, double a2, double a3, double xi, double w) {
+    a0 *= xi; a1 *= xi; a2 *= xi; a3 *= xi;
+    r.x0.push_back(a0 - a1);
+    r.x1.push_back(a1);
+    r.y0.push_back(a2 - a3);
+    r.y1.push_back(a3);
+    r.w.push_back(w);
+  };
+  for (int i1 = 0; i1 < n; i1++)
+    for (int i2 = 0; i2 < n; i2++)
+      for (int i3 = 0; i3 < n; i3++)
+        for (int i4 = 0; i4 < n; i4++) {
+          const double e1 = s[i1], e2 = s[i2], e3 = s[i3], xi = s[i4];
+          const double wt = sw[i1] * sw[i2] * sw[i3] * sw[i4];
+          const double x3 = xi * xi * xi;
+          if (kind == 0) {  // identical panel: 6 Duffy simplices
+            double w = x3 * e1 * e1 * e2 * wt;
+            add(1, 1 - e1 + e1 * e2, 1 - e1 * e2 * e3, 1 - e1, xi, w);
+            add(1 - e1 * e2 * e3, 1 - e1, 1, 1 - e1 + e1 * e2, xi, w);
+            add(1, e1 * (1 - e2 + e2 * e3), 1 - e1 * e2, e1 * (1 - e2), xi, w);
+            add(1 - e1 * e2, e1 * (1 - e2), 1, e1 * (1 - e2 + e2 * e3), xi, w);
+            add(1 - e1 * e2 * e3, e1 * (1 - e2 * e3), 1, e1 * (1 - e2), xi, w);
+            add(1, e1 * (1 - e2), 1 - e1 * e2 * e3, e1 * (1 - e2 * e3), xi, w);
+          } else if (kind == 1) {  // common edge: 5 simplices
+            double w0 = x3 * e1 * e1 * wt, w1 = x3 * e1 * e1 * e2 * wt;
+            add(1, e1 * e3, 1 - e1 * e2, e1 * (1 - e2), xi, w0);
+            add(1, e1, 1 - e1 * e2 * e3, e1 * e2 * (1 - e3), xi, w1);
+            add(1 - e1 * e2, e1 * (1 - e2), 1, e1 * e2 * e3, xi, w1);
+            add(1 - e1 * e2 * e3, e1 * e2 * (1 - e3), 1, e1, xi, w1);
+            add(1 - e1 * e2 * e3, e1 * (1 - e2 * e3), 1, e1 * e2, xi, w1);
+          } else {  // common vertex: 2 simplices
+            double w = x3 * e2 * wt;
+            add(1, e1, e2, e2 * e3, xi, w);
+            add(e2, e2 * e3, 1, e1, xi, w);
+          }
+        }
+  return r;
+}
+
+static void orbit3(TrigRule& r, double a, double wn) {
+  const double b = 1.0 - 2.0 * a;
+  const double pts[3][2] = {{b, a}, {a, b}, {a, a}};
+  for (auto& p : pts) {
+    r.xi.push_back(p[0]);
+    r.eta.push_back(p[1]);
+    r.w.push_back(0.5 * wn);
+  }
+}
+
+// Dunavant (1985) 6-point degree-4 rule.  The table's 15-digit values seed a
+// Newton solve of the S3-invariant moment equations: sum over the 6 points of
+// {1, sum l^2, sum l^3, sum l^4} must equal {1, 1/2, 3/10, 1/5}.
+static void dunavant_deg4(double& a, double& wa, double& b, double& wb) {
+  a = 0.445948490915965; wa = 0.223381589678011;
+  b = 0.091576213509771; wb = 0.109951743655322;
+  const double target[4] = {1.0, 0.5, 0.3, 0.2};
+  for (int it = 0; it < 8; it++) {
+    double Jm[4][5];
+    for (int k = 0; k < 4; k++) {
+      int d = k == 0 ? 0 : k + 1;
+      auto g = [d](double t) { return d == 0 ? 1.0 : std::pow(1 - 2 * t, d) + 2 * std::pow(t, d); };
+      auto dg = [d](double t) {
+        return d == 0 ? 0.0 : -2.0 * d * std::pow(1 - 2 * t, d - 1) + 2.0 * d * std::pow(t, d - 1);
+      };
+      Jm[k][0] = 3 * wa * dg(a);
+      Jm[k][1] = 3 * g(a);
+      Jm[k][2] = 3 * wb * dg(b);
+      Jm[k][3] = 3 * g(b);
+      Jm[k][4] = 3 * wa * g(a) + 3 * wb * g(b) - target[k];
+    }
+    for (int c = 0; c < 4; c++) {
+      int p = c;
+      for (int i = c + 1; i < 4; i++)
+        if (std::fabs(Jm[i][c]) > std::fabs(Jm[p][c])) p = i;
+      for (int j = 0; j < 5; j++) std::swap(Jm[c][j], Jm[p][j]);
+      for (int i = c + 1; i < 4; i++) {
+        double f = Jm[i][c] / Jm[c][c];
+        for (int j = c; j < 5; j++) Jm[i][j] -= f * Jm[c][j];
+      }
+    }
+    double d[4];
+    for (int i = 3; i >= 0; i--) {
+      double s = Jm[i][4];
+      for (int j = i + 1; j < 4; j++) s -= Jm[i][j] * d[j];
+      d[i] = s / Jm[i][i];
+    }
+    a -= d[0]; wa -= d[1]; b -= d[2]; wb -= d[3];
+  }
+}
+
+TrigRule trig_rule(int order) {
+  TrigRule r;
+  if (order <= 1) {
+    r.xi = {1.0 / 3}; r.eta = {1.0 / 3}; r.w = {0.5};
+  } else if (order == 2) {
+    orbit3(r, 1.0 / 6.0, 1.0 / 3.0);
+  } else if (order <= 4) {
+    double a, wa, b, wb;
+    dunavant_deg4(a, wa, b, wb);
+    orbit3(r, a, wa);
+    orbit3(r, b, wb);
+  } else if (order == 5) {  // Radon's 7-point degree-5 rule (closed form)
+    const double s15 = std::sqrt(15.0);
+    r.xi = {1.0 / 3}; r.eta = {1.0 / 3}; r.w = {0.5 * 9.0 / 40.0};
+    orbit3(r, (6.0 - s15) / 21.0, (155.0 - s15) / 1200.0);
+    orbit3(r, (6.0 + s15) / 21.0, (155.0 + s15) / 1200.0);
+  } else {  // collapsed (Duffy) Gauss-Legendre product rule, exact to degree order
+    std::vector<double> u, uw, v, vw;
+    gauss01((order + 3) / 2, u, uw);
+    gauss01((order + 2) / 2, v, vw);
+    for (size_t i = 0; i < u.size(); i++)
+      for (size_t j = 0; j < v.size(); j++) {
+        r.xi.push_back(u[i]);
+        r.eta.push_back((1.0 - u[i]) * v[j]);
+        r.w.push_back(uw[i] * vw[j] * (1.0 - u[i]));
+      }
+  }
+  return r;
+}
+
+// ------------------------------------------------------------------ tree --
+
+// cyclic Jacobi (rotation order (0,1),(0,2),(1,2)); eigenvectors as rows of V
+static void jacobi_sym3(const double Ain[9], double w[3], double V[9]) {
+  double A[9];
+  std::memcpy(A, Ain, sizeof(A));
+  for (int i = 0; i < 9; i++) V[i] = (i % 4 == 0) ? 1.0 : 0.0;
+  static const int pq[3][2] = {{0, 1}, {0, 2}, {1, 2}};
+  for (int sweep = 0; sweep < 60; sweep++) {
+    double off = A[1] * A[1] + A[2] * A[2] + A[5] * A[5];
+    double dia = A[0] * A[0] + A[4] * A[4] + A[8] * A[8];
+    if (off <= 1e-36 * dia || off == 0.0) break;
+    for (auto& t : pq) {
+      const int p = t[0], q = t[1];
+      const double apq = A[3 * p + q];
+      if (apq == 0.0) continue;
+      const double tau = (A[3 * q + q] - A[3 * p + p]) / (2.0 * apq);
+      const double tt = (tau >= 0 ? 1.0 : -1.0) / (std::fabs(tau) + std::sqrt(1.0 + tau * tau));
+      const double c = 1.0 / std::sqrt(1.0 + tt * tt), s = tt * c;
+      for (int k = 0; k < 3; k++) {
+        double x = A[3 * k + p], y = A[3 * k + q];
+        A[3 * k + p] = c * x - s * y;
+        A[3 * k + q] = s * x + c * y;
+      }
+      for (int k = 0; k < 3; k++) {
+        double x = A[3 * p + k], y = A[3 * q + k];
+        A[3 * p + k] = c * x - s * y;
+        A[3 * q + k] = s * x + c * y;
+      }
+      for (int k = 0; k < 3; k++) {
+        double x = V[3 * p + k], y = V[3 * q + k];
+        V[3 * p + k] = c * x - s * y;
+        V[3 * q + k] = s * x + c * y;
+      }
+    }
+  }
+  w[0] = A[0];
+  w[1] = A[4];
+  w[2] = A[8];
+}
+
+// eigenvector of the largest eigenvalue (ties: highest index); sign chosen so
+// the largest-magnitude component is positive
+static void split_axis(const double A[9], double ax[3]) {
+  double w[3], V[9];
+  jacobi_sym3(A, w, V);
+  int im = 0;
+  for (int i = 1; i < 3; i++)
+    if (w[i] >= w[im]) im = i;
+  for (int k = 0; k < 3; k++) ax[k] = V[3 * im + k];
+  int km = 0;
+  for (int k = 1; k < 3; k++)
+    if (std::fabs(ax[k]) > std::fabs(ax[km])) km = k;
+  if (ax[km] < 0)
+    for (int k = 0; k < 3; k++) ax[k] = -ax[k];
+}
+
+void Tree::build(const Space& s, int leafsize) {
+  const Mesh& m = *s.mesh;
+  // cluster data: weights = mean adjacent element area; points by support count
+  std::vector<double> G(s.ndof, 0.0), X(3 * (size_t)s.ndof, 0.0);
+  std::vector<int> cnt(s.ndof, 0);
+  for (int e = 0; e < m.nt; e++)
+    for (int l = 0; l < s.nloc; l++) {
+      int d = s.dofs[e * s.nloc + l];
+      G[d] += 0.5 * m.J[e];
+      cnt[d] += 1;
+    }
+  static const double xh[7][2] = {{1, 0}, {0, 1}, {0, 0}, {0.5, 0.5}, {0, 0.5}, {0.5, 0}, {1.0 / 3, 1.0 / 3}};
+  for (int e = 0; e < m.nt; e++) {
+    double mip[7][3];
+    for (int j = 0; j < 7; j++) m.point(e, xh[j][0], xh[j][1], mip[j]);
+    int vc = 0;
+    for (int l = 0; l < s.nloc; l++) {
+      int d = s.dofs[e * s.nloc + l];
+      if (cnt[d] > 2) {
+        for (int k = 0; k < 3; k++) X[3 * d + k] = mip[vc][k];
+        vc++;
+      }
+      if (cnt[d] == 1)
+        for (int k = 0; k < 3; k++) X[3 * d + k] = mip[6][k];
+    }
+  }
+  for (int d = 0; d < s.ndof; d++) G[d] /= (double)cnt[d];
+
+  perm.resize(s.ndof);
+  for (int i = 0; i < s.ndof; i++) perm[i] = i;
+  cl.clear();
+  cl.push_back(Cluster{0, -1, -1, -1, s.ndof, 0, 0.0, {0, 0, 0}});
+  for (size_t ic = 0; ic < cl.size(); ic++) {  // breadth-first, src/hmat.cpp:379-407
+    const long nclu = cl[ic].number, b = cl[ic].permupos, e = b + nclu - 1;
+    double mom[10] = {0};
+    for (long i = b; i <= e; i++) {
+      const int j = perm[i];
+      const double* x = &X[3 * j];
+      mom[0] += G[j];
+      mom[1] += G[j] * x[0];
+      mom[2] += G[j] * x[1];
+      mom[3] += G[j] * x[2];
+      mom[4] += G[j] * x[0] * x[0];
+      mom[5] += G[j] * x[0] * x[1];
+      mom[6] += G[j] * x[0] * x[2];
+      mom[7] += G[j] * x[1] * x[1];
+      mom[8] += G[j] * x[1] * x[2];
+      mom[9] += G[j] * x[2] * x[2];
+    }
+    const double ctr[3] = {mom[1] / mom[0], mom[2] / mom[0], mom[3] / mom[0]};
+    double A[9];
+    A[0] = mom[4] - ctr[0] * mom[1];
+    A[1] = mom[5] - ctr[0] * mom[2];
+    A[2] = mom[6] - ctr[0] * mom[3];
+    A[3] = A[1];
+    A[4] = mom[7] - ctr[1] * mom[2];
+    A[5] = mom[8] - ctr[1] * mom[3];
+    A[6] = A[2];
+    A[7] = A[5];
+    A[8] = mom[9] - ctr[2] * mom[3];
+    double ax[3];
+    split_axis(A, ax);
+    double radius = 0.0;
+    for (long i = b; i <= e; i++) {
+      const int j = perm[i];
+      double dx = X[3 * j] - ctr[0], dy = X[3 * j + 1] - ctr[1], dz = X[3 * j + 2] - ctr[2];
+      radius = std::fmax(radius, std::sqrt(dx * dx + dy * dy + dz * dz));
+    }
+    cl[ic].radius = radius;
+    for (int k = 0; k < 3; k++) cl[ic].centre[k] = ctr[k];
+    cl[ic].child1 = cl[ic].child2 = -1;
+    if (nclu <= leafsize) continue;
+    long num1 = 0, num2 = 0, icheck = b, iend = e;
+    const double crit = ctr[0] * ax[0] + ctr[1] * ax[1] + ctr[2] * ax[2];
+    for (long i = 0; i < nclu; i++) {
+      const int j = perm[icheck];
+      if (X[3 * j] * ax[0] + X[3 * j + 1] * ax[1] + X[3 * j + 2] * ax[2] >= crit) {
+        num1++;
+        icheck++;
+      } else {
+        num2++;
+        perm[icheck] = perm[iend];
+        perm[iend] = j;
+        iend--;
+      }
+    }
+    if (num1 == 0 || num2 == 0) continue;  // non-separable: leaf
+    const long lvl = cl[ic].level;
+    const int n0 = (int)cl.size();
+    cl[ic].child1 = n0;
+    cl[ic].child2 = n0 + 1;
+    cl.push_back(Cluster{lvl + 1, (long)ic, -1, -1, num1, b, 0.0, {0, 0, 0}});
+    cl.push_back(Cluster{lvl + 1, (long)ic, -1, -1, num2, b + num1, 0.0, {0, 0, 0}});
+  }
+  iperm.assign(s.ndof, 0);
+  for (int i = 0; i < s.ndof; i++) iperm[perm[i]] = i;
+  leaves.clear();
+  for (size_t i = 0; i < cl.size(); i++)
+    if (cl[i].child1 == -1) leaves.push_back((int)i);
+  std::sort(leaves.begin(), leaves.end(), [this](int a, int b) { return cl[a].permupos < cl[b].permupos; });
+}
+
+static bool admissible(const Cluster& a, const Cluster& b, double eta) {
+  double d[3] = {a.centre[0] - b.centre[0], a.centre[1] - b.centre[1], a.centre[2] - b.centre[2]};
+  double dist = std::sqrt(d[0] * d[0] + d[1] * d[1] + d[2] * d[2]);
+  dist = std::fmax(0.0, dist - a.radius - b.radius);
+  return std::fmin(a.radius, b.radius) < eta * dist;
+}
+
+static void part_rec(const Tree& rt, const Tree& ct, int i, int j, double eta, std::vector<Block>& out) {
+  const Cluster& r = rt.cl[i];
+  const Cluster& c = ct.cl[j];
+  int near;
+  if (r.child1 == -1 || c.child1 == -1) near = 1;
+  else if (admissible(r, c, eta)) near = 0;
+  else {
+    part_rec(rt, ct, (int)r.child1, (int)c.child1, eta, out);
+    part_rec(rt, ct, (int)r.child1, (int)c.child2, eta, out);
+    part_rec(rt, ct, (int)r.child2, (int)c.child1, eta, out);
+    part_rec(rt, ct, (int)r.child2, (int)c.child2, eta, out);
+    return;
+  }
+  out.push_back(Block{i, j, (int)r.permupos, (int)r.number, (int)c.permupos, (int)c.number, near, 0});
+}
+
+void partition(const Tree& rt, const Tree& ct, double eta, std::vector<Block>& out) {
+  out.clear();
+  part_rec(rt, ct, 0, 0, eta, out);
+}
+
+// --------------------------------------------------------------- kernels --
+
+int eval_dim(int ev) { return ev == EV_ID ? 1 : (ev == EV_ROT ? 3 : 4); }
+
+KernelInfo KernelInfo::make(int id, double kappa) {
+  (void)kappa;
+  KernelInfo k;
+  k.id = id;
+  k.complex_valued = id >= K_HSL;
+  k.ncomp = 1;
+  k.ev_test = k.ev_trial = EV_ID;
+  k.terms = {{1.0, 0, 0, 0}};
+  if (id == K_LHS) {  // src/ngbem.hpp:291-296
+    k.terms = {{1.0, 0, 0, 0}, {1.0, 0, 1, 1}, {1.0, 0, 2, 2}};
+    k.ev_test = k.ev_trial = EV_ROT;
+  } else if (id == K_HHS) {  // src/ngbem.hpp:389-395
+    k.ncomp = 2;
+    k.terms = {{1.0, 0, 0, 0}, {1.0, 0, 1, 1}, {1.0, 0, 2, 2}, {1.0, 1, 3, 3}};
+    k.ev_test = k.ev_trial = EV_HELM;
+  }
+  return k;
+}
+
+void patch(const Space& s, const int* dofs, int nd, std::vector<int>& out) {
+  out.clear();
+  for (int i = 0; i < nd; i++)
+    for (int k = s.ed_off[dofs[i]]; k < s.ed_off[dofs[i] + 1]; k++) out.push_back(s.ed_list[k]);
+  std::sort(out.begin(), out.end());
+  out.erase(std::unique(out.begin(), out.end()), out.end());
+}
+
+int classify_pair(const Mesh& m, int ei, int ej, int px[3], int py[3]) {
+  const int* vi = &m.tri[3 * ei];
+  const int* vj = &m.tri[3 * ej];
+  int nc = 0;
+  for (int a = 0; a < 3; a++)
+    for (int b = 0; b < 3; b++)
+      if (vi[a] == vj[b]) nc++;
+  px[0] = 0; px[1] = 1; px[2] = 2;
+  py[0] = 0; py[1] = 1; py[2] = 2;
+  if (nc == 2) {  // common edge: shared edge first, y edge reversed
+    static const int ed[3][2] = {{0, 1}, {1, 2}, {2, 0}};
+    int cex = 0, cey = 0;
+    for (int cx = 0; cx < 3; cx++)
+      for (int cy = 0; cy < 3; cy++) {
+        int a0 = vi[ed[cx][0]], a1 = vi[ed[cx][1]], b0 = vj[ed[cy][0]], b1 = vj[ed[cy][1]];
+        if (std::min(a0, a1) == std::min(b0, b1) && std::max(a0, a1) == std::max(b0, b1)) {
+          cex = cx;
+          cey = cy;
+        }
+      }
+    px[0] = ed[cex][0]; px[1] = ed[cex][1]; px[2] = 3 - px[0] - px[1];
+    py[0] = ed[cey][1]; py[1] = ed[cey][0]; py[2] = 3 - py[0] - py[1];
+  } else if (nc == 1) {  // common vertex first
+    int cvx = 0, cvy = 0;
+    for (int cx = 0; cx < 3; cx++)
+      for (int cy = 0; cy < 3; cy++)
+        if (vi[cx] == vj[cy]) { cvx = cx; cvy = cy; }
+    px[0] = cvx; px[1] = (cvx + 1) % 3; px[2] = 3 - px[0] - px[1];
+    py[0] = cvy; py[1] = (cvy + 1) % 3; py[2] = 3 - py[0] - py[1];
+  }
+  return nc;
+}
+
+}  // namespace bem

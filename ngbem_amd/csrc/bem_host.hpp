@@ -1,0 +1,85 @@
+// Host-side structures of the MI355X BEM library: mesh, spaces, quadrature
+// rules, cluster trees, block partition and the device work lists built from
+// them.  Compiled with -ffp-contract=off so that the cluster tree is reproducible
+// across compilers (the tree of a symmetric mesh is decided by rounding noise).
+#pragma once
+
+#include <cstdint>
+#include <string>
+#include <vector>
+
+namespace bem {
+
+static const double kPi = 3.14159265358979323846;
+
+enum SpaceKind { SP_P0 = 0, SP_P1 = 1 };
+enum EvalKind { EV_ID = 0, EV_ROT = 1, EV_HELM = 2 };
+enum KernelId { K_LSL = 0, K_LDL, K_LHS, K_HSL, K_HDL, K_HCF, K_HHS, K_COUNT };
+
+struct Mesh {
+  int nv = 0, nt = 0;
+  std::vector<double> xyz;                 // 3*nv
+  std::vector<int> tri;                    // 3*nt
+  std::vector<double> nrm, J;              // 3*nt, nt
+  std::vector<int> ve_off, ve_list;        // vertex -> elements (CSR)
+  void setup();
+  void point(int e, double xi, double eta, double* x) const;
+  void curls(int e, double c[3][3]) const; // n x grad(lambda_k)
+};
+
+struct Space {
+  const Mesh* mesh = nullptr;
+  int kind = SP_P0, ndof = 0, nloc = 1;
+  std::vector<int> dofs;                   // nt*nloc
+  std::vector<int> ed_off, ed_list;        // dof -> elements (CSR, ascending)
+  void setup(const Mesh* m, int k);
+};
+
+struct Rule4 {
+  std::vector<double> x0, x1, y0, y1, w;
+  int size() const { return (int)w.size(); }
+};
+struct TrigRule {
+  std::vector<double> xi, eta, w;
+  int size() const { return (int)w.size(); }
+};
+void gauss01(int n, std::vector<double>& x, std::vector<double>& w);
+Rule4 sauter_schwab(int kind, int order);  // 0 identical, 1 common edge, 2 common vertex
+TrigRule trig_rule(int order);
+
+struct Cluster {
+  long level, parent, child1, child2, number, permupos;
+  double radius, centre[3];
+};
+struct Tree {
+  std::vector<int> perm, iperm;  // cluster position <-> dof
+  std::vector<Cluster> cl;
+  std::vector<int> leaves;       // leaf clusters in position order
+  void build(const Space& s, int leafsize);
+};
+
+struct Block {
+  int row, col;           // test cluster, trial cluster
+  int r0, m, c0, n;       // position ranges
+  int near, fallback;
+};
+void partition(const Tree& rt, const Tree& ct, double eta, std::vector<Block>& out);
+
+struct KernelInfo {
+  int id;
+  bool complex_valued;
+  int ncomp;
+  int ev_test, ev_trial;
+  struct Term { double fac; int kc, trc, tec; };
+  std::vector<Term> terms;
+  static KernelInfo make(int id, double kappa);
+};
+int eval_dim(int ev);
+
+// sorted unique elements touching the given dofs
+void patch(const Space& s, const int* dofs, int ndofs, std::vector<int>& out);
+// number of common vertices and the Sauter-Schwab vertex permutations
+// (src/ngbem.cpp:527-531, 569-586, 627-642); returns the count
+int classify_pair(const Mesh& m, int ei, int ej, int px[3], int py[3]);
+
+}  // namespace bem

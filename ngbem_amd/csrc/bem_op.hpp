@@ -1,0 +1,154 @@
+// Operator object of the MI355X BEM library: host tables plus the device
+// buffers an assembled operator owns.  All device data are laid out in cluster
+// order, so every block's dof set is a contiguous range of rows/columns.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <memory>
+#include <stdexcept>
+#include <string>
+#include <vector>
+
+#include "bem_host.hpp"
+
+namespace bem {
+
+struct HipError : std::runtime_error {
+  using std::runtime_error::runtime_error;
+};
+
+#define BEM_HIP(call)                                                                          \
+  do {                                                                                         \
+    hipError_t _e = (call);                                                                    \
+    if (_e != hipSuccess)                                                                      \
+      throw ::bem::HipError(std::string(#call) + ": " + hipGetErrorString(_e) + " (" __FILE__ \
+                            ":" + std::to_string(__LINE__) + ")");                             \
+  } while (0)
+
+// owning device buffer
+struct DBuf {
+  void* p = nullptr;
+  size_t bytes = 0;
+  DBuf() = default;
+  DBuf(const DBuf&) = delete;
+  DBuf& operator=(const DBuf&) = delete;
+  ~DBuf() { release(); }
+  void release() {
+    if (p) (void)hipFree(p);
+    p = nullptr;
+    bytes = 0;
+  }
+  void alloc(size_t n) {
+    if (n == bytes && p) return;
+    release();
+    if (n == 0) return;
+    BEM_HIP(hipMalloc(&p, n));
+    bytes = n;
+  }
+  template <class T>
+  void upload(const std::vector<T>& v, hipStream_t s = nullptr) {
+    alloc(v.size() * sizeof(T));
+    if (!v.empty()) BEM_HIP(hipMemcpyAsync(p, v.data(), v.size() * sizeof(T), hipMemcpyHostToDevice, s));
+  }
+  template <class T>
+  T* as() const { return static_cast<T*>(p); }
+};
+
+// near block record (device)
+struct NearBlk {
+  int64_t off;     // offset of the m x n row-major values
+  int r0, m, c0, n;
+  int pI, pm;      // test patch: offset, length (P0: cluster slice of perm)
+  int pJ, pn;      // trial patch
+};
+
+// far block record (device)
+struct FarBlk {
+  int64_t uoff, voff;  // final factors: U m x R, V R x n (row-major)
+  int r0, m, c0, n;
+  int R;               // total rank (sum over terms)
+  int toff;            // offset in the matvec scratch for B^T x / A^T x
+  int pI, pm, pJ, pn;  // sorted patches (ACA point order)
+  int rmap, cmap;      // offsets of the row/col projection maps
+};
+
+// one ACA task = (far block, kernel component)
+struct AcaTask {
+  int64_t uoff, voff;  // workspace offsets (elements)
+  int blk, comp;
+  int np, nq;          // points (patch size * trig points)
+  int cap;             // rank capacity of this pass
+  int pad;
+};
+
+// matvec segment: the part of block `blk` that lies in one leaf tile
+struct Seg {
+  int blk;   // near: index in near table; far: index in far table
+  int far;   // 0 near, 1 far
+  int off;   // first row (forward) / column (transpose) of the tile inside the block
+  int pad;
+};
+
+struct Operator {
+  int kernel = K_LSL;
+  double kappa = 0.0;
+  KernelInfo kinfo;
+  bool complex_valued = false;
+  int intorder = 3, leafsize = 40;
+  double eta = 2.0, eps = 1e-6;
+  bool dense = false, mixed = false;
+
+  std::shared_ptr<Mesh> mesh;
+  std::shared_ptr<Space> trial, test;
+  bool same_space = true;
+  Tree trial_tree, test_tree_own;
+  const Tree* test_tree = nullptr;
+
+  std::vector<Block> blocks;
+  std::vector<int> blk_slot;  // block -> index in near or far table
+  std::vector<NearBlk> near;
+  std::vector<FarBlk> far;
+  std::vector<int> far_rank;  // per far block, total rank
+  int n_fallback = 0;
+  int64_t near_entries = 0, far_factor_entries = 0;
+  int n_sing[3] = {0, 0, 0};
+  double t_setup = 0, t_near = 0, t_far = 0;
+  double near_evals = 0, far_evals = 0;
+  bool assembled = false;
+
+  // rules
+  TrigRule trig;
+  Rule4 ss[3];
+
+  // device data
+  DBuf d_geo, d_vid, d_curl, d_quad, d_trig, d_ss[3];
+  DBuf d_perm_test, d_iperm_test, d_perm_trial, d_iperm_trial;
+  DBuf d_patch;            // near patches of P1 sides (test then trial)
+  DBuf d_near_blk, d_near_chunks, d_near_val, d_near_val32;
+  DBuf d_spairs[3];
+  DBuf d_far_blk, d_fU, d_fV, d_fU32, d_fV32, d_far_patch, d_far_maps;
+  // matvec plans
+  DBuf d_seg_fwd, d_tile_fwd, d_seg_trn, d_tile_trn;
+  int n_tiles_fwd = 0, n_tiles_trn = 0;
+  int t_total = 0;         // scratch length for far B^T x
+  DBuf d_xp, d_yp, d_t;    // scratch vectors (complex128 or double)
+
+  int ndof_trial() const { return trial->ndof; }
+  int ndof_test() const { return test->ndof; }
+};
+
+// device entry points (bem_device.hip)
+void setup_device(Operator& op, hipStream_t s);
+void assemble_near(Operator& op, hipStream_t s);
+void assemble_far(Operator& op, hipStream_t s);
+void build_matvec_plan(Operator& op, hipStream_t s);
+void make_mixed_copy(Operator& op, hipStream_t s);
+// y = alpha * op(A) x + beta * y on device vectors (complex128 or double)
+// ev (optional, 5 events): recorded before the gather and after the gather,
+// the low-rank B^T x stage, the row/column kernel and the scatter
+void apply_device(Operator& op, int flags, const double* alpha, const void* x, const double* beta, void* y,
+                  hipStream_t s, hipEvent_t* ev = nullptr);
+
+}  // namespace bem

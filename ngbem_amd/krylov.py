@@ -1,0 +1,157 @@
+"""Krylov solvers on GPU vectors, with ngsolve.krylovspace's argument names
+(CG: mat, rhs, pre, sol, tol, maxsteps, printrates, initialize; GMRes: A, b,
+pre, x, maxsteps, tol, printrates, restart), as the reference's scripts call
+them (tests/test_Laplace.py:60, demos/HelmholtzBurtonMiller.ipynb cell 9).
+
+``RefinedSolve`` is the mixed-precision form: an inner GMRES on the FP32 /
+complex64 copy of the operator, wrapped in FP64 iterative refinement where the
+residual b - A x is recomputed with the FP64 operator.
+"""
+from __future__ import annotations
+
+import math
+
+import numpy as np
+import torch
+
+from .operators import BaseMatrix, device
+
+
+def _dot(a, b):
+    return torch.vdot(a, b) if torch.is_complex(a) else torch.dot(a, b)
+
+
+def _apply(A, x):
+    if isinstance(A, BaseMatrix):
+        y = torch.zeros(A.height, dtype=torch.complex128 if (A.is_complex or torch.is_complex(x)) else torch.float64,
+                        device=x.device)
+        A.MultAdd(1.0, x, y)
+        return y
+    return A(x)
+
+
+def CG(mat, rhs, pre=None, sol=None, tol=1e-12, maxsteps=100, printrates=True, initialize=True, conjugate=False):
+    """Preconditioned conjugate gradients (relative residual tolerance)."""
+    b = rhs
+    x = sol if sol is not None else torch.zeros_like(b)
+    if initialize:
+        x.zero_()
+    r = b - _apply(mat, x)
+    z = _apply(pre, r) if pre is not None else r.clone()
+    p = z.clone()
+    rz = _dot(r, z)
+    err0 = math.sqrt(abs(float(rz.real if torch.is_complex(rz) else rz)))
+    if err0 == 0.0:
+        return x
+    for it in range(maxsteps):
+        Ap = _apply(mat, p)
+        alpha = rz / _dot(p, Ap)
+        x.add_(alpha * p)
+        r.sub_(alpha * Ap)
+        z = _apply(pre, r) if pre is not None else r
+        rz_new = _dot(r, z)
+        err = math.sqrt(abs(float(rz_new.real if torch.is_complex(rz_new) else rz_new)))
+        if printrates:
+            print(f"CG iteration {it + 1}, residual = {err}")
+        if err < tol * err0:
+            break
+        beta = rz_new / rz
+        rz = rz_new
+        p = z + beta * p
+    CG.iterations = it + 1
+    return x
+
+
+def GMRes(A, b, pre=None, freedofs=None, x=None, maxsteps=100, tol=None, innerproduct=None, callback=None,
+          restart=None, startiteration=0, printrates=True, reltol=None):
+    """Restarted right-preconditioned GMRES (modified Gram-Schmidt, Givens
+    rotations); stops when ||r|| <= tol * ||b||."""
+    if freedofs is not None or innerproduct is not None:
+        raise NotImplementedError("freedofs / innerproduct are not supported")
+    tol = 1e-7 if tol is None else tol
+    if reltol is not None:
+        tol = reltol
+    dt = torch.complex128 if (torch.is_complex(b) or getattr(A, "is_complex", False)) else torch.float64
+    b = b.to(dt)
+    if x is None:
+        x = torch.zeros_like(b)
+    m = restart or maxsteps
+    bnorm = float(torch.linalg.norm(b))
+    if bnorm == 0.0:
+        x.zero_()
+        return x
+    total = 0
+    cdt = np.complex128
+    while total < maxsteps:
+        r = b - _apply(A, x)
+        beta = float(torch.linalg.norm(r))
+        if beta <= tol * bnorm:
+            break
+        V = [r / beta]
+        Z = []
+        H = np.zeros((m + 1, m), dtype=cdt)
+        cs = np.zeros(m, dtype=cdt)
+        sn = np.zeros(m, dtype=cdt)
+        g = np.zeros(m + 1, dtype=cdt)
+        g[0] = beta
+        k_used = 0
+        for k in range(m):
+            z = _apply(pre, V[k]) if pre is not None else V[k]
+            Z.append(z)
+            w = _apply(A, z)
+            for i in range(k + 1):  # modified Gram-Schmidt
+                h = _dot(V[i], w)
+                H[i, k] = complex(h)
+                w = w - h * V[i]
+            hn = float(torch.linalg.norm(w))
+            H[k + 1, k] = hn
+            for i in range(k):  # previous rotations
+                hi, hj = H[i, k], H[i + 1, k]
+                H[i, k] = np.conj(cs[i]) * hi + np.conj(sn[i]) * hj
+                H[i + 1, k] = -sn[i] * hi + cs[i] * hj
+            a, bb = H[k, k], H[k + 1, k]
+            t = math.hypot(abs(a), abs(bb))
+            cs[k], sn[k] = (a / t, bb / t) if t > 0 else (1.0, 0.0)
+            H[k, k] = t
+            H[k + 1, k] = 0.0
+            g[k + 1] = -sn[k] * g[k]
+            g[k] = np.conj(cs[k]) * g[k]
+            k_used = k + 1
+            total += 1
+            res = abs(g[k + 1])
+            if printrates:
+                print(f"GMRes iteration {total}, residual = {res}")
+            if callback is not None:
+                callback(res)
+            if res <= tol * bnorm or hn == 0.0 or total >= maxsteps:
+                break
+            V.append(w / hn)
+        y = np.linalg.solve(np.triu(H[:k_used, :k_used]), g[:k_used])
+        for i in range(k_used):
+            x.add_(complex(y[i]) * Z[i] if torch.is_complex(x) else float(y[i].real) * Z[i])
+        if abs(g[k_used]) <= tol * bnorm:
+            break
+    GMRes.iterations = total
+    return x
+
+
+def RefinedSolve(A, A_inner, b, pre=None, tol=1e-10, inner_tol=1e-5, maxouter=20, maxsteps=500, printrates=False):
+    """FP64 iterative refinement around an inner GMRES on A_inner (typically
+    ``A.mixed``): x += GMRes(A_inner, b - A x).  Returns (x, history)."""
+    x = torch.zeros_like(b)
+    bnorm = float(torch.linalg.norm(b))
+    hist = []
+    inner_total = 0
+    for outer in range(maxouter):
+        r = b - _apply(A, x)
+        rn = float(torch.linalg.norm(r)) / bnorm
+        hist.append(rn)
+        if printrates:
+            print(f"refinement step {outer}: relative residual {rn:.3e}")
+        if rn <= tol:
+            break
+        d = GMRes(A_inner, r, pre=pre, maxsteps=maxsteps, tol=inner_tol, printrates=False)
+        inner_total += GMRes.iterations
+        x.add_(d)
+    RefinedSolve.inner_iterations = inner_total
+    return x, hist

@@ -1,0 +1,531 @@
+"""Python operator API, mirroring the reference's factories (src/python_bem.cpp:27-173)
+and the BaseMatrix interface NGSolve's solvers drive (src/hmat.hpp:139-197).
+
+Every factory assembles eagerly on the GPU, like the reference's constructor
+(src/ngbem.cpp:314-330).  ``op.mat`` is an :class:`HMatrix`: Mult / MultAdd /
+MultTrans / MultTransAdd, ``*`` with a vector, ``.T`` (plain transpose, as
+src/hmat.cpp:547-548), ``.H`` (adjoint), ``.nze``, ``CreateRowVector`` /
+``CreateColVector``, and algebra with scalars and other matrices
+(``0.5 * M + C.mat``).  Vectors are torch tensors on the GPU (numpy arrays are
+accepted and moved there).
+"""
+from __future__ import annotations
+
+import ctypes
+import math
+
+import numpy as np
+import torch
+
+from . import _capi
+from .spaces import FESpace, H1, SurfaceL2, mass_matrix_coo
+
+__all__ = ["BaseMatrix", "HMatrix", "IntegralOperator", "IntegralOperatorC", "MassMatrix",
+           "SingleLayerPotentialOperator", "DoubleLayerPotentialOperator", "HypersingularOperator",
+           "HelmholtzSingleLayerPotentialOperator", "HelmholtzDoubleLayerPotentialOperator",
+           "HelmholtzCombinedFieldOperator", "HelmholtzHypersingularOperator",
+           "MaxwellSingleLayerPotentialOperator", "MaxwellSingleLayerPotentialOperatorCurl",
+           "MaxwellDoubleLayerPotentialOperator", "device"]
+
+
+def device() -> torch.device:
+    if not torch.cuda.is_available():
+        raise RuntimeError("ngbem_amd needs a GPU (HIP device); none is visible")
+    return torch.device("cuda", torch.cuda.current_device())
+
+
+def _as_vec(x, dtype, dev):
+    if isinstance(x, np.ndarray):
+        x = torch.from_numpy(np.ascontiguousarray(x))
+    if not isinstance(x, torch.Tensor):
+        raise TypeError("vectors are torch tensors or numpy arrays")
+    return x.to(device=dev, dtype=dtype)
+
+
+# ------------------------------------------------------------- BaseMatrix --
+
+class BaseMatrix:
+    """Linear operator with NGSolve's BaseMatrix calling conventions."""
+
+    height: int
+    width: int
+    is_complex: bool
+
+    def dtype(self):
+        return torch.complex128 if self.is_complex else torch.float64
+
+    # y += s * A x
+    def MultAdd(self, s, x, y):
+        raise NotImplementedError
+
+    # y += s * A^T x
+    def MultTransAdd(self, s, x, y):
+        raise NotImplementedError
+
+    # y += s * A^H x = s * conj(A^T conj(x))
+    def MultAdjAdd(self, s, x, y):
+        tmp = torch.zeros_like(y)
+        self.MultTransAdd(1.0, x.conj() if torch.is_complex(x) else x, tmp)
+        y.add_(s * tmp.conj() if torch.is_complex(tmp) else s * tmp)
+
+    def Mult(self, x, y):
+        y.zero_()
+        self.MultAdd(1.0, x, y)
+
+    def MultTrans(self, x, y):
+        y.zero_()
+        self.MultTransAdd(1.0, x, y)
+
+    def CreateRowVector(self):
+        return torch.zeros(self.width, dtype=self.dtype(), device=device())
+
+    def CreateColVector(self):
+        return torch.zeros(self.height, dtype=self.dtype(), device=device())
+
+    def _out_dtype(self, x):
+        if self.is_complex or torch.is_complex(x):
+            return torch.complex128
+        return torch.float64
+
+    def __mul__(self, other):
+        if isinstance(other, (int, float, complex)):
+            return ScaledMatrix(other, self)
+        x = _as_vec(other, self._out_dtype(other if isinstance(other, torch.Tensor) else torch.from_numpy(other)),
+                    device())
+        y = torch.zeros(self.height, dtype=x.dtype, device=x.device)
+        self.MultAdd(1.0, x, y)
+        return y
+
+    def __rmul__(self, s):
+        if isinstance(s, (int, float, complex)):
+            return ScaledMatrix(s, self)
+        return NotImplemented
+
+    def __add__(self, other):
+        return SumMatrix(self, other, 1.0)
+
+    def __sub__(self, other):
+        return SumMatrix(self, other, -1.0)
+
+    def __neg__(self):
+        return ScaledMatrix(-1.0, self)
+
+    @property
+    def T(self):
+        return TransposeMatrix(self)
+
+    @property
+    def H(self):
+        return AdjointMatrix(self)
+
+    def Height(self):
+        return self.height
+
+    def Width(self):
+        return self.width
+
+
+class ScaledMatrix(BaseMatrix):
+    def __init__(self, s, A):
+        self.s, self.A = s, A
+        self.height, self.width = A.height, A.width
+        self.is_complex = A.is_complex or isinstance(s, complex)
+
+    def MultAdd(self, s, x, y):
+        self.A.MultAdd(s * self.s, x, y)
+
+    def MultTransAdd(self, s, x, y):
+        self.A.MultTransAdd(s * self.s, x, y)
+
+
+class SumMatrix(BaseMatrix):
+    def __init__(self, A, B, sign):
+        if A.height != B.height or A.width != B.width:
+            raise ValueError("matrix sum: shapes differ")
+        self.A, self.B, self.sign = A, B, sign
+        self.height, self.width = A.height, A.width
+        self.is_complex = A.is_complex or B.is_complex
+
+    def MultAdd(self, s, x, y):
+        self.A.MultAdd(s, x, y)
+        self.B.MultAdd(s * self.sign, x, y)
+
+    def MultTransAdd(self, s, x, y):
+        self.A.MultTransAdd(s, x, y)
+        self.B.MultTransAdd(s * self.sign, x, y)
+
+
+class TransposeMatrix(BaseMatrix):
+    def __init__(self, A):
+        self.A = A
+        self.height, self.width = A.width, A.height
+        self.is_complex = A.is_complex
+
+    def MultAdd(self, s, x, y):
+        self.A.MultTransAdd(s, x, y)
+
+    def MultTransAdd(self, s, x, y):
+        self.A.MultAdd(s, x, y)
+
+
+class AdjointMatrix(BaseMatrix):
+    def __init__(self, A):
+        self.A = A
+        self.height, self.width = A.width, A.height
+        self.is_complex = A.is_complex
+
+    def MultAdd(self, s, x, y):
+        if isinstance(self.A, HMatrix):
+            self.A._apply(_capi.APPLY_TRANS | _capi.APPLY_CONJ, s, x, y)
+        else:
+            self.A.MultAdjAdd(s, x, y)
+
+    def MultTransAdd(self, s, x, y):
+        # (A^H)^T = conj(A)
+        tmp = torch.zeros_like(y)
+        self.A.MultAdd(1.0, x.conj() if torch.is_complex(x) else x, tmp)
+        y.add_((s * tmp.conj()) if torch.is_complex(tmp) else s * tmp)
+
+
+class MassMatrix(BaseMatrix):
+    """Surface mass matrix  int u v ds  (test x trial) for SurfaceL2 p0 / H1 p1,
+    the ``Id`` of the reference's formulations (e.g. demos/HelmholtzBurtonMiller.ipynb)."""
+
+    def __init__(self, trial: FESpace, test: FESpace | None = None):
+        test = test or trial
+        r, c, v = mass_matrix_coo(trial, test)
+        dev = device()
+        self.height, self.width = test.ndof, trial.ndof
+        self.is_complex = False
+        self.rows = torch.from_numpy(r).to(dev)
+        self.cols = torch.from_numpy(c).to(dev)
+        self.vals = torch.from_numpy(v).to(dev)
+
+    def MultAdd(self, s, x, y):
+        x = x.to(y.dtype)
+        y.index_add_(0, self.rows, (s * self.vals).to(y.dtype) * x[self.cols])
+
+    def MultTransAdd(self, s, x, y):
+        x = x.to(y.dtype)
+        y.index_add_(0, self.cols, (s * self.vals).to(y.dtype) * x[self.rows])
+
+
+# ---------------------------------------------------------------- HMatrix --
+
+class HMatrix(BaseMatrix):
+    """The assembled H-matrix of one operator (src/hmat.hpp:163-201)."""
+
+    def __init__(self, op, mixed=False):
+        self._op = op
+        self.height, self.width = op.test_space.ndof, op.trial_space.ndof
+        self.is_complex = op.is_complex
+        self._mixed = mixed
+
+    @property
+    def nze(self):
+        return int(self._op.stats()["nze"])
+
+    @property
+    def mixed(self):
+        """View of this matrix applied with the FP32 / complex64 copy."""
+        if not self._op.has_mixed:
+            raise RuntimeError("operator built without mixed=True")
+        return HMatrix(self._op, mixed=True)
+
+    def IsComplex(self):
+        return self.is_complex
+
+    def _apply(self, flags, s, x, y):
+        op = self._op
+        trans = flags & _capi.APPLY_TRANS
+        nin, nout = (self.height, self.width) if trans else (self.width, self.height)
+        if x.shape[-1] != nin or y.shape[-1] != nout:
+            raise ValueError(f"vector sizes {tuple(x.shape)}, {tuple(y.shape)} do not match {nout}x{nin}")
+        if not y.is_cuda:
+            raise ValueError("output vector must live on the GPU")
+        if self._mixed:
+            flags |= _capi.APPLY_MIXED
+        s = complex(s)
+        if not self.is_complex:
+            # real operator: real and imaginary parts separately
+            if torch.is_complex(y):
+                xr = _as_vec(x.real if torch.is_complex(x) else x, torch.float64, y.device)
+                xi = _as_vec(x.imag, torch.float64, y.device) if torch.is_complex(x) else None
+                yr = torch.zeros(nout, dtype=torch.float64, device=y.device)
+                op._call_apply(flags, 1.0, xr, 0.0, yr)
+                res = yr.to(torch.complex128)
+                if xi is not None:
+                    yi = torch.zeros(nout, dtype=torch.float64, device=y.device)
+                    op._call_apply(flags, 1.0, xi, 0.0, yi)
+                    res = res + 1j * yi
+                y.add_(s * res)
+                return
+            if s.imag != 0.0:
+                raise TypeError("complex scalar with a real vector")
+            xr = _as_vec(x, torch.float64, y.device).contiguous()
+            op._call_apply(flags, s.real, xr, 1.0, y)
+            return
+        if not torch.is_complex(y):
+            raise TypeError("complex operator needs a complex output vector")
+        xc = _as_vec(x, torch.complex128, y.device).contiguous()
+        if y.is_contiguous():
+            op._call_apply(flags, s, xc, 1.0, y)
+        else:
+            tmp = y.contiguous()
+            op._call_apply(flags, s, xc, 1.0, tmp)
+            y.copy_(tmp)
+
+    def MultAdd(self, s, x, y):
+        self._apply(0, s, x, y)
+
+    def MultTransAdd(self, s, x, y):
+        self._apply(_capi.APPLY_TRANS, s, x, y)
+
+    def MultAdjAdd(self, s, x, y):
+        self._apply(_capi.APPLY_TRANS | _capi.APPLY_CONJ, s, x, y)
+
+    def blocks(self):
+        return self._op.blocks()
+
+
+# ------------------------------------------------------ integral operator --
+
+class IntegralOperator:
+    """Assembled boundary integral operator (real-valued: Laplace)."""
+
+    is_complex = False
+
+    def __init__(self, kernel, kappa, trial_space, test_space, intorder, leafsize, eta, eps, dense=False,
+                 mixed=False, testhmatrix=False):
+        if not isinstance(trial_space, FESpace) or not isinstance(test_space, FESpace):
+            raise TypeError("spaces must be ngbem_amd SurfaceL2 / H1 spaces")
+        self.trial_space, self.test_space = trial_space, test_space
+        self.kernel, self.kappa = kernel, float(kappa)
+        self.params = dict(intorder=intorder, leafsize=leafsize, eta=eta, eps=eps, dense=dense)
+        self.has_mixed = bool(mixed)
+        L = _capi.lib()
+        p = _capi.Params(int(intorder), int(leafsize), float(eta), float(eps), int(bool(dense)), int(bool(mixed)))
+        out = ctypes.c_void_p()
+        _capi.check(L.bem_op_create(kernel, float(kappa), trial_space.handle, test_space.handle, ctypes.byref(p),
+                                    ctypes.byref(out)), "bem_op_create")
+        self._h = out
+        dev = device()
+        with torch.cuda.device(dev):
+            stream = torch.cuda.current_stream().cuda_stream
+            _capi.check(L.bem_op_assemble(self._h, ctypes.c_void_p(stream)), "bem_op_assemble")
+        self._mat = HMatrix(self)
+        if testhmatrix:
+            self._test_hmatrix()
+
+    def __del__(self):
+        h = getattr(self, "_h", None)
+        if h:
+            _capi.lib().bem_op_destroy(h)
+            self._h = None
+
+    @property
+    def mat(self) -> HMatrix:
+        return self._mat
+
+    def GetMatrix(self):
+        return self._mat
+
+    def GetPotential(self, gf):
+        raise NotImplementedError("potential evaluation (PotentialCF) is outside this library's scope")
+
+    def stats(self):
+        st = _capi.Stats()
+        _capi.check(_capi.lib().bem_op_stats(self._h, ctypes.byref(st)), "bem_op_stats")
+        return st.as_dict()
+
+    def _call_apply(self, flags, alpha, x, beta, y):
+        a = np.array([complex(alpha).real, complex(alpha).imag], dtype=np.float64)
+        b = np.array([complex(beta).real, complex(beta).imag], dtype=np.float64)
+        dtype = _capi.C128 if self.is_complex else _capi.F64
+        with torch.cuda.device(y.device):
+            stream = torch.cuda.current_stream().cuda_stream
+            _capi.check(_capi.lib().bem_op_apply(self._h, flags | _capi.APPLY_ON_DEVICE, _capi.dptr(a),
+                                                 ctypes.c_void_p(x.data_ptr()), _capi.dptr(b),
+                                                 ctypes.c_void_p(y.data_ptr()), 1, 0, dtype,
+                                                 ctypes.c_void_p(stream)), "bem_op_apply")
+
+    # -- introspection (parity tests) --
+    def num_blocks(self):
+        return _capi.lib().bem_op_num_blocks(self._h)
+
+    def block_info(self, b):
+        bi = _capi.BlockInfo()
+        _capi.check(_capi.lib().bem_op_block_info(self._h, b, ctypes.byref(bi)))
+        return {k: getattr(bi, k) for k, _ in bi._fields_}
+
+    def blocks(self):
+        return [self.block_info(b) for b in range(self.num_blocks())]
+
+    def block_dofs(self, b):
+        i = self.block_info(b)
+        rows = np.zeros(i["m"], dtype=np.int32)
+        cols = np.zeros(i["n"], dtype=np.int32)
+        _capi.check(_capi.lib().bem_op_block_dofs(self._h, b, _capi.iptr(rows), _capi.iptr(cols)))
+        return rows, cols
+
+    def block_data(self, b):
+        i = self.block_info(b)
+        m, n, r = i["m"], i["n"], i["rank"]
+        dt = np.complex128 if self.is_complex else np.float64
+        size = m * n if r < 0 else r * (m + n)
+        buf = np.zeros(max(size, 1), dtype=dt)
+        _capi.check(_capi.lib().bem_op_block_data(self._h, b, ctypes.c_void_p(buf.ctypes.data)))
+        if r < 0:
+            return buf[: m * n].reshape(m, n)
+        return buf[: m * r].reshape(m, r), buf[m * r: m * r + r * n].reshape(r, n)
+
+    def tree(self, which=0):
+        n = self.trial_space.ndof if which == 0 else self.test_space.ndof
+        perm = np.zeros(n, dtype=np.int32)
+        L = _capi.lib()
+        ncl = L.bem_op_tree(self._h, which, _capi.iptr(perm), None, None, 0)
+        cl = np.zeros(7 * ncl)
+        cen = np.zeros(3 * ncl)
+        L.bem_op_tree(self._h, which, _capi.iptr(perm), _capi.dptr(cl), _capi.dptr(cen), ncl)
+        return perm, cl.reshape(ncl, 7), cen.reshape(ncl, 3)
+
+    def densify(self):
+        """Full matrix from the stored blocks (small problems only)."""
+        dt = np.complex128 if self.is_complex else np.float64
+        A = np.zeros((self.test_space.ndof, self.trial_space.ndof), dtype=dt)
+        for b in range(self.num_blocks()):
+            rows, cols = self.block_dofs(b)
+            blk = self.block_data(b)
+            if isinstance(blk, tuple):
+                blk = blk[0] @ blk[1]
+            A[np.ix_(rows, cols)] += blk
+        return A
+
+    def _test_hmatrix(self):
+        """Spectral-norm error of H against the dense operator (30 power
+        iterations on (D-H)^H (D-H)) and compression rate, as the reference's
+        testhmatrix diagnostic (src/ngbem.cpp:332-369)."""
+        p = dict(self.params)
+        dense = type(self)(self.kernel, self.kappa, self.trial_space, self.test_space, p["intorder"], p["leafsize"],
+                           p["eta"], p["eps"], dense=True)
+        print(f"dense: {self.test_space.ndof} x {self.trial_space.ndof}")
+        D, H = dense.mat, self.mat
+        x = torch.ones(self.trial_space.ndof, dtype=self.mat.dtype(), device=device())
+        for _ in range(30):
+            x = x / torch.linalg.norm(x)
+            y = D * x
+            H.MultAdd(-1.0, x, y)
+            y = y.conj()
+            x2 = D.T * y
+            H.MultTransAdd(-1.0, y, x2)
+            x = x2.conj()
+        st = self.stats()
+        rate = st["nze"] / (self.test_space.ndof * self.trial_space.ndof)
+        print(f"compression rate {rate}  2-norm error {math.sqrt(float(torch.linalg.norm(x)))}")
+
+
+class IntegralOperatorC(IntegralOperator):
+    """Assembled boundary integral operator (complex-valued: Helmholtz)."""
+
+    is_complex = True
+
+
+def _dense_remap(method, leafsize, eta):
+    if method == "dense":
+        return True
+    if method != "aca":
+        raise ValueError(f"method must be 'aca' or 'dense', got {method!r}")
+    return False
+
+
+def _no_region(*regions):
+    for r in regions:
+        if r is not None:
+            raise NotImplementedError("definedon regions are not supported")
+
+
+def SingleLayerPotentialOperator(space, intorder=3, leafsize=40, eta=2., eps=1e-6, method="aca", testhmatrix=False,
+                                 mixed=False):
+    """Laplace single layer V (src/python_bem.cpp:27-38, src/ngbem.hpp:230-246)."""
+    dense = _dense_remap(method, leafsize, eta)
+    return IntegralOperator(_capi.LAPLACE_SL, 0.0, space, space, intorder, leafsize, eta, eps, dense, mixed,
+                            testhmatrix)
+
+
+def DoubleLayerPotentialOperator(trial_space, test_space, trial_definedon=None, test_definedon=None, intorder=3,
+                                 leafsize=40, eta=2., eps=1e-6, method="aca", testhmatrix=False, mixed=False):
+    """Laplace double layer K (src/python_bem.cpp:41-60, src/ngbem.hpp:257-274)."""
+    _no_region(trial_definedon, test_definedon)
+    dense = _dense_remap(method, leafsize, eta)
+    return IntegralOperator(_capi.LAPLACE_DL, 0.0, trial_space, test_space, intorder, leafsize, eta, eps, dense,
+                            mixed, testhmatrix)
+
+
+def HypersingularOperator(space, definedon=None, intorder=3, leafsize=40, eta=2., eps=1e-6, method="aca",
+                          testhmatrix=False, mixed=False):
+    """Laplace hypersingular W with n x grad evaluators (src/python_bem.cpp:63-75)."""
+    _no_region(definedon)
+    if method not in ("aca", "dense"):
+        raise ValueError("method must be 'aca' or 'dense'")
+    return IntegralOperator(_capi.LAPLACE_HS, 0.0, space, space, intorder, leafsize, eta, eps, False, mixed,
+                            testhmatrix)
+
+
+def _helmholtz(kernel, trial_space, test_space, kappa, intorder, leafsize, eta, eps, method, testhmatrix, mixed):
+    if kappa is None:
+        raise TypeError("kappa is required")
+    if method not in ("aca", "dense"):
+        raise ValueError("method must be 'aca' or 'dense'")
+    test_space = test_space if test_space is not None else trial_space
+    return IntegralOperatorC(kernel, kappa, trial_space, test_space, intorder, leafsize, eta, eps, False, mixed,
+                             testhmatrix)
+
+
+def HelmholtzSingleLayerPotentialOperator(trial_space, test_space=None, kappa=None, intorder=3, leafsize=40, eta=2.,
+                                          eps=1e-6, method="aca", testhmatrix=False, mixed=False):
+    """Helmholtz single layer, e^{ikr}/(4 pi r) (src/python_bem.cpp:79-87)."""
+    return _helmholtz(_capi.HELMHOLTZ_SL, trial_space, test_space, kappa, intorder, leafsize, eta, eps, method,
+                      testhmatrix, mixed)
+
+
+def HelmholtzDoubleLayerPotentialOperator(trial_space, test_space=None, kappa=None, intorder=3, leafsize=40, eta=2.,
+                                          eps=1e-6, method="aca", testhmatrix=False, mixed=False):
+    """Helmholtz double layer (src/python_bem.cpp:91-99)."""
+    return _helmholtz(_capi.HELMHOLTZ_DL, trial_space, test_space, kappa, intorder, leafsize, eta, eps, method,
+                      testhmatrix, mixed)
+
+
+def HelmholtzCombinedFieldOperator(trial_space, test_space=None, kappa=None, intorder=3, leafsize=40, eta=2.,
+                                   eps=1e-6, method="aca", testhmatrix=False, mixed=False):
+    """Combined field K - i kappa V (src/python_bem.cpp:102-110, src/ngbem.hpp:412-434)."""
+    return _helmholtz(_capi.HELMHOLTZ_CF, trial_space, test_space, kappa, intorder, leafsize, eta, eps, method,
+                      testhmatrix, mixed)
+
+
+def HelmholtzHypersingularOperator(trial_space, test_space=None, kappa=None, intorder=3, leafsize=40, eta=2.,
+                                   eps=1e-6, method="aca", testhmatrix=False, mixed=False):
+    """Helmholtz hypersingular with [n x grad, u] evaluators (src/python_bem.cpp:113-119)."""
+    return _helmholtz(_capi.HELMHOLTZ_HS, trial_space, test_space, kappa, intorder, leafsize, eta, eps, method,
+                      testhmatrix, mixed)
+
+
+def _maxwell(*a, **k):
+    raise NotImplementedError("Maxwell operators need HDivSurface spaces, which this library does not provide yet")
+
+
+def MaxwellSingleLayerPotentialOperator(space, kappa, definedon=None, intorder=3, leafsize=40, eta=2., eps=1e-6,
+                                        method="aca", testhmatrix=False):
+    """Not available: needs HDivSurface (src/python_bem.cpp:122-134)."""
+    _maxwell()
+
+
+def MaxwellSingleLayerPotentialOperatorCurl(space, kappa, definedon=None, intorder=3, leafsize=40, eta=2., eps=1e-6,
+                                            method="aca", testhmatrix=False):
+    """Not available: needs HCurl surface spaces (src/python_bem.cpp:137-149)."""
+    _maxwell()
+
+
+def MaxwellDoubleLayerPotentialOperator(trial_space, test_space, kappa, trial_definedon=None, test_definedon=None,
+                                        intorder=3, leafsize=40, eta=2., eps=1e-6, method="aca", testhmatrix=False):
+    """Not available: needs HDivSurface / HCurl spaces (src/python_bem.cpp:154-173)."""
+    _maxwell()

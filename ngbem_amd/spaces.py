@@ -1,0 +1,107 @@
+"""Boundary finite element spaces on flat triangle meshes.
+
+``SurfaceL2(mesh, order=0)``: one dof per triangle, shape function 1
+(NGSolve SurfaceL2, order 0).  ``H1(mesh, order=1)``: one dof per vertex,
+barycentric hat functions (NGSolve H1, order 1, on a surface mesh).  These are
+the spaces the reference's operators take (src/python_bem.cpp:27-173); dof
+numbering: triangle index (SurfaceL2) and vertex index (H1).
+"""
+from __future__ import annotations
+
+import ctypes
+
+import numpy as np
+
+from . import _capi
+from .mesh import Mesh
+
+
+def _mesh_handle(mesh: Mesh):
+    h = getattr(mesh, "_bem_handle", None)
+    if h is None:
+        L = _capi.lib()
+        out = ctypes.c_void_p()
+        _capi.check(L.bem_mesh_create(_capi.dptr(mesh.xyz), mesh.nv, _capi.iptr(mesh.tri), mesh.nt,
+                                      ctypes.byref(out)), "bem_mesh_create")
+        h = _Handle(out, L.bem_mesh_destroy)
+        mesh._bem_handle = h
+    return h
+
+
+class _Handle:
+    def __init__(self, ptr, destroy):
+        self.ptr = ptr
+        self._destroy = destroy
+
+    def __del__(self):
+        if self.ptr:
+            self._destroy(self.ptr)
+            self.ptr = None
+
+
+class FESpace:
+    kind = -1
+    order = -1
+
+    def __init__(self, mesh: Mesh, order: int, complex: bool = False, definedon=None, dirichlet=None,
+                 dual_mapping: bool = False):
+        if definedon is not None:
+            raise NotImplementedError("definedon regions are not supported: the space covers the whole mesh")
+        if dual_mapping:
+            raise NotImplementedError("dual_mapping=True is not supported")
+        if order != self.order:
+            raise NotImplementedError(f"{type(self).__name__} supports order {self.order} only")
+        self.mesh = mesh
+        self.is_complex = bool(complex)
+        self.dirichlet = dirichlet
+        self._mesh_h = _mesh_handle(mesh)
+        L = _capi.lib()
+        out = ctypes.c_void_p()
+        _capi.check(L.bem_space_create(self._mesh_h.ptr, self.kind, order, ctypes.byref(out)), "bem_space_create")
+        self._h = _Handle(out, L.bem_space_destroy)
+        self.ndof = int(L.bem_space_ndof(out))
+
+    @property
+    def handle(self):
+        return self._h.ptr
+
+    def __repr__(self):
+        return f"{type(self).__name__}(ndof={self.ndof}, order={self.order})"
+
+
+class SurfaceL2(FESpace):
+    kind = _capi.SPACE_P0
+    order = 0
+
+    def __init__(self, mesh, order=0, complex=False, definedon=None, dirichlet=None, dual_mapping=False):
+        super().__init__(mesh, order, complex, definedon, dirichlet, dual_mapping)
+
+
+class H1(FESpace):
+    kind = _capi.SPACE_P1
+    order = 1
+
+    def __init__(self, mesh, order=1, complex=False, definedon=None, dirichlet=None):
+        super().__init__(mesh, order, complex, definedon, dirichlet)
+
+
+def mass_matrix_coo(trial: FESpace, test: FESpace):
+    """Entries of the surface mass matrix  int u v ds  (test x trial), exact for
+    P0/P1 on flat triangles: rows, cols, values."""
+    m = trial.mesh
+    area = m.areas()
+    tri = m.tri.astype(np.int64)
+    nt = m.nt
+    e = np.arange(nt, dtype=np.int64)
+    if isinstance(trial, SurfaceL2) and isinstance(test, SurfaceL2):
+        return e, e, area
+    if isinstance(trial, H1) and isinstance(test, SurfaceL2):
+        return np.repeat(e, 3), tri.reshape(-1), np.repeat(area / 3.0, 3)
+    if isinstance(trial, SurfaceL2) and isinstance(test, H1):
+        return tri.reshape(-1), np.repeat(e, 3), np.repeat(area / 3.0, 3)
+    # H1 x H1: area/12 * (1 + delta_ij)
+    loc = np.array([[2, 1, 1], [1, 2, 1], [1, 1, 2]], dtype=np.float64) / 12.0
+    rows = np.repeat(tri, 3, axis=1).reshape(-1)
+    cols = np.tile(tri, (1, 3)).reshape(-1)
+    vals = (area[:, None, None] * loc[None]).reshape(-1)
+    return rows, cols, vals

@@ -1,0 +1,155 @@
+"""GPU parity: the HIP assembly and apply against the CPU FP64 oracle, through
+the C-ABI (libngbem_amd.so), on the same meshes and parameters.
+
+Bars: near-field entries within 1e-12 of the oracle (max abs difference over
+the block's max entry); ACA factors: same ranks (pivot paths can flip on
+rounding ties, so 90%) and ||U V - U' V'||_F <= 10 eps ||U' V'||_F per block;
+H-matvec with identical blocks: 1e-13 relative (summation order only).
+"""
+import numpy as np
+import pytest
+import torch
+
+import ngbem_amd as nb
+from oracle import oracle as O
+
+pytestmark = pytest.mark.gpu
+
+FACT = {
+    "LSL": (nb.SingleLayerPotentialOperator, O.LAPLACE_SL, False),
+    "LDL": (nb.DoubleLayerPotentialOperator, O.LAPLACE_DL, False),
+    "LHS": (nb.HypersingularOperator, O.LAPLACE_HS, False),
+    "HSL": (nb.HelmholtzSingleLayerPotentialOperator, O.HELMHOLTZ_SL, True),
+    "HDL": (nb.HelmholtzDoubleLayerPotentialOperator, O.HELMHOLTZ_DL, True),
+    "HCF": (nb.HelmholtzCombinedFieldOperator, O.HELMHOLTZ_CF, True),
+    "HHS": (nb.HelmholtzHypersingularOperator, O.HELMHOLTZ_HS, True),
+}
+
+# (kernel, trial space, test space, level, intorder, leafsize, eta, eps, kappa)
+CASES = [
+    ("LSL", 0, 0, 2, 4, 16, 2.0, 1e-8, 0.0),
+    ("LSL", 0, 0, 3, 3, 40, 3.0, 1e-6, 0.0),
+    ("LSL", 1, 1, 2, 4, 16, 2.0, 1e-8, 0.0),
+    ("LDL", 0, 0, 2, 4, 16, 2.0, 1e-8, 0.0),
+    ("LDL", 1, 0, 2, 4, 16, 2.0, 1e-8, 0.0),
+    ("LHS", 1, 1, 2, 4, 16, 2.0, 1e-8, 0.0),
+    ("HSL", 0, 0, 2, 4, 16, 2.0, 1e-8, 4.0),
+    ("HDL", 0, 0, 2, 4, 16, 2.0, 1e-8, 4.0),
+    ("HDL", 1, 0, 2, 4, 16, 2.0, 1e-8, 4.0),
+    ("HCF", 0, 0, 2, 4, 16, 2.0, 1e-8, 4.0),
+    ("HCF", 0, 0, 3, 6, 32, 2.0, 1e-6, 10.0),
+    ("HHS", 1, 1, 2, 4, 16, 2.0, 1e-8, 4.0),
+]
+
+
+def _space(mesh, kind):
+    return nb.SurfaceL2(mesh, order=0) if kind == 0 else nb.H1(mesh, order=1)
+
+
+def _build(case):
+    name, trk, tek, level, order, leaf, eta, eps, kappa = case
+    mesh = nb.icosphere(level)
+    fac, kid, cplx = FACT[name]
+    trial, test = _space(mesh, trk), _space(mesh, tek)
+    kw = dict(intorder=order, leafsize=leaf, eta=eta, eps=eps)
+    print(f"case {case}", flush=True)
+    if name in ("LSL",):
+        op = fac(trial, **kw)
+    elif name == "LHS":
+        op = fac(trial, **kw)
+    elif name == "LDL":
+        op = fac(trial, test, **kw)
+    else:
+        op = fac(trial, test, kappa=kappa, **kw)
+    ref = O.OracleOp(mesh, kid, kappa=kappa, trial=trk, test=tek, **kw)
+    return op, ref
+
+
+@pytest.mark.parametrize("case", CASES, ids=[f"{c[0]}-{c[1]}{c[2]}-L{c[3]}-o{c[4]}" for c in CASES])
+def test_blocks_match_oracle(gpu, case):
+    op, ref = _build(case)
+    eps = case[7]
+    ob = ref.blocks()
+    assert op.num_blocks() == len(ob)
+    same_rank = 0
+    nfar = 0
+    for b, rb in enumerate(ob):
+        info = op.block_info(b)
+        assert (info["row_cluster"], info["col_cluster"]) == (rb["row"], rb["col"])
+        rows, cols = op.block_dofs(b)
+        orows, ocols = ref.block_dofs(b)
+        assert np.array_equal(rows, orows) and np.array_equal(cols, ocols)
+        data = op.block_data(b)
+        if info["rank"] < 0:
+            D = ref.dense(cols, rows)
+            if not op.is_complex:
+                assert np.abs(D.imag).max() == 0.0
+                D = D.real
+            scale = np.abs(D).max()
+            assert np.abs(data - D).max() <= 1e-12 * scale, f"block {b}"
+        else:
+            nfar += 1
+            UV = ref.aca(cols, rows)
+            assert UV is not None
+            U, V = UV
+            same_rank += U.shape[1] == info["rank"]
+            A_ref = U @ V
+            A = data[0] @ data[1]
+            assert np.linalg.norm(A - A_ref) <= 10 * eps * np.linalg.norm(A_ref), f"far block {b}"
+    if nfar:
+        assert same_rank >= 0.9 * nfar
+
+
+@pytest.mark.parametrize("case", CASES[:1] + CASES[4:6] + CASES[9:10] + CASES[11:12],
+                         ids=lambda c: f"{c[0]}-{c[1]}{c[2]}")
+def test_matvec_matches_oracle(gpu, case):
+    op, ref = _build(case)
+    # load the GPU-assembled blocks into the oracle, so only the apply differs
+    for b in range(op.num_blocks()):
+        d = op.block_data(b)
+        if isinstance(d, tuple):
+            ref.set_block(b, np.concatenate([d[0].ravel(), d[1].ravel()]), d[0].shape[1])
+        else:
+            ref.set_block(b, d.ravel(), -1)
+    rng = np.random.default_rng(1)
+    cplx = op.is_complex
+    for trans in (False, True):
+        n_in = op.test_space.ndof if trans else op.trial_space.ndof
+        x = rng.standard_normal(n_in) + (1j * rng.standard_normal(n_in) if cplx else 0)
+        y_ref = ref.apply(x.astype(np.complex128), trans=trans)
+        M = op.mat.T if trans else op.mat
+        y = (M * torch.from_numpy(x).to(gpu)).cpu().numpy()
+        assert np.linalg.norm(y - y_ref) <= 1e-13 * np.linalg.norm(y_ref)
+    if cplx:  # adjoint = conj of the plain transpose of conj(x)
+        x = rng.standard_normal(op.test_space.ndof) + 1j * rng.standard_normal(op.test_space.ndof)
+        y = (op.mat.H * torch.from_numpy(x).to(gpu)).cpu().numpy()
+        y_ref = np.conj(ref.apply(np.conj(x), trans=True))
+        assert np.linalg.norm(y - y_ref) <= 1e-13 * np.linalg.norm(y_ref)
+
+
+def test_hmatrix_vs_dense(gpu):
+    """The reference's ACA criterion (tests/test_aca.py:21-28): ||(V_dense - V_H) x|| < eps."""
+    mesh = nb.icosphere(3)
+    fes = nb.SurfaceL2(mesh)
+    eps = 1e-6
+    Vd = nb.SingleLayerPotentialOperator(fes, intorder=4, method="dense")
+    Vh = nb.SingleLayerPotentialOperator(fes, intorder=4, leafsize=40, eta=3.0, eps=eps)
+    assert Vh.stats()["n_far"] > 0
+    x = torch.from_numpy(np.random.default_rng(4711).random(fes.ndof)).to(gpu)
+    err = torch.linalg.norm(Vd.mat * x - Vh.mat * x).item()
+    assert err < eps
+    assert Vh.mat.nze < Vd.mat.nze
+
+
+def test_mixed_matches_fp64(gpu):
+    mesh = nb.icosphere(3)
+    fes = nb.SurfaceL2(mesh)
+    C = nb.HelmholtzCombinedFieldOperator(fes, fes, kappa=5.0, intorder=4, leafsize=32, eps=1e-6, mixed=True)
+    x = torch.randn(fes.ndof, dtype=torch.complex128, device=gpu)
+    y64 = C.mat * x
+    y32 = C.mat.mixed * x
+    rel = (torch.linalg.norm(y64 - y32) / torch.linalg.norm(y64)).item()
+    assert rel < 1e-5
+    yt64 = C.mat.T * x
+    yt32 = C.mat.mixed.T * x
+    assert (torch.linalg.norm(yt64 - yt32) / torch.linalg.norm(yt64)).item() < 1e-5
