@@ -335,12 +335,21 @@ int bem_op_block_data(const bem_op* h, int b, void* out) {
     const int slot = op.blk_slot[b];
     if (bl.near || bl.fallback) {
       const bem::NearBlk& nb = op.near[slot];
-      BEM_HIP(hipMemcpy(out, (const char*)op.d_near_val.p + nb.off * esz, (size_t)nb.m * nb.n * esz,
-                        hipMemcpyDeviceToHost));
+      // stored column-major; exported row-major
+      std::vector<char> tmp((size_t)nb.m * nb.n * esz);
+      BEM_HIP(hipMemcpy(tmp.data(), (const char*)op.d_near_val.p + nb.off * esz, tmp.size(), hipMemcpyDeviceToHost));
+      for (int c = 0; c < nb.n; c++)
+        for (int r = 0; r < nb.m; r++)
+          std::memcpy((char*)out + ((size_t)r * nb.n + c) * esz, tmp.data() + ((size_t)c * nb.m + r) * esz, esz);
     } else {
       const bem::FarBlk& fb = op.far[slot];
       const size_t nu = (size_t)fb.m * fb.R, nv = (size_t)fb.R * fb.n;
-      if (nu) BEM_HIP(hipMemcpy(out, (const char*)op.d_fU.p + fb.uoff * esz, nu * esz, hipMemcpyDeviceToHost));
+      // U is stored transposed (R x m); exported as m x R
+      std::vector<char> tmp(nu * esz);
+      if (nu) BEM_HIP(hipMemcpy(tmp.data(), (const char*)op.d_fU.p + fb.uoff * esz, nu * esz, hipMemcpyDeviceToHost));
+      for (int l = 0; l < fb.R; l++)
+        for (int i = 0; i < fb.m; i++)
+          std::memcpy((char*)out + ((size_t)i * fb.R + l) * esz, tmp.data() + ((size_t)l * fb.m + i) * esz, esz);
       if (nv)
         BEM_HIP(hipMemcpy((char*)out + nu * esz, (const char*)op.d_fV.p + fb.voff * esz, nv * esz,
                           hipMemcpyDeviceToHost));

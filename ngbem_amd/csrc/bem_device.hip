@@ -286,7 +286,7 @@ __global__ __launch_bounds__(256) void k_near_regular(const NearBlk* __restrict_
   const NearBlk nb = blks[ch.x];
   const int p = ch.y + (int)threadIdx.x;
   if (p >= nb.pm * nb.pn) return;
-  const int ia = p / nb.pn, ja = p - ia * nb.pn;
+  const int ja = p / nb.pm, ia = p - ja * nb.pm;  // consecutive lanes: consecutive test rows
   const int e = pIb[nb.pI + ia], f = pJb[nb.pJ + ja];
   const int4 ve = vid[e], vf = vid[f];
   const bool touch = ve.x == vf.x || ve.x == vf.y || ve.x == vf.z || ve.y == vf.x || ve.y == vf.y || ve.y == vf.z ||
@@ -323,7 +323,7 @@ __global__ __launch_bounds__(256) void k_near_regular(const NearBlk* __restrict_
     }
   }
   if (DIRECT) {
-    val[nb.off + (int64_t)ia * nb.n + ja] = el[0][0];
+    val[nb.off + (int64_t)ja * nb.m + ia] = el[0][0];
   } else {
     const int vi[3] = {ve.x, ve.y, ve.z}, vj[3] = {vf.x, vf.y, vf.z};
 #pragma unroll
@@ -334,7 +334,7 @@ __global__ __launch_bounds__(256) void k_near_regular(const NearBlk* __restrict_
       for (int jj = 0; jj < EJ::NL; jj++) {
         const int col = EJ::P1 ? ipr[vj[jj]] - nb.c0 : ja;
         if (col < 0 || col >= nb.n) continue;
-        atomic_add(&val[nb.off + (int64_t)row * nb.n + col], el[ii][jj]);
+        atomic_add(&val[nb.off + (int64_t)col * nb.m + row], el[ii][jj]);
       }
     }
   }
@@ -401,7 +401,7 @@ __global__ __launch_bounds__(256) void k_near_singular(const int4* __restrict__ 
   }
   if (DIRECT) {
     const int row = ipt[e] - nb.r0, col = ipr[f] - nb.c0;
-    val[nb.off + (int64_t)row * nb.n + col] = el[0][0];
+    val[nb.off + (int64_t)col * nb.m + row] = el[0][0];
   } else {
 #pragma unroll
     for (int kx = 0; kx < EI::NL; kx++) {
@@ -411,7 +411,7 @@ __global__ __launch_bounds__(256) void k_near_singular(const int4* __restrict__ 
       for (int ky = 0; ky < EJ::NL; ky++) {
         const int col = EJ::P1 ? ipr[vidf[4 * f + py[ky]]] - nb.c0 : ipr[f] - nb.c0;
         if (col < 0 || col >= nb.n) continue;
-        atomic_add(&val[nb.off + (int64_t)row * nb.n + col], el[kx][ky]);
+        atomic_add(&val[nb.off + (int64_t)col * nb.m + row], el[kx][ky]);
       }
     }
   }
@@ -698,7 +698,7 @@ __global__ __launch_bounds__(256) void k_project(const ProjTask* __restrict__ ta
   const int* cmo = maps + b.cmap;
   // U2: m x R
   for (int idx = threadIdx.x; idx < b.m * R; idx += 256) {
-    const int ii = idx / R, col = idx - ii * R;
+    const int col = idx / b.m, ii = idx - col * b.m;
     int t = 0;
     while (t + 1 < KER::NTERM && col >= r0[t + 1]) t++;
     const TermC tm = KER::term(t);
@@ -719,7 +719,7 @@ __global__ __launch_bounds__(256) void k_project(const ProjTask* __restrict__ ta
         s += rmul(Uc[a * ntq + q], sh[kl * EI::DIM + tm.tec] * w);
       }
     }
-    oU[tk.ou + (size_t)ii * R + col] = rmul(s, tm.fac);
+    oU[tk.ou + (size_t)col * b.m + ii] = rmul(s, tm.fac);
   }
   // V2: R x n
   for (int idx = threadIdx.x; idx < b.n * R; idx += 256) {
@@ -773,145 +773,166 @@ __global__ void k_scatter(const int* __restrict__ perm, int n, const VT* __restr
   y[d] = r;
 }
 
-// t_b = V_b x[c0 : c0+n]  (forward)  or  t_b = U_b^T x[r0 : r0+m] (transpose)
+// Storage used by the apply: dense blocks column-major (entry (r, c) of an
+// m x n block at off + c*m + r), low-rank blocks as Ut (R x m) and V (R x n),
+// both row-major.  Forward y = A x: lanes own rows, so every load of a column
+// (dense) or of a rank row (Ut) is one coalesced transaction and no
+// cross-lane reduction is needed.
+
+// t_b = V_b x[c0 : c0+n] (forward)  or  t_b = Ut_b x[r0 : r0+m] (transpose);
+// one wave per rank row, lanes along the contiguous row
 template <class ST, class VT, bool TRANS, bool CONJ>
-__global__ __launch_bounds__(256) void k_mv_far_t(const FarBlk* __restrict__ fb, const ST* __restrict__ fU,
+__global__ __launch_bounds__(256) void k_mv_far_t(const FarBlk* __restrict__ fb, const ST* __restrict__ fUt,
                                                   const ST* __restrict__ fV, const VT* __restrict__ xp,
                                                   VT* __restrict__ tbuf) {
   const FarBlk b = fb[blockIdx.x];
-  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
-  if (!TRANS) {
-    const ST* V = fV + b.voff;
-    for (int l = wv; l < b.R; l += 4) {
-      VT acc = zero_<VT>();
-      const ST* row = V + (size_t)l * b.n;
-      for (int j = lane; j < b.n; j += 64) {
-        VT a;
-        cvt(a, row[j]);
-        acc += a * xp[b.c0 + j];
-      }
-      acc = wave_sum(acc);
-      if (lane == 0) tbuf[b.toff + l] = acc;
+  const int lane = threadIdx.x & 63;
+  const int wv = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+  const ST* F = TRANS ? fUt + b.uoff : fV + b.voff;
+  const int len = TRANS ? b.m : b.n;
+  const VT* xx = xp + (TRANS ? b.r0 : b.c0);
+  for (int l = wv; l < b.R; l += 4) {
+    const ST* row = F + (int64_t)l * len;
+    VT a0 = zero_<VT>(), a1 = zero_<VT>();
+    int j = lane;
+    for (; j + 64 < len; j += 128) {
+      VT u, v;
+      cvt(u, row[j]);
+      cvt(v, row[j + 64]);
+      if (CONJ) { u = cconj(u); v = cconj(v); }
+      a0 += u * xx[j];
+      a1 += v * xx[j + 64];
     }
-  } else {
-    // lanes over rank columns, waves over row ranges; U is m x R row-major
-    __shared__ VT sh[4][64];
-    const ST* U = fU + b.uoff;
-    for (int l0 = 0; l0 < b.R; l0 += 64) {
-      const int l = l0 + lane;
-      VT acc = zero_<VT>();
-      if (l < b.R)
-        for (int i = wv; i < b.m; i += 4) {
-          VT a;
-          cvt(a, U[(size_t)i * b.R + l]);
-          if (CONJ) a = cconj(a);
-          acc += a * xp[b.r0 + i];
-        }
-      sh[wv][lane] = acc;
-      __syncthreads();
-      if (wv == 0 && l < b.R) tbuf[b.toff + l] = sh[0][lane] + sh[1][lane] + sh[2][lane] + sh[3][lane];
-      __syncthreads();
+    if (j < len) {
+      VT u;
+      cvt(u, row[j]);
+      if (CONJ) u = cconj(u);
+      a0 += u * xx[j];
     }
+    a0 = wave_sum(a0 + a1);
+    if (lane == 0) tbuf[b.toff + l] = a0;
   }
 }
 
 struct Tile {
-  int p0, len;       // positions
+  int p0, len;       // positions owned (len <= 64)
   int s0, s1;        // segment range
+  int roff;          // offset of this 64-chunk inside its leaf
+  int pad;
 };
 
-// forward: one workgroup per leaf row cluster; each wave owns rows, lanes run
-// along the row of every covering block
+// forward: one workgroup per (leaf row cluster, 64-row chunk); lane r owns row
+// p0 + r, the four waves split the columns (dense) / rank rows (low rank) of
+// every covering block and are summed through LDS once at the end
 template <class ST, class VT>
 __global__ __launch_bounds__(256) void k_mv_rows(const Tile* __restrict__ tiles, const Seg* __restrict__ segs,
                                                  const NearBlk* __restrict__ nbk, const FarBlk* __restrict__ fbk,
-                                                 const ST* __restrict__ nval, const ST* __restrict__ fU,
+                                                 const ST* __restrict__ nval, const ST* __restrict__ fUt,
                                                  const VT* __restrict__ xp, const VT* __restrict__ tbuf,
                                                  VT* __restrict__ yp) {
+  __shared__ VT red[4][64];
   const Tile tl = tiles[blockIdx.x];
-  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
-  for (int r = wv; r < tl.len; r += 4) {
-    VT acc = zero_<VT>();
-    for (int s = tl.s0; s < tl.s1; s++) {
-      const Seg sg = segs[s];
-      if (!sg.far) {
-        const NearBlk b = nbk[sg.blk];
-        const ST* row = nval + b.off + (int64_t)(sg.off + r) * b.n;
-        const VT* xx = xp + b.c0;
-        int j = lane;
-        for (; j + 192 < b.n; j += 256) {
-          VT a0, a1, a2, a3;
-          cvt(a0, row[j]);
-          cvt(a1, row[j + 64]);
-          cvt(a2, row[j + 128]);
-          cvt(a3, row[j + 192]);
-          acc += a0 * xx[j];
-          acc += a1 * xx[j + 64];
-          acc += a2 * xx[j + 128];
-          acc += a3 * xx[j + 192];
-        }
-        for (; j < b.n; j += 64) {
-          VT a;
-          cvt(a, row[j]);
-          acc += a * xx[j];
-        }
-      } else {
-        const FarBlk b = fbk[sg.blk];
-        const ST* row = fU + b.uoff + (int64_t)(sg.off + r) * b.R;
-        for (int l = lane; l < b.R; l += 64) {
-          VT a;
-          cvt(a, row[l]);
-          acc += a * tbuf[b.toff + l];
-        }
+  const int lane = threadIdx.x & 63;
+  const int wv = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+  const bool act = lane < tl.len;
+  const int r = act ? lane : 0;
+  VT acc0 = zero_<VT>(), acc1 = zero_<VT>();
+  for (int s = tl.s0; s < tl.s1; s++) {
+    const Seg sg = segs[s];
+    if (!sg.far) {
+      const NearBlk b = nbk[sg.blk];
+      const ST* col = nval + b.off + sg.off + tl.roff + r;
+      const VT* xx = xp + b.c0;
+      const int64_t ld = b.m;
+      int j = wv;
+      for (; j + 4 < b.n; j += 8) {
+        VT u, v;
+        cvt(u, col[j * ld]);
+        cvt(v, col[(j + 4) * ld]);
+        acc0 += u * xx[j];
+        acc1 += v * xx[j + 4];
+      }
+      if (j < b.n) {
+        VT u;
+        cvt(u, col[j * ld]);
+        acc0 += u * xx[j];
+      }
+    } else {
+      const FarBlk b = fbk[sg.blk];
+      const ST* col = fUt + b.uoff + sg.off + tl.roff + r;
+      const VT* tt = tbuf + b.toff;
+      const int64_t ld = b.m;
+      int l = wv;
+      for (; l + 4 < b.R; l += 8) {
+        VT u, v;
+        cvt(u, col[l * ld]);
+        cvt(v, col[(l + 4) * ld]);
+        acc0 += u * tt[l];
+        acc1 += v * tt[l + 4];
+      }
+      if (l < b.R) {
+        VT u;
+        cvt(u, col[l * ld]);
+        acc0 += u * tt[l];
       }
     }
-    acc = wave_sum(acc);
-    if (lane == 0) yp[tl.p0 + r] = acc;
   }
+  red[wv][lane] = acc0 + acc1;
+  __syncthreads();
+  if (wv == 0 && act) yp[tl.p0 + lane] = red[0][lane] + red[1][lane] + red[2][lane] + red[3][lane];
 }
 
-// transpose: one workgroup per leaf column cluster; lanes own columns, waves
-// split the rows of each covering block and combine through LDS
+// transpose: one workgroup per (leaf column cluster, 64-column chunk); dense
+// blocks: one wave per column, lanes along the contiguous column, wave sum;
+// low rank: lanes own columns, waves split the rank rows of V
 template <class ST, class VT, bool CONJ>
 __global__ __launch_bounds__(256) void k_mv_cols(const Tile* __restrict__ tiles, const Seg* __restrict__ segs,
                                                  const NearBlk* __restrict__ nbk, const FarBlk* __restrict__ fbk,
                                                  const ST* __restrict__ nval, const ST* __restrict__ fV,
                                                  const VT* __restrict__ xp, const VT* __restrict__ tbuf,
                                                  VT* __restrict__ yp) {
-  __shared__ VT sh[4][64];
+  __shared__ VT red[4][64];
+  __shared__ VT colsum[64];
   const Tile tl = tiles[blockIdx.x];
-  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
-  for (int c0 = 0; c0 < tl.len; c0 += 64) {
-    const int c = c0 + lane;
-    VT acc = zero_<VT>();
-    if (c < tl.len)
-      for (int s = tl.s0; s < tl.s1; s++) {
-        const Seg sg = segs[s];
-        if (!sg.far) {
-          const NearBlk b = nbk[sg.blk];
-          const ST* col = nval + b.off + sg.off + c;
-          for (int i = wv; i < b.m; i += 4) {
-            VT a;
-            cvt(a, col[(int64_t)i * b.n]);
-            if (CONJ) a = cconj(a);
-            acc += a * xp[b.r0 + i];
-          }
-        } else {
-          const FarBlk b = fbk[sg.blk];
-          const ST* col = fV + b.voff + sg.off + c;
-          for (int l = wv; l < b.R; l += 4) {
-            VT a;
-            cvt(a, col[(int64_t)l * b.n]);
-            if (CONJ) a = cconj(a);
-            acc += a * tbuf[b.toff + l];
-          }
+  const int lane = threadIdx.x & 63;
+  const int wv = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+  const bool act = lane < tl.len;
+  const int c = act ? lane : 0;
+  if (wv == 0) colsum[lane] = zero_<VT>();
+  __syncthreads();
+  VT acc = zero_<VT>();
+  for (int s = tl.s0; s < tl.s1; s++) {
+    const Seg sg = segs[s];
+    if (!sg.far) {
+      const NearBlk b = nbk[sg.blk];
+      const VT* xx = xp + b.r0;
+      for (int cc = wv; cc < tl.len; cc += 4) {  // wave-owned column (same wave every segment)
+        const ST* col = nval + b.off + (int64_t)(sg.off + tl.roff + cc) * b.m;
+        VT a = zero_<VT>();
+        for (int i = lane; i < b.m; i += 64) {
+          VT u;
+          cvt(u, col[i]);
+          if (CONJ) u = cconj(u);
+          a += u * xx[i];
         }
+        a = wave_sum(a);
+        if (lane == 0) colsum[cc] += a;
       }
-    sh[wv][lane] = acc;
-    __syncthreads();
-    if (wv == 0 && c < tl.len) yp[tl.p0 + c] = sh[0][lane] + sh[1][lane] + sh[2][lane] + sh[3][lane];
-    __syncthreads();
+    } else {
+      const FarBlk b = fbk[sg.blk];
+      const ST* row = fV + b.voff + sg.off + tl.roff + c;
+      const VT* tt = tbuf + b.toff;
+      for (int l = wv; l < b.R; l += 4) {
+        VT u;
+        cvt(u, row[(int64_t)l * b.n]);
+        if (CONJ) u = cconj(u);
+        acc += u * tt[l];
+      }
+    }
   }
+  red[wv][lane] = acc;
+  __syncthreads();
+  if (wv == 0 && act) yp[tl.p0 + lane] = colsum[lane] + red[0][lane] + red[1][lane] + red[2][lane] + red[3][lane];
 }
 
 template <class S, class D>
@@ -1396,10 +1417,11 @@ static void plan(const Operator& op, bool trans, std::vector<Tile>& tiles, std::
   segs.clear();
   for (int i = 0; i < nleaf; i++) {
     const Cluster& c = tree.cl[tree.leaves[i]];
-    Tile t{(int)c.permupos, (int)c.number, (int)segs.size(), 0};
+    const int s0 = (int)segs.size();
     segs.insert(segs.end(), per[i].begin(), per[i].end());
-    t.s1 = (int)segs.size();
-    tiles.push_back(t);
+    const int s1 = (int)segs.size();
+    for (int ro = 0; ro < (int)c.number; ro += 64)  // leaves above 64 dofs: one tile per 64-chunk
+      tiles.push_back(Tile{(int)c.permupos + ro, std::min(64, (int)c.number - ro), s0, s1, ro, 0});
   }
 }
 
