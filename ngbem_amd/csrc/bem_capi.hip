@@ -52,18 +52,18 @@ double now() {
 // reference's exception path (src/ngbem.cpp:839-842, 143-153)
 void mark_fallbacks(bem::Operator& op) {
   const bem::Mesh& m = *op.mesh;
-  std::vector<int> stamp(m.nv, -1), pI, pJ;
+  std::vector<int> stamp(m.nv, -1);
+  const bem::ClusterPatches& ci = *op.cp_test;
+  const bem::ClusterPatches& cj = op.cp_trial;
   for (size_t b = 0; b < op.blocks.size(); b++) {
     bem::Block& bl = op.blocks[b];
     if (bl.near) continue;
-    bem::patch(*op.test, &op.test_tree->perm[bl.r0], bl.m, pI);
-    bem::patch(*op.trial, &op.trial_tree.perm[bl.c0], bl.n, pJ);
-    for (int e : pI)
-      for (int k = 0; k < 3; k++) stamp[m.tri[3 * e + k]] = (int)b;
+    for (int k = ci.off[bl.row]; k < ci.off[bl.row + 1]; k++)
+      for (int c = 0; c < 3; c++) stamp[m.tri[3 * ci.list[k] + c]] = (int)b;
     bool shared = false;
-    for (int e : pJ)
-      for (int k = 0; k < 3; k++)
-        if (stamp[m.tri[3 * e + k]] == (int)b) shared = true;
+    for (int k = cj.off[bl.col]; k < cj.off[bl.col + 1] && !shared; k++)
+      for (int c = 0; c < 3; c++)
+        if (stamp[m.tri[3 * cj.list[k] + c]] == (int)b) shared = true;
     if (shared) {
       bl.fallback = 1;
       op.n_fallback++;
@@ -152,24 +152,38 @@ int bem_op_create(int kernel, double kappa, bem_space* trial, bem_space* test, c
     op.trial = trial->s;
     op.test = test->s;
     op.same_space = trial->s == test->s || (trial->s->kind == test->s->kind);
+    bem::PhaseTimer pt_("create");
     op.trial_tree.build(*op.trial, op.leafsize);
     if (op.same_space) op.test_tree = &op.trial_tree;
     else {
       op.test_tree_own.build(*op.test, op.leafsize);
       op.test_tree = &op.test_tree_own;
     }
+    pt_.mark("cluster trees");
     bem::partition(*op.test_tree, op.trial_tree, op.eta, op.blocks);
+    pt_.mark("block partition");
+    op.cp_trial.build(*op.trial, op.trial_tree);
+    if (op.same_space) op.cp_test = &op.cp_trial;
+    else {
+      op.cp_test_own.build(*op.test, *op.test_tree);
+      op.cp_test = &op.cp_test_own;
+    }
+    op.lookup.build(*op.test_tree, op.blocks);
+    pt_.mark("cluster patches + lookup");
     mark_fallbacks(op);
+    pt_.mark("fallback check");
     op.blk_slot.resize(op.blocks.size());
     for (size_t b = 0; b < op.blocks.size(); b++) {
       const bem::Block& bl = op.blocks[b];
       if (bl.near || bl.fallback) {
         op.blk_slot[b] = (int)op.near.size();
+        op.near_block.push_back((int)b);
         bem::NearBlk nb{};
         nb.r0 = bl.r0; nb.m = bl.m; nb.c0 = bl.c0; nb.n = bl.n;
         op.near.push_back(nb);
       } else {
         op.blk_slot[b] = (int)op.far.size();
+        op.far_block.push_back((int)b);
         bem::FarBlk fb{};
         fb.r0 = bl.r0; fb.m = bl.m; fb.c0 = bl.c0; fb.n = bl.n;
         op.far.push_back(fb);
@@ -188,11 +202,17 @@ int bem_op_assemble(bem_op* h, void* stream) {
     bem::Operator& op = h->op;
     hipStream_t s = (hipStream_t)stream;
     const double t0 = now();
+    bem::PhaseTimer pt_("assemble");
     bem::setup_device(op, s);
+    pt_.mark("device setup");
     bem::assemble_near(op, s);
+    pt_.mark("near field");
     bem::assemble_far(op, s);
+    pt_.mark("far field");
     bem::build_matvec_plan(op, s);
+    pt_.mark("matvec plan");
     if (op.mixed) bem::make_mixed_copy(op, s);
+    pt_.mark("mixed copy");
     op.t_setup += now() - t0 - op.t_near - op.t_far;
     op.assembled = true;
   });

@@ -21,6 +21,8 @@
 
 #include <algorithm>
 #include <chrono>
+#include <cstdio>
+#include <cstdlib>
 #include <cmath>
 #include <cstring>
 #include <numeric>
@@ -420,10 +422,10 @@ __global__ __launch_bounds__(256) void k_near_singular(const int4* __restrict__ 
 // ------------------------------------------------------------------- ACA --
 // src/ngbem.cpp:902-1049 for one (far block, component) per workgroup.
 // Workspace: U column l at U[l*np + i] (point rows), V row l at V[l*nq + j].
+// A task that reaches its rank capacity saves (k, ik, ikm1, jkm1, norm2) and is
+// resumed with a larger workspace, so it never recomputes its first steps.
 
-constexpr int ACA_THREADS = 256;
-constexpr int ACA_WAVES = ACA_THREADS / 64;
-constexpr int IP_CHUNK = 8;  // inner products reduced together
+constexpr int IP_CHUNK = 4;  // inner products reduced together
 
 struct ArgMax {
   double a2;
@@ -451,27 +453,16 @@ __device__ __forceinline__ zc div_(double a, zc b) {  // a / b, a real
   return {a * b.x / d, -a * b.y / d};
 }
 
-template <class T>
+template <int NW>
 __device__ ArgMax block_argmax(ArgMax v, ArgMax* sh) {
   v = wave_argmax(v);
   const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
   if (lane == 0) sh[wv] = v;
   __syncthreads();
   ArgMax r = sh[0];
-  for (int w = 1; w < ACA_WAVES; w++)
+#pragma unroll
+  for (int w = 1; w < NW; w++)
     if (better(sh[w], r)) r = sh[w];
-  __syncthreads();
-  return r;
-}
-
-template <class T>
-__device__ __forceinline__ T block_sum(T v, T* sh) {
-  v = wave_sum(v);
-  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
-  if (lane == 0) sh[wv] = v;
-  __syncthreads();
-  T r = sh[0];
-  for (int w = 1; w < ACA_WAVES; w++) r += sh[w];
   __syncthreads();
   return r;
 }
@@ -500,21 +491,21 @@ __device__ __forceinline__ typename KER::T kcomp(const KER& ker, P3 x, P3 y, P3 
   return r;
 }
 
-template <class KER>
-__global__ __launch_bounds__(ACA_THREADS) void k_aca(const AcaTask* __restrict__ tasks, const FarBlk* __restrict__ fb,
-                                                     const int* __restrict__ fpatch, const double* __restrict__ quad,
-                                                     const double* __restrict__ geo, int ntq, KER ker, double eps,
-                                                     typename KER::T* __restrict__ wsU, typename KER::T* __restrict__ wsV,
-                                                     int2* __restrict__ result) {
+template <class KER, int NTH>
+__global__ __launch_bounds__(NTH) void k_aca(AcaTask* __restrict__ tasks, const FarBlk* __restrict__ fb,
+                                             const int* __restrict__ fpatch, const double* __restrict__ quad,
+                                             const double* __restrict__ geo, int ntq, KER ker, double eps,
+                                             typename KER::T* __restrict__ wsU, typename KER::T* __restrict__ wsV) {
   typedef typename KER::T T;
+  constexpr int NW = NTH / 64;
   extern __shared__ unsigned char smem_raw[];
   T* s_coef = reinterpret_cast<T*>(smem_raw);  // cap entries
-  __shared__ ArgMax s_am[ACA_WAVES];
-  __shared__ T s_red[ACA_WAVES * 2 * IP_CHUNK];
-  __shared__ double s_dred[ACA_WAVES * 2];
+  __shared__ ArgMax s_am[NW];
+  __shared__ T s_red[NW * 2 * IP_CHUNK];
+  __shared__ double s_dred[NW * 2];
   __shared__ int s_flag;
 
-  const AcaTask tk = tasks[blockIdx.x];
+  AcaTask tk = tasks[blockIdx.x];
   const FarBlk b = fb[tk.blk];
   const PointSet X{fpatch + b.pI, quad, geo, ntq}, Y{fpatch + b.pJ, quad, geo, ntq};
   const int np = tk.np, nq = tk.nq, comp = tk.comp;
@@ -524,27 +515,44 @@ __global__ __launch_bounds__(ACA_THREADS) void k_aca(const AcaTask* __restrict__
   const int kmax = min(pmax, tk.cap);
   const int tid = threadIdx.x;
 
-  int ik = 0, ikm1 = 0, jkm1 = nq + 1, rank = -1;
-  double norm2 = 0.0;
-  for (int k = 0; k < kmax; k++) {
-    for (int l = tid; l < k; l += ACA_THREADS) s_coef[l] = U[(size_t)l * np + ik];
+  int ik = tk.ik, ikm1 = tk.ikm1, jkm1 = tk.jkm1, rank = -1;
+  double norm2 = tk.norm2;
+  int k = tk.k0;
+  for (; k < kmax; k++) {
+    for (int l = tid; l < k; l += NTH) s_coef[l] = U[(size_t)l * np + ik];
     __syncthreads();
     // row ik of the residual: v_j = K(x_ik, y_j) - sum_l V[l][j] U[ik][l]
     P3 xi, nxi;
     X.get(ik, xi, nxi);
     ArgMax best{-1.0, 0x7fffffff, 0.0, 0.0};
     T* vk = V + (size_t)k * nq;
-    for (int j = tid; j < nq; j += ACA_THREADS) {
-      P3 yj, nyj;
-      Y.get(j, yj, nyj);
-      T v = kcomp(ker, xi, yj, nxi, nyj, comp);
-      for (int l = 0; l < k; l++) v -= V[(size_t)l * nq + j] * s_coef[l];
-      vk[j] = v;
-      ArgMax c{abs2(v), j, 0.0, 0.0};
-      setv(c, v);
-      if (j != jkm1 && better(c, best)) best = c;
+    for (int j0 = tid; j0 < nq; j0 += 2 * NTH) {
+      const int j1 = j0 + NTH;
+      const bool h1 = j1 < nq;
+      const int j1c = h1 ? j1 : j0;
+      P3 y0, n0, y1, n1;
+      Y.get(j0, y0, n0);
+      Y.get(j1c, y1, n1);
+      T v0 = kcomp(ker, xi, y0, nxi, n0, comp);
+      T v1 = kcomp(ker, xi, y1, nxi, n1, comp);
+      for (int l = 0; l < k; l++) {
+        const T c = s_coef[l];
+        const T* Vl = V + (size_t)l * nq;
+        v0 -= Vl[j0] * c;
+        v1 -= Vl[j1c] * c;
+      }
+      vk[j0] = v0;
+      ArgMax c0{abs2(v0), j0, 0.0, 0.0};
+      setv(c0, v0);
+      if (j0 != jkm1 && better(c0, best)) best = c0;
+      if (h1) {
+        vk[j1] = v1;
+        ArgMax c1{abs2(v1), j1, 0.0, 0.0};
+        setv(c1, v1);
+        if (j1 != jkm1 && better(c1, best)) best = c1;
+      }
     }
-    const ArgMax am = block_argmax<T>(best, s_am);
+    const ArgMax am = block_argmax<NW>(best, s_am);
     if (!(am.a2 > 0.0)) {  // zero pivot: rank k (src/ngbem.cpp:977-980)
       rank = k;
       break;
@@ -555,7 +563,7 @@ __global__ __launch_bounds__(ACA_THREADS) void k_aca(const AcaTask* __restrict__
     getv(am, piv);
     const T sc = div_(sqrt(vkj), piv);
     const double rs = 1.0 / sqrt(vkj);
-    for (int l = tid; l < k; l += ACA_THREADS) s_coef[l] = V[(size_t)l * nq + jk];
+    for (int l = tid; l < k; l += NTH) s_coef[l] = V[(size_t)l * nq + jk];
     __syncthreads();
     // column jk: u_i = (K(x_i, y_jk) - sum_l U[i][l] V[l][jk]) / sqrt(|v_jk|)
     P3 yk, nyk;
@@ -563,25 +571,42 @@ __global__ __launch_bounds__(ACA_THREADS) void k_aca(const AcaTask* __restrict__
     ArgMax bu{-1.0, 0x7fffffff, 0.0, 0.0};
     T* uk = U + (size_t)k * np;
     double nu2 = 0.0;
-    for (int i = tid; i < np; i += ACA_THREADS) {
-      P3 x, nx;
-      X.get(i, x, nx);
-      T u = kcomp(ker, x, yk, nx, nyk, comp);
-      for (int l = 0; l < k; l++) u -= U[(size_t)l * np + i] * s_coef[l];
-      u = rmul(u, rs);
-      uk[i] = u;
-      nu2 += abs2(u);
-      ArgMax c{abs2(u), i, 0.0, 0.0};
-      if (i != ikm1 && better(c, bu)) bu = c;
+    for (int i0 = tid; i0 < np; i0 += 2 * NTH) {
+      const int i1 = i0 + NTH;
+      const bool h1 = i1 < np;
+      const int i1c = h1 ? i1 : i0;
+      P3 x0, m0, x1, m1;
+      X.get(i0, x0, m0);
+      X.get(i1c, x1, m1);
+      T u0 = kcomp(ker, x0, yk, m0, nyk, comp);
+      T u1 = kcomp(ker, x1, yk, m1, nyk, comp);
+      for (int l = 0; l < k; l++) {
+        const T c = s_coef[l];
+        const T* Ul = U + (size_t)l * np;
+        u0 -= Ul[i0] * c;
+        u1 -= Ul[i1c] * c;
+      }
+      u0 = rmul(u0, rs);
+      uk[i0] = u0;
+      nu2 += abs2(u0);
+      ArgMax c0{abs2(u0), i0, 0.0, 0.0};
+      if (i0 != ikm1 && better(c0, bu)) bu = c0;
+      if (h1) {
+        u1 = rmul(u1, rs);
+        uk[i1] = u1;
+        nu2 += abs2(u1);
+        ArgMax c1{abs2(u1), i1, 0.0, 0.0};
+        if (i1 != ikm1 && better(c1, bu)) bu = c1;
+      }
     }
     // scale row k by sqrt(|v_jk|)/v_jk and take its norm
     double nv2 = 0.0;
-    for (int j = tid; j < nq; j += ACA_THREADS) {
+    for (int j = tid; j < nq; j += NTH) {
       T v = vk[j] * sc;
       vk[j] = v;
       nv2 += abs2(v);
     }
-    const ArgMax amu = block_argmax<T>(bu, s_am);
+    const ArgMax amu = block_argmax<NW>(bu, s_am);
     nv2 = wave_sum(nv2);
     nu2 = wave_sum(nu2);
     if ((tid & 63) == 0) {
@@ -595,13 +620,13 @@ __global__ __launch_bounds__(ACA_THREADS) void k_aca(const AcaTask* __restrict__
       T iv[IP_CHUNK], iu[IP_CHUNK];
 #pragma unroll
       for (int c = 0; c < IP_CHUNK; c++) { iv[c] = zero_<T>(); iu[c] = zero_<T>(); }
-      for (int j = tid; j < nq; j += ACA_THREADS) {
+      for (int j = tid; j < nq; j += NTH) {
         const T v = vk[j];
 #pragma unroll
         for (int c = 0; c < IP_CHUNK; c++)
           if (c < nl) iv[c] += v * cconj(V[(size_t)(l0 + c) * nq + j]);
       }
-      for (int i = tid; i < np; i += ACA_THREADS) {
+      for (int i = tid; i < np; i += NTH) {
         const T u = uk[i];
 #pragma unroll
         for (int c = 0; c < IP_CHUNK; c++)
@@ -622,7 +647,7 @@ __global__ __launch_bounds__(ACA_THREADS) void k_aca(const AcaTask* __restrict__
       if (tid == 0)
         for (int c = 0; c < nl; c++) {
           T sv = s_red[c], su = s_red[IP_CHUNK + c];
-          for (int w = 1; w < ACA_WAVES; w++) {
+          for (int w = 1; w < NW; w++) {
             sv += s_red[w * 2 * IP_CHUNK + c];
             su += s_red[w * 2 * IP_CHUNK + IP_CHUNK + c];
           }
@@ -633,7 +658,7 @@ __global__ __launch_bounds__(ACA_THREADS) void k_aca(const AcaTask* __restrict__
     __syncthreads();
     if (tid == 0) {
       double v2 = 0.0, u2 = 0.0;
-      for (int w = 0; w < ACA_WAVES; w++) {
+      for (int w = 0; w < NW; w++) {
         v2 += s_dred[2 * w];
         u2 += s_dred[2 * w + 1];
       }
@@ -656,9 +681,21 @@ __global__ __launch_bounds__(ACA_THREADS) void k_aca(const AcaTask* __restrict__
     int status = 0;
     if (rank < 0) {
       if (kmax == pmax) rank = pmax;
-      else { rank = kmax; status = 1; }  // capacity reached: rerun with a larger cap
+      else {
+        rank = kmax;
+        status = 1;  // capacity reached: resume with a larger workspace
+      }
     }
-    result[blockIdx.x] = make_int2(rank, status);
+    if (tid == 0) {
+      tk.rank = rank;
+      tk.status = status;
+      tk.k0 = (status == 1) ? kmax : rank;
+      tk.ik = ik;
+      tk.ikm1 = ikm1;
+      tk.jkm1 = jkm1;
+      tk.norm2 = norm2;
+      tasks[blockIdx.x] = tk;
+    }
   }
 }
 
@@ -946,6 +983,14 @@ static double now() {
   return std::chrono::duration<double>(std::chrono::steady_clock::now().time_since_epoch()).count();
 }
 
+PhaseTimer::PhaseTimer(const char* sc) : on(std::getenv("NGBEM_TIMING") != nullptr), t(now()), scope(sc) {}
+void PhaseTimer::mark(const char* what) {
+  if (!on) return;
+  const double t1 = now();
+  std::fprintf(stderr, "[ngbem %s] %-28s %9.3f ms\n", scope, what, (t1 - t) * 1e3);
+  t = t1;
+}
+
 template <class F>
 static void dispatch_kernel(int id, double kappa, F&& f) {
   switch (id) {
@@ -1048,32 +1093,29 @@ void assemble_near(Operator& op, hipStream_t s) {
   const Tree& rt = op.trial_tree;
   const bool p1i = te.kind == SP_P1, p1j = tr.kind == SP_P1;
   const bool direct = !p1i && !p1j;
-  // near tables + patches
-  std::vector<int> patch_all;  // P1 patches; P0 sides index the permutation
+  PhaseTimer pt_("near");
+  // near tables; P1 sides use the per-cluster patches, P0 sides the permutation
   std::vector<int2> chunks;
   std::vector<int4> pairs[3];
   int64_t off = 0;
-  std::vector<int> pI, pJ, neigh;
-  std::vector<char> mark(m.nt, 0);
+  const ClusterPatches& cpi = *op.cp_test;
+  const ClusterPatches& cpj = op.cp_trial;
+  const int ncl_test = (int)tt.cl.size();
   for (size_t k = 0; k < op.near.size(); k++) {
     NearBlk& nb = op.near[k];
+    const Block& bl = op.blocks[op.near_block[k]];
     nb.off = off;
     off += (int64_t)nb.m * nb.n;
     if (p1i) {
-      patch(te, &tt.perm[nb.r0], nb.m, pI);
-      nb.pI = (int)patch_all.size();
-      nb.pm = (int)pI.size();
-      patch_all.insert(patch_all.end(), pI.begin(), pI.end());
+      nb.pI = cpi.off[bl.row];
+      nb.pm = cpi.size(bl.row);
     } else {
-      pI.assign(tt.perm.begin() + nb.r0, tt.perm.begin() + nb.r0 + nb.m);
       nb.pI = nb.r0;
       nb.pm = nb.m;
     }
-    if (p1j) {
-      patch(tr, &rt.perm[nb.c0], nb.n, pJ);
-      nb.pJ = (int)patch_all.size();
-      nb.pn = (int)pJ.size();
-      patch_all.insert(patch_all.end(), pJ.begin(), pJ.end());
+    if (p1j) {  // trial patches follow the test patches in the device array
+      nb.pJ = (op.same_space ? 0 : cpi.off[ncl_test]) + cpj.off[bl.col];
+      nb.pn = cpj.size(bl.col);
     } else {
       nb.pJ = nb.c0;
       nb.pn = nb.n;
@@ -1081,19 +1123,14 @@ void assemble_near(Operator& op, hipStream_t s) {
     const int64_t np = (int64_t)nb.pm * nb.pn;
     if (np >= (int64_t)1 << 31) throw std::runtime_error("near block too large for one launch grid");
     for (int64_t st = 0; st < np; st += 256) chunks.push_back(make_int2((int)k, (int)st));
-    // touching pairs among the patches
-    auto in_trial = [&](int f) {
-      if (!p1j) {
-        const int pos = rt.iperm[f];
-        return pos >= nb.c0 && pos < nb.c0 + nb.n;
-      }
-      for (int c = 0; c < 3; c++) {
-        const int pos = rt.iperm[m.tri[3 * f + c]];
-        if (pos >= nb.c0 && pos < nb.c0 + nb.n) return true;
-      }
-      return false;
-    };
-    for (int e : pI) {
+  }
+  // touching element pairs: for each test element, its vertex neighbours; a
+  // pair belongs to every near block holding one of its dof combinations
+  // (the reference recomputes it in each such block, src/ngbem.cpp:501-737)
+  {
+    std::vector<char> mark(m.nt, 0);
+    std::vector<int> neigh, got;
+    for (int e = 0; e < m.nt; e++) {
       neigh.clear();
       for (int c = 0; c < 3; c++) {
         const int v = m.tri[3 * e + c];
@@ -1104,7 +1141,6 @@ void assemble_near(Operator& op, hipStream_t s) {
       }
       for (int f : neigh) {
         mark[f] = 0;
-        if (!in_trial(f)) continue;
         int px[3], py[3];
         const int nc = classify_pair(m, e, f, px, py);
         int cls;
@@ -1115,10 +1151,20 @@ void assemble_near(Operator& op, hipStream_t s) {
         } else if (nc == 2) cls = 1;
         else if (nc == 1) cls = 2;
         else continue;
-        pairs[cls].push_back(make_int4((int)k, e, f, perm_code(px, py)));
+        got.clear();
+        for (int i = 0; i < te.nloc; i++)
+          for (int j = 0; j < tr.nloc; j++) {
+            const int rp = tt.iperm[te.dofs[e * te.nloc + i]], cp = rt.iperm[tr.dofs[f * tr.nloc + j]];
+            const int b = op.lookup.find(rp, cp);
+            if (b < 0 || !(op.blocks[b].near || op.blocks[b].fallback)) continue;
+            const int slot = op.blk_slot[b];
+            if (std::find(got.begin(), got.end(), slot) == got.end()) got.push_back(slot);
+          }
+        for (int slot : got) pairs[cls].push_back(make_int4(slot, e, f, perm_code(px, py)));
       }
     }
   }
+  pt_.mark("host work lists");
   op.near_entries = off;
   for (int c = 0; c < 3; c++) op.n_sing[c] = (int)pairs[c].size();
   const int ntq = op.trig.size();
@@ -1127,8 +1173,13 @@ void assemble_near(Operator& op, hipStream_t s) {
   for (int c = 0; c < 3; c++) op.near_evals += (double)pairs[c].size() * (op.ss[c].size() - (double)ntq * ntq);
 
   op.d_near_blk.upload(op.near, s);
+  pt_.mark("stats");
   op.d_near_chunks.upload(chunks, s);
-  op.d_patch.upload(patch_all, s);
+  {
+    std::vector<int> patch_all(cpi.list);  // test patches, then trial patches when the spaces differ
+    if (!op.same_space) patch_all.insert(patch_all.end(), cpj.list.begin(), cpj.list.end());
+    op.d_patch.upload(patch_all, s);
+  }
   for (int c = 0; c < 3; c++) op.d_spairs[c].upload(pairs[c], s);
   const size_t esz = op.complex_valued ? 16 : 8;
   op.d_near_val.alloc((size_t)off * esz);
@@ -1158,11 +1209,13 @@ void assemble_near(Operator& op, hipStream_t s) {
                            op.d_iperm_test.as<int>(), op.d_iperm_trial.as<int>(), ker, val);
       }
     };
+    pt_.mark("uploads + alloc");
     if constexpr (!EI::P1 && !EJ::P1) launch(std::true_type{});
     else launch(std::false_type{});
   });
   BEM_HIP(hipGetLastError());
   BEM_HIP(hipStreamSynchronize(s));
+  pt_.mark("kernels");
   op.t_near = now() - t0;
 }
 
@@ -1183,56 +1236,47 @@ void assemble_far(Operator& op, hipStream_t s) {
   const Tree& rt = op.trial_tree;
   const int ntq = op.trig.size();
   const int ncomp = op.kinfo.ncomp;
-  // sorted patches (ACA point order) and projection maps
-  std::vector<int> fpatch, maps, pI, pJ, dofs;
-  auto build_map = [&](const Space& sp, const Tree& tree, int p0, int len, const std::vector<int>& P) {
-    const int base = (int)maps.size();
-    maps.resize(base + len + 1);
-    std::vector<int> ents;
-    for (int ii = 0; ii < len; ii++) {
-      maps[base + ii] = (int)ents.size();
-      const int d = tree.perm[p0 + ii];
-      for (int q = sp.ed_off[d]; q < sp.ed_off[d + 1]; q++) {
-        const int e = sp.ed_list[q];
-        const int a = (int)(std::lower_bound(P.begin(), P.end(), e) - P.begin());
-        int kl = 0;
-        for (int l = 0; l < sp.nloc; l++)
-          if (sp.dofs[e * sp.nloc + l] == d) kl = l;
-        ents.push_back(a * 4 + kl);
-      }
-    }
-    maps[base + len] = (int)ents.size();
-    maps.insert(maps.end(), ents.begin(), ents.end());
-    return base;
-  };
+  ClusterPatches& ci = *op.cp_test;
+  ClusterPatches& cj = op.cp_trial;
+  PhaseTimer pt_("far");
+  // patches live in d_patch (test list, then the trial list when the spaces differ)
+  const int joff = op.same_space ? 0 : ci.off[tt.cl.size()];
   for (int k = 0; k < nfar; k++) {
     FarBlk& fb = op.far[k];
-    patch(te, &tt.perm[fb.r0], fb.m, pI);
-    patch(tr, &rt.perm[fb.c0], fb.n, pJ);
-    fb.pI = (int)fpatch.size();
-    fb.pm = (int)pI.size();
-    fpatch.insert(fpatch.end(), pI.begin(), pI.end());
-    fb.pJ = (int)fpatch.size();
-    fb.pn = (int)pJ.size();
-    fpatch.insert(fpatch.end(), pJ.begin(), pJ.end());
-    fb.rmap = build_map(te, tt, fb.r0, fb.m, pI);
-    fb.cmap = build_map(tr, rt, fb.c0, fb.n, pJ);
+    const Block& bl = op.blocks[op.far_block[k]];
+    fb.pI = ci.off[bl.row];
+    fb.pm = ci.size(bl.row);
+    fb.pJ = joff + cj.off[bl.col];
+    fb.pn = cj.size(bl.col);
+    fb.rmap = ci.map_of(te, tt, bl.row);
+    fb.cmap = cj.map_of(tr, rt, bl.col);
   }
-  op.d_far_patch.upload(fpatch, s);
-  op.d_far_maps.upload(maps, s);
+  {
+    std::vector<int> maps(ci.maps);
+    if (!op.same_space) {
+      const int shift = (int)ci.maps.size();
+      for (auto& fb : op.far) fb.cmap += shift;
+      maps.insert(maps.end(), cj.maps.begin(), cj.maps.end());
+    }
+    op.d_far_maps.upload(maps, s);
+  }
   op.d_far_blk.upload(op.far, s);
 
   size_t free_b = 0, total_b = 0;
   BEM_HIP(hipMemGetInfo(&free_b, &total_b));
   const size_t esz = op.complex_valued ? 16 : 8;
-  const size_t budget = std::max<size_t>(std::min<size_t>(free_b / 4, (size_t)24 << 30), (size_t)256 << 20);
+  const size_t budget = std::max<size_t>(free_b / 3, (size_t)256 << 20);
+  // initial rank capacity (NGBEM_ACA_CAP0 overrides); points above which a block gets 512 threads
+  const char* cap_env = std::getenv("NGBEM_ACA_CAP0");
+  const int CAP0 = cap_env ? std::max(1, std::atoi(cap_env)) : 48, BIG = 6144;
+  pt_.mark("host tables + uploads");
 
-  struct Pending {
-    int blk, cap;
-  };
-  std::vector<Pending> todo;
-  const int cap0 = 32;
-  for (int k = 0; k < nfar; k++) todo.push_back({k, cap0});
+  // blocks by ACA size, largest first (longest-processing-time order)
+  std::vector<int> order(nfar);
+  std::iota(order.begin(), order.end(), 0);
+  auto pts = [&](int k) { return (op.far[k].pm + op.far[k].pn) * ntq; };
+  std::stable_sort(order.begin(), order.end(), [&](int a, int b) { return pts(a) > pts(b); });
+
   struct Piece {
     DBuf U, V;
     size_t nu = 0, nv = 0;
@@ -1241,112 +1285,207 @@ void assemble_far(Operator& op, hipStream_t s) {
   std::vector<int> piece_of(nfar, -1);
   std::vector<int64_t> loc_u(nfar), loc_v(nfar);
   op.far_rank.assign(nfar, 0);
-  DBuf wsU, wsV, d_tasks, d_res, d_ptasks;
-  while (!todo.empty()) {
-    // take a batch within the workspace budget
+  hipStream_t s2;
+  BEM_HIP(hipStreamCreateWithFlags(&s2, hipStreamNonBlocking));
+  hipEvent_t ev_s, ev_s2;
+  BEM_HIP(hipEventCreateWithFlags(&ev_s, hipEventDisableTiming));
+  BEM_HIP(hipEventCreateWithFlags(&ev_s2, hipEventDisableTiming));
+  DBuf wsU, wsV, d_tasks, d_ptasks;
+
+  // launch tasks[begin, end) (all small or all large) on one stream
+  auto launch = [&](AcaTask* dtasks, int count, bool big, int maxcap, hipStream_t st, DBuf& U, DBuf& V) {
+    if (count == 0) return;
+    dispatch_kernel(op.kernel, op.kappa, [&](auto ker) {
+      using KER = decltype(ker);
+      using T = typename KER::T;
+      if (big)
+        hipLaunchKernelGGL((k_aca<KER, 512>), dim3(count), dim3(512), maxcap * sizeof(T), st, dtasks,
+                           op.d_far_blk.as<FarBlk>(), op.d_patch.as<int>(), op.d_quad.as<double>(),
+                           op.d_geo.as<double>(), ntq, ker, op.eps, U.as<T>(), V.as<T>());
+      else
+        hipLaunchKernelGGL((k_aca<KER, 256>), dim3(count), dim3(256), maxcap * sizeof(T), st, dtasks,
+                           op.d_far_blk.as<FarBlk>(), op.d_patch.as<int>(), op.d_quad.as<double>(),
+                           op.d_geo.as<double>(), ntq, ker, op.eps, U.as<T>(), V.as<T>());
+    });
+    BEM_HIP(hipGetLastError());
+  };
+  // run a task list to completion; tasks are ordered big (first nbig) then small
+  auto run = [&](std::vector<AcaTask>& tasks, int nbig, DBuf& U, DBuf& V) {
+    int maxcap = 0;
+    for (auto& t : tasks) maxcap = std::max(maxcap, t.cap);
+    d_tasks.upload(tasks, s);
+    BEM_HIP(hipEventRecord(ev_s, s));
+    BEM_HIP(hipStreamWaitEvent(s2, ev_s, 0));
+    launch(d_tasks.as<AcaTask>(), nbig, true, maxcap, s2, U, V);
+    launch(d_tasks.as<AcaTask>() + nbig, (int)tasks.size() - nbig, false, maxcap, s, U, V);
+    BEM_HIP(hipEventRecord(ev_s2, s2));
+    BEM_HIP(hipStreamWaitEvent(s, ev_s2, 0));
+    BEM_HIP(hipMemcpyAsync(tasks.data(), d_tasks.p, tasks.size() * sizeof(AcaTask), hipMemcpyDeviceToHost, s));
+    BEM_HIP(hipStreamSynchronize(s));
+  };
+
+  size_t next = 0;
+  while (next < order.size()) {
+    // batch of whole blocks (all components) within the workspace budget
     std::vector<AcaTask> tasks;
-    std::vector<Pending> batch;
+    std::vector<int> bblocks;
     size_t usedU = 0, usedV = 0;
-    size_t i = 0;
-    for (; i < todo.size(); i++) {
-      const FarBlk& fb = op.far[todo[i].blk];
+    while (next < order.size()) {
+      const int k = order[next];
+      const FarBlk& fb = op.far[k];
       const int np = fb.pm * ntq, nq = fb.pn * ntq;
-      const int cap = std::min(todo[i].cap, std::min(np, nq));
-      if (cap > 4096) throw std::runtime_error("ACA rank above 4096 requested; raise eps or lower leafsize");
-      const size_t need = (size_t)cap * (np + nq) * esz * ncomp;
-      if (!batch.empty() && (usedU + usedV) * esz + need > budget) break;
-      batch.push_back({todo[i].blk, cap});
+      const int cap = std::min(CAP0, std::min(np, nq));
+      const size_t need = (size_t)cap * (np + nq) * ncomp;
+      if (!bblocks.empty() && (usedU + usedV + need) * esz > budget) break;
+      bblocks.push_back(k);
       for (int c = 0; c < ncomp; c++) {
         AcaTask t{};
         t.uoff = (int64_t)usedU;
         t.voff = (int64_t)usedV;
-        t.blk = todo[i].blk;
+        t.norm2 = 0.0;
+        t.blk = k;
         t.comp = c;
         t.np = np;
         t.nq = nq;
         t.cap = cap;
+        t.k0 = 0;
+        t.ik = 0;
+        t.ikm1 = 0;
+        t.jkm1 = nq + 1;
         tasks.push_back(t);
         usedU += (size_t)cap * np;
         usedV += (size_t)cap * nq;
       }
+      next++;
     }
-    todo.erase(todo.begin(), todo.begin() + i);
     wsU.alloc(std::max(usedU * esz, wsU.bytes));
     wsV.alloc(std::max(usedV * esz, wsV.bytes));
-    d_tasks.upload(tasks, s);
-    d_res.alloc(tasks.size() * sizeof(int2));
-    int maxcap = 0;
-    for (auto& t : tasks) maxcap = std::max(maxcap, t.cap);
-    dispatch_kernel(op.kernel, op.kappa, [&](auto ker) {
-      using KER = decltype(ker);
-      using T = typename KER::T;
-      hipLaunchKernelGGL((k_aca<KER>), dim3((unsigned)tasks.size()), dim3(ACA_THREADS), maxcap * sizeof(T), s,
-                         d_tasks.as<AcaTask>(), op.d_far_blk.as<FarBlk>(), op.d_far_patch.as<int>(),
-                         op.d_quad.as<double>(), op.d_geo.as<double>(), ntq, ker, op.eps, wsU.as<T>(), wsV.as<T>(),
-                         d_res.as<int2>());
-    });
-    BEM_HIP(hipGetLastError());
-    std::vector<int2> res(tasks.size());
-    BEM_HIP(hipMemcpyAsync(res.data(), d_res.p, res.size() * sizeof(int2), hipMemcpyDeviceToHost, s));
-    BEM_HIP(hipStreamSynchronize(s));
-    // finished blocks -> projection into a batch piece; capped blocks -> retry
-    std::vector<ProjTask> ptasks;
-    auto piece = std::make_unique<Piece>();
-    for (size_t bi = 0; bi < batch.size(); bi++) {
-      const int blk = batch[bi].blk;
-      bool capped = false;
-      int rk[2] = {0, 0};
-      for (int c = 0; c < ncomp; c++) {
-        const int2 r = res[bi * ncomp + c];
-        rk[c] = r.x;
-        capped |= r.y != 0;
+    int nbig = 0;
+    while (nbig < (int)tasks.size() && tasks[nbig].np + tasks[nbig].nq >= BIG) nbig++;
+    pt_.mark("batch setup");
+    run(tasks, nbig, wsU, wsV);
+    pt_.mark("aca kernels");
+    // workspace that holds each task: 0 = wsU/wsV, r > 0 = resume buffer r-1
+    std::vector<int> where(tasks.size(), 0);
+    std::vector<std::unique_ptr<std::pair<DBuf, DBuf>>> rbuf;
+    auto bufU = [&](int w) -> DBuf& { return w == 0 ? wsU : rbuf[w - 1]->first; };
+    auto bufV = [&](int w) -> DBuf& { return w == 0 ? wsV : rbuf[w - 1]->second; };
+    // resume blocks whose ACA ran out of rank capacity: all their components
+    // move to a larger buffer (finished columns copied), capped ones continue
+    for (;;) {
+      std::vector<int> blks;
+      for (size_t bi = 0; bi < bblocks.size(); bi++)
+        for (int c = 0; c < ncomp; c++)
+          if (tasks[bi * ncomp + c].status == 1) {
+            blks.push_back((int)bi);
+            break;
+          }
+      if (blks.empty()) break;
+      std::vector<AcaTask> rt2;
+      std::vector<int> src;
+      size_t u2 = 0, v2 = 0;
+      for (int bi : blks)
+        for (int c = 0; c < ncomp; c++) {
+          const int t = bi * ncomp + c;
+          AcaTask r = tasks[t];
+          const int pmax = std::min(r.np, r.nq);
+          if (r.status == 1) r.cap = std::min(pmax, std::max(2 * r.cap, r.cap + 64));
+          else r.cap = r.k0;
+          if (r.cap > 4096) throw std::runtime_error("ACA rank above 4096 requested; raise eps or lower leafsize");
+          r.uoff = (int64_t)u2;
+          r.voff = (int64_t)v2;
+          u2 += (size_t)r.cap * r.np;
+          v2 += (size_t)r.cap * r.nq;
+          rt2.push_back(r);
+          src.push_back(t);
+        }
+      rbuf.push_back(std::make_unique<std::pair<DBuf, DBuf>>());
+      DBuf& U2 = rbuf.back()->first;
+      DBuf& V2 = rbuf.back()->second;
+      U2.alloc(std::max<size_t>(u2, 1) * esz);
+      V2.alloc(std::max<size_t>(v2, 1) * esz);
+      std::vector<AcaTask> launchq;
+      std::vector<int> lpos;
+      for (size_t q = 0; q < rt2.size(); q++) {
+        const AcaTask& o = tasks[src[q]];
+        DBuf& SU = bufU(where[src[q]]);
+        DBuf& SV = bufV(where[src[q]]);
+        if (o.k0)
+          BEM_HIP(hipMemcpyAsync((char*)U2.p + rt2[q].uoff * esz, (char*)SU.p + o.uoff * esz,
+                                 (size_t)o.k0 * o.np * esz, hipMemcpyDeviceToDevice, s));
+        if (o.k0)
+          BEM_HIP(hipMemcpyAsync((char*)V2.p + rt2[q].voff * esz, (char*)SV.p + o.voff * esz,
+                                 (size_t)o.k0 * o.nq * esz, hipMemcpyDeviceToDevice, s));
+        if (o.status == 1) {
+          launchq.push_back(rt2[q]);
+          lpos.push_back((int)q);
+        }
       }
-      const FarBlk& fb = op.far[blk];
-      const int np = fb.pm * ntq, nq = fb.pn * ntq;
-      for (int c = 0; c < ncomp; c++) op.far_evals += (double)rk[c] * (np + nq);
-      if (capped) {
-        todo.push_back({blk, batch[bi].cap * 4});
-        continue;
+      run(launchq, 0, U2, V2);
+      for (size_t q = 0; q < lpos.size(); q++) rt2[lpos[q]] = launchq[q];
+      for (size_t q = 0; q < rt2.size(); q++) {
+        tasks[src[q]] = rt2[q];
+        where[src[q]] = (int)rbuf.size();
       }
-      int R = 0;
-      for (auto& t : op.kinfo.terms) R += rk[t.kc];
-      ProjTask pt{};
-      pt.blk = blk;
-      pt.ncomp = ncomp;
-      for (int c = 0; c < ncomp; c++) {
-        pt.uoff[c] = tasks[bi * ncomp + c].uoff;
-        pt.voff[c] = tasks[bi * ncomp + c].voff;
-        pt.rank[c] = rk[c];
-      }
-      pt.np = np;
-      pt.nq = nq;
-      pt.ou = (int64_t)piece->nu;
-      pt.ov = (int64_t)piece->nv;
-      piece->nu += (size_t)fb.m * R;
-      piece->nv += (size_t)R * fb.n;
-      op.far_rank[blk] = R;
-      piece_of[blk] = (int)pieces.size();
-      loc_u[blk] = pt.ou;
-      loc_v[blk] = pt.ov;
-      ptasks.push_back(pt);
     }
-    if (!ptasks.empty()) {
+    pt_.mark("aca resume");
+    // projection, one launch per workspace buffer
+    for (int w = 0; w <= (int)rbuf.size(); w++) {
+      std::vector<ProjTask> ptasks;
+      auto piece = std::make_unique<Piece>();
+      for (size_t bi = 0; bi < bblocks.size(); bi++) {
+        if (where[bi * ncomp] != w) continue;
+        const int blk = bblocks[bi];
+        int rk[2] = {0, 0};
+        for (int c = 0; c < ncomp; c++) rk[c] = tasks[bi * ncomp + c].rank;
+        const FarBlk& fb = op.far[blk];
+        const int np = fb.pm * ntq, nq = fb.pn * ntq;
+        for (int c = 0; c < ncomp; c++) op.far_evals += (double)rk[c] * (np + nq);
+        int R = 0;
+        for (auto& t : op.kinfo.terms) R += rk[t.kc];
+        ProjTask pt{};
+        pt.blk = blk;
+        pt.ncomp = ncomp;
+        for (int c = 0; c < ncomp; c++) {
+          pt.uoff[c] = tasks[bi * ncomp + c].uoff;
+          pt.voff[c] = tasks[bi * ncomp + c].voff;
+          pt.rank[c] = rk[c];
+        }
+        pt.np = np;
+        pt.nq = nq;
+        pt.ou = (int64_t)piece->nu;
+        pt.ov = (int64_t)piece->nv;
+        piece->nu += (size_t)fb.m * R;
+        piece->nv += (size_t)R * fb.n;
+        op.far_rank[blk] = R;
+        piece_of[blk] = (int)pieces.size();
+        loc_u[blk] = pt.ou;
+        loc_v[blk] = pt.ov;
+        ptasks.push_back(pt);
+      }
+      if (ptasks.empty()) continue;
       piece->U.alloc(std::max<size_t>(piece->nu, 1) * esz);
       piece->V.alloc(std::max<size_t>(piece->nv, 1) * esz);
       d_ptasks.upload(ptasks, s);
+      DBuf& SU = bufU(w);
+      DBuf& SV = bufV(w);
       dispatch_full(op, [&](auto ker, auto ei, auto ej) {
         using KER = decltype(ker);
         using T = typename KER::T;
         hipLaunchKernelGGL((k_project<KER, decltype(ei), decltype(ej)>), dim3((unsigned)ptasks.size()), dim3(256), 0,
-                           s, d_ptasks.as<ProjTask>(), op.d_far_blk.as<FarBlk>(), op.d_far_patch.as<int>(),
+                           s, d_ptasks.as<ProjTask>(), op.d_far_blk.as<FarBlk>(), op.d_patch.as<int>(),
                            op.d_far_maps.as<int>(), op.d_quad.as<double>(), op.d_curl.as<double>(),
-                           op.d_trig.as<double>(), ntq, wsU.as<T>(), wsV.as<T>(), piece->U.as<T>(), piece->V.as<T>());
+                           op.d_trig.as<double>(), ntq, SU.as<T>(), SV.as<T>(), piece->U.as<T>(), piece->V.as<T>());
       });
       BEM_HIP(hipGetLastError());
       BEM_HIP(hipStreamSynchronize(s));
       pieces.push_back(std::move(piece));
     }
   }
+  pt_.mark("projection");
+  (void)hipEventDestroy(ev_s);
+  (void)hipEventDestroy(ev_s2);
+  (void)hipStreamDestroy(s2);
   wsU.release();
   wsV.release();
   // concatenate pieces into the final factor arrays
@@ -1383,6 +1522,7 @@ void assemble_far(Operator& op, hipStream_t s) {
   op.t_total = toff;
   op.d_far_blk.upload(op.far, s);
   BEM_HIP(hipStreamSynchronize(s));
+  pt_.mark("concatenate factors");
   op.t_far = now() - t0;
 }
 

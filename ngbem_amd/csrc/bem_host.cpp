@@ -476,6 +476,92 @@ void patch(const Space& s, const int* dofs, int nd, std::vector<int>& out) {
   out.erase(std::unique(out.begin(), out.end()), out.end());
 }
 
+void ClusterPatches::build(const Space& s, const Tree& t) {
+  const int ncl = (int)t.cl.size();
+  std::vector<std::vector<int>> tmp(ncl);
+  std::vector<int> a;
+  for (int c = ncl - 1; c >= 0; c--) {  // children have larger indices (breadth-first)
+    const Cluster& cl = t.cl[c];
+    if (cl.child1 < 0) {
+      patch(s, &t.perm[cl.permupos], (int)cl.number, tmp[c]);
+    } else {
+      const std::vector<int>& p = tmp[cl.child1];
+      const std::vector<int>& q = tmp[cl.child2];
+      a.resize(p.size() + q.size());
+      std::merge(p.begin(), p.end(), q.begin(), q.end(), a.begin());
+      a.erase(std::unique(a.begin(), a.end()), a.end());
+      tmp[c] = a;
+    }
+  }
+  off.assign(ncl + 1, 0);
+  for (int c = 0; c < ncl; c++) off[c + 1] = off[c] + (int)tmp[c].size();
+  list.resize(off[ncl]);
+  for (int c = 0; c < ncl; c++) std::copy(tmp[c].begin(), tmp[c].end(), list.begin() + off[c]);
+  moff.assign(ncl, -1);
+  maps.clear();
+}
+
+int ClusterPatches::map_of(const Space& s, const Tree& t, int c) {
+  if (moff[c] >= 0) return moff[c];
+  const Cluster& cl = t.cl[c];
+  const int len = (int)cl.number, base = (int)maps.size();
+  const int* P = &list[off[c]];
+  const int np = size(c);
+  maps.resize(base + len + 1);
+  std::vector<int> ents;
+  for (int ii = 0; ii < len; ii++) {
+    maps[base + ii] = (int)ents.size();
+    const int d = t.perm[cl.permupos + ii];
+    for (int q = s.ed_off[d]; q < s.ed_off[d + 1]; q++) {
+      const int e = s.ed_list[q];
+      const int a = (int)(std::lower_bound(P, P + np, e) - P);
+      int kl = 0;
+      for (int l = 0; l < s.nloc; l++)
+        if (s.dofs[e * s.nloc + l] == d) kl = l;
+      ents.push_back(a * 4 + kl);
+    }
+  }
+  maps[base + len] = (int)ents.size();
+  maps.insert(maps.end(), ents.begin(), ents.end());
+  moff[c] = base;
+  return base;
+}
+
+void BlockLookup::build(const Tree& rows, const std::vector<Block>& blocks) {
+  const int nleaf = (int)rows.leaves.size();
+  int n = 0;
+  for (int l : rows.leaves) n += (int)rows.cl[l].number;
+  pos2leaf.assign(n, 0);
+  std::vector<int> lp0(nleaf);
+  for (int i = 0; i < nleaf; i++) {
+    const Cluster& c = rows.cl[rows.leaves[i]];
+    lp0[i] = (int)c.permupos;
+    for (long p = c.permupos; p < c.permupos + c.number; p++) pos2leaf[p] = i;
+  }
+  std::vector<std::vector<std::pair<int, int>>> per(nleaf);
+  for (size_t b = 0; b < blocks.size(); b++) {
+    const Block& bl = blocks[b];
+    int li = pos2leaf[bl.r0];
+    while (li < nleaf && lp0[li] < bl.r0 + bl.m) per[li++].push_back({bl.c0, (int)b});
+  }
+  starts.assign(nleaf, {});
+  ids.assign(nleaf, {});
+  for (int i = 0; i < nleaf; i++) {
+    std::sort(per[i].begin(), per[i].end());
+    for (auto& pr : per[i]) {
+      starts[i].push_back(pr.first);
+      ids[i].push_back(pr.second);
+    }
+  }
+}
+
+int BlockLookup::find(int rowpos, int colpos) const {
+  const int li = pos2leaf[rowpos];
+  const std::vector<int>& st = starts[li];
+  const int k = (int)(std::upper_bound(st.begin(), st.end(), colpos) - st.begin()) - 1;
+  return k >= 0 ? ids[li][k] : -1;
+}
+
 int classify_pair(const Mesh& m, int ei, int ej, int px[3], int py[3]) {
   const int* vi = &m.tri[3 * ei];
   const int* vj = &m.tri[3 * ej];

@@ -78,6 +78,27 @@ int eval_dim(int ev);
 
 // sorted unique elements touching the given dofs
 void patch(const Space& s, const int* dofs, int ndofs, std::vector<int>& out);
+
+// Patches of every cluster of a tree (sorted unique elements touching the
+// cluster's dofs, src/ngbem.cpp:415-443), built bottom-up by merging the
+// children's lists, and on request the dof -> (patch index, local dof) maps the
+// far-field projection needs (layout: number+1 offsets, then entries a*4+k).
+struct ClusterPatches {
+  std::vector<int> off, list;
+  std::vector<int> moff;  // per cluster offset into maps, -1 if not built
+  std::vector<int> maps;
+  void build(const Space& s, const Tree& t);
+  int map_of(const Space& s, const Tree& t, int c);
+  int size(int c) const { return off[c + 1] - off[c]; }
+};
+
+// near/far lookup of the block holding (row position, col position)
+struct BlockLookup {
+  std::vector<int> pos2leaf;                 // test positions -> leaf index
+  std::vector<std::vector<int>> starts, ids; // per leaf: column starts (sorted), block index
+  void build(const Tree& rows, const std::vector<Block>& blocks);
+  int find(int rowpos, int colpos) const;
+};
 // number of common vertices and the Sauter-Schwab vertex permutations
 // (src/ngbem.cpp:527-531, 569-586, 627-642); returns the count
 int classify_pair(const Mesh& m, int ei, int ej, int px[3], int py[3]);

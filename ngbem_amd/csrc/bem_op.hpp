@@ -74,12 +74,16 @@ struct FarBlk {
   int rmap, cmap;      // offsets of the row/col projection maps
 };
 
-// one ACA task = (far block, kernel component)
+// one ACA task = (far block, kernel component); the resume state is written
+// back by the kernel
 struct AcaTask {
   int64_t uoff, voff;  // workspace offsets (elements)
+  double norm2;        // resume state: squared Frobenius norm estimate
   int blk, comp;
   int np, nq;          // points (patch size * trig points)
   int cap;             // rank capacity of this pass
+  int k0, ik, ikm1, jkm1;  // resume state
+  int rank, status;    // outputs: rank, 1 = capacity reached
   int pad;
 };
 
@@ -108,6 +112,10 @@ struct Operator {
 
   std::vector<Block> blocks;
   std::vector<int> blk_slot;  // block -> index in near or far table
+  std::vector<int> near_block, far_block;  // table slot -> block
+  ClusterPatches cp_trial, cp_test_own;
+  ClusterPatches* cp_test = nullptr;
+  BlockLookup lookup;
   std::vector<NearBlk> near;
   std::vector<FarBlk> far;
   std::vector<int> far_rank;  // per far block, total rank
@@ -137,6 +145,15 @@ struct Operator {
 
   int ndof_trial() const { return trial->ndof; }
   int ndof_test() const { return test->ndof; }
+};
+
+// phase timing on stderr when NGBEM_TIMING is set
+struct PhaseTimer {
+  bool on;
+  double t;
+  const char* scope;
+  explicit PhaseTimer(const char* sc);
+  void mark(const char* what);
 };
 
 // device entry points (bem_device.hip)

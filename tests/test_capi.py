@@ -1,0 +1,106 @@
+"""C-ABI checks that need no GPU: the library loads, exports every function
+include/ngbem_amd.h declares, rejects bad arguments with a message, and its
+host-side cluster trees and block partitions are identical to the oracle's."""
+import ctypes
+import os
+import re
+
+import numpy as np
+import pytest
+
+import ngbem_amd as nb
+from ngbem_amd import _capi
+from oracle import oracle as O
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+
+
+def header_functions():
+    txt = open(os.path.join(ROOT, "include", "ngbem_amd.h")).read()
+    txt = re.sub(r"/\*.*?\*/", "", txt, flags=re.S)
+    return sorted(set(re.findall(r"\b(bem_[a-z_]+)\s*\(", txt)))
+
+
+def test_library_exports_every_declared_symbol():
+    L = _capi.lib()
+    names = header_functions()
+    assert len(names) >= 18
+    for n in names:
+        assert hasattr(L, n), n
+    assert sorted(names) == sorted(_capi.EXPORTS)
+
+
+def test_bad_arguments_are_reported():
+    L = _capi.lib()
+    out = ctypes.c_void_p()
+    xyz = np.zeros((3, 3))
+    tri = np.array([[0, 1, 5]], dtype=np.int32)
+    rc = L.bem_mesh_create(_capi.dptr(xyz), 3, _capi.iptr(tri), 1, ctypes.byref(out))
+    assert rc == -1 and b"out of range" in L.bem_last_error()
+    tri = np.array([[0, 1, 2]], dtype=np.int32)
+    rc = L.bem_mesh_create(_capi.dptr(xyz), 3, _capi.iptr(tri), 1, ctypes.byref(out))
+    assert rc == -1 and b"degenerate" in L.bem_last_error()
+    m = nb.icosphere(1)
+    with pytest.raises(RuntimeError, match="hypersingular operators need H1"):
+        p0 = nb.SurfaceL2(m)
+        params = _capi.Params(3, 40, 2.0, 1e-6, 0, 0)
+        _capi.check(L.bem_op_create(_capi.LAPLACE_HS, 0.0, p0.handle, p0.handle, ctypes.byref(params),
+                                    ctypes.byref(out)))
+    with pytest.raises(NotImplementedError):
+        nb.H1(m, order=2)
+    with pytest.raises(NotImplementedError):
+        nb.MaxwellSingleLayerPotentialOperator(None, 1.0)
+
+
+def op_create(space_trial, space_test, kernel, **p):
+    L = _capi.lib()
+    params = _capi.Params(p.get("intorder", 3), p.get("leafsize", 40), p.get("eta", 2.0), p.get("eps", 1e-6),
+                          int(p.get("dense", False)), 0)
+    out = ctypes.c_void_p()
+    _capi.check(L.bem_op_create(kernel, p.get("kappa", 0.0), space_trial.handle, space_test.handle,
+                                ctypes.byref(params), ctypes.byref(out)))
+    return out
+
+
+@pytest.mark.parametrize("level,trk,tek,leaf,eta", [(3, 0, 0, 40, 3.0), (4, 0, 0, 64, 2.0), (3, 1, 0, 20, 2.0),
+                                                    (4, 1, 1, 40, 2.0), (3, 0, 1, 16, 3.0)])
+def test_tree_and_blocks_identical_to_oracle(level, trk, tek, leaf, eta):
+    m = nb.icosphere(level)
+    sp = {0: nb.SurfaceL2, 1: nb.H1}
+    tr, te = sp[trk](m), sp[tek](m)
+    h = op_create(tr, te, _capi.LAPLACE_DL, leafsize=leaf, eta=eta)
+    L = _capi.lib()
+    try:
+        ref = O.OracleOp(m, O.LAPLACE_DL, trial=trk, test=tek, leafsize=leaf, eta=eta)
+        for which, n in ((0, tr.ndof), (1, te.ndof)):
+            perm = np.zeros(n, dtype=np.int32)
+            ncl = L.bem_op_tree(h, which, _capi.iptr(perm), None, None, 0)
+            cl = np.zeros(7 * ncl)
+            cen = np.zeros(3 * ncl)
+            L.bem_op_tree(h, which, _capi.iptr(perm), _capi.dptr(cl), _capi.dptr(cen), ncl)
+            operm, ocl, ocen = ref.tree(which)
+            assert np.array_equal(perm, operm)
+            assert np.array_equal(cl.reshape(-1, 7), ocl)  # bit-identical radii too
+            assert np.array_equal(cen.reshape(-1, 3), ocen)
+        nblk = L.bem_op_num_blocks(h)
+        assert nblk == ref.nblocks
+        for b in range(nblk):
+            bi = _capi.BlockInfo()
+            L.bem_op_block_info(h, b, ctypes.byref(bi))
+            rb = ref.block_info(b)
+            assert (bi.row_cluster, bi.col_cluster) == (rb["row"], rb["col"])
+            assert bool(bi.near and not bi.fallback) == rb["near"]
+    finally:
+        L.bem_op_destroy(h)
+
+
+def test_dense_method_is_one_block():
+    m = nb.icosphere(2)
+    s = nb.SurfaceL2(m)
+    h = op_create(s, s, _capi.LAPLACE_SL, dense=True)
+    L = _capi.lib()
+    assert L.bem_op_num_blocks(h) == 1
+    bi = _capi.BlockInfo()
+    L.bem_op_block_info(h, 0, ctypes.byref(bi))
+    assert bi.near == 1 and bi.m == m.nt and bi.n == m.nt
+    L.bem_op_destroy(h)

@@ -153,3 +153,24 @@ def test_mixed_matches_fp64(gpu):
     yt64 = C.mat.T * x
     yt32 = C.mat.mixed.T * x
     assert (torch.linalg.norm(yt64 - yt32) / torch.linalg.norm(yt64)).item() < 1e-5
+
+
+@pytest.mark.parametrize("case", [CASES[9], CASES[11], CASES[4]], ids=lambda c: f"{c[0]}-{c[1]}{c[2]}")
+def test_aca_resume_matches_oracle(gpu, case, monkeypatch):
+    """A tiny initial rank capacity forces every far block through the
+    resume path (capacity reached -> larger workspace -> continue)."""
+    monkeypatch.setenv("NGBEM_ACA_CAP0", "3")
+    op, ref = _build(case)
+    eps = case[7]
+    nfar = 0
+    for b in range(op.num_blocks()):
+        info = op.block_info(b)
+        if info["rank"] < 0:
+            continue
+        nfar += 1
+        rows, cols = op.block_dofs(b)
+        U, V = ref.aca(cols, rows)
+        A_ref = U @ V
+        d = op.block_data(b)
+        assert np.linalg.norm(d[0] @ d[1] - A_ref) <= 10 * eps * np.linalg.norm(A_ref)
+    assert nfar > 0

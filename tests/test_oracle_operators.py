@@ -1,0 +1,207 @@
+"""Pins for the oracle's operators, H-matrix and apply (CPU only).
+
+What the mathematics fixes, chosen so that a dropped term, a wrong sign or a
+transposed operand fails:
+- Laplace on the unit sphere: V1 = 1 (capacitance 4 pi), K1 = -1/2, W1 = 0
+  exactly (surface curl of a constant), and the spherical-harmonic eigenvalues
+  V: 1/(2n+1), K: -1/(2(2n+1)), W: n(n+1)/(2n+1) through Rayleigh quotients.
+- Helmholtz kernels reduce to Laplace as kappa -> 0; combined field =
+  DL - i kappa SL entry by entry.
+- ACA: ||D - U V||_F <= 10 eps ||D||_F against the dense block, and the best
+  rank-R approximation (SVD) is never worse than the ACA one.
+- Tree and partition: permutation is a bijection, children partition the
+  parent, leaves <= leafsize, blocks tile the matrix once, admissible blocks
+  satisfy the admissibility test.
+- Apply: equals the densified H-matrix; <A x, y> = <x, A^T y>.
+"""
+import numpy as np
+import pytest
+
+import ngbem_amd.mesh as M
+from oracle import oracle as O
+
+EPS_ROUND = 1e-13
+
+
+@pytest.fixture(scope="module")
+def sphere3():
+    return M.icosphere(3)
+
+
+def mass_p0(m):
+    return np.diag(m.areas())
+
+
+def mass_p1(m):
+    n = m.nv
+    A = np.zeros((n, n))
+    loc = (np.ones((3, 3)) + np.eye(3)) / 12.0
+    for t, a in zip(m.tri, m.areas()):
+        A[np.ix_(t, t)] += a * loc
+    return A
+
+
+def test_single_layer_sphere(sphere3):
+    m = sphere3
+    V = O.OracleOp(m, O.LAPLACE_SL, intorder=4).dense().real
+    area = m.areas()
+    # V 1 = 1 on the unit sphere (Galerkin: V 1 ~ M 1)
+    assert np.abs(V.sum(1) - area).max() / area.max() < 5e-3
+    # capacitance 4 pi
+    sigma = np.linalg.solve(V, area)
+    cap = area @ sigma
+    assert abs(cap - 4 * np.pi) / (4 * np.pi) < 5e-3
+    # eigenvalue of Y_1 (z): 1/3
+    z = m.xyz[m.tri].mean(1)[:, 2]
+    rq = (z @ V @ z) / (z @ np.diag(area) @ z)
+    assert abs(rq - 1.0 / 3.0) < 2e-2
+    # symmetric up to quadrature error
+    assert np.abs(V - V.T).max() / np.abs(V).max() < 1e-4
+
+
+def test_single_layer_converges_with_level():
+    errs = []
+    for lvl in (1, 2, 3):
+        m = M.icosphere(lvl)
+        V = O.OracleOp(m, O.LAPLACE_SL, intorder=4).dense().real
+        a = m.areas()
+        errs.append(abs(a @ np.linalg.solve(V, a) - 4 * np.pi))
+    assert errs[2] < errs[1] < errs[0]
+
+
+def test_double_layer_sphere(sphere3):
+    m = sphere3
+    K = O.OracleOp(m, O.LAPLACE_DL, trial=O.P0, test=O.P0, intorder=4).dense().real
+    area = m.areas()
+    # K 1 = -1/2 (Galerkin: K 1 ~ -1/2 M 1)
+    assert np.abs(K.sum(1) + 0.5 * area).max() / area.max() < 2e-2
+    z = m.xyz[m.tri].mean(1)[:, 2]
+    rq = (z @ K @ z) / (z @ np.diag(area) @ z)
+    assert abs(rq + 1.0 / 6.0) < 2e-2
+
+
+def test_double_layer_h1_trial(sphere3):
+    m = sphere3
+    K = O.OracleOp(m, O.LAPLACE_DL, trial=O.P1, test=O.P0, intorder=4).dense().real
+    assert K.shape == (m.nt, m.nv)
+    # K 1 = -1/2 M_{L2,H1} 1 = -area/2 per triangle
+    assert np.abs(K.sum(1) + 0.5 * m.areas()).max() / m.areas().max() < 2e-2
+
+
+def test_hypersingular_sphere():
+    m = M.icosphere(2)
+    W = O.OracleOp(m, O.LAPLACE_HS, trial=O.P1, test=O.P1, intorder=4).dense().real
+    # W 1 = 0 exactly: the surface curl of a constant vanishes element by element
+    assert np.abs(W.sum(1)).max() <= EPS_ROUND * np.abs(W).max() * m.nv
+    z = m.xyz[:, 2]
+    rq = (z @ W @ z) / (z @ mass_p1(m) @ z)
+    assert abs(rq - 2.0 / 3.0) < 3e-2
+    assert np.abs(W - W.T).max() / np.abs(W).max() < 1e-3
+
+
+@pytest.mark.parametrize("trial,test", [(O.P0, O.P0), (O.P1, O.P0)])
+def test_combined_field_is_dl_minus_i_kappa_sl(trial, test):
+    m = M.icosphere(2)
+    kappa = 3.7
+    kw = dict(trial=trial, test=test, intorder=4, kappa=kappa)
+    C = O.OracleOp(m, O.HELMHOLTZ_CF, **kw).dense()
+    K = O.OracleOp(m, O.HELMHOLTZ_DL, **kw).dense()
+    V = O.OracleOp(m, O.HELMHOLTZ_SL, **kw).dense()
+    assert np.abs(C - (K - 1j * kappa * V)).max() <= 1e-13 * np.abs(C).max()
+
+
+@pytest.mark.parametrize("hk,lk,trial", [(O.HELMHOLTZ_SL, O.LAPLACE_SL, O.P0), (O.HELMHOLTZ_DL, O.LAPLACE_DL, O.P0),
+                                         (O.HELMHOLTZ_HS, O.LAPLACE_HS, O.P1)])
+def test_helmholtz_reduces_to_laplace(hk, lk, trial):
+    m = M.icosphere(1)
+    kappa = 1e-6
+    H = O.OracleOp(m, hk, kappa=kappa, trial=trial, test=trial, intorder=4).dense()
+    L = O.OracleOp(m, lk, trial=trial, test=trial, intorder=4).dense()
+    assert np.abs(L.imag).max() == 0.0
+    assert np.abs(H - L).max() <= 1e-5 * np.abs(L).max()
+    # first-order term: imag(G_kappa) -> kappa/(4 pi) for SL
+    if hk == O.HELMHOLTZ_SL:
+        w = np.outer(m.areas(), m.areas())
+        assert np.allclose(H.imag / kappa, w / (4 * np.pi), rtol=1e-6, atol=0)
+
+
+def test_aca_vs_dense_and_svd():
+    m = M.icosphere(3)
+    eps = 1e-6
+    op = O.OracleOp(m, O.HELMHOLTZ_CF, kappa=6.0, intorder=4, leafsize=32, eta=2.0, eps=eps)
+    far = [b for b in range(op.nblocks) if not op.block_info(b)["near"]]
+    assert len(far) > 10
+    for b in far[:: max(1, len(far) // 12)]:
+        rows, cols = op.block_dofs(b)
+        D = op.dense(cols, rows)
+        U, V = op.aca(cols, rows)
+        R = U.shape[1]
+        err = np.linalg.norm(D - U @ V)
+        assert err <= 10 * eps * np.linalg.norm(D)
+        s = np.linalg.svd(D, compute_uv=False)
+        best = np.sqrt(np.sum(s[R:] ** 2))
+        assert best <= err * (1 + 1e-9) + 1e-300
+
+
+def test_aca_reference_criterion():
+    """tests/test_aca.py:21-28: ||(V_dense - V_H) x|| < eps with x random in [0,1)."""
+    m = M.icosphere(3)
+    eps = 1e-6
+    op = O.OracleOp(m, O.LAPLACE_SL, intorder=4, leafsize=40, eta=3.0, eps=eps)
+    op.assemble()
+    D = O.OracleOp(m, O.LAPLACE_SL, intorder=4, leafsize=10 ** 9, eta=0.0).dense()
+    x = np.random.default_rng(4711).random(m.nt)
+    err = np.linalg.norm(D @ x - op.apply(x))
+    assert err < eps
+    assert op.nze() < m.nt ** 2
+
+
+def tree_invariants(perm, cl, leafsize, n):
+    assert np.array_equal(np.sort(perm), np.arange(n))
+    level, parent, c1, c2, number, pos = (cl[:, k].astype(int) for k in range(6))
+    assert number[0] == n and pos[0] == 0
+    for i in range(len(cl)):
+        if c1[i] >= 0:
+            assert pos[c1[i]] == pos[i] and pos[c2[i]] == pos[i] + number[c1[i]]
+            assert number[c1[i]] + number[c2[i]] == number[i]
+            assert parent[c1[i]] == i and parent[c2[i]] == i
+            assert number[c1[i]] > 0 and number[c2[i]] > 0
+        else:
+            assert number[i] <= leafsize
+
+
+@pytest.mark.parametrize("trial,test,leaf,eta", [(O.P0, O.P0, 16, 2.0), (O.P1, O.P0, 20, 3.0), (O.P1, O.P1, 16, 2.0)])
+def test_tree_and_partition(trial, test, leaf, eta):
+    m = M.icosphere(3)
+    op = O.OracleOp(m, O.LAPLACE_DL, trial=trial, test=test, intorder=3, leafsize=leaf, eta=eta)
+    perm, cl, cen = op.tree(0)
+    tree_invariants(perm, cl, leaf, op.ntrial)
+    tperm, tcl, tcen = op.tree(1)
+    tree_invariants(tperm, tcl, leaf, op.ntest)
+    cover = np.zeros((op.ntest, op.ntrial), dtype=np.int32)
+    for b in range(op.nblocks):
+        info = op.block_info(b)
+        rows, cols = op.block_dofs(b)
+        cover[np.ix_(rows, cols)] += 1
+        if not info["near"]:
+            r, c = info["row"], info["col"]
+            assert tcl[r, 2] >= 0 and cl[c, 2] >= 0  # both non-leaf
+            dist = max(0.0, np.linalg.norm(tcen[r] - cen[c]) - tcl[r, 6] - cl[c, 6])
+            assert min(tcl[r, 6], cl[c, 6]) < eta * dist
+    assert (cover == 1).all()
+
+
+def test_apply_matches_densified_and_transpose():
+    m = M.icosphere(2)
+    op = O.OracleOp(m, O.HELMHOLTZ_DL, kappa=2.0, trial=O.P1, test=O.P0, intorder=4, leafsize=12, eta=2.0, eps=1e-8)
+    op.assemble()
+    A = op.densify()
+    rng = np.random.default_rng(3)
+    x = rng.standard_normal(op.ntrial) + 1j * rng.standard_normal(op.ntrial)
+    y = rng.standard_normal(op.ntest) + 1j * rng.standard_normal(op.ntest)
+    Ax = op.apply(x)
+    assert np.linalg.norm(Ax - A @ x) <= 1e-13 * np.linalg.norm(A @ x)
+    Aty = op.apply(y, trans=True)
+    assert np.linalg.norm(Aty - A.T @ y) <= 1e-13 * np.linalg.norm(A.T @ y)
+    # <A x, y> = <x, A^T y> (bilinear, plain transpose)
+    assert abs(np.sum(Ax * y) - np.sum(x * Aty)) <= 1e-12 * abs(np.sum(Ax * y))
