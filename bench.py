@@ -199,6 +199,8 @@ def main():
         pre = lambda r: r / area  # noqa: E731  (diagonal mass preconditioner)
         A64 = 0.5 * Mass + M64
         A32 = 0.5 * Mass + M32
+        nb.GMRes(A64, f, pre=pre, tol=1e-4, maxsteps=5, printrates=False)  # warm up torch's kernels
+        nb.RefinedSolve(A64, A32, f, pre=pre, tol=1e-3, inner_tol=1e-2, maxouter=1)
         torch.cuda.synchronize()
         t0 = time.perf_counter()
         u64 = nb.GMRes(A64, f, pre=pre, tol=1e-10, maxsteps=400, printrates=False)

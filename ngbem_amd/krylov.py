@@ -64,8 +64,12 @@ def CG(mat, rhs, pre=None, sol=None, tol=1e-12, maxsteps=100, printrates=True, i
 
 def GMRes(A, b, pre=None, freedofs=None, x=None, maxsteps=100, tol=None, innerproduct=None, callback=None,
           restart=None, startiteration=0, printrates=True, reltol=None):
-    """Restarted right-preconditioned GMRES (modified Gram-Schmidt, Givens
-    rotations); stops when ||r|| <= tol * ||b||."""
+    """Restarted right-preconditioned GMRES; stops when ||r|| <= tol * ||b||.
+
+    The Krylov basis is one (m+1) x n tensor on the GPU; each step orthogonalises
+    with classical Gram-Schmidt applied twice (two stacked matrix-vector
+    products), so the host reads back one small coefficient vector per step.
+    """
     if freedofs is not None or innerproduct is not None:
         raise NotImplementedError("freedofs / innerproduct are not supported")
     tol = 1e-7 if tol is None else tol
@@ -75,20 +79,23 @@ def GMRes(A, b, pre=None, freedofs=None, x=None, maxsteps=100, tol=None, innerpr
     b = b.to(dt)
     if x is None:
         x = torch.zeros_like(b)
-    m = restart or maxsteps
+    m = min(restart or maxsteps, maxsteps)
+    n = b.shape[0]
     bnorm = float(torch.linalg.norm(b))
     if bnorm == 0.0:
         x.zero_()
+        GMRes.iterations = 0
         return x
-    total = 0
+    Vb = torch.empty(m + 1, n, dtype=dt, device=b.device)
+    Zb = torch.empty(m, n, dtype=dt, device=b.device) if pre is not None else None
     cdt = np.complex128
+    total = 0
     while total < maxsteps:
         r = b - _apply(A, x)
         beta = float(torch.linalg.norm(r))
         if beta <= tol * bnorm:
             break
-        V = [r / beta]
-        Z = []
+        Vb[0] = r / beta
         H = np.zeros((m + 1, m), dtype=cdt)
         cs = np.zeros(m, dtype=cdt)
         sn = np.zeros(m, dtype=cdt)
@@ -96,15 +103,19 @@ def GMRes(A, b, pre=None, freedofs=None, x=None, maxsteps=100, tol=None, innerpr
         g[0] = beta
         k_used = 0
         for k in range(m):
-            z = _apply(pre, V[k]) if pre is not None else V[k]
-            Z.append(z)
+            z = _apply(pre, Vb[k]) if pre is not None else Vb[k]
+            if Zb is not None:
+                Zb[k] = z
             w = _apply(A, z)
-            for i in range(k + 1):  # modified Gram-Schmidt
-                h = _dot(V[i], w)
-                H[i, k] = complex(h)
-                w = w - h * V[i]
-            hn = float(torch.linalg.norm(w))
-            H[k + 1, k] = hn
+            Vk = Vb[: k + 1]
+            h = Vk.conj() @ w
+            w = w - h @ Vk
+            h2 = Vk.conj() @ w  # second pass (CGS2)
+            w = w - h2 @ Vk
+            hn_t = torch.linalg.norm(w)
+            hcol = torch.cat([h + h2, hn_t.reshape(1).to(h.dtype)]).cpu().numpy()
+            hn = float(hcol[k + 1].real)
+            H[: k + 2, k] = hcol
             for i in range(k):  # previous rotations
                 hi, hj = H[i, k], H[i + 1, k]
                 H[i, k] = np.conj(cs[i]) * hi + np.conj(sn[i]) * hj
@@ -125,10 +136,11 @@ def GMRes(A, b, pre=None, freedofs=None, x=None, maxsteps=100, tol=None, innerpr
                 callback(res)
             if res <= tol * bnorm or hn == 0.0 or total >= maxsteps:
                 break
-            V.append(w / hn)
+            Vb[k + 1] = w / hn
         y = np.linalg.solve(np.triu(H[:k_used, :k_used]), g[:k_used])
-        for i in range(k_used):
-            x.add_(complex(y[i]) * Z[i] if torch.is_complex(x) else float(y[i].real) * Z[i])
+        yt = torch.from_numpy(y if torch.is_complex(x) else y.real.copy()).to(device=x.device, dtype=dt)
+        basis = Zb[:k_used] if Zb is not None else Vb[:k_used]
+        x.add_(yt @ basis)
         if abs(g[k_used]) <= tol * bnorm:
             break
     GMRes.iterations = total
