@@ -535,7 +535,18 @@ __global__ __launch_bounds__(NTH) void k_aca(AcaTask* __restrict__ tasks, const 
       Y.get(j1c, y1, n1);
       T v0 = kcomp(ker, xi, y0, nxi, n0, comp);
       T v1 = kcomp(ker, xi, y1, nxi, n1, comp);
-      for (int l = 0; l < k; l++) {
+      int l = 0;
+      for (; l + 1 < k; l += 2) {  // four independent loads in flight
+        const T c = s_coef[l], d = s_coef[l + 1];
+        const T* Vl = V + (size_t)l * nq;
+        const T* Vm = Vl + nq;
+        const T a0 = Vl[j0], a1 = Vl[j1c], b0 = Vm[j0], b1 = Vm[j1c];
+        v0 -= a0 * c;
+        v1 -= a1 * c;
+        v0 -= b0 * d;
+        v1 -= b1 * d;
+      }
+      if (l < k) {
         const T c = s_coef[l];
         const T* Vl = V + (size_t)l * nq;
         v0 -= Vl[j0] * c;
@@ -580,7 +591,18 @@ __global__ __launch_bounds__(NTH) void k_aca(AcaTask* __restrict__ tasks, const 
       X.get(i1c, x1, m1);
       T u0 = kcomp(ker, x0, yk, m0, nyk, comp);
       T u1 = kcomp(ker, x1, yk, m1, nyk, comp);
-      for (int l = 0; l < k; l++) {
+      int l = 0;
+      for (; l + 1 < k; l += 2) {
+        const T c = s_coef[l], d = s_coef[l + 1];
+        const T* Ul = U + (size_t)l * np;
+        const T* Um = Ul + np;
+        const T a0 = Ul[i0], a1 = Ul[i1c], b0 = Um[i0], b1 = Um[i1c];
+        u0 -= a0 * c;
+        u1 -= a1 * c;
+        u0 -= b0 * d;
+        u1 -= b1 * d;
+      }
+      if (l < k) {
         const T c = s_coef[l];
         const T* Ul = U + (size_t)l * np;
         u0 -= Ul[i0] * c;
@@ -711,77 +733,70 @@ struct ProjTask {
   int64_t ou, ov;            // output offsets (batch buffer)
 };
 
+constexpr int PROJ_CHUNK = 2048;  // outputs per workgroup
+
+// one workgroup per (task, chunk of PROJ_CHUNK outputs); outputs are Ut
+// (R x m, ii fastest) followed by V (R x n, jj fastest)
 template <class KER, class EI, class EJ>
-__global__ __launch_bounds__(256) void k_project(const ProjTask* __restrict__ tasks, const FarBlk* __restrict__ fb,
-                                                 const int* __restrict__ fpatch, const int* __restrict__ maps,
-                                                 const double* __restrict__ quad, const double* __restrict__ curl,
-                                                 const double* __restrict__ trig, int ntq,
-                                                 const typename KER::T* __restrict__ wsU,
-                                                 const typename KER::T* __restrict__ wsV, typename KER::T* __restrict__ oU,
-                                                 typename KER::T* __restrict__ oV) {
+__global__ __launch_bounds__(256) void k_project(const ProjTask* __restrict__ tasks, const int2* __restrict__ items,
+                                                 const FarBlk* __restrict__ fb, const int* __restrict__ fpatch,
+                                                 const int* __restrict__ maps, const double* __restrict__ quad,
+                                                 const double* __restrict__ curl, const double* __restrict__ trig,
+                                                 int ntq, const typename KER::T* __restrict__ wsU,
+                                                 const typename KER::T* __restrict__ wsV,
+                                                 typename KER::T* __restrict__ oU, typename KER::T* __restrict__ oV) {
   typedef typename KER::T T;
-  const ProjTask tk = tasks[blockIdx.x];
+  const int2 it = items[blockIdx.x];
+  const ProjTask tk = tasks[it.x];
   const FarBlk b = fb[tk.blk];
-  int r0[KER::NTERM], rk[KER::NTERM], R = 0;
+  int r0[KER::NTERM], R = 0;
 #pragma unroll
   for (int t = 0; t < KER::NTERM; t++) {
     r0[t] = R;
-    rk[t] = tk.rank[KER::term(t).kc];
-    R += rk[t];
+    R += tk.rank[KER::term(t).kc];
   }
-  const int* PI = fpatch + b.pI;
-  const int* PJ = fpatch + b.pJ;
-  const int* rmo = maps + b.rmap;  // m+1 offsets then entries a*4+k
-  const int* cmo = maps + b.cmap;
-  // U2: m x R
-  for (int idx = threadIdx.x; idx < b.m * R; idx += 256) {
-    const int col = idx / b.m, ii = idx - col * b.m;
+  const int nU = b.m * R, total = nU + R * b.n;
+  const int end = min(total, it.y + PROJ_CHUNK);
+  for (int idx = it.y + (int)threadIdx.x; idx < end; idx += 256) {
+    const bool isU = idx < nU;
+    const int len = isU ? b.m : b.n;
+    const int loc = isU ? idx : idx - nU;
+    const int col = loc / len, pos = loc - col * len;
     int t = 0;
-    while (t + 1 < KER::NTERM && col >= r0[t + 1]) t++;
+#pragma unroll
+    for (int q = 1; q < KER::NTERM; q++)
+      if (col >= r0[q]) t = q;
     const TermC tm = KER::term(t);
     const int l = col - r0[t];
-    const T* Uc = wsU + tk.uoff[tm.kc] + (size_t)l * tk.np;
+    const T* W = isU ? wsU + tk.uoff[tm.kc] + (size_t)l * tk.np : wsV + tk.voff[tm.kc] + (size_t)l * tk.nq;
+    const int* mp = maps + (isU ? b.rmap : b.cmap);
+    const int* P = fpatch + (isU ? b.pI : b.pJ);
+    const int comp = isU ? tm.tec : tm.trc;
     T s = zero_<T>();
-    for (int k2 = rmo[ii]; k2 < rmo[ii + 1]; k2++) {
-      const int ent = rmo[b.m + 1 + k2];
-      const int a = ent >> 2, kl = ent & 3, e = PI[a];
+    for (int k2 = mp[pos]; k2 < mp[pos + 1]; k2++) {
+      const int ent = mp[len + 1 + k2];
+      const int a = ent >> 2, kl = ent & 3, e = P[a];
       double c9[9];
-      if (EI::DIM > 1)
+      if ((isU ? EI::DIM : EJ::DIM) > 1)
         for (int c = 0; c < 9; c++) c9[c] = curl[(size_t)e * 9 + c];
       for (int q = 0; q < ntq; q++) {
         const double lam[3] = {trig[3 * q], trig[3 * q + 1], 1.0 - trig[3 * q] - trig[3 * q + 1]};
-        double sh[EI::NL * EI::DIM];
-        EI::shape(lam, c9, sh);
+        double sh;
+        if (isU) {
+          double v[EI::NL * EI::DIM];
+          EI::shape(lam, c9, v);
+          sh = v[kl * EI::DIM + comp];
+        } else {
+          double v[EJ::NL * EJ::DIM];
+          EJ::shape(lam, c9, v);
+          sh = v[kl * EJ::DIM + comp];
+        }
         const double w = quad[((size_t)e * ntq + q) * 4 + 3];
-        s += rmul(Uc[a * ntq + q], sh[kl * EI::DIM + tm.tec] * w);
+        s += rmul(W[a * ntq + q], sh * w);
       }
     }
-    oU[tk.ou + (size_t)col * b.m + ii] = rmul(s, tm.fac);
-  }
-  // V2: R x n
-  for (int idx = threadIdx.x; idx < b.n * R; idx += 256) {
-    const int col = idx / b.n, jj = idx - col * b.n;
-    int t = 0;
-    while (t + 1 < KER::NTERM && col >= r0[t + 1]) t++;
-    const TermC tm = KER::term(t);
-    const int l = col - r0[t];
-    const T* Vc = wsV + tk.voff[tm.kc] + (size_t)l * tk.nq;
-    T s = zero_<T>();
-    for (int k2 = cmo[jj]; k2 < cmo[jj + 1]; k2++) {
-      const int ent = cmo[b.n + 1 + k2];
-      const int a = ent >> 2, kl = ent & 3, e = PJ[a];
-      double c9[9];
-      if (EJ::DIM > 1)
-        for (int c = 0; c < 9; c++) c9[c] = curl[(size_t)e * 9 + c];
-      for (int q = 0; q < ntq; q++) {
-        const double lam[3] = {trig[3 * q], trig[3 * q + 1], 1.0 - trig[3 * q] - trig[3 * q + 1]};
-        double sh[EJ::NL * EJ::DIM];
-        EJ::shape(lam, c9, sh);
-        const double w = quad[((size_t)e * ntq + q) * 4 + 3];
-        s += rmul(Vc[a * ntq + q], sh[kl * EJ::DIM + tm.trc] * w);
-      }
-    }
-    oV[tk.ov + (size_t)col * b.n + jj] = s;
+    if (isU) oU[tk.ou + (size_t)col * b.m + pos] = rmul(s, tm.fac);
+    else oV[tk.ov + (size_t)col * b.n + pos] = s;
   }
 }
 
@@ -817,28 +832,32 @@ __global__ void k_scatter(const int* __restrict__ perm, int n, const VT* __restr
 // cross-lane reduction is needed.
 
 // t_b = V_b x[c0 : c0+n] (forward)  or  t_b = Ut_b x[r0 : r0+m] (transpose);
-// one wave per rank row, lanes along the contiguous row
+// one wave per low-rank block (four per workgroup), its 16-lane groups own rank
+// rows (a 16-wide slice of a row is one cache line)
 template <class ST, class VT, bool TRANS, bool CONJ>
-__global__ __launch_bounds__(256) void k_mv_far_t(const FarBlk* __restrict__ fb, const ST* __restrict__ fUt,
+__global__ __launch_bounds__(256) void k_mv_far_t(const FarBlk* __restrict__ fb, int nfar, const ST* __restrict__ fUt,
                                                   const ST* __restrict__ fV, const VT* __restrict__ xp,
                                                   VT* __restrict__ tbuf) {
-  const FarBlk b = fb[blockIdx.x];
-  const int lane = threadIdx.x & 63;
-  const int wv = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+  const int blk = blockIdx.x * 4 + (threadIdx.x >> 6);
+  if (blk >= nfar) return;
+  const FarBlk b = fb[blk];
+  const int g = (threadIdx.x >> 4) & 3, l16 = threadIdx.x & 15;
   const ST* F = TRANS ? fUt + b.uoff : fV + b.voff;
   const int len = TRANS ? b.m : b.n;
   const VT* xx = xp + (TRANS ? b.r0 : b.c0);
-  for (int l = wv; l < b.R; l += 4) {
-    const ST* row = F + (int64_t)l * len;
+  for (int l0 = 0; l0 < b.R; l0 += 4) {
+    const int l = l0 + g;
+    const bool act = l < b.R;
+    const ST* row = F + (int64_t)(act ? l : l0) * len;
     VT a0 = zero_<VT>(), a1 = zero_<VT>();
-    int j = lane;
-    for (; j + 64 < len; j += 128) {
+    int j = l16;
+    for (; j + 16 < len; j += 32) {
       VT u, v;
       cvt(u, row[j]);
-      cvt(v, row[j + 64]);
+      cvt(v, row[j + 16]);
       if (CONJ) { u = cconj(u); v = cconj(v); }
       a0 += u * xx[j];
-      a1 += v * xx[j + 64];
+      a1 += v * xx[j + 16];
     }
     if (j < len) {
       VT u;
@@ -846,8 +865,10 @@ __global__ __launch_bounds__(256) void k_mv_far_t(const FarBlk* __restrict__ fb,
       if (CONJ) u = cconj(u);
       a0 += u * xx[j];
     }
-    a0 = wave_sum(a0 + a1);
-    if (lane == 0) tbuf[b.toff + l] = a0;
+    a0 += a1;
+#pragma unroll
+    for (int m = 8; m >= 1; m >>= 1) a0 += shfl_xor(a0, m);
+    if (l16 == 0 && act) tbuf[b.toff + l] = a0;
   }
 }
 
@@ -860,61 +881,51 @@ struct Tile {
 
 // forward: one workgroup per (leaf row cluster, 64-row chunk); lane r owns row
 // p0 + r, the four waves split the columns (dense) / rank rows (low rank) of
-// every covering block and are summed through LDS once at the end
+// every covering block, four loads in flight per wave, summed through LDS
+// once at the end
 template <class ST, class VT>
 __global__ __launch_bounds__(256) void k_mv_rows(const Tile* __restrict__ tiles, const Seg* __restrict__ segs,
-                                                 const NearBlk* __restrict__ nbk, const FarBlk* __restrict__ fbk,
                                                  const ST* __restrict__ nval, const ST* __restrict__ fUt,
                                                  const VT* __restrict__ xp, const VT* __restrict__ tbuf,
                                                  VT* __restrict__ yp) {
   __shared__ VT red[4][64];
+  __shared__ Seg s_seg[64];
   const Tile tl = tiles[blockIdx.x];
   const int lane = threadIdx.x & 63;
   const int wv = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
   const bool act = lane < tl.len;
-  const int r = act ? lane : 0;
-  VT acc0 = zero_<VT>(), acc1 = zero_<VT>();
-  for (int s = tl.s0; s < tl.s1; s++) {
-    const Seg sg = segs[s];
-    if (!sg.far) {
-      const NearBlk b = nbk[sg.blk];
-      const ST* col = nval + b.off + sg.off + tl.roff + r;
-      const VT* xx = xp + b.c0;
-      const int64_t ld = b.m;
+  const int r = (act ? lane : 0) + tl.roff;
+  VT acc0 = zero_<VT>(), acc1 = zero_<VT>(), acc2 = zero_<VT>(), acc3 = zero_<VT>();
+  for (int sb = tl.s0; sb < tl.s1; sb += 64) {
+    const int ns = min(64, tl.s1 - sb);
+    __syncthreads();
+    if ((int)threadIdx.x < ns) s_seg[threadIdx.x] = segs[sb + threadIdx.x];
+    __syncthreads();
+    for (int k = 0; k < ns; k++) {
+      const Seg sg = s_seg[k];
+      const ST* col = (sg.far ? fUt : nval) + sg.base + r;
+      const VT* xs = (sg.far ? tbuf : xp) + sg.xoff;
+      const int64_t ld = sg.ld;
       int j = wv;
-      for (; j + 4 < b.n; j += 8) {
-        VT u, v;
-        cvt(u, col[j * ld]);
-        cvt(v, col[(j + 4) * ld]);
-        acc0 += u * xx[j];
-        acc1 += v * xx[j + 4];
+      for (; j + 12 < sg.cnt; j += 16) {
+        VT u0, u1, u2, u3;
+        cvt(u0, col[j * ld]);
+        cvt(u1, col[(j + 4) * ld]);
+        cvt(u2, col[(j + 8) * ld]);
+        cvt(u3, col[(j + 12) * ld]);
+        acc0 += u0 * xs[j];
+        acc1 += u1 * xs[j + 4];
+        acc2 += u2 * xs[j + 8];
+        acc3 += u3 * xs[j + 12];
       }
-      if (j < b.n) {
+      for (; j < sg.cnt; j += 4) {
         VT u;
         cvt(u, col[j * ld]);
-        acc0 += u * xx[j];
-      }
-    } else {
-      const FarBlk b = fbk[sg.blk];
-      const ST* col = fUt + b.uoff + sg.off + tl.roff + r;
-      const VT* tt = tbuf + b.toff;
-      const int64_t ld = b.m;
-      int l = wv;
-      for (; l + 4 < b.R; l += 8) {
-        VT u, v;
-        cvt(u, col[l * ld]);
-        cvt(v, col[(l + 4) * ld]);
-        acc0 += u * tt[l];
-        acc1 += v * tt[l + 4];
-      }
-      if (l < b.R) {
-        VT u;
-        cvt(u, col[l * ld]);
-        acc0 += u * tt[l];
+        acc0 += u * xs[j];
       }
     }
   }
-  red[wv][lane] = acc0 + acc1;
+  red[wv][lane] = (acc0 + acc1) + (acc2 + acc3);
   __syncthreads();
   if (wv == 0 && act) yp[tl.p0 + lane] = red[0][lane] + red[1][lane] + red[2][lane] + red[3][lane];
 }
@@ -924,46 +935,58 @@ __global__ __launch_bounds__(256) void k_mv_rows(const Tile* __restrict__ tiles,
 // low rank: lanes own columns, waves split the rank rows of V
 template <class ST, class VT, bool CONJ>
 __global__ __launch_bounds__(256) void k_mv_cols(const Tile* __restrict__ tiles, const Seg* __restrict__ segs,
-                                                 const NearBlk* __restrict__ nbk, const FarBlk* __restrict__ fbk,
                                                  const ST* __restrict__ nval, const ST* __restrict__ fV,
                                                  const VT* __restrict__ xp, const VT* __restrict__ tbuf,
                                                  VT* __restrict__ yp) {
   __shared__ VT red[4][64];
   __shared__ VT colsum[64];
+  __shared__ Seg s_seg[64];
   const Tile tl = tiles[blockIdx.x];
   const int lane = threadIdx.x & 63;
   const int wv = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
   const bool act = lane < tl.len;
-  const int c = act ? lane : 0;
+  const int c = (act ? lane : 0) + tl.roff;
   if (wv == 0) colsum[lane] = zero_<VT>();
-  __syncthreads();
   VT acc = zero_<VT>();
-  for (int s = tl.s0; s < tl.s1; s++) {
-    const Seg sg = segs[s];
-    if (!sg.far) {
-      const NearBlk b = nbk[sg.blk];
-      const VT* xx = xp + b.r0;
-      for (int cc = wv; cc < tl.len; cc += 4) {  // wave-owned column (same wave every segment)
-        const ST* col = nval + b.off + (int64_t)(sg.off + tl.roff + cc) * b.m;
-        VT a = zero_<VT>();
-        for (int i = lane; i < b.m; i += 64) {
-          VT u;
-          cvt(u, col[i]);
-          if (CONJ) u = cconj(u);
-          a += u * xx[i];
+  for (int sb = tl.s0; sb < tl.s1; sb += 64) {
+    const int ns = min(64, tl.s1 - sb);
+    __syncthreads();
+    if ((int)threadIdx.x < ns) s_seg[threadIdx.x] = segs[sb + threadIdx.x];
+    __syncthreads();
+    for (int k = 0; k < ns; k++) {
+      const Seg sg = s_seg[k];
+      if (!sg.far) {
+        const VT* xx = xp + sg.xoff;
+        for (int cc = wv; cc < tl.len; cc += 4) {  // wave-owned column (same wave in every segment)
+          const ST* col = nval + sg.base + (int64_t)(tl.roff + cc) * sg.ld;
+          VT a = zero_<VT>(), a2 = zero_<VT>();
+          int i = lane;
+          for (; i + 64 < sg.cnt; i += 128) {
+            VT u, v;
+            cvt(u, col[i]);
+            cvt(v, col[i + 64]);
+            if (CONJ) { u = cconj(u); v = cconj(v); }
+            a += u * xx[i];
+            a2 += v * xx[i + 64];
+          }
+          if (i < sg.cnt) {
+            VT u;
+            cvt(u, col[i]);
+            if (CONJ) u = cconj(u);
+            a += u * xx[i];
+          }
+          a = wave_sum(a + a2);
+          if (lane == 0) colsum[cc] += a;
         }
-        a = wave_sum(a);
-        if (lane == 0) colsum[cc] += a;
-      }
-    } else {
-      const FarBlk b = fbk[sg.blk];
-      const ST* row = fV + b.voff + sg.off + tl.roff + c;
-      const VT* tt = tbuf + b.toff;
-      for (int l = wv; l < b.R; l += 4) {
-        VT u;
-        cvt(u, row[(int64_t)l * b.n]);
-        if (CONJ) u = cconj(u);
-        acc += u * tt[l];
+      } else {
+        const ST* row = fV + sg.base + c;
+        const VT* tt = tbuf + sg.xoff;
+        for (int l = wv; l < sg.cnt; l += 4) {
+          VT u;
+          cvt(u, row[(int64_t)l * sg.ld]);
+          if (CONJ) u = cconj(u);
+          acc += u * tt[l];
+        }
       }
     }
   }
@@ -1071,6 +1094,11 @@ void setup_device(Operator& op, hipStream_t s) {
     }
     op.d_ss[k].upload(rr, s);
   }
+  {
+    std::vector<int> patch_all(op.cp_test->list);  // test patches, then trial patches when the spaces differ
+    if (!op.same_space) patch_all.insert(patch_all.end(), op.cp_trial.list.begin(), op.cp_trial.list.end());
+    op.d_patch.upload(patch_all, s);
+  }
   op.d_perm_trial.upload(op.trial_tree.perm, s);
   op.d_iperm_trial.upload(op.trial_tree.iperm, s);
   op.d_perm_test.upload(op.test_tree->perm, s);
@@ -1175,11 +1203,7 @@ void assemble_near(Operator& op, hipStream_t s) {
   op.d_near_blk.upload(op.near, s);
   pt_.mark("stats");
   op.d_near_chunks.upload(chunks, s);
-  {
-    std::vector<int> patch_all(cpi.list);  // test patches, then trial patches when the spaces differ
-    if (!op.same_space) patch_all.insert(patch_all.end(), cpj.list.begin(), cpj.list.end());
-    op.d_patch.upload(patch_all, s);
-  }
+
   for (int c = 0; c < 3; c++) op.d_spairs[c].upload(pairs[c], s);
   const size_t esz = op.complex_valued ? 16 : 8;
   op.d_near_val.alloc((size_t)off * esz);
@@ -1467,16 +1491,27 @@ void assemble_far(Operator& op, hipStream_t s) {
       piece->U.alloc(std::max<size_t>(piece->nu, 1) * esz);
       piece->V.alloc(std::max<size_t>(piece->nv, 1) * esz);
       d_ptasks.upload(ptasks, s);
+      std::vector<int2> items;
+      for (size_t t = 0; t < ptasks.size(); t++) {
+        const FarBlk& fb = op.far[ptasks[t].blk];
+        const int R = op.far_rank[ptasks[t].blk];
+        const int total = R * (fb.m + fb.n);
+        for (int st = 0; st < total; st += PROJ_CHUNK) items.push_back(make_int2((int)t, st));
+      }
+      DBuf d_items;
+      d_items.upload(items, s);
       DBuf& SU = bufU(w);
       DBuf& SV = bufV(w);
-      dispatch_full(op, [&](auto ker, auto ei, auto ej) {
-        using KER = decltype(ker);
-        using T = typename KER::T;
-        hipLaunchKernelGGL((k_project<KER, decltype(ei), decltype(ej)>), dim3((unsigned)ptasks.size()), dim3(256), 0,
-                           s, d_ptasks.as<ProjTask>(), op.d_far_blk.as<FarBlk>(), op.d_patch.as<int>(),
-                           op.d_far_maps.as<int>(), op.d_quad.as<double>(), op.d_curl.as<double>(),
-                           op.d_trig.as<double>(), ntq, SU.as<T>(), SV.as<T>(), piece->U.as<T>(), piece->V.as<T>());
-      });
+      if (!items.empty())
+        dispatch_full(op, [&](auto ker, auto ei, auto ej) {
+          using KER = decltype(ker);
+          using T = typename KER::T;
+          hipLaunchKernelGGL((k_project<KER, decltype(ei), decltype(ej)>), dim3((unsigned)items.size()), dim3(256),
+                             0, s, d_ptasks.as<ProjTask>(), d_items.as<int2>(), op.d_far_blk.as<FarBlk>(),
+                             op.d_patch.as<int>(), op.d_far_maps.as<int>(), op.d_quad.as<double>(),
+                             op.d_curl.as<double>(), op.d_trig.as<double>(), ntq, SU.as<T>(), SV.as<T>(),
+                             piece->U.as<T>(), piece->V.as<T>());
+        });
       BEM_HIP(hipGetLastError());
       BEM_HIP(hipStreamSynchronize(s));
       pieces.push_back(std::move(piece));
@@ -1534,10 +1569,12 @@ static void plan(const Operator& op, bool trans, std::vector<Tile>& tiles, std::
   std::vector<int> lp0(nleaf);
   for (int i = 0; i < nleaf; i++) lp0[i] = (int)tree.cl[tree.leaves[i]].permupos;
   std::vector<std::vector<Seg>> per(nleaf);
-  auto add = [&](int slot, int far, int p0, int len) {
+  // p0/len: the block's position range on the tiled side; mk(o): segment for
+  // a leaf starting o positions into the block
+  auto add = [&](int p0, int len, auto mk) {
     int li = (int)(std::upper_bound(lp0.begin(), lp0.end(), p0) - lp0.begin()) - 1;
     while (li < nleaf && lp0[li] < p0 + len) {
-      per[li].push_back(Seg{slot, far, lp0[li] - p0, 0});
+      per[li].push_back(mk(lp0[li] - p0));
       li++;
     }
   };
@@ -1545,12 +1582,16 @@ static void plan(const Operator& op, bool trans, std::vector<Tile>& tiles, std::
     const int slot = op.blk_slot[b];
     if (op.blocks[b].near || op.blocks[b].fallback) {
       const NearBlk& nb = op.near[slot];
-      if (trans) add(slot, 0, nb.c0, nb.n);
-      else add(slot, 0, nb.r0, nb.m);
+      if (!trans)
+        add(nb.r0, nb.m, [&](int o) { return Seg{nb.off + o, nb.m, nb.n, nb.c0, 0}; });
+      else
+        add(nb.c0, nb.n, [&](int o) { return Seg{nb.off + (int64_t)o * nb.m, nb.m, nb.m, nb.r0, 0}; });
     } else {
       const FarBlk& fb = op.far[slot];
-      if (trans) add(slot, 1, fb.c0, fb.n);
-      else add(slot, 1, fb.r0, fb.m);
+      if (!trans)
+        add(fb.r0, fb.m, [&](int o) { return Seg{fb.uoff + o, fb.m, fb.R, fb.toff, 1}; });
+      else
+        add(fb.c0, fb.n, [&](int o) { return Seg{fb.voff + o, fb.n, fb.R, fb.toff, 1}; });
     }
   }
   tiles.clear();
@@ -1634,28 +1675,25 @@ static void apply_t(Operator& op, int flags, const XT* x, XT* y, XT alpha, XT be
   mark(1);
   if (!trans) {
     if (nfar)
-      hipLaunchKernelGGL((k_mv_far_t<ST, VT, false, false>), dim3(nfar), dim3(256), 0, s, op.d_far_blk.as<FarBlk>(),
-                         fU, fV, xp, tb);
+      hipLaunchKernelGGL((k_mv_far_t<ST, VT, false, false>), dim3((nfar + 3) / 4), dim3(256), 0, s,
+                         op.d_far_blk.as<FarBlk>(), nfar, fU, fV, xp, tb);
     mark(2);
     hipLaunchKernelGGL((k_mv_rows<ST, VT>), dim3(op.n_tiles_fwd), dim3(256), 0, s, op.d_tile_fwd.as<Tile>(),
-                       op.d_seg_fwd.as<Seg>(), op.d_near_blk.as<NearBlk>(), op.d_far_blk.as<FarBlk>(), nval, fU, xp,
-                       tb, yp);
+                       op.d_seg_fwd.as<Seg>(), nval, fU, xp, tb, yp);
   } else if (!conj) {
     if (nfar)
-      hipLaunchKernelGGL((k_mv_far_t<ST, VT, true, false>), dim3(nfar), dim3(256), 0, s, op.d_far_blk.as<FarBlk>(),
-                         fU, fV, xp, tb);
+      hipLaunchKernelGGL((k_mv_far_t<ST, VT, true, false>), dim3((nfar + 3) / 4), dim3(256), 0, s,
+                         op.d_far_blk.as<FarBlk>(), nfar, fU, fV, xp, tb);
     mark(2);
     hipLaunchKernelGGL((k_mv_cols<ST, VT, false>), dim3(op.n_tiles_trn), dim3(256), 0, s, op.d_tile_trn.as<Tile>(),
-                       op.d_seg_trn.as<Seg>(), op.d_near_blk.as<NearBlk>(), op.d_far_blk.as<FarBlk>(), nval, fV, xp,
-                       tb, yp);
+                       op.d_seg_trn.as<Seg>(), nval, fV, xp, tb, yp);
   } else {
     if (nfar)
-      hipLaunchKernelGGL((k_mv_far_t<ST, VT, true, true>), dim3(nfar), dim3(256), 0, s, op.d_far_blk.as<FarBlk>(),
-                         fU, fV, xp, tb);
+      hipLaunchKernelGGL((k_mv_far_t<ST, VT, true, true>), dim3((nfar + 3) / 4), dim3(256), 0, s,
+                         op.d_far_blk.as<FarBlk>(), nfar, fU, fV, xp, tb);
     mark(2);
     hipLaunchKernelGGL((k_mv_cols<ST, VT, true>), dim3(op.n_tiles_trn), dim3(256), 0, s, op.d_tile_trn.as<Tile>(),
-                       op.d_seg_trn.as<Seg>(), op.d_near_blk.as<NearBlk>(), op.d_far_blk.as<FarBlk>(), nval, fV, xp,
-                       tb, yp);
+                       op.d_seg_trn.as<Seg>(), nval, fV, xp, tb, yp);
   }
   mark(3);
   hipLaunchKernelGGL((k_scatter<XT, VT>), dim3((nout + 255) / 256), dim3(256), 0, s, pout, nout, yp, y, alpha, beta,

@@ -88,11 +88,14 @@ struct AcaTask {
 };
 
 // matvec segment: the part of block `blk` that lies in one leaf tile
+// Self-contained, so a tile can stage all of its segments in LDS with one load
+// round.  Forward (lanes own rows): value (r, j) of the leaf's rows sits at
+// base + r + j*ld, j < cnt, multiplied by xs[xoff + j].  Transpose: dense
+// columns start at base + c*ld and hold cnt rows (x at xoff); low-rank rows
+// of V start at base + l*ld (+ c), l < cnt (t at xoff).
 struct Seg {
-  int blk;   // near: index in near table; far: index in far table
-  int far;   // 0 near, 1 far
-  int off;   // first row (forward) / column (transpose) of the tile inside the block
-  int pad;
+  int64_t base;
+  int ld, cnt, xoff, far;
 };
 
 struct Operator {

@@ -39,6 +39,10 @@ CASES = [
     ("HCF", 0, 0, 2, 4, 16, 2.0, 1e-8, 4.0),
     ("HCF", 0, 0, 3, 6, 32, 2.0, 1e-6, 10.0),
     ("HHS", 1, 1, 2, 4, 16, 2.0, 1e-8, 4.0),
+    # H1 cases with admissible blocks (multi-entry projection maps, curl shapes)
+    ("LHS", 1, 1, 3, 3, 16, 2.0, 1e-8, 0.0),
+    ("HHS", 1, 1, 3, 3, 16, 2.0, 1e-8, 4.0),
+    ("LDL", 1, 0, 3, 3, 16, 2.0, 1e-8, 0.0),
 ]
 
 
@@ -100,7 +104,7 @@ def test_blocks_match_oracle(gpu, case):
         assert same_rank >= 0.9 * nfar
 
 
-@pytest.mark.parametrize("case", CASES[:1] + CASES[4:6] + CASES[9:10] + CASES[11:12],
+@pytest.mark.parametrize("case", CASES[:1] + CASES[4:6] + CASES[9:10] + CASES[11:15],
                          ids=lambda c: f"{c[0]}-{c[1]}{c[2]}")
 def test_matvec_matches_oracle(gpu, case):
     op, ref = _build(case)
@@ -155,7 +159,7 @@ def test_mixed_matches_fp64(gpu):
     assert (torch.linalg.norm(yt64 - yt32) / torch.linalg.norm(yt64)).item() < 1e-5
 
 
-@pytest.mark.parametrize("case", [CASES[9], CASES[11], CASES[4]], ids=lambda c: f"{c[0]}-{c[1]}{c[2]}")
+@pytest.mark.parametrize("case", [CASES[9], CASES[13], CASES[14]], ids=lambda c: f"{c[0]}-{c[1]}{c[2]}")
 def test_aca_resume_matches_oracle(gpu, case, monkeypatch):
     """A tiny initial rank capacity forces every far block through the
     resume path (capacity reached -> larger workspace -> continue)."""
