@@ -831,45 +831,61 @@ __global__ void k_scatter(const int* __restrict__ perm, int n, const VT* __restr
 // (dense) or of a rank row (Ut) is one coalesced transaction and no
 // cross-lane reduction is needed.
 
-// t_b = V_b x[c0 : c0+n] (forward)  or  t_b = Ut_b x[r0 : r0+m] (transpose);
-// one wave per low-rank block (four per workgroup), its 16-lane groups own rank
-// rows (a 16-wide slice of a row is one cache line)
+// t_b = V_b x[c0 : c0+n] (forward)  or  t_b = Ut_b x[r0 : r0+m] (transpose).
+// Work items are waves: a long row (>= 256 entries) gets a whole wave, short
+// rows go four to a wave with 16 lanes each (a 16-wide slice of a row is one
+// cache line); item = (far block, first rank row, rows, lanes per row).
+struct RowItem {
+  int blk, l0, nrows, lanes;
+};
+
 template <class ST, class VT, bool TRANS, bool CONJ>
-__global__ __launch_bounds__(256) void k_mv_far_t(const FarBlk* __restrict__ fb, int nfar, const ST* __restrict__ fUt,
+__global__ __launch_bounds__(256) void k_mv_far_t(const RowItem* __restrict__ items, int nitems,
+                                                  const FarBlk* __restrict__ fb, const ST* __restrict__ fUt,
                                                   const ST* __restrict__ fV, const VT* __restrict__ xp,
                                                   VT* __restrict__ tbuf) {
-  const int blk = blockIdx.x * 4 + (threadIdx.x >> 6);
-  if (blk >= nfar) return;
-  const FarBlk b = fb[blk];
-  const int g = (threadIdx.x >> 4) & 3, l16 = threadIdx.x & 15;
+  const int w = blockIdx.x * 4 + (threadIdx.x >> 6);
+  if (w >= nitems) return;
+  const RowItem it = items[w];
+  const FarBlk b = fb[it.blk];
+  const int lane = threadIdx.x & 63;
+  const int lanes = it.lanes;  // 64 or 16
+  const int g = lane / lanes, li = lane - g * lanes;
   const ST* F = TRANS ? fUt + b.uoff : fV + b.voff;
   const int len = TRANS ? b.m : b.n;
   const VT* xx = xp + (TRANS ? b.r0 : b.c0);
-  for (int l0 = 0; l0 < b.R; l0 += 4) {
-    const int l = l0 + g;
-    const bool act = l < b.R;
-    const ST* row = F + (int64_t)(act ? l : l0) * len;
-    VT a0 = zero_<VT>(), a1 = zero_<VT>();
-    int j = l16;
-    for (; j + 16 < len; j += 32) {
-      VT u, v;
-      cvt(u, row[j]);
-      cvt(v, row[j + 16]);
-      if (CONJ) { u = cconj(u); v = cconj(v); }
-      a0 += u * xx[j];
-      a1 += v * xx[j + 16];
-    }
-    if (j < len) {
-      VT u;
-      cvt(u, row[j]);
-      if (CONJ) u = cconj(u);
-      a0 += u * xx[j];
-    }
-    a0 += a1;
+  const bool act = g < it.nrows;
+  const int l = it.l0 + (act ? g : 0);
+  const ST* row = F + (int64_t)l * len;
+  VT a0 = zero_<VT>(), a1 = zero_<VT>(), a2 = zero_<VT>(), a3 = zero_<VT>();
+  int j = li;
+  const int st = lanes;
+  for (; j + 3 * st < len; j += 4 * st) {
+    VT u0, u1, u2, u3;
+    cvt(u0, row[j]);
+    cvt(u1, row[j + st]);
+    cvt(u2, row[j + 2 * st]);
+    cvt(u3, row[j + 3 * st]);
+    if (CONJ) { u0 = cconj(u0); u1 = cconj(u1); u2 = cconj(u2); u3 = cconj(u3); }
+    a0 += u0 * xx[j];
+    a1 += u1 * xx[j + st];
+    a2 += u2 * xx[j + 2 * st];
+    a3 += u3 * xx[j + 3 * st];
+  }
+  for (; j < len; j += st) {
+    VT u;
+    cvt(u, row[j]);
+    if (CONJ) u = cconj(u);
+    a0 += u * xx[j];
+  }
+  a0 = (a0 + a1) + (a2 + a3);
+  if (lanes == 64) {
+    a0 = wave_sum(a0);
+  } else {
 #pragma unroll
     for (int m = 8; m >= 1; m >>= 1) a0 += shfl_xor(a0, m);
-    if (l16 == 0 && act) tbuf[b.toff + l] = a0;
   }
+  if (li == 0 && act) tbuf[b.toff + l] = a0;
 }
 
 struct Tile {
@@ -1617,6 +1633,19 @@ void build_matvec_plan(Operator& op, hipStream_t s) {
   op.d_tile_trn.upload(tiles, s);
   op.d_seg_trn.upload(segs, s);
   op.n_tiles_trn = (int)tiles.size();
+  for (int trans = 0; trans < 2; trans++) {
+    std::vector<RowItem> items;
+    for (size_t k = 0; k < op.far.size(); k++) {
+      const FarBlk& fb = op.far[k];
+      const int len = trans ? fb.m : fb.n;
+      if (len >= 256)
+        for (int l = 0; l < fb.R; l++) items.push_back(RowItem{(int)k, l, 1, 64});
+      else
+        for (int l = 0; l < fb.R; l += 4) items.push_back(RowItem{(int)k, l, std::min(4, fb.R - l), 16});
+    }
+    (trans ? op.d_items_trn : op.d_items_fwd).upload(items, s);
+    (trans ? op.n_items_trn : op.n_items_fwd) = (int)items.size();
+  }
   const size_t esz = op.complex_valued ? 16 : 8;
   const int nmax = std::max(op.ndof_trial(), op.ndof_test());
   op.d_xp.alloc((size_t)nmax * esz);
@@ -1675,22 +1704,22 @@ static void apply_t(Operator& op, int flags, const XT* x, XT* y, XT alpha, XT be
   mark(1);
   if (!trans) {
     if (nfar)
-      hipLaunchKernelGGL((k_mv_far_t<ST, VT, false, false>), dim3((nfar + 3) / 4), dim3(256), 0, s,
-                         op.d_far_blk.as<FarBlk>(), nfar, fU, fV, xp, tb);
+      hipLaunchKernelGGL((k_mv_far_t<ST, VT, false, false>), dim3((op.n_items_fwd + 3) / 4), dim3(256), 0, s,
+                         op.d_items_fwd.as<RowItem>(), op.n_items_fwd, op.d_far_blk.as<FarBlk>(), fU, fV, xp, tb);
     mark(2);
     hipLaunchKernelGGL((k_mv_rows<ST, VT>), dim3(op.n_tiles_fwd), dim3(256), 0, s, op.d_tile_fwd.as<Tile>(),
                        op.d_seg_fwd.as<Seg>(), nval, fU, xp, tb, yp);
   } else if (!conj) {
     if (nfar)
-      hipLaunchKernelGGL((k_mv_far_t<ST, VT, true, false>), dim3((nfar + 3) / 4), dim3(256), 0, s,
-                         op.d_far_blk.as<FarBlk>(), nfar, fU, fV, xp, tb);
+      hipLaunchKernelGGL((k_mv_far_t<ST, VT, true, false>), dim3((op.n_items_trn + 3) / 4), dim3(256), 0, s,
+                         op.d_items_trn.as<RowItem>(), op.n_items_trn, op.d_far_blk.as<FarBlk>(), fU, fV, xp, tb);
     mark(2);
     hipLaunchKernelGGL((k_mv_cols<ST, VT, false>), dim3(op.n_tiles_trn), dim3(256), 0, s, op.d_tile_trn.as<Tile>(),
                        op.d_seg_trn.as<Seg>(), nval, fV, xp, tb, yp);
   } else {
     if (nfar)
-      hipLaunchKernelGGL((k_mv_far_t<ST, VT, true, true>), dim3((nfar + 3) / 4), dim3(256), 0, s,
-                         op.d_far_blk.as<FarBlk>(), nfar, fU, fV, xp, tb);
+      hipLaunchKernelGGL((k_mv_far_t<ST, VT, true, true>), dim3((op.n_items_trn + 3) / 4), dim3(256), 0, s,
+                         op.d_items_trn.as<RowItem>(), op.n_items_trn, op.d_far_blk.as<FarBlk>(), fU, fV, xp, tb);
     mark(2);
     hipLaunchKernelGGL((k_mv_cols<ST, VT, true>), dim3(op.n_tiles_trn), dim3(256), 0, s, op.d_tile_trn.as<Tile>(),
                        op.d_seg_trn.as<Seg>(), nval, fV, xp, tb, yp);

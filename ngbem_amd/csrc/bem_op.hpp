@@ -143,6 +143,8 @@ struct Operator {
   // matvec plans
   DBuf d_seg_fwd, d_tile_fwd, d_seg_trn, d_tile_trn;
   int n_tiles_fwd = 0, n_tiles_trn = 0;
+  DBuf d_items_fwd, d_items_trn;  // low-rank B^T x work items
+  int n_items_fwd = 0, n_items_trn = 0;
   int t_total = 0;         // scratch length for far B^T x
   DBuf d_xp, d_yp, d_t;    // scratch vectors (complex128 or double)
 
