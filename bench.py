@@ -67,6 +67,10 @@ def main():
     ap.add_argument("--kappa", type=float, default=10.0)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-solve", action="store_true")
+    ap.add_argument("--calib-copy", action="store_true",
+                    help="end with one plain 512 MiB device copy (calibrates rocprof FETCH_SIZE/WRITE_SIZE)")
+    ap.add_argument("--traffic", type=str, default=None,
+                    help="JSON file with measured HBM bytes per k_mv_rows launch (from the PMC passes)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only (asked for --gpus {args.gpus}); the multi-GPU sharding is a design "
@@ -230,6 +234,16 @@ def main():
     if not args.no_cpu_baseline:
         out["cpu_baseline"] = cpu_baseline(C, mesh, kappa, params, st)
 
+    if args.traffic and os.path.exists(args.traffic):
+        tr = json.load(open(args.traffic))
+        out["roofline"]["traffic"] = tr.get("bytes_per_launch")
+        out["roofline"]["traffic_note"] = tr.get("note")
+    if args.calib_copy:
+        a = torch.empty(512 << 20, dtype=torch.uint8, device=dev).fill_(1)
+        b = torch.empty_like(a)
+        torch.cuda.synchronize()
+        b.copy_(a)
+        torch.cuda.synchronize()
     print(json.dumps(out), flush=True)
 
 
