@@ -37,13 +37,18 @@ typedef struct bem_op bem_op;
 
 /* space kinds (src/python_bem.cpp callers pass NGSolve spaces of these types) */
 enum {
-  BEM_SPACE_P0 = 0, /* SurfaceL2, order 0: one dof per triangle, shape 1 */
-  BEM_SPACE_P1 = 1  /* H1, order 1: one dof per vertex, barycentric hats */
+  BEM_SPACE_P0 = 0,    /* SurfaceL2, order 0: one dof per triangle, shape 1 */
+  BEM_SPACE_P1 = 1,    /* H1, order 1: one dof per vertex, barycentric hats */
+  BEM_SPACE_HDIV = 2,  /* HDivSurface, order 0: one dof per edge, unit-flux RT0 */
+  BEM_SPACE_HCURL = 3  /* HCurl, order 0, on the surface: one dof per edge, Whitney */
 };
 
-/* kernel ids (src/ngbem.hpp:230-431); evaluators follow the factories:
+/* kernel ids (src/ngbem.hpp:230-510); evaluators follow the factories:
  * LAPLACE_HS uses n x grad (DiffOpBoundaryRot), HELMHOLTZ_HS uses
- * [n x grad, u] (DiffOpHelmholtz), all others the plain trace. */
+ * [n x grad, u] (DiffOpHelmholtz), MAXWELL_SL uses [phi, div phi]
+ * (DiffOpMaxwellNew on HDivSurface, DiffOpMaxwell on HCurl), MAXWELL_DL the
+ * rotated HCurl trace (trial) and the HDivSurface trace (test); all others
+ * the plain trace. */
 enum {
   BEM_LAPLACE_SL = 0,
   BEM_LAPLACE_DL = 1,
@@ -51,7 +56,9 @@ enum {
   BEM_HELMHOLTZ_SL = 3,
   BEM_HELMHOLTZ_DL = 4,
   BEM_HELMHOLTZ_CF = 5,
-  BEM_HELMHOLTZ_HS = 6
+  BEM_HELMHOLTZ_HS = 6,
+  BEM_MAXWELL_SL = 7,
+  BEM_MAXWELL_DL = 8
 };
 
 /* apply flags */

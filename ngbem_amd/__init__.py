@@ -8,7 +8,7 @@ far field) and apply run in hand-written HIP kernels for gfx950 behind the
 C-ABI in include/ngbem_amd.h.
 """
 from .mesh import Mesh, icosphere, ellipsoid
-from .spaces import SurfaceL2, H1
+from .spaces import SurfaceL2, H1, HDivSurface, HCurl
 from .operators import (BaseMatrix, HMatrix, IntegralOperator, IntegralOperatorC, MassMatrix,
                         SingleLayerPotentialOperator, DoubleLayerPotentialOperator, HypersingularOperator,
                         HelmholtzSingleLayerPotentialOperator, HelmholtzDoubleLayerPotentialOperator,
@@ -17,7 +17,7 @@ from .operators import (BaseMatrix, HMatrix, IntegralOperator, IntegralOperatorC
                         MaxwellDoubleLayerPotentialOperator)
 from .krylov import CG, GMRes, RefinedSolve
 
-__all__ = ["Mesh", "icosphere", "ellipsoid", "SurfaceL2", "H1", "BaseMatrix", "HMatrix", "IntegralOperator",
+__all__ = ["Mesh", "icosphere", "ellipsoid", "SurfaceL2", "H1", "HDivSurface", "HCurl", "BaseMatrix", "HMatrix", "IntegralOperator",
            "IntegralOperatorC", "MassMatrix", "SingleLayerPotentialOperator", "DoubleLayerPotentialOperator",
            "HypersingularOperator", "HelmholtzSingleLayerPotentialOperator", "HelmholtzDoubleLayerPotentialOperator",
            "HelmholtzCombinedFieldOperator", "HelmholtzHypersingularOperator", "MaxwellSingleLayerPotentialOperator",

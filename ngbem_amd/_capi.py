@@ -11,8 +11,9 @@ import os
 _LIB_PATH = os.path.join(os.path.dirname(os.path.abspath(__file__)), "_lib", "libngbem_amd.so")
 _lib = None
 
-SPACE_P0, SPACE_P1 = 0, 1
-LAPLACE_SL, LAPLACE_DL, LAPLACE_HS, HELMHOLTZ_SL, HELMHOLTZ_DL, HELMHOLTZ_CF, HELMHOLTZ_HS = range(7)
+SPACE_P0, SPACE_P1, SPACE_HDIV, SPACE_HCURL = 0, 1, 2, 3
+(LAPLACE_SL, LAPLACE_DL, LAPLACE_HS, HELMHOLTZ_SL, HELMHOLTZ_DL, HELMHOLTZ_CF, HELMHOLTZ_HS, MAXWELL_SL,
+ MAXWELL_DL) = range(9)
 APPLY_TRANS, APPLY_CONJ, APPLY_MIXED, APPLY_ON_DEVICE = 1, 2, 4, 8
 F64, C128 = 0, 1
 

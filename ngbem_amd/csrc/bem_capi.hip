@@ -107,11 +107,13 @@ int bem_mesh_destroy(bem_mesh* m) {
 
 int bem_space_create(bem_mesh* m, int kind, int order, bem_space** out) {
   if (!m || !out) return fail(-1, "bem_space_create: bad arguments");
-  if (!((kind == BEM_SPACE_P0 && order == 0) || (kind == BEM_SPACE_P1 && order == 1)))
-    return fail(-1, "bem_space_create: supported spaces are SurfaceL2 order 0 and H1 order 1");
+  if (!((kind == BEM_SPACE_P0 && order == 0) || (kind == BEM_SPACE_P1 && order == 1) ||
+        (kind == BEM_SPACE_HDIV && order == 0) || (kind == BEM_SPACE_HCURL && order == 0)))
+    return fail(-1, "bem_space_create: supported spaces are SurfaceL2 order 0, H1 order 1, HDivSurface order 0 "
+                    "and HCurl order 0");
   return guarded([&] {
     auto s = std::make_shared<bem::Space>();
-    s->setup(m->m.get(), kind == BEM_SPACE_P0 ? bem::SP_P0 : bem::SP_P1);
+    s->setup(m->m.get(), kind);
     *out = new bem_space{s, m->m};
   });
 }
@@ -135,13 +137,23 @@ int bem_op_create(int kernel, double kappa, bem_space* trial, bem_space* test, c
       (trial->s->kind != bem::SP_P1 || test->s->kind != bem::SP_P1))
     return fail(-1, "bem_op_create: hypersingular operators need H1 order 1 spaces");
   if (kernel >= bem::K_HSL && !(kappa > 0.0)) return fail(-1, "bem_op_create: kappa must be positive");
+  {
+    const int tk = trial->s->kind, sk = test->s->kind;
+    const bool scalar = tk <= bem::SP_P1 && sk <= bem::SP_P1;
+    if (kernel <= bem::K_HHS && !scalar)
+      return fail(-1, "bem_op_create: Laplace/Helmholtz kernels need SurfaceL2 / H1 spaces");
+    if (kernel == bem::K_MSL && !((tk == bem::SP_RWG && sk == bem::SP_RWG) || (tk == bem::SP_NED && sk == bem::SP_NED)))
+      return fail(-1, "bem_op_create: Maxwell single layer needs HDivSurface (or HCurl for the curl form) on both sides");
+    if (kernel == bem::K_MDL && !(tk == bem::SP_NED && sk == bem::SP_RWG))
+      return fail(-1, "bem_op_create: Maxwell double layer needs an HCurl trial and an HDivSurface test space");
+  }
   return guarded([&] {
     const double t0 = now();
     auto h = std::make_unique<bem_op>();
     bem::Operator& op = h->op;
     op.kernel = kernel;
     op.kappa = kappa;
-    op.kinfo = bem::KernelInfo::make(kernel, kappa);
+    op.kinfo = bem::KernelInfo::make(kernel, kappa, trial->s->kind, test->s->kind);
     op.complex_valued = op.kinfo.complex_valued;
     op.intorder = p->intorder;
     op.leafsize = p->dense ? INT_MAX : p->leafsize;

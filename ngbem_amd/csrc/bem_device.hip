@@ -122,7 +122,7 @@ struct KLapSL {
   typedef double T;
   static constexpr int NC = 1, NTERM = 1;
   double kappa;
-  __device__ static constexpr TermC term(int) { return {1.0, 0, 0, 0}; }
+  __device__ TermC term(int) const { return {1.0, 0, 0, 0}; }
   __device__ __forceinline__ void eval(P3 x, P3 y, P3, P3, T* o) const {
     double dx = x.x - y.x, dy = x.y - y.y, dz = x.z - y.z;
     double r = sqrt(dx * dx + dy * dy + dz * dz);
@@ -131,13 +131,13 @@ struct KLapSL {
 };
 struct KLapHS : KLapSL {  // src/ngbem.hpp:279-300: 1/(4 pi r), curl . curl terms
   static constexpr int NTERM = 3;
-  __device__ static constexpr TermC term(int t) { return {1.0, 0, t, t}; }
+  __device__ TermC term(int t) const { return {1.0, 0, t, t}; }
 };
 struct KLapDL {
   typedef double T;
   static constexpr int NC = 1, NTERM = 1;
   double kappa;
-  __device__ static constexpr TermC term(int) { return {1.0, 0, 0, 0}; }
+  __device__ TermC term(int) const { return {1.0, 0, 0, 0}; }
   __device__ __forceinline__ void eval(P3 x, P3 y, P3, P3 ny, T* o) const {
     double dx = x.x - y.x, dy = x.y - y.y, dz = x.z - y.z;
     double r = sqrt(dx * dx + dy * dy + dz * dz);
@@ -149,7 +149,7 @@ struct KHelSL {
   typedef zc T;
   static constexpr int NC = 1, NTERM = 1;
   double kappa;
-  __device__ static constexpr TermC term(int) { return {1.0, 0, 0, 0}; }
+  __device__ TermC term(int) const { return {1.0, 0, 0, 0}; }
   __device__ __forceinline__ void eval(P3 x, P3 y, P3, P3, T* o) const {
     double dx = x.x - y.x, dy = x.y - y.y, dz = x.z - y.z;
     double r = sqrt(dx * dx + dy * dy + dz * dz);
@@ -163,7 +163,7 @@ struct KHelDL {  // e^{ikr}/(4 pi r^3) <n_y, x-y> (1 - i k r)
   typedef zc T;
   static constexpr int NC = 1, NTERM = 1;
   double kappa;
-  __device__ static constexpr TermC term(int) { return {1.0, 0, 0, 0}; }
+  __device__ TermC term(int) const { return {1.0, 0, 0, 0}; }
   __device__ __forceinline__ void eval(P3 x, P3 y, P3, P3 ny, T* o) const {
     double dx = x.x - y.x, dy = x.y - y.y, dz = x.z - y.z;
     double r = sqrt(dx * dx + dy * dy + dz * dz);
@@ -178,7 +178,7 @@ struct KHelCF {  // e^{ikr}/(4 pi r^3) (<n_y, x-y>(1 - i k r) - i k r^2)
   typedef zc T;
   static constexpr int NC = 1, NTERM = 1;
   double kappa;
-  __device__ static constexpr TermC term(int) { return {1.0, 0, 0, 0}; }
+  __device__ TermC term(int) const { return {1.0, 0, 0, 0}; }
   __device__ __forceinline__ void eval(P3 x, P3 y, P3, P3 ny, T* o) const {
     double dx = x.x - y.x, dy = x.y - y.y, dz = x.z - y.z;
     double r2 = dx * dx + dy * dy + dz * dz;
@@ -195,7 +195,7 @@ struct KHelHS {  // {G, -k^2 G <n_x,n_y>}, src/ngbem.hpp:371-397
   typedef zc T;
   static constexpr int NC = 2, NTERM = 4;
   double kappa;
-  __device__ static constexpr TermC term(int t) { return t < 3 ? TermC{1.0, 0, t, t} : TermC{1.0, 1, 3, 3}; }
+  __device__ TermC term(int t) const { return t < 3 ? TermC{1.0, 0, t, t} : TermC{1.0, 1, 3, 3}; }
   __device__ __forceinline__ void eval(P3 x, P3 y, P3 nx, P3 ny, T* o) const {
     double dx = x.x - y.x, dy = x.y - y.y, dz = x.z - y.z;
     double r = sqrt(dx * dx + dy * dy + dz * dz);
@@ -208,57 +208,140 @@ struct KHelHS {  // {G, -k^2 G <n_x,n_y>}, src/ngbem.hpp:371-397
   }
 };
 
+struct KMaxSL {  // G; terms kappa phi_c phi_c (c < 3), -1/kappa div div (src/ngbem.hpp:442-468)
+  typedef zc T;
+  static constexpr int NC = 1, NTERM = 4;
+  double kappa;
+  __device__ TermC term(int t) const { return t < 3 ? TermC{kappa, 0, t, t} : TermC{-1.0 / kappa, 0, 3, 3}; }
+  __device__ __forceinline__ void eval(P3 x, P3 y, P3, P3, T* o) const {
+    double dx = x.x - y.x, dy = x.y - y.y, dz = x.z - y.z;
+    double r = sqrt(dx * dx + dy * dy + dz * dz);
+    double s, c;
+    sincos(kappa * r, &s, &c);
+    double g = kInv4Pi / r;
+    o[0] = {c * g, s * g};
+  }
+};
+struct KMaxDL {  // e^{ikr}(ikr - 1)(x-y)/(4 pi r^3), cross-product terms (src/ngbem.hpp:482-510)
+  typedef zc T;
+  static constexpr int NC = 3, NTERM = 6;
+  double kappa;
+  __device__ TermC term(int t) const {
+    switch (t) {
+      case 0: return {1.0, 0, 1, 2};
+      case 1: return {-1.0, 0, 2, 1};
+      case 2: return {1.0, 1, 2, 0};
+      case 3: return {-1.0, 1, 0, 2};
+      case 4: return {1.0, 2, 0, 1};
+      default: return {-1.0, 2, 1, 0};
+    }
+  }
+  __device__ __forceinline__ void eval(P3 x, P3 y, P3, P3, T* o) const {
+    double dx = x.x - y.x, dy = x.y - y.y, dz = x.z - y.z;
+    double r2 = dx * dx + dy * dy + dz * dz;
+    double r = sqrt(r2);
+    double kr = kappa * r, s, c;
+    sincos(kr, &s, &c);
+    double g = kInv4Pi / (r2 * r);
+    // e^{ikr} (i kr - 1) = (c + i s)(-1 + i kr) = (-c - kr s) + i (kr c - s)
+    const zc f = {g * (-c - kr * s), g * (kr * c - s)};
+    o[0] = {f.x * dx, f.y * dx};
+    o[1] = {f.x * dy, f.y * dy};
+    o[2] = {f.x * dz, f.y * dz};
+  }
+};
+
 // ------------------------------------------------------------- evaluators --
-// shape rows at one point, layout dof*DIM + comp (src/diffops.hpp:12-165)
+// shape rows at one point, layout dof*DIM + comp (src/diffops.hpp:12-338).
+// Per-element shape data (stride SDS doubles): [n x grad lambda_k (9),
+// edge scale s_k = sign_k / J (3), vertex opposite edge k (9)].
+// load(): copy what the evaluator needs into registers; PERM evaluators index
+// their dofs by Sauter-Schwab vertex (px), the others by element-local dof.
+// MULTI: more than one dof per element (scatter through the dof tables).
+constexpr int SDS = 21;
 
 template <int SP, int EV>
 struct Ev;
 template <>
 struct Ev<SP_P0, EV_ID> {
-  static constexpr int NL = 1, DIM = 1;
-  static constexpr bool P1 = false;
-  __device__ __forceinline__ static void shape(const double*, const double*, double* o) { o[0] = 1.0; }
+  static constexpr int NL = 1, DIM = 1, NB = 1;
+  static constexpr bool MULTI = false, PERM = false;
+  __device__ __forceinline__ static void load(const double*, const int*, double*) {}
+  __device__ __forceinline__ static void shape(const double*, P3, const double*, double* o) { o[0] = 1.0; }
 };
 template <>
 struct Ev<SP_P1, EV_ID> {
-  static constexpr int NL = 3, DIM = 1;
-  static constexpr bool P1 = true;
-  __device__ __forceinline__ static void shape(const double* lam, const double*, double* o) {
+  static constexpr int NL = 3, DIM = 1, NB = 1;
+  static constexpr bool MULTI = true, PERM = true;
+  __device__ __forceinline__ static void load(const double*, const int*, double*) {}
+  __device__ __forceinline__ static void shape(const double* lam, P3, const double*, double* o) {
     o[0] = lam[0]; o[1] = lam[1]; o[2] = lam[2];
   }
 };
 template <>
 struct Ev<SP_P1, EV_ROT> {
-  static constexpr int NL = 3, DIM = 3;
-  static constexpr bool P1 = true;
-  __device__ __forceinline__ static void shape(const double*, const double* curl, double* o) {
+  static constexpr int NL = 3, DIM = 3, NB = 9;
+  static constexpr bool MULTI = true, PERM = true;
+  __device__ __forceinline__ static void load(const double* sd, const int* px, double* b) {
+    for (int k = 0; k < 3; k++)
+      for (int c = 0; c < 3; c++) b[3 * k + c] = sd[3 * px[k] + c];
+  }
+  __device__ __forceinline__ static void shape(const double*, P3, const double* b, double* o) {
 #pragma unroll
-    for (int i = 0; i < 9; i++) o[i] = curl[i];
+    for (int i = 0; i < 9; i++) o[i] = b[i];
   }
 };
 template <>
 struct Ev<SP_P1, EV_HELM> {
-  static constexpr int NL = 3, DIM = 4;
-  static constexpr bool P1 = true;
-  __device__ __forceinline__ static void shape(const double* lam, const double* curl, double* o) {
+  static constexpr int NL = 3, DIM = 4, NB = 9;
+  static constexpr bool MULTI = true, PERM = true;
+  __device__ __forceinline__ static void load(const double* sd, const int* px, double* b) {
+    Ev<SP_P1, EV_ROT>::load(sd, px, b);
+  }
+  __device__ __forceinline__ static void shape(const double* lam, P3, const double* b, double* o) {
 #pragma unroll
     for (int k = 0; k < 3; k++) {
-      o[4 * k + 0] = curl[3 * k + 0];
-      o[4 * k + 1] = curl[3 * k + 1];
-      o[4 * k + 2] = curl[3 * k + 2];
+      o[4 * k + 0] = b[3 * k + 0];
+      o[4 * k + 1] = b[3 * k + 1];
+      o[4 * k + 2] = b[3 * k + 2];
       o[4 * k + 3] = lam[k];
     }
   }
 };
+// edge elements: phi_k(X) = s_k (X - p_opp(k)) (unit-flux RT0), div = 2 s_k.
+// HDivSurface: [phi, div] (DiffOpMaxwellNew) or phi (BND); HCurl: n x N_k =
+// -phi_k, so DiffOpMaxwell gives [-phi, div] and RotatedTrace gives -phi.
+template <int SGN, bool DIV>
+struct EvEdge {
+  static constexpr int NL = 3, DIM = DIV ? 4 : 3, NB = 12;
+  static constexpr bool MULTI = true, PERM = false;
+  __device__ __forceinline__ static void load(const double* sd, const int*, double* b) {
+    for (int i = 0; i < 12; i++) b[i] = sd[9 + i];
+  }
+  __device__ __forceinline__ static void shape(const double*, P3 X, const double* b, double* o) {
+#pragma unroll
+    for (int k = 0; k < 3; k++) {
+      const double sk = SGN * b[k];
+      o[DIM * k + 0] = sk * (X.x - b[3 + 3 * k]);
+      o[DIM * k + 1] = sk * (X.y - b[4 + 3 * k]);
+      o[DIM * k + 2] = sk * (X.z - b[5 + 3 * k]);
+      if (DIV) o[DIM * k + 3] = 2.0 * b[k];
+    }
+  }
+};
+template <> struct Ev<SP_RWG, EV_MAXW> : EvEdge<1, true> {};
+template <> struct Ev<SP_NED, EV_MAXW> : EvEdge<-1, true> {};
+template <> struct Ev<SP_RWG, EV_TRACE3> : EvEdge<1, false> {};
+template <> struct Ev<SP_NED, EV_TRACE3> : EvEdge<-1, false> {};
 
 // elmat[ii][jj] += sum_t fac_t * f * K_{kc_t} * phi_i[ii,tec_t] * phi_j[jj,trc_t]
 template <class KER, class EI, class EJ>
-__device__ __forceinline__ void accum(typename KER::T (&el)[EI::NL][EJ::NL], const typename KER::T* kv, double f,
-                                      const double* si, const double* sj) {
+__device__ __forceinline__ void accum(const KER& ker, typename KER::T (&el)[EI::NL][EJ::NL],
+                                      const typename KER::T* kv, double f, const double* si, const double* sj) {
   typedef typename KER::T T;
 #pragma unroll
   for (int t = 0; t < KER::NTERM; t++) {
-    const TermC tm = KER::term(t);
+    const TermC tm = ker.term(t);
     T k = rmul(kv[tm.kc], tm.fac * f);
 #pragma unroll
     for (int ii = 0; ii < EI::NL; ii++) {
@@ -279,8 +362,9 @@ template <class KER, class EI, class EJ, bool DIRECT>
 __global__ __launch_bounds__(256) void k_near_regular(const NearBlk* __restrict__ blks, const int2* __restrict__ chunks,
                                                       const int* __restrict__ pIb, const int* __restrict__ pJb,
                                                       const double* __restrict__ geo, const int4* __restrict__ vid,
-                                                      const double* __restrict__ quad, const double* __restrict__ curl,
+                                                      const double* __restrict__ quad, const double* __restrict__ sdat,
                                                       const double* __restrict__ trig, int ntq,
+                                                      const int* __restrict__ dofs_t, const int* __restrict__ dofs_r,
                                                       const int* __restrict__ ipt, const int* __restrict__ ipr, KER ker,
                                                       typename KER::T* __restrict__ val) {
   typedef typename KER::T T;
@@ -295,11 +379,10 @@ __global__ __launch_bounds__(256) void k_near_regular(const NearBlk* __restrict_
                      ve.z == vf.x || ve.z == vf.y || ve.z == vf.z;
   if (touch) return;  // singular pair: written by k_near_singular
   const P3 ne = ld3(geo + (size_t)e * GEO + 9), nf = ld3(geo + (size_t)f * GEO + 9);
-  double ce[9], cf[9];
-  if (EI::DIM > 1)
-    for (int i = 0; i < 9; i++) ce[i] = curl[(size_t)e * 9 + i];
-  if (EJ::DIM > 1)
-    for (int i = 0; i < 9; i++) cf[i] = curl[(size_t)f * 9 + i];
+  const int idp[3] = {0, 1, 2};
+  double be[EI::NB], bf[EJ::NB];
+  EI::load(sdat + (size_t)e * SDS, idp, be);
+  EJ::load(sdat + (size_t)f * SDS, idp, bf);
   T el[EI::NL][EJ::NL];
 #pragma unroll
   for (int a = 0; a < EI::NL; a++)
@@ -312,29 +395,28 @@ __global__ __launch_bounds__(256) void k_near_regular(const NearBlk* __restrict_
     const double wx = qe[4 * ix + 3];
     double lx[3] = {trig[3 * ix], trig[3 * ix + 1], 1.0 - trig[3 * ix] - trig[3 * ix + 1]};
     double si[EI::NL * EI::DIM];
-    EI::shape(lx, ce, si);
+    EI::shape(lx, x, be, si);
     for (int iy = 0; iy < ntq; iy++) {
       const P3 y = ld3(qf + 4 * iy);
       const double wy = qf[4 * iy + 3];
       double ly[3] = {trig[3 * iy], trig[3 * iy + 1], 1.0 - trig[3 * iy] - trig[3 * iy + 1]};
       double sj[EJ::NL * EJ::DIM];
-      EJ::shape(ly, cf, sj);
+      EJ::shape(ly, y, bf, sj);
       T kv[KER::NC];
       ker.eval(x, y, ne, nf, kv);
-      accum<KER, EI, EJ>(el, kv, wx * wy, si, sj);
+      accum<KER, EI, EJ>(ker, el, kv, wx * wy, si, sj);
     }
   }
   if (DIRECT) {
     val[nb.off + (int64_t)ja * nb.m + ia] = el[0][0];
   } else {
-    const int vi[3] = {ve.x, ve.y, ve.z}, vj[3] = {vf.x, vf.y, vf.z};
 #pragma unroll
     for (int ii = 0; ii < EI::NL; ii++) {
-      const int row = EI::P1 ? ipt[vi[ii]] - nb.r0 : ia;
+      const int row = EI::MULTI ? ipt[dofs_t[4 * e + ii]] - nb.r0 : ia;
       if (row < 0 || row >= nb.m) continue;
 #pragma unroll
       for (int jj = 0; jj < EJ::NL; jj++) {
-        const int col = EJ::P1 ? ipr[vj[jj]] - nb.c0 : ja;
+        const int col = EJ::MULTI ? ipr[dofs_r[4 * f + jj]] - nb.c0 : ja;
         if (col < 0 || col >= nb.n) continue;
         atomic_add(&val[nb.off + (int64_t)col * nb.m + row], el[ii][jj]);
       }
@@ -351,8 +433,9 @@ template <class KER, class EI, class EJ, bool DIRECT>
 __global__ __launch_bounds__(256) void k_near_singular(const int4* __restrict__ pairs, int npairs,
                                                        const NearBlk* __restrict__ blks, const double* __restrict__ rule,
                                                        int Q, const double* __restrict__ geo,
-                                                       const int* __restrict__ vidf, const double* __restrict__ curl,
-                                                       const int* __restrict__ ipt, const int* __restrict__ ipr, KER ker,
+                                                       const double* __restrict__ sdat, const int* __restrict__ dofs_t,
+                                                       const int* __restrict__ dofs_r, const int* __restrict__ ipt,
+                                                       const int* __restrict__ ipr, KER ker,
                                                        typename KER::T* __restrict__ val) {
   typedef typename KER::T T;
   const int p = blockIdx.x * 256 + threadIdx.x;
@@ -372,14 +455,9 @@ __global__ __launch_bounds__(256) void k_near_singular(const int4* __restrict__ 
   const P3 F0 = ld3(gf + 3 * py[0]), F1 = ld3(gf + 3 * py[1]), F2 = ld3(gf + 3 * py[2]);
   const P3 ne = ld3(ge + 9), nf = ld3(gf + 9);
   const double JJ = ge[12] * gf[12];
-  // curls in Sauter-Schwab vertex order
-  double ce[9], cf[9];
-  if (EI::DIM > 1)
-    for (int k = 0; k < 3; k++)
-      for (int c = 0; c < 3; c++) ce[3 * k + c] = curl[(size_t)e * 9 + 3 * px[k] + c];
-  if (EJ::DIM > 1)
-    for (int k = 0; k < 3; k++)
-      for (int c = 0; c < 3; c++) cf[3 * k + c] = curl[(size_t)f * 9 + 3 * py[k] + c];
+  double be[EI::NB], bf[EJ::NB];  // shape data, in Sauter-Schwab vertex order for PERM evaluators
+  EI::load(sdat + (size_t)e * SDS, px, be);
+  EJ::load(sdat + (size_t)f * SDS, py, bf);
   T el[EI::NL][EJ::NL];
 #pragma unroll
   for (int a = 0; a < EI::NL; a++)
@@ -395,11 +473,11 @@ __global__ __launch_bounds__(256) void k_near_singular(const int4* __restrict__ 
     const P3 Y = {ly[0] * F0.x + ly[1] * F1.x + ly[2] * F2.x, ly[0] * F0.y + ly[1] * F1.y + ly[2] * F2.y,
                   ly[0] * F0.z + ly[1] * F1.z + ly[2] * F2.z};
     double si[EI::NL * EI::DIM], sj[EJ::NL * EJ::DIM];
-    EI::shape(lx, ce, si);
-    EJ::shape(ly, cf, sj);
+    EI::shape(lx, X, be, si);
+    EJ::shape(ly, Y, bf, sj);
     T kv[KER::NC];
     ker.eval(X, Y, ne, nf, kv);
-    accum<KER, EI, EJ>(el, kv, JJ * w, si, sj);
+    accum<KER, EI, EJ>(ker, el, kv, JJ * w, si, sj);
   }
   if (DIRECT) {
     const int row = ipt[e] - nb.r0, col = ipr[f] - nb.c0;
@@ -407,11 +485,13 @@ __global__ __launch_bounds__(256) void k_near_singular(const int4* __restrict__ 
   } else {
 #pragma unroll
     for (int kx = 0; kx < EI::NL; kx++) {
-      const int row = EI::P1 ? ipt[vidf[4 * e + px[kx]]] - nb.r0 : ipt[e] - nb.r0;
+      const int dx = EI::PERM ? px[kx] : kx;
+      const int row = EI::MULTI ? ipt[dofs_t[4 * e + dx]] - nb.r0 : ipt[e] - nb.r0;
       if (row < 0 || row >= nb.m) continue;
 #pragma unroll
       for (int ky = 0; ky < EJ::NL; ky++) {
-        const int col = EJ::P1 ? ipr[vidf[4 * f + py[ky]]] - nb.c0 : ipr[f] - nb.c0;
+        const int dy = EJ::PERM ? py[ky] : ky;
+        const int col = EJ::MULTI ? ipr[dofs_r[4 * f + dy]] - nb.c0 : ipr[f] - nb.c0;
         if (col < 0 || col >= nb.n) continue;
         atomic_add(&val[nb.off + (int64_t)col * nb.m + row], el[kx][ky]);
       }
@@ -741,8 +821,8 @@ template <class KER, class EI, class EJ>
 __global__ __launch_bounds__(256) void k_project(const ProjTask* __restrict__ tasks, const int2* __restrict__ items,
                                                  const FarBlk* __restrict__ fb, const int* __restrict__ fpatch,
                                                  const int* __restrict__ maps, const double* __restrict__ quad,
-                                                 const double* __restrict__ curl, const double* __restrict__ trig,
-                                                 int ntq, const typename KER::T* __restrict__ wsU,
+                                                 const double* __restrict__ sdat, const double* __restrict__ trig,
+                                                 int ntq, KER ker, const typename KER::T* __restrict__ wsU,
                                                  const typename KER::T* __restrict__ wsV,
                                                  typename KER::T* __restrict__ oU, typename KER::T* __restrict__ oV) {
   typedef typename KER::T T;
@@ -753,10 +833,11 @@ __global__ __launch_bounds__(256) void k_project(const ProjTask* __restrict__ ta
 #pragma unroll
   for (int t = 0; t < KER::NTERM; t++) {
     r0[t] = R;
-    R += tk.rank[KER::term(t).kc];
+    R += tk.rank[ker.term(t).kc];
   }
   const int nU = b.m * R, total = nU + R * b.n;
   const int end = min(total, it.y + PROJ_CHUNK);
+  const int idp[3] = {0, 1, 2};
   for (int idx = it.y + (int)threadIdx.x; idx < end; idx += 256) {
     const bool isU = idx < nU;
     const int len = isU ? b.m : b.n;
@@ -766,7 +847,7 @@ __global__ __launch_bounds__(256) void k_project(const ProjTask* __restrict__ ta
 #pragma unroll
     for (int q = 1; q < KER::NTERM; q++)
       if (col >= r0[q]) t = q;
-    const TermC tm = KER::term(t);
+    const TermC tm = ker.term(t);
     const int l = col - r0[t];
     const T* W = isU ? wsU + tk.uoff[tm.kc] + (size_t)l * tk.np : wsV + tk.voff[tm.kc] + (size_t)l * tk.nq;
     const int* mp = maps + (isU ? b.rmap : b.cmap);
@@ -776,23 +857,24 @@ __global__ __launch_bounds__(256) void k_project(const ProjTask* __restrict__ ta
     for (int k2 = mp[pos]; k2 < mp[pos + 1]; k2++) {
       const int ent = mp[len + 1 + k2];
       const int a = ent >> 2, kl = ent & 3, e = P[a];
-      double c9[9];
-      if ((isU ? EI::DIM : EJ::DIM) > 1)
-        for (int c = 0; c < 9; c++) c9[c] = curl[(size_t)e * 9 + c];
+      double bb[EI::NB > EJ::NB ? EI::NB : EJ::NB];
+      if (isU) EI::load(sdat + (size_t)e * SDS, idp, bb);
+      else EJ::load(sdat + (size_t)e * SDS, idp, bb);
       for (int q = 0; q < ntq; q++) {
         const double lam[3] = {trig[3 * q], trig[3 * q + 1], 1.0 - trig[3 * q] - trig[3 * q + 1]};
+        const double* qp = quad + ((size_t)e * ntq + q) * 4;
+        const P3 X = ld3(qp);
         double sh;
         if (isU) {
           double v[EI::NL * EI::DIM];
-          EI::shape(lam, c9, v);
+          EI::shape(lam, X, bb, v);
           sh = v[kl * EI::DIM + comp];
         } else {
           double v[EJ::NL * EJ::DIM];
-          EJ::shape(lam, c9, v);
+          EJ::shape(lam, X, bb, v);
           sh = v[kl * EJ::DIM + comp];
         }
-        const double w = quad[((size_t)e * ntq + q) * 4 + 3];
-        s += rmul(W[a * ntq + q], sh * w);
+        s += rmul(W[a * ntq + q], sh * qp[3]);
       }
     }
     if (isU) oU[tk.ou + (size_t)col * b.m + pos] = rmul(s, tm.fac);
@@ -1040,6 +1122,8 @@ static void dispatch_kernel(int id, double kappa, F&& f) {
     case K_HDL: f(KHelDL{kappa}); break;
     case K_HCF: f(KHelCF{kappa}); break;
     case K_HHS: f(KHelHS{kappa}); break;
+    case K_MSL: f(KMaxSL{kappa}); break;
+    case K_MDL: f(KMaxDL{kappa}); break;
     default: throw std::runtime_error("unknown kernel id");
   }
 }
@@ -1054,6 +1138,11 @@ static void dispatch_full(const Operator& op, F&& f) {
       f(ker, Ev<SP_P1, EV_ROT>{}, Ev<SP_P1, EV_ROT>{});
     } else if constexpr (std::is_same<KER, KHelHS>::value) {
       f(ker, Ev<SP_P1, EV_HELM>{}, Ev<SP_P1, EV_HELM>{});
+    } else if constexpr (std::is_same<KER, KMaxSL>::value) {
+      if (st == SP_RWG) f(ker, Ev<SP_RWG, EV_MAXW>{}, Ev<SP_RWG, EV_MAXW>{});
+      else f(ker, Ev<SP_NED, EV_MAXW>{}, Ev<SP_NED, EV_MAXW>{});
+    } else if constexpr (std::is_same<KER, KMaxDL>::value) {
+      f(ker, Ev<SP_RWG, EV_TRACE3>{}, Ev<SP_NED, EV_TRACE3>{});  // test: HDivSurface trace, trial: rotated HCurl trace
     } else {
       if (st == SP_P0 && sr == SP_P0) f(ker, Ev<SP_P0, EV_ID>{}, Ev<SP_P0, EV_ID>{});
       else if (st == SP_P0 && sr == SP_P1) f(ker, Ev<SP_P0, EV_ID>{}, Ev<SP_P1, EV_ID>{});
@@ -1068,7 +1157,7 @@ static void dispatch_full(const Operator& op, F&& f) {
 void setup_device(Operator& op, hipStream_t s) {
   const Mesh& m = *op.mesh;
   const int nt = m.nt, ntq = op.trig.size();
-  std::vector<double> geo((size_t)nt * GEO, 0.0), quad((size_t)nt * ntq * 4), curl((size_t)nt * 9);
+  std::vector<double> geo((size_t)nt * GEO, 0.0), quad((size_t)nt * ntq * 4), sdat((size_t)nt * SDS, 0.0);
   std::vector<int> vid((size_t)nt * 4, 0);
   for (int e = 0; e < nt; e++) {
     for (int k = 0; k < 3; k++) {
@@ -1087,9 +1176,25 @@ void setup_device(Operator& op, hipStream_t s) {
     }
     double c[3][3];
     m.curls(e, c);
+    double* sd = &sdat[(size_t)e * SDS];
     for (int k = 0; k < 3; k++)
-      for (int d = 0; d < 3; d++) curl[(size_t)e * 9 + 3 * k + d] = c[k][d];
+      for (int d = 0; d < 3; d++) sd[3 * k + d] = c[k][d];
+    // edge spaces: s_k = sign_k / J and the vertex opposite edge k (= vertex k+2)
+    const Space& es = (op.test->kind >= SP_RWG) ? *op.test : *op.trial;
+    for (int k = 0; k < 3; k++) {
+      sd[9 + k] = es.sign.empty() ? 0.0 : es.sign[3 * e + k] / m.J[e];
+      const int vo = m.tri[3 * e + (k + 2) % 3];
+      for (int d = 0; d < 3; d++) sd[12 + 3 * k + d] = m.xyz[3 * vo + d];
+    }
   }
+  auto dof_table = [&](const Space& sp) {
+    std::vector<int> t((size_t)nt * 4, 0);
+    for (int e = 0; e < nt; e++)
+      for (int l = 0; l < sp.nloc; l++) t[4 * (size_t)e + l] = sp.dofs[(size_t)e * sp.nloc + l];
+    return t;
+  };
+  op.d_dofs_test.upload(dof_table(*op.test), s);
+  op.d_dofs_trial.upload(dof_table(*op.trial), s);
   std::vector<double> trig((size_t)ntq * 3);
   for (int q = 0; q < ntq; q++) {
     trig[3 * q] = op.trig.xi[q];
@@ -1098,7 +1203,7 @@ void setup_device(Operator& op, hipStream_t s) {
   }
   op.d_geo.upload(geo, s);
   op.d_vid.upload(vid, s);
-  op.d_curl.upload(curl, s);
+  op.d_sdat.upload(sdat, s);
   op.d_quad.upload(quad, s);
   op.d_trig.upload(trig, s);
   for (int k = 0; k < 3; k++) {
@@ -1135,7 +1240,7 @@ void assemble_near(Operator& op, hipStream_t s) {
   const Space& tr = *op.trial;
   const Tree& tt = *op.test_tree;
   const Tree& rt = op.trial_tree;
-  const bool p1i = te.kind == SP_P1, p1j = tr.kind == SP_P1;
+  const bool p1i = te.kind != SP_P0, p1j = tr.kind != SP_P0;  // multi-dof elements use cluster patches
   const bool direct = !p1i && !p1j;
   PhaseTimer pt_("near");
   // near tables; P1 sides use the per-cluster patches, P0 sides the permutation
@@ -1237,20 +1342,21 @@ void assemble_near(Operator& op, hipStream_t s) {
       if (!chunks.empty())
         hipLaunchKernelGGL((k_near_regular<KER, EI, EJ, D>), dim3((unsigned)chunks.size()), dim3(256), 0, s,
                            op.d_near_blk.as<NearBlk>(), op.d_near_chunks.as<int2>(), pIb, pJb, op.d_geo.as<double>(),
-                           op.d_vid.as<int4>(), op.d_quad.as<double>(), op.d_curl.as<double>(),
-                           op.d_trig.as<double>(), ntq, op.d_iperm_test.as<int>(), op.d_iperm_trial.as<int>(), ker,
-                           val);
+                           op.d_vid.as<int4>(), op.d_quad.as<double>(), op.d_sdat.as<double>(),
+                           op.d_trig.as<double>(), ntq, op.d_dofs_test.as<int>(), op.d_dofs_trial.as<int>(),
+                           op.d_iperm_test.as<int>(), op.d_iperm_trial.as<int>(), ker, val);
       for (int c = 0; c < 3; c++) {
         const int np = (int)pairs[c].size();
         if (!np) continue;
         hipLaunchKernelGGL((k_near_singular<KER, EI, EJ, D>), dim3((np + 255) / 256), dim3(256), 0, s,
                            op.d_spairs[c].as<int4>(), np, op.d_near_blk.as<NearBlk>(), op.d_ss[c].as<double>(),
-                           op.ss[c].size(), op.d_geo.as<double>(), op.d_vid.as<int>(), op.d_curl.as<double>(),
-                           op.d_iperm_test.as<int>(), op.d_iperm_trial.as<int>(), ker, val);
+                           op.ss[c].size(), op.d_geo.as<double>(), op.d_sdat.as<double>(),
+                           op.d_dofs_test.as<int>(), op.d_dofs_trial.as<int>(), op.d_iperm_test.as<int>(),
+                           op.d_iperm_trial.as<int>(), ker, val);
       }
     };
     pt_.mark("uploads + alloc");
-    if constexpr (!EI::P1 && !EJ::P1) launch(std::true_type{});
+    if constexpr (!EI::MULTI && !EJ::MULTI) launch(std::true_type{});
     else launch(std::false_type{});
   });
   BEM_HIP(hipGetLastError());
@@ -1525,7 +1631,7 @@ void assemble_far(Operator& op, hipStream_t s) {
           hipLaunchKernelGGL((k_project<KER, decltype(ei), decltype(ej)>), dim3((unsigned)items.size()), dim3(256),
                              0, s, d_ptasks.as<ProjTask>(), d_items.as<int2>(), op.d_far_blk.as<FarBlk>(),
                              op.d_patch.as<int>(), op.d_far_maps.as<int>(), op.d_quad.as<double>(),
-                             op.d_curl.as<double>(), op.d_trig.as<double>(), ntq, SU.as<T>(), SV.as<T>(),
+                             op.d_sdat.as<double>(), op.d_trig.as<double>(), ntq, ker, SU.as<T>(), SV.as<T>(),
                              piece->U.as<T>(), piece->V.as<T>());
         });
       BEM_HIP(hipGetLastError());

@@ -82,10 +82,38 @@ void Space::setup(const Mesh* m, int k) {
     nloc = 1;
     dofs.resize(m->nt);
     for (int e = 0; e < m->nt; e++) dofs[e] = e;
-  } else {
+  } else if (k == SP_P1) {
     ndof = m->nv;
     nloc = 3;
     dofs.assign(m->tri.begin(), m->tri.end());
+  } else {
+    // edges sorted by (min vertex, max vertex); local dof k of a triangle is
+    // its edge (v_k, v_{k+1}), signed +1 when v_k < v_{k+1}
+    nloc = 3;
+    std::vector<long long> keys;
+    keys.reserve(3 * (size_t)m->nt);
+    auto key = [m](int e, int l) {
+      const int a = m->tri[3 * e + l], b = m->tri[3 * e + (l + 1) % 3];
+      return (long long)std::min(a, b) * m->nv + std::max(a, b);
+    };
+    for (int e = 0; e < m->nt; e++)
+      for (int l = 0; l < 3; l++) keys.push_back(key(e, l));
+    std::vector<long long> uk(keys);
+    std::sort(uk.begin(), uk.end());
+    uk.erase(std::unique(uk.begin(), uk.end()), uk.end());
+    ndof = (int)uk.size();
+    edges.resize(2 * (size_t)ndof);
+    for (int i = 0; i < ndof; i++) {
+      edges[2 * i] = (int)(uk[i] / m->nv);
+      edges[2 * i + 1] = (int)(uk[i] % m->nv);
+    }
+    dofs.resize(3 * (size_t)m->nt);
+    sign.resize(3 * (size_t)m->nt);
+    for (int e = 0; e < m->nt; e++)
+      for (int l = 0; l < 3; l++) {
+        dofs[3 * e + l] = (int)(std::lower_bound(uk.begin(), uk.end(), key(e, l)) - uk.begin());
+        sign[3 * e + l] = m->tri[3 * e + l] < m->tri[3 * e + (l + 1) % 3] ? 1 : -1;
+      }
   }
   std::vector<int> cnt(ndof + 1, 0);
   for (int e = 0; e < m->nt; e++)
@@ -330,12 +358,16 @@ void Tree::build(const Space& s, int leafsize) {
   for (int e = 0; e < m.nt; e++) {
     double mip[7][3];
     for (int j = 0; j < 7; j++) m.point(e, xh[j][0], xh[j][1], mip[j]);
-    int vc = 0;
+    int vc = 0, ec = 0;
     for (int l = 0; l < s.nloc; l++) {
       int d = s.dofs[e * s.nloc + l];
       if (cnt[d] > 2) {
         for (int k = 0; k < 3; k++) X[3 * d + k] = mip[vc][k];
         vc++;
+      }
+      if (cnt[d] == 2) {  // edge dof: midpoint of the element's ec-th edge (src/hmat.cpp:120-140)
+        for (int k = 0; k < 3; k++) X[3 * d + k] = mip[3 + ec][k];
+        ec++;
       }
       if (cnt[d] == 1)
         for (int k = 0; k < 3; k++) X[3 * d + k] = mip[6][k];
@@ -447,10 +479,11 @@ void partition(const Tree& rt, const Tree& ct, double eta, std::vector<Block>& o
 
 // --------------------------------------------------------------- kernels --
 
-int eval_dim(int ev) { return ev == EV_ID ? 1 : (ev == EV_ROT ? 3 : 4); }
+int eval_dim(int ev) { return ev == EV_ID ? 1 : (ev == EV_ROT || ev == EV_TRACE3 ? 3 : 4); }
 
-KernelInfo KernelInfo::make(int id, double kappa) {
-  (void)kappa;
+KernelInfo KernelInfo::make(int id, double kappa, int trial_kind, int test_kind) {
+  (void)trial_kind;
+  (void)test_kind;
   KernelInfo k;
   k.id = id;
   k.complex_valued = id >= K_HSL;
@@ -464,6 +497,13 @@ KernelInfo KernelInfo::make(int id, double kappa) {
     k.ncomp = 2;
     k.terms = {{1.0, 0, 0, 0}, {1.0, 0, 1, 1}, {1.0, 0, 2, 2}, {1.0, 1, 3, 3}};
     k.ev_test = k.ev_trial = EV_HELM;
+  } else if (id == K_MSL) {  // src/ngbem.hpp:442-468: kappa phi.phi - 1/kappa div.div
+    k.terms = {{kappa, 0, 0, 0}, {kappa, 0, 1, 1}, {kappa, 0, 2, 2}, {-1.0 / kappa, 0, 3, 3}};
+    k.ev_test = k.ev_trial = EV_MAXW;
+  } else if (id == K_MDL) {  // src/ngbem.hpp:482-510: (factor, kernel comp, trial comp, test comp)
+    k.ncomp = 3;
+    k.terms = {{1.0, 0, 1, 2}, {-1.0, 0, 2, 1}, {1.0, 1, 2, 0}, {-1.0, 1, 0, 2}, {1.0, 2, 0, 1}, {-1.0, 2, 1, 0}};
+    k.ev_test = k.ev_trial = EV_TRACE3;
   }
   return k;
 }

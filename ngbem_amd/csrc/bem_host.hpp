@@ -12,9 +12,13 @@ namespace bem {
 
 static const double kPi = 3.14159265358979323846;
 
-enum SpaceKind { SP_P0 = 0, SP_P1 = 1 };
-enum EvalKind { EV_ID = 0, EV_ROT = 1, EV_HELM = 2 };
-enum KernelId { K_LSL = 0, K_LDL, K_LHS, K_HSL, K_HDL, K_HCF, K_HHS, K_COUNT };
+// SP_RWG: HDivSurface order 0 (unit-flux Raviart-Thomas on edges);
+// SP_NED: HCurl order 0 on the surface (Whitney/Nedelec edge functions)
+enum SpaceKind { SP_P0 = 0, SP_P1 = 1, SP_RWG = 2, SP_NED = 3 };
+// EV_MAXW: [phi, div phi] (DiffOpMaxwellNew on HDivSurface, DiffOpMaxwell on
+// HCurl); EV_TRACE3: vector trace (HDivSurface BND, RotatedTrace on HCurl)
+enum EvalKind { EV_ID = 0, EV_ROT = 1, EV_HELM = 2, EV_MAXW = 3, EV_TRACE3 = 4 };
+enum KernelId { K_LSL = 0, K_LDL, K_LHS, K_HSL, K_HDL, K_HCF, K_HHS, K_MSL, K_MDL, K_COUNT };
 
 struct Mesh {
   int nv = 0, nt = 0;
@@ -31,6 +35,8 @@ struct Space {
   const Mesh* mesh = nullptr;
   int kind = SP_P0, ndof = 0, nloc = 1;
   std::vector<int> dofs;                   // nt*nloc
+  std::vector<int> sign;                   // nt*3 edge orientation signs (RWG/NED)
+  std::vector<int> edges;                  // 2*nedges (RWG/NED): sorted vertex pairs
   std::vector<int> ed_off, ed_list;        // dof -> elements (CSR, ascending)
   void setup(const Mesh* m, int k);
 };
@@ -72,7 +78,8 @@ struct KernelInfo {
   int ev_test, ev_trial;
   struct Term { double fac; int kc, trc, tec; };
   std::vector<Term> terms;
-  static KernelInfo make(int id, double kappa);
+  // trial/test space kinds decide the evaluators of the Maxwell kernels
+  static KernelInfo make(int id, double kappa, int trial_kind = SP_P0, int test_kind = SP_P0);
 };
 int eval_dim(int ev);
 

@@ -134,7 +134,8 @@ struct Operator {
   Rule4 ss[3];
 
   // device data
-  DBuf d_geo, d_vid, d_curl, d_quad, d_trig, d_ss[3];
+  DBuf d_geo, d_vid, d_sdat, d_quad, d_trig, d_ss[3];
+  DBuf d_dofs_test, d_dofs_trial;  // int4 per element: the element's dofs
   DBuf d_perm_test, d_iperm_test, d_perm_trial, d_iperm_trial;
   DBuf d_patch;            // near patches of P1 sides (test then trial)
   DBuf d_near_blk, d_near_chunks, d_near_val, d_near_val32;

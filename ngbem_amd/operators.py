@@ -18,7 +18,7 @@ import numpy as np
 import torch
 
 from . import _capi
-from .spaces import FESpace, H1, SurfaceL2, mass_matrix_coo
+from .spaces import FESpace, H1, HCurl, HDivSurface, SurfaceL2, mass_matrix_coo
 
 __all__ = ["BaseMatrix", "HMatrix", "IntegralOperator", "IntegralOperatorC", "MassMatrix",
            "SingleLayerPotentialOperator", "DoubleLayerPotentialOperator", "HypersingularOperator",
@@ -298,7 +298,7 @@ class IntegralOperator:
     def __init__(self, kernel, kappa, trial_space, test_space, intorder, leafsize, eta, eps, dense=False,
                  mixed=False, testhmatrix=False):
         if not isinstance(trial_space, FESpace) or not isinstance(test_space, FESpace):
-            raise TypeError("spaces must be ngbem_amd SurfaceL2 / H1 spaces")
+            raise TypeError("spaces must be ngbem_amd SurfaceL2 / H1 / HDivSurface / HCurl spaces")
         self.trial_space, self.test_space = trial_space, test_space
         self.kernel, self.kappa = kernel, float(kappa)
         self.params = dict(intorder=intorder, leafsize=leafsize, eta=eta, eps=eps, dense=dense)
@@ -509,23 +509,34 @@ def HelmholtzHypersingularOperator(trial_space, test_space=None, kappa=None, int
                       testhmatrix, mixed)
 
 
-def _maxwell(*a, **k):
-    raise NotImplementedError("Maxwell operators need HDivSurface spaces, which this library does not provide yet")
-
-
 def MaxwellSingleLayerPotentialOperator(space, kappa, definedon=None, intorder=3, leafsize=40, eta=2., eps=1e-6,
-                                        method="aca", testhmatrix=False):
-    """Not available: needs HDivSurface (src/python_bem.cpp:122-134)."""
-    _maxwell()
+                                        method="aca", testhmatrix=False, mixed=False):
+    """Maxwell single layer on HDivSurface: kappa <G phi, phi> - 1/kappa <G div, div>
+    (src/python_bem.cpp:122-134, src/ngbem.hpp:442-468, DiffOpMaxwellNew)."""
+    _no_region(definedon)
+    if not isinstance(space, HDivSurface):
+        raise TypeError("MaxwellSingleLayerPotentialOperator needs an HDivSurface space")
+    return _helmholtz(_capi.MAXWELL_SL, space, space, kappa, intorder, leafsize, eta, eps, method, testhmatrix, mixed)
 
 
 def MaxwellSingleLayerPotentialOperatorCurl(space, kappa, definedon=None, intorder=3, leafsize=40, eta=2., eps=1e-6,
-                                            method="aca", testhmatrix=False):
-    """Not available: needs HCurl surface spaces (src/python_bem.cpp:137-149)."""
-    _maxwell()
+                                            method="aca", testhmatrix=False, mixed=False):
+    """Maxwell single layer on HCurl with the rotated trace and the surface curl
+    (src/python_bem.cpp:137-149, DiffOpMaxwell)."""
+    _no_region(definedon)
+    if not isinstance(space, HCurl):
+        raise TypeError("MaxwellSingleLayerPotentialOperatorCurl needs an HCurl space")
+    return _helmholtz(_capi.MAXWELL_SL, space, space, kappa, intorder, leafsize, eta, eps, method, testhmatrix, mixed)
 
 
 def MaxwellDoubleLayerPotentialOperator(trial_space, test_space, kappa, trial_definedon=None, test_definedon=None,
-                                        intorder=3, leafsize=40, eta=2., eps=1e-6, method="aca", testhmatrix=False):
-    """Not available: needs HDivSurface / HCurl spaces (src/python_bem.cpp:154-173)."""
-    _maxwell()
+                                        intorder=3, leafsize=40, eta=2., eps=1e-6, method="aca", testhmatrix=False,
+                                        mixed=False):
+    """Maxwell double layer: rotated HCurl trace (trial) against the HDivSurface
+    trace (test) (src/python_bem.cpp:154-173, src/ngbem.hpp:482-510)."""
+    _no_region(trial_definedon, test_definedon)
+    if not (isinstance(trial_space, HCurl) and isinstance(test_space, HDivSurface)):
+        raise TypeError("MaxwellDoubleLayerPotentialOperator needs an HCurl trial and an HDivSurface test space")
+    dense = _dense_remap(method, leafsize, eta)
+    return IntegralOperatorC(_capi.MAXWELL_DL, kappa, trial_space, test_space, intorder, leafsize, eta, eps, dense,
+                             mixed, testhmatrix)

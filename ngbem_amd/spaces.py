@@ -85,10 +85,68 @@ class H1(FESpace):
         super().__init__(mesh, order, complex, definedon, dirichlet)
 
 
+class _EdgeSpace(FESpace):
+    """Lowest-order edge elements: dof = mesh edge (sorted (min, max) vertex
+    pairs); local dof l of a triangle is its edge (v_l, v_{l+1}), signed +1 when
+    v_l < v_{l+1}."""
+
+    def __init__(self, mesh, order=0, complex=False, definedon=None, dirichlet=None):
+        super().__init__(mesh, order, complex, definedon, dirichlet)
+        tri = mesh.tri
+        loc = np.stack([tri, np.roll(tri, -1, axis=1)], axis=2)  # nt x 3 x 2
+        lo, hi = loc.min(2), loc.max(2)
+        self.edges = mesh.edges()
+        key = lo.astype(np.int64) * mesh.nv + hi
+        ekey = self.edges[:, 0].astype(np.int64) * mesh.nv + self.edges[:, 1]
+        self.elem_dofs = np.searchsorted(ekey, key).astype(np.int64)
+        self.elem_sign = np.where(loc[:, :, 0] < loc[:, :, 1], 1.0, -1.0)
+        assert self.ndof == len(self.edges)
+
+    def rt_shapes(self, pts_bary):
+        """phi_l at barycentric points (q x 3) of every triangle: nt x q x 3(l) x 3(xyz),
+        phi_l = sign_l / J (x - p_opp(l)) (unit-flux Raviart-Thomas)."""
+        m = self.mesh
+        P = m.xyz[m.tri]  # nt x 3 x 3
+        J = 2.0 * m.areas()
+        X = np.einsum("qk,tkc->tqc", pts_bary, P)
+        popp = P[:, [2, 0, 1], :]  # vertex opposite local edge l = vertex l+2
+        s = self.elem_sign / J[:, None]
+        return s[:, None, :, None] * (X[:, :, None, :] - popp[:, None, :, :])
+
+
+class HDivSurface(_EdgeSpace):
+    """HDivSurface, order 0: unit-flux Raviart-Thomas (RWG-type) functions."""
+    kind = _capi.SPACE_HDIV
+    order = 0
+
+
+class HCurl(_EdgeSpace):
+    """HCurl, order 0, on the surface: Whitney edge functions N_l = n x phi_l."""
+    kind = _capi.SPACE_HCURL
+    order = 0
+
+
 def mass_matrix_coo(trial: FESpace, test: FESpace):
-    """Entries of the surface mass matrix  int u v ds  (test x trial), exact for
-    P0/P1 on flat triangles: rows, cols, values."""
+    """Entries of the surface mass matrix  int u . v ds  (test x trial), exact for
+    P0/P1 and the lowest-order edge spaces on flat triangles: rows, cols, values."""
     m = trial.mesh
+    if isinstance(trial, _EdgeSpace) or isinstance(test, _EdgeSpace):
+        if not (isinstance(trial, _EdgeSpace) and isinstance(test, _EdgeSpace)):
+            raise NotImplementedError("mass matrix between an edge space and a scalar space")
+        # edge-midpoint rule, exact for the quadratic integrands
+        bary = np.array([[0.5, 0.5, 0.0], [0.0, 0.5, 0.5], [0.5, 0.0, 0.5]])
+        w = m.areas()[:, None] / 3.0
+        n = m.normals()
+        phi_i = test.rt_shapes(bary)
+        phi_j = trial.rt_shapes(bary)
+        if isinstance(test, HCurl):
+            phi_i = np.cross(n[:, None, None, :], phi_i)
+        if isinstance(trial, HCurl):
+            phi_j = np.cross(n[:, None, None, :], phi_j)
+        loc = np.einsum("tq,tqic,tqjc->tij", w, phi_i, phi_j)  # nt x 3 x 3
+        rows = np.repeat(test.elem_dofs, 3, axis=1).reshape(-1)
+        cols = np.tile(trial.elem_dofs, (1, 3)).reshape(-1)
+        return rows, cols, loc.reshape(-1)
     area = m.areas()
     tri = m.tri.astype(np.int64)
     nt = m.nt

@@ -288,12 +288,16 @@ struct Mesh {
   }
 };
 
-enum { SP_P0 = 0, SP_P1 = 1 };
-enum { EV_ID = 0, EV_ROT = 1, EV_HELM = 2 };
+// SP_RWG: HDivSurface order 0 (unit-flux RT0 per edge); SP_NED: HCurl order 0
+enum { SP_P0 = 0, SP_P1 = 1, SP_RWG = 2, SP_NED = 3 };
+// EV_MAXW: [phi, div] (DiffOpMaxwellNew, src/diffops.hpp:263-338; DiffOpMaxwell :170-255)
+// EV_TR3: vector trace (HDivSurface BND; RotatedTrace :62-107 on HCurl)
+enum { EV_ID = 0, EV_ROT = 1, EV_HELM = 2, EV_MAXW = 3, EV_TR3 = 4 };
 
 struct Space {
   int kind = SP_P0, ndof = 0, nloc = 1;
   std::vector<int> dofs;                     // nt * nloc
+  std::vector<int> sign;                     // edge spaces: +1 when the local edge runs low -> high vertex
   std::vector<std::vector<int>> elems4dof;  // src/ngbem.cpp:70-79
   void setup(const Mesh& m, int k) {
     kind = k;
@@ -301,9 +305,28 @@ struct Space {
       ndof = m.nt; nloc = 1;
       dofs.resize(m.nt);
       for (int e = 0; e < m.nt; e++) dofs[e] = e;
-    } else {
+    } else if (k == SP_P1) {
       ndof = m.nv; nloc = 3;
       dofs.assign(m.tri.begin(), m.tri.end());
+    } else {  // edges numbered in (min vertex, max vertex) order; local edge l = (v_l, v_{l+1})
+      nloc = 3;
+      std::vector<std::pair<int, int>> ed;
+      for (int e = 0; e < m.nt; e++)
+        for (int l = 0; l < 3; l++) {
+          int a = m.tri[3 * e + l], b = m.tri[3 * e + (l + 1) % 3];
+          ed.push_back({std::min(a, b), std::max(a, b)});
+        }
+      std::vector<std::pair<int, int>> u(ed);
+      std::sort(u.begin(), u.end());
+      u.erase(std::unique(u.begin(), u.end()), u.end());
+      ndof = (int)u.size();
+      dofs.resize(3 * (size_t)m.nt);
+      sign.resize(3 * (size_t)m.nt);
+      for (int e = 0; e < m.nt; e++)
+        for (int l = 0; l < 3; l++) {
+          dofs[3 * e + l] = (int)(std::lower_bound(u.begin(), u.end(), ed[3 * e + l]) - u.begin());
+          sign[3 * e + l] = m.tri[3 * e + l] < m.tri[3 * e + (l + 1) % 3] ? 1 : -1;
+        }
     }
     elems4dof.assign(ndof, {});
     for (int e = 0; e < m.nt; e++)
@@ -311,12 +334,27 @@ struct Space {
   }
 };
 
-static int ev_dim(int ev) { return ev == EV_ID ? 1 : (ev == EV_ROT ? 3 : 4); }
+static int ev_dim(int ev) { return ev == EV_ID ? 1 : (ev == EV_ROT || ev == EV_TR3 ? 3 : 4); }
 
 // shape rows [nloc*dim] at reference point; layout dof*dim + comp
 // (DiffOpBoundaryRot src/diffops.hpp:12-57, DiffOpHelmholtz :110-165)
 static void eval_shape(const Mesh& m, const Space& s, int ev, int e, double xi, double eta, double* out) {
   if (s.kind == SP_P0) { out[0] = 1.0; return; }
+  if (s.kind == SP_RWG || s.kind == SP_NED) {
+    // RT0 with unit flux: phi_l = sign_l / J (x - p_opp(l)), div = 2 sign_l / J; the
+    // rotated trace of the Whitney function of an HCurl edge is -phi_l
+    double X[3];
+    m.point(e, xi, eta, X);
+    const double sg = (s.kind == SP_NED) ? -1.0 : 1.0;
+    const int dim = ev_dim(ev);
+    for (int l = 0; l < 3; l++) {
+      const double sl = s.sign[3 * e + l] / m.J[e];
+      const double* po = &m.xyz[3 * m.tri[3 * e + (l + 2) % 3]];
+      for (int c = 0; c < 3; c++) out[l * dim + c] = sg * sl * (X[c] - po[c]);
+      if (ev == EV_MAXW) out[l * dim + 3] = 2.0 * sl;
+    }
+    return;
+  }
   double lam[3] = {xi, eta, 1.0 - xi - eta};
   if (ev == EV_ID) { for (int k = 0; k < 3; k++) out[k] = lam[k]; return; }
   double g[3][3];
@@ -334,7 +372,7 @@ static void eval_shape(const Mesh& m, const Space& s, int ev, int e, double xi, 
 
 // ---------------------------------------------------------------- kernels --
 
-enum { K_LSL = 0, K_LDL, K_LHS, K_HSL, K_HDL, K_HCF, K_HHS };
+enum { K_LSL = 0, K_LDL, K_LHS, K_HSL, K_HDL, K_HCF, K_HHS, K_MSL, K_MDL };
 
 struct Term { double fac; int kc, trc, tec; };
 
@@ -356,6 +394,15 @@ struct Kernel {
       ncomp = 2;
       terms = {{1.0, 0, 0, 0}, {1.0, 0, 1, 1}, {1.0, 0, 2, 2}, {1.0, 1, 3, 3}};
       ev_trial = ev_test = EV_HELM;
+    } else if (kid == K_MSL) {  // src/ngbem.hpp:448-457
+      terms.clear();
+      for (int i = 0; i < 3; i++) terms.push_back({kappa, 0, i, i});
+      terms.push_back({-1.0 / kappa, 0, 3, 3});
+      ev_trial = ev_test = EV_MAXW;
+    } else if (kid == K_MDL) {  // src/ngbem.hpp:500-508
+      ncomp = 3;
+      terms = {{1.0, 0, 1, 2}, {-1.0, 0, 2, 1}, {1.0, 1, 2, 0}, {-1.0, 1, 0, 2}, {1.0, 2, 0, 1}, {-1.0, 2, 1, 0}};
+      ev_trial = ev_test = EV_TR3;
     }
   }
   // src/ngbem.hpp:238-431
@@ -386,6 +433,11 @@ struct Kernel {
       double nxny = nx[0] * ny[0] + nx[1] * ny[1] + nx[2] * ny[2];
       out[0] = g;
       out[1] = -kappa * kappa * g * nxny;
+    } else if (id == K_MSL) {
+      out[0] = e / (4 * PI * r);
+    } else if (id == K_MDL) {
+      cplx f = e / (4 * PI * r * r * r) * cplx(-1.0, kr);
+      for (int c = 0; c < 3; c++) out[c] = f * d[c];
     }
   }
 };
@@ -473,10 +525,11 @@ static void cluster_data(const Mesh& m, const Space& s, std::vector<double>& G, 
   for (int e = 0; e < m.nt; e++) {
     double mip[7][3];
     for (int j = 0; j < 7; j++) m.point(e, xh[j][0], xh[j][1], mip[j]);
-    int vc = 0;
+    int vc = 0, ec = 0;
     for (int l = 0; l < s.nloc; l++) {
       int d = s.dofs[e * s.nloc + l];
       if (cnt[d] > 2) { for (int k = 0; k < 3; k++) X[3 * d + k] = mip[vc][k]; vc++; }
+      if (cnt[d] == 2) { for (int k = 0; k < 3; k++) X[3 * d + k] = mip[3 + ec][k]; ec++; }  // edge midpoints
       if (cnt[d] == 1) for (int k = 0; k < 3; k++) X[3 * d + k] = mip[6][k];
     }
   }

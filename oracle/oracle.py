@@ -16,8 +16,9 @@ _LIB_PATH = os.path.join(_HERE, "_build", "libbem_oracle.so")
 _lib = None
 
 # kernel ids (same numbering as the product's C-ABI)
-LAPLACE_SL, LAPLACE_DL, LAPLACE_HS, HELMHOLTZ_SL, HELMHOLTZ_DL, HELMHOLTZ_CF, HELMHOLTZ_HS = range(7)
-P0, P1 = 0, 1
+(LAPLACE_SL, LAPLACE_DL, LAPLACE_HS, HELMHOLTZ_SL, HELMHOLTZ_DL, HELMHOLTZ_CF, HELMHOLTZ_HS, MAXWELL_SL,
+ MAXWELL_DL) = range(9)
+P0, P1, HDIV, HCURL = 0, 1, 2, 3
 
 
 def build():
@@ -111,8 +112,11 @@ class OracleOp:
         self.mesh = mesh
         self.kernel = kernel
         self.h = lib().orc_op_new(mesh.h, kernel, float(kappa), trial, test, intorder, leafsize, float(eta), float(eps))
-        self.ntrial = len(mesh.tri) if trial == P0 else len(mesh.xyz)
-        self.ntest = len(mesh.tri) if test == P0 else len(mesh.xyz)
+        nedge = len(np.unique(np.sort(np.concatenate([mesh.tri[:, [0, 1]], mesh.tri[:, [1, 2]],
+                                                       mesh.tri[:, [2, 0]]]), axis=1), axis=0))
+        size = {P0: len(mesh.tri), P1: len(mesh.xyz), HDIV: nedge, HCURL: nedge}
+        self.ntrial = size[trial]
+        self.ntest = size[test]
 
     def __del__(self):
         if getattr(self, "h", None) and _lib is not None:

@@ -48,8 +48,18 @@ def test_bad_arguments_are_reported():
                                     ctypes.byref(out)))
     with pytest.raises(NotImplementedError):
         nb.H1(m, order=2)
+    with pytest.raises(TypeError):
+        nb.MaxwellSingleLayerPotentialOperator(nb.SurfaceL2(m), 1.0)
+    hd, hc = nb.HDivSurface(m), nb.HCurl(m)
+    params = _capi.Params(3, 40, 2.0, 1e-6, 0, 0)
+    for kern, tr, te, msg in ((_capi.MAXWELL_SL, p0, p0, "Maxwell single layer needs"),
+                              (_capi.MAXWELL_SL, hd, hc, "Maxwell single layer needs"),
+                              (_capi.MAXWELL_DL, hd, hc, "Maxwell double layer needs"),
+                              (_capi.HELMHOLTZ_SL, hd, hd, "Laplace/Helmholtz kernels need")):
+        with pytest.raises(RuntimeError, match=msg):
+            _capi.check(L.bem_op_create(kern, 1.0, tr.handle, te.handle, ctypes.byref(params), ctypes.byref(out)))
     with pytest.raises(NotImplementedError):
-        nb.MaxwellSingleLayerPotentialOperator(None, 1.0)
+        nb.HDivSurface(m, order=1)
 
 
 def op_create(space_trial, space_test, kernel, **p):
@@ -63,15 +73,17 @@ def op_create(space_trial, space_test, kernel, **p):
 
 
 @pytest.mark.parametrize("level,trk,tek,leaf,eta", [(3, 0, 0, 40, 3.0), (4, 0, 0, 64, 2.0), (3, 1, 0, 20, 2.0),
-                                                    (4, 1, 1, 40, 2.0), (3, 0, 1, 16, 3.0)])
+                                                    (4, 1, 1, 40, 2.0), (3, 0, 1, 16, 3.0), (3, 3, 2, 32, 2.0),
+                                                    (4, 2, 2, 64, 2.0)])
 def test_tree_and_blocks_identical_to_oracle(level, trk, tek, leaf, eta):
     m = nb.icosphere(level)
-    sp = {0: nb.SurfaceL2, 1: nb.H1}
+    sp = {0: nb.SurfaceL2, 1: nb.H1, 2: nb.HDivSurface, 3: nb.HCurl}
     tr, te = sp[trk](m), sp[tek](m)
-    h = op_create(tr, te, _capi.LAPLACE_DL, leafsize=leaf, eta=eta)
+    kern = _capi.LAPLACE_DL if trk < 2 else (_capi.MAXWELL_DL if trk == 3 else _capi.MAXWELL_SL)
+    h = op_create(tr, te, kern, leafsize=leaf, eta=eta, kappa=1.0)
     L = _capi.lib()
     try:
-        ref = O.OracleOp(m, O.LAPLACE_DL, trial=trk, test=tek, leafsize=leaf, eta=eta)
+        ref = O.OracleOp(m, kern, kappa=1.0, trial=trk, test=tek, leafsize=leaf, eta=eta)
         for which, n in ((0, tr.ndof), (1, te.ndof)):
             perm = np.zeros(n, dtype=np.int32)
             ncl = L.bem_op_tree(h, which, _capi.iptr(perm), None, None, 0)

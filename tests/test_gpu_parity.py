@@ -23,6 +23,9 @@ FACT = {
     "HDL": (nb.HelmholtzDoubleLayerPotentialOperator, O.HELMHOLTZ_DL, True),
     "HCF": (nb.HelmholtzCombinedFieldOperator, O.HELMHOLTZ_CF, True),
     "HHS": (nb.HelmholtzHypersingularOperator, O.HELMHOLTZ_HS, True),
+    "MSL": (nb.MaxwellSingleLayerPotentialOperator, O.MAXWELL_SL, True),
+    "MSLC": (nb.MaxwellSingleLayerPotentialOperatorCurl, O.MAXWELL_SL, True),
+    "MDL": (nb.MaxwellDoubleLayerPotentialOperator, O.MAXWELL_DL, True),
 }
 
 # (kernel, trial space, test space, level, intorder, leafsize, eta, eps, kappa)
@@ -43,11 +46,17 @@ CASES = [
     ("LHS", 1, 1, 3, 3, 16, 2.0, 1e-8, 0.0),
     ("HHS", 1, 1, 3, 3, 16, 2.0, 1e-8, 4.0),
     ("LDL", 1, 0, 3, 3, 16, 2.0, 1e-8, 0.0),
+    # Maxwell: spaces 2 = HDivSurface, 3 = HCurl (edge dofs)
+    ("MSL", 2, 2, 2, 4, 16, 2.0, 1e-8, 3.0),
+    ("MSLC", 3, 3, 2, 4, 16, 2.0, 1e-8, 3.0),
+    ("MDL", 3, 2, 2, 4, 16, 2.0, 1e-8, 3.0),
+    ("MSL", 2, 2, 3, 3, 32, 2.0, 1e-6, 6.0),
+    ("MDL", 3, 2, 3, 3, 32, 2.0, 1e-6, 6.0),
 ]
 
 
 def _space(mesh, kind):
-    return nb.SurfaceL2(mesh, order=0) if kind == 0 else nb.H1(mesh, order=1)
+    return [nb.SurfaceL2, nb.H1, nb.HDivSurface, nb.HCurl][kind](mesh)
 
 
 def _build(case):
@@ -57,7 +66,11 @@ def _build(case):
     trial, test = _space(mesh, trk), _space(mesh, tek)
     kw = dict(intorder=order, leafsize=leaf, eta=eta, eps=eps)
     print(f"case {case}", flush=True)
-    if name in ("LSL",):
+    if name in ("MSL", "MSLC"):
+        op = fac(trial, kappa, **kw)
+    elif name == "MDL":
+        op = fac(trial, test, kappa, **kw)
+    elif name in ("LSL",):
         op = fac(trial, **kw)
     elif name == "LHS":
         op = fac(trial, **kw)
@@ -104,7 +117,7 @@ def test_blocks_match_oracle(gpu, case):
         assert same_rank >= 0.9 * nfar
 
 
-@pytest.mark.parametrize("case", CASES[:1] + CASES[4:6] + CASES[9:10] + CASES[11:15],
+@pytest.mark.parametrize("case", CASES[:1] + CASES[4:6] + CASES[9:10] + CASES[11:15] + CASES[18:20],
                          ids=lambda c: f"{c[0]}-{c[1]}{c[2]}")
 def test_matvec_matches_oracle(gpu, case):
     op, ref = _build(case)
@@ -159,7 +172,8 @@ def test_mixed_matches_fp64(gpu):
     assert (torch.linalg.norm(yt64 - yt32) / torch.linalg.norm(yt64)).item() < 1e-5
 
 
-@pytest.mark.parametrize("case", [CASES[9], CASES[13], CASES[14]], ids=lambda c: f"{c[0]}-{c[1]}{c[2]}")
+@pytest.mark.parametrize("case", [CASES[9], CASES[13], CASES[14], CASES[18], CASES[19]],
+                         ids=lambda c: f"{c[0]}-{c[1]}{c[2]}")
 def test_aca_resume_matches_oracle(gpu, case, monkeypatch):
     """A tiny initial rank capacity forces every far block through the
     resume path (capacity reached -> larger workspace -> continue)."""

@@ -84,7 +84,8 @@ def test_dl_hs_h1_mixed_refinement(gpu):
     assert float(torch.linalg.norm(v64 - v32.real) / torch.linalg.norm(v64)) < 1e-6
     # W annihilates constants exactly (surface curl of a constant is zero)
     w1 = W.mat * ones
-    assert float(torch.linalg.norm(w1)) <= 1e-10 * float(torch.linalg.norm(W.mat * g))
+    # exact for the dense part; the ACA blocks (eps 1e-6) keep it only to their tolerance
+    assert float(torch.linalg.norm(w1)) <= 1e-4 * float(torch.linalg.norm(W.mat * g))
 
 
 def mie_dudn(kappa, cos_t):

@@ -1,0 +1,230 @@
+"""Pins of the oracle's Maxwell operators (CPU only).
+
+- Planar RT identity: on a flat open mesh the lowest-order HDivSurface space
+  reproduces every in-plane field u = c + alpha (x - x0) exactly, with
+  coefficients = edge fluxes computed from geometry alone.  Then
+      a^T V_maxwell a = kappa sum_k U_k^T V_H1 U_k - (2 alpha)^2 / kappa 1^T V_P0 1
+  where V_H1 / V_P0 are the oracle's Helmholtz single layer on H1 p1 / SurfaceL2
+  p0 with the same rules.  Both sides evaluate the same quadrature points, so
+  they agree to rounding: a wrong factor, sign, dropped term or a wrong RT
+  shape fails it.  The MaxwellSL kernel terms are src/ngbem.hpp:452-458.
+- The curl form on HCurl (rotated trace n x N, surface curl) equals the div
+  form on HDivSurface for N = n x phi; the double layer vanishes on a plane
+  ((x - y) . (u x v) = 0 for in-plane x - y, u, v; src/ngbem.hpp:482-510).
+- Far element pairs: entries against an independent numpy quadrature of the
+  kernels as the reference states them (src/ngbem.hpp:459-463, 492-499).
+- Manufactured solution (reference tests/test_Maxwell_Box.py): the radiating
+  dipole field E with source inside the sphere satisfies
+  V j = (-1/2 M + K) m  for m = -E_t (edge moments) and j = n x curl E / kappa;
+  the L2 error of j must fall like h^2 with the oracle's dense matrices.
+"""
+import numpy as np
+import pytest
+
+import ngbem_amd as nb
+from ngbem_amd.spaces import mass_matrix_coo
+from oracle import oracle as O
+
+
+def planar_mesh(n=4, seed=0):
+    rng = np.random.default_rng(seed)
+    g = np.linspace(0, 1, n + 1)
+    X, Y = np.meshgrid(g, g, indexing="ij")
+    P = np.stack([X.ravel(), Y.ravel()], 1)
+    inner = (P[:, 0] > 0) & (P[:, 0] < 1) & (P[:, 1] > 0) & (P[:, 1] < 1)
+    P[inner] += 0.2 / n * rng.uniform(-1, 1, (inner.sum(), 2))
+    tri = []
+    for i in range(n):
+        for j in range(n):
+            a, b, c, d = i * (n + 1) + j, (i + 1) * (n + 1) + j, (i + 1) * (n + 1) + j + 1, i * (n + 1) + j + 1
+            tri += [[a, b, c], [a, c, d]]
+    xyz = np.concatenate([P, np.zeros((len(P), 1))], 1)
+    th, ph = 0.7, 0.4
+    R1 = np.array([[1, 0, 0], [0, np.cos(th), -np.sin(th)], [0, np.sin(th), np.cos(th)]])
+    R2 = np.array([[np.cos(ph), -np.sin(ph), 0], [np.sin(ph), np.cos(ph), 0], [0, 0, 1]])
+    return nb.Mesh(xyz @ (R2 @ R1).T + np.array([0.3, -0.2, 0.1]), np.array(tri))
+
+
+def edge_fluxes(space, u):
+    """Coefficients of the RT interpolant of a linear field: flux of u through
+    each edge, edge oriented from its lower to its higher vertex, normal t x n."""
+    mesh = space.mesh
+    n = mesh.normals()[0]
+    A, B = mesh.xyz[space.edges[:, 0]], mesh.xyz[space.edges[:, 1]]
+    return np.einsum("ec,ec->e", u(0.5 * (A + B)), np.cross(B - A, n))
+
+
+def dense_mass(trial, test):
+    r, c, v = mass_matrix_coo(trial, test)
+    M = np.zeros((test.ndof, trial.ndof))
+    np.add.at(M, (r, c), v)
+    return M
+
+
+@pytest.fixture(scope="module")
+def plane():
+    return planar_mesh(4)
+
+
+def _in_plane_field(mesh):
+    n = mesh.normals()[0]
+    t1 = mesh.xyz[1] - mesh.xyz[0]
+    t1 /= np.linalg.norm(t1)
+    t2 = np.cross(n, t1)
+    c, alpha, x0 = 0.8 * t1 - 0.3 * t2, 0.65, mesh.xyz.mean(0)
+    return (lambda X: c + alpha * (X - x0)), alpha
+
+
+def test_rt_interpolation_reproduces_linear_fields(plane):
+    hd = nb.HDivSurface(plane)
+    u, _ = _in_plane_field(plane)
+    a = edge_fluxes(hd, u)
+    bary = np.array([[0.2, 0.3, 0.5], [0.6, 0.1, 0.3]])
+    X = np.einsum("qk,tkc->tqc", bary, plane.xyz[plane.tri])
+    uh = np.einsum("tl,tqlc->tqc", a[hd.elem_dofs], hd.rt_shapes(bary))
+    assert np.abs(uh - u(X)).max() < 1e-14
+    # and the mass matrix integrates |u|^2 exactly (quadratic integrand)
+    Mh = dense_mass(hd, hd)
+    tri_rule = O.trig_rule(4)
+    bq = np.stack([tri_rule[:, 0], tri_rule[:, 1], 1 - tri_rule[:, 0] - tri_rule[:, 1]], 1)
+    Xq = np.einsum("qk,tkc->tqc", bq, plane.xyz[plane.tri])
+    ref = np.sum(tri_rule[:, 2][None] * 2 * plane.areas()[:, None] * (u(Xq) ** 2).sum(-1))
+    assert abs(a @ Mh @ a - ref) < 1e-13 * ref
+
+
+@pytest.mark.parametrize("intorder", [3, 5])
+def test_maxwell_sl_planar_identity(plane, intorder):
+    kappa = 2.0
+    hd = nb.HDivSurface(plane)
+    u, alpha = _in_plane_field(plane)
+    a = edge_fluxes(hd, u)
+    V = O.OracleOp(plane, O.MAXWELL_SL, kappa, O.HDIV, O.HDIV, intorder=intorder).dense()
+    V1 = O.OracleOp(plane, O.HELMHOLTZ_SL, kappa, O.P1, O.P1, intorder=intorder).dense()
+    V0 = O.OracleOp(plane, O.HELMHOLTZ_SL, kappa, O.P0, O.P0, intorder=intorder).dense()
+    U = u(plane.xyz)
+    lhs = a @ V @ a
+    rhs = kappa * sum(U[:, k] @ V1 @ U[:, k] for k in range(3)) - (2 * alpha) ** 2 / kappa * V0.sum()
+    assert abs(lhs - rhs) < 1e-12 * abs(rhs)
+
+
+def test_maxwell_curl_form_and_planar_double_layer(plane):
+    kappa = 1.5
+    V = O.OracleOp(plane, O.MAXWELL_SL, kappa, O.HDIV, O.HDIV, intorder=4).dense()
+    Vc = O.OracleOp(plane, O.MAXWELL_SL, kappa, O.HCURL, O.HCURL, intorder=4).dense()
+    assert np.abs(Vc - V).max() <= 1e-14 * np.abs(V).max()
+    K = O.OracleOp(plane, O.MAXWELL_DL, kappa, O.HCURL, O.HDIV, intorder=4).dense()
+    assert np.abs(K).max() < 1e-14 * np.abs(V).max()
+
+
+def _collapsed_gauss(n):
+    g, w = np.polynomial.legendre.leggauss(n)
+    g, w = 0.5 * (g + 1), 0.5 * w
+    s, t = np.meshgrid(g, g, indexing="ij")
+    ws = np.outer(w, w)
+    xi, eta = s.ravel(), (t * (1 - s)).ravel()
+    return np.stack([xi, eta, 1 - xi - eta], 1), (ws * (1 - s)).ravel()
+
+
+def test_maxwell_far_pairs_vs_numpy():
+    mesh = nb.icosphere(1)
+    hd = nb.HDivSurface(mesh)
+    kappa, intorder = 2.5, 6
+    bary, w = _collapsed_gauss(12)
+    P = mesh.xyz[mesh.tri]
+    X = np.einsum("qk,tkc->tqc", bary, P)
+    W = w[None, :] * 2 * mesh.areas()[:, None]
+    phi = hd.rt_shapes(bary)
+    div = hd.elem_sign / mesh.areas()[:, None]          # div phi_l = 2 sign_l / J
+    rot = -phi                                           # n x (n x phi) for tangential phi
+    cen = P.mean(1)
+    supp = [np.argwhere(hd.elem_dofs == d) for d in range(hd.ndof)]
+    msl = O.OracleOp(mesh, O.MAXWELL_SL, kappa, O.HDIV, O.HDIV, intorder=intorder)
+    mdl = O.OracleOp(mesh, O.MAXWELL_DL, kappa, O.HCURL, O.HDIV, intorder=intorder)
+    rng = np.random.default_rng(0)
+    res = []
+    while len(res) < 60:
+        di, dj = rng.integers(0, hd.ndof, 2)
+        if not all(np.linalg.norm(cen[a] - cen[b]) > 1.0 for a, _ in supp[di] for b, _ in supp[dj]):
+            continue
+        vs = vd = 0j
+        for a, la in supp[di]:          # test element (x)
+            for b, lb in supp[dj]:      # trial element (y)
+                d = X[a][:, None, :] - X[b][None, :, :]
+                r = np.linalg.norm(d, axis=-1)
+                G = np.exp(1j * kappa * r) / (4 * np.pi * r)
+                ww = W[a][:, None] * W[b][None, :]
+                vs += kappa * np.sum(ww * G * (phi[a, :, la] @ phi[b, :, lb].T))
+                vs -= div[a, la] * div[b, lb] / kappa * np.sum(ww * G)
+                kv = (np.exp(1j * kappa * r) / (4 * np.pi * r ** 3) * (1j * kappa * r - 1))[..., None] * d
+                cr = np.cross(rot[b, :, lb][None, :, :], phi[a, :, la][:, None, :])
+                vd += np.sum(ww * np.einsum("pqc,pqc->pq", kv, cr))
+        res.append((msl.dense([dj], [di])[0, 0], vs, mdl.dense([dj], [di])[0, 0], vd))
+    r = np.array(res)
+    assert np.abs(r[:, 0] - r[:, 1]).max() < 5e-6 * np.abs(r[:, 1]).max()
+    assert np.abs(r[:, 2] - r[:, 3]).max() < 5e-6 * np.abs(r[:, 3]).max()
+
+
+def dipole_field(X, kappa, e):
+    """Radiating dipole E = -curl curl (e exp(i kappa r) / r) and its curl
+    (reference tests/test_Maxwell_Box.py:29-35, source at the origin)."""
+    r = np.linalg.norm(X, axis=-1)[..., None]
+    ex = (X * e).sum(-1)[..., None]
+    f = (-kappa ** 2 * r ** 2 - 1j * kappa * r + 1) / r ** 3
+    g = ex * (-kappa ** 2 * r ** 2 - 3j * kappa * r + 3) / r ** 5
+    E = np.exp(1j * kappa * r) * (f * e - g * X)
+    curlE = -kappa ** 2 * np.exp(1j * kappa * r) * (1j * kappa * r - 1) / r ** 3 * np.cross(X, e)
+    return E, curlE
+
+
+def test_dipole_curl_closed_form():
+    kappa, e = 1.3, np.array([0.3, -0.5, 0.8])
+    X0, h = np.array([0.3, -0.7, 0.5]), 1e-5
+    J = np.zeros((3, 3), complex)
+    for k in range(3):
+        d = np.zeros(3)
+        d[k] = h
+        J[:, k] = (dipole_field(X0 + d, kappa, e)[0] - dipole_field(X0 - d, kappa, e)[0]) / (2 * h)
+    curl = np.array([J[2, 1] - J[1, 2], J[0, 2] - J[2, 0], J[1, 0] - J[0, 1]])
+    assert np.abs(curl - dipole_field(X0, kappa, e)[1]).max() < 1e-8
+
+
+class DipoleProblem:
+    """Data of the manufactured exterior problem on a mesh of the unit sphere."""
+
+    def __init__(self, mesh, kappa):
+        self.mesh, self.kappa = mesh, kappa
+        self.e = np.array([1.192, -0.189, 2.745]) / np.linalg.norm([1.192, -0.189, 2.745])
+        self.hd, self.hc = nb.HDivSurface(mesh), nb.HCurl(mesh)
+        # m = -E_t as edge moments int_e m . (p_hi - p_lo) dlambda (the dual basis of N)
+        g, gw = np.polynomial.legendre.leggauss(8)
+        g, gw = 0.5 * (g + 1), 0.5 * gw
+        A, B = mesh.xyz[self.hc.edges[:, 0]], mesh.xyz[self.hc.edges[:, 1]]
+        Ee, _ = dipole_field(A[None] + g[:, None, None] * (B - A)[None], kappa, self.e)
+        self.m = -np.einsum("q,qec,ec->e", gw, Ee, B - A)
+        self.M = dense_mass(self.hc, self.hd)
+
+    def rhs(self, K):
+        return -0.5 * (self.M @ self.m) + K(self.m)
+
+    def error(self, j):
+        rule = O.trig_rule(10)
+        bary = np.stack([rule[:, 0], rule[:, 1], 1 - rule[:, 0] - rule[:, 1]], 1)
+        w = rule[:, 2][None, :] * 2 * self.mesh.areas()[:, None]
+        X = np.einsum("qk,tkc->tqc", bary, self.mesh.xyz[self.mesh.tri])
+        n = self.mesh.normals()[:, None, :]
+        jex = np.cross(n, dipole_field(X, self.kappa, self.e)[1] / self.kappa)
+        jh = np.einsum("tl,tqlc->tqc", j[self.hd.elem_dofs], self.hd.rt_shapes(bary))
+        return np.sqrt(np.sum(w * (np.abs(jh - jex) ** 2).sum(-1)) / np.sum(w * (np.abs(jex) ** 2).sum(-1)))
+
+
+def test_maxwell_dipole_convergence():
+    kappa, errs = 3.0, []
+    for lvl in (1, 2):
+        mesh = nb.icosphere(lvl)
+        pb = DipoleProblem(mesh, kappa)
+        V = O.OracleOp(mesh, O.MAXWELL_SL, kappa, O.HDIV, O.HDIV, intorder=4).dense()
+        K = O.OracleOp(mesh, O.MAXWELL_DL, kappa, O.HCURL, O.HDIV, intorder=4).dense()
+        errs.append(pb.error(np.linalg.solve(V, pb.rhs(lambda m: K @ m))))
+    # measured 0.059 and 0.016: second order in h
+    assert errs[0] < 0.08 and errs[1] < 0.025
+    assert errs[1] < 0.35 * errs[0]
