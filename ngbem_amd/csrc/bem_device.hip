@@ -804,11 +804,13 @@ __global__ __launch_bounds__(NTH) void k_aca(AcaTask* __restrict__ tasks, const 
 // dof-level factors from the point-level ACA factors (src/ngbem.cpp:1045-1160):
 // U2[ii][r0_t + l] = fac_t sum_{(a,k) in rowmap(ii)} sum_q phi_k,tec(q) w_aq U_c[l][a*ntq+q]
 // V2[r0_t + l][jj] =       sum_{(a,k) in colmap(jj)} sum_q phi_k,trc(q) w_aq V_c[l][a*ntq+q]
+constexpr int MAX_KCOMP = 3;  // kernel components (MaxwellDL: 3)
+
 struct ProjTask {
   int blk;
   int ncomp;
-  int64_t uoff[2], voff[2];  // workspace offsets per component
-  int rank[2];
+  int64_t uoff[MAX_KCOMP], voff[MAX_KCOMP];  // workspace offsets per component
+  int rank[MAX_KCOMP];
   int np, nq;
   int64_t ou, ov;            // output offsets (batch buffer)
 };
@@ -826,6 +828,7 @@ __global__ __launch_bounds__(256) void k_project(const ProjTask* __restrict__ ta
                                                  const typename KER::T* __restrict__ wsV,
                                                  typename KER::T* __restrict__ oU, typename KER::T* __restrict__ oV) {
   typedef typename KER::T T;
+  static_assert(KER::NC <= MAX_KCOMP, "ProjTask holds MAX_KCOMP kernel components");
   const int2 it = items[blockIdx.x];
   const ProjTask tk = tasks[it.x];
   const FarBlk b = fb[tk.blk];
@@ -1382,6 +1385,7 @@ void assemble_far(Operator& op, hipStream_t s) {
   const Tree& rt = op.trial_tree;
   const int ntq = op.trig.size();
   const int ncomp = op.kinfo.ncomp;
+  if (ncomp > MAX_KCOMP) throw std::runtime_error("assemble_far: more kernel components than MAX_KCOMP");
   ClusterPatches& ci = *op.cp_test;
   ClusterPatches& cj = op.cp_trial;
   PhaseTimer pt_("far");
@@ -1582,7 +1586,7 @@ void assemble_far(Operator& op, hipStream_t s) {
       for (size_t bi = 0; bi < bblocks.size(); bi++) {
         if (where[bi * ncomp] != w) continue;
         const int blk = bblocks[bi];
-        int rk[2] = {0, 0};
+        int rk[MAX_KCOMP] = {0, 0, 0};
         for (int c = 0; c < ncomp; c++) rk[c] = tasks[bi * ncomp + c].rank;
         const FarBlk& fb = op.far[blk];
         const int np = fb.pm * ntq, nq = fb.pn * ntq;
