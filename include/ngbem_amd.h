@@ -66,7 +66,8 @@ enum {
   BEM_APPLY_TRANS = 1,     /* y = alpha A^T x + beta y (plain transpose, MultTransAdd) */
   BEM_APPLY_CONJ = 2,      /* with TRANS: A^H (adjoint) */
   BEM_APPLY_MIXED = 4,     /* use the FP32 / complex64 copy of the operator */
-  BEM_APPLY_ON_DEVICE = 8  /* x and y are device pointers (else host pointers) */
+  BEM_APPLY_ON_DEVICE = 8, /* x and y are device pointers (else host pointers) */
+  BEM_APPLY_PER_VECTOR = 16 /* nrhs > 1: one vector at a time instead of the block product */
 };
 
 /* vector dtypes */
@@ -117,7 +118,11 @@ int bem_op_create(int kernel, double kappa, bem_space* trial, bem_space* test, c
 /* near-field quadrature + ACA on the device; stream is a hipStream_t (NULL = default) */
 int bem_op_assemble(bem_op* op, void* stream);
 /* y = alpha op(A) x + beta y for nrhs right-hand sides (columns of length ld).
- * alpha/beta point to 2 doubles (re, im); dtype BEM_F64 only for real operators. */
+ * alpha/beta point to 2 doubles (re, im); dtype BEM_F64 only for real operators.
+ * The forward product with nrhs > 1 runs as a block product on the matrix
+ * cores (MFMA), 64 columns per pass; transposed products go vector by vector.
+ * Replaces HMatrix::Mult/MultAdd/MultTrans/MultTransAdd (src/hmat.cpp:632-666),
+ * which the reference calls once per vector (MultiVector applies loop). */
 int bem_op_apply(bem_op* op, int flags, const double* alpha, const void* x, const double* beta, void* y,
                  int nrhs, int ld, int dtype, void* stream);
 int bem_op_stats(const bem_op* op, bem_stats* st);

@@ -14,7 +14,7 @@ _lib = None
 SPACE_P0, SPACE_P1, SPACE_HDIV, SPACE_HCURL = 0, 1, 2, 3
 (LAPLACE_SL, LAPLACE_DL, LAPLACE_HS, HELMHOLTZ_SL, HELMHOLTZ_DL, HELMHOLTZ_CF, HELMHOLTZ_HS, MAXWELL_SL,
  MAXWELL_DL) = range(9)
-APPLY_TRANS, APPLY_CONJ, APPLY_MIXED, APPLY_ON_DEVICE = 1, 2, 4, 8
+APPLY_TRANS, APPLY_CONJ, APPLY_MIXED, APPLY_ON_DEVICE, APPLY_PER_VECTOR = 1, 2, 4, 8, 16
 F64, C128 = 0, 1
 
 

@@ -267,10 +267,26 @@ int bem_op_apply(bem_op* h, int flags, const double* alpha, const void* x, const
     hipStream_t s = (hipStream_t)stream;
     const size_t esz = op.complex_valued ? 16 : 8;
     const int dflags = flags & (BEM_APPLY_TRANS | BEM_APPLY_CONJ | BEM_APPLY_MIXED);
+    // several right-hand sides of the forward product go through the matrix cores
+    const bool block = nrhs > 1 && !trans && !(flags & BEM_APPLY_PER_VECTOR);
     if (flags & BEM_APPLY_ON_DEVICE) {
-      for (int r = 0; r < nrhs; r++)
-        bem::apply_device(op, dflags, alpha, (const char*)x + (size_t)r * ld * esz, beta,
-                          (char*)y + (size_t)r * ld * esz, s);
+      if (block) {
+        bem::apply_device_mm(op, dflags, alpha, x, beta, y, nrhs, ld, s);
+      } else {
+        for (int r = 0; r < nrhs; r++)
+          bem::apply_device(op, dflags, alpha, (const char*)x + (size_t)r * ld * esz, beta,
+                            (char*)y + (size_t)r * ld * esz, s);
+      }
+    } else if (block) {
+      const size_t xin = ((size_t)ld * (nrhs - 1) + nin) * esz, yout = ((size_t)ld * (nrhs - 1) + nout) * esz;
+      bem::DBuf dx, dy;
+      dx.alloc(xin);
+      dy.alloc(yout);
+      BEM_HIP(hipMemcpyAsync(dx.p, x, xin, hipMemcpyHostToDevice, s));
+      if (!(beta[0] == 0.0 && beta[1] == 0.0)) BEM_HIP(hipMemcpyAsync(dy.p, y, yout, hipMemcpyHostToDevice, s));
+      bem::apply_device_mm(op, dflags, alpha, dx.p, beta, dy.p, nrhs, ld, s);
+      BEM_HIP(hipMemcpyAsync(y, dy.p, yout, hipMemcpyDeviceToHost, s));
+      BEM_HIP(hipStreamSynchronize(s));
     } else {
       bem::DBuf dx, dy;
       dx.alloc((size_t)nin * esz);

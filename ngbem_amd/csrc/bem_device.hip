@@ -1096,6 +1096,219 @@ __global__ __launch_bounds__(256) void k_mv_cols(const Tile* __restrict__ tiles,
   if (wv == 0 && act) yp[tl.p0 + lane] = colsum[lane] + red[0][lane] + red[1][lane] + red[2][lane] + red[3][lane];
 }
 
+// ----------------------------------------------------------- block apply --
+// Y = A X for MM_W = 64 right-hand sides at once, on the matrix cores: MFMA
+// 16x16x4 (f32 for the FP32 copy, f64 for the FP64 one), complex products as
+// four real MFMAs.  X, T and Y are in cluster order, row-major [position][64],
+// so the right-hand-side tile of a block is one contiguous slab.  Operand
+// mapping: in MFMA k-slot g (= lane / 16) of substep s, a lane carries column
+// kk + 4 g + s of its row, so four consecutive columns of one row come from one
+// lane (one vector load for row-major V).  Result lane layout: D[4 g + v][lane % 16].
+constexpr int MM_W = 64;
+
+template <class R>
+struct MF;
+template <>
+struct MF<float> {
+  typedef float V4 __attribute__((ext_vector_type(4)));
+  __device__ __forceinline__ static V4 mma(float a, float b, V4 c) {
+    return __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, c, 0, 0, 0);
+  }
+};
+template <>
+struct MF<double> {
+  typedef double V4 __attribute__((ext_vector_type(4)));
+  __device__ __forceinline__ static V4 mma(double a, double b, V4 c) {
+    return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0);
+  }
+};
+
+// 16 rows x 64 columns of a wave's partial result
+template <class VT>
+struct MAcc;
+template <class R>
+struct MAcc<cx<R>> {
+  typename MF<R>::V4 re[4], im[4];
+  __device__ __forceinline__ void zero() {
+#pragma unroll
+    for (int t = 0; t < 4; t++)
+#pragma unroll
+      for (int v = 0; v < 4; v++) re[t][v] = im[t][v] = R(0);
+  }
+  __device__ __forceinline__ void step(cx<R> a, const cx<R> (&b)[4]) {
+#pragma unroll
+    for (int t = 0; t < 4; t++) {
+      re[t] = MF<R>::mma(a.x, b[t].x, re[t]);
+      re[t] = MF<R>::mma(-a.y, b[t].y, re[t]);
+      im[t] = MF<R>::mma(a.x, b[t].y, im[t]);
+      im[t] = MF<R>::mma(a.y, b[t].x, im[t]);
+    }
+  }
+  __device__ __forceinline__ cx<R> get(int t, int v) const { return {re[t][v], im[t][v]}; }
+};
+template <class R>
+struct MAccReal {
+  typename MF<R>::V4 re[4];
+  __device__ __forceinline__ void zero() {
+#pragma unroll
+    for (int t = 0; t < 4; t++)
+#pragma unroll
+      for (int v = 0; v < 4; v++) re[t][v] = R(0);
+  }
+  __device__ __forceinline__ void step(R a, const R (&b)[4]) {
+#pragma unroll
+    for (int t = 0; t < 4; t++) re[t] = MF<R>::mma(a, b[t], re[t]);
+  }
+  __device__ __forceinline__ R get(int t, int v) const { return re[t][v]; }
+};
+template <>
+struct MAcc<double> : MAccReal<double> {};
+template <>
+struct MAcc<float> : MAccReal<float> {};
+
+// one 16-column group (four MFMA substeps) of A (16 rows, element (i, k) at
+// A[k * lda] for this lane's row i) times X rows [xk .. xk + 16)
+template <class ST, class VT>
+__device__ __forceinline__ void mm_group(MAcc<VT>& acc, const ST* __restrict__ A, int64_t lda, bool rok,
+                                         const VT* __restrict__ X, int kk, int cnt, int g, int i) {
+  VT a[4];
+#pragma unroll
+  for (int s = 0; s < 4; s++) {
+    const int k = kk + 4 * g + s;
+    a[s] = zero_<VT>();
+    if (rok && k < cnt) cvt(a[s], A[(int64_t)k * lda]);
+  }
+#pragma unroll
+  for (int s = 0; s < 4; s++) {
+    const int k = kk + 4 * g + s;
+    VT b[4];
+#pragma unroll
+    for (int t = 0; t < 4; t++) b[t] = (k < cnt) ? X[(int64_t)k * MM_W + 16 * t + i] : zero_<VT>();
+    acc.step(a[s], b);
+  }
+}
+
+struct MMItem {
+  int blk, l0;  // far block, first of 16 rank rows
+};
+
+// T[toff + l0 + r][:] = V[l0 + r][:] X[c0 : c0 + n][:], one workgroup per 16
+// rank rows of a far block; the four waves take every fourth 16-column group
+// and are summed through LDS
+template <class ST, class VT>
+__global__ __launch_bounds__(256) void k_mm_far_t(const MMItem* __restrict__ items, const FarBlk* __restrict__ fb,
+                                                  const ST* __restrict__ fV, const VT* __restrict__ xm,
+                                                  VT* __restrict__ tm) {
+  __shared__ VT red[3][16][MM_W];
+  const MMItem it = items[blockIdx.x];
+  const FarBlk b = fb[it.blk];
+  const int lane = threadIdx.x & 63;
+  const int wv = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+  const int i = lane & 15, g = lane >> 4;
+  const bool rok = it.l0 + i < b.R;
+  const ST* A = fV + b.voff + (int64_t)(rok ? it.l0 + i : 0) * b.n;  // row-major: column k at A[k]
+  const VT* X = xm + (int64_t)b.c0 * MM_W;
+  MAcc<VT> acc;
+  acc.zero();
+  for (int kk = 16 * wv; kk < b.n; kk += 64) mm_group<ST, VT>(acc, A, 1, rok, X, kk, b.n, g, i);
+  if (wv > 0)
+#pragma unroll
+    for (int t = 0; t < 4; t++)
+#pragma unroll
+      for (int v = 0; v < 4; v++) red[wv - 1][4 * g + v][16 * t + i] = acc.get(t, v);
+  __syncthreads();
+  if (wv == 0) {
+#pragma unroll
+    for (int t = 0; t < 4; t++)
+#pragma unroll
+      for (int v = 0; v < 4; v++) {
+        const int r = 4 * g + v, c = 16 * t + i;
+        if (it.l0 + r < b.R)
+          tm[(int64_t)(b.toff + it.l0 + r) * MM_W + c] = acc.get(t, v) + red[0][r][c] + red[1][r][c] + red[2][r][c];
+      }
+  }
+}
+
+// forward block apply over the row tiles of the single-vector plan: the tile's
+// rows go in 16-row strips to the waves; with fewer than three strips the
+// spare waves split the columns and are summed through LDS
+template <class ST, class VT>
+__global__ __launch_bounds__(256) void k_mm_rows(const Tile* __restrict__ tiles, const Seg* __restrict__ segs,
+                                                 const ST* __restrict__ nval, const ST* __restrict__ fUt,
+                                                 const VT* __restrict__ xm, const VT* __restrict__ tm,
+                                                 VT* __restrict__ ym) {
+  __shared__ VT red[3][16][MM_W];
+  __shared__ Seg s_seg[64];
+  const Tile tl = tiles[blockIdx.x];
+  const int lane = threadIdx.x & 63;
+  const int wv = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+  const int i = lane & 15, g = lane >> 4;
+  const int ns = (tl.len + 15) >> 4;
+  const int nsplit = ns == 1 ? 4 : (ns == 2 ? 2 : 1);
+  const int strip = wv % ns, ksub = wv / ns;
+  const bool active = ksub < nsplit;
+  const bool rok = active && strip * 16 + i < tl.len;
+  const int r = tl.roff + strip * 16 + (rok ? i : 0);
+  MAcc<VT> acc;
+  acc.zero();
+  for (int sb = tl.s0; sb < tl.s1; sb += 64) {
+    const int nseg = min(64, tl.s1 - sb);
+    __syncthreads();
+    if ((int)threadIdx.x < nseg) s_seg[threadIdx.x] = segs[sb + threadIdx.x];
+    __syncthreads();
+    if (!active) continue;
+    for (int k = 0; k < nseg; k++) {
+      const Seg sg = s_seg[k];
+      const ST* A = (sg.far ? fUt : nval) + sg.base + r;  // column-major: column k at A[k * ld]
+      const VT* X = (sg.far ? tm : xm) + (int64_t)sg.xoff * MM_W;
+      for (int kk = 16 * ksub; kk < sg.cnt; kk += 16 * nsplit)
+        mm_group<ST, VT>(acc, A, sg.ld, rok, X, kk, sg.cnt, g, i);
+    }
+  }
+  if (active && ksub > 0)
+#pragma unroll
+    for (int t = 0; t < 4; t++)
+#pragma unroll
+      for (int v = 0; v < 4; v++) red[(ksub - 1) * ns + strip][4 * g + v][16 * t + i] = acc.get(t, v);
+  __syncthreads();
+  if (active && ksub == 0) {
+#pragma unroll
+    for (int t = 0; t < 4; t++)
+#pragma unroll
+      for (int v = 0; v < 4; v++) {
+        const int rr = 4 * g + v, c = 16 * t + i;
+        VT sum = acc.get(t, v);
+        for (int q = 1; q < nsplit; q++) sum += red[(q - 1) * ns + strip][rr][c];
+        if (strip * 16 + rr < tl.len) ym[(int64_t)(tl.p0 + strip * 16 + rr) * MM_W + c] = sum;
+      }
+  }
+}
+
+// xm[p][r] = x[perm[p] + (r0 + r) ld] for r < nr, zero beyond
+template <class VT, class XT>
+__global__ void k_gather_mm(const int* __restrict__ perm, int n, const XT* __restrict__ x, int64_t ld, int r0, int nr,
+                            VT* __restrict__ xm) {
+  const int64_t idx = (int64_t)blockIdx.x * 256 + threadIdx.x;
+  if (idx >= (int64_t)n * MM_W) return;
+  const int p = (int)(idx / MM_W), r = (int)(idx % MM_W);
+  VT v = zero_<VT>();
+  if (r < nr) cvt(v, x[(int64_t)perm[p] + (int64_t)(r0 + r) * ld]);
+  xm[idx] = v;
+}
+
+template <class YT, class VT>
+__global__ void k_scatter_mm(const int* __restrict__ perm, int n, const VT* __restrict__ ym, YT* __restrict__ y,
+                             int64_t ld, int r0, int nr, YT alpha, YT beta, int beta_zero) {
+  const int64_t idx = (int64_t)blockIdx.x * 256 + threadIdx.x;
+  if (idx >= (int64_t)n * MM_W) return;
+  const int p = (int)(idx / MM_W), r = (int)(idx % MM_W);
+  if (r >= nr) return;
+  YT v;
+  cvt(v, ym[idx]);
+  YT* dst = y + (int64_t)perm[p] + (int64_t)(r0 + r) * ld;
+  *dst = beta_zero ? alpha * v : alpha * v + beta * *dst;
+}
+
 template <class S, class D>
 __global__ void k_convert(const S* __restrict__ a, D* __restrict__ b, int64_t n) {
   for (int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x; i < n; i += (int64_t)gridDim.x * 256) cvt(b[i], a[i]);
@@ -1838,6 +2051,69 @@ static void apply_t(Operator& op, int flags, const XT* x, XT* y, XT alpha, XT be
   hipLaunchKernelGGL((k_scatter<XT, VT>), dim3((nout + 255) / 256), dim3(256), 0, s, pout, nout, yp, y, alpha, beta,
                      (int)beta_zero, 0);
   mark(4);
+}
+
+// forward block apply, 64 right-hand sides per pass
+template <class ST, class VT, class XT>
+static void apply_mm_t(Operator& op, const XT* x, XT* y, int nrhs, int64_t ld, XT alpha, XT beta, bool beta_zero,
+                       const ST* nval, const ST* fU, const ST* fV, hipStream_t s) {
+  const int nin = op.ndof_trial(), nout = op.ndof_test();
+  const size_t vsz = sizeof(VT);
+  if (op.n_mm_items < 0) {
+    std::vector<MMItem> items;
+    for (size_t k = 0; k < op.far.size(); k++)
+      for (int l = 0; l < op.far[k].R; l += 16) items.push_back(MMItem{(int)k, l});
+    op.d_mm_items.upload(items, s);
+    BEM_HIP(hipStreamSynchronize(s));
+    op.n_mm_items = (int)items.size();
+  }
+  const size_t need_v = (size_t)std::max(nin, nout) * MM_W * vsz;
+  const size_t need_t = (size_t)std::max(op.t_total, 1) * MM_W * vsz;
+  if (op.d_xm.bytes < need_v) op.d_xm.alloc(need_v);
+  if (op.d_ym.bytes < need_v) op.d_ym.alloc(need_v);
+  if (op.d_tm.bytes < need_t) op.d_tm.alloc(need_t);
+  VT* xm = op.d_xm.as<VT>();
+  VT* ym = op.d_ym.as<VT>();
+  VT* tm = op.d_tm.as<VT>();
+  for (int r0 = 0; r0 < nrhs; r0 += MM_W) {
+    const int nr = std::min(MM_W, nrhs - r0);
+    hipLaunchKernelGGL((k_gather_mm<VT, XT>), dim3((unsigned)(((int64_t)nin * MM_W + 255) / 256)), dim3(256), 0, s,
+                       op.d_perm_trial.as<int>(), nin, x, ld, r0, nr, xm);
+    if (op.n_mm_items > 0)
+      hipLaunchKernelGGL((k_mm_far_t<ST, VT>), dim3(op.n_mm_items), dim3(256), 0, s, op.d_mm_items.as<MMItem>(),
+                         op.d_far_blk.as<FarBlk>(), fV, xm, tm);
+    hipLaunchKernelGGL((k_mm_rows<ST, VT>), dim3(op.n_tiles_fwd), dim3(256), 0, s, op.d_tile_fwd.as<Tile>(),
+                       op.d_seg_fwd.as<Seg>(), nval, fU, xm, tm, ym);
+    hipLaunchKernelGGL((k_scatter_mm<XT, VT>), dim3((unsigned)(((int64_t)nout * MM_W + 255) / 256)), dim3(256), 0, s,
+                       op.d_perm_test.as<int>(), nout, ym, y, ld, r0, nr, alpha, beta, (int)beta_zero);
+  }
+}
+
+void apply_device_mm(Operator& op, int flags, const double* alpha, const void* x, const double* beta, void* y,
+                     int nrhs, int64_t ld, hipStream_t s) {
+  if (!op.assembled) throw std::runtime_error("operator not assembled");
+  if (flags & 3) throw std::runtime_error("block apply is forward only");
+  const bool mixed = (flags & 4) != 0;
+  if (mixed && !op.mixed) throw std::runtime_error("operator was created without the mixed-precision copy");
+  const bool beta_zero = beta[0] == 0.0 && beta[1] == 0.0;
+  if (op.complex_valued) {
+    const zc a{alpha[0], alpha[1]}, b{beta[0], beta[1]};
+    if (!mixed)
+      apply_mm_t<zc, zc, zc>(op, (const zc*)x, (zc*)y, nrhs, ld, a, b, beta_zero, op.d_near_val.as<zc>(),
+                             op.d_fU.as<zc>(), op.d_fV.as<zc>(), s);
+    else
+      apply_mm_t<fc, fc, zc>(op, (const zc*)x, (zc*)y, nrhs, ld, a, b, beta_zero, op.d_near_val32.as<fc>(),
+                             op.d_fU32.as<fc>(), op.d_fV32.as<fc>(), s);
+  } else {
+    if (alpha[1] != 0.0 || beta[1] != 0.0) throw std::runtime_error("complex scalars on a real operator");
+    if (!mixed)
+      apply_mm_t<double, double, double>(op, (const double*)x, (double*)y, nrhs, ld, alpha[0], beta[0], beta_zero,
+                                         op.d_near_val.as<double>(), op.d_fU.as<double>(), op.d_fV.as<double>(), s);
+    else
+      apply_mm_t<float, float, double>(op, (const double*)x, (double*)y, nrhs, ld, alpha[0], beta[0], beta_zero,
+                                       op.d_near_val32.as<float>(), op.d_fU32.as<float>(), op.d_fV32.as<float>(), s);
+  }
+  BEM_HIP(hipGetLastError());
 }
 
 void apply_device(Operator& op, int flags, const double* alpha, const void* x, const double* beta, void* y,

@@ -148,6 +148,9 @@ struct Operator {
   int n_items_fwd = 0, n_items_trn = 0;
   int t_total = 0;         // scratch length for far B^T x
   DBuf d_xp, d_yp, d_t;    // scratch vectors (complex128 or double)
+  // block apply (64 right-hand sides): work items built on first use, slabs
+  DBuf d_mm_items, d_xm, d_ym, d_tm;
+  int n_mm_items = -1;
 
   int ndof_trial() const { return trial->ndof; }
   int ndof_test() const { return test->ndof; }
@@ -173,5 +176,9 @@ void make_mixed_copy(Operator& op, hipStream_t s);
 // the low-rank B^T x stage, the row/column kernel and the scatter
 void apply_device(Operator& op, int flags, const double* alpha, const void* x, const double* beta, void* y,
                   hipStream_t s, hipEvent_t* ev = nullptr);
+// forward y = alpha A x + beta y for nrhs columns (stride ld) on the matrix
+// cores, 64 columns per pass
+void apply_device_mm(Operator& op, int flags, const double* alpha, const void* x, const double* beta, void* y,
+                     int nrhs, int64_t ld, hipStream_t s);
 
 }  // namespace bem

@@ -92,7 +92,8 @@ class BaseMatrix:
             return ScaledMatrix(other, self)
         x = _as_vec(other, self._out_dtype(other if isinstance(other, torch.Tensor) else torch.from_numpy(other)),
                     device())
-        y = torch.zeros(self.height, dtype=x.dtype, device=x.device)
+        # a (k, width) batch of row vectors gives a (k, height) batch
+        y = torch.zeros(x.shape[:-1] + (self.height,), dtype=x.dtype, device=x.device)
         self.MultAdd(1.0, x, y)
         return y
 
@@ -203,11 +204,11 @@ class MassMatrix(BaseMatrix):
 
     def MultAdd(self, s, x, y):
         x = x.to(y.dtype)
-        y.index_add_(0, self.rows, (s * self.vals).to(y.dtype) * x[self.cols])
+        y.index_add_(-1, self.rows, (s * self.vals).to(y.dtype) * x[..., self.cols])
 
     def MultTransAdd(self, s, x, y):
         x = x.to(y.dtype)
-        y.index_add_(0, self.cols, (s * self.vals).to(y.dtype) * x[self.rows])
+        y.index_add_(-1, self.cols, (s * self.vals).to(y.dtype) * x[..., self.rows])
 
 
 # ---------------------------------------------------------------- HMatrix --
@@ -239,8 +240,10 @@ class HMatrix(BaseMatrix):
         op = self._op
         trans = flags & _capi.APPLY_TRANS
         nin, nout = (self.height, self.width) if trans else (self.width, self.height)
-        if x.shape[-1] != nin or y.shape[-1] != nout:
+        if x.shape[-1] != nin or y.shape[-1] != nout or x.dim() != y.dim() or x.dim() > 2 or (
+                x.dim() == 2 and x.shape[0] != y.shape[0]):
             raise ValueError(f"vector sizes {tuple(x.shape)}, {tuple(y.shape)} do not match {nout}x{nin}")
+        shape_out = y.shape
         if not y.is_cuda:
             raise ValueError("output vector must live on the GPU")
         if self._mixed:
@@ -251,11 +254,12 @@ class HMatrix(BaseMatrix):
             if torch.is_complex(y):
                 xr = _as_vec(x.real if torch.is_complex(x) else x, torch.float64, y.device)
                 xi = _as_vec(x.imag, torch.float64, y.device) if torch.is_complex(x) else None
-                yr = torch.zeros(nout, dtype=torch.float64, device=y.device)
-                op._call_apply(flags, 1.0, xr, 0.0, yr)
+                yr = torch.zeros(shape_out, dtype=torch.float64, device=y.device)
+                op._call_apply(flags, 1.0, xr.contiguous(), 0.0, yr)
                 res = yr.to(torch.complex128)
                 if xi is not None:
-                    yi = torch.zeros(nout, dtype=torch.float64, device=y.device)
+                    xi = xi.contiguous()
+                    yi = torch.zeros(shape_out, dtype=torch.float64, device=y.device)
                     op._call_apply(flags, 1.0, xi, 0.0, yi)
                     res = res + 1j * yi
                 y.add_(s * res)
@@ -339,15 +343,30 @@ class IntegralOperator:
         return st.as_dict()
 
     def _call_apply(self, flags, alpha, x, beta, y):
+        """x, y: device vectors, or batches (k, n) of k row vectors (the forward
+        product of a batch runs as one block product on the matrix cores)."""
         a = np.array([complex(alpha).real, complex(alpha).imag], dtype=np.float64)
         b = np.array([complex(beta).real, complex(beta).imag], dtype=np.float64)
         dtype = _capi.C128 if self.is_complex else _capi.F64
+        nrhs, ld = 1, 0
+        xb, yb = x, y
+        if x.dim() == 2:
+            nrhs, nin, nout = x.shape[0], x.shape[1], y.shape[1]
+            ld = max(nin, nout)
+            if nin != nout:  # one leading dimension for both sides
+                xb = torch.zeros(nrhs, ld, dtype=x.dtype, device=x.device)
+                xb[:, :nin] = x
+                yb = torch.zeros(nrhs, ld, dtype=y.dtype, device=y.device)
+                if b.any():
+                    yb[:, :nout] = y
         with torch.cuda.device(y.device):
             stream = torch.cuda.current_stream().cuda_stream
             _capi.check(_capi.lib().bem_op_apply(self._h, flags | _capi.APPLY_ON_DEVICE, _capi.dptr(a),
-                                                 ctypes.c_void_p(x.data_ptr()), _capi.dptr(b),
-                                                 ctypes.c_void_p(y.data_ptr()), 1, 0, dtype,
+                                                 ctypes.c_void_p(xb.data_ptr()), _capi.dptr(b),
+                                                 ctypes.c_void_p(yb.data_ptr()), nrhs, ld, dtype,
                                                  ctypes.c_void_p(stream)), "bem_op_apply")
+        if yb is not y:
+            y.copy_(yb[:, : y.shape[1]])
 
     # -- introspection (parity tests) --
     def num_blocks(self):
