@@ -1114,6 +1114,7 @@ struct MF<float> {
   __device__ __forceinline__ static V4 mma(float a, float b, V4 c) {
     return __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, c, 0, 0, 0);
   }
+  __device__ __forceinline__ static int row(int g, int v) { return 4 * g + v; }
 };
 template <>
 struct MF<double> {
@@ -1121,6 +1122,7 @@ struct MF<double> {
   __device__ __forceinline__ static V4 mma(double a, double b, V4 c) {
     return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0);
   }
+  __device__ __forceinline__ static int row(int g, int v) { return 4 * v + g; }
 };
 
 // 16 rows x 64 columns of a wave's partial result
@@ -1145,6 +1147,7 @@ struct MAcc<cx<R>> {
     }
   }
   __device__ __forceinline__ cx<R> get(int t, int v) const { return {re[t][v], im[t][v]}; }
+  __device__ __forceinline__ static int row(int g, int v) { return MF<R>::row(g, v); }
 };
 template <class R>
 struct MAccReal {
@@ -1160,6 +1163,7 @@ struct MAccReal {
     for (int t = 0; t < 4; t++) re[t] = MF<R>::mma(a, b[t], re[t]);
   }
   __device__ __forceinline__ R get(int t, int v) const { return re[t][v]; }
+  __device__ __forceinline__ static int row(int g, int v) { return MF<R>::row(g, v); }
 };
 template <>
 struct MAcc<double> : MAccReal<double> {};
@@ -1215,14 +1219,14 @@ __global__ __launch_bounds__(256) void k_mm_far_t(const MMItem* __restrict__ ite
 #pragma unroll
     for (int t = 0; t < 4; t++)
 #pragma unroll
-      for (int v = 0; v < 4; v++) red[wv - 1][4 * g + v][16 * t + i] = acc.get(t, v);
+      for (int v = 0; v < 4; v++) red[wv - 1][MAcc<VT>::row(g, v)][16 * t + i] = acc.get(t, v);
   __syncthreads();
   if (wv == 0) {
 #pragma unroll
     for (int t = 0; t < 4; t++)
 #pragma unroll
       for (int v = 0; v < 4; v++) {
-        const int r = 4 * g + v, c = 16 * t + i;
+        const int r = MAcc<VT>::row(g, v), c = 16 * t + i;
         if (it.l0 + r < b.R)
           tm[(int64_t)(b.toff + it.l0 + r) * MM_W + c] = acc.get(t, v) + red[0][r][c] + red[1][r][c] + red[2][r][c];
       }
@@ -1269,14 +1273,14 @@ __global__ __launch_bounds__(256) void k_mm_rows(const Tile* __restrict__ tiles,
 #pragma unroll
     for (int t = 0; t < 4; t++)
 #pragma unroll
-      for (int v = 0; v < 4; v++) red[(ksub - 1) * ns + strip][4 * g + v][16 * t + i] = acc.get(t, v);
+      for (int v = 0; v < 4; v++) red[(ksub - 1) * ns + strip][MAcc<VT>::row(g, v)][16 * t + i] = acc.get(t, v);
   __syncthreads();
   if (active && ksub == 0) {
 #pragma unroll
     for (int t = 0; t < 4; t++)
 #pragma unroll
       for (int v = 0; v < 4; v++) {
-        const int rr = 4 * g + v, c = 16 * t + i;
+        const int rr = MAcc<VT>::row(g, v), c = 16 * t + i;
         VT sum = acc.get(t, v);
         for (int q = 1; q < nsplit; q++) sum += red[(q - 1) * ns + strip][rr][c];
         if (strip * 16 + rr < tl.len) ym[(int64_t)(tl.p0 + strip * 16 + rr) * MM_W + c] = sum;
