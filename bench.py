@@ -71,11 +71,16 @@ def main():
                     help="end with one plain 512 MiB device copy (calibrates rocprof FETCH_SIZE/WRITE_SIZE)")
     ap.add_argument("--traffic", type=str, default=None,
                     help="JSON file with measured HBM bytes per k_mv_rows launch (from the PMC passes)")
+    ap.add_argument("--workload", choices=["helmholtz_cf", "maxwell_efie_64rhs"], default="helmholtz_cf",
+                    help="helmholtz_cf: the headline; maxwell_efie_64rhs: block product of 64 right-hand sides")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only (asked for --gpus {args.gpus}); the multi-GPU sharding is a design "
               f"note, not part of this build", file=sys.stderr)
         sys.exit(2)
+    if args.workload == "maxwell_efie_64rhs":
+        bench_maxwell_block(args)
+        return
 
     import torch
 
@@ -244,6 +249,90 @@ def main():
         torch.cuda.synchronize()
         b.copy_(a)
         torch.cuda.synchronize()
+    print(json.dumps(out), flush=True)
+
+
+def bench_maxwell_block(args):
+    """Maxwell EFIE single layer, HDivSurface p0, kappa 5, intorder 4,
+    leafsize 40, eta 2, ACA eps 1e-5; one step = one block product of 64
+    right-hand sides (MFMA path), FP64 and FP32 copies, against 64 vector
+    products."""
+    import torch
+
+    import ngbem_amd as nb
+
+    dev = torch.device("cuda", 0)
+    torch.cuda.set_device(dev)
+    level = args.level
+    kappa = 5.0
+    nrhs = 64
+    mesh = nb.icosphere(level)
+    hd = nb.HDivSurface(mesh)
+    params = dict(intorder=4, leafsize=40, eta=2.0, eps=1e-5)
+    log(f"bench maxwell: icosphere L{level}, {hd.ndof} RWG dofs, kappa {kappa}, {params}, {nrhs} rhs")
+    torch.zeros(1, device=dev)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    V = nb.MaxwellSingleLayerPotentialOperator(hd, kappa, mixed=True, **params)
+    torch.cuda.synchronize()
+    t_assembly = time.perf_counter() - t0
+    st = V.stats()
+    log(f"assembly {t_assembly:.3f} s  near {st['t_near']:.3f} s  far {st['t_far']:.3f} s  "
+        f"blocks near/far {st['n_near']}/{st['n_far']}  max rank {st['max_rank']}")
+    g = torch.Generator(device="cpu").manual_seed(0)
+    X = torch.randn(nrhs, hd.ndof, dtype=torch.complex128, generator=g).to(dev)
+    Y = torch.zeros_like(X)
+
+    def timed(fn, steps, warmup):
+        for _ in range(warmup):
+            fn()
+        torch.cuda.synchronize()
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        e0.record()
+        for _ in range(steps):
+            fn()
+        e1.record()
+        torch.cuda.synchronize()
+        return e0.elapsed_time(e1) / 1e3 / steps
+
+    t64 = timed(lambda: V.mat.Mult(X, Y), args.steps, args.warmup)
+    t32 = timed(lambda: V.mat.mixed.Mult(X, Y), args.steps, args.warmup)
+    y = torch.zeros(hd.ndof, dtype=torch.complex128, device=dev)
+
+    def per_vector(M):
+        for r in range(nrhs):
+            M.Mult(X[r], y)
+
+    tv64 = timed(lambda: per_vector(V.mat), max(2, args.steps // 10), 1)
+    tv32 = timed(lambda: per_vector(V.mat.mixed), max(2, args.steps // 10), 1)
+    Y64 = V.mat * X
+    Y32 = V.mat.mixed * X
+    err = (torch.linalg.norm(Y64 - Y32) / torch.linalg.norm(Y64)).item()
+    entries = st["near_entries"] + st["far_factor_entries"]
+    flops = 8.0 * nrhs * entries  # complex multiply-add = 8 real flops
+    log(f"block apply fp64 {t64 * 1e3:.3f} ms ({flops / t64 / 1e12:.1f} TFLOP/s)  "
+        f"fp32 {t32 * 1e3:.3f} ms ({flops / t32 / 1e12:.1f} TFLOP/s); 64 vector applies fp64 {tv64 * 1e3:.3f} ms "
+        f"fp32 {tv32 * 1e3:.3f} ms; fp32 vs fp64 rel err {err:.2e}")
+    out = {
+        "metric": "block H-matvec/s (64 right-hand sides), Maxwell EFIE single layer",
+        "value": 1.0 / t32, "unit": "block H-matvec/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
+        "ms_per_step": t32 * 1e3, "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+        "dtype": "c64", "data": "synthetic",
+        "config": {"workload": f"Maxwell single layer, icosphere L{level} ({mesh.nt} triangles, {hd.ndof} "
+                               f"HDivSurface p0 dofs), kappa 5, intorder 4, leafsize 40, eta 2, ACA eps 1e-5, "
+                               f"{nrhs} right-hand sides per product"},
+        "assembly_s": t_assembly,
+        "fp64_block_per_s": 1.0 / t64,
+        "per_vector_ms": {"fp64": tv64 * 1e3, "fp32": tv32 * 1e3},
+        "block_vs_vector_speedup": {"fp64": tv64 / t64, "fp32": tv32 / t32},
+        "fp32_vs_fp64_rel_err": err,
+        "roofline": {"bound": "mfma", "achieved": flops / t32 / 1e12, "peak": 157.3, "unit": "TFLOP/s",
+                     "frac": flops / t32 / 1e12 / 157.3, "traffic": None,
+                     "flops_per_product": flops, "fp64_achieved": flops / t64 / 1e12, "fp64_peak": 78.6,
+                     "note": "whole block product (gather, V X, row tiles, scatter) timed"},
+        "hmatrix": {"near_entries": st["near_entries"], "far_factor_entries": st["far_factor_entries"],
+                    "n_near": st["n_near"], "n_far": st["n_far"], "max_rank": st["max_rank"]},
+    }
     print(json.dumps(out), flush=True)
 
 

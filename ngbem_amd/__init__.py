@@ -15,11 +15,11 @@ from .operators import (BaseMatrix, HMatrix, IntegralOperator, IntegralOperatorC
                         HelmholtzCombinedFieldOperator, HelmholtzHypersingularOperator,
                         MaxwellSingleLayerPotentialOperator, MaxwellSingleLayerPotentialOperatorCurl,
                         MaxwellDoubleLayerPotentialOperator)
-from .krylov import CG, GMRes, RefinedSolve
+from .krylov import CG, GMRes, GMResBatch, RefinedSolve
 
 __all__ = ["Mesh", "icosphere", "ellipsoid", "SurfaceL2", "H1", "HDivSurface", "HCurl", "BaseMatrix", "HMatrix", "IntegralOperator",
            "IntegralOperatorC", "MassMatrix", "SingleLayerPotentialOperator", "DoubleLayerPotentialOperator",
            "HypersingularOperator", "HelmholtzSingleLayerPotentialOperator", "HelmholtzDoubleLayerPotentialOperator",
            "HelmholtzCombinedFieldOperator", "HelmholtzHypersingularOperator", "MaxwellSingleLayerPotentialOperator",
-           "MaxwellSingleLayerPotentialOperatorCurl", "MaxwellDoubleLayerPotentialOperator", "CG", "GMRes",
+           "MaxwellSingleLayerPotentialOperatorCurl", "MaxwellDoubleLayerPotentialOperator", "CG", "GMRes", "GMResBatch",
            "RefinedSolve"]

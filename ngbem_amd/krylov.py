@@ -23,7 +23,8 @@ def _dot(a, b):
 
 def _apply(A, x):
     if isinstance(A, BaseMatrix):
-        y = torch.zeros(A.height, dtype=torch.complex128 if (A.is_complex or torch.is_complex(x)) else torch.float64,
+        y = torch.zeros(x.shape[:-1] + (A.height,),
+                        dtype=torch.complex128 if (A.is_complex or torch.is_complex(x)) else torch.float64,
                         device=x.device)
         A.MultAdd(1.0, x, y)
         return y
@@ -145,6 +146,92 @@ def GMRes(A, b, pre=None, freedofs=None, x=None, maxsteps=100, tol=None, innerpr
             break
     GMRes.iterations = total
     return x
+
+
+def GMResBatch(A, B, pre=None, maxsteps=100, tol=1e-7, restart=None, printrates=False):
+    """Right-preconditioned GMRES for the k right-hand sides in the rows of B
+    (k x n), each with its own Krylov space, all advanced together: every step
+    applies A to the k new basis vectors as one block product (the matrix-core
+    path of the H-matrix) and orthogonalises them with batched CGS2.  Stops when
+    every row has ||r_i|| <= tol ||b_i||; returns X (k x n)."""
+    dt = torch.complex128 if (torch.is_complex(B) or getattr(A, "is_complex", False)) else torch.float64
+    B = B.to(dt)
+    k, n = B.shape
+    X = torch.zeros_like(B)
+    m = min(restart or maxsteps, maxsteps)
+    bnorm = torch.linalg.norm(B, dim=1).cpu().numpy()
+    bnorm[bnorm == 0] = 1.0
+    V = torch.empty(m + 1, k, n, dtype=dt, device=B.device)
+    Z = torch.empty(m, k, n, dtype=dt, device=B.device) if pre is not None else None
+    total = 0
+    res = np.full(k, np.inf)
+    while total < maxsteps:
+        R = B - _apply(A, X)
+        beta = torch.linalg.norm(R, dim=1)
+        beta_h = beta.cpu().numpy()
+        if np.all(beta_h <= tol * bnorm):
+            res = beta_h / bnorm
+            break
+        V[0] = R / torch.clamp(beta, min=1e-300)[:, None]
+        H = np.zeros((k, m + 1, m), dtype=np.complex128)
+        cs = np.zeros((k, m), dtype=np.complex128)
+        sn = np.zeros((k, m), dtype=np.complex128)
+        g = np.zeros((k, m + 1), dtype=np.complex128)
+        g[:, 0] = beta_h
+        used = 0
+        for j in range(m):
+            z = _apply(pre, V[j]) if pre is not None else V[j]
+            if Z is not None:
+                Z[j] = z
+            w = _apply(A, z)
+            Vk = V[: j + 1].transpose(0, 1).contiguous()  # k x (j+1) x n
+            h = torch.bmm(Vk.conj(), w.unsqueeze(2)).squeeze(2)  # k x (j+1)
+            w = w - torch.bmm(h.unsqueeze(1), Vk).squeeze(1)
+            h2 = torch.bmm(Vk.conj(), w.unsqueeze(2)).squeeze(2)
+            w = w - torch.bmm(h2.unsqueeze(1), Vk).squeeze(1)
+            hn_t = torch.linalg.norm(w, dim=1)
+            hcol = torch.cat([h + h2, hn_t.to(h.dtype)[:, None]], 1).cpu().numpy()
+            hn = hcol[:, j + 1].real.copy()
+            H[:, : j + 2, j] = hcol
+            for i in range(j):
+                hi, hj = H[:, i, j].copy(), H[:, i + 1, j].copy()
+                H[:, i, j] = np.conj(cs[:, i]) * hi + np.conj(sn[:, i]) * hj
+                H[:, i + 1, j] = -sn[:, i] * hi + cs[:, i] * hj
+            a, bb = H[:, j, j].copy(), H[:, j + 1, j].copy()
+            t = np.hypot(np.abs(a), np.abs(bb))
+            ok = t > 0
+            cs[:, j] = np.where(ok, a / np.where(ok, t, 1.0), 1.0)
+            sn[:, j] = np.where(ok, bb / np.where(ok, t, 1.0), 0.0)
+            H[:, j, j] = t
+            H[:, j + 1, j] = 0.0
+            g[:, j + 1] = -sn[:, j] * g[:, j]
+            g[:, j] = np.conj(cs[:, j]) * g[:, j]
+            used = j + 1
+            total += 1
+            res = np.abs(g[:, j + 1]) / bnorm
+            if printrates:
+                print(f"GMResBatch iteration {total}, max relative residual = {res.max():.3e}")
+            if np.all(res <= tol) or total >= maxsteps:
+                break
+            V[j + 1] = w / torch.from_numpy(np.maximum(hn, 1e-300)).to(w.device)[:, None]
+        Y = np.zeros((k, used), dtype=np.complex128)
+        for i in range(k):
+            Hi = np.triu(H[i, :used, :used])
+            d = np.abs(np.diag(Hi))
+            if np.all(d > 0):
+                Y[i] = np.linalg.solve(Hi, g[i, :used])
+            else:  # converged early (exact breakdown): use the leading nonsingular part
+                q = int(np.argmax(d == 0))
+                if q:
+                    Y[i, :q] = np.linalg.solve(Hi[:q, :q], g[i, :q])
+        Yt = torch.from_numpy(Y if torch.is_complex(X) else Y.real.copy()).to(device=X.device, dtype=dt)
+        basis = (Z if Z is not None else V)[:used].transpose(0, 1).contiguous()  # k x used x n
+        X.add_(torch.bmm(Yt.unsqueeze(1), basis).squeeze(1))
+        if np.all(res <= tol):
+            break
+    GMResBatch.iterations = total
+    GMResBatch.residuals = res
+    return X
 
 
 def RefinedSolve(A, A_inner, b, pre=None, tol=1e-10, inner_tol=1e-5, maxouter=20, maxsteps=500, printrates=False):

@@ -1,0 +1,53 @@
+"""Krylov drivers on CPU tensors with a dense stand-in operator."""
+import numpy as np
+import torch
+
+import ngbem_amd.krylov as K
+from ngbem_amd.operators import BaseMatrix
+
+
+class Dense(BaseMatrix):
+    def __init__(self, A):
+        self.A = A
+        self.height, self.width = A.shape
+        self.is_complex = torch.is_complex(A)
+
+    def MultAdd(self, s, x, y):
+        y.add_(s * (x @ self.A.T))
+
+
+def _system(n=60, seed=0):
+    rng = np.random.default_rng(seed)
+    A = np.eye(n) * 3 + (rng.standard_normal((n, n)) + 1j * rng.standard_normal((n, n))) / np.sqrt(n)
+    return torch.from_numpy(A), rng
+
+
+def test_gmres_batch_matches_direct_solve():
+    A, rng = _system()
+    B = torch.from_numpy(rng.standard_normal((7, 60)) + 1j * rng.standard_normal((7, 60)))
+    X_ref = torch.linalg.solve(A, B.T).T
+    for kw in (dict(), dict(restart=10, pre=lambda r: r / 3.0)):
+        X = K.GMResBatch(Dense(A), B, tol=1e-12, maxsteps=200, **kw)
+        assert (torch.linalg.norm(X - X_ref) / torch.linalg.norm(X_ref)).item() < 1e-10
+        assert np.all(K.GMResBatch.residuals <= 1e-12)
+
+
+def test_gmres_batch_equals_single_gmres():
+    A, rng = _system(seed=1)
+    B = torch.from_numpy(rng.standard_normal((3, 60)) + 0j)
+    X = K.GMResBatch(Dense(A), B, tol=1e-13, maxsteps=100)
+    for r in range(3):
+        x = K.GMRes(Dense(A), B[r], tol=1e-13, maxsteps=100, printrates=False)
+        assert (torch.linalg.norm(X[r] - x) / torch.linalg.norm(x)).item() < 1e-11
+
+
+def test_cg_and_refined_solve():
+    rng = np.random.default_rng(2)
+    Q = rng.standard_normal((40, 40))
+    A = torch.from_numpy(Q @ Q.T + 40 * np.eye(40))
+    b = torch.from_numpy(rng.standard_normal(40))
+    x = K.CG(Dense(A), b, tol=1e-12, maxsteps=200, printrates=False)
+    assert (torch.linalg.norm(A @ x - b) / torch.linalg.norm(b)).item() < 1e-11
+    A32 = Dense(A.to(torch.float32).to(torch.float64))
+    x2, hist = K.RefinedSolve(Dense(A), A32, b, tol=1e-12, inner_tol=1e-4)
+    assert hist[-1] <= 1e-12

@@ -228,3 +228,85 @@ def test_maxwell_dipole_convergence():
     # measured 0.059 and 0.016: second order in h
     assert errs[0] < 0.08 and errs[1] < 0.025
     assert errs[1] < 0.35 * errs[0]
+
+
+def test_mie_series_checks():
+    """The vector-spherical-wave expansion reproduces the plane wave and its
+    curl, and the total tangential field vanishes on the conducting sphere."""
+    import mie_maxwell as M
+
+    rng = np.random.default_rng(0)
+    k = 5.0
+    X = rng.standard_normal((40, 3))
+    X *= (0.3 + 1.2 * rng.random(40))[:, None] / np.linalg.norm(X, axis=1)[:, None]
+    E, C = M.fields(X, k, parts=("inc",))
+    assert np.abs(E[:, 0] - np.exp(1j * k * X[:, 2])).max() < 1e-9
+    assert np.abs(E[:, 1:]).max() < 1e-9
+    assert np.abs(C[:, 1] - 1j * k * np.exp(1j * k * X[:, 2])).max() < 1e-8  # curl(x e^{ikz})
+    S = rng.standard_normal((40, 3))
+    S /= np.linalg.norm(S, axis=1)[:, None]
+    E, _ = M.fields(S, k)
+    assert np.abs(E - (E * S).sum(1)[:, None] * S).max() < 1e-12
+    Y, h = 1.3 * S[:4], 1e-5
+    _, C0 = M.fields(Y, k)
+    J = np.zeros((4, 3, 3), complex)
+    for c in range(3):
+        d = np.zeros(3)
+        d[c] = h
+        J[:, :, c] = (M.fields(Y + d, k)[0] - M.fields(Y - d, k)[0]) / (2 * h)
+    curl = np.stack([J[:, 2, 1] - J[:, 1, 2], J[:, 0, 2] - J[:, 2, 0], J[:, 1, 0] - J[:, 0, 1]], 1)
+    assert np.abs(curl - C0).max() < 1e-7 * np.abs(C0).max()
+    # rotated frame: wave p exp(i k d.x) seen from its own frame
+    d, p = np.array([0.3, -0.4, 0.866]), np.array([1.0, 0.5, 0.0])
+    R = M.rotation_to_z(d, p)
+    assert np.allclose(R @ R.T, np.eye(3)) and np.allclose(R @ (d / np.linalg.norm(d)), [0, 0, 1])
+
+
+def efie_plane_wave_rhs(hd, k, d, p, order=6):
+    """-int p exp(i k d.x) . phi_i over every triangle (rows: waves)."""
+    mesh = hd.mesh
+    rule = O.trig_rule(order)
+    bary = np.stack([rule[:, 0], rule[:, 1], 1 - rule[:, 0] - rule[:, 1]], 1)
+    w = rule[:, 2][None, :] * 2 * mesh.areas()[:, None]
+    X = np.einsum("qk,tkc->tqc", bary, mesh.xyz[mesh.tri])
+    phi = hd.rt_shapes(bary)
+    out = np.zeros((len(d), hd.ndof), complex)
+    for r in range(len(d)):
+        E = p[r][None, None, :] * np.exp(1j * k * (X @ d[r]))[..., None]
+        np.add.at(out[r], hd.elem_dofs, -np.einsum("tq,tqc,tqlc->tl", w, E, phi))
+    return out
+
+
+def efie_current_error(hd, J, k, d, p, order=6):
+    """Relative L2 error of the RT currents (rows of J) against the Mie current."""
+    import mie_maxwell as M
+
+    mesh = hd.mesh
+    rule = O.trig_rule(order)
+    bary = np.stack([rule[:, 0], rule[:, 1], 1 - rule[:, 0] - rule[:, 1]], 1)
+    w = rule[:, 2][None, :] * 2 * mesh.areas()[:, None]
+    X = np.einsum("qk,tkc->tqc", bary, mesh.xyz[mesh.tri])
+    Xs = (X / np.linalg.norm(X, axis=-1)[..., None]).reshape(-1, 3)
+    phi = hd.rt_shapes(bary)
+    errs = []
+    for r in range(len(d)):
+        jex = M.current_rotated(Xs, k, d[r], p[r]).reshape(X.shape)
+        jh = np.einsum("tl,tqlc->tqc", J[r][hd.elem_dofs], phi)
+        errs.append(np.sqrt(np.sum(w * (np.abs(jh - jex) ** 2).sum(-1)) / np.sum(w * (np.abs(jex) ** 2).sum(-1))))
+    return np.array(errs)
+
+
+def test_efie_vs_mie_oracle():
+    """EFIE V j = -<E_inc, phi> with the oracle's dense single layer: the RT
+    current converges to the Mie current at first order (0.29, 0.14 at L1, L2)."""
+    k = 2.0
+    d, p = np.array([[0.0, 0.0, 1.0], [0.6, 0.0, 0.8]]), np.array([[1.0, 0.0, 0.0], [0.0, 1.0, 0.0]])
+    errs = []
+    for lvl in (1, 2):
+        mesh = nb.icosphere(lvl)
+        hd = nb.HDivSurface(mesh)
+        V = O.OracleOp(mesh, O.MAXWELL_SL, k, O.HDIV, O.HDIV, intorder=4).dense()
+        J = np.linalg.solve(V, efie_plane_wave_rhs(hd, k, d, p).T).T
+        errs.append(efie_current_error(hd, J, k, d, p))
+    assert np.all(errs[0] < 0.32) and np.all(errs[1] < 0.16)
+    assert np.all(errs[1] < 0.55 * errs[0])
