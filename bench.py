@@ -69,8 +69,9 @@ def main():
     ap.add_argument("--no-solve", action="store_true")
     ap.add_argument("--calib-copy", action="store_true",
                     help="end with one plain 512 MiB device copy (calibrates rocprof FETCH_SIZE/WRITE_SIZE)")
-    ap.add_argument("--traffic", type=str, default=None,
-                    help="JSON file with measured HBM bytes per k_mv_rows launch (from the PMC passes)")
+    ap.add_argument("--traffic", type=str,
+                    default=os.path.join(ROOT, "profiles", "k_mv_rows_helmholtz_cf_L6_traffic.md"),
+                    help="summary of the PMC passes with a 'bytes_per_launch:' line for k_mv_rows<c64>")
     ap.add_argument("--workload", choices=["helmholtz_cf", "maxwell_efie_64rhs"], default="helmholtz_cf",
                     help="helmholtz_cf: the headline; maxwell_efie_64rhs: block product of 64 right-hand sides")
     args = ap.parse_args()
@@ -239,10 +240,11 @@ def main():
     if not args.no_cpu_baseline:
         out["cpu_baseline"] = cpu_baseline(C, mesh, kappa, params, st)
 
-    if args.traffic and os.path.exists(args.traffic):
-        tr = json.load(open(args.traffic))
-        out["roofline"]["traffic"] = tr.get("bytes_per_launch")
-        out["roofline"]["traffic_note"] = tr.get("note")
+    if args.traffic and os.path.exists(args.traffic) and args.level == 6 and kappa == 10.0:
+        for line in open(args.traffic):
+            if line.startswith("bytes_per_launch:"):
+                out["roofline"]["traffic"] = float(line.split(":")[1])
+                out["roofline"]["traffic_source"] = os.path.relpath(args.traffic, ROOT)
     if args.calib_copy:
         a = torch.empty(512 << 20, dtype=torch.uint8, device=dev).fill_(1)
         b = torch.empty_like(a)
