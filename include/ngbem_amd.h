@@ -34,6 +34,7 @@ extern "C" {
 typedef struct bem_mesh bem_mesh;
 typedef struct bem_space bem_space;
 typedef struct bem_op bem_op;
+typedef struct bem_batch bem_batch;
 
 /* space kinds (src/python_bem.cpp callers pass NGSolve spaces of these types) */
 enum {
@@ -58,7 +59,10 @@ enum {
   BEM_HELMHOLTZ_CF = 5,
   BEM_HELMHOLTZ_HS = 6,
   BEM_MAXWELL_SL = 7,
-  BEM_MAXWELL_DL = 8
+  BEM_MAXWELL_DL = 8,
+  /* d/dkappa of the Helmholtz single layer kernel, i exp(i kappa r)/(4 pi):
+   * no counterpart in the reference; the kappa-gradient of V(kappa) x */
+  BEM_HELMHOLTZ_SL_DKAPPA = 9
 };
 
 /* apply flags */
@@ -67,7 +71,8 @@ enum {
   BEM_APPLY_CONJ = 2,      /* with TRANS: A^H (adjoint) */
   BEM_APPLY_MIXED = 4,     /* use the FP32 / complex64 copy of the operator */
   BEM_APPLY_ON_DEVICE = 8, /* x and y are device pointers (else host pointers) */
-  BEM_APPLY_PER_VECTOR = 16 /* nrhs > 1: one vector at a time instead of the block product */
+  BEM_APPLY_PER_VECTOR = 16, /* nrhs > 1: one vector at a time instead of the block product */
+  BEM_BATCH_EAGER = 32        /* bem_batch_apply: launch the kernels directly, no graph replay */
 };
 
 /* vector dtypes */
@@ -145,6 +150,21 @@ int bem_op_block_data(const bem_op* op, int b, void* out);
 /* cluster tree: which 0 trial, 1 test; perm[ndof]; cl[7*ncl] = level,parent,child1,child2,
  * number,permupos,radius; cen[3*ncl].  Returns the cluster count. */
 int bem_op_tree(const bem_op* op, int which, int32_t* perm, double* cl, double* cen, int cap);
+
+/* Batches: the block-diagonal union of n assembled operators (all real or all
+ * complex), e.g. many small geometries.  The storage is copied into one set of
+ * arrays and one matvec plan covers every instance, so an apply is one launch
+ * of each kernel for the whole batch, captured once per flag combination in a
+ * hipGraph and replayed.  Vectors are the concatenation of the instances'
+ * vectors (device pointers).  The operators may be destroyed afterwards.
+ * No counterpart in the reference, which assembles and applies one operator
+ * per GenericIntegralOperator (src/ngbem.cpp:314-330, src/hmat.cpp:632-666).
+ * One stream at a time per batch. */
+int bem_batch_create(bem_op* const* ops, int n, bem_batch** out);
+int bem_batch_sizes(const bem_batch* b, int* width, int* height, int* is_complex);
+int bem_batch_apply(bem_batch* b, int flags, const double* alpha, const void* x, const double* beta, void* y,
+                    int dtype, void* stream);
+int bem_batch_destroy(bem_batch* b);
 
 #ifdef __cplusplus
 }

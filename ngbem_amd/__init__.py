@@ -14,12 +14,14 @@ from .operators import (BaseMatrix, HMatrix, IntegralOperator, IntegralOperatorC
                         HelmholtzSingleLayerPotentialOperator, HelmholtzDoubleLayerPotentialOperator,
                         HelmholtzCombinedFieldOperator, HelmholtzHypersingularOperator,
                         MaxwellSingleLayerPotentialOperator, MaxwellSingleLayerPotentialOperatorCurl,
-                        MaxwellDoubleLayerPotentialOperator)
+                        MaxwellDoubleLayerPotentialOperator, HelmholtzSingleLayerPotentialOperatorDKappa)
 from .krylov import CG, GMRes, GMResBatch, RefinedSolve
+from .batch import OperatorBatch, SingleLayerBatch, assemble_many
 
 __all__ = ["Mesh", "icosphere", "ellipsoid", "SurfaceL2", "H1", "HDivSurface", "HCurl", "BaseMatrix", "HMatrix", "IntegralOperator",
            "IntegralOperatorC", "MassMatrix", "SingleLayerPotentialOperator", "DoubleLayerPotentialOperator",
            "HypersingularOperator", "HelmholtzSingleLayerPotentialOperator", "HelmholtzDoubleLayerPotentialOperator",
            "HelmholtzCombinedFieldOperator", "HelmholtzHypersingularOperator", "MaxwellSingleLayerPotentialOperator",
-           "MaxwellSingleLayerPotentialOperatorCurl", "MaxwellDoubleLayerPotentialOperator", "CG", "GMRes", "GMResBatch",
-           "RefinedSolve"]
+           "MaxwellSingleLayerPotentialOperatorCurl", "MaxwellDoubleLayerPotentialOperator",
+           "HelmholtzSingleLayerPotentialOperatorDKappa", "CG", "GMRes", "GMResBatch", "RefinedSolve",
+           "OperatorBatch", "SingleLayerBatch", "assemble_many"]

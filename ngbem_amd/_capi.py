@@ -13,8 +13,8 @@ _lib = None
 
 SPACE_P0, SPACE_P1, SPACE_HDIV, SPACE_HCURL = 0, 1, 2, 3
 (LAPLACE_SL, LAPLACE_DL, LAPLACE_HS, HELMHOLTZ_SL, HELMHOLTZ_DL, HELMHOLTZ_CF, HELMHOLTZ_HS, MAXWELL_SL,
- MAXWELL_DL) = range(9)
-APPLY_TRANS, APPLY_CONJ, APPLY_MIXED, APPLY_ON_DEVICE, APPLY_PER_VECTOR = 1, 2, 4, 8, 16
+ MAXWELL_DL, HELMHOLTZ_SL_DKAPPA) = range(10)
+APPLY_TRANS, APPLY_CONJ, APPLY_MIXED, APPLY_ON_DEVICE, APPLY_PER_VECTOR, BATCH_EAGER = 1, 2, 4, 8, 16, 32
 F64, C128 = 0, 1
 
 
@@ -50,7 +50,8 @@ class BlockInfo(ctypes.Structure):
 EXPORTS = ["bem_last_error", "bem_device_count", "bem_mesh_create", "bem_mesh_destroy", "bem_space_create",
            "bem_space_ndof", "bem_space_destroy", "bem_op_create", "bem_op_assemble", "bem_op_apply",
            "bem_op_stats", "bem_op_profile_apply", "bem_op_destroy", "bem_op_num_blocks", "bem_op_block_info", "bem_op_block_dofs",
-           "bem_op_block_data", "bem_op_tree"]
+           "bem_op_block_data", "bem_op_tree", "bem_batch_create", "bem_batch_sizes", "bem_batch_apply",
+           "bem_batch_destroy"]
 
 
 def lib_path() -> str:
@@ -86,6 +87,11 @@ def lib():
     L.bem_op_block_dofs.argtypes = [P, ctypes.c_int, ip, ip]
     L.bem_op_block_data.argtypes = [P, ctypes.c_int, P]
     L.bem_op_tree.argtypes = [P, ctypes.c_int, ip, dp, dp, ctypes.c_int]
+    L.bem_batch_create.argtypes = [ctypes.POINTER(P), ctypes.c_int, ctypes.POINTER(P)]
+    L.bem_batch_sizes.argtypes = [P, ctypes.POINTER(ctypes.c_int), ctypes.POINTER(ctypes.c_int),
+                                  ctypes.POINTER(ctypes.c_int)]
+    L.bem_batch_apply.argtypes = [P, ctypes.c_int, dp, P, dp, P, ctypes.c_int, P]
+    L.bem_batch_destroy.argtypes = [P]
     _lib = L
     return L
 

@@ -22,6 +22,17 @@ struct bem_space {
 struct bem_op {
   bem::Operator op;
 };
+struct bem_batch {
+  bem::Operator op;                 // the merged operator
+  bem::DBuf dx, dy;                 // graph-owned input / output vectors
+  hipStream_t cs = nullptr;         // capture stream (default priority)
+  hipGraphExec_t exec[8] = {};      // per flag combination (trans, conj, mixed)
+  ~bem_batch() {
+    for (auto& e : exec)
+      if (e) (void)hipGraphExecDestroy(e);
+    if (cs) (void)hipStreamDestroy(cs);
+  }
+};
 
 namespace {
 thread_local std::string g_err;
@@ -140,7 +151,7 @@ int bem_op_create(int kernel, double kappa, bem_space* trial, bem_space* test, c
   {
     const int tk = trial->s->kind, sk = test->s->kind;
     const bool scalar = tk <= bem::SP_P1 && sk <= bem::SP_P1;
-    if (kernel <= bem::K_HHS && !scalar)
+    if ((kernel <= bem::K_HHS || kernel == bem::K_HSLK) && !scalar)
       return fail(-1, "bem_op_create: Laplace/Helmholtz kernels need SurfaceL2 / H1 spaces");
     if (kernel == bem::K_MSL && !((tk == bem::SP_RWG && sk == bem::SP_RWG) || (tk == bem::SP_NED && sk == bem::SP_NED)))
       return fail(-1, "bem_op_create: Maxwell single layer needs HDivSurface (or HCurl for the curl form) on both sides");
@@ -445,3 +456,78 @@ int bem_op_tree(const bem_op* h, int which, int32_t* perm, double* cl, double* c
 }
 
 }  // extern "C"
+
+// ------------------------------------------------------------------ batch --
+
+int bem_batch_create(bem_op* const* ops, int n, bem_batch** out) {
+  if (!ops || n < 1 || !out) return fail(-1, "bem_batch_create: bad arguments");
+  for (int i = 0; i < n; i++)
+    if (!ops[i]) return fail(-1, "bem_batch_create: null operator");
+  return guarded([&] {
+    auto b = std::make_unique<bem_batch>();
+    std::vector<bem::Operator*> v;
+    for (int i = 0; i < n; i++) v.push_back(&ops[i]->op);
+    BEM_HIP(hipStreamCreateWithFlags(&b->cs, hipStreamNonBlocking));
+    bem::build_batch(v, b->op, b->cs);
+    const size_t esz = b->op.complex_valued ? 16 : 8;
+    const int nmax = std::max(b->op.ndof_trial(), b->op.ndof_test());
+    b->dx.alloc((size_t)nmax * esz);
+    b->dy.alloc((size_t)nmax * esz);
+    *out = b.release();
+  });
+}
+
+int bem_batch_sizes(const bem_batch* b, int* width, int* height, int* is_complex) {
+  if (!b) return fail(-1, "bem_batch_sizes: bad arguments");
+  if (width) *width = b->op.ndof_trial();
+  if (height) *height = b->op.ndof_test();
+  if (is_complex) *is_complex = b->op.complex_valued ? 1 : 0;
+  return 0;
+}
+
+int bem_batch_apply(bem_batch* b, int flags, const double* alpha, const void* x, const double* beta, void* y,
+                    int dtype, void* stream) {
+  if (!b || !alpha || !beta || !x || !y) return fail(-1, "bem_batch_apply: bad arguments");
+  bem::Operator& op = b->op;
+  if (op.complex_valued != (dtype == BEM_C128)) return fail(-1, "bem_batch_apply: dtype does not match the batch");
+  return guarded([&] {
+    hipStream_t s = (hipStream_t)stream;
+    const int dflags = flags & (BEM_APPLY_TRANS | BEM_APPLY_CONJ | BEM_APPLY_MIXED);
+    if (flags & BEM_BATCH_EAGER) {
+      bem::apply_device(op, dflags, alpha, x, beta, y, s);
+      return;
+    }
+    const bool trans = flags & BEM_APPLY_TRANS;
+    const int nin = trans ? op.ndof_test() : op.ndof_trial();
+    const int nout = trans ? op.ndof_trial() : op.ndof_test();
+    const size_t esz = op.complex_valued ? 16 : 8;
+    hipGraphExec_t& ex = b->exec[dflags];
+    if (!ex) {  // capture gather .. scatter once per flag combination, replay after
+      const double one[2] = {1.0, 0.0}, zero[2] = {0.0, 0.0};
+      hipGraph_t g = nullptr;
+      BEM_HIP(hipStreamBeginCapture(b->cs, hipStreamCaptureModeThreadLocal));
+      try {
+        bem::apply_device(op, dflags, one, b->dx.p, zero, b->dy.p, b->cs);
+      } catch (...) {
+        (void)hipStreamEndCapture(b->cs, &g);
+        if (g) (void)hipGraphDestroy(g);
+        throw;
+      }
+      BEM_HIP(hipStreamEndCapture(b->cs, &g));
+      const hipError_t e = hipGraphInstantiate(&ex, g, nullptr, nullptr, 0);
+      (void)hipGraphDestroy(g);
+      if (e != hipSuccess) {
+        ex = nullptr;
+        throw bem::HipError(std::string("hipGraphInstantiate: ") + hipGetErrorString(e));
+      }
+    }
+    BEM_HIP(hipMemcpyAsync(b->dx.p, x, (size_t)nin * esz, hipMemcpyDeviceToDevice, s));
+    BEM_HIP(hipGraphLaunch(ex, s));
+    bem::axpby_device(op.complex_valued, y, b->dy.p, nout, alpha, beta, s);
+  });
+}
+
+int bem_batch_destroy(bem_batch* b) {
+  delete b;
+  return 0;
+}

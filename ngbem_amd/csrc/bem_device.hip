@@ -159,6 +159,19 @@ struct KHelSL {
     o[0] = {c * g, s * g};
   }
 };
+struct KHelSLdK {  // d/dkappa e^{ikr}/(4 pi r) = i e^{ikr}/(4 pi): smooth
+  typedef zc T;
+  static constexpr int NC = 1, NTERM = 1;
+  double kappa;
+  __device__ TermC term(int) const { return {1.0, 0, 0, 0}; }
+  __device__ __forceinline__ void eval(P3 x, P3 y, P3, P3, T* o) const {
+    double dx = x.x - y.x, dy = x.y - y.y, dz = x.z - y.z;
+    double r = sqrt(dx * dx + dy * dy + dz * dz);
+    double s, c;
+    sincos(kappa * r, &s, &c);
+    o[0] = {-s * kInv4Pi, c * kInv4Pi};
+  }
+};
 struct KHelDL {  // e^{ikr}/(4 pi r^3) <n_y, x-y> (1 - i k r)
   typedef zc T;
   static constexpr int NC = 1, NTERM = 1;
@@ -1344,6 +1357,7 @@ static void dispatch_kernel(int id, double kappa, F&& f) {
     case K_HHS: f(KHelHS{kappa}); break;
     case K_MSL: f(KMaxSL{kappa}); break;
     case K_MDL: f(KMaxDL{kappa}); break;
+    case K_HSLK: f(KHelSLdK{kappa}); break;
     default: throw std::runtime_error("unknown kernel id");
   }
 }
@@ -1949,17 +1963,8 @@ static void plan(const Operator& op, bool trans, std::vector<Tile>& tiles, std::
   }
 }
 
-void build_matvec_plan(Operator& op, hipStream_t s) {
-  std::vector<Tile> tiles;
-  std::vector<Seg> segs;
-  plan(op, false, tiles, segs);
-  op.d_tile_fwd.upload(tiles, s);
-  op.d_seg_fwd.upload(segs, s);
-  op.n_tiles_fwd = (int)tiles.size();
-  plan(op, true, tiles, segs);
-  op.d_tile_trn.upload(tiles, s);
-  op.d_seg_trn.upload(segs, s);
-  op.n_tiles_trn = (int)tiles.size();
+// low-rank B^T x work items of every far block (forward and transpose)
+static void upload_row_items(Operator& op, hipStream_t s) {
   for (int trans = 0; trans < 2; trans++) {
     std::vector<RowItem> items;
     for (size_t k = 0; k < op.far.size(); k++) {
@@ -1973,6 +1978,20 @@ void build_matvec_plan(Operator& op, hipStream_t s) {
     (trans ? op.d_items_trn : op.d_items_fwd).upload(items, s);
     (trans ? op.n_items_trn : op.n_items_fwd) = (int)items.size();
   }
+}
+
+void build_matvec_plan(Operator& op, hipStream_t s) {
+  std::vector<Tile> tiles;
+  std::vector<Seg> segs;
+  plan(op, false, tiles, segs);
+  op.d_tile_fwd.upload(tiles, s);
+  op.d_seg_fwd.upload(segs, s);
+  op.n_tiles_fwd = (int)tiles.size();
+  plan(op, true, tiles, segs);
+  op.d_tile_trn.upload(tiles, s);
+  op.d_seg_trn.upload(segs, s);
+  op.n_tiles_trn = (int)tiles.size();
+  upload_row_items(op, s);
   const size_t esz = op.complex_valued ? 16 : 8;
   const int nmax = std::max(op.ndof_trial(), op.ndof_test());
   op.d_xp.alloc((size_t)nmax * esz);
@@ -2007,6 +2026,139 @@ void make_mixed_copy(Operator& op, hipStream_t s) {
   conv(op.d_fV, op.d_fV32, tv);
   BEM_HIP(hipGetLastError());
   BEM_HIP(hipStreamSynchronize(s));
+}
+
+// ----------------------------------------------------------------- batch --
+// Block-diagonal union of assembled operators: storage copied into one set of
+// arrays, vectors concatenated, and one matvec plan whose tiles, segments and
+// low-rank items carry the per-instance offsets, so one launch of each apply
+// kernel serves every instance.
+static size_t factor_len(const Operator& op, bool u) {
+  size_t t = 0;
+  for (const FarBlk& fb : op.far)
+    t = std::max(t, (size_t)(u ? fb.uoff + (int64_t)fb.m * fb.R : fb.voff + (int64_t)fb.R * fb.n));
+  return t;
+}
+
+void build_batch(const std::vector<Operator*>& ops, Operator& b, hipStream_t s) {
+  if (ops.empty()) throw std::runtime_error("empty batch");
+  b.complex_valued = ops[0]->complex_valued;
+  b.mixed = true;
+  for (const Operator* o : ops) {
+    if (!o->assembled) throw std::runtime_error("batch: operator not assembled");
+    if (o->complex_valued != b.complex_valued) throw std::runtime_error("batch: real and complex operators mixed");
+    b.mixed = b.mixed && o->mixed;
+  }
+  const size_t esz = b.complex_valued ? 16 : 8, esz32 = esz / 2;
+  int64_t nnear = 0, nu = 0, nv = 0;
+  for (const Operator* o : ops) {
+    nnear += o->near_entries;
+    nu += (int64_t)factor_len(*o, true);
+    nv += (int64_t)factor_len(*o, false);
+  }
+  b.d_near_val.alloc((size_t)std::max<int64_t>(nnear, 1) * esz);
+  b.d_fU.alloc((size_t)std::max<int64_t>(nu, 1) * esz);
+  b.d_fV.alloc((size_t)std::max<int64_t>(nv, 1) * esz);
+  if (b.mixed) {
+    b.d_near_val32.alloc((size_t)std::max<int64_t>(nnear, 1) * esz32);
+    b.d_fU32.alloc((size_t)std::max<int64_t>(nu, 1) * esz32);
+    b.d_fV32.alloc((size_t)std::max<int64_t>(nv, 1) * esz32);
+  }
+  std::vector<int> perm_tr, perm_te;
+  std::vector<Tile> tf, tt, tiles;
+  std::vector<Seg> sf, st, segs;
+  b.far.clear();
+  int64_t NO = 0, UO = 0, VO = 0;
+  int TO = 0, XO = 0, YO = 0;
+  auto cp = [&](DBuf& dst, int64_t off, const DBuf& src, size_t n, size_t e) {
+    if (n) BEM_HIP(hipMemcpyAsync((char*)dst.p + off * e, src.p, n * e, hipMemcpyDeviceToDevice, s));
+  };
+  for (const Operator* op : ops) {
+    const size_t lu = factor_len(*op, true), lv = factor_len(*op, false);
+    cp(b.d_near_val, NO, op->d_near_val, (size_t)op->near_entries, esz);
+    cp(b.d_fU, UO, op->d_fU, lu, esz);
+    cp(b.d_fV, VO, op->d_fV, lv, esz);
+    if (b.mixed) {
+      cp(b.d_near_val32, NO, op->d_near_val32, (size_t)op->near_entries, esz32);
+      cp(b.d_fU32, UO, op->d_fU32, lu, esz32);
+      cp(b.d_fV32, VO, op->d_fV32, lv, esz32);
+    }
+    for (int p : op->trial_tree.perm) perm_tr.push_back(XO + p);
+    for (int p : op->test_tree->perm) perm_te.push_back(YO + p);
+    for (FarBlk fb : op->far) {
+      fb.uoff += UO;
+      fb.voff += VO;
+      fb.r0 += YO;
+      fb.c0 += XO;
+      fb.toff += TO;
+      b.far.push_back(fb);
+    }
+    for (int trans = 0; trans < 2; trans++) {
+      plan(*op, trans, tiles, segs);
+      std::vector<Tile>& T = trans ? tt : tf;
+      std::vector<Seg>& S = trans ? st : sf;
+      const int s0 = (int)S.size();
+      for (Tile t : tiles) {
+        t.p0 += trans ? XO : YO;
+        t.s0 += s0;
+        t.s1 += s0;
+        T.push_back(t);
+      }
+      for (Seg g : segs) {
+        g.base += g.far ? (trans ? VO : UO) : NO;
+        g.xoff += g.far ? TO : (trans ? YO : XO);
+        S.push_back(g);
+      }
+    }
+    NO += op->near_entries;
+    UO += (int64_t)lu;
+    VO += (int64_t)lv;
+    TO += op->t_total;
+    XO += op->ndof_trial();
+    YO += op->ndof_test();
+    b.near_entries += op->near_entries;
+    b.far_factor_entries += op->far_factor_entries;
+  }
+  b.trial = std::make_shared<Space>();
+  b.trial->ndof = XO;
+  b.test = std::make_shared<Space>();
+  b.test->ndof = YO;
+  b.t_total = TO;
+  b.d_perm_trial.upload(perm_tr, s);
+  b.d_perm_test.upload(perm_te, s);
+  b.d_far_blk.upload(b.far, s);
+  b.d_tile_fwd.upload(tf, s);
+  b.d_seg_fwd.upload(sf, s);
+  b.n_tiles_fwd = (int)tf.size();
+  b.d_tile_trn.upload(tt, s);
+  b.d_seg_trn.upload(st, s);
+  b.n_tiles_trn = (int)tt.size();
+  upload_row_items(b, s);
+  const int nmax = std::max(XO, YO);
+  b.d_xp.alloc((size_t)nmax * esz);
+  b.d_yp.alloc((size_t)nmax * esz);
+  b.d_t.alloc((size_t)std::max(TO, 1) * esz);
+  b.assembled = true;
+  BEM_HIP(hipStreamSynchronize(s));
+}
+
+template <class T>
+__global__ void k_axpby(T* __restrict__ y, const T* __restrict__ z, int n, T alpha, T beta, int beta_zero) {
+  const int i = blockIdx.x * 256 + threadIdx.x;
+  if (i >= n) return;
+  y[i] = beta_zero ? alpha * z[i] : alpha * z[i] + beta * y[i];
+}
+
+void axpby_device(bool cplx, void* y, const void* z, int n, const double* alpha, const double* beta, hipStream_t s) {
+  const int beta_zero = beta[0] == 0.0 && beta[1] == 0.0;
+  if (n <= 0) return;
+  if (cplx)
+    hipLaunchKernelGGL((k_axpby<zc>), dim3((n + 255) / 256), dim3(256), 0, s, (zc*)y, (const zc*)z, n,
+                       zc{alpha[0], alpha[1]}, zc{beta[0], beta[1]}, beta_zero);
+  else
+    hipLaunchKernelGGL((k_axpby<double>), dim3((n + 255) / 256), dim3(256), 0, s, (double*)y, (const double*)z, n,
+                       alpha[0], beta[0], beta_zero);
+  BEM_HIP(hipGetLastError());
 }
 
 // ----------------------------------------------------------------- apply --

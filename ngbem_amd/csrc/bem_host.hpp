@@ -18,7 +18,8 @@ enum SpaceKind { SP_P0 = 0, SP_P1 = 1, SP_RWG = 2, SP_NED = 3 };
 // EV_MAXW: [phi, div phi] (DiffOpMaxwellNew on HDivSurface, DiffOpMaxwell on
 // HCurl); EV_TRACE3: vector trace (HDivSurface BND, RotatedTrace on HCurl)
 enum EvalKind { EV_ID = 0, EV_ROT = 1, EV_HELM = 2, EV_MAXW = 3, EV_TRACE3 = 4 };
-enum KernelId { K_LSL = 0, K_LDL, K_LHS, K_HSL, K_HDL, K_HCF, K_HHS, K_MSL, K_MDL, K_COUNT };
+// K_HSLK: d/dkappa of the Helmholtz single layer kernel (no reference counterpart)
+enum KernelId { K_LSL = 0, K_LDL, K_LHS, K_HSL, K_HDL, K_HCF, K_HHS, K_MSL, K_MDL, K_HSLK, K_COUNT };
 
 struct Mesh {
   int nv = 0, nt = 0;

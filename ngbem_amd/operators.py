@@ -25,7 +25,7 @@ __all__ = ["BaseMatrix", "HMatrix", "IntegralOperator", "IntegralOperatorC", "Ma
            "HelmholtzSingleLayerPotentialOperator", "HelmholtzDoubleLayerPotentialOperator",
            "HelmholtzCombinedFieldOperator", "HelmholtzHypersingularOperator",
            "MaxwellSingleLayerPotentialOperator", "MaxwellSingleLayerPotentialOperatorCurl",
-           "MaxwellDoubleLayerPotentialOperator", "device"]
+           "MaxwellDoubleLayerPotentialOperator", "HelmholtzSingleLayerPotentialOperatorDKappa", "device"]
 
 
 def device() -> torch.device:
@@ -505,6 +505,15 @@ def HelmholtzSingleLayerPotentialOperator(trial_space, test_space=None, kappa=No
     """Helmholtz single layer, e^{ikr}/(4 pi r) (src/python_bem.cpp:79-87)."""
     return _helmholtz(_capi.HELMHOLTZ_SL, trial_space, test_space, kappa, intorder, leafsize, eta, eps, method,
                       testhmatrix, mixed)
+
+
+def HelmholtzSingleLayerPotentialOperatorDKappa(trial_space, test_space=None, kappa=None, intorder=3, leafsize=40,
+                                                eta=2., eps=1e-6, method="aca", testhmatrix=False, mixed=False):
+    """d/dkappa of the Helmholtz single layer: kernel i e^{ikr}/(4 pi), smooth.
+    No counterpart in the reference; it gives the kappa-gradient of V(kappa) x
+    (ngbem_amd.batch.SingleLayerBatch)."""
+    return _helmholtz(_capi.HELMHOLTZ_SL_DKAPPA, trial_space, test_space, kappa, intorder, leafsize, eta, eps,
+                      method, testhmatrix, mixed)
 
 
 def HelmholtzDoubleLayerPotentialOperator(trial_space, test_space=None, kappa=None, intorder=3, leafsize=40, eta=2.,

@@ -372,7 +372,7 @@ static void eval_shape(const Mesh& m, const Space& s, int ev, int e, double xi, 
 
 // ---------------------------------------------------------------- kernels --
 
-enum { K_LSL = 0, K_LDL, K_LHS, K_HSL, K_HDL, K_HCF, K_HHS, K_MSL, K_MDL };
+enum { K_LSL = 0, K_LDL, K_LHS, K_HSL, K_HDL, K_HCF, K_HHS, K_MSL, K_MDL, K_HSLK };
 
 struct Term { double fac; int kc, trc, tec; };
 
@@ -438,6 +438,8 @@ struct Kernel {
     } else if (id == K_MDL) {
       cplx f = e / (4 * PI * r * r * r) * cplx(-1.0, kr);
       for (int c = 0; c < 3; c++) out[c] = f * d[c];
+    } else if (id == K_HSLK) {  // d/dkappa of e^{i kappa r}/(4 pi r)
+      out[0] = cplx(0.0, 1.0) * e / (4 * PI);
     }
   }
 };

@@ -17,7 +17,7 @@ _lib = None
 
 # kernel ids (same numbering as the product's C-ABI)
 (LAPLACE_SL, LAPLACE_DL, LAPLACE_HS, HELMHOLTZ_SL, HELMHOLTZ_DL, HELMHOLTZ_CF, HELMHOLTZ_HS, MAXWELL_SL,
- MAXWELL_DL) = range(9)
+ MAXWELL_DL, HELMHOLTZ_SL_DKAPPA) = range(10)
 P0, P1, HDIV, HCURL = 0, 1, 2, 3
 
 

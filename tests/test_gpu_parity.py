@@ -26,6 +26,7 @@ FACT = {
     "MSL": (nb.MaxwellSingleLayerPotentialOperator, O.MAXWELL_SL, True),
     "MSLC": (nb.MaxwellSingleLayerPotentialOperatorCurl, O.MAXWELL_SL, True),
     "MDL": (nb.MaxwellDoubleLayerPotentialOperator, O.MAXWELL_DL, True),
+    "HSLK": (nb.HelmholtzSingleLayerPotentialOperatorDKappa, O.HELMHOLTZ_SL_DKAPPA, True),
 }
 
 # (kernel, trial space, test space, level, intorder, leafsize, eta, eps, kappa)
@@ -52,6 +53,7 @@ CASES = [
     ("MDL", 3, 2, 2, 4, 16, 2.0, 1e-8, 3.0),
     ("MSL", 2, 2, 3, 3, 32, 2.0, 1e-6, 6.0),
     ("MDL", 3, 2, 3, 3, 32, 2.0, 1e-6, 6.0),
+    ("HSLK", 0, 0, 3, 3, 32, 2.0, 1e-8, 5.0),
 ]
 
 

@@ -205,3 +205,25 @@ def test_apply_matches_densified_and_transpose():
     assert np.linalg.norm(Aty - A.T @ y) <= 1e-13 * np.linalg.norm(A.T @ y)
     # <A x, y> = <x, A^T y> (bilinear, plain transpose)
     assert abs(np.sum(Ax * y) - np.sum(x * Aty)) <= 1e-12 * abs(np.sum(Ax * y))
+
+
+def test_helmholtz_dkappa_operator_is_the_kappa_derivative():
+    """The d/dkappa operator (kernel i e^{ikr}/4pi) equals the central
+    difference of the Helmholtz single layer in kappa, entry by entry, with the
+    same quadrature (O(h^2) truncation), on P0 and P1."""
+    from ngbem_amd.mesh import ellipsoid
+
+    m = ellipsoid(1, (1.0, 0.7, 1.3))
+    for sp in (O.P0, O.P1):
+        kappa, h = 3.0, 1e-4
+        D = O.OracleOp(m, O.HELMHOLTZ_SL_DKAPPA, kappa, sp, sp, intorder=4).dense()
+        Vp = O.OracleOp(m, O.HELMHOLTZ_SL, kappa + h, sp, sp, intorder=4).dense()
+        Vm = O.OracleOp(m, O.HELMHOLTZ_SL, kappa - h, sp, sp, intorder=4).dense()
+        fd = (Vp - Vm) / (2 * h)
+        assert np.abs(D - fd).max() < 1e-7 * np.abs(D).max()
+        # closed form at coincident points: i/(4 pi) times the areas (P0 diagonal)
+        if sp == O.P0:
+            area = m.areas()
+            Dk0 = O.OracleOp(m, O.HELMHOLTZ_SL_DKAPPA, 1e-9, sp, sp, intorder=4).dense()
+            assert np.allclose(Dk0.imag, np.outer(area, area) / (4 * np.pi), rtol=1e-12, atol=0)
+            assert np.abs(Dk0.real).max() < 1e-8 * np.abs(Dk0.imag).max()  # -sin(kappa r)/(4 pi), kappa ~ 0
