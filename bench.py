@@ -72,8 +72,11 @@ def main():
     ap.add_argument("--traffic", type=str,
                     default=os.path.join(ROOT, "profiles", "k_mv_rows_helmholtz_cf_L6_traffic.md"),
                     help="summary of the PMC passes with a 'bytes_per_launch:' line for k_mv_rows<c64>")
-    ap.add_argument("--workload", choices=["helmholtz_cf", "maxwell_efie_64rhs"], default="helmholtz_cf",
-                    help="helmholtz_cf: the headline; maxwell_efie_64rhs: block product of 64 right-hand sides")
+    ap.add_argument("--workload", choices=["helmholtz_cf", "maxwell_efie_64rhs", "batch_sl_256"],
+                    default="helmholtz_cf",
+                    help="helmholtz_cf: the headline; maxwell_efie_64rhs: block product of 64 right-hand sides; "
+                         "batch_sl_256: 256 Helmholtz single layers on ellipsoids, batched apply + gradient")
+    ap.add_argument("--instances", type=int, default=256)
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only (asked for --gpus {args.gpus}); the multi-GPU sharding is a design "
@@ -81,6 +84,9 @@ def main():
         sys.exit(2)
     if args.workload == "maxwell_efie_64rhs":
         bench_maxwell_block(args)
+        return
+    if args.workload == "batch_sl_256":
+        bench_batch(args)
         return
 
     import torch
@@ -334,6 +340,82 @@ def bench_maxwell_block(args):
                      "note": "whole block product (gather, V X, row tiles, scatter) timed"},
         "hmatrix": {"near_entries": st["near_entries"], "far_factor_entries": st["far_factor_entries"],
                     "n_near": st["n_near"], "n_far": st["n_far"], "max_rank": st["max_rank"]},
+    }
+    print(json.dumps(out), flush=True)
+
+
+def bench_batch(args):
+    """Helmholtz single layers on ellipsoids (semi-axes U[0.5, 2], level
+    U{3, 4, 5}, kappa U[1, 8], seed 0), SurfaceL2 p0, intorder 4, leafsize 40,
+    eta 2, ACA eps 1e-6.  All instances assembled concurrently, merged into one
+    batch; one step = one batched apply (one hipGraph replay); also one
+    gradient step of ||V(kappa) x - t||^2 w.r.t. x and kappa."""
+    import torch
+
+    import ngbem_amd as nb
+
+    dev = torch.device("cuda", 0)
+    torch.cuda.set_device(dev)
+    rng = np.random.default_rng(0)
+    n_inst = args.instances
+    axes = rng.uniform(0.5, 2.0, (n_inst, 3))
+    levels = rng.integers(3, 6, n_inst)
+    kappas = rng.uniform(1.0, 8.0, n_inst)
+    meshes = [nb.ellipsoid(int(l), tuple(a)) for l, a in zip(levels, axes)]
+    spaces = [nb.SurfaceL2(m) for m in meshes]
+    ntri = sum(m.nt for m in meshes)
+    log(f"bench batch: {n_inst} instances, levels {np.bincount(levels, minlength=6)[3:].tolist()} (L3/L4/L5), "
+        f"{ntri} triangles in all")
+    torch.zeros(1, device=dev)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    slb = nb.SingleLayerBatch(spaces, kappas, threads=8, intorder=4, leafsize=40, eta=2.0, eps=1e-6)
+    torch.cuda.synchronize()
+    t_asm = time.perf_counter() - t0
+    log(f"assembly of {n_inst} operators + {n_inst} d/dkappa operators, batching: {t_asm:.2f} s")
+    n = slb.V.width
+    g = torch.Generator(device="cpu").manual_seed(1)
+    x = torch.randn(n, dtype=torch.complex128, generator=g).to(dev)
+    y = torch.zeros_like(x)
+    B = slb.V
+
+    def timed(fn, steps, warmup):
+        for _ in range(warmup):
+            fn()
+        torch.cuda.synchronize()
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        e0.record()
+        for _ in range(steps):
+            fn()
+        e1.record()
+        torch.cuda.synchronize()
+        return e0.elapsed_time(e1) / 1e3 / steps
+
+    t_apply = timed(lambda: B.Mult(x, y), args.steps, args.warmup)
+    t_eager = timed(lambda: B.view(graph=False).Mult(x, y), args.steps, args.warmup)
+    xr = torch.randn(n, dtype=torch.float64, generator=g).to(dev)
+    tgt = torch.randn(n, dtype=torch.complex128, generator=g).to(dev)
+    kap = torch.tensor(kappas, dtype=torch.float64, device=dev)
+
+    def grad_step():
+        xx = xr.clone().requires_grad_(True)
+        kk = kap.clone().requires_grad_(True)
+        loss = ((slb(xx, kk) - tgt).abs() ** 2).sum()
+        loss.backward()
+        return xx.grad, kk.grad
+
+    t_grad = timed(grad_step, max(3, args.steps // 5), 1)
+    log(f"batched apply {t_apply * 1e3:.3f} ms (graph), {t_eager * 1e3:.3f} ms (direct launches); "
+        f"forward+backward {t_grad * 1e3:.3f} ms")
+    out = {
+        "metric": "batched H-matvec/s over 256 Helmholtz single layers (one launch sequence)",
+        "value": 1.0 / t_apply, "unit": "batched H-matvec/s", "n_gpus": 1, "steps": args.steps,
+        "warmup": args.warmup, "ms_per_step": t_apply * 1e3, "higher_is_better": True, "scaling": "weak",
+        "vs_baseline": None, "dtype": "c128", "data": "synthetic",
+        "config": {"workload": f"{n_inst} Helmholtz single layers on ellipsoids, semi-axes U[0.5,2], level U{{3,4,5}}, "
+                               f"kappa U[1,8] (seed 0), SurfaceL2 p0, intorder 4, leafsize 40, eta 2, eps 1e-6; "
+                               f"{ntri} triangles, {n} dofs in all"},
+        "assembly_s": t_asm, "apply_direct_launch_ms": t_eager * 1e3, "grad_step_ms": t_grad * 1e3,
     }
     print(json.dumps(out), flush=True)
 
