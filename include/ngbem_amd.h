@@ -74,6 +74,9 @@ enum {
   BEM_APPLY_PER_VECTOR = 16, /* nrhs > 1: one vector at a time instead of the block product */
   BEM_BATCH_EAGER = 32        /* bem_batch_apply: launch the kernels directly, no graph replay */
 };
+enum {
+  BEM_BATCH_CONSUME = 1       /* bem_batch_create: move the operators' storage into the batch */
+};
 
 /* vector dtypes */
 enum { BEM_F64 = 0, BEM_C128 = 1 };
@@ -156,11 +159,13 @@ int bem_op_tree(const bem_op* op, int which, int32_t* perm, double* cl, double* 
  * arrays and one matvec plan covers every instance, so an apply is one launch
  * of each kernel for the whole batch, captured once per flag combination in a
  * hipGraph and replayed.  Vectors are the concatenation of the instances'
- * vectors (device pointers).  The operators may be destroyed afterwards.
+ * vectors (device pointers).  The operators may be destroyed afterwards; with
+ * flags = BEM_BATCH_CONSUME their block storage is freed as it is copied
+ * (peak memory: the batch plus one operator), and they can no longer be applied.
  * No counterpart in the reference, which assembles and applies one operator
  * per GenericIntegralOperator (src/ngbem.cpp:314-330, src/hmat.cpp:632-666).
  * One stream at a time per batch. */
-int bem_batch_create(bem_op* const* ops, int n, bem_batch** out);
+int bem_batch_create(bem_op* const* ops, int n, int flags, bem_batch** out);
 int bem_batch_sizes(const bem_batch* b, int* width, int* height, int* is_complex);
 int bem_batch_apply(bem_batch* b, int flags, const double* alpha, const void* x, const double* beta, void* y,
                     int dtype, void* stream);

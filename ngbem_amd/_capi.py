@@ -15,6 +15,7 @@ SPACE_P0, SPACE_P1, SPACE_HDIV, SPACE_HCURL = 0, 1, 2, 3
 (LAPLACE_SL, LAPLACE_DL, LAPLACE_HS, HELMHOLTZ_SL, HELMHOLTZ_DL, HELMHOLTZ_CF, HELMHOLTZ_HS, MAXWELL_SL,
  MAXWELL_DL, HELMHOLTZ_SL_DKAPPA) = range(10)
 APPLY_TRANS, APPLY_CONJ, APPLY_MIXED, APPLY_ON_DEVICE, APPLY_PER_VECTOR, BATCH_EAGER = 1, 2, 4, 8, 16, 32
+BATCH_CONSUME = 1
 F64, C128 = 0, 1
 
 
@@ -87,7 +88,7 @@ def lib():
     L.bem_op_block_dofs.argtypes = [P, ctypes.c_int, ip, ip]
     L.bem_op_block_data.argtypes = [P, ctypes.c_int, P]
     L.bem_op_tree.argtypes = [P, ctypes.c_int, ip, dp, dp, ctypes.c_int]
-    L.bem_batch_create.argtypes = [ctypes.POINTER(P), ctypes.c_int, ctypes.POINTER(P)]
+    L.bem_batch_create.argtypes = [ctypes.POINTER(P), ctypes.c_int, ctypes.c_int, ctypes.POINTER(P)]
     L.bem_batch_sizes.argtypes = [P, ctypes.POINTER(ctypes.c_int), ctypes.POINTER(ctypes.c_int),
                                   ctypes.POINTER(ctypes.c_int)]
     L.bem_batch_apply.argtypes = [P, ctypes.c_int, dp, P, dp, P, ctypes.c_int, P]

@@ -55,14 +55,18 @@ class OperatorBatch(BaseMatrix):
     copied into the batch, so they may be dropped afterwards.  ``mixed``
     selects the FP32 / complex64 copies (all operators built with mixed=True).
     ``graph=False`` launches the kernels directly instead of replaying a graph.
+    ``consume=True`` frees each operator's block storage as it is copied (the
+    operators can then no longer be applied); peak memory is the batch plus
+    one operator instead of twice the batch.
     """
 
-    def __init__(self, ops, mixed=False, graph=True):
+    def __init__(self, ops, mixed=False, graph=True, consume=False):
         ops = list(ops)
         L = _capi.lib()
         arr = (ctypes.c_void_p * len(ops))(*[o._h for o in ops])
         out = ctypes.c_void_p()
-        _capi.check(L.bem_batch_create(arr, len(ops), ctypes.byref(out)), "bem_batch_create")
+        _capi.check(L.bem_batch_create(arr, len(ops), _capi.BATCH_CONSUME if consume else 0, ctypes.byref(out)),
+                    "bem_batch_create")
         self._h = out
         w, h, c = ctypes.c_int(), ctypes.c_int(), ctypes.c_int()
         _capi.check(L.bem_batch_sizes(self._h, ctypes.byref(w), ctypes.byref(h), ctypes.byref(c)))
@@ -185,10 +189,12 @@ class SingleLayerBatch:
         items = list(zip(spaces, self.kappas))
         V = assemble_many(lambda it: HelmholtzSingleLayerPotentialOperator(it[0], it[0], kappa=it[1], **kw), items,
                           threads)
+        self.V = OperatorBatch(V, consume=True)
+        del V
         dV = assemble_many(lambda it: HelmholtzSingleLayerPotentialOperatorDKappa(it[0], it[0], kappa=it[1], **kw),
                            items, threads)
-        self.V = OperatorBatch(V)
-        self.dV = OperatorBatch(dV)
+        self.dV = OperatorBatch(dV, consume=True)
+        del dV
         self.ndofs = [s.ndof for s in spaces]
         seg = np.repeat(np.arange(len(spaces)), self.ndofs)
         self.seg = torch.from_numpy(seg).to(device())

@@ -459,7 +459,7 @@ int bem_op_tree(const bem_op* h, int which, int32_t* perm, double* cl, double* c
 
 // ------------------------------------------------------------------ batch --
 
-int bem_batch_create(bem_op* const* ops, int n, bem_batch** out) {
+int bem_batch_create(bem_op* const* ops, int n, int flags, bem_batch** out) {
   if (!ops || n < 1 || !out) return fail(-1, "bem_batch_create: bad arguments");
   for (int i = 0; i < n; i++)
     if (!ops[i]) return fail(-1, "bem_batch_create: null operator");
@@ -468,7 +468,7 @@ int bem_batch_create(bem_op* const* ops, int n, bem_batch** out) {
     std::vector<bem::Operator*> v;
     for (int i = 0; i < n; i++) v.push_back(&ops[i]->op);
     BEM_HIP(hipStreamCreateWithFlags(&b->cs, hipStreamNonBlocking));
-    bem::build_batch(v, b->op, b->cs);
+    bem::build_batch(v, b->op, b->cs, (flags & BEM_BATCH_CONSUME) != 0);
     const size_t esz = b->op.complex_valued ? 16 : 8;
     const int nmax = std::max(b->op.ndof_trial(), b->op.ndof_test());
     b->dx.alloc((size_t)nmax * esz);

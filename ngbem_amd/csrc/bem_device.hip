@@ -2040,7 +2040,7 @@ static size_t factor_len(const Operator& op, bool u) {
   return t;
 }
 
-void build_batch(const std::vector<Operator*>& ops, Operator& b, hipStream_t s) {
+void build_batch(const std::vector<Operator*>& ops, Operator& b, hipStream_t s, bool consume) {
   if (ops.empty()) throw std::runtime_error("empty batch");
   b.complex_valued = ops[0]->complex_valued;
   b.mixed = true;
@@ -2073,7 +2073,7 @@ void build_batch(const std::vector<Operator*>& ops, Operator& b, hipStream_t s) 
   auto cp = [&](DBuf& dst, int64_t off, const DBuf& src, size_t n, size_t e) {
     if (n) BEM_HIP(hipMemcpyAsync((char*)dst.p + off * e, src.p, n * e, hipMemcpyDeviceToDevice, s));
   };
-  for (const Operator* op : ops) {
+  for (Operator* op : ops) {
     const size_t lu = factor_len(*op, true), lv = factor_len(*op, false);
     cp(b.d_near_val, NO, op->d_near_val, (size_t)op->near_entries, esz);
     cp(b.d_fU, UO, op->d_fU, lu, esz);
@@ -2109,6 +2109,17 @@ void build_batch(const std::vector<Operator*>& ops, Operator& b, hipStream_t s) 
         g.xoff += g.far ? TO : (trans ? YO : XO);
         S.push_back(g);
       }
+    }
+    if (consume) {  // the operator's block storage moves into the batch
+      BEM_HIP(hipStreamSynchronize(s));
+      Operator* o = const_cast<Operator*>(op);
+      o->d_near_val.release();
+      o->d_fU.release();
+      o->d_fV.release();
+      o->d_near_val32.release();
+      o->d_fU32.release();
+      o->d_fV32.release();
+      o->assembled = false;
     }
     NO += op->near_entries;
     UO += (int64_t)lu;

@@ -177,7 +177,8 @@ void make_mixed_copy(Operator& op, hipStream_t s);
 void apply_device(Operator& op, int flags, const double* alpha, const void* x, const double* beta, void* y,
                   hipStream_t s, hipEvent_t* ev = nullptr);
 // block-diagonal union of assembled operators (storage copied, one plan)
-void build_batch(const std::vector<Operator*>& ops, Operator& b, hipStream_t s);
+// consume: free each operator's block storage once copied (it can no longer be applied)
+void build_batch(const std::vector<Operator*>& ops, Operator& b, hipStream_t s, bool consume);
 // y = alpha z + beta y (device vectors, complex128 or double)
 void axpby_device(bool cplx, void* y, const void* z, int n, const double* alpha, const double* beta, hipStream_t s);
 // forward y = alpha A x + beta y for nrhs columns (stride ld) on the matrix

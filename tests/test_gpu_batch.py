@@ -150,3 +150,14 @@ def test_autograd_hmatrix_gradient_close_to_dense(gpu):
         grads.append((kappa.grad.clone(), x.grad.clone()))
     assert (torch.linalg.norm(grads[0][0] - grads[1][0]) / torch.linalg.norm(grads[1][0])).item() < 1e-5
     assert (torch.linalg.norm(grads[0][1] - grads[1][1]) / torch.linalg.norm(grads[1][1])).item() < 1e-5
+
+
+def test_consumed_batch_equals_copied_batch(gpu, batch_setup):
+    inst, spaces, ops, B = batch_setup
+    kw = dict(intorder=3, leafsize=32, eta=2.0, eps=1e-8)
+    fresh = [nb.HelmholtzSingleLayerPotentialOperator(s, s, kappa=k, **kw) for s, (_, k) in zip(spaces, inst)]
+    C = nb.OperatorBatch(fresh, consume=True)
+    x = torch.randn(B.width, dtype=torch.complex128, device=gpu)
+    assert torch.equal(C * x, B * x)
+    with pytest.raises(RuntimeError, match="not assembled"):
+        fresh[0].mat * x[: spaces[0].ndof]
