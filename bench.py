@@ -77,6 +77,7 @@ def main():
                     help="helmholtz_cf: the headline; maxwell_efie_64rhs: block product of 64 right-hand sides; "
                          "batch_sl_256: 256 Helmholtz single layers on ellipsoids, batched apply + gradient")
     ap.add_argument("--instances", type=int, default=256)
+    ap.add_argument("--threads", type=int, default=8, help="host threads assembling instances concurrently")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only (asked for --gpus {args.gpus}); the multi-GPU sharding is a design "
@@ -102,8 +103,11 @@ def main():
     params = dict(intorder=4, leafsize=64, eta=2.0, eps=1e-6)
     log(f"bench: icosphere L{args.level} ({mesh.nt} triangles), kappa={kappa}, {params}")
 
-    # warm the runtime (first HIP call initialises the device)
+    # warm the runtime (first HIP call initialises the device) and load the
+    # assembly kernels' code objects with a small operator of the same kind
     torch.zeros(1, device=dev)
+    small = nb.SurfaceL2(nb.icosphere(2))
+    nb.HelmholtzCombinedFieldOperator(small, small, kappa=kappa, mixed=True, **dict(params, leafsize=16))
     torch.cuda.synchronize()
     t0 = time.perf_counter()
     C = nb.HelmholtzCombinedFieldOperator(fes, fes, kappa=kappa, mixed=True, **params)
@@ -369,7 +373,7 @@ def bench_batch(args):
     torch.zeros(1, device=dev)
     torch.cuda.synchronize()
     t0 = time.perf_counter()
-    slb = nb.SingleLayerBatch(spaces, kappas, threads=8, intorder=4, leafsize=40, eta=2.0, eps=1e-6)
+    slb = nb.SingleLayerBatch(spaces, kappas, threads=args.threads, intorder=4, leafsize=40, eta=2.0, eps=1e-6)
     torch.cuda.synchronize()
     t_asm = time.perf_counter() - t0
     log(f"assembly of {n_inst} operators + {n_inst} d/dkappa operators, batching: {t_asm:.2f} s")

@@ -5,8 +5,11 @@
 
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <cstdint>
+#include <cstdlib>
 #include <memory>
+#include <mutex>
 #include <stdexcept>
 #include <string>
 #include <vector>
@@ -27,6 +30,52 @@ struct HipError : std::runtime_error {
                             ":" + std::to_string(__LINE__) + ")");                             \
   } while (0)
 
+// Device memory comes from the device's default stream-ordered pool, which
+// keeps up to 64 GiB (NGBEM_POOL_KEEP_GB) of freed memory mapped: operators built one
+// after another (batches of geometries) then reuse it instead of mapping fresh
+// memory each time, which costs 100+ ms per large buffer.  Allocation and free
+// are ordered on the null stream and bracketed by synchronisation, so a buffer
+// behaves like hipMalloc/hipFree memory for every other stream.
+inline void pool_setup() {
+  static std::mutex mu;
+  static bool done[64] = {};
+  int dev = 0;
+  BEM_HIP(hipGetDevice(&dev));
+  std::lock_guard<std::mutex> lk(mu);
+  if (dev < 0 || dev >= 64 || done[dev]) return;
+  hipMemPool_t pool;
+  BEM_HIP(hipDeviceGetDefaultMemPool(&pool, dev));
+  const char* env = std::getenv("NGBEM_POOL_KEEP_GB");
+  uint64_t keep = (env ? (uint64_t)std::max(0.0, std::atof(env)) : 64ull) << 30;
+  BEM_HIP(hipMemPoolSetAttribute(pool, hipMemPoolAttrReleaseThreshold, &keep));
+  done[dev] = true;
+}
+
+inline void* dev_alloc(size_t n) {
+  pool_setup();
+  void* p = nullptr;
+  hipError_t e = hipMallocAsync(&p, n, nullptr);
+  if (e == hipErrorOutOfMemory) {  // give the pool's cached memory back and retry once
+    (void)hipGetLastError();
+    int dev = 0;
+    hipMemPool_t pool;
+    BEM_HIP(hipGetDevice(&dev));
+    BEM_HIP(hipDeviceGetDefaultMemPool(&pool, dev));
+    BEM_HIP(hipDeviceSynchronize());
+    BEM_HIP(hipMemPoolTrimTo(pool, 0));
+    e = hipMallocAsync(&p, n, nullptr);
+  }
+  BEM_HIP(e);
+  BEM_HIP(hipStreamSynchronize(nullptr));
+  return p;
+}
+
+inline void dev_free(void* p) {
+  if (!p) return;
+  (void)hipDeviceSynchronize();  // like hipFree: no stream may still use it
+  (void)hipFreeAsync(p, nullptr);
+}
+
 // owning device buffer
 struct DBuf {
   void* p = nullptr;
@@ -36,7 +85,7 @@ struct DBuf {
   DBuf& operator=(const DBuf&) = delete;
   ~DBuf() { release(); }
   void release() {
-    if (p) (void)hipFree(p);
+    dev_free(p);
     p = nullptr;
     bytes = 0;
   }
@@ -44,7 +93,7 @@ struct DBuf {
     if (n == bytes && p) return;
     release();
     if (n == 0) return;
-    BEM_HIP(hipMalloc(&p, n));
+    p = dev_alloc(n);
     bytes = n;
   }
   template <class T>
