@@ -70,8 +70,8 @@ def main():
     ap.add_argument("--calib-copy", action="store_true",
                     help="end with one plain 512 MiB device copy (calibrates rocprof FETCH_SIZE/WRITE_SIZE)")
     ap.add_argument("--traffic", type=str,
-                    default=os.path.join(ROOT, "profiles", "k_mv_rows_helmholtz_cf_L6_traffic.md"),
-                    help="summary of the PMC passes with a 'bytes_per_launch:' line for k_mv_rows<c64>")
+                    default=os.path.join(ROOT, "profiles", "hmatvec_helmholtz_cf_L6_traffic.md"),
+                    help="summary of the PMC passes with 'bytes_per_launch <kernel>:' lines (complex64 apply)")
     ap.add_argument("--workload", choices=["helmholtz_cf", "maxwell_efie_64rhs", "batch_sl_256"],
                     default="helmholtz_cf",
                     help="helmholtz_cf: the headline; maxwell_efie_64rhs: block product of 64 right-hand sides; "
@@ -143,7 +143,7 @@ def main():
 
     # per-kernel times from HIP events on the launch stream
     def profile(flags, iters=max(5, args.steps // 2)):
-        ms = np.zeros(5)
+        ms = np.zeros(6)
         stream = torch.cuda.current_stream().cuda_stream
         _capi.check(_capi.lib().bem_op_profile_apply(C._h, flags, iters, _capi.dptr(ms), ctypes.c_void_p(stream)))
         return ms
@@ -154,23 +154,32 @@ def main():
     near = st["near_entries"]
     far_u = far_v = 0
     R_tot = 0
+    m_tot = 0
     for b in range(C.num_blocks()):
         bi = C.block_info(b)
         if bi["rank"] >= 0:
             far_u += bi["m"] * bi["rank"]
             far_v += bi["rank"] * bi["n"]
             R_tot += bi["rank"]
+            m_tot += bi["m"]
 
-    def rows_bytes(s):  # dense rows + U rows + x + t + y, each once
-        return (near + far_u) * s + n * s + R_tot * s + n * s
+    # algorithmic bytes per launch (each operand once) of the three apply kernels
+    def kernel_bytes(s):
+        return {"k_mv_far_t": far_v * s + n * s + R_tot * s,          # V, x, t
+                "k_mv_far_z": far_u * s + R_tot * s + m_tot * s,      # Ut, t, z
+                "k_mv_rows": near * s + n * s + m_tot * s + n * s}    # dense, x, z, y
 
     def apply_bytes(s):
-        return (near + far_u + far_v) * s + 2 * n * s + 2 * R_tot * s
+        return (near + far_u + far_v) * s + 2 * n * s + 2 * R_tot * s + 2 * m_tot * s
 
-    rows_gbs32 = rows_bytes(8) / (prof32[2] * 1e-3) / 1e9
-    rows_gbs64 = rows_bytes(16) / (prof64[2] * 1e-3) / 1e9
-    log(f"k_mv_rows mixed {prof32[2]:.3f} ms = {rows_gbs32:.0f} GB/s; fp64 {prof64[2]:.3f} ms = {rows_gbs64:.0f} GB/s; "
-        f"far B^T x mixed {prof32[1]:.3f} ms fp64 {prof64[1]:.3f} ms")
+    kms = {"mixed": dict(zip(["gather", "k_mv_far_t", "k_mv_far_z", "k_mv_rows", "scatter", "apply"], prof32)),
+           "fp64": dict(zip(["gather", "k_mv_far_t", "k_mv_far_z", "k_mv_rows", "scatter", "apply"], prof64))}
+    kb32, kb64 = kernel_bytes(8), kernel_bytes(16)
+    gbs = {"mixed": {k: kb32[k] / (kms["mixed"][k] * 1e-3) / 1e9 for k in kb32},
+           "fp64": {k: kb64[k] / (kms["fp64"][k] * 1e-3) / 1e9 for k in kb64}}
+    dom = max(kb32, key=lambda k: kms["mixed"][k])  # dominant kernel of the measured (mixed) apply
+    log("mixed: " + "  ".join(f"{k} {kms['mixed'][k]:.3f} ms {gbs['mixed'][k]:.0f} GB/s" for k in kb32))
+    log("fp64:  " + "  ".join(f"{k} {kms['fp64'][k]:.3f} ms {gbs['fp64'][k]:.0f} GB/s" for k in kb64))
 
     out = {
         "metric": METRIC,
@@ -197,17 +206,16 @@ def main():
                     "n_far": st["n_far"], "max_rank": st["max_rank"], "nze": st["nze"],
                     "near_kernel_evals": st["near_kernel_evals"], "far_kernel_evals": st["far_kernel_evals"]},
         "near_field_evals_per_s": st["near_kernel_evals"] / st["t_near"] if st["t_near"] > 0 else None,
-        "kernel_ms": {"mixed": {"gather": prof32[0], "far_BtX": prof32[1], "rows": prof32[2], "scatter": prof32[3],
-                                "apply": prof32[4]},
-                      "fp64": {"gather": prof64[0], "far_BtX": prof64[1], "rows": prof64[2], "scatter": prof64[3],
-                               "apply": prof64[4]}},
-        "roofline": {"kernel": "k_mv_rows<cx<float>,cx<float>>", "bound": "hbm", "achieved": rows_gbs32,
-                     "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": rows_gbs32 / HBM_PEAK_GBS, "traffic": None,
-                     "algorithmic_bytes_per_launch": rows_bytes(8),
-                     "fp64_achieved": rows_gbs64, "fp64_frac": rows_gbs64 / HBM_PEAK_GBS,
+        "kernel_ms": kms,
+        "kernel_gbs": gbs,
+        "roofline": {"kernel": f"{dom}<cx<float>,cx<float>>", "bound": "hbm", "achieved": gbs["mixed"][dom],
+                     "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": gbs["mixed"][dom] / HBM_PEAK_GBS, "traffic": None,
+                     "algorithmic_bytes_per_launch": kb32[dom],
+                     "fp64_achieved": gbs["fp64"][dom], "fp64_frac": gbs["fp64"][dom] / HBM_PEAK_GBS,
                      "apply_bytes_mixed": apply_bytes(8), "apply_bytes_fp64": apply_bytes(16),
-                     "apply_gbs_mixed": apply_bytes(8) / (prof32[4] * 1e-3) / 1e9,
-                     "apply_gbs_fp64": apply_bytes(16) / (prof64[4] * 1e-3) / 1e9},
+                     "apply_gbs_mixed": apply_bytes(8) / (kms["mixed"]["apply"] * 1e-3) / 1e9,
+                     "apply_gbs_fp64": apply_bytes(16) / (kms["fp64"]["apply"] * 1e-3) / 1e9,
+                     "apply_frac_mixed": apply_bytes(8) / (kms["mixed"]["apply"] * 1e-3) / 1e9 / HBM_PEAK_GBS},
     }
 
     if not args.no_solve:
@@ -251,9 +259,10 @@ def main():
         out["cpu_baseline"] = cpu_baseline(C, mesh, kappa, params, st)
 
     if args.traffic and os.path.exists(args.traffic) and args.level == 6 and kappa == 10.0:
-        for line in open(args.traffic):
-            if line.startswith("bytes_per_launch:"):
-                out["roofline"]["traffic"] = float(line.split(":")[1])
+        for line in open(args.traffic):  # "bytes_per_launch <kernel>: <bytes>"
+            key, _, val = line.partition(":")
+            if key.strip() == f"bytes_per_launch {dom}":
+                out["roofline"]["traffic"] = float(val)
                 out["roofline"]["traffic_source"] = os.path.relpath(args.traffic, ROOT)
     if args.calib_copy:
         a = torch.empty(512 << 20, dtype=torch.uint8, device=dev).fill_(1)

@@ -134,9 +134,10 @@ int bem_op_assemble(bem_op* op, void* stream);
 int bem_op_apply(bem_op* op, int flags, const double* alpha, const void* x, const double* beta, void* y,
                  int nrhs, int ld, int dtype, void* stream);
 int bem_op_stats(const bem_op* op, bem_stats* st);
-/* measurement: run `iters` applies on device buffers of ones and return the
- * average milliseconds of [gather, low-rank B^T x, row/column kernel, scatter,
- * whole apply] from HIP events recorded on `stream` between the launches.
+/* measurement: run `iters` applies on device buffers of ones and return in
+ * ms[6] the average milliseconds of [gather, low-rank t = V x (Ut x),
+ * low-rank z = U t (V^T t), row/column kernel, scatter, whole apply] from HIP
+ * events recorded on `stream` between the launches.
  * No reference counterpart (the reference times with NGSolve Timers,
  * src/hmat.cpp:650-651). */
 int bem_op_profile_apply(bem_op* op, int flags, int iters, double* ms, void* stream);

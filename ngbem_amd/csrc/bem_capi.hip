@@ -328,21 +328,21 @@ int bem_op_profile_apply(bem_op* h, int flags, int iters, double* ms, void* stre
     dy.alloc((size_t)n * esz);
     std::vector<double> ones((size_t)n * esz / 8, 1.0);
     BEM_HIP(hipMemcpyAsync(dx.p, ones.data(), (size_t)n * esz, hipMemcpyHostToDevice, s));
-    hipEvent_t ev[5];
+    hipEvent_t ev[6];
     for (auto& e : ev) BEM_HIP(hipEventCreate(&e));
     const double one[2] = {1.0, 0.0}, zero[2] = {0.0, 0.0};
-    for (int k = 0; k < 5; k++) ms[k] = 0.0;
+    for (int k = 0; k < 6; k++) ms[k] = 0.0;
     const int dflags = flags & (BEM_APPLY_TRANS | BEM_APPLY_CONJ | BEM_APPLY_MIXED);
     for (int it = 0; it < iters; it++) {
       bem::apply_device(op, dflags, one, dx.p, zero, dy.p, s, ev);
-      BEM_HIP(hipEventSynchronize(ev[4]));
+      BEM_HIP(hipEventSynchronize(ev[5]));
       float t;
-      for (int k = 0; k < 4; k++) {
+      for (int k = 0; k < 5; k++) {
         BEM_HIP(hipEventElapsedTime(&t, ev[k], ev[k + 1]));
         ms[k] += t / iters;
       }
-      BEM_HIP(hipEventElapsedTime(&t, ev[0], ev[4]));
-      ms[4] += t / iters;
+      BEM_HIP(hipEventElapsedTime(&t, ev[0], ev[5]));
+      ms[5] += t / iters;
     }
     for (auto& e : ev) (void)hipEventDestroy(e);
   });

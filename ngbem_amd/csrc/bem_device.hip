@@ -986,6 +986,52 @@ __global__ __launch_bounds__(256) void k_mv_far_t(const RowItem* __restrict__ it
   if (li == 0 && act) tbuf[b.toff + l] = a0;
 }
 
+// z_b = U_b t_b (forward) or V_b^T t_b (transpose, conjugated for the
+// adjoint): one wave per 64 rows (columns) of a low-rank block, lanes along the
+// contiguous direction of Ut (V), rank loop unrolled by four; each block's
+// factor is read once, contiguously.
+struct ZItem {
+  int blk, r0;
+};
+
+template <class ST, class VT, bool TRANS, bool CONJ>
+__global__ __launch_bounds__(256) void k_mv_far_z(const ZItem* __restrict__ items, int nitems,
+                                                  const FarBlk* __restrict__ fb, const ST* __restrict__ fUt,
+                                                  const ST* __restrict__ fV, const VT* __restrict__ tbuf,
+                                                  VT* __restrict__ z) {
+  const int w = blockIdx.x * 4 + (threadIdx.x >> 6);
+  if (w >= nitems) return;
+  const ZItem it = items[w];
+  const FarBlk b = fb[it.blk];
+  const int lane = threadIdx.x & 63;
+  const int len = TRANS ? b.n : b.m;
+  const int r = it.r0 + lane;
+  if (r >= len) return;
+  const ST* F = (TRANS ? fV + b.voff : fUt + b.uoff) + r;
+  const VT* t = tbuf + b.toff;
+  VT a0 = zero_<VT>(), a1 = zero_<VT>(), a2 = zero_<VT>(), a3 = zero_<VT>();
+  int l = 0;
+  for (; l + 3 < b.R; l += 4) {
+    VT u0, u1, u2, u3;
+    cvt(u0, F[(int64_t)l * len]);
+    cvt(u1, F[(int64_t)(l + 1) * len]);
+    cvt(u2, F[(int64_t)(l + 2) * len]);
+    cvt(u3, F[(int64_t)(l + 3) * len]);
+    if (CONJ) { u0 = cconj(u0); u1 = cconj(u1); u2 = cconj(u2); u3 = cconj(u3); }
+    a0 += u0 * t[l];
+    a1 += u1 * t[l + 1];
+    a2 += u2 * t[l + 2];
+    a3 += u3 * t[l + 3];
+  }
+  for (; l < b.R; l++) {
+    VT u;
+    cvt(u, F[(int64_t)l * len]);
+    if (CONJ) u = cconj(u);
+    a0 += u * t[l];
+  }
+  z[(TRANS ? b.zoffT : b.zoff) + r] = (a0 + a1) + (a2 + a3);
+}
+
 struct Tile {
   int p0, len;       // positions owned (len <= 64)
   int s0, s1;        // segment range
@@ -999,9 +1045,8 @@ struct Tile {
 // once at the end
 template <class ST, class VT>
 __global__ __launch_bounds__(256) void k_mv_rows(const Tile* __restrict__ tiles, const Seg* __restrict__ segs,
-                                                 const ST* __restrict__ nval, const ST* __restrict__ fUt,
-                                                 const VT* __restrict__ xp, const VT* __restrict__ tbuf,
-                                                 VT* __restrict__ yp) {
+                                                 const ST* __restrict__ nval, const VT* __restrict__ z,
+                                                 const VT* __restrict__ xp, VT* __restrict__ yp) {
   __shared__ VT red[4][64];
   __shared__ Seg s_seg[64];
   const Tile tl = tiles[blockIdx.x];
@@ -1017,8 +1062,12 @@ __global__ __launch_bounds__(256) void k_mv_rows(const Tile* __restrict__ tiles,
     __syncthreads();
     for (int k = 0; k < ns; k++) {
       const Seg sg = s_seg[k];
-      const ST* col = (sg.far ? fUt : nval) + sg.base + r;
-      const VT* xs = (sg.far ? tbuf : xp) + sg.xoff;
+      if (sg.far) {  // the block's U t, computed by k_mv_far_z
+        if (wv == (k & 3)) acc1 += z[sg.base + r];
+        continue;
+      }
+      const ST* col = nval + sg.base + r;
+      const VT* xs = xp + sg.xoff;
       const int64_t ld = sg.ld;
       int j = wv;
       for (; j + 12 < sg.cnt; j += 16) {
@@ -1049,9 +1098,8 @@ __global__ __launch_bounds__(256) void k_mv_rows(const Tile* __restrict__ tiles,
 // low rank: lanes own columns, waves split the rank rows of V
 template <class ST, class VT, bool CONJ>
 __global__ __launch_bounds__(256) void k_mv_cols(const Tile* __restrict__ tiles, const Seg* __restrict__ segs,
-                                                 const ST* __restrict__ nval, const ST* __restrict__ fV,
-                                                 const VT* __restrict__ xp, const VT* __restrict__ tbuf,
-                                                 VT* __restrict__ yp) {
+                                                 const ST* __restrict__ nval, const VT* __restrict__ zT,
+                                                 const VT* __restrict__ xp, VT* __restrict__ yp) {
   __shared__ VT red[4][64];
   __shared__ VT colsum[64];
   __shared__ Seg s_seg[64];
@@ -1092,15 +1140,8 @@ __global__ __launch_bounds__(256) void k_mv_cols(const Tile* __restrict__ tiles,
           a = wave_sum(a + a2);
           if (lane == 0) colsum[cc] += a;
         }
-      } else {
-        const ST* row = fV + sg.base + c;
-        const VT* tt = tbuf + sg.xoff;
-        for (int l = wv; l < sg.cnt; l += 4) {
-          VT u;
-          cvt(u, row[(int64_t)l * sg.ld]);
-          if (CONJ) u = cconj(u);
-          acc += u * tt[l];
-        }
+      } else if (wv == (k & 3)) {  // the block's V^T t, computed by k_mv_far_z
+        acc += zT[sg.base + c];
       }
     }
   }
@@ -1920,7 +1961,10 @@ void assemble_far(Operator& op, hipStream_t s) {
 
 // ---------------------------------------------------------- matvec plans --
 
-static void plan(const Operator& op, bool trans, std::vector<Tile>& tiles, std::vector<Seg>& segs) {
+// zsegs: low-rank blocks as z slices (vector path); else as Ut columns times t
+// (block product, forward only)
+static void plan(const Operator& op, bool trans, std::vector<Tile>& tiles, std::vector<Seg>& segs,
+                 bool zsegs = true) {
   const Tree& tree = trans ? op.trial_tree : *op.test_tree;
   const int nleaf = (int)tree.leaves.size();
   std::vector<int> lp0(nleaf);
@@ -1945,10 +1989,12 @@ static void plan(const Operator& op, bool trans, std::vector<Tile>& tiles, std::
         add(nb.c0, nb.n, [&](int o) { return Seg{nb.off + (int64_t)o * nb.m, nb.m, nb.m, nb.r0, 0}; });
     } else {
       const FarBlk& fb = op.far[slot];
-      if (!trans)
+      if (!trans && zsegs)
+        add(fb.r0, fb.m, [&](int o) { return Seg{(int64_t)fb.zoff + o, 0, 1, 0, 2}; });
+      else if (!trans)
         add(fb.r0, fb.m, [&](int o) { return Seg{fb.uoff + o, fb.m, fb.R, fb.toff, 1}; });
       else
-        add(fb.c0, fb.n, [&](int o) { return Seg{fb.voff + o, fb.n, fb.R, fb.toff, 1}; });
+        add(fb.c0, fb.n, [&](int o) { return Seg{(int64_t)fb.zoffT + o, 0, 1, 0, 2}; });
     }
   }
   tiles.clear();
@@ -1980,13 +2026,41 @@ static void upload_row_items(Operator& op, hipStream_t s) {
   }
 }
 
+// z offsets of the low-rank blocks and the U t / V^T t work items
+static void upload_z_items(Operator& op, hipStream_t s) {
+  int zo = 0, zt = 0;
+  for (FarBlk& fb : op.far) {
+    fb.zoff = zo;
+    fb.zoffT = zt;
+    zo += fb.m;
+    zt += fb.n;
+  }
+  op.z_total = zo;
+  op.zT_total = zt;
+  op.d_far_blk.upload(op.far, s);
+  for (int trans = 0; trans < 2; trans++) {
+    std::vector<ZItem> items;
+    for (size_t k = 0; k < op.far.size(); k++) {
+      const int len = trans ? op.far[k].n : op.far[k].m;
+      for (int r = 0; r < len; r += 64) items.push_back(ZItem{(int)k, r});
+    }
+    (trans ? op.d_zitems_trn : op.d_zitems_fwd).upload(items, s);
+    (trans ? op.n_zitems_trn : op.n_zitems_fwd) = (int)items.size();
+  }
+}
+
 void build_matvec_plan(Operator& op, hipStream_t s) {
+  upload_z_items(op, s);
   std::vector<Tile> tiles;
   std::vector<Seg> segs;
+  plan(op, false, tiles, segs, false);
+  op.d_seg_fwd_mm.upload(segs, s);
+  BEM_HIP(hipStreamSynchronize(s));
   plan(op, false, tiles, segs);
   op.d_tile_fwd.upload(tiles, s);
   op.d_seg_fwd.upload(segs, s);
   op.n_tiles_fwd = (int)tiles.size();
+  BEM_HIP(hipStreamSynchronize(s));  // the host vectors are reused below
   plan(op, true, tiles, segs);
   op.d_tile_trn.upload(tiles, s);
   op.d_seg_trn.upload(segs, s);
@@ -1997,6 +2071,8 @@ void build_matvec_plan(Operator& op, hipStream_t s) {
   op.d_xp.alloc((size_t)nmax * esz);
   op.d_yp.alloc((size_t)nmax * esz);
   op.d_t.alloc((size_t)std::max(op.t_total, 1) * esz);
+  op.d_z.alloc((size_t)std::max(op.z_total, 1) * esz);
+  op.d_zT.alloc((size_t)std::max(op.zT_total, 1) * esz);
   BEM_HIP(hipStreamSynchronize(s));
 }
 
@@ -2066,10 +2142,10 @@ void build_batch(const std::vector<Operator*>& ops, Operator& b, hipStream_t s, 
   }
   std::vector<int> perm_tr, perm_te;
   std::vector<Tile> tf, tt, tiles;
-  std::vector<Seg> sf, st, segs;
+  std::vector<Seg> sf, st, smm, segs;
   b.far.clear();
   int64_t NO = 0, UO = 0, VO = 0;
-  int TO = 0, XO = 0, YO = 0;
+  int TO = 0, XO = 0, YO = 0, ZO = 0, ZOT = 0;
   auto cp = [&](DBuf& dst, int64_t off, const DBuf& src, size_t n, size_t e) {
     if (n) BEM_HIP(hipMemcpyAsync((char*)dst.p + off * e, src.p, n * e, hipMemcpyDeviceToDevice, s));
   };
@@ -2093,23 +2169,35 @@ void build_batch(const std::vector<Operator*>& ops, Operator& b, hipStream_t s, 
       fb.toff += TO;
       b.far.push_back(fb);
     }
-    for (int trans = 0; trans < 2; trans++) {
-      plan(*op, trans, tiles, segs);
-      std::vector<Tile>& T = trans ? tt : tf;
-      std::vector<Seg>& S = trans ? st : sf;
+    for (int pass = 0; pass < 3; pass++) {  // forward, transpose, forward of the block product
+      const bool trans = pass == 1, mm = pass == 2;
+      plan(*op, trans, tiles, segs, !mm);
+      std::vector<Seg>& S = mm ? smm : (trans ? st : sf);
       const int s0 = (int)S.size();
-      for (Tile t : tiles) {
-        t.p0 += trans ? XO : YO;
-        t.s0 += s0;
-        t.s1 += s0;
-        T.push_back(t);
+      if (!mm) {
+        std::vector<Tile>& T = trans ? tt : tf;
+        for (Tile t : tiles) {
+          t.p0 += trans ? XO : YO;
+          t.s0 += s0;
+          t.s1 += s0;
+          T.push_back(t);
+        }
       }
       for (Seg g : segs) {
-        g.base += g.far ? (trans ? VO : UO) : NO;
-        g.xoff += g.far ? TO : (trans ? YO : XO);
+        if (g.far == 2) {
+          g.base += trans ? ZOT : ZO;
+        } else if (g.far == 1) {
+          g.base += UO;
+          g.xoff += TO;
+        } else {
+          g.base += NO;
+          g.xoff += trans ? YO : XO;
+        }
         S.push_back(g);
       }
     }
+    ZO += op->z_total;
+    ZOT += op->zT_total;
     if (consume) {  // the operator's block storage moves into the batch
       BEM_HIP(hipStreamSynchronize(s));
       Operator* o = const_cast<Operator*>(op);
@@ -2137,9 +2225,11 @@ void build_batch(const std::vector<Operator*>& ops, Operator& b, hipStream_t s, 
   b.t_total = TO;
   b.d_perm_trial.upload(perm_tr, s);
   b.d_perm_test.upload(perm_te, s);
-  b.d_far_blk.upload(b.far, s);
+  upload_z_items(b, s);  // same cumulative z offsets as the shifted segments
+  if (b.z_total != ZO || b.zT_total != ZOT) throw std::runtime_error("batch: z offsets disagree");
   b.d_tile_fwd.upload(tf, s);
   b.d_seg_fwd.upload(sf, s);
+  b.d_seg_fwd_mm.upload(smm, s);
   b.n_tiles_fwd = (int)tf.size();
   b.d_tile_trn.upload(tt, s);
   b.d_seg_trn.upload(st, s);
@@ -2149,6 +2239,8 @@ void build_batch(const std::vector<Operator*>& ops, Operator& b, hipStream_t s, 
   b.d_xp.alloc((size_t)nmax * esz);
   b.d_yp.alloc((size_t)nmax * esz);
   b.d_t.alloc((size_t)std::max(TO, 1) * esz);
+  b.d_z.alloc((size_t)std::max(ZO, 1) * esz);
+  b.d_zT.alloc((size_t)std::max(ZOT, 1) * esz);
   b.assembled = true;
   BEM_HIP(hipStreamSynchronize(s));
 }
@@ -2189,35 +2281,55 @@ static void apply_t(Operator& op, int flags, const XT* x, XT* y, XT alpha, XT be
   const int* pin = trans ? op.d_perm_test.as<int>() : op.d_perm_trial.as<int>();
   const int* pout = trans ? op.d_perm_trial.as<int>() : op.d_perm_test.as<int>();
   const int nfar = (int)op.far.size();
+  VT* z = trans ? op.d_zT.as<VT>() : op.d_z.as<VT>();
   mark(0);
   hipLaunchKernelGGL((k_gather<VT, XT>), dim3((nin + 255) / 256), dim3(256), 0, s, pin, nin, x, xp, 0);
   mark(1);
+  // low rank: t = V x (forward) / Ut x (transpose, conjugated for the adjoint),
+  // then z = U t / V^T t per block; dense blocks and z slices per row tile
   if (!trans) {
-    if (nfar)
+    if (nfar) {
       hipLaunchKernelGGL((k_mv_far_t<ST, VT, false, false>), dim3((op.n_items_fwd + 3) / 4), dim3(256), 0, s,
                          op.d_items_fwd.as<RowItem>(), op.n_items_fwd, op.d_far_blk.as<FarBlk>(), fU, fV, xp, tb);
-    mark(2);
+      mark(2);
+      hipLaunchKernelGGL((k_mv_far_z<ST, VT, false, false>), dim3((op.n_zitems_fwd + 3) / 4), dim3(256), 0, s,
+                         op.d_zitems_fwd.as<ZItem>(), op.n_zitems_fwd, op.d_far_blk.as<FarBlk>(), fU, fV, tb, z);
+    } else {
+      mark(2);
+    }
+    mark(3);
     hipLaunchKernelGGL((k_mv_rows<ST, VT>), dim3(op.n_tiles_fwd), dim3(256), 0, s, op.d_tile_fwd.as<Tile>(),
-                       op.d_seg_fwd.as<Seg>(), nval, fU, xp, tb, yp);
-  } else if (!conj) {
-    if (nfar)
-      hipLaunchKernelGGL((k_mv_far_t<ST, VT, true, false>), dim3((op.n_items_trn + 3) / 4), dim3(256), 0, s,
-                         op.d_items_trn.as<RowItem>(), op.n_items_trn, op.d_far_blk.as<FarBlk>(), fU, fV, xp, tb);
-    mark(2);
-    hipLaunchKernelGGL((k_mv_cols<ST, VT, false>), dim3(op.n_tiles_trn), dim3(256), 0, s, op.d_tile_trn.as<Tile>(),
-                       op.d_seg_trn.as<Seg>(), nval, fV, xp, tb, yp);
+                       op.d_seg_fwd.as<Seg>(), nval, z, xp, yp);
   } else {
-    if (nfar)
-      hipLaunchKernelGGL((k_mv_far_t<ST, VT, true, true>), dim3((op.n_items_trn + 3) / 4), dim3(256), 0, s,
-                         op.d_items_trn.as<RowItem>(), op.n_items_trn, op.d_far_blk.as<FarBlk>(), fU, fV, xp, tb);
-    mark(2);
-    hipLaunchKernelGGL((k_mv_cols<ST, VT, true>), dim3(op.n_tiles_trn), dim3(256), 0, s, op.d_tile_trn.as<Tile>(),
-                       op.d_seg_trn.as<Seg>(), nval, fV, xp, tb, yp);
+    if (nfar) {
+      if (conj)
+        hipLaunchKernelGGL((k_mv_far_t<ST, VT, true, true>), dim3((op.n_items_trn + 3) / 4), dim3(256), 0, s,
+                           op.d_items_trn.as<RowItem>(), op.n_items_trn, op.d_far_blk.as<FarBlk>(), fU, fV, xp, tb);
+      else
+        hipLaunchKernelGGL((k_mv_far_t<ST, VT, true, false>), dim3((op.n_items_trn + 3) / 4), dim3(256), 0, s,
+                           op.d_items_trn.as<RowItem>(), op.n_items_trn, op.d_far_blk.as<FarBlk>(), fU, fV, xp, tb);
+      mark(2);
+      if (conj)
+        hipLaunchKernelGGL((k_mv_far_z<ST, VT, true, true>), dim3((op.n_zitems_trn + 3) / 4), dim3(256), 0, s,
+                           op.d_zitems_trn.as<ZItem>(), op.n_zitems_trn, op.d_far_blk.as<FarBlk>(), fU, fV, tb, z);
+      else
+        hipLaunchKernelGGL((k_mv_far_z<ST, VT, true, false>), dim3((op.n_zitems_trn + 3) / 4), dim3(256), 0, s,
+                           op.d_zitems_trn.as<ZItem>(), op.n_zitems_trn, op.d_far_blk.as<FarBlk>(), fU, fV, tb, z);
+    } else {
+      mark(2);
+    }
+    mark(3);
+    if (conj)
+      hipLaunchKernelGGL((k_mv_cols<ST, VT, true>), dim3(op.n_tiles_trn), dim3(256), 0, s, op.d_tile_trn.as<Tile>(),
+                         op.d_seg_trn.as<Seg>(), nval, z, xp, yp);
+    else
+      hipLaunchKernelGGL((k_mv_cols<ST, VT, false>), dim3(op.n_tiles_trn), dim3(256), 0, s, op.d_tile_trn.as<Tile>(),
+                         op.d_seg_trn.as<Seg>(), nval, z, xp, yp);
   }
-  mark(3);
+  mark(4);
   hipLaunchKernelGGL((k_scatter<XT, VT>), dim3((nout + 255) / 256), dim3(256), 0, s, pout, nout, yp, y, alpha, beta,
                      (int)beta_zero, 0);
-  mark(4);
+  mark(5);
 }
 
 // forward block apply, 64 right-hand sides per pass
@@ -2250,7 +2362,7 @@ static void apply_mm_t(Operator& op, const XT* x, XT* y, int nrhs, int64_t ld, X
       hipLaunchKernelGGL((k_mm_far_t<ST, VT>), dim3(op.n_mm_items), dim3(256), 0, s, op.d_mm_items.as<MMItem>(),
                          op.d_far_blk.as<FarBlk>(), fV, xm, tm);
     hipLaunchKernelGGL((k_mm_rows<ST, VT>), dim3(op.n_tiles_fwd), dim3(256), 0, s, op.d_tile_fwd.as<Tile>(),
-                       op.d_seg_fwd.as<Seg>(), nval, fU, xm, tm, ym);
+                       op.d_seg_fwd_mm.as<Seg>(), nval, fU, xm, tm, ym);
     hipLaunchKernelGGL((k_scatter_mm<XT, VT>), dim3((unsigned)(((int64_t)nout * MM_W + 255) / 256)), dim3(256), 0, s,
                        op.d_perm_test.as<int>(), nout, ym, y, ld, r0, nr, alpha, beta, (int)beta_zero);
   }

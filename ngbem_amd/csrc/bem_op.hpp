@@ -119,6 +119,7 @@ struct FarBlk {
   int r0, m, c0, n;
   int R;               // total rank (sum over terms)
   int toff;            // offset in the matvec scratch for B^T x / A^T x
+  int zoff, zoffT;     // offsets of U t (m entries) / V^T t (n entries) in the z scratch
   int pI, pm, pJ, pn;  // sorted patches (ACA point order)
   int rmap, cmap;      // offsets of the row/col projection maps
 };
@@ -138,10 +139,12 @@ struct AcaTask {
 
 // matvec segment: the part of block `blk` that lies in one leaf tile
 // Self-contained, so a tile can stage all of its segments in LDS with one load
-// round.  Forward (lanes own rows): value (r, j) of the leaf's rows sits at
-// base + r + j*ld, j < cnt, multiplied by xs[xoff + j].  Transpose: dense
-// columns start at base + c*ld and hold cnt rows (x at xoff); low-rank rows
-// of V start at base + l*ld (+ c), l < cnt (t at xoff).
+// round.  far = 0 (dense): forward (lanes own rows) value (r, j) of the leaf's
+// rows sits at base + r + j*ld, j < cnt, multiplied by x[xoff + j]; transpose:
+// columns start at base + c*ld and hold cnt rows (x at xoff).  far = 2 (low
+// rank, vector path): add z[base + r] (forward) / zT[base + c] (transpose), the
+// block's U t or V^T t computed beforehand.  far = 1 (low rank, block product):
+// Ut columns as in the dense forward case, multiplied by t[xoff + l].
 struct Seg {
   int64_t base;
   int ld, cnt, xoff, far;
@@ -192,7 +195,12 @@ struct Operator {
   DBuf d_far_blk, d_fU, d_fV, d_fU32, d_fV32, d_far_patch, d_far_maps;
   // matvec plans
   DBuf d_seg_fwd, d_tile_fwd, d_seg_trn, d_tile_trn;
+  DBuf d_seg_fwd_mm;       // forward segments of the block product (far = 1)
   int n_tiles_fwd = 0, n_tiles_trn = 0;
+  DBuf d_zitems_fwd, d_zitems_trn;  // U t / V^T t work items
+  int n_zitems_fwd = 0, n_zitems_trn = 0;
+  int z_total = 0, zT_total = 0;    // scratch lengths for U t / V^T t
+  DBuf d_z, d_zT;
   DBuf d_items_fwd, d_items_trn;  // low-rank B^T x work items
   int n_items_fwd = 0, n_items_trn = 0;
   int t_total = 0;         // scratch length for far B^T x
@@ -221,8 +229,9 @@ void assemble_far(Operator& op, hipStream_t s);
 void build_matvec_plan(Operator& op, hipStream_t s);
 void make_mixed_copy(Operator& op, hipStream_t s);
 // y = alpha * op(A) x + beta * y on device vectors (complex128 or double)
-// ev (optional, 5 events): recorded before the gather and after the gather,
-// the low-rank B^T x stage, the row/column kernel and the scatter
+// ev (optional, 6 events): recorded before the gather and after the gather,
+// the low-rank t stage, the low-rank z stage, the row/column kernel and the
+// scatter
 void apply_device(Operator& op, int flags, const double* alpha, const void* x, const double* beta, void* y,
                   hipStream_t s, hipEvent_t* ev = nullptr);
 // block-diagonal union of assembled operators (storage copied, one plan)
