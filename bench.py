@@ -163,14 +163,23 @@ def main():
             R_tot += bi["rank"]
             m_tot += bi["m"]
 
-    # algorithmic bytes per launch (each operand once) of the three apply kernels
+    # algorithmic bytes per launch (each operand once) of the apply kernels; the
+    # forward plan is the leaf-blocked U inside the row tiles unless
+    # NGBEM_MV_PLAN=z (then U t per block in k_mv_far_z, z slices in the tiles)
+    zplan = os.environ.get("NGBEM_MV_PLAN", "leaf") == "z"
+
     def kernel_bytes(s):
-        return {"k_mv_far_t": far_v * s + n * s + R_tot * s,          # V, x, t
-                "k_mv_far_z": far_u * s + R_tot * s + m_tot * s,      # Ut, t, z
-                "k_mv_rows": near * s + n * s + m_tot * s + n * s}    # dense, x, z, y
+        kb = {"k_mv_far_t": far_v * s + n * s + R_tot * s}            # V, x, t
+        if zplan:
+            kb["k_mv_far_z"] = far_u * s + R_tot * s + m_tot * s      # Ut, t, z
+            kb["k_mv_rows"] = near * s + n * s + m_tot * s + n * s    # dense, x, z, y
+        else:
+            kb["k_mv_rows"] = (near + far_u) * s + n * s + R_tot * s + n * s  # dense, U, x, t, y
+        return kb
 
     def apply_bytes(s):
-        return (near + far_u + far_v) * s + 2 * n * s + 2 * R_tot * s + 2 * m_tot * s
+        z = 2 * m_tot * s if zplan else 0
+        return (near + far_u + far_v) * s + 2 * n * s + 2 * R_tot * s + z
 
     kms = {"mixed": dict(zip(["gather", "k_mv_far_t", "k_mv_far_z", "k_mv_rows", "scatter", "apply"], prof32)),
            "fp64": dict(zip(["gather", "k_mv_far_t", "k_mv_far_z", "k_mv_rows", "scatter", "apply"], prof64))}

@@ -376,6 +376,7 @@ int bem_op_stats(const bem_op* h, bem_stats* st) {
 }
 
 int bem_op_destroy(bem_op* h) {
+  if (h) (void)hipDeviceSynchronize();  // no apply of this operator may still be running
   delete h;
   return 0;
 }
@@ -528,6 +529,7 @@ int bem_batch_apply(bem_batch* b, int flags, const double* alpha, const void* x,
 }
 
 int bem_batch_destroy(bem_batch* b) {
+  if (b) (void)hipDeviceSynchronize();
   delete b;
   return 0;
 }

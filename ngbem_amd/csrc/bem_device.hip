@@ -1045,8 +1045,9 @@ struct Tile {
 // once at the end
 template <class ST, class VT>
 __global__ __launch_bounds__(256) void k_mv_rows(const Tile* __restrict__ tiles, const Seg* __restrict__ segs,
-                                                 const ST* __restrict__ nval, const VT* __restrict__ z,
-                                                 const VT* __restrict__ xp, VT* __restrict__ yp) {
+                                                 const ST* __restrict__ nval, const ST* __restrict__ fUt,
+                                                 const VT* __restrict__ z, const VT* __restrict__ xp,
+                                                 const VT* __restrict__ tbuf, VT* __restrict__ yp) {
   __shared__ VT red[4][64];
   __shared__ Seg s_seg[64];
   const Tile tl = tiles[blockIdx.x];
@@ -1062,12 +1063,13 @@ __global__ __launch_bounds__(256) void k_mv_rows(const Tile* __restrict__ tiles,
     __syncthreads();
     for (int k = 0; k < ns; k++) {
       const Seg sg = s_seg[k];
-      if (sg.far) {  // the block's U t, computed by k_mv_far_z
+      if (sg.far == 2) {  // the block's U t, computed by k_mv_far_z
         if (wv == (k & 3)) acc1 += z[sg.base + r];
         continue;
       }
-      const ST* col = nval + sg.base + r;
-      const VT* xs = xp + sg.xoff;
+      // dense columns, or the columns of U (Ut rows) times t
+      const ST* col = (sg.far ? fUt : nval) + sg.base + r;
+      const VT* xs = (sg.far ? tbuf : xp) + sg.xoff;
       const int64_t ld = sg.ld;
       int j = wv;
       for (; j + 12 < sg.cnt; j += 16) {
@@ -1961,21 +1963,25 @@ void assemble_far(Operator& op, hipStream_t s) {
 
 // ---------------------------------------------------------- matvec plans --
 
-// zsegs: low-rank blocks as z slices (vector path); else as Ut columns times t
-// (block product, forward only)
+// How the forward plan covers the low-rank blocks: PLAN_Z as slices of z = U t
+// (computed per block beforehand), PLAN_UT as columns of U (Ut rows, stride m)
+// times t, PLAN_LEAF as columns of the leaf-blocked copy of U (the rows of one
+// leaf stored together, ld = leaf size) times t.  The transpose always uses z.
+enum PlanKind { PLAN_Z = 0, PLAN_UT = 1, PLAN_LEAF = 2 };
+
 static void plan(const Operator& op, bool trans, std::vector<Tile>& tiles, std::vector<Seg>& segs,
-                 bool zsegs = true) {
+                 int kind = PLAN_Z) {
   const Tree& tree = trans ? op.trial_tree : *op.test_tree;
   const int nleaf = (int)tree.leaves.size();
   std::vector<int> lp0(nleaf);
   for (int i = 0; i < nleaf; i++) lp0[i] = (int)tree.cl[tree.leaves[i]].permupos;
   std::vector<std::vector<Seg>> per(nleaf);
-  // p0/len: the block's position range on the tiled side; mk(o): segment for
-  // a leaf starting o positions into the block
+  // p0/len: the block's position range on the tiled side; mk(o, ls): segment
+  // for a leaf of ls positions starting o positions into the block
   auto add = [&](int p0, int len, auto mk) {
     int li = (int)(std::upper_bound(lp0.begin(), lp0.end(), p0) - lp0.begin()) - 1;
     while (li < nleaf && lp0[li] < p0 + len) {
-      per[li].push_back(mk(lp0[li] - p0));
+      per[li].push_back(mk(lp0[li] - p0, (int)tree.cl[tree.leaves[li]].number));
       li++;
     }
   };
@@ -1984,17 +1990,19 @@ static void plan(const Operator& op, bool trans, std::vector<Tile>& tiles, std::
     if (op.blocks[b].near || op.blocks[b].fallback) {
       const NearBlk& nb = op.near[slot];
       if (!trans)
-        add(nb.r0, nb.m, [&](int o) { return Seg{nb.off + o, nb.m, nb.n, nb.c0, 0}; });
+        add(nb.r0, nb.m, [&](int o, int) { return Seg{nb.off + o, nb.m, nb.n, nb.c0, 0}; });
       else
-        add(nb.c0, nb.n, [&](int o) { return Seg{nb.off + (int64_t)o * nb.m, nb.m, nb.m, nb.r0, 0}; });
+        add(nb.c0, nb.n, [&](int o, int) { return Seg{nb.off + (int64_t)o * nb.m, nb.m, nb.m, nb.r0, 0}; });
     } else {
       const FarBlk& fb = op.far[slot];
-      if (!trans && zsegs)
-        add(fb.r0, fb.m, [&](int o) { return Seg{(int64_t)fb.zoff + o, 0, 1, 0, 2}; });
-      else if (!trans)
-        add(fb.r0, fb.m, [&](int o) { return Seg{fb.uoff + o, fb.m, fb.R, fb.toff, 1}; });
+      if (trans)
+        add(fb.c0, fb.n, [&](int o, int) { return Seg{(int64_t)fb.zoffT + o, 0, 1, 0, 2}; });
+      else if (kind == PLAN_Z)
+        add(fb.r0, fb.m, [&](int o, int) { return Seg{(int64_t)fb.zoff + o, 0, 1, 0, 2}; });
+      else if (kind == PLAN_UT)
+        add(fb.r0, fb.m, [&](int o, int) { return Seg{fb.uoff + o, fb.m, fb.R, fb.toff, 1}; });
       else
-        add(fb.c0, fb.n, [&](int o) { return Seg{(int64_t)fb.zoffT + o, 0, 1, 0, 2}; });
+        add(fb.r0, fb.m, [&](int o, int ls) { return Seg{fb.uoff + (int64_t)o * fb.R, ls, fb.R, fb.toff, 1}; });
     }
   }
   tiles.clear();
@@ -2026,6 +2034,53 @@ static void upload_row_items(Operator& op, hipStream_t s) {
   }
 }
 
+// leaf-blocked copy of U: for every low-rank block and every leaf of its row
+// cluster, the leaf's rows of U (ls x R, column-major) stored together at
+// uoff + o R, o = the leaf's first row inside the block
+struct LChunk {
+  int64_t uoff;
+  int m, R, o, ls;
+};
+
+template <class T>
+__global__ __launch_bounds__(256) void k_leaf_blocked(const LChunk* __restrict__ chunks, const T* __restrict__ ut,
+                                                      T* __restrict__ ul) {
+  const LChunk c = chunks[blockIdx.x];
+  const int total = c.ls * c.R;
+  for (int idx = threadIdx.x; idx < total; idx += 256) {
+    const int l = idx / c.ls, i = idx - l * c.ls;
+    ul[c.uoff + (int64_t)c.o * c.R + idx] = ut[c.uoff + (int64_t)l * c.m + c.o + i];
+  }
+}
+
+static void build_leaf_blocked(Operator& op, hipStream_t s) {
+  const Tree& tree = *op.test_tree;
+  const int nleaf = (int)tree.leaves.size();
+  std::vector<int> lp0(nleaf);
+  for (int i = 0; i < nleaf; i++) lp0[i] = (int)tree.cl[tree.leaves[i]].permupos;
+  std::vector<LChunk> chunks;
+  size_t tu = 0;
+  for (const FarBlk& fb : op.far) {
+    tu = std::max(tu, (size_t)(fb.uoff + (int64_t)fb.m * fb.R));
+    int li = (int)(std::upper_bound(lp0.begin(), lp0.end(), fb.r0) - lp0.begin()) - 1;
+    for (; li < nleaf && lp0[li] < fb.r0 + fb.m; li++)
+      if (fb.R > 0) chunks.push_back(LChunk{fb.uoff, fb.m, fb.R, lp0[li] - fb.r0, (int)tree.cl[tree.leaves[li]].number});
+  }
+  const size_t esz = op.complex_valued ? 16 : 8;
+  op.d_fUL.alloc(std::max<size_t>(tu, 1) * esz);
+  if (chunks.empty()) return;
+  DBuf d_chunks;
+  d_chunks.upload(chunks, s);
+  if (op.complex_valued)
+    hipLaunchKernelGGL((k_leaf_blocked<zc>), dim3((unsigned)chunks.size()), dim3(256), 0, s, d_chunks.as<LChunk>(),
+                       op.d_fU.as<zc>(), op.d_fUL.as<zc>());
+  else
+    hipLaunchKernelGGL((k_leaf_blocked<double>), dim3((unsigned)chunks.size()), dim3(256), 0, s,
+                       d_chunks.as<LChunk>(), op.d_fU.as<double>(), op.d_fUL.as<double>());
+  BEM_HIP(hipGetLastError());
+  BEM_HIP(hipStreamSynchronize(s));
+}
+
 // z offsets of the low-rank blocks and the U t / V^T t work items
 static void upload_z_items(Operator& op, hipStream_t s) {
   int zo = 0, zt = 0;
@@ -2051,10 +2106,14 @@ static void upload_z_items(Operator& op, hipStream_t s) {
 
 void build_matvec_plan(Operator& op, hipStream_t s) {
   upload_z_items(op, s);
+  build_leaf_blocked(op, s);
   std::vector<Tile> tiles;
   std::vector<Seg> segs;
-  plan(op, false, tiles, segs, false);
+  plan(op, false, tiles, segs, PLAN_UT);
   op.d_seg_fwd_mm.upload(segs, s);
+  BEM_HIP(hipStreamSynchronize(s));
+  plan(op, false, tiles, segs, PLAN_LEAF);
+  op.d_seg_fwd_leaf.upload(segs, s);
   BEM_HIP(hipStreamSynchronize(s));
   plan(op, false, tiles, segs);
   op.d_tile_fwd.upload(tiles, s);
@@ -2099,6 +2158,7 @@ void make_mixed_copy(Operator& op, hipStream_t s) {
     tv = std::max(tv, (size_t)(fb.voff + (int64_t)fb.R * fb.n));
   }
   conv(op.d_fU, op.d_fU32, tu);
+  conv(op.d_fUL, op.d_fUL32, tu);
   conv(op.d_fV, op.d_fV32, tv);
   BEM_HIP(hipGetLastError());
   BEM_HIP(hipStreamSynchronize(s));
@@ -2134,15 +2194,17 @@ void build_batch(const std::vector<Operator*>& ops, Operator& b, hipStream_t s, 
   }
   b.d_near_val.alloc((size_t)std::max<int64_t>(nnear, 1) * esz);
   b.d_fU.alloc((size_t)std::max<int64_t>(nu, 1) * esz);
+  b.d_fUL.alloc((size_t)std::max<int64_t>(nu, 1) * esz);
   b.d_fV.alloc((size_t)std::max<int64_t>(nv, 1) * esz);
   if (b.mixed) {
     b.d_near_val32.alloc((size_t)std::max<int64_t>(nnear, 1) * esz32);
     b.d_fU32.alloc((size_t)std::max<int64_t>(nu, 1) * esz32);
+    b.d_fUL32.alloc((size_t)std::max<int64_t>(nu, 1) * esz32);
     b.d_fV32.alloc((size_t)std::max<int64_t>(nv, 1) * esz32);
   }
   std::vector<int> perm_tr, perm_te;
   std::vector<Tile> tf, tt, tiles;
-  std::vector<Seg> sf, st, smm, segs;
+  std::vector<Seg> sf, st, smm, sleaf, segs;
   b.far.clear();
   int64_t NO = 0, UO = 0, VO = 0;
   int TO = 0, XO = 0, YO = 0, ZO = 0, ZOT = 0;
@@ -2153,10 +2215,12 @@ void build_batch(const std::vector<Operator*>& ops, Operator& b, hipStream_t s, 
     const size_t lu = factor_len(*op, true), lv = factor_len(*op, false);
     cp(b.d_near_val, NO, op->d_near_val, (size_t)op->near_entries, esz);
     cp(b.d_fU, UO, op->d_fU, lu, esz);
+    cp(b.d_fUL, UO, op->d_fUL, lu, esz);
     cp(b.d_fV, VO, op->d_fV, lv, esz);
     if (b.mixed) {
       cp(b.d_near_val32, NO, op->d_near_val32, (size_t)op->near_entries, esz32);
       cp(b.d_fU32, UO, op->d_fU32, lu, esz32);
+      cp(b.d_fUL32, UO, op->d_fUL32, lu, esz32);
       cp(b.d_fV32, VO, op->d_fV32, lv, esz32);
     }
     for (int p : op->trial_tree.perm) perm_tr.push_back(XO + p);
@@ -2169,12 +2233,13 @@ void build_batch(const std::vector<Operator*>& ops, Operator& b, hipStream_t s, 
       fb.toff += TO;
       b.far.push_back(fb);
     }
-    for (int pass = 0; pass < 3; pass++) {  // forward, transpose, forward of the block product
-      const bool trans = pass == 1, mm = pass == 2;
-      plan(*op, trans, tiles, segs, !mm);
-      std::vector<Seg>& S = mm ? smm : (trans ? st : sf);
+    // forward (z), transpose, forward over Ut columns (block product), forward over leaf-blocked U
+    for (int pass = 0; pass < 4; pass++) {
+      const bool trans = pass == 1, extra = pass >= 2;
+      plan(*op, trans, tiles, segs, pass == 2 ? PLAN_UT : (pass == 3 ? PLAN_LEAF : PLAN_Z));
+      std::vector<Seg>& S = pass == 2 ? smm : (pass == 3 ? sleaf : (trans ? st : sf));
       const int s0 = (int)S.size();
-      if (!mm) {
+      if (!extra) {
         std::vector<Tile>& T = trans ? tt : tf;
         for (Tile t : tiles) {
           t.p0 += trans ? XO : YO;
@@ -2203,9 +2268,11 @@ void build_batch(const std::vector<Operator*>& ops, Operator& b, hipStream_t s, 
       Operator* o = const_cast<Operator*>(op);
       o->d_near_val.release();
       o->d_fU.release();
+      o->d_fUL.release();
       o->d_fV.release();
       o->d_near_val32.release();
       o->d_fU32.release();
+      o->d_fUL32.release();
       o->d_fV32.release();
       o->assembled = false;
     }
@@ -2230,6 +2297,7 @@ void build_batch(const std::vector<Operator*>& ops, Operator& b, hipStream_t s, 
   b.d_tile_fwd.upload(tf, s);
   b.d_seg_fwd.upload(sf, s);
   b.d_seg_fwd_mm.upload(smm, s);
+  b.d_seg_fwd_leaf.upload(sleaf, s);
   b.n_tiles_fwd = (int)tf.size();
   b.d_tile_trn.upload(tt, s);
   b.d_seg_trn.upload(st, s);
@@ -2266,9 +2334,21 @@ void axpby_device(bool cplx, void* y, const void* z, int n, const double* alpha,
 
 // ----------------------------------------------------------------- apply --
 
+// forward plan of the vector apply: leaf-blocked U (default), NGBEM_MV_PLAN=z
+// or =cols for the other two forms
+static int mv_plan() {
+  static const int kind = [] {
+    const char* e = std::getenv("NGBEM_MV_PLAN");
+    if (e && std::string(e) == "z") return (int)PLAN_Z;
+    if (e && std::string(e) == "cols") return (int)PLAN_UT;
+    return (int)PLAN_LEAF;
+  }();
+  return kind;
+}
+
 template <class ST, class VT, class XT>
 static void apply_t(Operator& op, int flags, const XT* x, XT* y, XT alpha, XT beta, bool beta_zero, const ST* nval,
-                    const ST* fU, const ST* fV, hipStream_t s, hipEvent_t* ev) {
+                    const ST* fU, const ST* fUL, const ST* fV, hipStream_t s, hipEvent_t* ev) {
   auto mark = [&](int i) {
     if (ev) BEM_HIP(hipEventRecord(ev[i], s));
   };
@@ -2288,18 +2368,22 @@ static void apply_t(Operator& op, int flags, const XT* x, XT* y, XT alpha, XT be
   // low rank: t = V x (forward) / Ut x (transpose, conjugated for the adjoint),
   // then z = U t / V^T t per block; dense blocks and z slices per row tile
   if (!trans) {
+    const int pk = mv_plan();
+    const bool zpath = pk == PLAN_Z;
     if (nfar) {
       hipLaunchKernelGGL((k_mv_far_t<ST, VT, false, false>), dim3((op.n_items_fwd + 3) / 4), dim3(256), 0, s,
                          op.d_items_fwd.as<RowItem>(), op.n_items_fwd, op.d_far_blk.as<FarBlk>(), fU, fV, xp, tb);
       mark(2);
-      hipLaunchKernelGGL((k_mv_far_z<ST, VT, false, false>), dim3((op.n_zitems_fwd + 3) / 4), dim3(256), 0, s,
-                         op.d_zitems_fwd.as<ZItem>(), op.n_zitems_fwd, op.d_far_blk.as<FarBlk>(), fU, fV, tb, z);
+      if (zpath)
+        hipLaunchKernelGGL((k_mv_far_z<ST, VT, false, false>), dim3((op.n_zitems_fwd + 3) / 4), dim3(256), 0, s,
+                           op.d_zitems_fwd.as<ZItem>(), op.n_zitems_fwd, op.d_far_blk.as<FarBlk>(), fU, fV, tb, z);
     } else {
       mark(2);
     }
     mark(3);
+    const DBuf& segs = pk == PLAN_Z ? op.d_seg_fwd : (pk == PLAN_UT ? op.d_seg_fwd_mm : op.d_seg_fwd_leaf);
     hipLaunchKernelGGL((k_mv_rows<ST, VT>), dim3(op.n_tiles_fwd), dim3(256), 0, s, op.d_tile_fwd.as<Tile>(),
-                       op.d_seg_fwd.as<Seg>(), nval, z, xp, yp);
+                       segs.as<Seg>(), nval, pk == PLAN_LEAF ? fUL : fU, z, xp, tb, yp);
   } else {
     if (nfar) {
       if (conj)
@@ -2348,6 +2432,8 @@ static void apply_mm_t(Operator& op, const XT* x, XT* y, int nrhs, int64_t ld, X
   }
   const size_t need_v = (size_t)std::max(nin, nout) * MM_W * vsz;
   const size_t need_t = (size_t)std::max(op.t_total, 1) * MM_W * vsz;
+  if (op.d_xm.bytes < need_v || op.d_ym.bytes < need_v || op.d_tm.bytes < need_t)
+    BEM_HIP(hipDeviceSynchronize());  // earlier block products may still use the slabs
   if (op.d_xm.bytes < need_v) op.d_xm.alloc(need_v);
   if (op.d_ym.bytes < need_v) op.d_ym.alloc(need_v);
   if (op.d_tm.bytes < need_t) op.d_tm.alloc(need_t);
@@ -2405,18 +2491,20 @@ void apply_device(Operator& op, int flags, const double* alpha, const void* x, c
     const zc a{alpha[0], alpha[1]}, b{beta[0], beta[1]};
     if (!mixed)
       apply_t<zc, zc, zc>(op, flags, (const zc*)x, (zc*)y, a, b, beta_zero, op.d_near_val.as<zc>(), op.d_fU.as<zc>(),
-                          op.d_fV.as<zc>(), s, ev);
+                          op.d_fUL.as<zc>(), op.d_fV.as<zc>(), s, ev);
     else
       apply_t<fc, fc, zc>(op, flags, (const zc*)x, (zc*)y, a, b, beta_zero, op.d_near_val32.as<fc>(),
-                          op.d_fU32.as<fc>(), op.d_fV32.as<fc>(), s, ev);
+                          op.d_fU32.as<fc>(), op.d_fUL32.as<fc>(), op.d_fV32.as<fc>(), s, ev);
   } else {
     if (alpha[1] != 0.0 || beta[1] != 0.0) throw std::runtime_error("complex scalars on a real operator");
     if (!mixed)
       apply_t<double, double, double>(op, flags & 1, (const double*)x, (double*)y, alpha[0], beta[0], beta_zero,
-                                      op.d_near_val.as<double>(), op.d_fU.as<double>(), op.d_fV.as<double>(), s, ev);
+                                      op.d_near_val.as<double>(), op.d_fU.as<double>(), op.d_fUL.as<double>(),
+                                      op.d_fV.as<double>(), s, ev);
     else
       apply_t<float, float, double>(op, flags & 1, (const double*)x, (double*)y, alpha[0], beta[0], beta_zero,
-                                    op.d_near_val32.as<float>(), op.d_fU32.as<float>(), op.d_fV32.as<float>(), s, ev);
+                                    op.d_near_val32.as<float>(), op.d_fU32.as<float>(), op.d_fUL32.as<float>(),
+                                    op.d_fV32.as<float>(), s, ev);
   }
   BEM_HIP(hipGetLastError());
 }

@@ -30,50 +30,70 @@ struct HipError : std::runtime_error {
                             ":" + std::to_string(__LINE__) + ")");                             \
   } while (0)
 
-// Device memory comes from the device's default stream-ordered pool, which
-// keeps up to 64 GiB (NGBEM_POOL_KEEP_GB) of freed memory mapped: operators built one
-// after another (batches of geometries) then reuse it instead of mapping fresh
-// memory each time, which costs 100+ ms per large buffer.  Allocation and free
-// are ordered on the null stream and bracketed by synchronisation, so a buffer
-// behaves like hipMalloc/hipFree memory for every other stream.
-inline void pool_setup() {
+// Device memory: by default plain hipMalloc/hipFree.  NGBEM_ALLOC=pool takes
+// it from the device's stream-ordered pool instead, which keeps up to 64 GiB
+// (NGBEM_POOL_KEEP_GB) of freed memory mapped, so operators built one after
+// another (batches of geometries) reuse it instead of mapping fresh memory.
+// Pool allocations are ordered on a private stream and synchronised at once;
+// frees are ordered on that stream too, so a buffer must be idle when it is
+// released (the library synchronises its streams before temporaries go out of
+// scope and the device before an operator is destroyed).
+inline bool use_pool() {
+  static const bool on = [] {
+    const char* e = std::getenv("NGBEM_ALLOC");
+    return e && std::string(e) == "pool";
+  }();
+  return on;
+}
+
+inline hipStream_t pool_stream() {
   static std::mutex mu;
-  static bool done[64] = {};
+  static hipStream_t st[64] = {};
   int dev = 0;
   BEM_HIP(hipGetDevice(&dev));
+  if (dev < 0 || dev >= 64) throw HipError("device index out of range");
   std::lock_guard<std::mutex> lk(mu);
-  if (dev < 0 || dev >= 64 || done[dev]) return;
-  hipMemPool_t pool;
-  BEM_HIP(hipDeviceGetDefaultMemPool(&pool, dev));
-  const char* env = std::getenv("NGBEM_POOL_KEEP_GB");
-  uint64_t keep = (env ? (uint64_t)std::max(0.0, std::atof(env)) : 64ull) << 30;
-  BEM_HIP(hipMemPoolSetAttribute(pool, hipMemPoolAttrReleaseThreshold, &keep));
-  done[dev] = true;
+  if (!st[dev]) {
+    hipMemPool_t pool;
+    BEM_HIP(hipDeviceGetDefaultMemPool(&pool, dev));
+    const char* env = std::getenv("NGBEM_POOL_KEEP_GB");
+    uint64_t keep = (env ? (uint64_t)std::max(0.0, std::atof(env)) : 64ull) << 30;
+    BEM_HIP(hipMemPoolSetAttribute(pool, hipMemPoolAttrReleaseThreshold, &keep));
+    BEM_HIP(hipStreamCreateWithFlags(&st[dev], hipStreamNonBlocking));
+  }
+  return st[dev];
 }
 
 inline void* dev_alloc(size_t n) {
-  pool_setup();
   void* p = nullptr;
-  hipError_t e = hipMallocAsync(&p, n, nullptr);
+  if (!use_pool()) {
+    BEM_HIP(hipMalloc(&p, n));
+    return p;
+  }
+  hipStream_t as = pool_stream();
+  hipError_t e = hipMallocAsync(&p, n, as);
   if (e == hipErrorOutOfMemory) {  // give the pool's cached memory back and retry once
     (void)hipGetLastError();
     int dev = 0;
     hipMemPool_t pool;
     BEM_HIP(hipGetDevice(&dev));
     BEM_HIP(hipDeviceGetDefaultMemPool(&pool, dev));
-    BEM_HIP(hipDeviceSynchronize());
+    BEM_HIP(hipStreamSynchronize(as));
     BEM_HIP(hipMemPoolTrimTo(pool, 0));
-    e = hipMallocAsync(&p, n, nullptr);
+    e = hipMallocAsync(&p, n, as);
   }
   BEM_HIP(e);
-  BEM_HIP(hipStreamSynchronize(nullptr));
+  BEM_HIP(hipStreamSynchronize(as));
   return p;
 }
 
 inline void dev_free(void* p) {
   if (!p) return;
-  (void)hipDeviceSynchronize();  // like hipFree: no stream may still use it
-  (void)hipFreeAsync(p, nullptr);
+  if (!use_pool()) {
+    (void)hipFree(p);
+    return;
+  }
+  (void)hipFreeAsync(p, pool_stream());
 }
 
 // owning device buffer
@@ -195,7 +215,9 @@ struct Operator {
   DBuf d_far_blk, d_fU, d_fV, d_fU32, d_fV32, d_far_patch, d_far_maps;
   // matvec plans
   DBuf d_seg_fwd, d_tile_fwd, d_seg_trn, d_tile_trn;
-  DBuf d_seg_fwd_mm;       // forward segments of the block product (far = 1)
+  DBuf d_seg_fwd_mm;       // forward segments of the block product (far = 1, Ut columns)
+  DBuf d_seg_fwd_leaf;     // forward segments over the leaf-blocked U (far = 1)
+  DBuf d_fUL, d_fUL32;     // leaf-blocked copy of U (FP64 / FP32)
   int n_tiles_fwd = 0, n_tiles_trn = 0;
   DBuf d_zitems_fwd, d_zitems_trn;  // U t / V^T t work items
   int n_zitems_fwd = 0, n_zitems_trn = 0;
