@@ -77,6 +77,10 @@ def main():
                     help="helmholtz_cf: the headline; maxwell_efie_64rhs: block product of 64 right-hand sides; "
                          "batch_sl_256: 256 Helmholtz single layers on ellipsoids, batched apply + gradient")
     ap.add_argument("--instances", type=int, default=256)
+    ap.add_argument("--reserve-gib", type=float, default=0.0,
+                    help="map this much device memory into the library's pool before timing (NGBEM_ALLOC=pool)")
+    ap.add_argument("--assemble-twice", action="store_true",
+                    help="also time a second assembly of the same operator (steady state)")
     ap.add_argument("--threads", type=int, default=8, help="host threads assembling instances concurrently")
     args = ap.parse_args()
     if args.gpus != 1:
@@ -106,6 +110,8 @@ def main():
     # warm the runtime (first HIP call initialises the device) and load the
     # assembly kernels' code objects with a small operator of the same kind
     torch.zeros(1, device=dev)
+    if args.reserve_gib > 0:
+        nb.reserve(args.reserve_gib)
     small = nb.SurfaceL2(nb.icosphere(2))
     nb.HelmholtzCombinedFieldOperator(small, small, kappa=kappa, mixed=True, **dict(params, leafsize=16))
     torch.cuda.synchronize()
@@ -116,6 +122,17 @@ def main():
     st = C.stats()
     log(f"assembly {t_assembly:.3f} s  near {st['t_near']:.3f} s  far {st['t_far']:.3f} s  "
         f"setup {st['t_setup']:.3f} s  blocks near/far {st['n_near']}/{st['n_far']}  max rank {st['max_rank']}")
+    t_assembly2 = None
+    if args.assemble_twice:
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        C2 = nb.HelmholtzCombinedFieldOperator(fes, fes, kappa=kappa, mixed=True, **params)
+        torch.cuda.synchronize()
+        t_assembly2 = time.perf_counter() - t0
+        st2 = C2.stats()
+        del C2
+        log(f"second assembly {t_assembly2:.3f} s  near {st2['t_near']:.3f} s  far {st2['t_far']:.3f} s  "
+            f"setup {st2['t_setup']:.3f} s")
 
     x = torch.randn(fes.ndof, dtype=torch.complex128, device=dev)
     y = torch.zeros_like(x)
@@ -164,9 +181,9 @@ def main():
             m_tot += bi["m"]
 
     # algorithmic bytes per launch (each operand once) of the apply kernels; the
-    # forward plan is the leaf-blocked U inside the row tiles unless
-    # NGBEM_MV_PLAN=z (then U t per block in k_mv_far_z, z slices in the tiles)
-    zplan = os.environ.get("NGBEM_MV_PLAN", "leaf") == "z"
+    # forward plan reads U inside the row tiles unless NGBEM_MV_PLAN=z (then
+    # U t per block in k_mv_far_z, z slices in the tiles)
+    zplan = os.environ.get("NGBEM_MV_PLAN", "cols") == "z"
 
     def kernel_bytes(s):
         kb = {"k_mv_far_t": far_v * s + n * s + R_tot * s}            # V, x, t
@@ -208,6 +225,7 @@ def main():
                                f"eta 2, ACA eps 1e-6; mixed = complex64 dense leaves + factors, FP32 accumulation"},
         "assembly_s": t_assembly,
         "assembly_breakdown_s": {"setup": st["t_setup"], "near": st["t_near"], "far_aca": st["t_far"]},
+        "second_assembly_s": t_assembly2,
         "fp64_matvec_per_s": args.steps / t64,
         "mixed_vs_fp64_matvec_speedup": t64 / t32,
         "mixed_matvec_rel_err": mv_err,

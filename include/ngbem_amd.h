@@ -114,6 +114,13 @@ typedef struct {
 const char* bem_last_error(void);
 int bem_device_count(void);
 
+/* Device memory of the library comes from hipMalloc, or with the environment
+ * variable NGBEM_ALLOC=pool from the device's stream-ordered pool, which keeps
+ * freed memory mapped (NGBEM_POOL_KEEP_GB, default 64).  bem_reserve maps
+ * `bytes` into that pool up front (no effect without it).  No counterpart in
+ * the reference. */
+int bem_reserve(double bytes);
+
 int bem_mesh_create(const double* xyz, int nv, const int32_t* tri, int nt, bem_mesh** out);
 int bem_mesh_destroy(bem_mesh* m);
 
@@ -142,6 +149,19 @@ int bem_op_stats(const bem_op* op, bem_stats* st);
  * src/hmat.cpp:650-651). */
 int bem_op_profile_apply(bem_op* op, int flags, int iters, double* ms, void* stream);
 int bem_op_destroy(bem_op* op);
+
+/* Potential of a trial-side density at arbitrary points: replaces
+ * GenericIntegralOperator::GetPotential / PotentialCF (src/ngbem.cpp:1166-1235).
+ * u_c(x) = sum over the trial side's quadrature points y (rule of the
+ * operator's intorder) of J w K(x, y, n_x, n_y) vals(y) per kernel term, c =
+ * the term's test component, term factors not applied (as the reference).
+ * coef: device vector of trial coefficients (double or complex128 like the
+ * operator); points / normals: device arrays npoints x 3 (normals may be NULL:
+ * zero); out: device array npoints x bem_op_potential_dim(op).  Synchronises
+ * the stream before returning. */
+int bem_op_potential_dim(const bem_op* op);
+int bem_op_potential(bem_op* op, const void* coef, const double* points, const double* normals, int npoints, void* out,
+                     int dtype, void* stream);
 
 /* introspection for parity tests */
 int bem_op_num_blocks(const bem_op* op);

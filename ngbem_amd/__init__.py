@@ -18,10 +18,18 @@ from .operators import (BaseMatrix, HMatrix, IntegralOperator, IntegralOperatorC
 from .krylov import CG, GMRes, GMResBatch, RefinedSolve
 from .batch import OperatorBatch, SingleLayerBatch, assemble_many
 
+
+def reserve(gib: float) -> None:
+    """Map ``gib`` GiB of device memory into the library's memory pool
+    (NGBEM_ALLOC=pool) so later assemblies reuse it (bem_reserve)."""
+    from . import _capi
+
+    _capi.check(_capi.lib().bem_reserve(float(gib) * 2 ** 30), "bem_reserve")
+
 __all__ = ["Mesh", "icosphere", "ellipsoid", "SurfaceL2", "H1", "HDivSurface", "HCurl", "BaseMatrix", "HMatrix", "IntegralOperator",
            "IntegralOperatorC", "MassMatrix", "SingleLayerPotentialOperator", "DoubleLayerPotentialOperator",
            "HypersingularOperator", "HelmholtzSingleLayerPotentialOperator", "HelmholtzDoubleLayerPotentialOperator",
            "HelmholtzCombinedFieldOperator", "HelmholtzHypersingularOperator", "MaxwellSingleLayerPotentialOperator",
            "MaxwellSingleLayerPotentialOperatorCurl", "MaxwellDoubleLayerPotentialOperator",
            "HelmholtzSingleLayerPotentialOperatorDKappa", "CG", "GMRes", "GMResBatch", "RefinedSolve",
-           "OperatorBatch", "SingleLayerBatch", "assemble_many"]
+           "OperatorBatch", "SingleLayerBatch", "assemble_many", "reserve"]

@@ -48,10 +48,10 @@ class BlockInfo(ctypes.Structure):
 
 
 # every symbol include/ngbem_amd.h declares
-EXPORTS = ["bem_last_error", "bem_device_count", "bem_mesh_create", "bem_mesh_destroy", "bem_space_create",
+EXPORTS = ["bem_last_error", "bem_device_count", "bem_reserve", "bem_mesh_create", "bem_mesh_destroy", "bem_space_create",
            "bem_space_ndof", "bem_space_destroy", "bem_op_create", "bem_op_assemble", "bem_op_apply",
            "bem_op_stats", "bem_op_profile_apply", "bem_op_destroy", "bem_op_num_blocks", "bem_op_block_info", "bem_op_block_dofs",
-           "bem_op_block_data", "bem_op_tree", "bem_batch_create", "bem_batch_sizes", "bem_batch_apply",
+           "bem_op_block_data", "bem_op_tree", "bem_op_potential_dim", "bem_op_potential", "bem_batch_create", "bem_batch_sizes", "bem_batch_apply",
            "bem_batch_destroy"]
 
 
@@ -88,6 +88,9 @@ def lib():
     L.bem_op_block_dofs.argtypes = [P, ctypes.c_int, ip, ip]
     L.bem_op_block_data.argtypes = [P, ctypes.c_int, P]
     L.bem_op_tree.argtypes = [P, ctypes.c_int, ip, dp, dp, ctypes.c_int]
+    L.bem_reserve.argtypes = [ctypes.c_double]
+    L.bem_op_potential_dim.argtypes = [P]
+    L.bem_op_potential.argtypes = [P, P, P, P, ctypes.c_int, P, ctypes.c_int, P]
     L.bem_batch_create.argtypes = [ctypes.POINTER(P), ctypes.c_int, ctypes.c_int, ctypes.POINTER(P)]
     L.bem_batch_sizes.argtypes = [P, ctypes.POINTER(ctypes.c_int), ctypes.POINTER(ctypes.c_int),
                                   ctypes.POINTER(ctypes.c_int)]

@@ -458,6 +458,24 @@ int bem_op_tree(const bem_op* h, int which, int32_t* perm, double* cl, double* c
 
 }  // extern "C"
 
+int bem_reserve(double bytes) {
+  if (!(bytes >= 0.0)) return fail(-1, "bem_reserve: bad arguments");
+  return guarded([&] { bem::pool_reserve((size_t)bytes); });
+}
+
+int bem_op_potential_dim(const bem_op* h) {
+  if (!h) return fail(-1, "bem_op_potential_dim: bad arguments");
+  return bem::potential_dim(h->op);
+}
+
+int bem_op_potential(bem_op* h, const void* coef, const double* points, const double* normals, int npoints, void* out,
+                     int dtype, void* stream) {
+  if (!h || !coef || !points || !out || npoints < 0) return fail(-1, "bem_op_potential: bad arguments");
+  if (h->op.complex_valued != (dtype == BEM_C128)) return fail(-1, "bem_op_potential: dtype does not match");
+  if (npoints == 0) return 0;
+  return guarded([&] { bem::potential_device(h->op, coef, points, normals, npoints, out, (hipStream_t)stream); });
+}
+
 // ------------------------------------------------------------------ batch --
 
 int bem_batch_create(bem_op* const* ops, int n, int flags, bem_batch** out) {

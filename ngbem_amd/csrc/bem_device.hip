@@ -1706,6 +1706,7 @@ void assemble_far(Operator& op, hipStream_t s) {
     size_t nu = 0, nv = 0;
   };
   std::vector<std::unique_ptr<Piece>> pieces;
+  bool direct = false;  // factors projected straight into op.d_fU / op.d_fV
   std::vector<int> piece_of(nfar, -1);
   std::vector<int64_t> loc_u(nfar), loc_v(nfar);
   op.far_rank.assign(nfar, 0);
@@ -1853,7 +1854,25 @@ void assemble_far(Operator& op, hipStream_t s) {
       }
     }
     pt_.mark("aca resume");
-    // projection, one launch per workspace buffer
+    // projection, one launch per workspace buffer.  When this batch holds
+    // every low-rank block (the usual case), all ranks are known now and the
+    // factors are projected straight into the final arrays; otherwise into
+    // per-buffer pieces that are concatenated at the end.
+    const bool direct_now = pieces.empty() && !direct && next == order.size();
+    if (direct_now) {
+      size_t totU = 0, totV = 0;
+      for (size_t bi = 0; bi < bblocks.size(); bi++) {
+        const FarBlk& fb = op.far[bblocks[bi]];
+        int R = 0;
+        for (auto& t : op.kinfo.terms) R += tasks[bi * ncomp + t.kc].rank;
+        totU += (size_t)fb.m * R;
+        totV += (size_t)R * fb.n;
+      }
+      op.d_fU.alloc(std::max<size_t>(totU, 1) * esz);
+      op.d_fV.alloc(std::max<size_t>(totV, 1) * esz);
+      direct = true;
+    }
+    size_t gu = 0, gv = 0;
     for (int w = 0; w <= (int)rbuf.size(); w++) {
       std::vector<ProjTask> ptasks;
       auto piece = std::make_unique<Piece>();
@@ -1877,19 +1896,23 @@ void assemble_far(Operator& op, hipStream_t s) {
         }
         pt.np = np;
         pt.nq = nq;
-        pt.ou = (int64_t)piece->nu;
-        pt.ov = (int64_t)piece->nv;
-        piece->nu += (size_t)fb.m * R;
-        piece->nv += (size_t)R * fb.n;
+        pt.ou = (int64_t)(direct_now ? gu : piece->nu);
+        pt.ov = (int64_t)(direct_now ? gv : piece->nv);
+        (direct_now ? gu : piece->nu) += (size_t)fb.m * R;
+        (direct_now ? gv : piece->nv) += (size_t)R * fb.n;
         op.far_rank[blk] = R;
-        piece_of[blk] = (int)pieces.size();
+        piece_of[blk] = direct_now ? -1 : (int)pieces.size();
         loc_u[blk] = pt.ou;
         loc_v[blk] = pt.ov;
         ptasks.push_back(pt);
       }
       if (ptasks.empty()) continue;
-      piece->U.alloc(std::max<size_t>(piece->nu, 1) * esz);
-      piece->V.alloc(std::max<size_t>(piece->nv, 1) * esz);
+      if (!direct_now) {
+        piece->U.alloc(std::max<size_t>(piece->nu, 1) * esz);
+        piece->V.alloc(std::max<size_t>(piece->nv, 1) * esz);
+      }
+      DBuf& OU = direct_now ? op.d_fU : piece->U;
+      DBuf& OV = direct_now ? op.d_fV : piece->V;
       d_ptasks.upload(ptasks, s);
       std::vector<int2> items;
       for (size_t t = 0; t < ptasks.size(); t++) {
@@ -1910,11 +1933,11 @@ void assemble_far(Operator& op, hipStream_t s) {
                              0, s, d_ptasks.as<ProjTask>(), d_items.as<int2>(), op.d_far_blk.as<FarBlk>(),
                              op.d_patch.as<int>(), op.d_far_maps.as<int>(), op.d_quad.as<double>(),
                              op.d_sdat.as<double>(), op.d_trig.as<double>(), ntq, ker, SU.as<T>(), SV.as<T>(),
-                             piece->U.as<T>(), piece->V.as<T>());
+                             OU.as<T>(), OV.as<T>());
         });
       BEM_HIP(hipGetLastError());
       BEM_HIP(hipStreamSynchronize(s));
-      pieces.push_back(std::move(piece));
+      if (!direct_now) pieces.push_back(std::move(piece));
     }
   }
   pt_.mark("projection");
@@ -1923,33 +1946,36 @@ void assemble_far(Operator& op, hipStream_t s) {
   (void)hipStreamDestroy(s2);
   wsU.release();
   wsV.release();
-  // concatenate pieces into the final factor arrays
+  // concatenate pieces into the final factor arrays (not needed when the
+  // factors were projected in place)
   size_t tu = 0, tv = 0;
   std::vector<size_t> bu(pieces.size()), bv(pieces.size());
-  for (size_t p = 0; p < pieces.size(); p++) {
-    bu[p] = tu;
-    bv[p] = tv;
-    tu += pieces[p]->nu;
-    tv += pieces[p]->nv;
+  if (!direct) {
+    for (size_t p = 0; p < pieces.size(); p++) {
+      bu[p] = tu;
+      bv[p] = tv;
+      tu += pieces[p]->nu;
+      tv += pieces[p]->nv;
+    }
+    op.d_fU.alloc(std::max<size_t>(tu, 1) * esz);
+    op.d_fV.alloc(std::max<size_t>(tv, 1) * esz);
+    for (size_t p = 0; p < pieces.size(); p++) {
+      if (pieces[p]->nu)
+        BEM_HIP(hipMemcpyAsync((char*)op.d_fU.p + bu[p] * esz, pieces[p]->U.p, pieces[p]->nu * esz,
+                               hipMemcpyDeviceToDevice, s));
+      if (pieces[p]->nv)
+        BEM_HIP(hipMemcpyAsync((char*)op.d_fV.p + bv[p] * esz, pieces[p]->V.p, pieces[p]->nv * esz,
+                               hipMemcpyDeviceToDevice, s));
+    }
+    BEM_HIP(hipStreamSynchronize(s));
+    pieces.clear();
   }
-  op.d_fU.alloc(std::max<size_t>(tu, 1) * esz);
-  op.d_fV.alloc(std::max<size_t>(tv, 1) * esz);
-  for (size_t p = 0; p < pieces.size(); p++) {
-    if (pieces[p]->nu)
-      BEM_HIP(hipMemcpyAsync((char*)op.d_fU.p + bu[p] * esz, pieces[p]->U.p, pieces[p]->nu * esz,
-                             hipMemcpyDeviceToDevice, s));
-    if (pieces[p]->nv)
-      BEM_HIP(hipMemcpyAsync((char*)op.d_fV.p + bv[p] * esz, pieces[p]->V.p, pieces[p]->nv * esz,
-                             hipMemcpyDeviceToDevice, s));
-  }
-  BEM_HIP(hipStreamSynchronize(s));
-  pieces.clear();
   int toff = 0;
   for (int k = 0; k < nfar; k++) {
     FarBlk& fb = op.far[k];
     fb.R = op.far_rank[k];
-    fb.uoff = (int64_t)bu[piece_of[k]] + loc_u[k];
-    fb.voff = (int64_t)bv[piece_of[k]] + loc_v[k];
+    fb.uoff = (direct ? 0 : (int64_t)bu[piece_of[k]]) + loc_u[k];
+    fb.voff = (direct ? 0 : (int64_t)bv[piece_of[k]]) + loc_v[k];
     fb.toff = toff;
     toff += fb.R;
     op.far_factor_entries += (int64_t)fb.R * (fb.m + fb.n);
@@ -1963,11 +1989,10 @@ void assemble_far(Operator& op, hipStream_t s) {
 
 // ---------------------------------------------------------- matvec plans --
 
-// How the forward plan covers the low-rank blocks: PLAN_Z as slices of z = U t
-// (computed per block beforehand), PLAN_UT as columns of U (Ut rows, stride m)
-// times t, PLAN_LEAF as columns of the leaf-blocked copy of U (the rows of one
-// leaf stored together, ld = leaf size) times t.  The transpose always uses z.
-enum PlanKind { PLAN_Z = 0, PLAN_UT = 1, PLAN_LEAF = 2 };
+// How the forward plan covers the low-rank blocks: PLAN_UT as columns of U
+// (Ut rows, stride m) times t inside the row tiles, PLAN_Z as slices of
+// z = U t computed per block beforehand.  The transpose always uses z.
+enum PlanKind { PLAN_Z = 0, PLAN_UT = 1 };
 
 static void plan(const Operator& op, bool trans, std::vector<Tile>& tiles, std::vector<Seg>& segs,
                  int kind = PLAN_Z) {
@@ -1999,10 +2024,8 @@ static void plan(const Operator& op, bool trans, std::vector<Tile>& tiles, std::
         add(fb.c0, fb.n, [&](int o, int) { return Seg{(int64_t)fb.zoffT + o, 0, 1, 0, 2}; });
       else if (kind == PLAN_Z)
         add(fb.r0, fb.m, [&](int o, int) { return Seg{(int64_t)fb.zoff + o, 0, 1, 0, 2}; });
-      else if (kind == PLAN_UT)
-        add(fb.r0, fb.m, [&](int o, int) { return Seg{fb.uoff + o, fb.m, fb.R, fb.toff, 1}; });
       else
-        add(fb.r0, fb.m, [&](int o, int ls) { return Seg{fb.uoff + (int64_t)o * fb.R, ls, fb.R, fb.toff, 1}; });
+        add(fb.r0, fb.m, [&](int o, int) { return Seg{fb.uoff + o, fb.m, fb.R, fb.toff, 1}; });
     }
   }
   tiles.clear();
@@ -2034,53 +2057,6 @@ static void upload_row_items(Operator& op, hipStream_t s) {
   }
 }
 
-// leaf-blocked copy of U: for every low-rank block and every leaf of its row
-// cluster, the leaf's rows of U (ls x R, column-major) stored together at
-// uoff + o R, o = the leaf's first row inside the block
-struct LChunk {
-  int64_t uoff;
-  int m, R, o, ls;
-};
-
-template <class T>
-__global__ __launch_bounds__(256) void k_leaf_blocked(const LChunk* __restrict__ chunks, const T* __restrict__ ut,
-                                                      T* __restrict__ ul) {
-  const LChunk c = chunks[blockIdx.x];
-  const int total = c.ls * c.R;
-  for (int idx = threadIdx.x; idx < total; idx += 256) {
-    const int l = idx / c.ls, i = idx - l * c.ls;
-    ul[c.uoff + (int64_t)c.o * c.R + idx] = ut[c.uoff + (int64_t)l * c.m + c.o + i];
-  }
-}
-
-static void build_leaf_blocked(Operator& op, hipStream_t s) {
-  const Tree& tree = *op.test_tree;
-  const int nleaf = (int)tree.leaves.size();
-  std::vector<int> lp0(nleaf);
-  for (int i = 0; i < nleaf; i++) lp0[i] = (int)tree.cl[tree.leaves[i]].permupos;
-  std::vector<LChunk> chunks;
-  size_t tu = 0;
-  for (const FarBlk& fb : op.far) {
-    tu = std::max(tu, (size_t)(fb.uoff + (int64_t)fb.m * fb.R));
-    int li = (int)(std::upper_bound(lp0.begin(), lp0.end(), fb.r0) - lp0.begin()) - 1;
-    for (; li < nleaf && lp0[li] < fb.r0 + fb.m; li++)
-      if (fb.R > 0) chunks.push_back(LChunk{fb.uoff, fb.m, fb.R, lp0[li] - fb.r0, (int)tree.cl[tree.leaves[li]].number});
-  }
-  const size_t esz = op.complex_valued ? 16 : 8;
-  op.d_fUL.alloc(std::max<size_t>(tu, 1) * esz);
-  if (chunks.empty()) return;
-  DBuf d_chunks;
-  d_chunks.upload(chunks, s);
-  if (op.complex_valued)
-    hipLaunchKernelGGL((k_leaf_blocked<zc>), dim3((unsigned)chunks.size()), dim3(256), 0, s, d_chunks.as<LChunk>(),
-                       op.d_fU.as<zc>(), op.d_fUL.as<zc>());
-  else
-    hipLaunchKernelGGL((k_leaf_blocked<double>), dim3((unsigned)chunks.size()), dim3(256), 0, s,
-                       d_chunks.as<LChunk>(), op.d_fU.as<double>(), op.d_fUL.as<double>());
-  BEM_HIP(hipGetLastError());
-  BEM_HIP(hipStreamSynchronize(s));
-}
-
 // z offsets of the low-rank blocks and the U t / V^T t work items
 static void upload_z_items(Operator& op, hipStream_t s) {
   int zo = 0, zt = 0;
@@ -2106,14 +2082,10 @@ static void upload_z_items(Operator& op, hipStream_t s) {
 
 void build_matvec_plan(Operator& op, hipStream_t s) {
   upload_z_items(op, s);
-  build_leaf_blocked(op, s);
   std::vector<Tile> tiles;
   std::vector<Seg> segs;
   plan(op, false, tiles, segs, PLAN_UT);
   op.d_seg_fwd_mm.upload(segs, s);
-  BEM_HIP(hipStreamSynchronize(s));
-  plan(op, false, tiles, segs, PLAN_LEAF);
-  op.d_seg_fwd_leaf.upload(segs, s);
   BEM_HIP(hipStreamSynchronize(s));
   plan(op, false, tiles, segs);
   op.d_tile_fwd.upload(tiles, s);
@@ -2158,10 +2130,139 @@ void make_mixed_copy(Operator& op, hipStream_t s) {
     tv = std::max(tv, (size_t)(fb.voff + (int64_t)fb.R * fb.n));
   }
   conv(op.d_fU, op.d_fU32, tu);
-  conv(op.d_fUL, op.d_fUL32, tu);
   conv(op.d_fV, op.d_fV32, tv);
   BEM_HIP(hipGetLastError());
   BEM_HIP(hipStreamSynchronize(s));
+}
+
+// ------------------------------------------------------------- potential --
+// PotentialCF (src/ngbem.cpp:1166-1235): u_c(x) = sum over the trial side's
+// quadrature points y (rule of the operator's intorder) of J w K_kc(x, y, n_x,
+// n_y) vals_trc(y), per kernel term with c = its test component (the term
+// factors are not applied, as in the reference).  Sources (point, normal,
+// weight, evaluator values) are built once per call; targets are evaluated
+// N-body style: a thread per target, sources staged through LDS, the source
+// list split across workgroups (grid y) and the partial sums reduced in a
+// fixed order.
+template <class T, int D>
+struct PotSrc {
+  double y[3], ny[3], w, pad;
+  T v[D];
+};
+
+template <class KER, class EJ>
+__global__ __launch_bounds__(256) void k_pot_src(int nt, int ntq, const double* __restrict__ quad,
+                                                 const double* __restrict__ geo, const double* __restrict__ sdat,
+                                                 const double* __restrict__ trig, const int* __restrict__ dofs,
+                                                 const typename KER::T* __restrict__ coef,
+                                                 PotSrc<typename KER::T, EJ::DIM>* __restrict__ src) {
+  typedef typename KER::T T;
+  const int idx = blockIdx.x * 256 + threadIdx.x;
+  if (idx >= nt * ntq) return;
+  const int e = idx / ntq, q = idx - e * ntq;
+  const double* qp = quad + (size_t)idx * 4;
+  const double lam[3] = {trig[3 * q], trig[3 * q + 1], 1.0 - trig[3 * q] - trig[3 * q + 1]};
+  const int idp[3] = {0, 1, 2};
+  double bb[EJ::NB > 0 ? EJ::NB : 1];
+  EJ::load(sdat + (size_t)e * SDS, idp, bb);
+  double v[EJ::NL * EJ::DIM];
+  EJ::shape(lam, ld3(qp), bb, v);
+  PotSrc<T, EJ::DIM> o;
+  for (int k = 0; k < 3; k++) {
+    o.y[k] = qp[k];
+    o.ny[k] = geo[(size_t)e * GEO + 9 + k];
+  }
+  o.w = qp[3];
+  o.pad = 0.0;
+#pragma unroll
+  for (int c = 0; c < EJ::DIM; c++) {
+    T a = zero_<T>();
+#pragma unroll
+    for (int l = 0; l < EJ::NL; l++) a += rmul(coef[dofs[4 * (size_t)e + l]], v[l * EJ::DIM + c]);
+    o.v[c] = a;
+  }
+  src[idx] = o;
+}
+
+template <class KER, int D>
+__global__ __launch_bounds__(256) void k_potential(const PotSrc<typename KER::T, D>* __restrict__ src, int nsrc,
+                                                   int chunk, const double* __restrict__ pts,
+                                                   const double* __restrict__ nrm, int np, KER ker,
+                                                   typename KER::T* __restrict__ part) {
+  typedef typename KER::T T;
+  __shared__ PotSrc<T, D> tile[128];
+  const int i = blockIdx.x * 256 + threadIdx.x;
+  const bool act = i < np;
+  const int ii = act ? i : 0;
+  const P3 x = ld3(pts + 3 * (size_t)ii);
+  const P3 nx = nrm ? ld3(nrm + 3 * (size_t)ii) : P3{0.0, 0.0, 0.0};
+  T acc[D];
+#pragma unroll
+  for (int c = 0; c < D; c++) acc[c] = zero_<T>();
+  const int s0 = blockIdx.y * chunk, s1 = min(nsrc, s0 + chunk);
+  for (int sb = s0; sb < s1; sb += 128) {
+    const int nsb = min(128, s1 - sb);
+    __syncthreads();
+    if ((int)threadIdx.x < nsb) tile[threadIdx.x] = src[sb + threadIdx.x];
+    __syncthreads();
+    for (int k = 0; k < nsb; k++) {
+      const PotSrc<T, D>& sc = tile[k];
+      T kv[KER::NC];
+      ker.eval(x, P3{sc.y[0], sc.y[1], sc.y[2]}, nx, P3{sc.ny[0], sc.ny[1], sc.ny[2]}, kv);
+#pragma unroll
+      for (int t = 0; t < KER::NTERM; t++) {
+        const TermC tm = ker.term(t);
+        acc[tm.tec] += rmul(kv[tm.kc], sc.w) * sc.v[tm.trc];
+      }
+    }
+  }
+  if (act)
+#pragma unroll
+    for (int c = 0; c < D; c++) part[((size_t)blockIdx.y * np + i) * D + c] = acc[c];
+}
+
+template <class T>
+__global__ void k_pot_reduce(const T* __restrict__ part, int nsplit, int64_t n, T* __restrict__ out) {
+  const int64_t j = (int64_t)blockIdx.x * 256 + threadIdx.x;
+  if (j >= n) return;
+  T a = part[j];
+  for (int k = 1; k < nsplit; k++) a += part[(int64_t)k * n + j];
+  out[j] = a;
+}
+
+int potential_dim(const Operator& op) {
+  int d = 0;
+  dispatch_full(op, [&](auto, auto, auto ej) { d = decltype(ej)::DIM; });
+  return d;
+}
+
+void potential_device(Operator& op, const void* coef, const double* pts, const double* nrm, int np, void* out,
+                      hipStream_t s) {
+  if (!op.assembled) throw std::runtime_error("operator not assembled");
+  const int nt = op.mesh->nt, ntq = op.trig.size(), nsrc = nt * ntq;
+  dispatch_full(op, [&](auto ker, auto, auto ej) {
+    using KER = decltype(ker);
+    using EJ = decltype(ej);
+    using T = typename KER::T;
+    constexpr int D = EJ::DIM;
+    DBuf src, part;
+    src.alloc((size_t)nsrc * sizeof(PotSrc<T, D>));
+    hipLaunchKernelGGL((k_pot_src<KER, EJ>), dim3((nsrc + 255) / 256), dim3(256), 0, s, nt, ntq,
+                       op.d_quad.as<double>(), op.d_geo.as<double>(), op.d_sdat.as<double>(), op.d_trig.as<double>(),
+                       op.d_dofs_trial.as<int>(), (const T*)coef, src.as<PotSrc<T, D>>());
+    // enough workgroups to fill the chip: split the sources when targets are few
+    const int gx = (np + 255) / 256;
+    const int nsplit = std::max(1, std::min(std::max(1, 2048 / gx), (nsrc + 1023) / 1024));
+    const int chunk = (nsrc + nsplit - 1) / nsplit;
+    part.alloc((size_t)nsplit * np * D * sizeof(T));
+    hipLaunchKernelGGL((k_potential<KER, D>), dim3(gx, nsplit), dim3(256), 0, s, src.as<PotSrc<T, D>>(), nsrc,
+                       chunk, pts, nrm, np, ker, part.as<T>());
+    const int64_t n = (int64_t)np * D;
+    hipLaunchKernelGGL((k_pot_reduce<T>), dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, part.as<T>(), nsplit, n,
+                       (T*)out);
+    BEM_HIP(hipGetLastError());
+    BEM_HIP(hipStreamSynchronize(s));
+  });
 }
 
 // ----------------------------------------------------------------- batch --
@@ -2194,17 +2295,15 @@ void build_batch(const std::vector<Operator*>& ops, Operator& b, hipStream_t s, 
   }
   b.d_near_val.alloc((size_t)std::max<int64_t>(nnear, 1) * esz);
   b.d_fU.alloc((size_t)std::max<int64_t>(nu, 1) * esz);
-  b.d_fUL.alloc((size_t)std::max<int64_t>(nu, 1) * esz);
   b.d_fV.alloc((size_t)std::max<int64_t>(nv, 1) * esz);
   if (b.mixed) {
     b.d_near_val32.alloc((size_t)std::max<int64_t>(nnear, 1) * esz32);
     b.d_fU32.alloc((size_t)std::max<int64_t>(nu, 1) * esz32);
-    b.d_fUL32.alloc((size_t)std::max<int64_t>(nu, 1) * esz32);
     b.d_fV32.alloc((size_t)std::max<int64_t>(nv, 1) * esz32);
   }
   std::vector<int> perm_tr, perm_te;
   std::vector<Tile> tf, tt, tiles;
-  std::vector<Seg> sf, st, smm, sleaf, segs;
+  std::vector<Seg> sf, st, smm, segs;
   b.far.clear();
   int64_t NO = 0, UO = 0, VO = 0;
   int TO = 0, XO = 0, YO = 0, ZO = 0, ZOT = 0;
@@ -2215,12 +2314,10 @@ void build_batch(const std::vector<Operator*>& ops, Operator& b, hipStream_t s, 
     const size_t lu = factor_len(*op, true), lv = factor_len(*op, false);
     cp(b.d_near_val, NO, op->d_near_val, (size_t)op->near_entries, esz);
     cp(b.d_fU, UO, op->d_fU, lu, esz);
-    cp(b.d_fUL, UO, op->d_fUL, lu, esz);
     cp(b.d_fV, VO, op->d_fV, lv, esz);
     if (b.mixed) {
       cp(b.d_near_val32, NO, op->d_near_val32, (size_t)op->near_entries, esz32);
       cp(b.d_fU32, UO, op->d_fU32, lu, esz32);
-      cp(b.d_fUL32, UO, op->d_fUL32, lu, esz32);
       cp(b.d_fV32, VO, op->d_fV32, lv, esz32);
     }
     for (int p : op->trial_tree.perm) perm_tr.push_back(XO + p);
@@ -2233,11 +2330,11 @@ void build_batch(const std::vector<Operator*>& ops, Operator& b, hipStream_t s, 
       fb.toff += TO;
       b.far.push_back(fb);
     }
-    // forward (z), transpose, forward over Ut columns (block product), forward over leaf-blocked U
-    for (int pass = 0; pass < 4; pass++) {
-      const bool trans = pass == 1, extra = pass >= 2;
-      plan(*op, trans, tiles, segs, pass == 2 ? PLAN_UT : (pass == 3 ? PLAN_LEAF : PLAN_Z));
-      std::vector<Seg>& S = pass == 2 ? smm : (pass == 3 ? sleaf : (trans ? st : sf));
+    // forward over z slices, transpose, forward over Ut columns (vector apply and block product)
+    for (int pass = 0; pass < 3; pass++) {
+      const bool trans = pass == 1, extra = pass == 2;
+      plan(*op, trans, tiles, segs, pass == 2 ? PLAN_UT : PLAN_Z);
+      std::vector<Seg>& S = pass == 2 ? smm : (trans ? st : sf);
       const int s0 = (int)S.size();
       if (!extra) {
         std::vector<Tile>& T = trans ? tt : tf;
@@ -2268,11 +2365,9 @@ void build_batch(const std::vector<Operator*>& ops, Operator& b, hipStream_t s, 
       Operator* o = const_cast<Operator*>(op);
       o->d_near_val.release();
       o->d_fU.release();
-      o->d_fUL.release();
       o->d_fV.release();
       o->d_near_val32.release();
       o->d_fU32.release();
-      o->d_fUL32.release();
       o->d_fV32.release();
       o->assembled = false;
     }
@@ -2297,7 +2392,6 @@ void build_batch(const std::vector<Operator*>& ops, Operator& b, hipStream_t s, 
   b.d_tile_fwd.upload(tf, s);
   b.d_seg_fwd.upload(sf, s);
   b.d_seg_fwd_mm.upload(smm, s);
-  b.d_seg_fwd_leaf.upload(sleaf, s);
   b.n_tiles_fwd = (int)tf.size();
   b.d_tile_trn.upload(tt, s);
   b.d_seg_trn.upload(st, s);
@@ -2334,21 +2428,19 @@ void axpby_device(bool cplx, void* y, const void* z, int n, const double* alpha,
 
 // ----------------------------------------------------------------- apply --
 
-// forward plan of the vector apply: leaf-blocked U (default), NGBEM_MV_PLAN=z
-// or =cols for the other two forms
+// forward plan of the vector apply: Ut columns in the row tiles (default) or
+// NGBEM_MV_PLAN=z
 static int mv_plan() {
   static const int kind = [] {
     const char* e = std::getenv("NGBEM_MV_PLAN");
-    if (e && std::string(e) == "z") return (int)PLAN_Z;
-    if (e && std::string(e) == "cols") return (int)PLAN_UT;
-    return (int)PLAN_LEAF;
+    return (e && std::string(e) == "z") ? (int)PLAN_Z : (int)PLAN_UT;
   }();
   return kind;
 }
 
 template <class ST, class VT, class XT>
 static void apply_t(Operator& op, int flags, const XT* x, XT* y, XT alpha, XT beta, bool beta_zero, const ST* nval,
-                    const ST* fU, const ST* fUL, const ST* fV, hipStream_t s, hipEvent_t* ev) {
+                    const ST* fU, const ST* fV, hipStream_t s, hipEvent_t* ev) {
   auto mark = [&](int i) {
     if (ev) BEM_HIP(hipEventRecord(ev[i], s));
   };
@@ -2381,9 +2473,9 @@ static void apply_t(Operator& op, int flags, const XT* x, XT* y, XT alpha, XT be
       mark(2);
     }
     mark(3);
-    const DBuf& segs = pk == PLAN_Z ? op.d_seg_fwd : (pk == PLAN_UT ? op.d_seg_fwd_mm : op.d_seg_fwd_leaf);
+    const DBuf& segs = pk == PLAN_Z ? op.d_seg_fwd : op.d_seg_fwd_mm;
     hipLaunchKernelGGL((k_mv_rows<ST, VT>), dim3(op.n_tiles_fwd), dim3(256), 0, s, op.d_tile_fwd.as<Tile>(),
-                       segs.as<Seg>(), nval, pk == PLAN_LEAF ? fUL : fU, z, xp, tb, yp);
+                       segs.as<Seg>(), nval, fU, z, xp, tb, yp);
   } else {
     if (nfar) {
       if (conj)
@@ -2491,20 +2583,20 @@ void apply_device(Operator& op, int flags, const double* alpha, const void* x, c
     const zc a{alpha[0], alpha[1]}, b{beta[0], beta[1]};
     if (!mixed)
       apply_t<zc, zc, zc>(op, flags, (const zc*)x, (zc*)y, a, b, beta_zero, op.d_near_val.as<zc>(), op.d_fU.as<zc>(),
-                          op.d_fUL.as<zc>(), op.d_fV.as<zc>(), s, ev);
+                          op.d_fV.as<zc>(), s, ev);
     else
       apply_t<fc, fc, zc>(op, flags, (const zc*)x, (zc*)y, a, b, beta_zero, op.d_near_val32.as<fc>(),
-                          op.d_fU32.as<fc>(), op.d_fUL32.as<fc>(), op.d_fV32.as<fc>(), s, ev);
+                          op.d_fU32.as<fc>(), op.d_fV32.as<fc>(), s, ev);
   } else {
     if (alpha[1] != 0.0 || beta[1] != 0.0) throw std::runtime_error("complex scalars on a real operator");
     if (!mixed)
       apply_t<double, double, double>(op, flags & 1, (const double*)x, (double*)y, alpha[0], beta[0], beta_zero,
-                                      op.d_near_val.as<double>(), op.d_fU.as<double>(), op.d_fUL.as<double>(),
-                                      op.d_fV.as<double>(), s, ev);
+                                      op.d_near_val.as<double>(), op.d_fU.as<double>(), op.d_fV.as<double>(), s,
+                                      ev);
     else
       apply_t<float, float, double>(op, flags & 1, (const double*)x, (double*)y, alpha[0], beta[0], beta_zero,
-                                    op.d_near_val32.as<float>(), op.d_fU32.as<float>(), op.d_fUL32.as<float>(),
-                                    op.d_fV32.as<float>(), s, ev);
+                                    op.d_near_val32.as<float>(), op.d_fU32.as<float>(), op.d_fV32.as<float>(), s,
+                                    ev);
   }
   BEM_HIP(hipGetLastError());
 }

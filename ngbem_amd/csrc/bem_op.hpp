@@ -96,6 +96,27 @@ inline void dev_free(void* p) {
   (void)hipFreeAsync(p, pool_stream());
 }
 
+// map `bytes` of device memory into the pool once (allocate, free back to the
+// pool, raise the release threshold to keep it); no effect without the pool
+inline void pool_reserve(size_t bytes) {
+  if (!use_pool() || bytes == 0) return;
+  hipStream_t as = pool_stream();
+  int dev = 0;
+  hipMemPool_t pool;
+  BEM_HIP(hipGetDevice(&dev));
+  BEM_HIP(hipDeviceGetDefaultMemPool(&pool, dev));
+  uint64_t keep = 0;
+  BEM_HIP(hipMemPoolGetAttribute(pool, hipMemPoolAttrReleaseThreshold, &keep));
+  if (keep < bytes) {
+    keep = bytes;
+    BEM_HIP(hipMemPoolSetAttribute(pool, hipMemPoolAttrReleaseThreshold, &keep));
+  }
+  void* p = nullptr;
+  BEM_HIP(hipMallocAsync(&p, bytes, as));
+  BEM_HIP(hipFreeAsync(p, as));
+  BEM_HIP(hipStreamSynchronize(as));
+}
+
 // owning device buffer
 struct DBuf {
   void* p = nullptr;
@@ -215,9 +236,7 @@ struct Operator {
   DBuf d_far_blk, d_fU, d_fV, d_fU32, d_fV32, d_far_patch, d_far_maps;
   // matvec plans
   DBuf d_seg_fwd, d_tile_fwd, d_seg_trn, d_tile_trn;
-  DBuf d_seg_fwd_mm;       // forward segments of the block product (far = 1, Ut columns)
-  DBuf d_seg_fwd_leaf;     // forward segments over the leaf-blocked U (far = 1)
-  DBuf d_fUL, d_fUL32;     // leaf-blocked copy of U (FP64 / FP32)
+  DBuf d_seg_fwd_mm;       // forward segments over Ut columns (far = 1): vector apply and block product
   int n_tiles_fwd = 0, n_tiles_trn = 0;
   DBuf d_zitems_fwd, d_zitems_trn;  // U t / V^T t work items
   int n_zitems_fwd = 0, n_zitems_trn = 0;
@@ -256,6 +275,11 @@ void make_mixed_copy(Operator& op, hipStream_t s);
 // scatter
 void apply_device(Operator& op, int flags, const double* alpha, const void* x, const double* beta, void* y,
                   hipStream_t s, hipEvent_t* ev = nullptr);
+// potential of the trial-side density coef at np points (PotentialCF):
+// out is np x potential_dim(op), double or complex128 like the operator
+int potential_dim(const Operator& op);
+void potential_device(Operator& op, const void* coef, const double* pts, const double* nrm, int np, void* out,
+                      hipStream_t s);
 // block-diagonal union of assembled operators (storage copied, one plan)
 // consume: free each operator's block storage once copied (it can no longer be applied)
 void build_batch(const std::vector<Operator*>& ops, Operator& b, hipStream_t s, bool consume);

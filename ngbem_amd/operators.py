@@ -292,6 +292,38 @@ class HMatrix(BaseMatrix):
         return self._op.blocks()
 
 
+# -------------------------------------------------------------- potential --
+
+class PotentialCF:
+    """u_c(x) = sum over the trial side's quadrature points y of
+    J w K(x, y, n_x, n_y) vals(y) per kernel term (PotentialCF::T_Evaluate,
+    src/ngbem.cpp:1187-1235; the term factors are not applied, as there).
+    Call with points (npts x 3) and optional normals n_x (zero otherwise);
+    returns (npts,) for scalar evaluators, else (npts, dim)."""
+
+    def __init__(self, op, coef):
+        self._op = op
+        dt = torch.complex128 if op.is_complex else torch.float64
+        self.coef = _as_vec(coef, dt, device()).contiguous()
+        if self.coef.shape != (op.trial_space.ndof,):
+            raise ValueError("coefficient vector does not match the trial space")
+        self.dim = int(_capi.lib().bem_op_potential_dim(op._h))
+
+    def __call__(self, points, normals=None):
+        op = self._op
+        dev = self.coef.device
+        P = _as_vec(points, torch.float64, dev).reshape(-1, 3).contiguous()
+        N = None if normals is None else _as_vec(normals, torch.float64, dev).reshape(-1, 3).contiguous()
+        out = torch.zeros(P.shape[0], self.dim, dtype=self.coef.dtype, device=dev)
+        with torch.cuda.device(dev):
+            stream = torch.cuda.current_stream().cuda_stream
+            _capi.check(_capi.lib().bem_op_potential(
+                op._h, ctypes.c_void_p(self.coef.data_ptr()), ctypes.c_void_p(P.data_ptr()),
+                None if N is None else ctypes.c_void_p(N.data_ptr()), P.shape[0], ctypes.c_void_p(out.data_ptr()),
+                _capi.C128 if op.is_complex else _capi.F64, ctypes.c_void_p(stream)), "bem_op_potential")
+        return out[:, 0] if self.dim == 1 else out
+
+
 # ------------------------------------------------------ integral operator --
 
 class IntegralOperator:
@@ -335,7 +367,10 @@ class IntegralOperator:
         return self._mat
 
     def GetPotential(self, gf):
-        raise NotImplementedError("potential evaluation (PotentialCF) is outside this library's scope")
+        """The potential of the trial-side density ``gf`` (its coefficient
+        vector) as a callable ``pot(points, normals=None)`` evaluated on the GPU
+        (GenericIntegralOperator::GetPotential, src/ngbem.cpp:1166-1235)."""
+        return PotentialCF(self, gf)
 
     def stats(self):
         st = _capi.Stats()

@@ -1244,6 +1244,54 @@ double orc_apply(void* p, int trans, const double* x, double* y, int nthreads) {
   apply(*(Op*)p, trans, (const cplx*)x, (cplx*)y, nthreads);
   return std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
 }
+// PotentialCF::T_Evaluate (src/ngbem.cpp:1187-1235): at a point x with normal
+// n_x, u_c(x) = sum_T sum_q J_T w_q K_kc(x, y_q, n_x, n_T) vals_trc(y_q) summed
+// over the kernel terms with c = the term's test component; vals = the trial
+// evaluator applied to the coefficient vector; the rule is IntegrationRule(ET_TRIG,
+// intorder).  The term factors are not applied (the reference multiplies by
+// the weight and the kernel value only, :1228).  coef: complex (interleaved)
+// trial coefficients; out: npts x dim complex.
+int orc_potential(void* p, const double* coef, const double* pts, const double* nrm, int npts, double* out) {
+  const Op& op = *(Op*)p;
+  const Mesh& m = *op.mesh;
+  const TrigRule& tr = op.tr;
+  const int nq = tr.size();
+  const int dim = ev_dim(op.K.ev_trial), nl = op.trial.nloc;
+  const cplx* c = (const cplx*)coef;
+  std::vector<double> Y((size_t)m.nt * nq * 3), W((size_t)m.nt * nq);
+  std::vector<cplx> vals((size_t)m.nt * nq * dim);
+  for (int e = 0; e < m.nt; e++) {
+    std::vector<double> sh(12);
+    for (int q = 0; q < nq; q++) {
+      m.point(e, tr.xi[q], tr.eta[q], &Y[3 * ((size_t)e * nq + q)]);
+      W[(size_t)e * nq + q] = m.J[e] * tr.w[q];
+      eval_shape(m, op.trial, op.K.ev_trial, e, tr.xi[q], tr.eta[q], sh.data());
+      for (int d = 0; d < dim; d++) {
+        cplx v = 0.0;
+        for (int l = 0; l < nl; l++) v += c[op.trial.dofs[(size_t)e * nl + l]] * sh[l * dim + d];
+        vals[((size_t)e * nq + q) * dim + d] = v;
+      }
+    }
+  }
+  cplx* o = (cplx*)out;
+#pragma omp parallel for schedule(dynamic, 16)
+  for (int i = 0; i < npts; i++) {
+    const double zero[3] = {0.0, 0.0, 0.0};
+    const double* x = pts + 3 * (size_t)i;
+    const double* nx = nrm ? nrm + 3 * (size_t)i : zero;
+    std::vector<cplx> acc(dim, 0.0);
+    cplx kv[3];
+    for (int e = 0; e < m.nt; e++)
+      for (int q = 0; q < nq; q++) {
+        const size_t sq = (size_t)e * nq + q;
+        op.K.eval(x, &Y[3 * sq], nx, &m.nrm[3 * e], kv);
+        for (const Term& t : op.K.terms) acc[t.tec] += W[sq] * kv[t.kc] * vals[sq * dim + t.trc];
+      }
+    for (int d = 0; d < dim; d++) o[(size_t)i * dim + d] = acc[d];
+  }
+  return dim;
+}
+
 int orc_max_threads() {
 #ifdef _OPENMP
   return omp_get_max_threads();

@@ -56,6 +56,7 @@ def lib():
         L.orc_set_block.argtypes = [P, ctypes.c_int, ctypes.c_int, dp]
         L.orc_nze.restype = ctypes.c_long
         L.orc_nze.argtypes = [P]
+        L.orc_potential.argtypes = [P, dp, dp, dp, ctypes.c_int, dp]
         L.orc_apply.restype = ctypes.c_double
         L.orc_apply.argtypes = [P, ctypes.c_int, dp, dp, ctypes.c_int]
         _lib = L
@@ -205,6 +206,17 @@ class OracleOp:
         t = lib().orc_apply(self.h, int(trans), _dp(x.view(np.float64)), _dp(y.view(np.float64)), nthreads)
         self.last_apply_seconds = t
         return y
+
+    def potential(self, coef, points, normals=None):
+        """PotentialCF values at points (npts x dim, complex); dim = trial evaluator dim."""
+        c = np.ascontiguousarray(coef, dtype=np.complex128)
+        P = np.ascontiguousarray(points, dtype=np.float64).reshape(-1, 3)
+        N = None if normals is None else np.ascontiguousarray(normals, dtype=np.float64).reshape(-1, 3)
+        dim = 4
+        out = np.zeros((len(P), dim), dtype=np.complex128)  # large enough for every evaluator
+        d = lib().orc_potential(self.h, _dp(c.view(np.float64)), _dp(P), None if N is None else _dp(N), len(P),
+                                _dp(out.view(np.float64)))
+        return out.reshape(-1)[: len(P) * d].reshape(len(P), d)
 
     def densify(self):
         """Full matrix from the assembled blocks (small problems only)."""
