@@ -68,7 +68,8 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-solve", action="store_true")
     ap.add_argument("--calib-copy", action="store_true",
-                    help="end with one plain 512 MiB device copy (calibrates rocprof FETCH_SIZE/WRITE_SIZE)")
+                    help="end with one elementwise kernel reading and writing 512 MiB (calibrates rocprof "
+                         "FETCH_SIZE/WRITE_SIZE)")
     ap.add_argument("--traffic", type=str,
                     default=os.path.join(ROOT, "profiles", "hmatvec_helmholtz_cf_L6_traffic.md"),
                     help="summary of the PMC passes with 'bytes_per_launch <kernel>:' lines (complex64 apply)")
@@ -292,10 +293,12 @@ def main():
                 out["roofline"]["traffic"] = float(val)
                 out["roofline"]["traffic_source"] = os.path.relpath(args.traffic, ROOT)
     if args.calib_copy:
-        a = torch.empty(512 << 20, dtype=torch.uint8, device=dev).fill_(1)
+        # counter calibration: one elementwise kernel that reads 512 MiB and
+        # writes 512 MiB with ordinary vector loads and stores
+        a = torch.ones(128 << 20, dtype=torch.float32, device=dev)
         b = torch.empty_like(a)
         torch.cuda.synchronize()
-        b.copy_(a)
+        torch.mul(a, 2.0, out=b)
         torch.cuda.synchronize()
     print(json.dumps(out), flush=True)
 

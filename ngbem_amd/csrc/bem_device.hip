@@ -1689,7 +1689,14 @@ void assemble_far(Operator& op, hipStream_t s) {
   size_t free_b = 0, total_b = 0;
   BEM_HIP(hipMemGetInfo(&free_b, &total_b));
   const size_t esz = op.complex_valued ? 16 : 8;
-  const size_t budget = std::max<size_t>(free_b / 3, (size_t)256 << 20);
+  // workspace budget: a third of the free memory, at most NGBEM_ACA_BUDGET_GB
+  // (default 16): blocks beyond it go in further batches that reuse the buffer
+  size_t budget = std::max<size_t>(free_b / 3, (size_t)256 << 20);
+  {
+    const char* be = std::getenv("NGBEM_ACA_BUDGET_GB");
+    const double gb = be ? std::max(0.25, std::atof(be)) : 16.0;
+    budget = std::min(budget, (size_t)(gb * (double)(1ull << 30)));
+  }
   // initial rank capacity (NGBEM_ACA_CAP0 overrides); points above which a block gets 512 threads
   const char* cap_env = std::getenv("NGBEM_ACA_CAP0");
   const int CAP0 = cap_env ? std::max(1, std::atoi(cap_env)) : 48, BIG = 6144;
