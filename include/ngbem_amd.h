@@ -130,6 +130,19 @@ int bem_space_destroy(bem_space* s);
 
 /* tree + partition + device setup; no quadrature yet */
 int bem_op_create(int kernel, double kappa, bem_space* trial, bem_space* test, const bem_params* p, bem_op** out);
+/* Several geometries in one operator: the mesh holds ninst surfaces one after
+ * another (elements [elem_offsets[i], elem_offsets[i+1]) form geometry i, which
+ * shares no vertex with the others; dofs numbered geometry by geometry), each
+ * with its own wavenumber kappas[i].  The cluster tree is a forest (split
+ * between geometries first, then the usual PCA bisection), no block couples two
+ * geometries, and each kernel evaluation uses its geometry's wavenumber, so
+ * assembly and apply of all geometries run in the launches of one operator and
+ * give the blocks the geometries' own operators would.  Not for the Maxwell
+ * single layer (kappa in its term factors) nor for bem_op_potential.  No
+ * counterpart in the reference (one GenericIntegralOperator per geometry,
+ * src/ngbem.cpp:314-330). */
+int bem_op_create_multi(int kernel, const double* kappas, int ninst, const int32_t* elem_offsets, bem_space* trial,
+                        bem_space* test, const bem_params* p, bem_op** out);
 /* near-field quadrature + ACA on the device; stream is a hipStream_t (NULL = default) */
 int bem_op_assemble(bem_op* op, void* stream);
 /* y = alpha op(A) x + beta y for nrhs right-hand sides (columns of length ld).

@@ -14,7 +14,8 @@ from .operators import (BaseMatrix, HMatrix, IntegralOperator, IntegralOperatorC
                         HelmholtzSingleLayerPotentialOperator, HelmholtzDoubleLayerPotentialOperator,
                         HelmholtzCombinedFieldOperator, HelmholtzHypersingularOperator,
                         MaxwellSingleLayerPotentialOperator, MaxwellSingleLayerPotentialOperatorCurl,
-                        MaxwellDoubleLayerPotentialOperator, HelmholtzSingleLayerPotentialOperatorDKappa)
+                        MaxwellDoubleLayerPotentialOperator, HelmholtzSingleLayerPotentialOperatorDKappa,
+                        MultiGeometryOperator, union_mesh)
 from .krylov import CG, GMRes, GMResBatch, RefinedSolve
 from .batch import OperatorBatch, SingleLayerBatch, assemble_many
 
@@ -32,4 +33,4 @@ __all__ = ["Mesh", "icosphere", "ellipsoid", "SurfaceL2", "H1", "HDivSurface", "
            "HelmholtzCombinedFieldOperator", "HelmholtzHypersingularOperator", "MaxwellSingleLayerPotentialOperator",
            "MaxwellSingleLayerPotentialOperatorCurl", "MaxwellDoubleLayerPotentialOperator",
            "HelmholtzSingleLayerPotentialOperatorDKappa", "CG", "GMRes", "GMResBatch", "RefinedSolve",
-           "OperatorBatch", "SingleLayerBatch", "assemble_many", "reserve"]
+           "OperatorBatch", "SingleLayerBatch", "assemble_many", "reserve", "MultiGeometryOperator", "union_mesh"]

@@ -49,7 +49,7 @@ class BlockInfo(ctypes.Structure):
 
 # every symbol include/ngbem_amd.h declares
 EXPORTS = ["bem_last_error", "bem_device_count", "bem_reserve", "bem_mesh_create", "bem_mesh_destroy", "bem_space_create",
-           "bem_space_ndof", "bem_space_destroy", "bem_op_create", "bem_op_assemble", "bem_op_apply",
+           "bem_space_ndof", "bem_space_destroy", "bem_op_create", "bem_op_create_multi", "bem_op_assemble", "bem_op_apply",
            "bem_op_stats", "bem_op_profile_apply", "bem_op_destroy", "bem_op_num_blocks", "bem_op_block_info", "bem_op_block_dofs",
            "bem_op_block_data", "bem_op_tree", "bem_op_potential_dim", "bem_op_potential", "bem_batch_create", "bem_batch_sizes", "bem_batch_apply",
            "bem_batch_destroy"]
@@ -78,6 +78,8 @@ def lib():
     L.bem_space_ndof.argtypes = [P]
     L.bem_space_destroy.argtypes = [P]
     L.bem_op_create.argtypes = [ctypes.c_int, ctypes.c_double, P, P, ctypes.POINTER(Params), ctypes.POINTER(P)]
+    L.bem_op_create_multi.argtypes = [ctypes.c_int, dp, ctypes.c_int, ip, P, P, ctypes.POINTER(Params),
+                                      ctypes.POINTER(P)]
     L.bem_op_assemble.argtypes = [P, P]
     L.bem_op_apply.argtypes = [P, ctypes.c_int, dp, P, dp, P, ctypes.c_int, ctypes.c_int, ctypes.c_int, P]
     L.bem_op_stats.argtypes = [P, ctypes.POINTER(Stats)]

@@ -10,9 +10,9 @@ vectors.
 own GPU stream, so the instances' assemblies overlap on the device.
 
 ``SingleLayerBatch``: differentiable y_i = V_i(kappa_i) x_i over a batch of
-Helmholtz single layers, a ``torch.autograd.Function`` whose backward applies
-the adjoint batch (gradient w.r.t. x) and the batch of d/dkappa operators
-(gradient w.r.t. kappa).  None of this has a counterpart in the reference,
+Helmholtz single layers (one multi-geometry operator for all of them), a
+``torch.autograd.Function`` whose backward applies the adjoint (gradient
+w.r.t. x) and the d/dkappa operator (gradient w.r.t. kappa).  None of this has a counterpart in the reference,
 which builds one ``GenericIntegralOperator`` at a time (src/ngbem.cpp:314-330)
 and is not differentiable.
 """
@@ -180,22 +180,38 @@ class _BatchApply(torch.autograd.Function):
 
 class SingleLayerBatch:
     """Helmholtz single layers V_i(kappa_i) on spaces_i, with their d/dkappa
-    operators, as two ``OperatorBatch`` objects; ``__call__(x, kappa)`` is the
-    differentiable y = blockdiag(V_i) x for concatenated x (kappa: a tensor of
-    the wavenumbers the batch was built with, which receives the gradient)."""
+    operators; ``__call__(x, kappa)`` is the differentiable y = blockdiag(V_i) x
+    for concatenated x (kappa: a tensor of the wavenumbers the batch was built
+    with, which receives the gradient).
 
-    def __init__(self, spaces, kappas, threads=8, **kw):
+    By default all geometries go into one operator each for V and dV/dkappa
+    (``MultiGeometryOperator``: one assembly, one apply for the whole batch);
+    ``multi=False`` assembles one operator per geometry (host threads) and
+    merges them into ``OperatorBatch`` objects instead."""
+
+    def __init__(self, spaces, kappas, threads=8, multi=True, **kw):
         self.kappas = [float(k) for k in kappas]
-        items = list(zip(spaces, self.kappas))
-        V = assemble_many(lambda it: HelmholtzSingleLayerPotentialOperator(it[0], it[0], kappa=it[1], **kw), items,
-                          threads)
-        self.V = OperatorBatch(V, consume=True)
-        del V
-        dV = assemble_many(lambda it: HelmholtzSingleLayerPotentialOperatorDKappa(it[0], it[0], kappa=it[1], **kw),
-                           items, threads)
-        self.dV = OperatorBatch(dV, consume=True)
-        del dV
         self.ndofs = [s.ndof for s in spaces]
+        if multi:
+            from .operators import MultiGeometryOperator
+
+            meshes = [s.mesh for s in spaces]
+            cls = type(spaces[0])
+            if any(type(s) is not cls for s in spaces):
+                raise TypeError("all spaces of a batch must be of one kind")
+            self._ops = (MultiGeometryOperator(_capi.HELMHOLTZ_SL, meshes, self.kappas, cls, **kw),
+                         MultiGeometryOperator(_capi.HELMHOLTZ_SL_DKAPPA, meshes, self.kappas, cls, **kw))
+            self.V, self.dV = self._ops[0].mat, self._ops[1].mat
+        else:
+            items = list(zip(spaces, self.kappas))
+            V = assemble_many(lambda it: HelmholtzSingleLayerPotentialOperator(it[0], it[0], kappa=it[1], **kw),
+                              items, threads)
+            self.V = OperatorBatch(V, consume=True)
+            del V
+            dV = assemble_many(lambda it: HelmholtzSingleLayerPotentialOperatorDKappa(it[0], it[0], kappa=it[1],
+                                                                                      **kw), items, threads)
+            self.dV = OperatorBatch(dV, consume=True)
+            del dV
         seg = np.repeat(np.arange(len(spaces)), self.ndofs)
         self.seg = torch.from_numpy(seg).to(device())
 

@@ -136,7 +136,9 @@ int bem_space_destroy(bem_space* s) {
   return 0;
 }
 
-int bem_op_create(int kernel, double kappa, bem_space* trial, bem_space* test, const bem_params* p, bem_op** out) {
+// inst_of_dof / kappa_el: several geometries in one mesh (bem_op_create_multi)
+static int create_op(int kernel, double kappa, bem_space* trial, bem_space* test, const bem_params* p, bem_op** out,
+                     const std::vector<int>* inst_of_dof, const std::vector<double>* kappa_el) {
   if (!trial || !out || !p) return fail(-1, "bem_op_create: bad arguments");
   if (kernel < 0 || kernel >= bem::K_COUNT) return fail(-1, "bem_op_create: unknown kernel id");
   if (!test) test = trial;
@@ -176,11 +178,20 @@ int bem_op_create(int kernel, double kappa, bem_space* trial, bem_space* test, c
     op.trial = trial->s;
     op.test = test->s;
     op.same_space = trial->s == test->s || (trial->s->kind == test->s->kind);
+    if (kappa_el) op.kappa_el = *kappa_el;
     bem::PhaseTimer pt_("create");
-    op.trial_tree.build(*op.trial, op.leafsize);
+    op.trial_tree.build(*op.trial, op.leafsize, inst_of_dof);
     if (op.same_space) op.test_tree = &op.trial_tree;
     else {
-      op.test_tree_own.build(*op.test, op.leafsize);
+      // the test space's dofs need their own instance map
+      std::vector<int> inst_test;
+      if (inst_of_dof) {
+        inst_test.assign(op.test->ndof, 0);
+        const int nloc = op.test->nloc;
+        for (int e = 0; e < op.mesh->nt; e++)
+          for (int l = 0; l < nloc; l++) inst_test[op.test->dofs[(size_t)e * nloc + l]] = (*inst_of_dof)[op.trial->dofs[(size_t)e * op.trial->nloc]];
+      }
+      op.test_tree_own.build(*op.test, op.leafsize, inst_of_dof ? &inst_test : nullptr);
       op.test_tree = &op.test_tree_own;
     }
     pt_.mark("cluster trees");
@@ -218,6 +229,50 @@ int bem_op_create(int kernel, double kappa, bem_space* trial, bem_space* test, c
     op.t_setup = now() - t0;
     *out = h.release();
   });
+}
+
+int bem_op_create(int kernel, double kappa, bem_space* trial, bem_space* test, const bem_params* p, bem_op** out) {
+  return create_op(kernel, kappa, trial, test, p, out, nullptr, nullptr);
+}
+
+int bem_op_create_multi(int kernel, const double* kappas, int ninst, const int32_t* elem_offsets, bem_space* trial,
+                        bem_space* test, const bem_params* p, bem_op** out) {
+  if (!kappas || ninst < 1 || !elem_offsets || !trial) return fail(-1, "bem_op_create_multi: bad arguments");
+  if (kernel == bem::K_MSL) return fail(-1, "bem_op_create_multi: the Maxwell single layer has kappa in its terms");
+  const bem::Mesh& m = *trial->m;
+  if (elem_offsets[0] != 0 || elem_offsets[ninst] != m.nt)
+    return fail(-1, "bem_op_create_multi: element offsets must run from 0 to the element count");
+  std::vector<double> kel(m.nt);
+  std::vector<int> inst_el(m.nt);
+  for (int i = 0; i < ninst; i++) {
+    if (elem_offsets[i + 1] <= elem_offsets[i]) return fail(-1, "bem_op_create_multi: empty or decreasing instance");
+    if (!(kappas[i] >= 0.0)) return fail(-1, "bem_op_create_multi: negative kappa");
+    for (int e = elem_offsets[i]; e < elem_offsets[i + 1]; e++) {
+      kel[e] = kappas[i];
+      inst_el[e] = i;
+    }
+  }
+  // instance of every trial dof; the instances must not share dofs and must be
+  // numbered instance by instance
+  const bem::Space& sp = *trial->s;
+  std::vector<int> inst(sp.ndof, -1);
+  for (int e = 0; e < m.nt; e++)
+    for (int l = 0; l < sp.nloc; l++) {
+      int& d = inst[sp.dofs[(size_t)e * sp.nloc + l]];
+      if (d >= 0 && d != inst_el[e]) return fail(-1, "bem_op_create_multi: geometries share a dof");
+      d = inst_el[e];
+    }
+  for (int d = 1; d < sp.ndof; d++)
+    if (inst[d] < inst[d - 1]) return fail(-1, "bem_op_create_multi: dofs are not numbered geometry by geometry");
+  // the element sets must not touch (no shared vertices)
+  std::vector<int> vinst(m.nv, -1);
+  for (int e = 0; e < m.nt; e++)
+    for (int k = 0; k < 3; k++) {
+      int& v = vinst[m.tri[3 * e + k]];
+      if (v >= 0 && v != inst_el[e]) return fail(-1, "bem_op_create_multi: geometries share a vertex");
+      v = inst_el[e];
+    }
+  return create_op(kernel, kappas[0], trial, test, p, out, &inst, &kel);
 }
 
 int bem_op_assemble(bem_op* h, void* stream) {
@@ -472,6 +527,7 @@ int bem_op_potential(bem_op* h, const void* coef, const double* points, const do
                      int dtype, void* stream) {
   if (!h || !coef || !points || !out || npoints < 0) return fail(-1, "bem_op_potential: bad arguments");
   if (h->op.complex_valued != (dtype == BEM_C128)) return fail(-1, "bem_op_potential: dtype does not match");
+  if (!h->op.kappa_el.empty()) return fail(-1, "bem_op_potential: not available for several geometries in one operator");
   if (npoints == 0) return 0;
   return guarded([&] { bem::potential_device(h->op, coef, points, normals, npoints, out, (hipStream_t)stream); });
 }

@@ -366,7 +366,7 @@ __device__ __forceinline__ void accum(const KER& ker, typename KER::T (&el)[EI::
 }
 
 // element data layout
-constexpr int GEO = 16;  // p0 p1 p2 n J pad
+constexpr int GEO = 16;  // p0 p1 p2 n J kappa pad
 __device__ __forceinline__ P3 ld3(const double* p) { return {p[0], p[1], p[2]}; }
 
 // ------------------------------------------------------ regular near field --
@@ -391,6 +391,7 @@ __global__ __launch_bounds__(256) void k_near_regular(const NearBlk* __restrict_
   const bool touch = ve.x == vf.x || ve.x == vf.y || ve.x == vf.z || ve.y == vf.x || ve.y == vf.y || ve.y == vf.z ||
                      ve.z == vf.x || ve.z == vf.y || ve.z == vf.z;
   if (touch) return;  // singular pair: written by k_near_singular
+  ker.kappa = geo[(size_t)e * GEO + 13];  // the element's geometry's wavenumber
   const P3 ne = ld3(geo + (size_t)e * GEO + 9), nf = ld3(geo + (size_t)f * GEO + 9);
   const int idp[3] = {0, 1, 2};
   double be[EI::NB], bf[EJ::NB];
@@ -464,6 +465,7 @@ __global__ __launch_bounds__(256) void k_near_singular(const int4* __restrict__ 
   }
   const double* ge = geo + (size_t)e * GEO;
   const double* gf = geo + (size_t)f * GEO;
+  ker.kappa = ge[13];  // the element's geometry's wavenumber
   const P3 E0 = ld3(ge + 3 * px[0]), E1 = ld3(ge + 3 * px[1]), E2 = ld3(ge + 3 * px[2]);
   const P3 F0 = ld3(gf + 3 * py[0]), F1 = ld3(gf + 3 * py[1]), F2 = ld3(gf + 3 * py[2]);
   const P3 ne = ld3(ge + 9), nf = ld3(gf + 9);
@@ -600,6 +602,7 @@ __global__ __launch_bounds__(NTH) void k_aca(AcaTask* __restrict__ tasks, const 
 
   AcaTask tk = tasks[blockIdx.x];
   const FarBlk b = fb[tk.blk];
+  ker.kappa = geo[(size_t)fpatch[b.pI] * GEO + 13];  // one geometry per block
   const PointSet X{fpatch + b.pI, quad, geo, ntq}, Y{fpatch + b.pJ, quad, geo, ntq};
   const int np = tk.np, nq = tk.nq, comp = tk.comp;
   T* U = wsU + tk.uoff;
@@ -1444,6 +1447,8 @@ void setup_device(Operator& op, hipStream_t s) {
     }
     for (int c = 0; c < 3; c++) geo[(size_t)e * GEO + 9 + c] = m.nrm[3 * e + c];
     geo[(size_t)e * GEO + 12] = m.J[e];
+    // wavenumber of the element's geometry (several geometries in one operator)
+    geo[(size_t)e * GEO + 13] = op.kappa_el.empty() ? op.kappa : op.kappa_el[e];
     for (int q = 0; q < ntq; q++) {
       double x[3];
       m.point(e, op.trig.xi[q], op.trig.eta[q], x);

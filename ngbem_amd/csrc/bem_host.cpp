@@ -343,7 +343,7 @@ static void split_axis(const double A[9], double ax[3]) {
     for (int k = 0; k < 3; k++) ax[k] = -ax[k];
 }
 
-void Tree::build(const Space& s, int leafsize) {
+void Tree::build(const Space& s, int leafsize, const std::vector<int>* inst) {
   const Mesh& m = *s.mesh;
   // cluster data: weights = mean adjacent element area; points by support count
   std::vector<double> G(s.ndof, 0.0), X(3 * (size_t)s.ndof, 0.0);
@@ -418,6 +418,29 @@ void Tree::build(const Space& s, int leafsize) {
     cl[ic].radius = radius;
     for (int k = 0; k < 3; k++) cl[ic].centre[k] = ctr[k];
     cl[ic].child1 = cl[ic].child2 = -1;
+    if (inst) {  // instance range; a cluster over several instances splits between them
+      int lo = (*inst)[perm[b]], hi = lo;
+      for (long i = b; i <= e; i++) {
+        lo = std::min(lo, (*inst)[perm[i]]);
+        hi = std::max(hi, (*inst)[perm[i]]);
+      }
+      ilo.resize(cl.size());
+      ihi.resize(cl.size());
+      ilo[ic] = lo;
+      ihi[ic] = hi + 1;
+      if (hi > lo) {  // positions are still in dof order here, i.e. sorted by instance
+        const int mid = (lo + hi + 1) / 2;
+        long split = b;
+        while (split <= e && (*inst)[perm[split]] < mid) split++;
+        const long lvl = cl[ic].level;
+        const int n0 = (int)cl.size();
+        cl[ic].child1 = n0;
+        cl[ic].child2 = n0 + 1;
+        cl.push_back(Cluster{lvl + 1, (long)ic, -1, -1, split - b, b, 0.0, {0, 0, 0}});
+        cl.push_back(Cluster{lvl + 1, (long)ic, -1, -1, e + 1 - split, split, 0.0, {0, 0, 0}});
+        continue;
+      }
+    }
     if (nclu <= leafsize) continue;
     long num1 = 0, num2 = 0, icheck = b, iend = e;
     const double crit = ctr[0] * ax[0] + ctr[1] * ax[1] + ctr[2] * ax[2];
@@ -441,6 +464,10 @@ void Tree::build(const Space& s, int leafsize) {
     cl.push_back(Cluster{lvl + 1, (long)ic, -1, -1, num1, b, 0.0, {0, 0, 0}});
     cl.push_back(Cluster{lvl + 1, (long)ic, -1, -1, num2, b + num1, 0.0, {0, 0, 0}});
   }
+  if (inst) {
+    ilo.resize(cl.size());
+    ihi.resize(cl.size());
+  }
   iperm.assign(s.ndof, 0);
   for (int i = 0; i < s.ndof; i++) iperm[perm[i]] = i;
   leaves.clear();
@@ -459,6 +486,17 @@ static bool admissible(const Cluster& a, const Cluster& b, double eta) {
 static void part_rec(const Tree& rt, const Tree& ct, int i, int j, double eta, std::vector<Block>& out) {
   const Cluster& r = rt.cl[i];
   const Cluster& c = ct.cl[j];
+  if (!rt.ilo.empty() && !ct.ilo.empty()) {  // several instances: no block couples two of them
+    if (rt.ihi[i] <= ct.ilo[j] || ct.ihi[j] <= rt.ilo[i]) return;
+    const bool mr = rt.ihi[i] - rt.ilo[i] > 1, mc = ct.ihi[j] - ct.ilo[j] > 1;
+    if (mr || mc) {
+      for (int a = 0; a < (mr ? 2 : 1); a++)
+        for (int bb = 0; bb < (mc ? 2 : 1); bb++)
+          part_rec(rt, ct, mr ? (int)(a ? r.child2 : r.child1) : i, mc ? (int)(bb ? c.child2 : c.child1) : j, eta,
+                   out);
+      return;
+    }
+  }
   int near;
   if (r.child1 == -1 || c.child1 == -1) near = 1;
   else if (admissible(r, c, eta)) near = 0;

@@ -62,7 +62,12 @@ struct Tree {
   std::vector<int> perm, iperm;  // cluster position <-> dof
   std::vector<Cluster> cl;
   std::vector<int> leaves;       // leaf clusters in position order
-  void build(const Space& s, int leafsize);
+  // several geometries in one space (dofs ordered instance by instance):
+  // instance range [ilo, ihi) per cluster; clusters spanning several instances
+  // are split between instances (never leaves), single-instance clusters by
+  // the usual PCA bisection
+  std::vector<int> ilo, ihi;
+  void build(const Space& s, int leafsize, const std::vector<int>* inst_of_dof = nullptr);
 };
 
 struct Block {

@@ -194,6 +194,7 @@ struct Seg {
 struct Operator {
   int kernel = K_LSL;
   double kappa = 0.0;
+  std::vector<double> kappa_el;  // per element, when the mesh holds several geometries
   KernelInfo kinfo;
   bool complex_valued = false;
   int intorder = 3, leafsize = 40;

@@ -18,6 +18,7 @@ import numpy as np
 import torch
 
 from . import _capi
+from .mesh import Mesh
 from .spaces import FESpace, H1, HCurl, HDivSurface, SurfaceL2, mass_matrix_coo
 
 __all__ = ["BaseMatrix", "HMatrix", "IntegralOperator", "IntegralOperatorC", "MassMatrix",
@@ -25,7 +26,7 @@ __all__ = ["BaseMatrix", "HMatrix", "IntegralOperator", "IntegralOperatorC", "Ma
            "HelmholtzSingleLayerPotentialOperator", "HelmholtzDoubleLayerPotentialOperator",
            "HelmholtzCombinedFieldOperator", "HelmholtzHypersingularOperator",
            "MaxwellSingleLayerPotentialOperator", "MaxwellSingleLayerPotentialOperatorCurl",
-           "MaxwellDoubleLayerPotentialOperator", "HelmholtzSingleLayerPotentialOperatorDKappa", "device"]
+           "MaxwellDoubleLayerPotentialOperator", "HelmholtzSingleLayerPotentialOperatorDKappa", "MultiGeometryOperator", "union_mesh", "device"]
 
 
 def device() -> torch.device:
@@ -292,6 +293,45 @@ class HMatrix(BaseMatrix):
         return self._op.blocks()
 
 
+# ------------------------------------------------- several geometries --
+
+def union_mesh(meshes):
+    """One mesh holding the given surfaces one after another: (mesh, element
+    offsets, vertex offsets)."""
+    xyz, tri, eoff, voff = [], [], [0], [0]
+    for m in meshes:
+        tri.append(m.tri.astype(np.int64) + voff[-1])
+        xyz.append(m.xyz)
+        eoff.append(eoff[-1] + m.nt)
+        voff.append(voff[-1] + m.nv)
+    return Mesh(np.concatenate(xyz), np.concatenate(tri)), np.array(eoff), np.array(voff)
+
+
+def MultiGeometryOperator(factory_kernel, meshes, kappas=None, trial_space=SurfaceL2, test_space=None, intorder=3,
+                          leafsize=40, eta=2., eps=1e-6, mixed=False):
+    """One operator over several geometries (block diagonal): every geometry
+    gets the blocks its own operator would have, with its own wavenumber, and
+    assembly and apply of all of them run in the launches of one operator
+    (bem_op_create_multi).  ``factory_kernel``: a kernel id of ``_capi``
+    (LAPLACE_SL, HELMHOLTZ_SL, HELMHOLTZ_SL_DKAPPA, ...); the spaces are
+    classes (SurfaceL2, H1, ...) instantiated on the union mesh.  Vectors are
+    the concatenation of the geometries' vectors.  Returns the operator; its
+    ``dof_offsets`` split vectors per geometry.  No counterpart in the
+    reference."""
+    mesh, eoff, _ = union_mesh(meshes)
+    n = len(meshes)
+    kap = np.zeros(n) if kappas is None else np.asarray(kappas, dtype=np.float64)
+    if kap.shape != (n,):
+        raise ValueError("one wavenumber per geometry")
+    trial = trial_space(mesh)
+    test = trial if test_space is None else test_space(mesh)
+    cls = IntegralOperatorC if factory_kernel >= _capi.HELMHOLTZ_SL else IntegralOperator
+    op = cls(factory_kernel, kap, trial, test, intorder, leafsize, eta, eps, False, mixed, False, _multi=(kap, eoff))
+    op.dof_offsets = np.cumsum([0] + [trial_space(m).ndof for m in meshes])
+    op.meshes = list(meshes)
+    return op
+
+
 # -------------------------------------------------------------- potential --
 
 class PotentialCF:
@@ -332,18 +372,26 @@ class IntegralOperator:
     is_complex = False
 
     def __init__(self, kernel, kappa, trial_space, test_space, intorder, leafsize, eta, eps, dense=False,
-                 mixed=False, testhmatrix=False):
+                 mixed=False, testhmatrix=False, _multi=None):
         if not isinstance(trial_space, FESpace) or not isinstance(test_space, FESpace):
             raise TypeError("spaces must be ngbem_amd SurfaceL2 / H1 / HDivSurface / HCurl spaces")
         self.trial_space, self.test_space = trial_space, test_space
-        self.kernel, self.kappa = kernel, float(kappa)
+        self.kernel, self.kappa = kernel, float(np.atleast_1d(kappa)[0])
         self.params = dict(intorder=intorder, leafsize=leafsize, eta=eta, eps=eps, dense=dense)
         self.has_mixed = bool(mixed)
         L = _capi.lib()
         p = _capi.Params(int(intorder), int(leafsize), float(eta), float(eps), int(bool(dense)), int(bool(mixed)))
         out = ctypes.c_void_p()
-        _capi.check(L.bem_op_create(kernel, float(kappa), trial_space.handle, test_space.handle, ctypes.byref(p),
-                                    ctypes.byref(out)), "bem_op_create")
+        if _multi is None:
+            _capi.check(L.bem_op_create(kernel, float(kappa), trial_space.handle, test_space.handle, ctypes.byref(p),
+                                        ctypes.byref(out)), "bem_op_create")
+        else:  # several geometries: per-geometry wavenumbers and element offsets
+            kap, eoff = _multi
+            kap = np.ascontiguousarray(kap, dtype=np.float64)
+            eoff = np.ascontiguousarray(eoff, dtype=np.int32)
+            _capi.check(L.bem_op_create_multi(kernel, _capi.dptr(kap), len(kap), _capi.iptr(eoff),
+                                              trial_space.handle, test_space.handle, ctypes.byref(p),
+                                              ctypes.byref(out)), "bem_op_create_multi")
         self._h = out
         dev = device()
         with torch.cuda.device(dev):

@@ -116,3 +116,74 @@ def test_dense_method_is_one_block():
     L.bem_op_block_info(h, 0, ctypes.byref(bi))
     assert bi.near == 1 and bi.m == m.nt and bi.n == m.nt
     L.bem_op_destroy(h)
+
+
+def _blocks_as_sets(h, offset=0):
+    L = _capi.lib()
+    out = set()
+    for b in range(L.bem_op_num_blocks(h)):
+        bi = _capi.BlockInfo()
+        L.bem_op_block_info(h, b, ctypes.byref(bi))
+        rows = np.zeros(bi.m, dtype=np.int32)
+        cols = np.zeros(bi.n, dtype=np.int32)
+        L.bem_op_block_dofs(h, b, _capi.iptr(rows), _capi.iptr(cols))
+        out.add((tuple(sorted(rows - offset)), tuple(sorted(cols - offset)), bool(bi.near)))
+    return out
+
+
+@pytest.mark.parametrize("kind", [0, 1])
+def test_multi_geometry_partition_is_per_geometry(kind):
+    """Several geometries in one operator: no block couples two of them, and
+    each geometry gets exactly the blocks of its own operator."""
+    meshes = [nb.ellipsoid(2, (1.0, 0.7, 1.3)), nb.ellipsoid(3, (0.6, 1.1, 0.9)), nb.ellipsoid(2, (1.5, 1.0, 0.5))]
+    U, eoff, voff = nb.union_mesh(meshes)
+    sp = nb.SurfaceL2 if kind == 0 else nb.H1
+    su = sp(U)
+    L = _capi.lib()
+    params = _capi.Params(3, 24, 2.0, 1e-6, 0, 0)
+    kap = np.array([1.0, 2.0, 3.0])
+    out = ctypes.c_void_p()
+    _capi.check(L.bem_op_create_multi(_capi.HELMHOLTZ_SL, _capi.dptr(kap), 3, _capi.iptr(eoff.astype(np.int32)),
+                                      su.handle, su.handle, ctypes.byref(params), ctypes.byref(out)))
+    try:
+        doff = eoff if kind == 0 else voff
+        got = _blocks_as_sets(out)
+        for i, m in enumerate(meshes):
+            si = sp(m)
+            h = op_create(si, si, _capi.HELMHOLTZ_SL, leafsize=24, kappa=kap[i])
+            try:
+                want = {(tuple(r + doff[i] for r in rows), tuple(c + doff[i] for c in cols), near)
+                        for rows, cols, near in _blocks_as_sets(h)}
+            finally:
+                L.bem_op_destroy(h)
+            mine = {b for b in got if doff[i] <= b[0][0] < doff[i + 1]}
+            assert mine == want
+            for rows, cols, _ in mine:
+                assert doff[i] <= min(cols) and max(cols) < doff[i + 1]
+        assert sum(1 for _ in got) == len(got)
+    finally:
+        L.bem_op_destroy(out)
+
+
+def test_multi_geometry_rejects_bad_input():
+    L = _capi.lib()
+    m1 = nb.icosphere(1)
+    U, eoff, _ = nb.union_mesh([m1, m1])
+    su = nb.SurfaceL2(U)
+    params = _capi.Params(3, 24, 2.0, 1e-6, 0, 0)
+    out = ctypes.c_void_p()
+    kap = np.array([1.0, 2.0])
+    with pytest.raises(RuntimeError, match="element offsets"):
+        bad = np.array([0, 10, 30], dtype=np.int32)
+        _capi.check(L.bem_op_create_multi(_capi.HELMHOLTZ_SL, _capi.dptr(kap), 2, _capi.iptr(bad), su.handle,
+                                          su.handle, ctypes.byref(params), ctypes.byref(out)))
+    with pytest.raises(RuntimeError, match="share a vertex"):
+        shared = nb.Mesh(U.xyz, np.where(U.tri >= m1.nv, U.tri - m1.nv, U.tri))  # second copy reuses the vertices
+        ss = nb.SurfaceL2(shared)
+        _capi.check(L.bem_op_create_multi(_capi.HELMHOLTZ_SL, _capi.dptr(kap), 2,
+                                          _capi.iptr(eoff.astype(np.int32)), ss.handle, ss.handle,
+                                          ctypes.byref(params), ctypes.byref(out)))
+    hd = nb.HDivSurface(U)
+    with pytest.raises(RuntimeError, match="Maxwell single layer"):
+        _capi.check(L.bem_op_create_multi(_capi.MAXWELL_SL, _capi.dptr(kap), 2, _capi.iptr(eoff.astype(np.int32)),
+                                          hd.handle, hd.handle, ctypes.byref(params), ctypes.byref(out)))

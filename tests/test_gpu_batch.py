@@ -161,3 +161,46 @@ def test_consumed_batch_equals_copied_batch(gpu, batch_setup):
     assert torch.equal(C * x, B * x)
     with pytest.raises(RuntimeError, match="not assembled"):
         fresh[0].mat * x[: spaces[0].ndof]
+
+
+@pytest.mark.parametrize("kernel,trial,test", [
+    ("HSL", nb.SurfaceL2, None), ("HHS", nb.H1, None), ("LDL", nb.H1, nb.SurfaceL2), ("HCF", nb.SurfaceL2, None)])
+def test_multi_geometry_operator_equals_single_operators(gpu, kernel, trial, test):
+    """One operator over several geometries (one assembly, one apply) gives
+    every geometry the result of its own operator, with its own wavenumber."""
+    from ngbem_amd import _capi
+
+    inst = _instances(5, levels=(2, 3), seed=9)
+    meshes = [m for m, _ in inst]
+    kap = [k for _, k in inst]
+    kid = {"HSL": _capi.HELMHOLTZ_SL, "HHS": _capi.HELMHOLTZ_HS, "LDL": _capi.LAPLACE_DL,
+           "HCF": _capi.HELMHOLTZ_CF}[kernel]
+    fac = {"HSL": nb.HelmholtzSingleLayerPotentialOperator, "HHS": nb.HelmholtzHypersingularOperator,
+           "LDL": nb.DoubleLayerPotentialOperator, "HCF": nb.HelmholtzCombinedFieldOperator}[kernel]
+    kw = dict(intorder=3, leafsize=24, eta=2.0, eps=1e-8)
+    print(f"multi-geometry {kernel}: {[m.nt for m in meshes]} triangles", flush=True)
+    M = nb.MultiGeometryOperator(kid, meshes, kap if kid >= _capi.HELMHOLTZ_SL else None, trial, test, **kw)
+    singles = []
+    for m, k in zip(meshes, kap):
+        tr, te = trial(m), (test(m) if test else None)
+        if kernel == "LDL":
+            singles.append(fac(tr, te, **kw))
+        elif kernel == "HHS":
+            singles.append(fac(tr, kappa=k, **kw))
+        else:
+            singles.append(fac(tr, tr, kappa=k, **kw))
+    toff = np.cumsum([0] + [s.test_space.ndof for s in singles])
+    coff = np.cumsum([0] + [s.trial_space.ndof for s in singles])
+    dt = torch.complex128 if M.is_complex else torch.float64
+    g = torch.Generator(device="cpu").manual_seed(1)
+    x = torch.randn(M.trial_space.ndof, dtype=dt, generator=g).to(gpu)
+    xt = torch.randn(M.test_space.ndof, dtype=dt, generator=g).to(gpu)
+    y = M.mat * x
+    yt = M.mat.T * xt
+    ref = torch.cat([s.mat * x[coff[i]:coff[i + 1]] for i, s in enumerate(singles)])
+    ref_t = torch.cat([s.mat.T * xt[toff[i]:toff[i + 1]] for i, s in enumerate(singles)])
+    assert (torch.linalg.norm(y - ref) / torch.linalg.norm(ref)).item() <= 1e-13
+    assert (torch.linalg.norm(yt - ref_t) / torch.linalg.norm(ref_t)).item() <= 1e-13
+    st = M.stats()
+    assert st["n_near"] == sum(s.stats()["n_near"] for s in singles)
+    assert st["n_far"] == sum(s.stats()["n_far"] for s in singles)
