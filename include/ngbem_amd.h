@@ -114,6 +114,17 @@ typedef struct {
 const char* bem_last_error(void);
 int bem_device_count(void);
 
+/* TestCompression (src/python_bem.cpp:177-193, src/test_compression.cpp):
+ * bem_test_points fills the two clouds of CreateMatrix (x: nx points, y: ny
+ * points, 3 doubles each) from the C library's rand() after srand(seed); the
+ * reference calls rand() without seeding, so its first call in a process sees
+ * seed 1.  bem_test_compression_aca runs TestCompressionACA's ACA of
+ * 1/|x_i - y_j| on the GPU and returns the rank, the Frobenius error of U V
+ * against the matrix and the kernel seconds. */
+int bem_test_points(int nx, int ny, double eta, unsigned seed, double* X, double* Y);
+int bem_test_compression_aca(const double* X, int nx, const double* Y, int ny, double eps, int* rank, double* err,
+                             double* seconds);
+
 /* Device memory of the library comes from hipMalloc, or with the environment
  * variable NGBEM_ALLOC=pool from the device's stream-ordered pool, which keeps
  * freed memory mapped (NGBEM_POOL_KEEP_GB, default 64).  bem_reserve maps

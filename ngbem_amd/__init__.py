@@ -18,6 +18,7 @@ from .operators import (BaseMatrix, HMatrix, IntegralOperator, IntegralOperatorC
                         MultiGeometryOperator, union_mesh)
 from .krylov import CG, GMRes, GMResBatch, RefinedSolve
 from .batch import OperatorBatch, SingleLayerBatch, assemble_many
+from .compression import TestCompression
 
 
 def reserve(gib: float) -> None:
@@ -33,4 +34,5 @@ __all__ = ["Mesh", "icosphere", "ellipsoid", "SurfaceL2", "H1", "HDivSurface", "
            "HelmholtzCombinedFieldOperator", "HelmholtzHypersingularOperator", "MaxwellSingleLayerPotentialOperator",
            "MaxwellSingleLayerPotentialOperatorCurl", "MaxwellDoubleLayerPotentialOperator",
            "HelmholtzSingleLayerPotentialOperatorDKappa", "CG", "GMRes", "GMResBatch", "RefinedSolve",
-           "OperatorBatch", "SingleLayerBatch", "assemble_many", "reserve", "MultiGeometryOperator", "union_mesh"]
+           "OperatorBatch", "SingleLayerBatch", "assemble_many", "reserve", "MultiGeometryOperator", "union_mesh",
+           "TestCompression"]

@@ -48,7 +48,7 @@ class BlockInfo(ctypes.Structure):
 
 
 # every symbol include/ngbem_amd.h declares
-EXPORTS = ["bem_last_error", "bem_device_count", "bem_reserve", "bem_mesh_create", "bem_mesh_destroy", "bem_space_create",
+EXPORTS = ["bem_last_error", "bem_device_count", "bem_reserve", "bem_test_points", "bem_test_compression_aca", "bem_mesh_create", "bem_mesh_destroy", "bem_space_create",
            "bem_space_ndof", "bem_space_destroy", "bem_op_create", "bem_op_create_multi", "bem_op_assemble", "bem_op_apply",
            "bem_op_stats", "bem_op_profile_apply", "bem_op_destroy", "bem_op_num_blocks", "bem_op_block_info", "bem_op_block_dofs",
            "bem_op_block_data", "bem_op_tree", "bem_op_potential_dim", "bem_op_potential", "bem_batch_create", "bem_batch_sizes", "bem_batch_apply",
@@ -91,6 +91,9 @@ def lib():
     L.bem_op_block_data.argtypes = [P, ctypes.c_int, P]
     L.bem_op_tree.argtypes = [P, ctypes.c_int, ip, dp, dp, ctypes.c_int]
     L.bem_reserve.argtypes = [ctypes.c_double]
+    L.bem_test_points.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.c_double, ctypes.c_uint, dp, dp]
+    L.bem_test_compression_aca.argtypes = [dp, ctypes.c_int, dp, ctypes.c_int, ctypes.c_double,
+                                           ctypes.POINTER(ctypes.c_int), dp, dp]
     L.bem_op_potential_dim.argtypes = [P]
     L.bem_op_potential.argtypes = [P, P, P, P, ctypes.c_int, P, ctypes.c_int, P]
     L.bem_batch_create.argtypes = [ctypes.POINTER(P), ctypes.c_int, ctypes.c_int, ctypes.POINTER(P)]

@@ -3,6 +3,8 @@
 #include <hip/hip_runtime.h>
 
 #include <chrono>
+#include <cmath>
+#include <cstdlib>
 #include <climits>
 #include <cstring>
 #include <memory>
@@ -512,6 +514,24 @@ int bem_op_tree(const bem_op* h, int which, int32_t* perm, double* cl, double* c
 }
 
 }  // extern "C"
+
+int bem_test_points(int nx, int ny, double eta, unsigned seed, double* X, double* Y) {
+  if (nx < 1 || ny < 1 || !(eta > 0.0) || !X || !Y) return fail(-1, "bem_test_points: bad arguments");
+  // CreateMatrix (src/test_compression.cpp:16-43): rand() / RAND_MAX in the unit
+  // cube, x points first; y shifted by 1 + sqrt(3) / eta along the first axis
+  std::srand(seed);
+  for (int i = 0; i < 3 * nx; i++) X[i] = (double)std::rand() / RAND_MAX;
+  for (int i = 0; i < 3 * ny; i++) Y[i] = (double)std::rand() / RAND_MAX;
+  for (int j = 0; j < ny; j++) Y[3 * j] += 1.0 + std::sqrt(3.0) / eta;
+  return 0;
+}
+
+int bem_test_compression_aca(const double* X, int nx, const double* Y, int ny, double eps, int* rank, double* err,
+                             double* seconds) {
+  if (!X || !Y || nx < 1 || ny < 1 || nx > 4096 || !rank || !err || !seconds)
+    return fail(-1, "bem_test_compression_aca: bad arguments (1 <= nx <= 4096)");
+  return guarded([&] { bem::test_compression_aca(X, nx, Y, ny, eps, rank, err, seconds); });
+}
 
 int bem_reserve(double bytes) {
   if (!(bytes >= 0.0)) return fail(-1, "bem_reserve: bad arguments");

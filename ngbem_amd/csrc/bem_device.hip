@@ -2277,6 +2277,145 @@ void potential_device(Operator& op, const void* coef, const double* pts, const d
   });
 }
 
+// ------------------------------------------------------ compression test --
+// TestCompressionACA (src/test_compression.cpp:170-244): partially pivoted ACA
+// of M_ij = 1 / |x_i - y_j| between two point clouds, with that file's rules
+// (start at row 0; pivot scaled by 1 / v_jk; exclude the previous row and
+// column pivots; eps scaled by 2/3 / sqrt(m n); stop when |u_k| |v_k| <
+// eps sqrt(norm2)).  One workgroup; entries generated on the fly; U column-
+// major (m x cap), V row-major (cap x n).
+__device__ __forceinline__ double inv_dist(const double* x, const double* y) {
+  const double d0 = x[0] - y[0], d1 = x[1] - y[1], d2 = x[2] - y[2];
+  return 1.0 / sqrt(d0 * d0 + d1 * d1 + d2 * d2);
+}
+
+__device__ double block_sum(double v, double* sh) {
+  v = wave_sum(v);
+  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+  __syncthreads();
+  if (lane == 0) sh[wv] = v;
+  __syncthreads();
+  return sh[0] + sh[1] + sh[2] + sh[3];
+}
+
+__global__ __launch_bounds__(256) void k_aca_points(const double* __restrict__ X, int nx, const double* __restrict__ Y,
+                                                    int ny, double eps, int cap, double* __restrict__ U,
+                                                    double* __restrict__ V, int* __restrict__ rank_out) {
+  __shared__ ArgMax s_am[4];
+  __shared__ double s_red[4];
+  extern __shared__ double s_c[];  // cap coefficients
+  const double epsk = 2.0 / 3.0 * eps / sqrt((double)nx * (double)ny);
+  int ik = 0, jk = 0, ikm1 = -1, jkm1 = -1, rank = cap;
+  double norm2 = 0.0;
+  for (int k = 0; k < cap; k++) {
+    // row ik of the residual
+    for (int l = threadIdx.x; l < k; l += 256) s_c[l] = U[(size_t)l * nx + ik];
+    __syncthreads();
+    ArgMax am{-1.0, 0, 0.0, 0.0};
+    for (int j = threadIdx.x; j < ny; j += 256) {
+      double v = inv_dist(X + 3 * ik, Y + 3 * j);
+      for (int l = 0; l < k; l++) v -= s_c[l] * V[(size_t)l * ny + j];
+      V[(size_t)k * ny + j] = v;
+      const double a2 = v * v;
+      if (j != jkm1 && a2 > 0.0 && (a2 > am.a2 || (a2 == am.a2 && j < am.idx))) am = ArgMax{a2, j, v, 0.0};
+    }
+    am = block_argmax<4>(am, s_am);
+    if (am.a2 > 0.0) jk = am.idx;  // the reference keeps the last pivot when the row vanishes
+    const double vjk = V[(size_t)k * ny + jk];
+    __syncthreads();
+    for (int j = threadIdx.x; j < ny; j += 256) V[(size_t)k * ny + j] *= 1.0 / vjk;
+    // column jk of the residual
+    for (int l = threadIdx.x; l < k; l += 256) s_c[l] = V[(size_t)l * ny + jk];
+    __syncthreads();
+    ArgMax bm{-1.0, 0, 0.0, 0.0};
+    for (int i = threadIdx.x; i < nx; i += 256) {
+      double u = inv_dist(X + 3 * i, Y + 3 * jk);
+      for (int l = 0; l < k; l++) u -= U[(size_t)l * nx + i] * s_c[l];
+      U[(size_t)k * nx + i] = u;
+      const double a2 = u * u;
+      if (i != ikm1 && a2 > 0.0 && (a2 > bm.a2 || (a2 == bm.a2 && i < bm.idx))) bm = ArgMax{a2, i, u, 0.0};
+    }
+    bm = block_argmax<4>(bm, s_am);
+    if (bm.a2 > 0.0) ik = bm.idx;
+    __syncthreads();
+    // norm update
+    double pv = 0.0, pu = 0.0;
+    for (int j = threadIdx.x; j < ny; j += 256) pv += V[(size_t)k * ny + j] * V[(size_t)k * ny + j];
+    for (int i = threadIdx.x; i < nx; i += 256) pu += U[(size_t)k * nx + i] * U[(size_t)k * nx + i];
+    const double nv = block_sum(pv, s_red), nu = block_sum(pu, s_red);
+    const double norm_k = sqrt(nv) * sqrt(nu);
+    norm2 += norm_k * norm_k;
+    for (int l = 0; l < k; l++) {
+      double a = 0.0, b = 0.0;
+      for (int j = threadIdx.x; j < ny; j += 256) a += V[(size_t)k * ny + j] * V[(size_t)l * ny + j];
+      for (int i = threadIdx.x; i < nx; i += 256) b += U[(size_t)k * nx + i] * U[(size_t)l * nx + i];
+      const double va = block_sum(a, s_red), ub = block_sum(b, s_red);
+      norm2 += 2.0 * va * ub;
+    }
+    ikm1 = ik;
+    jkm1 = jk;
+    if (norm_k < epsk * sqrt(norm2)) {
+      rank = k + 1;
+      break;
+    }
+  }
+  if (threadIdx.x == 0) *rank_out = rank;
+}
+
+// per row i: sum_j (M_ij - sum_l U_il V_lj)^2
+__global__ __launch_bounds__(256) void k_aca_points_err(const double* __restrict__ X, int nx,
+                                                        const double* __restrict__ Y, int ny,
+                                                        const double* __restrict__ U, const double* __restrict__ V,
+                                                        int rank, double* __restrict__ rowerr) {
+  const int i = blockIdx.x * 256 + threadIdx.x;
+  if (i >= nx) return;
+  double s = 0.0;
+  for (int j = 0; j < ny; j++) {
+    double r = inv_dist(X + 3 * i, Y + 3 * j);
+    for (int l = 0; l < rank; l++) r -= U[(size_t)l * nx + i] * V[(size_t)l * ny + j];
+    s += r * r;
+  }
+  rowerr[i] = s;
+}
+
+void test_compression_aca(const double* X, int nx, const double* Y, int ny, double eps, int* rank, double* err,
+                          double* seconds) {
+  const int cap = nx;  // U has nx columns in the reference
+  DBuf dX, dY, dU, dV, dR, dE;
+  std::vector<double> hx(X, X + 3 * (size_t)nx), hy(Y, Y + 3 * (size_t)ny);
+  hipStream_t s = nullptr;
+  dX.upload(hx, s);
+  dY.upload(hy, s);
+  dU.alloc((size_t)cap * nx * sizeof(double));
+  dV.alloc((size_t)cap * ny * sizeof(double));
+  dR.alloc(sizeof(int));
+  dE.alloc((size_t)nx * sizeof(double));
+  hipEvent_t e0, e1;
+  BEM_HIP(hipEventCreate(&e0));
+  BEM_HIP(hipEventCreate(&e1));
+  BEM_HIP(hipEventRecord(e0, s));
+  hipLaunchKernelGGL(k_aca_points, dim3(1), dim3(256), (size_t)cap * sizeof(double), s, dX.as<double>(), nx,
+                     dY.as<double>(), ny, eps, cap, dU.as<double>(), dV.as<double>(), dR.as<int>());
+  BEM_HIP(hipEventRecord(e1, s));
+  BEM_HIP(hipGetLastError());
+  BEM_HIP(hipMemcpyAsync(rank, dR.p, sizeof(int), hipMemcpyDeviceToHost, s));
+  BEM_HIP(hipStreamSynchronize(s));
+  float ms = 0.0f;
+  BEM_HIP(hipEventElapsedTime(&ms, e0, e1));
+  *seconds = ms * 1e-3;
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
+  hipLaunchKernelGGL(k_aca_points_err, dim3((nx + 255) / 256), dim3(256), 0, s, dX.as<double>(), nx, dY.as<double>(),
+                     ny, dU.as<double>(), dV.as<double>(), *rank, dE.as<double>());
+  BEM_HIP(hipGetLastError());
+  std::vector<double> re(nx);
+  BEM_HIP(hipMemcpyAsync(re.data(), dE.p, (size_t)nx * sizeof(double), hipMemcpyDeviceToHost, s));
+  BEM_HIP(hipStreamSynchronize(s));
+  double t = 0.0;
+  for (double v : re) t += v;
+  *err = std::sqrt(t);
+}
+
 // ----------------------------------------------------------------- batch --
 // Block-diagonal union of assembled operators: storage copied into one set of
 // arrays, vectors concatenated, and one matvec plan whose tiles, segments and

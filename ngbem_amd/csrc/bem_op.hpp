@@ -281,6 +281,9 @@ void apply_device(Operator& op, int flags, const double* alpha, const void* x, c
 int potential_dim(const Operator& op);
 void potential_device(Operator& op, const void* coef, const double* pts, const double* nrm, int np, void* out,
                       hipStream_t s);
+// ACA of 1/|x_i - y_j| between two point clouds (TestCompression, method aca)
+void test_compression_aca(const double* X, int nx, const double* Y, int ny, double eps, int* rank, double* err,
+                          double* seconds);
 // block-diagonal union of assembled operators (storage copied, one plan)
 // consume: free each operator's block storage once copied (it can no longer be applied)
 void build_batch(const std::vector<Operator*>& ops, Operator& b, hipStream_t s, bool consume);

@@ -38,6 +38,7 @@
 #include <algorithm>
 #include <chrono>
 #include <cmath>
+#include <cstdlib>
 #include <complex>
 #include <cstdint>
 #include <cstring>
@@ -1290,6 +1291,96 @@ int orc_potential(void* p, const double* coef, const double* pts, const double* 
     for (int d = 0; d < dim; d++) o[(size_t)i * dim + d] = acc[d];
   }
   return dim;
+}
+
+// TestCompression (src/test_compression.cpp): CreateMatrix (:16-43) from the C
+// library's rand() after srand(seed), and CalcACA (:170-226) on it; returns the
+// rank, writes the Frobenius error of U V.
+static void test_points(int nx, int ny, double eta, unsigned seed, std::vector<double>& X, std::vector<double>& Y) {
+  std::srand(seed);
+  X.resize(3 * (size_t)nx);
+  Y.resize(3 * (size_t)ny);
+  for (auto& v : X) v = (double)std::rand() / RAND_MAX;
+  for (auto& v : Y) v = (double)std::rand() / RAND_MAX;
+  for (int j = 0; j < ny; j++) Y[3 * j] += 1.0 + std::sqrt(3.0) / eta;
+}
+
+void orc_test_points(int nx, int ny, double eta, unsigned seed, double* X, double* Y) {
+  std::vector<double> x, y;
+  test_points(nx, ny, eta, seed, x, y);
+  std::copy(x.begin(), x.end(), X);
+  std::copy(y.begin(), y.end(), Y);
+}
+
+int orc_test_compression_aca(int nx, int ny, double eta, double eps, unsigned seed, double* err, double* Uo,
+                             double* Vo, int* pivi, int* pivj) {
+  std::vector<double> X, Y;
+  test_points(nx, ny, eta, seed, X, Y);
+  auto M = [&](int i, int j) {
+    double d0 = X[3 * i] - Y[3 * j], d1 = X[3 * i + 1] - Y[3 * j + 1], d2 = X[3 * i + 2] - Y[3 * j + 2];
+    return 1.0 / std::sqrt(d0 * d0 + d1 * d1 + d2 * d2);
+  };
+  const int cap = nx;
+  std::vector<double> U((size_t)nx * cap), V((size_t)cap * ny);  // U(i,k) at U[k*nx+i], V(k,j) at V[k*ny+j]
+  int ik = 0, jk = 0, ikm1 = -1, jkm1 = -1, rank = cap;
+  double norm2 = 0.0;
+  eps = 2.0 / 3.0 * eps / std::sqrt((double)nx * ny);
+  for (int k = 0; k < cap; k++) {
+    for (int j = 0; j < ny; j++) {
+      double v = M(ik, j);
+      for (int l = 0; l < k; l++) v -= V[(size_t)l * ny + j] * U[(size_t)l * nx + ik];
+      V[(size_t)k * ny + j] = v;
+    }
+    double vkj = 0.0;
+    for (int j = 0; j < ny; j++)
+      if (std::fabs(V[(size_t)k * ny + j]) > vkj && j != jkm1) {
+        vkj = std::fabs(V[(size_t)k * ny + j]);
+        jk = j;
+      }
+    const double piv = V[(size_t)k * ny + jk];
+    if (pivi) pivi[k] = ik;
+    if (pivj) pivj[k] = jk;
+    for (int j = 0; j < ny; j++) V[(size_t)k * ny + j] *= 1.0 / piv;
+    for (int i = 0; i < nx; i++) {
+      double u = M(i, jk);
+      for (int l = 0; l < k; l++) u -= U[(size_t)l * nx + i] * V[(size_t)l * ny + jk];
+      U[(size_t)k * nx + i] = u;
+    }
+    double uik = 0.0;
+    for (int i = 0; i < nx; i++)
+      if (std::fabs(U[(size_t)k * nx + i]) > uik && i != ikm1) {
+        uik = std::fabs(U[(size_t)k * nx + i]);
+        ik = i;
+      }
+    double nv = 0.0, nu = 0.0;
+    for (int j = 0; j < ny; j++) nv += V[(size_t)k * ny + j] * V[(size_t)k * ny + j];
+    for (int i = 0; i < nx; i++) nu += U[(size_t)k * nx + i] * U[(size_t)k * nx + i];
+    const double norm_k = std::sqrt(nv) * std::sqrt(nu);
+    norm2 += norm_k * norm_k;
+    for (int l = 0; l < k; l++) {
+      double a = 0.0, b = 0.0;
+      for (int j = 0; j < ny; j++) a += V[(size_t)k * ny + j] * V[(size_t)l * ny + j];
+      for (int i = 0; i < nx; i++) b += U[(size_t)k * nx + i] * U[(size_t)l * nx + i];
+      norm2 += 2.0 * a * b;
+    }
+    ikm1 = ik;
+    jkm1 = jk;
+    if (norm_k < eps * std::sqrt(norm2)) {
+      rank = k + 1;
+      break;
+    }
+  }
+  double e2 = 0.0;
+  for (int i = 0; i < nx; i++)
+    for (int j = 0; j < ny; j++) {
+      double r = M(i, j);
+      for (int l = 0; l < rank; l++) r -= U[(size_t)l * nx + i] * V[(size_t)l * ny + j];
+      e2 += r * r;
+    }
+  *err = std::sqrt(e2);
+  if (Uo) std::copy(U.begin(), U.begin() + (size_t)rank * nx, Uo);
+  if (Vo) std::copy(V.begin(), V.begin() + (size_t)rank * ny, Vo);
+  return rank;
 }
 
 int orc_max_threads() {

@@ -57,6 +57,9 @@ def lib():
         L.orc_nze.restype = ctypes.c_long
         L.orc_nze.argtypes = [P]
         L.orc_potential.argtypes = [P, dp, dp, dp, ctypes.c_int, dp]
+        L.orc_test_points.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.c_double, ctypes.c_uint, dp, dp]
+        L.orc_test_compression_aca.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.c_double, ctypes.c_double,
+                                               ctypes.c_uint, dp, dp, dp, ip, ip]
         L.orc_apply.restype = ctypes.c_double
         L.orc_apply.argtypes = [P, ctypes.c_int, dp, dp, ctypes.c_int]
         _lib = L
@@ -89,6 +92,27 @@ def gauss01(n):
     x, w = np.zeros(n), np.zeros(n)
     lib().orc_gauss01(n, _dp(x), _dp(w))
     return x, w
+
+
+def test_points(nx, ny, eta, seed=1):
+    X, Y = np.zeros((nx, 3)), np.zeros((ny, 3))
+    lib().orc_test_points(nx, ny, float(eta), seed, _dp(X), _dp(Y))
+    return X, Y
+
+
+def test_compression_aca(nx, ny, eta, eps, seed=1, factors=False):
+    """(rank, Frobenius error) of TestCompressionACA restated; with factors=True
+    also U (nx x r), V (r x ny) and the row / column pivots."""
+    err = np.zeros(1)
+    U, V = np.zeros((nx, nx)), np.zeros((nx, ny))
+    pi, pj = np.zeros(nx, dtype=np.int32), np.zeros(nx, dtype=np.int32)
+    r = lib().orc_test_compression_aca(nx, ny, float(eta), float(eps), seed, _dp(err), _dp(U), _dp(V), _ip(pi),
+                                       _ip(pj))
+    if not factors:
+        return r, float(err[0])
+    Uc = U.reshape(-1)[: r * nx].reshape(r, nx).T.copy()  # stored column by column
+    Vr = V.reshape(-1)[: r * ny].reshape(r, ny).copy()
+    return r, float(err[0]), Uc, Vr, pi[:r].copy(), pj[:r].copy()
 
 
 class OracleMesh:
