@@ -100,3 +100,157 @@ def ellipsoid(level: int, axes=(1.0, 1.0, 1.0)) -> Mesh:
     """Icosphere of ``level`` scaled by the semi-axes (orientation preserved)."""
     m = icosphere(level)
     return Mesh(m.xyz * np.asarray(axes, dtype=np.float64)[None, :], m.tri)
+
+
+# ------------------------------------------------------------ file formats --
+# Surface triangle meshes from files.  The reference reads geometry through
+# NGSolve/Netgen (e.g. the ship STL of demos/Maxwell_Ship.ipynb); here the
+# common formats are read directly into flat-triangle meshes.  Vertex order is
+# kept as stored (STL: counter-clockwise seen from the facet normal), so that
+# (p0 - p2) x (p1 - p2) follows the file's orientation; ``orient_outward``
+# flips a closed surface whose enclosed volume comes out negative.
+
+
+def _merge_points(P, tri):
+    """Merge coincident vertices (exact coordinate match)."""
+    uniq, inv = np.unique(P, axis=0, return_inverse=True)
+    return uniq, inv.reshape(-1)[tri]
+
+
+def read_stl(path) -> Mesh:
+    """ASCII or binary STL; coincident facet corners become one vertex."""
+    data = open(path, "rb").read()
+    is_ascii = data[:5].lower() == b"solid" and b"facet" in data[:2048]
+    if not is_ascii:
+        n = int(np.frombuffer(data[80:84], dtype="<u4")[0])
+        if len(data) < 84 + 50 * n:
+            raise ValueError("truncated binary STL")
+        rec = np.frombuffer(data[84:84 + 50 * n], dtype=np.dtype([("n", "<f4", 3), ("v", "<f4", (3, 3)),
+                                                                   ("a", "<u2")]))
+        P = rec["v"].reshape(-1, 3).astype(np.float64)
+    else:
+        vals = [line.split()[1:4] for line in data.decode("ascii", "replace").splitlines()
+                if line.strip().lower().startswith("vertex")]
+        P = np.array(vals, dtype=np.float64)
+        if len(P) % 3:
+            raise ValueError("ASCII STL: vertex count is not a multiple of 3")
+    tri = np.arange(len(P)).reshape(-1, 3)
+    xyz, tri = _merge_points(P, tri)
+    return Mesh(xyz, tri)
+
+
+def read_gmsh(path) -> Mesh:
+    """Gmsh ASCII .msh, versions 2.2 and 4.1: the 3-node triangles (element type
+    2); the physical (v2) / entity (v4) tag becomes the element's region."""
+    lines = open(path).read().split("\n")
+    ver = 2.2
+    nodes, tris, regions = {}, [], []
+    i = 0
+    while i < len(lines):
+        s = lines[i].strip()
+        if s == "$MeshFormat":
+            ver = float(lines[i + 1].split()[0])
+        elif s == "$Nodes":
+            if ver < 4:
+                n = int(lines[i + 1])
+                for k in range(n):
+                    f = lines[i + 2 + k].split()
+                    nodes[int(f[0])] = [float(v) for v in f[1:4]]
+                i += n + 2
+            else:
+                nb, _, _, _ = (int(v) for v in lines[i + 1].split())
+                j = i + 2
+                for _ in range(nb):
+                    _, _, _, m = (int(v) for v in lines[j].split())
+                    tags = [int(lines[j + 1 + k]) for k in range(m)]
+                    for k in range(m):
+                        nodes[tags[k]] = [float(v) for v in lines[j + 1 + m + k].split()[:3]]
+                    j += 1 + 2 * m
+                i = j
+            continue
+        elif s == "$Elements":
+            if ver < 4:
+                n = int(lines[i + 1])
+                for k in range(n):
+                    f = [int(v) for v in lines[i + 2 + k].split()]
+                    if f[1] == 2:
+                        nt = f[2]
+                        tris.append(f[3 + nt:6 + nt])
+                        regions.append(f[3] if nt > 0 else 0)
+                i += n + 2
+            else:
+                nb, _, _, _ = (int(v) for v in lines[i + 1].split())
+                j = i + 2
+                for _ in range(nb):
+                    _, tag, etype, m = (int(v) for v in lines[j].split())
+                    for k in range(m):
+                        f = [int(v) for v in lines[j + 1 + k].split()]
+                        if etype == 2:
+                            tris.append(f[1:4])
+                            regions.append(tag)
+                    j += 1 + m
+                i = j
+            continue
+        i += 1
+    if not tris:
+        raise ValueError("no triangles in the Gmsh file")
+    keys = sorted(nodes)
+    index = {k: n for n, k in enumerate(keys)}
+    xyz = np.array([nodes[k] for k in keys])
+    tri = np.array([[index[v] for v in t] for t in tris])
+    m = Mesh(xyz, tri)
+    m.regions = np.array(regions, dtype=np.int32)
+    return m
+
+
+def read_netgen_vol(path) -> Mesh:
+    """Netgen .vol: the "surfaceelements" (triangles; the boundary condition
+    number becomes the region) over the "points" section."""
+    lines = [line.strip() for line in open(path).read().split("\n")]
+    tris, regions, pts = [], [], []
+    i = 0
+    while i < len(lines):
+        s = lines[i]
+        if s in ("surfaceelements", "surfaceelementsgi", "surfaceelementsuv"):
+            n = int(lines[i + 1])
+            for k in range(n):
+                f = lines[i + 2 + k].split()
+                npnt = int(f[4])
+                if npnt == 3:
+                    tris.append([int(v) - 1 for v in f[5:8]])
+                    regions.append(int(f[1]))
+            i += n + 2
+            continue
+        if s == "points":
+            n = int(lines[i + 1])
+            pts = [[float(v) for v in lines[i + 2 + k].split()[:3]] for k in range(n)]
+            i += n + 2
+            continue
+        i += 1
+    if not tris or not pts:
+        raise ValueError("no surface triangles / points in the .vol file")
+    m = Mesh(np.array(pts), np.array(tris))
+    m.regions = np.array(regions, dtype=np.int32)
+    return m
+
+
+def read_mesh(path) -> Mesh:
+    """Dispatch on the file extension (.stl, .msh, .vol)."""
+    p = str(path).lower()
+    if p.endswith(".stl"):
+        return read_stl(path)
+    if p.endswith(".msh"):
+        return read_gmsh(path)
+    if p.endswith(".vol"):
+        return read_netgen_vol(path)
+    raise ValueError(f"unknown mesh format: {path}")
+
+
+def orient_outward(mesh: Mesh) -> Mesh:
+    """Flip every triangle of a closed, consistently oriented surface whose
+    enclosed (signed) volume comes out negative."""
+    p0, p1, p2 = (mesh.xyz[mesh.tri[:, k]] for k in range(3))
+    vol = np.einsum("ij,ij->i", p0, np.cross(p1, p2)).sum() / 6.0
+    if vol >= 0:
+        return mesh
+    return Mesh(mesh.xyz, mesh.tri[:, [1, 0, 2]])
