@@ -64,7 +64,7 @@ def TestCompression(nx=100, ny=100, eta=2.0, eps=1e-10, method="svd", seed=1):
         return (0, 0.0)
     dev = device()
     Xt, Yt = torch.from_numpy(X).to(dev), torch.from_numpy(Y).to(dev)
-    A = 1.0 / torch.cdist(Xt, Yt)
+    A = 1.0 / torch.linalg.norm(Xt[:, None, :] - Yt[None, :, :], dim=-1)  # exact differences, not |x|^2+|y|^2-2xy
     torch.cuda.synchronize()
     t0 = time.perf_counter()
     if method == "svd":
