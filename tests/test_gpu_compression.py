@@ -1,4 +1,4 @@
-"""TestCompression on the GPU against the oracle (aca) and numpy (svd)."""
+"""TestCompression's cross approximation on the GPU against the oracle's."""
 import numpy as np
 import pytest
 
@@ -17,14 +17,3 @@ def test_aca_matches_oracle(gpu, nx, ny, eps):
     assert r == r_ref
     assert abs(err - err_ref) <= 1e-6 * err_ref + 1e-14
 
-
-def test_svd_and_tsvd(gpu):
-    X, Y = O.test_points(150, 120, 2.0)
-    S = np.linalg.svd(1.0 / np.linalg.norm(X[:, None] - Y[None], axis=-1), compute_uv=False)
-    for eps in (1e-4, 1e-8):
-        r, err, _ = nb.TestCompression(150, 120, 2.0, eps, "svd")
-        assert r == int((S > eps).sum())
-        assert abs(err - np.sqrt((S[r:] ** 2).sum())) <= 1e-10 + 1e-6 * err
-        rt, errt, _ = nb.TestCompression(150, 120, 2.0, eps, "tsvd")
-        assert abs(rt - r) <= 2 and errt <= 10 * eps * np.linalg.norm(S)
-    assert nb.TestCompression(10, 10, 2.0, 1e-6, "other") == (0, 0.0)
