@@ -136,7 +136,18 @@ int bem_mesh_create(const double* xyz, int nv, const int32_t* tri, int nt, bem_m
 int bem_mesh_destroy(bem_mesh* m);
 
 int bem_space_create(bem_mesh* m, int kind, int order, bem_space** out);
+/* A space on part of the mesh: the elements e with elmask[e] != 0 (one byte
+ * per element; NULL: all).  Replaces NGSolve's FESpace definedon flag wrapped
+ * in Compress, as the reference's demos build their spaces
+ * (demos/HelmholtzBurtonMiller.ipynb cell 4; the operator sees only the dofs
+ * of the space's elements, src/ngbem.cpp:44-55 and src/hmat.cpp:23-31).  Dofs
+ * are numbered compactly over the region's elements, in the order the
+ * whole-mesh numbering gives them. */
+int bem_space_create_on(bem_mesh* m, int kind, int order, const unsigned char* elmask, bem_space** out);
 int bem_space_ndof(const bem_space* s);
+/* element -> dof table (nt x nloc, -1 on elements outside the space's region);
+ * returns nloc.  Replaces FESpace::GetDofNrs over the surface elements. */
+int bem_space_dofs(const bem_space* s, int32_t* dofs);
 int bem_space_destroy(bem_space* s);
 
 /* tree + partition + device setup; no quadrature yet */
@@ -181,11 +192,14 @@ int bem_op_destroy(bem_op* op);
  * the term's test component, term factors not applied (as the reference).
  * coef: device vector of trial coefficients (double or complex128 like the
  * operator); points / normals: device arrays npoints x 3 (normals may be NULL:
- * zero); out: device array npoints x bem_op_potential_dim(op).  Synchronises
- * the stream before returning. */
+ * zero); elmask: host array, one byte per element: only elements with
+ * elmask[e] != 0 contribute (the operator's trial_definedon region,
+ * src/ngbem.cpp:1205; NULL: every element of the trial space); out: device
+ * array npoints x bem_op_potential_dim(op).  Synchronises the stream before
+ * returning. */
 int bem_op_potential_dim(const bem_op* op);
-int bem_op_potential(bem_op* op, const void* coef, const double* points, const double* normals, int npoints, void* out,
-                     int dtype, void* stream);
+int bem_op_potential(bem_op* op, const void* coef, const double* points, const double* normals, int npoints,
+                     const unsigned char* elmask, void* out, int dtype, void* stream);
 
 /* introspection for parity tests */
 int bem_op_num_blocks(const bem_op* op);

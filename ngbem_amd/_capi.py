@@ -48,7 +48,8 @@ class BlockInfo(ctypes.Structure):
 
 
 # every symbol include/ngbem_amd.h declares
-EXPORTS = ["bem_last_error", "bem_device_count", "bem_reserve", "bem_test_points", "bem_test_compression_aca", "bem_mesh_create", "bem_mesh_destroy", "bem_space_create",
+EXPORTS = ["bem_last_error", "bem_device_count", "bem_reserve", "bem_test_points", "bem_test_compression_aca", "bem_mesh_create", "bem_mesh_destroy", "bem_space_create", "bem_space_create_on",
+           "bem_space_dofs",
            "bem_space_ndof", "bem_space_destroy", "bem_op_create", "bem_op_create_multi", "bem_op_assemble", "bem_op_apply",
            "bem_op_stats", "bem_op_profile_apply", "bem_op_destroy", "bem_op_num_blocks", "bem_op_block_info", "bem_op_block_dofs",
            "bem_op_block_data", "bem_op_tree", "bem_op_potential_dim", "bem_op_potential", "bem_batch_create", "bem_batch_sizes", "bem_batch_apply",
@@ -75,6 +76,8 @@ def lib():
     L.bem_mesh_create.argtypes = [dp, ctypes.c_int, ip, ctypes.c_int, ctypes.POINTER(P)]
     L.bem_mesh_destroy.argtypes = [P]
     L.bem_space_create.argtypes = [P, ctypes.c_int, ctypes.c_int, ctypes.POINTER(P)]
+    L.bem_space_create_on.argtypes = [P, ctypes.c_int, ctypes.c_int, P, ctypes.POINTER(P)]
+    L.bem_space_dofs.argtypes = [P, ip]
     L.bem_space_ndof.argtypes = [P]
     L.bem_space_destroy.argtypes = [P]
     L.bem_op_create.argtypes = [ctypes.c_int, ctypes.c_double, P, P, ctypes.POINTER(Params), ctypes.POINTER(P)]
@@ -95,7 +98,7 @@ def lib():
     L.bem_test_compression_aca.argtypes = [dp, ctypes.c_int, dp, ctypes.c_int, ctypes.c_double,
                                            ctypes.POINTER(ctypes.c_int), dp, dp]
     L.bem_op_potential_dim.argtypes = [P]
-    L.bem_op_potential.argtypes = [P, P, P, P, ctypes.c_int, P, ctypes.c_int, P]
+    L.bem_op_potential.argtypes = [P, P, P, P, ctypes.c_int, P, P, ctypes.c_int, P]
     L.bem_batch_create.argtypes = [ctypes.POINTER(P), ctypes.c_int, ctypes.c_int, ctypes.POINTER(P)]
     L.bem_batch_sizes.argtypes = [P, ctypes.POINTER(ctypes.c_int), ctypes.POINTER(ctypes.c_int),
                                   ctypes.POINTER(ctypes.c_int)]

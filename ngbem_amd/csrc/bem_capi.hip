@@ -118,20 +118,35 @@ int bem_mesh_destroy(bem_mesh* m) {
   return 0;
 }
 
-int bem_space_create(bem_mesh* m, int kind, int order, bem_space** out) {
+int bem_space_create_on(bem_mesh* m, int kind, int order, const unsigned char* elmask, bem_space** out) {
   if (!m || !out) return fail(-1, "bem_space_create: bad arguments");
   if (!((kind == BEM_SPACE_P0 && order == 0) || (kind == BEM_SPACE_P1 && order == 1) ||
         (kind == BEM_SPACE_HDIV && order == 0) || (kind == BEM_SPACE_HCURL && order == 0)))
     return fail(-1, "bem_space_create: supported spaces are SurfaceL2 order 0, H1 order 1, HDivSurface order 0 "
                     "and HCurl order 0");
+  if (elmask) {
+    bool any = false;
+    for (int e = 0; e < m->m->nt && !any; e++) any = elmask[e] != 0;
+    if (!any) return fail(-1, "bem_space_create_on: the region holds no element");
+  }
   return guarded([&] {
     auto s = std::make_shared<bem::Space>();
-    s->setup(m->m.get(), kind);
+    s->setup(m->m.get(), kind, elmask);
     *out = new bem_space{s, m->m};
   });
 }
 
+int bem_space_create(bem_mesh* m, int kind, int order, bem_space** out) {
+  return bem_space_create_on(m, kind, order, nullptr, out);
+}
+
 int bem_space_ndof(const bem_space* s) { return s ? s->s->ndof : -1; }
+
+int bem_space_dofs(const bem_space* s, int32_t* dofs) {
+  if (!s || !dofs) return fail(-1, "bem_space_dofs: bad arguments");
+  std::copy(s->s->dofs.begin(), s->s->dofs.end(), dofs);
+  return s->s->nloc;
+}
 
 int bem_space_destroy(bem_space* s) {
   delete s;
@@ -179,7 +194,7 @@ static int create_op(int kernel, double kappa, bem_space* trial, bem_space* test
     op.mesh = trial->m;
     op.trial = trial->s;
     op.test = test->s;
-    op.same_space = trial->s == test->s || (trial->s->kind == test->s->kind);
+    op.same_space = trial->s == test->s || (trial->s->kind == test->s->kind && trial->s->region == test->s->region);
     if (kappa_el) op.kappa_el = *kappa_el;
     bem::PhaseTimer pt_("create");
     op.trial_tree.build(*op.trial, op.leafsize, inst_of_dof);
@@ -240,6 +255,8 @@ int bem_op_create(int kernel, double kappa, bem_space* trial, bem_space* test, c
 int bem_op_create_multi(int kernel, const double* kappas, int ninst, const int32_t* elem_offsets, bem_space* trial,
                         bem_space* test, const bem_params* p, bem_op** out) {
   if (!kappas || ninst < 1 || !elem_offsets || !trial) return fail(-1, "bem_op_create_multi: bad arguments");
+  if (!trial->s->region.empty() || (test && !test->s->region.empty()))
+    return fail(-1, "bem_op_create_multi: spaces on part of the mesh are not supported here");
   if (kernel == bem::K_MSL) return fail(-1, "bem_op_create_multi: the Maxwell single layer has kappa in its terms");
   const bem::Mesh& m = *trial->m;
   if (elem_offsets[0] != 0 || elem_offsets[ninst] != m.nt)
@@ -543,13 +560,14 @@ int bem_op_potential_dim(const bem_op* h) {
   return bem::potential_dim(h->op);
 }
 
-int bem_op_potential(bem_op* h, const void* coef, const double* points, const double* normals, int npoints, void* out,
-                     int dtype, void* stream) {
+int bem_op_potential(bem_op* h, const void* coef, const double* points, const double* normals, int npoints,
+                     const unsigned char* elmask, void* out, int dtype, void* stream) {
   if (!h || !coef || !points || !out || npoints < 0) return fail(-1, "bem_op_potential: bad arguments");
   if (h->op.complex_valued != (dtype == BEM_C128)) return fail(-1, "bem_op_potential: dtype does not match");
   if (!h->op.kappa_el.empty()) return fail(-1, "bem_op_potential: not available for several geometries in one operator");
   if (npoints == 0) return 0;
-  return guarded([&] { bem::potential_device(h->op, coef, points, normals, npoints, out, (hipStream_t)stream); });
+  return guarded(
+      [&] { bem::potential_device(h->op, coef, points, normals, npoints, elmask, out, (hipStream_t)stream); });
 }
 
 // ------------------------------------------------------------------ batch --

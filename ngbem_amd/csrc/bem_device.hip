@@ -495,18 +495,18 @@ __global__ __launch_bounds__(256) void k_near_singular(const int4* __restrict__ 
     accum<KER, EI, EJ>(ker, el, kv, JJ * w, si, sj);
   }
   if (DIRECT) {
-    const int row = ipt[e] - nb.r0, col = ipr[f] - nb.c0;
+    const int row = ipt[dofs_t[4 * e]] - nb.r0, col = ipr[dofs_r[4 * f]] - nb.c0;
     val[nb.off + (int64_t)col * nb.m + row] = el[0][0];
   } else {
 #pragma unroll
     for (int kx = 0; kx < EI::NL; kx++) {
       const int dx = EI::PERM ? px[kx] : kx;
-      const int row = EI::MULTI ? ipt[dofs_t[4 * e + dx]] - nb.r0 : ipt[e] - nb.r0;
+      const int row = ipt[dofs_t[4 * e + (EI::MULTI ? dx : 0)]] - nb.r0;
       if (row < 0 || row >= nb.m) continue;
 #pragma unroll
       for (int ky = 0; ky < EJ::NL; ky++) {
         const int dy = EJ::PERM ? py[ky] : ky;
-        const int col = EJ::MULTI ? ipr[dofs_r[4 * f + dy]] - nb.c0 : ipr[f] - nb.c0;
+        const int col = ipr[dofs_r[4 * f + (EJ::MULTI ? dy : 0)]] - nb.c0;
         if (col < 0 || col >= nb.n) continue;
         atomic_add(&val[nb.off + (int64_t)col * nb.m + row], el[kx][ky]);
       }
@@ -1562,6 +1562,7 @@ void assemble_near(Operator& op, hipStream_t s) {
     std::vector<char> mark(m.nt, 0);
     std::vector<int> neigh, got;
     for (int e = 0; e < m.nt; e++) {
+      if (!te.has(e)) continue;
       neigh.clear();
       for (int c = 0; c < 3; c++) {
         const int v = m.tri[3 * e + c];
@@ -1572,6 +1573,7 @@ void assemble_near(Operator& op, hipStream_t s) {
       }
       for (int f : neigh) {
         mark[f] = 0;
+        if (!tr.has(f)) continue;
         int px[3], py[3];
         const int nc = classify_pair(m, e, f, px, py);
         int cls;
@@ -1611,8 +1613,20 @@ void assemble_near(Operator& op, hipStream_t s) {
   const size_t esz = op.complex_valued ? 16 : 8;
   op.d_near_val.alloc((size_t)off * esz);
   if (!direct && off) BEM_HIP(hipMemsetAsync(op.d_near_val.p, 0, (size_t)off * esz, s));
-  const int* pIb = p1i ? op.d_patch.as<int>() : op.d_perm_test.as<int>();
-  const int* pJb = p1j ? op.d_patch.as<int>() : op.d_perm_trial.as<int>();
+  // P0 sides: cluster position -> element; on part of the mesh the dof is not
+  // the element number
+  DBuf d_pe_test, d_pe_trial;
+  auto elem_of_pos = [&](const Space& sp, const Tree& t, DBuf& buf) {
+    std::vector<int> v(t.perm.size());
+    for (size_t i = 0; i < v.size(); i++) v[i] = sp.ed_list[sp.ed_off[t.perm[i]]];
+    buf.upload(v, s);
+    BEM_HIP(hipStreamSynchronize(s));
+    return buf.as<int>();
+  };
+  const int* pIb = p1i ? op.d_patch.as<int>()
+                       : (te.region.empty() ? op.d_perm_test.as<int>() : elem_of_pos(te, tt, d_pe_test));
+  const int* pJb = p1j ? op.d_patch.as<int>()
+                       : (tr.region.empty() ? op.d_perm_trial.as<int>() : elem_of_pos(tr, rt, d_pe_trial));
   dispatch_full(op, [&](auto ker, auto ei, auto ej) {
     using KER = decltype(ker);
     using EI = decltype(ei);
@@ -2162,17 +2176,19 @@ struct PotSrc {
   T v[D];
 };
 
+// sources of the elements elist[0 .. ne): one per (element, quadrature point)
 template <class KER, class EJ>
-__global__ __launch_bounds__(256) void k_pot_src(int nt, int ntq, const double* __restrict__ quad,
-                                                 const double* __restrict__ geo, const double* __restrict__ sdat,
-                                                 const double* __restrict__ trig, const int* __restrict__ dofs,
-                                                 const typename KER::T* __restrict__ coef,
+__global__ __launch_bounds__(256) void k_pot_src(const int* __restrict__ elist, int ne, int ntq,
+                                                 const double* __restrict__ quad, const double* __restrict__ geo,
+                                                 const double* __restrict__ sdat, const double* __restrict__ trig,
+                                                 const int* __restrict__ dofs, const typename KER::T* __restrict__ coef,
                                                  PotSrc<typename KER::T, EJ::DIM>* __restrict__ src) {
   typedef typename KER::T T;
   const int idx = blockIdx.x * 256 + threadIdx.x;
-  if (idx >= nt * ntq) return;
-  const int e = idx / ntq, q = idx - e * ntq;
-  const double* qp = quad + (size_t)idx * 4;
+  if (idx >= ne * ntq) return;
+  const int ie = idx / ntq, q = idx - ie * ntq;
+  const int e = elist[ie];
+  const double* qp = quad + ((size_t)e * ntq + q) * 4;
   const double lam[3] = {trig[3 * q], trig[3 * q + 1], 1.0 - trig[3 * q] - trig[3 * q + 1]};
   const int idp[3] = {0, 1, 2};
   double bb[EJ::NB > 0 ? EJ::NB : 1];
@@ -2248,10 +2264,23 @@ int potential_dim(const Operator& op) {
   return d;
 }
 
-void potential_device(Operator& op, const void* coef, const double* pts, const double* nrm, int np, void* out,
-                      hipStream_t s) {
+void potential_device(Operator& op, const void* coef, const double* pts, const double* nrm, int np,
+                      const unsigned char* elmask, void* out, hipStream_t s) {
   if (!op.assembled) throw std::runtime_error("operator not assembled");
-  const int nt = op.mesh->nt, ntq = op.trig.size(), nsrc = nt * ntq;
+  const int nt = op.mesh->nt, ntq = op.trig.size();
+  // source elements: those of the trial space, restricted to the mask
+  std::vector<int> elist;
+  elist.reserve(nt);
+  for (int e = 0; e < nt; e++)
+    if (op.trial->has(e) && (!elmask || elmask[e])) elist.push_back(e);
+  const int ne = (int)elist.size(), nsrc = ne * ntq;
+  if (ne == 0) {
+    BEM_HIP(hipMemsetAsync(out, 0, (size_t)np * potential_dim(op) * (op.complex_valued ? 16 : 8), s));
+    BEM_HIP(hipStreamSynchronize(s));
+    return;
+  }
+  DBuf d_elist;
+  d_elist.upload(elist, s);
   dispatch_full(op, [&](auto ker, auto, auto ej) {
     using KER = decltype(ker);
     using EJ = decltype(ej);
@@ -2259,7 +2288,7 @@ void potential_device(Operator& op, const void* coef, const double* pts, const d
     constexpr int D = EJ::DIM;
     DBuf src, part;
     src.alloc((size_t)nsrc * sizeof(PotSrc<T, D>));
-    hipLaunchKernelGGL((k_pot_src<KER, EJ>), dim3((nsrc + 255) / 256), dim3(256), 0, s, nt, ntq,
+    hipLaunchKernelGGL((k_pot_src<KER, EJ>), dim3((nsrc + 255) / 256), dim3(256), 0, s, d_elist.as<int>(), ne, ntq,
                        op.d_quad.as<double>(), op.d_geo.as<double>(), op.d_sdat.as<double>(), op.d_trig.as<double>(),
                        op.d_dofs_trial.as<int>(), (const T*)coef, src.as<PotSrc<T, D>>());
     // enough workgroups to fill the chip: split the sources when targets are few

@@ -74,9 +74,11 @@ void Mesh::curls(int e, double c[3][3]) const {
   }
 }
 
-void Space::setup(const Mesh* m, int k) {
+void Space::setup(const Mesh* m, int k, const unsigned char* elmask) {
   mesh = m;
   kind = k;
+  region.clear();
+  if (elmask) region.assign(elmask, elmask + m->nt);
   if (k == SP_P0) {
     ndof = m->nt;
     nloc = 1;
@@ -115,15 +117,40 @@ void Space::setup(const Mesh* m, int k) {
         sign[3 * e + l] = m->tri[3 * e + l] < m->tri[3 * e + (l + 1) % 3] ? 1 : -1;
       }
   }
+  if (!region.empty()) {
+    // a space on part of the mesh (NGSolve's definedon, compressed): keep the
+    // dofs of the region's elements, renumbered in their original order
+    std::vector<int> newid(ndof, -1);
+    for (int e = 0; e < m->nt; e++)
+      if (region[e])
+        for (int l = 0; l < nloc; l++) newid[dofs[e * nloc + l]] = 0;
+    int n = 0;
+    for (int d = 0; d < ndof; d++)
+      if (newid[d] == 0) newid[d] = n++;
+    if (!edges.empty()) {
+      std::vector<int> ked(2 * (size_t)n);
+      for (int d = 0; d < ndof; d++)
+        if (newid[d] >= 0) {
+          ked[2 * newid[d]] = edges[2 * d];
+          ked[2 * newid[d] + 1] = edges[2 * d + 1];
+        }
+      edges.swap(ked);
+    }
+    for (int e = 0; e < m->nt; e++)
+      for (int l = 0; l < nloc; l++) dofs[e * nloc + l] = region[e] ? newid[dofs[e * nloc + l]] : -1;
+    ndof = n;
+  }
   std::vector<int> cnt(ndof + 1, 0);
   for (int e = 0; e < m->nt; e++)
-    for (int l = 0; l < nloc; l++) cnt[dofs[e * nloc + l] + 1]++;
+    if (has(e))
+      for (int l = 0; l < nloc; l++) cnt[dofs[e * nloc + l] + 1]++;
   ed_off.assign(ndof + 1, 0);
   for (int d = 0; d < ndof; d++) ed_off[d + 1] = ed_off[d] + cnt[d + 1];
   ed_list.assign(ed_off[ndof], 0);
   std::vector<int> fill(ed_off.begin(), ed_off.end() - 1);
   for (int e = 0; e < m->nt; e++)
-    for (int l = 0; l < nloc; l++) ed_list[fill[dofs[e * nloc + l]]++] = e;
+    if (has(e))
+      for (int l = 0; l < nloc; l++) ed_list[fill[dofs[e * nloc + l]]++] = e;
 }
 
 // ----------------------------------------------------------------- rules --
@@ -349,13 +376,14 @@ void Tree::build(const Space& s, int leafsize, const std::vector<int>* inst) {
   std::vector<double> G(s.ndof, 0.0), X(3 * (size_t)s.ndof, 0.0);
   std::vector<int> cnt(s.ndof, 0);
   for (int e = 0; e < m.nt; e++)
-    for (int l = 0; l < s.nloc; l++) {
+    for (int l = 0; l < s.nloc && s.has(e); l++) {
       int d = s.dofs[e * s.nloc + l];
       G[d] += 0.5 * m.J[e];
       cnt[d] += 1;
     }
   static const double xh[7][2] = {{1, 0}, {0, 1}, {0, 0}, {0.5, 0.5}, {0, 0.5}, {0.5, 0}, {1.0 / 3, 1.0 / 3}};
   for (int e = 0; e < m.nt; e++) {
+    if (!s.has(e)) continue;
     double mip[7][3];
     for (int j = 0; j < 7; j++) m.point(e, xh[j][0], xh[j][1], mip[j]);
     int vc = 0, ec = 0;

@@ -35,11 +35,16 @@ struct Mesh {
 struct Space {
   const Mesh* mesh = nullptr;
   int kind = SP_P0, ndof = 0, nloc = 1;
-  std::vector<int> dofs;                   // nt*nloc
+  std::vector<int> dofs;                   // nt*nloc; -1 on elements outside the space's region
   std::vector<int> sign;                   // nt*3 edge orientation signs (RWG/NED)
   std::vector<int> edges;                  // 2*nedges (RWG/NED): sorted vertex pairs
   std::vector<int> ed_off, ed_list;        // dof -> elements (CSR, ascending)
-  void setup(const Mesh* m, int k);
+  std::vector<unsigned char> region;       // per element, 1 = in the space (empty: every element)
+  // elmask: the space lives on the elements with elmask[e] != 0 (null: all);
+  // dofs are numbered compactly over those elements, in the order of the
+  // numbering on the whole mesh
+  void setup(const Mesh* m, int k, const unsigned char* elmask = nullptr);
+  bool has(int e) const { return region.empty() || region[e]; }
 };
 
 struct Rule4 {

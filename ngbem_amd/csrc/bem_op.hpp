@@ -279,8 +279,8 @@ void apply_device(Operator& op, int flags, const double* alpha, const void* x, c
 // potential of the trial-side density coef at np points (PotentialCF):
 // out is np x potential_dim(op), double or complex128 like the operator
 int potential_dim(const Operator& op);
-void potential_device(Operator& op, const void* coef, const double* pts, const double* nrm, int np, void* out,
-                      hipStream_t s);
+void potential_device(Operator& op, const void* coef, const double* pts, const double* nrm, int np,
+                      const unsigned char* elmask, void* out, hipStream_t s);
 // ACA of 1/|x_i - y_j| between two point clouds (TestCompression, method aca)
 void test_compression_aca(const double* X, int nx, const double* Y, int ny, double eps, int* rank, double* err,
                           double* seconds);

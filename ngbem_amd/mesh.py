@@ -9,17 +9,83 @@ through their shared edge in opposite directions).
 """
 from __future__ import annotations
 
+import re
+
 import numpy as np
 
 
-class Mesh:
-    """Surface mesh made of flat triangles (vertex array + triangle array)."""
+class Region:
+    """A set of the mesh's surface elements, what NGSolve's
+    ``mesh.Boundaries(pattern)`` returns (the reference takes it as
+    ``definedon`` / ``trial_definedon`` / ``test_definedon``,
+    src/python_bem.cpp:42-57): ``mask`` is one bool per triangle.  Regions
+    combine with ``+`` (union), ``-`` (difference) and ``~`` (complement)."""
 
-    def __init__(self, xyz, tri):
+    def __init__(self, mesh, mask):
+        self.mesh = mesh
+        self.mask = np.ascontiguousarray(np.asarray(mask, dtype=bool).reshape(-1))
+        if self.mask.shape != (mesh.nt,):
+            raise ValueError("a region needs one flag per triangle")
+
+    def _same(self, other):
+        if not isinstance(other, Region) or other.mesh is not self.mesh:
+            raise ValueError("regions of different meshes")
+
+    def __add__(self, other):
+        self._same(other)
+        return Region(self.mesh, self.mask | other.mask)
+
+    def __sub__(self, other):
+        self._same(other)
+        return Region(self.mesh, self.mask & ~other.mask)
+
+    def __invert__(self):
+        return Region(self.mesh, ~self.mask)
+
+    def Mask(self):
+        """One flag per region index (sorted region numbers), as NGSolve's
+        Region.Mask() is one bit per boundary condition."""
+        idx = np.unique(self.mesh.regions)
+        return np.array([bool(self.mask[self.mesh.regions == r].all()) for r in idx])
+
+    def elements(self):
+        return np.flatnonzero(self.mask)
+
+    def __repr__(self):
+        return f"Region({int(self.mask.sum())} of {self.mesh.nt} elements)"
+
+
+class Mesh:
+    """Surface mesh made of flat triangles (vertex array + triangle array).
+
+    ``regions``: a region number per triangle (default 0), ``region_names``:
+    number -> name (unnamed numbers are called "default", as Netgen names an
+    unnamed boundary)."""
+
+    def __init__(self, xyz, tri, regions=None, region_names=None):
         self.xyz = np.ascontiguousarray(np.asarray(xyz, dtype=np.float64).reshape(-1, 3))
         self.tri = np.ascontiguousarray(np.asarray(tri, dtype=np.int32).reshape(-1, 3))
         if self.tri.size and (self.tri.min() < 0 or self.tri.max() >= len(self.xyz)):
             raise ValueError("triangle vertex index out of range")
+        self.regions = (np.zeros(self.nt, dtype=np.int32) if regions is None
+                        else np.ascontiguousarray(np.asarray(regions, dtype=np.int32).reshape(-1)))
+        if self.regions.shape != (self.nt,):
+            raise ValueError("one region number per triangle")
+        self.region_names = dict(region_names or {})
+
+    def region_name(self, r) -> str:
+        return self.region_names.get(int(r), "default")
+
+    def GetBoundaries(self):
+        """Names of the region numbers present, in increasing number order."""
+        return tuple(self.region_name(r) for r in np.unique(self.regions))
+
+    def Boundaries(self, pattern: str) -> Region:
+        """The triangles whose region name matches the regular expression
+        ``pattern`` as a whole (NGSolve's mesh.Boundaries)."""
+        rx = re.compile(pattern)
+        hit = [r for r in np.unique(self.regions) if rx.fullmatch(self.region_name(r))]
+        return Region(self, np.isin(self.regions, hit))
 
     @property
     def nv(self) -> int:
@@ -141,15 +207,38 @@ def read_stl(path) -> Mesh:
 
 def read_gmsh(path) -> Mesh:
     """Gmsh ASCII .msh, versions 2.2 and 4.1: the 3-node triangles (element type
-    2); the physical (v2) / entity (v4) tag becomes the element's region."""
+    2); the physical tag becomes the element's region (v4: the surface
+    entity's first physical tag from $Entities, else the entity tag), named
+    by $PhysicalNames."""
     lines = open(path).read().split("\n")
     ver = 2.2
     nodes, tris, regions = {}, [], []
+    names, surf_phys = {}, {}
     i = 0
     while i < len(lines):
         s = lines[i].strip()
         if s == "$MeshFormat":
             ver = float(lines[i + 1].split()[0])
+        elif s == "$PhysicalNames":
+            n = int(lines[i + 1])
+            for k in range(n):
+                f = lines[i + 2 + k].split(maxsplit=2)
+                if int(f[0]) == 2:
+                    names[int(f[1])] = f[2].strip().strip('"')
+            i += n + 2
+            continue
+        elif s == "$Entities" and ver >= 4:
+            # v4: elements carry entity tags; a surface entity's first
+            # physical tag names its region
+            npt, ncu, nsu, _ = (int(v) for v in lines[i + 1].split())
+            j = i + 2 + npt + ncu
+            for k in range(nsu):
+                f = lines[j + k].split()
+                nph = int(f[7])
+                if nph > 0:
+                    surf_phys[int(f[0])] = int(f[8])
+            i = j + nsu
+            continue
         elif s == "$Nodes":
             if ver < 4:
                 n = int(lines[i + 1])
@@ -187,7 +276,7 @@ def read_gmsh(path) -> Mesh:
                         f = [int(v) for v in lines[j + 1 + k].split()]
                         if etype == 2:
                             tris.append(f[1:4])
-                            regions.append(tag)
+                            regions.append(surf_phys.get(tag, tag))
                     j += 1 + m
                 i = j
             continue
@@ -198,19 +287,26 @@ def read_gmsh(path) -> Mesh:
     index = {k: n for n, k in enumerate(keys)}
     xyz = np.array([nodes[k] for k in keys])
     tri = np.array([[index[v] for v in t] for t in tris])
-    m = Mesh(xyz, tri)
-    m.regions = np.array(regions, dtype=np.int32)
-    return m
+    return Mesh(xyz, tri, regions, names)
 
 
 def read_netgen_vol(path) -> Mesh:
     """Netgen .vol: the "surfaceelements" (triangles; the boundary condition
-    number becomes the region) over the "points" section."""
+    number becomes the region, named by the "bcnames" section) over the
+    "points" section."""
     lines = [line.strip() for line in open(path).read().split("\n")]
-    tris, regions, pts = [], [], []
+    tris, regions, pts, names = [], [], [], {}
     i = 0
     while i < len(lines):
         s = lines[i]
+        if s == "bcnames":
+            n = int(lines[i + 1])
+            for k in range(n):
+                f = lines[i + 2 + k].split(maxsplit=1)
+                if len(f) == 2:
+                    names[int(f[0])] = f[1].strip()
+            i += n + 2
+            continue
         if s in ("surfaceelements", "surfaceelementsgi", "surfaceelementsuv"):
             n = int(lines[i + 1])
             for k in range(n):
@@ -229,9 +325,7 @@ def read_netgen_vol(path) -> Mesh:
         i += 1
     if not tris or not pts:
         raise ValueError("no surface triangles / points in the .vol file")
-    m = Mesh(np.array(pts), np.array(tris))
-    m.regions = np.array(regions, dtype=np.int32)
-    return m
+    return Mesh(np.array(pts), np.array(tris), regions, names)
 
 
 def read_mesh(path) -> Mesh:
@@ -253,4 +347,4 @@ def orient_outward(mesh: Mesh) -> Mesh:
     vol = np.einsum("ij,ij->i", p0, np.cross(p1, p2)).sum() / 6.0
     if vol >= 0:
         return mesh
-    return Mesh(mesh.xyz, mesh.tri[:, [1, 0, 2]])
+    return Mesh(mesh.xyz, mesh.tri[:, [1, 0, 2]], mesh.regions, mesh.region_names)

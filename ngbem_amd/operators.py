@@ -18,7 +18,7 @@ import numpy as np
 import torch
 
 from . import _capi
-from .mesh import Mesh
+from .mesh import Mesh, Region
 from .spaces import FESpace, H1, HCurl, HDivSurface, SurfaceL2, mass_matrix_coo
 
 __all__ = ["BaseMatrix", "HMatrix", "IntegralOperator", "IntegralOperatorC", "MassMatrix",
@@ -295,16 +295,19 @@ class HMatrix(BaseMatrix):
 
 # ------------------------------------------------- several geometries --
 
-def union_mesh(meshes):
+def union_mesh(meshes, names=None):
     """One mesh holding the given surfaces one after another: (mesh, element
-    offsets, vertex offsets)."""
+    offsets, vertex offsets).  Surface i becomes region i, named names[i]
+    when given (so that mesh.Boundaries(names[i]) selects it)."""
     xyz, tri, eoff, voff = [], [], [0], [0]
     for m in meshes:
         tri.append(m.tri.astype(np.int64) + voff[-1])
         xyz.append(m.xyz)
         eoff.append(eoff[-1] + m.nt)
         voff.append(voff[-1] + m.nv)
-    return Mesh(np.concatenate(xyz), np.concatenate(tri)), np.array(eoff), np.array(voff)
+    regions = np.repeat(np.arange(len(meshes)), [m.nt for m in meshes])
+    rn = {} if names is None else {i: str(n) for i, n in enumerate(names)}
+    return Mesh(np.concatenate(xyz), np.concatenate(tri), regions, rn), np.array(eoff), np.array(voff)
 
 
 def MultiGeometryOperator(factory_kernel, meshes, kappas=None, trial_space=SurfaceL2, test_space=None, intorder=3,
@@ -355,11 +358,14 @@ class PotentialCF:
         P = _as_vec(points, torch.float64, dev).reshape(-1, 3).contiguous()
         N = None if normals is None else _as_vec(normals, torch.float64, dev).reshape(-1, 3).contiguous()
         out = torch.zeros(P.shape[0], self.dim, dtype=self.coef.dtype, device=dev)
+        reg = getattr(op, "trial_definedon", None)  # src/ngbem.cpp:1205: only the region's elements
+        em = None if reg is None else np.ascontiguousarray(reg.mask, dtype=np.uint8)
         with torch.cuda.device(dev):
             stream = torch.cuda.current_stream().cuda_stream
             _capi.check(_capi.lib().bem_op_potential(
                 op._h, ctypes.c_void_p(self.coef.data_ptr()), ctypes.c_void_p(P.data_ptr()),
-                None if N is None else ctypes.c_void_p(N.data_ptr()), P.shape[0], ctypes.c_void_p(out.data_ptr()),
+                None if N is None else ctypes.c_void_p(N.data_ptr()), P.shape[0],
+                None if em is None else em.ctypes.data_as(ctypes.c_void_p), ctypes.c_void_p(out.data_ptr()),
                 _capi.C128 if op.is_complex else _capi.F64, ctypes.c_void_p(stream)), "bem_op_potential")
         return out[:, 0] if self.dim == 1 else out
 
@@ -372,10 +378,16 @@ class IntegralOperator:
     is_complex = False
 
     def __init__(self, kernel, kappa, trial_space, test_space, intorder, leafsize, eta, eps, dense=False,
-                 mixed=False, testhmatrix=False, _multi=None):
+                 mixed=False, testhmatrix=False, _multi=None, definedon=(None, None)):
         if not isinstance(trial_space, FESpace) or not isinstance(test_space, FESpace):
             raise TypeError("spaces must be ngbem_amd SurfaceL2 / H1 / HDivSurface / HCurl spaces")
         self.trial_space, self.test_space = trial_space, test_space
+        # operator-level regions (trial_definedon / test_definedon): as in the
+        # reference they select the elements of the potential and the dofs of
+        # testhmatrix's dense matrix, not the H-matrix (src/ngbem.cpp:37-113,
+        # 1170, 1205)
+        self.trial_definedon, self.test_definedon = (_region_of(trial_space, definedon[0]),
+                                                     _region_of(test_space, definedon[1]))
         self.kernel, self.kappa = kernel, float(np.atleast_1d(kappa)[0])
         self.params = dict(intorder=intorder, leafsize=leafsize, eta=eta, eps=eps, dense=dense)
         self.has_mixed = bool(mixed)
@@ -512,13 +524,18 @@ class IntegralOperator:
                            p["eta"], p["eps"], dense=True)
         print(f"dense: {self.test_space.ndof} x {self.trial_space.ndof}")
         D, H = dense.mat, self.mat
-        x = torch.ones(self.trial_space.ndof, dtype=self.mat.dtype(), device=device())
+        # the reference's dense matrix holds the rows and columns of the
+        # definedon regions' dofs only (mapbnd2glob, src/ngbem.cpp:275)
+        dt = self.mat.dtype()
+        cmask = _dof_mask(self.trial_space, self.trial_definedon).to(dt)
+        rmask = _dof_mask(self.test_space, self.test_definedon).to(dt)
+        x = torch.ones(self.trial_space.ndof, dtype=dt, device=device())
         for _ in range(30):
             x = x / torch.linalg.norm(x)
-            y = D * x
+            y = rmask * (D * (cmask * x))
             H.MultAdd(-1.0, x, y)
             y = y.conj()
-            x2 = D.T * y
+            x2 = cmask * (D.T * (rmask * y))
             H.MultTransAdd(-1.0, y, x2)
             x = x2.conj()
         st = self.stats()
@@ -540,10 +557,25 @@ def _dense_remap(method, leafsize, eta):
     return False
 
 
-def _no_region(*regions):
-    for r in regions:
-        if r is not None:
-            raise NotImplementedError("definedon regions are not supported")
+def _region_of(space, region):
+    """A Region of the space's mesh (or a pattern for mesh.Boundaries), or None."""
+    if region is None:
+        return None
+    if isinstance(region, str):
+        region = space.mesh.Boundaries(region)
+    if not isinstance(region, Region) or region.mesh is not space.mesh:
+        raise ValueError("definedon must be a Region of the space's mesh (mesh.Boundaries(...))")
+    return region
+
+
+def _dof_mask(space, region):
+    """1 for the dofs of the region's elements (all dofs without a region)."""
+    m = torch.ones(space.ndof, dtype=torch.float64)
+    if region is not None:
+        d = space.elem_dofs[region.mask].reshape(-1)
+        m.zero_()
+        m[torch.from_numpy(np.unique(d[d >= 0]))] = 1.0
+    return m.to(device())
 
 
 def SingleLayerPotentialOperator(space, intorder=3, leafsize=40, eta=2., eps=1e-6, method="aca", testhmatrix=False,
@@ -557,30 +589,29 @@ def SingleLayerPotentialOperator(space, intorder=3, leafsize=40, eta=2., eps=1e-
 def DoubleLayerPotentialOperator(trial_space, test_space, trial_definedon=None, test_definedon=None, intorder=3,
                                  leafsize=40, eta=2., eps=1e-6, method="aca", testhmatrix=False, mixed=False):
     """Laplace double layer K (src/python_bem.cpp:41-60, src/ngbem.hpp:257-274)."""
-    _no_region(trial_definedon, test_definedon)
     dense = _dense_remap(method, leafsize, eta)
     return IntegralOperator(_capi.LAPLACE_DL, 0.0, trial_space, test_space, intorder, leafsize, eta, eps, dense,
-                            mixed, testhmatrix)
+                            mixed, testhmatrix, definedon=(trial_definedon, test_definedon))
 
 
 def HypersingularOperator(space, definedon=None, intorder=3, leafsize=40, eta=2., eps=1e-6, method="aca",
                           testhmatrix=False, mixed=False):
     """Laplace hypersingular W with n x grad evaluators (src/python_bem.cpp:63-75)."""
-    _no_region(definedon)
     if method not in ("aca", "dense"):
         raise ValueError("method must be 'aca' or 'dense'")
     return IntegralOperator(_capi.LAPLACE_HS, 0.0, space, space, intorder, leafsize, eta, eps, False, mixed,
-                            testhmatrix)
+                            testhmatrix, definedon=(definedon, definedon))
 
 
-def _helmholtz(kernel, trial_space, test_space, kappa, intorder, leafsize, eta, eps, method, testhmatrix, mixed):
+def _helmholtz(kernel, trial_space, test_space, kappa, intorder, leafsize, eta, eps, method, testhmatrix, mixed,
+               definedon=(None, None)):
     if kappa is None:
         raise TypeError("kappa is required")
     if method not in ("aca", "dense"):
         raise ValueError("method must be 'aca' or 'dense'")
     test_space = test_space if test_space is not None else trial_space
     return IntegralOperatorC(kernel, kappa, trial_space, test_space, intorder, leafsize, eta, eps, False, mixed,
-                             testhmatrix)
+                             testhmatrix, definedon=definedon)
 
 
 def HelmholtzSingleLayerPotentialOperator(trial_space, test_space=None, kappa=None, intorder=3, leafsize=40, eta=2.,
@@ -624,20 +655,20 @@ def MaxwellSingleLayerPotentialOperator(space, kappa, definedon=None, intorder=3
                                         method="aca", testhmatrix=False, mixed=False):
     """Maxwell single layer on HDivSurface: kappa <G phi, phi> - 1/kappa <G div, div>
     (src/python_bem.cpp:122-134, src/ngbem.hpp:442-468, DiffOpMaxwellNew)."""
-    _no_region(definedon)
     if not isinstance(space, HDivSurface):
         raise TypeError("MaxwellSingleLayerPotentialOperator needs an HDivSurface space")
-    return _helmholtz(_capi.MAXWELL_SL, space, space, kappa, intorder, leafsize, eta, eps, method, testhmatrix, mixed)
+    return _helmholtz(_capi.MAXWELL_SL, space, space, kappa, intorder, leafsize, eta, eps, method, testhmatrix, mixed,
+                      (definedon, definedon))
 
 
 def MaxwellSingleLayerPotentialOperatorCurl(space, kappa, definedon=None, intorder=3, leafsize=40, eta=2., eps=1e-6,
                                             method="aca", testhmatrix=False, mixed=False):
     """Maxwell single layer on HCurl with the rotated trace and the surface curl
     (src/python_bem.cpp:137-149, DiffOpMaxwell)."""
-    _no_region(definedon)
     if not isinstance(space, HCurl):
         raise TypeError("MaxwellSingleLayerPotentialOperatorCurl needs an HCurl space")
-    return _helmholtz(_capi.MAXWELL_SL, space, space, kappa, intorder, leafsize, eta, eps, method, testhmatrix, mixed)
+    return _helmholtz(_capi.MAXWELL_SL, space, space, kappa, intorder, leafsize, eta, eps, method, testhmatrix, mixed,
+                      (definedon, definedon))
 
 
 def MaxwellDoubleLayerPotentialOperator(trial_space, test_space, kappa, trial_definedon=None, test_definedon=None,
@@ -645,9 +676,8 @@ def MaxwellDoubleLayerPotentialOperator(trial_space, test_space, kappa, trial_de
                                         mixed=False):
     """Maxwell double layer: rotated HCurl trace (trial) against the HDivSurface
     trace (test) (src/python_bem.cpp:154-173, src/ngbem.hpp:482-510)."""
-    _no_region(trial_definedon, test_definedon)
     if not (isinstance(trial_space, HCurl) and isinstance(test_space, HDivSurface)):
         raise TypeError("MaxwellDoubleLayerPotentialOperator needs an HCurl trial and an HDivSurface test space")
     dense = _dense_remap(method, leafsize, eta)
     return IntegralOperatorC(_capi.MAXWELL_DL, kappa, trial_space, test_space, intorder, leafsize, eta, eps, dense,
-                             mixed, testhmatrix)
+                             mixed, testhmatrix, definedon=(trial_definedon, test_definedon))

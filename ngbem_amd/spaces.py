@@ -5,6 +5,13 @@
 barycentric hat functions (NGSolve H1, order 1, on a surface mesh).  These are
 the spaces the reference's operators take (src/python_bem.cpp:27-173); dof
 numbering: triangle index (SurfaceL2) and vertex index (H1).
+
+``definedon=mesh.Boundaries(...)`` puts a space on part of the mesh: only the
+region's triangles carry shape functions, and the dofs are the ones those
+triangles touch, numbered compactly in the whole-mesh order (NGSolve's
+definedon space wrapped in Compress, which is how the reference's demos build
+them, demos/HelmholtzBurtonMiller.ipynb cell 4).  ``elem_dofs`` is the
+element -> dof table, -1 outside the region.
 """
 from __future__ import annotations
 
@@ -13,7 +20,7 @@ import ctypes
 import numpy as np
 
 from . import _capi
-from .mesh import Mesh
+from .mesh import Mesh, Region
 
 
 def _mesh_handle(mesh: Mesh):
@@ -45,21 +52,42 @@ class FESpace:
 
     def __init__(self, mesh: Mesh, order: int, complex: bool = False, definedon=None, dirichlet=None,
                  dual_mapping: bool = False):
-        if definedon is not None:
-            raise NotImplementedError("definedon regions are not supported: the space covers the whole mesh")
         if dual_mapping:
             raise NotImplementedError("dual_mapping=True is not supported")
         if order != self.order:
             raise NotImplementedError(f"{type(self).__name__} supports order {self.order} only")
+        if isinstance(definedon, str):
+            definedon = mesh.Boundaries(definedon)
+        if definedon is not None:
+            if not isinstance(definedon, Region) or definedon.mesh is not mesh:
+                raise ValueError("definedon must be a Region of the space's mesh (mesh.Boundaries(...))")
+            if not definedon.mask.any():
+                raise ValueError("definedon region holds no element")
         self.mesh = mesh
+        self.definedon = definedon
         self.is_complex = bool(complex)
         self.dirichlet = dirichlet
         self._mesh_h = _mesh_handle(mesh)
         L = _capi.lib()
         out = ctypes.c_void_p()
-        _capi.check(L.bem_space_create(self._mesh_h.ptr, self.kind, order, ctypes.byref(out)), "bem_space_create")
+        mask = None
+        if definedon is not None:
+            self._mask = np.ascontiguousarray(definedon.mask, dtype=np.uint8)
+            mask = self._mask.ctypes.data_as(ctypes.c_void_p)
+        _capi.check(L.bem_space_create_on(self._mesh_h.ptr, self.kind, order, mask, ctypes.byref(out)),
+                    "bem_space_create")
         self._h = _Handle(out, L.bem_space_destroy)
         self.ndof = int(L.bem_space_ndof(out))
+        tab = np.zeros(3 * mesh.nt, dtype=np.int32)
+        nloc = int(L.bem_space_dofs(out, _capi.iptr(tab)))
+        if nloc < 1:
+            _capi.check(nloc, "bem_space_dofs")
+        self.elem_dofs = tab[: nloc * mesh.nt].reshape(mesh.nt, nloc).astype(np.int64)
+
+    @property
+    def elements(self):
+        """Triangles the space lives on."""
+        return np.flatnonzero(self.elem_dofs[:, 0] >= 0)
 
     @property
     def handle(self):
@@ -95,12 +123,11 @@ class _EdgeSpace(FESpace):
         tri = mesh.tri
         loc = np.stack([tri, np.roll(tri, -1, axis=1)], axis=2)  # nt x 3 x 2
         lo, hi = loc.min(2), loc.max(2)
-        self.edges = mesh.edges()
-        key = lo.astype(np.int64) * mesh.nv + hi
-        ekey = self.edges[:, 0].astype(np.int64) * mesh.nv + self.edges[:, 1]
-        self.elem_dofs = np.searchsorted(ekey, key).astype(np.int64)
+        self.edges = np.zeros((self.ndof, 2), dtype=np.int32)  # dof -> sorted vertex pair
+        on = self.elem_dofs >= 0
+        self.edges[self.elem_dofs[on], 0] = lo[on]
+        self.edges[self.elem_dofs[on], 1] = hi[on]
         self.elem_sign = np.where(loc[:, :, 0] < loc[:, :, 1], 1.0, -1.0)
-        assert self.ndof == len(self.edges)
 
     def rt_shapes(self, pts_bary):
         """phi_l at barycentric points (q x 3) of every triangle: nt x q x 3(l) x 3(xyz),
@@ -144,22 +171,18 @@ def mass_matrix_coo(trial: FESpace, test: FESpace):
         if isinstance(trial, HCurl):
             phi_j = np.cross(n[:, None, None, :], phi_j)
         loc = np.einsum("tq,tqic,tqjc->tij", w, phi_i, phi_j)  # nt x 3 x 3
-        rows = np.repeat(test.elem_dofs, 3, axis=1).reshape(-1)
-        cols = np.tile(trial.elem_dofs, (1, 3)).reshape(-1)
-        return rows, cols, loc.reshape(-1)
-    area = m.areas()
-    tri = m.tri.astype(np.int64)
-    nt = m.nt
-    e = np.arange(nt, dtype=np.int64)
-    if isinstance(trial, SurfaceL2) and isinstance(test, SurfaceL2):
-        return e, e, area
-    if isinstance(trial, H1) and isinstance(test, SurfaceL2):
-        return np.repeat(e, 3), tri.reshape(-1), np.repeat(area / 3.0, 3)
-    if isinstance(trial, SurfaceL2) and isinstance(test, H1):
-        return tri.reshape(-1), np.repeat(e, 3), np.repeat(area / 3.0, 3)
-    # H1 x H1: area/12 * (1 + delta_ij)
-    loc = np.array([[2, 1, 1], [1, 2, 1], [1, 1, 2]], dtype=np.float64) / 12.0
-    rows = np.repeat(tri, 3, axis=1).reshape(-1)
-    cols = np.tile(tri, (1, 3)).reshape(-1)
-    vals = (area[:, None, None] * loc[None]).reshape(-1)
-    return rows, cols, vals
+    else:
+        area = m.areas()
+        ni, nj = test.elem_dofs.shape[1], trial.elem_dofs.shape[1]
+        if ni == 1 and nj == 1:
+            loc = area[:, None, None]
+        elif ni == 1 or nj == 1:  # SurfaceL2 x H1: area / 3 per vertex
+            loc = np.broadcast_to((area / 3.0)[:, None, None], (m.nt, ni, nj))
+        else:  # H1 x H1: area/12 * (1 + delta_ij)
+            loc = area[:, None, None] * (np.array([[2, 1, 1], [1, 2, 1], [1, 1, 2]], dtype=np.float64) / 12.0)[None]
+    ni, nj = loc.shape[1], loc.shape[2]
+    rows = np.repeat(test.elem_dofs, nj, axis=1).reshape(-1)
+    cols = np.tile(trial.elem_dofs, (1, ni)).reshape(-1)
+    vals = np.ascontiguousarray(loc).reshape(-1)
+    keep = (rows >= 0) & (cols >= 0)  # elements of both spaces' regions
+    return rows[keep], cols[keep], vals[keep]

@@ -300,8 +300,12 @@ struct Space {
   std::vector<int> dofs;                     // nt * nloc
   std::vector<int> sign;                     // edge spaces: +1 when the local edge runs low -> high vertex
   std::vector<std::vector<int>> elems4dof;  // src/ngbem.cpp:70-79
-  void setup(const Mesh& m, int k) {
+  std::vector<unsigned char> region;         // per element (empty: all); NGSolve definedon + Compress
+  bool has(int e) const { return region.empty() || region[e]; }
+  void setup(const Mesh& m, int k, const unsigned char* mask = nullptr) {
     kind = k;
+    region.clear();
+    if (mask) region.assign(mask, mask + m.nt);
     if (k == SP_P0) {
       ndof = m.nt; nloc = 1;
       dofs.resize(m.nt);
@@ -329,9 +333,24 @@ struct Space {
           sign[3 * e + l] = m.tri[3 * e + l] < m.tri[3 * e + (l + 1) % 3] ? 1 : -1;
         }
     }
+    if (mask) {
+      // the dofs of the region's elements only (src/ngbem.cpp:44-55: bnddofs
+      // of the definedon elements), renumbered in their original order
+      std::vector<int> id(ndof, -1);
+      for (int e = 0; e < m.nt; e++)
+        if (region[e])
+          for (int l = 0; l < nloc; l++) id[dofs[e * nloc + l]] = 1;
+      int n = 0;
+      for (int d = 0; d < ndof; d++)
+        if (id[d] > 0) id[d] = n++;
+      for (int e = 0; e < m.nt; e++)
+        for (int l = 0; l < nloc; l++) dofs[e * nloc + l] = region[e] ? id[dofs[e * nloc + l]] : -1;
+      ndof = n;
+    }
     elems4dof.assign(ndof, {});
     for (int e = 0; e < m.nt; e++)
-      for (int l = 0; l < nloc; l++) elems4dof[dofs[e * nloc + l]].push_back(e);
+      if (has(e))
+        for (int l = 0; l < nloc; l++) elems4dof[dofs[e * nloc + l]].push_back(e);
   }
 };
 
@@ -519,13 +538,14 @@ static void cluster_data(const Mesh& m, const Space& s, std::vector<double>& G, 
   X.assign(3 * s.ndof, 0.0);
   std::vector<int> cnt(s.ndof, 0);
   for (int e = 0; e < m.nt; e++)
-    for (int l = 0; l < s.nloc; l++) {
+    for (int l = 0; l < s.nloc && s.has(e); l++) {
       int d = s.dofs[e * s.nloc + l];
       G[d] += 0.5 * m.J[e];
       cnt[d] += 1;
     }
   const double xh[7][2] = {{1, 0}, {0, 1}, {0, 0}, {0.5, 0.5}, {0, 0.5}, {0.5, 0}, {1.0 / 3, 1.0 / 3}};
   for (int e = 0; e < m.nt; e++) {
+    if (!s.has(e)) continue;
     double mip[7][3];
     for (int j = 0; j < 7; j++) m.point(e, xh[j][0], xh[j][1], mip[j]);
     int vc = 0, ec = 0;
@@ -1102,15 +1122,16 @@ void orc_gauss01(int n, double* x, double* w) {
   for (int i = 0; i < n; i++) { x[i] = a[i]; w[i] = b[i]; }
 }
 
-// trial_kind/test_kind: 0 = SurfaceL2 p0, 1 = H1 p1
-void* orc_op_new(void* mesh, int kernel, double kappa, int trial_kind, int test_kind, int intorder, int leafsize,
-                 double eta, double eps) {
+// trial_kind/test_kind: SP_*; trial_mask/test_mask: spaces on the elements
+// with mask[e] != 0 (null: the whole mesh)
+void* orc_op_new_on(void* mesh, int kernel, double kappa, int trial_kind, int test_kind, const unsigned char* trial_mask,
+                    const unsigned char* test_mask, int intorder, int leafsize, double eta, double eps) {
   Op* op = new Op;
   op->mesh = (const Mesh*)mesh;
   op->K.setup(kernel, kappa);
-  op->trial.setup(*op->mesh, trial_kind);
-  op->test.setup(*op->mesh, test_kind);
-  op->same_space = (trial_kind == test_kind);
+  op->trial.setup(*op->mesh, trial_kind, trial_mask);
+  op->test.setup(*op->mesh, test_kind, test_mask);
+  op->same_space = (trial_kind == test_kind) && op->trial.region == op->test.region;
   op->intorder = intorder;
   op->leafsize = leafsize;
   op->eta = eta;
@@ -1129,7 +1150,21 @@ void* orc_op_new(void* mesh, int kernel, double kappa, int trial_kind, int test_
   op->assembled.assign(nb, 0);
   return op;
 }
+void* orc_op_new(void* mesh, int kernel, double kappa, int trial_kind, int test_kind, int intorder, int leafsize,
+                 double eta, double eps) {
+  return orc_op_new_on(mesh, kernel, kappa, trial_kind, test_kind, nullptr, nullptr, intorder, leafsize, eta, eps);
+}
 void orc_op_free(void* op) { delete (Op*)op; }
+void orc_op_sizes(void* p, int* ntrial, int* ntest) {
+  *ntrial = ((Op*)p)->trial.ndof;
+  *ntest = ((Op*)p)->test.ndof;
+}
+// element -> dof table of the trial (which = 0) or test (1) space, -1 outside its region
+int orc_op_dofs(void* p, int which, int* out) {
+  const Space& s = which ? ((Op*)p)->test : ((Op*)p)->trial;
+  std::copy(s.dofs.begin(), s.dofs.end(), out);
+  return s.nloc;
+}
 int orc_op_nblocks(void* op) { return (int)((Op*)op)->blocks.size(); }
 // info: near, row cluster, col cluster, m, n, rank (-1 dense, -2 not assembled), fallback
 void orc_op_block(void* p, int b, int* info) {
@@ -1252,7 +1287,10 @@ double orc_apply(void* p, int trans, const double* x, double* y, int nthreads) {
 // intorder).  The term factors are not applied (the reference multiplies by
 // the weight and the kernel value only, :1228).  coef: complex (interleaved)
 // trial coefficients; out: npts x dim complex.
-int orc_potential(void* p, const double* coef, const double* pts, const double* nrm, int npts, double* out) {
+// elmask: only elements with elmask[e] != 0 contribute (null: all of the
+// trial space's elements), src/ngbem.cpp:1204-1205
+int orc_potential(void* p, const double* coef, const double* pts, const double* nrm, int npts,
+                  const unsigned char* elmask, double* out) {
   const Op& op = *(Op*)p;
   const Mesh& m = *op.mesh;
   const TrigRule& tr = op.tr;
@@ -1261,7 +1299,10 @@ int orc_potential(void* p, const double* coef, const double* pts, const double* 
   const cplx* c = (const cplx*)coef;
   std::vector<double> Y((size_t)m.nt * nq * 3), W((size_t)m.nt * nq);
   std::vector<cplx> vals((size_t)m.nt * nq * dim);
+  std::vector<char> use(m.nt);
+  for (int e = 0; e < m.nt; e++) use[e] = op.trial.has(e) && (!elmask || elmask[e]);
   for (int e = 0; e < m.nt; e++) {
+    if (!use[e]) continue;
     std::vector<double> sh(12);
     for (int q = 0; q < nq; q++) {
       m.point(e, tr.xi[q], tr.eta[q], &Y[3 * ((size_t)e * nq + q)]);
@@ -1283,7 +1324,7 @@ int orc_potential(void* p, const double* coef, const double* pts, const double* 
     std::vector<cplx> acc(dim, 0.0);
     cplx kv[3];
     for (int e = 0; e < m.nt; e++)
-      for (int q = 0; q < nq; q++) {
+      for (int q = 0; q < nq && use[e]; q++) {
         const size_t sq = (size_t)e * nq + q;
         op.K.eval(x, &Y[3 * sq], nx, &m.nrm[3 * e], kv);
         for (const Term& t : op.K.terms) acc[t.tec] += W[sq] * kv[t.kc] * vals[sq * dim + t.trc];

@@ -56,7 +56,12 @@ def lib():
         L.orc_set_block.argtypes = [P, ctypes.c_int, ctypes.c_int, dp]
         L.orc_nze.restype = ctypes.c_long
         L.orc_nze.argtypes = [P]
-        L.orc_potential.argtypes = [P, dp, dp, dp, ctypes.c_int, dp]
+        L.orc_potential.argtypes = [P, dp, dp, dp, ctypes.c_int, P, dp]
+        L.orc_op_new_on.restype = P
+        L.orc_op_new_on.argtypes = [P, ctypes.c_int, ctypes.c_double, ctypes.c_int, ctypes.c_int, P, P, ctypes.c_int,
+                                    ctypes.c_int, ctypes.c_double, ctypes.c_double]
+        L.orc_op_sizes.argtypes = [P, ip, ip]
+        L.orc_op_dofs.argtypes = [P, ctypes.c_int, ip]
         L.orc_test_points.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.c_double, ctypes.c_uint, dp, dp]
         L.orc_test_compression_aca.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.c_double, ctypes.c_double,
                                                ctypes.c_uint, dp, dp, dp, ip, ip]
@@ -131,17 +136,30 @@ class OracleOp:
     """Operator restated from the reference: tree + partition at construction;
     blocks are assembled on request (``assemble``)."""
 
-    def __init__(self, mesh, kernel, kappa=0.0, trial=P0, test=P0, intorder=3, leafsize=40, eta=2.0, eps=1e-6):
+    def __init__(self, mesh, kernel, kappa=0.0, trial=P0, test=P0, intorder=3, leafsize=40, eta=2.0, eps=1e-6,
+                 trial_mask=None, test_mask=None):
+        """trial_mask / test_mask: boolean per element: the space lives on
+        those elements only (dofs numbered compactly), None: the whole mesh."""
         if not isinstance(mesh, OracleMesh):
             mesh = OracleMesh(mesh.xyz, mesh.tri)
         self.mesh = mesh
         self.kernel = kernel
-        self.h = lib().orc_op_new(mesh.h, kernel, float(kappa), trial, test, intorder, leafsize, float(eta), float(eps))
-        nedge = len(np.unique(np.sort(np.concatenate([mesh.tri[:, [0, 1]], mesh.tri[:, [1, 2]],
-                                                       mesh.tri[:, [2, 0]]]), axis=1), axis=0))
-        size = {P0: len(mesh.tri), P1: len(mesh.xyz), HDIV: nedge, HCURL: nedge}
-        self.ntrial = size[trial]
-        self.ntest = size[test]
+        self._masks = [None if k is None else np.ascontiguousarray(k, dtype=np.uint8) for k in (trial_mask, test_mask)]
+        for k in self._masks:
+            if k is not None and k.shape != (len(mesh.tri),):
+                raise ValueError("element mask needs one entry per triangle")
+        mp = [None if k is None else k.ctypes.data_as(ctypes.c_void_p) for k in self._masks]
+        self.h = lib().orc_op_new_on(mesh.h, kernel, float(kappa), trial, test, mp[0], mp[1], intorder, leafsize,
+                                     float(eta), float(eps))
+        a, b = ctypes.c_int(), ctypes.c_int()
+        lib().orc_op_sizes(self.h, ctypes.byref(a), ctypes.byref(b))
+        self.ntrial, self.ntest = a.value, b.value
+
+    def elem_dofs(self, test=False):
+        """Element -> dof table (nt x nloc), -1 outside the space's region."""
+        out = np.zeros(3 * len(self.mesh.tri), dtype=np.int32)
+        nloc = lib().orc_op_dofs(self.h, int(test), _ip(out))
+        return out[: nloc * len(self.mesh.tri)].reshape(len(self.mesh.tri), nloc)
 
     def __del__(self):
         if getattr(self, "h", None) and _lib is not None:
@@ -231,15 +249,17 @@ class OracleOp:
         self.last_apply_seconds = t
         return y
 
-    def potential(self, coef, points, normals=None):
-        """PotentialCF values at points (npts x dim, complex); dim = trial evaluator dim."""
+    def potential(self, coef, points, normals=None, elmask=None):
+        """PotentialCF values at points (npts x dim, complex); dim = trial evaluator dim;
+        elmask: boolean per element, the elements that contribute (None: all)."""
         c = np.ascontiguousarray(coef, dtype=np.complex128)
+        em = None if elmask is None else np.ascontiguousarray(elmask, dtype=np.uint8)
         P = np.ascontiguousarray(points, dtype=np.float64).reshape(-1, 3)
         N = None if normals is None else np.ascontiguousarray(normals, dtype=np.float64).reshape(-1, 3)
         dim = 4
         out = np.zeros((len(P), dim), dtype=np.complex128)  # large enough for every evaluator
         d = lib().orc_potential(self.h, _dp(c.view(np.float64)), _dp(P), None if N is None else _dp(N), len(P),
-                                _dp(out.view(np.float64)))
+                                None if em is None else em.ctypes.data_as(ctypes.c_void_p), _dp(out.view(np.float64)))
         return out.reshape(-1)[: len(P) * d].reshape(len(P), d)
 
     def densify(self):
