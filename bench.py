@@ -205,6 +205,10 @@ def main():
     gbs = {"mixed": {k: kb32[k] / (kms["mixed"][k] * 1e-3) / 1e9 for k in kb32},
            "fp64": {k: kb64[k] / (kms["fp64"][k] * 1e-3) / 1e9 for k in kb64}}
     dom = max(kb32, key=lambda k: kms["mixed"][k])  # dominant kernel of the measured (mixed) apply
+    # the row tiles run as k_mv_rows_flat (one column list per tile) unless
+    # the z plan or NGBEM_MV_ROWS=segments selects the segment loop
+    flat = not zplan and os.environ.get("NGBEM_MV_ROWS", "flat") != "segments"
+    dom_name = "k_mv_rows_flat" if dom == "k_mv_rows" and flat else dom
     log("mixed: " + "  ".join(f"{k} {kms['mixed'][k]:.3f} ms {gbs['mixed'][k]:.0f} GB/s" for k in kb32))
     log("fp64:  " + "  ".join(f"{k} {kms['fp64'][k]:.3f} ms {gbs['fp64'][k]:.0f} GB/s" for k in kb64))
 
@@ -236,7 +240,7 @@ def main():
         "near_field_evals_per_s": st["near_kernel_evals"] / st["t_near"] if st["t_near"] > 0 else None,
         "kernel_ms": kms,
         "kernel_gbs": gbs,
-        "roofline": {"kernel": f"{dom}<cx<float>,cx<float>>", "bound": "hbm", "achieved": gbs["mixed"][dom],
+        "roofline": {"kernel": f"{dom_name}<cx<float>,cx<float>>", "bound": "hbm", "achieved": gbs["mixed"][dom],
                      "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": gbs["mixed"][dom] / HBM_PEAK_GBS, "traffic": None,
                      "algorithmic_bytes_per_launch": kb32[dom],
                      "fp64_achieved": gbs["fp64"][dom], "fp64_frac": gbs["fp64"][dom] / HBM_PEAK_GBS,
@@ -289,7 +293,7 @@ def main():
     if args.traffic and os.path.exists(args.traffic) and args.level == 6 and kappa == 10.0:
         for line in open(args.traffic):  # "bytes_per_launch <kernel>: <bytes>"
             key, _, val = line.partition(":")
-            if key.strip() == f"bytes_per_launch {dom}":
+            if key.strip() == f"bytes_per_launch {dom_name}":
                 out["roofline"]["traffic"] = float(val)
                 out["roofline"]["traffic_source"] = os.path.relpath(args.traffic, ROOT)
     if args.calib_copy:

@@ -934,59 +934,67 @@ __global__ void k_scatter(const int* __restrict__ perm, int n, const VT* __restr
 
 // t_b = V_b x[c0 : c0+n] (forward)  or  t_b = Ut_b x[r0 : r0+m] (transpose).
 // Work items are waves: a long row (>= 256 entries) gets a whole wave, short
-// rows go four to a wave with 16 lanes each (a 16-wide slice of a row is one
-// cache line); item = (far block, first rank row, rows, lanes per row).
+// rows go 64 / lanes to a wave (lanes per row: 16 by default); an item holds
+// all a wave needs (first row's element offset in the factor array, row
+// length, x offset, t index of the first row, rows, lanes), so one load
+// precedes the data loads.  Each lane keeps eight loads in flight, the last
+// ones predicated off past the row end.
 struct RowItem {
-  int blk, l0, nrows, lanes;
+  int64_t off;
+  int len, xoff, toff, nrows, lanes, pad;
 };
 
-template <class ST, class VT, bool TRANS, bool CONJ>
+// Workgroup b of a grid of G runs on XCD b % 8 (round robin); this maps it to
+// the logical index that gives every XCD one contiguous range of G / 8 work
+// units, so neighbouring row tiles (adjacent leaves, whose U slices share
+// cache lines) use one L2.  Identity when remap == 0.  Only for work units of
+// similar cost: contiguous ranges of uneven units unbalance the XCDs (the
+// low-rank row items run 25% slower with it).
+__device__ __forceinline__ int xcd_major(int b, int G, int remap) {
+  if (!remap) return b;
+  const int q = G >> 3, rem = G & 7, x = b & 7, k = b >> 3;
+  return x * q + min(x, rem) + k;
+}
+
+template <class ST, class VT, bool CONJ>
 __global__ __launch_bounds__(256) void k_mv_far_t(const RowItem* __restrict__ items, int nitems,
-                                                  const FarBlk* __restrict__ fb, const ST* __restrict__ fUt,
-                                                  const ST* __restrict__ fV, const VT* __restrict__ xp,
+                                                  const ST* __restrict__ F, const VT* __restrict__ xp,
                                                   VT* __restrict__ tbuf) {
   const int w = blockIdx.x * 4 + (threadIdx.x >> 6);
   if (w >= nitems) return;
   const RowItem it = items[w];
-  const FarBlk b = fb[it.blk];
   const int lane = threadIdx.x & 63;
-  const int lanes = it.lanes;  // 64 or 16
+  const int lanes = it.lanes;
   const int g = lane / lanes, li = lane - g * lanes;
-  const ST* F = TRANS ? fUt + b.uoff : fV + b.voff;
-  const int len = TRANS ? b.m : b.n;
-  const VT* xx = xp + (TRANS ? b.r0 : b.c0);
   const bool act = g < it.nrows;
-  const int l = it.l0 + (act ? g : 0);
-  const ST* row = F + (int64_t)l * len;
-  VT a0 = zero_<VT>(), a1 = zero_<VT>(), a2 = zero_<VT>(), a3 = zero_<VT>();
-  int j = li;
-  const int st = lanes;
-  for (; j + 3 * st < len; j += 4 * st) {
-    VT u0, u1, u2, u3;
-    cvt(u0, row[j]);
-    cvt(u1, row[j + st]);
-    cvt(u2, row[j + 2 * st]);
-    cvt(u3, row[j + 3 * st]);
-    if (CONJ) { u0 = cconj(u0); u1 = cconj(u1); u2 = cconj(u2); u3 = cconj(u3); }
-    a0 += u0 * xx[j];
-    a1 += u1 * xx[j + st];
-    a2 += u2 * xx[j + 2 * st];
-    a3 += u3 * xx[j + 3 * st];
-  }
-  for (; j < len; j += st) {
-    VT u;
-    cvt(u, row[j]);
-    if (CONJ) u = cconj(u);
-    a0 += u * xx[j];
-  }
-  a0 = (a0 + a1) + (a2 + a3);
-  if (lanes == 64) {
-    a0 = wave_sum(a0);
-  } else {
+  const int len = it.len;
+  const ST* row = F + it.off + (int64_t)(act ? g : 0) * len;
+  const VT* xx = xp + it.xoff;
+  VT a[8];
 #pragma unroll
-    for (int m = 8; m >= 1; m >>= 1) a0 += shfl_xor(a0, m);
+  for (int k = 0; k < 8; k++) a[k] = zero_<VT>();
+  for (int j0 = li; j0 < len; j0 += 8 * lanes) {
+    VT u[8], xv[8];
+#pragma unroll
+    for (int k = 0; k < 8; k++) {
+      const int j = j0 + k * lanes;
+      u[k] = zero_<VT>();
+      xv[k] = zero_<VT>();
+      if (j < len) {
+        cvt(u[k], row[j]);
+        xv[k] = xx[j];
+      }
+    }
+#pragma unroll
+    for (int k = 0; k < 8; k++) a[k] += (CONJ ? cconj(u[k]) : u[k]) * xv[k];
   }
-  if (li == 0 && act) tbuf[b.toff + l] = a0;
+  VT r = ((a[0] + a[1]) + (a[2] + a[3])) + ((a[4] + a[5]) + (a[6] + a[7]));
+  if (lanes == 64) {
+    r = wave_sum(r);
+  } else {
+    for (int m = lanes >> 1; m >= 1; m >>= 1) r += shfl_xor(r, m);
+  }
+  if (li == 0 && act) tbuf[it.toff + g] = r;
 }
 
 // z_b = U_b t_b (forward) or V_b^T t_b (transpose, conjugated for the
@@ -1094,6 +1102,66 @@ __global__ __launch_bounds__(256) void k_mv_rows(const Tile* __restrict__ tiles,
     }
   }
   red[wv][lane] = (acc0 + acc1) + (acc2 + acc3);
+  __syncthreads();
+  if (wv == 0 && act) yp[tl.p0 + lane] = red[0][lane] + red[1][lane] + red[2][lane] + red[3][lane];
+}
+
+// forward, flat form of the same plan: the tile's dense columns and U
+// columns (Ut rows) in one list of (element offset, x or t index | COL_FAR);
+// wave w takes columns 8w + 32k .. +8 of the list, so every wave streams
+// eight 64-row column loads per step whatever the block sizes are; summed
+// through LDS once at the end.  Column metadata is wave-uniform (scalar loads).
+constexpr int COL_FAR = 1 << 30;
+
+template <class ST, class VT>
+__device__ __forceinline__ void col_step(const int2 c, const ST* __restrict__ nval, const ST* __restrict__ fUt,
+                                         const VT* __restrict__ xp, const VT* __restrict__ tbuf, int r, VT& u,
+                                         VT& xv) {
+  const bool far = (c.y & COL_FAR) != 0;
+  cvt(u, (far ? fUt : nval)[c.x + r]);
+  xv = (far ? tbuf : xp)[c.y & (COL_FAR - 1)];
+}
+
+template <class ST, class VT>
+__global__ __launch_bounds__(256) void k_mv_rows_flat(const Tile* __restrict__ tiles, const int2* __restrict__ frange,
+                                                      const int2* __restrict__ fcols, const ST* __restrict__ nval,
+                                                      const ST* __restrict__ fUt, const VT* __restrict__ xp,
+                                                      const VT* __restrict__ tbuf, VT* __restrict__ yp, int remap) {
+  __shared__ VT red[4][64];
+  const int tb = xcd_major(blockIdx.x, gridDim.x, remap);
+  const Tile tl = tiles[tb];
+  const int2 rg = frange[tb];
+  const int lane = threadIdx.x & 63;
+  const int wv = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+  const bool act = lane < tl.len;
+  const int r = (act ? lane : 0) + tl.roff;
+  VT a[8];
+#pragma unroll
+  for (int i = 0; i < 8; i++) a[i] = zero_<VT>();
+  int c = rg.x + 8 * wv;
+  int2 m[8];
+  if (c + 8 <= rg.y) {
+#pragma unroll
+    for (int i = 0; i < 8; i++) m[i] = fcols[c + i];
+  }
+  for (; c + 8 <= rg.y; c += 32) {
+    VT u[8], xv[8];
+#pragma unroll
+    for (int i = 0; i < 8; i++) col_step(m[i], nval, fUt, xp, tbuf, r, u[i], xv[i]);
+    // next group's column list, fetched while this group's loads are in flight
+    if (c + 40 <= rg.y) {
+#pragma unroll
+      for (int i = 0; i < 8; i++) m[i] = fcols[c + 32 + i];
+    }
+#pragma unroll
+    for (int i = 0; i < 8; i++) a[i] += u[i] * xv[i];
+  }
+  for (int i = 0; c + i < rg.y && i < 8; i++) {  // the wave's last, partial group
+    VT u, xv;
+    col_step(fcols[c + i], nval, fUt, xp, tbuf, r, u, xv);
+    a[0] += u * xv;
+  }
+  red[wv][lane] = ((a[0] + a[1]) + (a[2] + a[3])) + ((a[4] + a[5]) + (a[6] + a[7]));
   __syncthreads();
   if (wv == 0 && act) yp[tl.p0 + lane] = red[0][lane] + red[1][lane] + red[2][lane] + red[3][lane];
 }
@@ -2066,17 +2134,31 @@ static void plan(const Operator& op, bool trans, std::vector<Tile>& tiles, std::
   }
 }
 
-// low-rank B^T x work items of every far block (forward and transpose)
+// low-rank B^T x work items of every far block (forward: rows of V times
+// x[c0 ..], transpose: rows of Ut times x[r0 ..])
+static int far_lanes() {
+  static const int l = [] {
+    const char* e = std::getenv("NGBEM_FAR_LANES");
+    const int v = e ? std::atoi(e) : 16;
+    return (v == 4 || v == 8 || v == 16 || v == 32) ? v : 16;
+  }();
+  return l;
+}
+
 static void upload_row_items(Operator& op, hipStream_t s) {
+  const int L = far_lanes(), per = 64 / L;
   for (int trans = 0; trans < 2; trans++) {
     std::vector<RowItem> items;
     for (size_t k = 0; k < op.far.size(); k++) {
       const FarBlk& fb = op.far[k];
       const int len = trans ? fb.m : fb.n;
+      const int64_t base = trans ? fb.uoff : fb.voff;
+      const int xoff = trans ? fb.r0 : fb.c0;
       if (len >= 256)
-        for (int l = 0; l < fb.R; l++) items.push_back(RowItem{(int)k, l, 1, 64});
+        for (int l = 0; l < fb.R; l++) items.push_back(RowItem{base + (int64_t)l * len, len, xoff, fb.toff + l, 1, 64, 0});
       else
-        for (int l = 0; l < fb.R; l += 4) items.push_back(RowItem{(int)k, l, std::min(4, fb.R - l), 16});
+        for (int l = 0; l < fb.R; l += per)
+          items.push_back(RowItem{base + (int64_t)l * len, len, xoff, fb.toff + l, std::min(per, fb.R - l), L, 0});
     }
     (trans ? op.d_items_trn : op.d_items_fwd).upload(items, s);
     (trans ? op.n_items_trn : op.n_items_fwd) = (int)items.size();
@@ -2106,6 +2188,41 @@ static void upload_z_items(Operator& op, hipStream_t s) {
   }
 }
 
+// flat column lists of the forward Ut plan (k_mv_rows_flat); tiles of one
+// leaf share the leaf's list.  Left empty when an offset needs more than 31 bits.
+static void upload_flat_columns(Operator& op, const std::vector<Tile>& tiles, const std::vector<Seg>& segs,
+                                hipStream_t s) {
+  std::vector<int2> cols, range(tiles.size());
+  int ls0 = -1, ls1 = -1, c0 = 0;
+  for (size_t t = 0; t < tiles.size(); t++) {
+    const Tile& tl = tiles[t];
+    if (tl.s0 != ls0 || tl.s1 != ls1) {
+      c0 = (int)cols.size();
+      for (int k = tl.s0; k < tl.s1; k++) {
+        const Seg& sg = segs[k];
+        if (sg.far == 2) throw std::runtime_error("flat column plan: z segment in the Ut plan");
+        for (int j = 0; j < sg.cnt; j++) {
+          const int64_t off = sg.base + (int64_t)j * sg.ld;
+          const int64_t xi = (int64_t)sg.xoff + j;
+          if (off + sg.ld > (int64_t)INT32_MAX || xi >= COL_FAR) {
+            op.d_fcols.release();
+            op.d_frange.release();
+            return;
+          }
+          cols.push_back(make_int2((int)off, (int)xi | (sg.far ? COL_FAR : 0)));
+        }
+      }
+      ls0 = tl.s0;
+      ls1 = tl.s1;
+    }
+    range[t] = make_int2(c0, (int)cols.size());
+  }
+  if (cols.empty()) cols.push_back(make_int2(0, 0));
+  op.d_fcols.upload(cols, s);
+  op.d_frange.upload(range, s);
+  BEM_HIP(hipStreamSynchronize(s));
+}
+
 void build_matvec_plan(Operator& op, hipStream_t s) {
   upload_z_items(op, s);
   std::vector<Tile> tiles;
@@ -2113,6 +2230,7 @@ void build_matvec_plan(Operator& op, hipStream_t s) {
   plan(op, false, tiles, segs, PLAN_UT);
   op.d_seg_fwd_mm.upload(segs, s);
   BEM_HIP(hipStreamSynchronize(s));
+  upload_flat_columns(op, tiles, segs, s);
   plan(op, false, tiles, segs);
   op.d_tile_fwd.upload(tiles, s);
   op.d_seg_fwd.upload(segs, s);
@@ -2618,6 +2736,26 @@ static int mv_plan() {
   return kind;
 }
 
+// XCD-major workgroup order of the forward row tiles (default);
+// NGBEM_XCD_REMAP=0 keeps the launch order
+static int xcd_remap() {
+  static const int r = [] {
+    const char* e = std::getenv("NGBEM_XCD_REMAP");
+    return (e && std::string(e) == "0") ? 0 : 1;
+  }();
+  return r;
+}
+
+// row tiles of the forward apply from the flat column lists (default) or,
+// with NGBEM_MV_ROWS=segments, segment by segment
+static bool rows_flat() {
+  static const bool flat = [] {
+    const char* e = std::getenv("NGBEM_MV_ROWS");
+    return !(e && std::string(e) == "segments");
+  }();
+  return flat;
+}
+
 template <class ST, class VT, class XT>
 static void apply_t(Operator& op, int flags, const XT* x, XT* y, XT alpha, XT beta, bool beta_zero, const ST* nval,
                     const ST* fU, const ST* fV, hipStream_t s, hipEvent_t* ev) {
@@ -2643,8 +2781,8 @@ static void apply_t(Operator& op, int flags, const XT* x, XT* y, XT alpha, XT be
     const int pk = mv_plan();
     const bool zpath = pk == PLAN_Z;
     if (nfar) {
-      hipLaunchKernelGGL((k_mv_far_t<ST, VT, false, false>), dim3((op.n_items_fwd + 3) / 4), dim3(256), 0, s,
-                         op.d_items_fwd.as<RowItem>(), op.n_items_fwd, op.d_far_blk.as<FarBlk>(), fU, fV, xp, tb);
+      hipLaunchKernelGGL((k_mv_far_t<ST, VT, false>), dim3((op.n_items_fwd + 3) / 4), dim3(256), 0, s,
+                         op.d_items_fwd.as<RowItem>(), op.n_items_fwd, fV, xp, tb);
       mark(2);
       if (zpath)
         hipLaunchKernelGGL((k_mv_far_z<ST, VT, false, false>), dim3((op.n_zitems_fwd + 3) / 4), dim3(256), 0, s,
@@ -2654,16 +2792,20 @@ static void apply_t(Operator& op, int flags, const XT* x, XT* y, XT alpha, XT be
     }
     mark(3);
     const DBuf& segs = pk == PLAN_Z ? op.d_seg_fwd : op.d_seg_fwd_mm;
-    hipLaunchKernelGGL((k_mv_rows<ST, VT>), dim3(op.n_tiles_fwd), dim3(256), 0, s, op.d_tile_fwd.as<Tile>(),
-                       segs.as<Seg>(), nval, fU, z, xp, tb, yp);
+    if (!zpath && op.d_frange.p && rows_flat())
+      hipLaunchKernelGGL((k_mv_rows_flat<ST, VT>), dim3(op.n_tiles_fwd), dim3(256), 0, s, op.d_tile_fwd.as<Tile>(),
+                         op.d_frange.as<int2>(), op.d_fcols.as<int2>(), nval, fU, xp, tb, yp, xcd_remap());
+    else
+      hipLaunchKernelGGL((k_mv_rows<ST, VT>), dim3(op.n_tiles_fwd), dim3(256), 0, s, op.d_tile_fwd.as<Tile>(),
+                         segs.as<Seg>(), nval, fU, z, xp, tb, yp);
   } else {
     if (nfar) {
       if (conj)
-        hipLaunchKernelGGL((k_mv_far_t<ST, VT, true, true>), dim3((op.n_items_trn + 3) / 4), dim3(256), 0, s,
-                           op.d_items_trn.as<RowItem>(), op.n_items_trn, op.d_far_blk.as<FarBlk>(), fU, fV, xp, tb);
+        hipLaunchKernelGGL((k_mv_far_t<ST, VT, true>), dim3((op.n_items_trn + 3) / 4), dim3(256), 0, s,
+                           op.d_items_trn.as<RowItem>(), op.n_items_trn, fU, xp, tb);
       else
-        hipLaunchKernelGGL((k_mv_far_t<ST, VT, true, false>), dim3((op.n_items_trn + 3) / 4), dim3(256), 0, s,
-                           op.d_items_trn.as<RowItem>(), op.n_items_trn, op.d_far_blk.as<FarBlk>(), fU, fV, xp, tb);
+        hipLaunchKernelGGL((k_mv_far_t<ST, VT, false>), dim3((op.n_items_trn + 3) / 4), dim3(256), 0, s,
+                           op.d_items_trn.as<RowItem>(), op.n_items_trn, fU, xp, tb);
       mark(2);
       if (conj)
         hipLaunchKernelGGL((k_mv_far_z<ST, VT, true, true>), dim3((op.n_zitems_trn + 3) / 4), dim3(256), 0, s,

@@ -238,6 +238,10 @@ struct Operator {
   // matvec plans
   DBuf d_seg_fwd, d_tile_fwd, d_seg_trn, d_tile_trn;
   DBuf d_seg_fwd_mm;       // forward segments over Ut columns (far = 1): vector apply and block product
+  // the same forward plan as one flat column list per row tile: (element
+  // offset of the column, x / t index | COL_FAR); empty when the offsets
+  // overflow 31 bits
+  DBuf d_fcols, d_frange;
   int n_tiles_fwd = 0, n_tiles_trn = 0;
   DBuf d_zitems_fwd, d_zitems_trn;  // U t / V^T t work items
   int n_zitems_fwd = 0, n_zitems_trn = 0;

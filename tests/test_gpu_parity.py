@@ -163,6 +163,27 @@ def test_hmatrix_vs_dense(gpu):
     assert Vh.mat.nze < Vd.mat.nze
 
 
+@pytest.mark.parametrize("kind", ["LSL-P0", "LDL-P1P0", "LHS-P1"])
+def test_mixed_real_matches_fp64(gpu, kind):
+    """FP32 factor copies (4-byte elements, 16-byte chunk loads in the
+    forward low-rank product, rows starting at every alignment)."""
+    mesh = nb.icosphere(3)
+    kw = dict(intorder=3, leafsize=24, eps=1e-7, mixed=True)
+    if kind == "LSL-P0":
+        A = nb.SingleLayerPotentialOperator(nb.SurfaceL2(mesh), **kw)
+    elif kind == "LDL-P1P0":
+        A = nb.DoubleLayerPotentialOperator(nb.H1(mesh), nb.SurfaceL2(mesh), **kw)
+    else:
+        A = nb.HypersingularOperator(nb.H1(mesh), **kw)
+    assert A.stats()["n_far"] > 0
+    for trans in (False, True):
+        M = A.mat.T if trans else A.mat
+        x = torch.randn(M.width, dtype=torch.float64, device=gpu)
+        y64 = M * x
+        y32 = (A.mat.mixed.T if trans else A.mat.mixed) * x
+        assert (torch.linalg.norm(y64 - y32) / torch.linalg.norm(y64)).item() < 2e-6, trans
+
+
 def test_mixed_matches_fp64(gpu):
     mesh = nb.icosphere(3)
     fes = nb.SurfaceL2(mesh)
