@@ -120,10 +120,13 @@ int bem_mesh_destroy(bem_mesh* m) {
 
 int bem_space_create_on(bem_mesh* m, int kind, int order, const unsigned char* elmask, bem_space** out) {
   if (!m || !out) return fail(-1, "bem_space_create: bad arguments");
-  if (!((kind == BEM_SPACE_P0 && order == 0) || (kind == BEM_SPACE_P1 && order == 1) ||
+  if (!((kind == BEM_SPACE_P0 && order >= 0 && order <= 2) || (kind == BEM_SPACE_P1 && (order == 1 || order == 2)) ||
         (kind == BEM_SPACE_HDIV && order == 0) || (kind == BEM_SPACE_HCURL && order == 0)))
-    return fail(-1, "bem_space_create: supported spaces are SurfaceL2 order 0, H1 order 1, HDivSurface order 0 "
-                    "and HCurl order 0");
+    return fail(-1, "bem_space_create: supported spaces are SurfaceL2 orders 0-2, H1 orders 1-2, HDivSurface "
+                    "order 0 and HCurl order 0");
+  int ik = kind;  // internal kind
+  if (kind == BEM_SPACE_P0) ik = order == 0 ? bem::SP_P0 : (order == 1 ? bem::SP_L2_1 : bem::SP_L2_2);
+  if (kind == BEM_SPACE_P1) ik = order == 1 ? bem::SP_P1 : bem::SP_P2;
   if (elmask) {
     bool any = false;
     for (int e = 0; e < m->m->nt && !any; e++) any = elmask[e] != 0;
@@ -131,7 +134,7 @@ int bem_space_create_on(bem_mesh* m, int kind, int order, const unsigned char* e
   }
   return guarded([&] {
     auto s = std::make_shared<bem::Space>();
-    s->setup(m->m.get(), kind, elmask);
+    s->setup(m->m.get(), ik, elmask);
     *out = new bem_space{s, m->m};
   });
 }
@@ -163,13 +166,12 @@ static int create_op(int kernel, double kappa, bem_space* trial, bem_space* test
   if (p->intorder < 0 || p->intorder > 40) return fail(-1, "bem_op_create: intorder out of range [0, 40]");
   if (p->leafsize < 1) return fail(-1, "bem_op_create: leafsize must be positive");
   if (!(p->eps > 0.0)) return fail(-1, "bem_op_create: eps must be positive");
-  if ((kernel == bem::K_LHS || kernel == bem::K_HHS) &&
-      (trial->s->kind != bem::SP_P1 || test->s->kind != bem::SP_P1))
-    return fail(-1, "bem_op_create: hypersingular operators need H1 order 1 spaces");
+  if ((kernel == bem::K_LHS || kernel == bem::K_HHS) && (!bem::is_h1_kind(trial->s->kind) || !bem::is_h1_kind(test->s->kind)))
+    return fail(-1, "bem_op_create: hypersingular operators need H1 spaces");
   if (kernel >= bem::K_HSL && !(kappa > 0.0)) return fail(-1, "bem_op_create: kappa must be positive");
   {
     const int tk = trial->s->kind, sk = test->s->kind;
-    const bool scalar = tk <= bem::SP_P1 && sk <= bem::SP_P1;
+    const bool scalar = !bem::is_edge_kind(tk) && !bem::is_edge_kind(sk);
     if ((kernel <= bem::K_HHS || kernel == bem::K_HSLK) && !scalar)
       return fail(-1, "bem_op_create: Laplace/Helmholtz kernels need SurfaceL2 / H1 spaces");
     if (kernel == bem::K_MSL && !((tk == bem::SP_RWG && sk == bem::SP_RWG) || (tk == bem::SP_NED && sk == bem::SP_NED)))

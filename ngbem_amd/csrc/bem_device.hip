@@ -267,11 +267,13 @@ struct KMaxDL {  // e^{ikr}(ikr - 1)(x-y)/(4 pi r^3), cross-product terms (src/n
 // ------------------------------------------------------------- evaluators --
 // shape rows at one point, layout dof*DIM + comp (src/diffops.hpp:12-338).
 // Per-element shape data (stride SDS doubles): [n x grad lambda_k (9),
-// edge scale s_k = sign_k / J (3), vertex opposite edge k (9)].
+// edge scale s_k = sign_k / J (3), vertex opposite edge k (9), test space
+// kind, trial space kind].
 // load(): copy what the evaluator needs into registers; PERM evaluators index
-// their dofs by Sauter-Schwab vertex (px), the others by element-local dof.
+// their dofs by Sauter-Schwab vertex (px) and take barycentrics in that order,
+// the others index by element-local dof and take local barycentrics.
 // MULTI: more than one dof per element (scatter through the dof tables).
-constexpr int SDS = 21;
+constexpr int SDS = 23;
 
 template <int SP, int EV>
 struct Ev;
@@ -342,6 +344,76 @@ struct EvEdge {
     }
   }
 };
+// Higher-order scalar spaces (SurfaceL2 orders 1-2, H1 order 2, and the
+// lowest orders when paired with one of them): one evaluator whose space kind
+// is read at run time from the element data (slot 21 + SIDE; uniform across
+// the launch, so the switch does not diverge); NL = NLOC_MAX, unused rows
+// are zero.  EVK: EV_ID (values), EV_ROT (n x grad_Gamma, H1 only), EV_HELM
+// ([n x grad_Gamma, value], H1 only).
+//   SurfaceL2: Dubiner basis in x = lambda_0, y = lambda_1 (orders i + j <= p,
+//     i outer): 1 | 3y-1, 2x+y-1 | P2^(1,0)(2y-1), (2x+y-1)(5y-1),
+//     (3(2x+y-1)^2 - (1-y)^2)/2, with P2^(1,0)(t) = (5t^2 + 2t - 1)/2;
+//   H1: lambda_0..2, then lambda_l lambda_(l+1) for the edges l = 0, 1, 2.
+template <int SIDE, int EVK>
+struct EvGen {
+  static constexpr int NL = NLOC_MAX, DIM = EVK == EV_ID ? 1 : (EVK == EV_ROT ? 3 : 4), NB = 10;
+  static constexpr bool MULTI = true, PERM = false;
+  __device__ __forceinline__ static void load(const double* sd, const int*, double* b) {
+    for (int i = 0; i < 9; i++) b[i] = sd[i];
+    b[9] = sd[21 + SIDE];
+  }
+  __device__ __forceinline__ static void shape(const double* lam, P3, const double* b, double* o) {
+    const int kind = (int)b[9];
+#pragma unroll
+    for (int i = 0; i < NL * DIM; i++) o[i] = 0.0;
+    if (EVK == EV_ID && (kind == SP_P0 || kind == SP_L2_1 || kind == SP_L2_2)) {
+      const double x = lam[0], y = lam[1];
+      o[0] = 1.0;
+      if (kind == SP_P0) return;
+      const double t = 2.0 * y - 1.0, sx = 2.0 * x + y - 1.0;
+      if (kind == SP_L2_1) {
+        o[1] = 3.0 * y - 1.0;
+        o[2] = sx;
+        return;
+      }
+      const double w = 1.0 - y;
+      o[1] = 3.0 * y - 1.0;
+      o[2] = 0.5 * (5.0 * t * t + 2.0 * t - 1.0);
+      o[3] = sx;
+      o[4] = sx * (5.0 * y - 1.0);
+      o[5] = 0.5 * (3.0 * sx * sx - w * w);
+      return;
+    }
+    // H1 orders 1, 2 (vertex functions, then edge bubbles)
+    const int nl = kind == SP_P2 ? 6 : 3;
+#pragma unroll
+    for (int k = 0; k < 6; k++) {
+      if (k >= nl) break;
+      double v, g[3];
+      if (k < 3) {
+        v = lam[k];
+        g[0] = b[3 * k];
+        g[1] = b[3 * k + 1];
+        g[2] = b[3 * k + 2];
+      } else {
+        const int a = k - 3, c = (k - 2) % 3;  // edge (a, a + 1)
+        v = lam[a] * lam[c];
+        g[0] = lam[a] * b[3 * c] + lam[c] * b[3 * a];
+        g[1] = lam[a] * b[3 * c + 1] + lam[c] * b[3 * a + 1];
+        g[2] = lam[a] * b[3 * c + 2] + lam[c] * b[3 * a + 2];
+      }
+      if (EVK == EV_ID) {
+        o[k] = v;
+      } else {
+        o[DIM * k] = g[0];
+        o[DIM * k + 1] = g[1];
+        o[DIM * k + 2] = g[2];
+        if (EVK == EV_HELM) o[DIM * k + 3] = v;
+      }
+    }
+  }
+};
+
 template <> struct Ev<SP_RWG, EV_MAXW> : EvEdge<1, true> {};
 template <> struct Ev<SP_NED, EV_MAXW> : EvEdge<-1, true> {};
 template <> struct Ev<SP_RWG, EV_TRACE3> : EvEdge<1, false> {};
@@ -426,11 +498,15 @@ __global__ __launch_bounds__(256) void k_near_regular(const NearBlk* __restrict_
   } else {
 #pragma unroll
     for (int ii = 0; ii < EI::NL; ii++) {
-      const int row = EI::MULTI ? ipt[dofs_t[4 * e + ii]] - nb.r0 : ia;
+      const int di = EI::MULTI ? dofs_t[DOF_STRIDE * e + ii] : 0;
+      if (di < 0) continue;  // unused row of a generic evaluator
+      const int row = EI::MULTI ? ipt[di] - nb.r0 : ia;
       if (row < 0 || row >= nb.m) continue;
 #pragma unroll
       for (int jj = 0; jj < EJ::NL; jj++) {
-        const int col = EJ::MULTI ? ipr[dofs_r[4 * f + jj]] - nb.c0 : ja;
+        const int dj = EJ::MULTI ? dofs_r[DOF_STRIDE * f + jj] : 0;
+        if (dj < 0) continue;
+        const int col = EJ::MULTI ? ipr[dj] - nb.c0 : ja;
         if (col < 0 || col >= nb.n) continue;
         atomic_add(&val[nb.off + (int64_t)col * nb.m + row], el[ii][jj]);
       }
@@ -488,25 +564,41 @@ __global__ __launch_bounds__(256) void k_near_singular(const int4* __restrict__ 
     const P3 Y = {ly[0] * F0.x + ly[1] * F1.x + ly[2] * F2.x, ly[0] * F0.y + ly[1] * F1.y + ly[2] * F2.y,
                   ly[0] * F0.z + ly[1] * F1.z + ly[2] * F2.z};
     double si[EI::NL * EI::DIM], sj[EJ::NL * EJ::DIM];
-    EI::shape(lx, X, be, si);
-    EJ::shape(ly, Y, bf, sj);
+    if (EI::PERM) {
+      EI::shape(lx, X, be, si);
+    } else {  // element-local barycentrics
+      double l[3];
+      l[px[0]] = lx[0]; l[px[1]] = lx[1]; l[px[2]] = lx[2];
+      EI::shape(l, X, be, si);
+    }
+    if (EJ::PERM) {
+      EJ::shape(ly, Y, bf, sj);
+    } else {
+      double l[3];
+      l[py[0]] = ly[0]; l[py[1]] = ly[1]; l[py[2]] = ly[2];
+      EJ::shape(l, Y, bf, sj);
+    }
     T kv[KER::NC];
     ker.eval(X, Y, ne, nf, kv);
     accum<KER, EI, EJ>(ker, el, kv, JJ * w, si, sj);
   }
   if (DIRECT) {
-    const int row = ipt[dofs_t[4 * e]] - nb.r0, col = ipr[dofs_r[4 * f]] - nb.c0;
+    const int row = ipt[dofs_t[DOF_STRIDE * e]] - nb.r0, col = ipr[dofs_r[DOF_STRIDE * f]] - nb.c0;
     val[nb.off + (int64_t)col * nb.m + row] = el[0][0];
   } else {
 #pragma unroll
     for (int kx = 0; kx < EI::NL; kx++) {
       const int dx = EI::PERM ? px[kx] : kx;
-      const int row = ipt[dofs_t[4 * e + (EI::MULTI ? dx : 0)]] - nb.r0;
+      const int di = dofs_t[DOF_STRIDE * e + (EI::MULTI ? dx : 0)];
+      if (di < 0) continue;  // unused row of a generic evaluator
+      const int row = ipt[di] - nb.r0;
       if (row < 0 || row >= nb.m) continue;
 #pragma unroll
       for (int ky = 0; ky < EJ::NL; ky++) {
         const int dy = EJ::PERM ? py[ky] : ky;
-        const int col = ipr[dofs_r[4 * f + (EJ::MULTI ? dy : 0)]] - nb.c0;
+        const int dj = dofs_r[DOF_STRIDE * f + (EJ::MULTI ? dy : 0)];
+        if (dj < 0) continue;
+        const int col = ipr[dj] - nb.c0;
         if (col < 0 || col >= nb.n) continue;
         atomic_add(&val[nb.off + (int64_t)col * nb.m + row], el[kx][ky]);
       }
@@ -875,7 +967,7 @@ __global__ __launch_bounds__(256) void k_project(const ProjTask* __restrict__ ta
     T s = zero_<T>();
     for (int k2 = mp[pos]; k2 < mp[pos + 1]; k2++) {
       const int ent = mp[len + 1 + k2];
-      const int a = ent >> 2, kl = ent & 3, e = P[a];
+      const int a = ent / DOF_STRIDE, kl = ent % DOF_STRIDE, e = P[a];
       double bb[EI::NB > EJ::NB ? EI::NB : EJ::NB];
       if (isU) EI::load(sdat + (size_t)e * SDS, idp, bb);
       else EJ::load(sdat + (size_t)e * SDS, idp, bb);
@@ -1482,15 +1574,20 @@ static void dispatch_full(const Operator& op, F&& f) {
   const int st = op.test->kind, sr = op.trial->kind;
   dispatch_kernel(op.kernel, op.kappa, [&](auto ker) {
     using KER = decltype(ker);
+    const bool hi = is_higher_kind(st) || is_higher_kind(sr);
     if constexpr (std::is_same<KER, KLapHS>::value) {
-      f(ker, Ev<SP_P1, EV_ROT>{}, Ev<SP_P1, EV_ROT>{});
+      if (hi) f(ker, EvGen<0, EV_ROT>{}, EvGen<1, EV_ROT>{});
+      else f(ker, Ev<SP_P1, EV_ROT>{}, Ev<SP_P1, EV_ROT>{});
     } else if constexpr (std::is_same<KER, KHelHS>::value) {
-      f(ker, Ev<SP_P1, EV_HELM>{}, Ev<SP_P1, EV_HELM>{});
+      if (hi) f(ker, EvGen<0, EV_HELM>{}, EvGen<1, EV_HELM>{});
+      else f(ker, Ev<SP_P1, EV_HELM>{}, Ev<SP_P1, EV_HELM>{});
     } else if constexpr (std::is_same<KER, KMaxSL>::value) {
       if (st == SP_RWG) f(ker, Ev<SP_RWG, EV_MAXW>{}, Ev<SP_RWG, EV_MAXW>{});
       else f(ker, Ev<SP_NED, EV_MAXW>{}, Ev<SP_NED, EV_MAXW>{});
     } else if constexpr (std::is_same<KER, KMaxDL>::value) {
       f(ker, Ev<SP_RWG, EV_TRACE3>{}, Ev<SP_NED, EV_TRACE3>{});  // test: HDivSurface trace, trial: rotated HCurl trace
+    } else if (hi) {
+      f(ker, EvGen<0, EV_ID>{}, EvGen<1, EV_ID>{});
     } else {
       if (st == SP_P0 && sr == SP_P0) f(ker, Ev<SP_P0, EV_ID>{}, Ev<SP_P0, EV_ID>{});
       else if (st == SP_P0 && sr == SP_P1) f(ker, Ev<SP_P0, EV_ID>{}, Ev<SP_P1, EV_ID>{});
@@ -1530,17 +1627,20 @@ void setup_device(Operator& op, hipStream_t s) {
     for (int k = 0; k < 3; k++)
       for (int d = 0; d < 3; d++) sd[3 * k + d] = c[k][d];
     // edge spaces: s_k = sign_k / J and the vertex opposite edge k (= vertex k+2)
-    const Space& es = (op.test->kind >= SP_RWG) ? *op.test : *op.trial;
+    const Space& es = is_edge_kind(op.test->kind) ? *op.test : *op.trial;
     for (int k = 0; k < 3; k++) {
       sd[9 + k] = es.sign.empty() ? 0.0 : es.sign[3 * e + k] / m.J[e];
       const int vo = m.tri[3 * e + (k + 2) % 3];
       for (int d = 0; d < 3; d++) sd[12 + 3 * k + d] = m.xyz[3 * vo + d];
     }
+    sd[21] = op.test->kind;
+    sd[22] = op.trial->kind;
   }
+  // unused slots (l >= nloc; the generic evaluators' zero rows) hold -1
   auto dof_table = [&](const Space& sp) {
-    std::vector<int> t((size_t)nt * 4, 0);
+    std::vector<int> t((size_t)nt * DOF_STRIDE, -1);
     for (int e = 0; e < nt; e++)
-      for (int l = 0; l < sp.nloc; l++) t[4 * (size_t)e + l] = sp.dofs[(size_t)e * sp.nloc + l];
+      for (int l = 0; l < sp.nloc; l++) t[DOF_STRIDE * (size_t)e + l] = sp.dofs[(size_t)e * sp.nloc + l];
     return t;
   };
   op.d_dofs_test.upload(dof_table(*op.test), s);
@@ -2324,7 +2424,10 @@ __global__ __launch_bounds__(256) void k_pot_src(const int* __restrict__ elist, 
   for (int c = 0; c < EJ::DIM; c++) {
     T a = zero_<T>();
 #pragma unroll
-    for (int l = 0; l < EJ::NL; l++) a += rmul(coef[dofs[4 * (size_t)e + l]], v[l * EJ::DIM + c]);
+    for (int l = 0; l < EJ::NL; l++) {
+      const int d = dofs[DOF_STRIDE * (size_t)e + l];
+      if (d >= 0) a += rmul(coef[d], v[l * EJ::DIM + c]);
+    }
     o.v[c] = a;
   }
   src[idx] = o;

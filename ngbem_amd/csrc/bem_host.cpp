@@ -84,10 +84,41 @@ void Space::setup(const Mesh* m, int k, const unsigned char* elmask) {
     nloc = 1;
     dofs.resize(m->nt);
     for (int e = 0; e < m->nt; e++) dofs[e] = e;
+  } else if (k == SP_L2_1 || k == SP_L2_2) {
+    // constants first (dof e), then each element's higher functions in a block
+    nloc = k == SP_L2_1 ? 3 : 6;
+    ndof = m->nt * nloc;
+    dofs.resize((size_t)m->nt * nloc);
+    for (int e = 0; e < m->nt; e++) {
+      dofs[(size_t)e * nloc] = e;
+      for (int l = 1; l < nloc; l++) dofs[(size_t)e * nloc + l] = m->nt + e * (nloc - 1) + l - 1;
+    }
   } else if (k == SP_P1) {
     ndof = m->nv;
     nloc = 3;
     dofs.assign(m->tri.begin(), m->tri.end());
+  } else if (k == SP_P2) {
+    // vertex dofs, then one dof per edge (v_l, v_l+1), edges numbered in
+    // sorted (min, max) order after the vertices
+    nloc = 6;
+    auto key = [m](int e, int l) {
+      const int a = m->tri[3 * e + l], b = m->tri[3 * e + (l + 1) % 3];
+      return (long long)std::min(a, b) * m->nv + std::max(a, b);
+    };
+    std::vector<long long> uk;
+    uk.reserve(3 * (size_t)m->nt);
+    for (int e = 0; e < m->nt; e++)
+      for (int l = 0; l < 3; l++) uk.push_back(key(e, l));
+    std::sort(uk.begin(), uk.end());
+    uk.erase(std::unique(uk.begin(), uk.end()), uk.end());
+    ndof = m->nv + (int)uk.size();
+    dofs.resize(6 * (size_t)m->nt);
+    for (int e = 0; e < m->nt; e++)
+      for (int l = 0; l < 3; l++) {
+        dofs[6 * (size_t)e + l] = m->tri[3 * e + l];
+        dofs[6 * (size_t)e + 3 + l] =
+            m->nv + (int)(std::lower_bound(uk.begin(), uk.end(), key(e, l)) - uk.begin());
+      }
   } else {
     // edges sorted by (min vertex, max vertex); local dof k of a triangle is
     // its edge (v_k, v_{k+1}), signed +1 when v_k < v_{k+1}
@@ -624,7 +655,7 @@ int ClusterPatches::map_of(const Space& s, const Tree& t, int c) {
       int kl = 0;
       for (int l = 0; l < s.nloc; l++)
         if (s.dofs[e * s.nloc + l] == d) kl = l;
-      ents.push_back(a * 4 + kl);
+      ents.push_back(a * DOF_STRIDE + kl);
     }
   }
   maps[base + len] = (int)ents.size();

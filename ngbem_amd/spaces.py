@@ -47,15 +47,16 @@ class _Handle:
 
 
 class FESpace:
-    kind = -1
-    order = -1
+    kind = -1        # C-ABI space kind (BEM_SPACE_*)
+    orders = ()      # supported orders
 
     def __init__(self, mesh: Mesh, order: int, complex: bool = False, definedon=None, dirichlet=None,
                  dual_mapping: bool = False):
         if dual_mapping:
             raise NotImplementedError("dual_mapping=True is not supported")
-        if order != self.order:
-            raise NotImplementedError(f"{type(self).__name__} supports order {self.order} only")
+        if order not in self.orders:
+            raise NotImplementedError(f"{type(self).__name__} supports orders {list(self.orders)}")
+        self.order = int(order)
         if isinstance(definedon, str):
             definedon = mesh.Boundaries(definedon)
         if definedon is not None:
@@ -78,7 +79,7 @@ class FESpace:
                     "bem_space_create")
         self._h = _Handle(out, L.bem_space_destroy)
         self.ndof = int(L.bem_space_ndof(out))
-        tab = np.zeros(3 * mesh.nt, dtype=np.int32)
+        tab = np.zeros(6 * mesh.nt, dtype=np.int32)
         nloc = int(L.bem_space_dofs(out, _capi.iptr(tab)))
         if nloc < 1:
             _capi.check(nloc, "bem_space_dofs")
@@ -97,20 +98,64 @@ class FESpace:
         return f"{type(self).__name__}(ndof={self.ndof}, order={self.order})"
 
 
+def _jacobi(n, a, t):
+    """Jacobi polynomial P_n^(a,0)(t) (three-term recurrence)."""
+    p0, p1 = np.ones_like(t), 0.5 * ((a + 2.0) * t + a)
+    if n == 0:
+        return p0
+    for k in range(2, n + 1):
+        c = 2.0 * k + a
+        p0, p1 = p1, ((c - 1.0) * (c * (c - 2.0) * t + a * a) * p1 - 2.0 * (k + a - 1.0) * (k - 1.0) * c * p0) / \
+            (2.0 * k * (k + a) * (c - 2.0))
+    return p1
+
+
+def _dubiner(p, x, y):
+    """Dubiner basis of degree <= p in x = lambda_0, y = lambda_1 (i outer, j
+    inner): P_i((2x+y-1)/(1-y)) (1-y)^i P_j^(2i+1,0)(2y-1); rows = points."""
+    w, s = 1.0 - y, 2.0 * x + y - 1.0
+    out = []
+    for i in range(p + 1):
+        q = [np.ones_like(s), s]
+        for k in range(2, i + 1):
+            q.append(((2 * k - 1) * s * q[-1] - (k - 1) * w * w * q[-2]) / k)
+        for j in range(p - i + 1):
+            out.append(q[i] * _jacobi(j, 2.0 * i + 1.0, 2 * y - 1))
+    return np.stack(out, axis=-1)
+
+
 class SurfaceL2(FESpace):
+    """SurfaceL2, orders 0-2: constants (dof = triangle), plus for orders 1, 2
+    the triangle's higher Dubiner functions (in the element's barycentrics
+    x = lambda_0, y = lambda_1; dofs nt + e (nloc - 1) + k - 1)."""
     kind = _capi.SPACE_P0
-    order = 0
+    orders = (0, 1, 2)
 
     def __init__(self, mesh, order=0, complex=False, definedon=None, dirichlet=None, dual_mapping=False):
         super().__init__(mesh, order, complex, definedon, dirichlet, dual_mapping)
+        self.ikind = (0, 4, 5)[self.order]
+
+    def shapes(self, bary):
+        """Shape values at barycentric points (q x 3): q x nloc."""
+        return _dubiner(self.order, bary[:, 0], bary[:, 1])
 
 
 class H1(FESpace):
+    """H1, orders 1-2 on the surface: vertex hats lambda_k, plus for order 2
+    the edge bubbles lambda_l lambda_(l+1) (dofs nv + edge number, edges in
+    sorted (min, max) order)."""
     kind = _capi.SPACE_P1
-    order = 1
+    orders = (1, 2)
 
     def __init__(self, mesh, order=1, complex=False, definedon=None, dirichlet=None):
         super().__init__(mesh, order, complex, definedon, dirichlet)
+        self.ikind = (None, 1, 6)[self.order]
+
+    def shapes(self, bary):
+        v = [bary[:, 0], bary[:, 1], bary[:, 2]]
+        if self.order == 2:
+            v += [bary[:, 0] * bary[:, 1], bary[:, 1] * bary[:, 2], bary[:, 2] * bary[:, 0]]
+        return np.stack(v, axis=-1)
 
 
 class _EdgeSpace(FESpace):
@@ -118,8 +163,11 @@ class _EdgeSpace(FESpace):
     pairs); local dof l of a triangle is its edge (v_l, v_{l+1}), signed +1 when
     v_l < v_{l+1}."""
 
+    orders = (0,)
+
     def __init__(self, mesh, order=0, complex=False, definedon=None, dirichlet=None):
         super().__init__(mesh, order, complex, definedon, dirichlet)
+        self.ikind = self.kind
         tri = mesh.tri
         loc = np.stack([tri, np.roll(tri, -1, axis=1)], axis=2)  # nt x 3 x 2
         lo, hi = loc.min(2), loc.max(2)
@@ -144,13 +192,11 @@ class _EdgeSpace(FESpace):
 class HDivSurface(_EdgeSpace):
     """HDivSurface, order 0: unit-flux Raviart-Thomas (RWG-type) functions."""
     kind = _capi.SPACE_HDIV
-    order = 0
 
 
 class HCurl(_EdgeSpace):
     """HCurl, order 0, on the surface: Whitney edge functions N_l = n x phi_l."""
     kind = _capi.SPACE_HCURL
-    order = 0
 
 
 def mass_matrix_coo(trial: FESpace, test: FESpace):
@@ -171,6 +217,18 @@ def mass_matrix_coo(trial: FESpace, test: FESpace):
         if isinstance(trial, HCurl):
             phi_j = np.cross(n[:, None, None, :], phi_j)
         loc = np.einsum("tq,tqic,tqjc->tij", w, phi_i, phi_j)  # nt x 3 x 3
+    elif trial.order > 1 or test.order > 1 or isinstance(trial, SurfaceL2) and trial.order or \
+            isinstance(test, SurfaceL2) and test.order:
+        # higher orders: reference Gram matrix by a collapsed Gauss rule
+        # (exact to degree 7), scaled by 2 * area
+        g, gw = np.polynomial.legendre.leggauss(5)
+        g, gw = 0.5 * (g + 1.0), 0.5 * gw
+        U, Vv = np.meshgrid(g, g, indexing="ij")
+        W = (gw[:, None] * gw[None, :]) * (1.0 - U)
+        x, y = U.ravel(), (Vv * (1.0 - U)).ravel()
+        bary = np.stack([x, y, 1.0 - x - y], axis=1)
+        G = np.einsum("q,qi,qj->ij", W.ravel(), test.shapes(bary), trial.shapes(bary))
+        loc = 2.0 * m.areas()[:, None, None] * G[None]
     else:
         area = m.areas()
         ni, nj = test.elem_dofs.shape[1], trial.elem_dofs.shape[1]

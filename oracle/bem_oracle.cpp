@@ -289,8 +289,10 @@ struct Mesh {
   }
 };
 
-// SP_RWG: HDivSurface order 0 (unit-flux RT0 per edge); SP_NED: HCurl order 0
-enum { SP_P0 = 0, SP_P1 = 1, SP_RWG = 2, SP_NED = 3 };
+// SP_RWG: HDivSurface order 0 (unit-flux RT0 per edge); SP_NED: HCurl order 0;
+// SP_L2_1 / SP_L2_2: SurfaceL2 order 1 / 2 (Dubiner basis); SP_P2: H1 order 2
+// (vertex hats, then the edge bubbles lambda_a lambda_b)
+enum { SP_P0 = 0, SP_P1 = 1, SP_RWG = 2, SP_NED = 3, SP_L2_1 = 4, SP_L2_2 = 5, SP_P2 = 6 };
 // EV_MAXW: [phi, div] (DiffOpMaxwellNew, src/diffops.hpp:263-338; DiffOpMaxwell :170-255)
 // EV_TR3: vector trace (HDivSurface BND; RotatedTrace :62-107 on HCurl)
 enum { EV_ID = 0, EV_ROT = 1, EV_HELM = 2, EV_MAXW = 3, EV_TR3 = 4 };
@@ -313,6 +315,32 @@ struct Space {
     } else if (k == SP_P1) {
       ndof = m.nv; nloc = 3;
       dofs.assign(m.tri.begin(), m.tri.end());
+    } else if (k == SP_L2_1 || k == SP_L2_2) {  // constants e, then the higher functions element by element
+      nloc = (k == SP_L2_1) ? 3 : 6;
+      ndof = m.nt * nloc;
+      dofs.resize((size_t)m.nt * nloc);
+      for (int e = 0; e < m.nt; e++) {
+        dofs[(size_t)e * nloc] = e;
+        for (int l = 1; l < nloc; l++) dofs[(size_t)e * nloc + l] = m.nt + e * (nloc - 1) + (l - 1);
+      }
+    } else if (k == SP_P2) {  // vertices, then the edges (v_l, v_l+1) in (min, max) order
+      nloc = 6;
+      std::vector<std::pair<int, int>> ed;
+      for (int e = 0; e < m.nt; e++)
+        for (int l = 0; l < 3; l++) {
+          int a = m.tri[3 * e + l], b = m.tri[3 * e + (l + 1) % 3];
+          ed.push_back({std::min(a, b), std::max(a, b)});
+        }
+      std::vector<std::pair<int, int>> u(ed);
+      std::sort(u.begin(), u.end());
+      u.erase(std::unique(u.begin(), u.end()), u.end());
+      ndof = m.nv + (int)u.size();
+      dofs.resize(6 * (size_t)m.nt);
+      for (int e = 0; e < m.nt; e++)
+        for (int l = 0; l < 3; l++) {
+          dofs[6 * (size_t)e + l] = m.tri[3 * e + l];
+          dofs[6 * (size_t)e + 3 + l] = m.nv + (int)(std::lower_bound(u.begin(), u.end(), ed[3 * e + l]) - u.begin());
+        }
     } else {  // edges numbered in (min vertex, max vertex) order; local edge l = (v_l, v_{l+1})
       nloc = 3;
       std::vector<std::pair<int, int>> ed;
@@ -358,8 +386,42 @@ static int ev_dim(int ev) { return ev == EV_ID ? 1 : (ev == EV_ROT || ev == EV_T
 
 // shape rows [nloc*dim] at reference point; layout dof*dim + comp
 // (DiffOpBoundaryRot src/diffops.hpp:12-57, DiffOpHelmholtz :110-165)
+// Jacobi polynomial P_n^(a,0)(t) by the three-term recurrence
+static double jacobi(int n, double a, double t) {
+  double p0 = 1.0, p1 = 0.5 * ((a + 2.0) * t + a);
+  if (n == 0) return p0;
+  for (int k = 2; k <= n; k++) {
+    const double c = 2.0 * k + a;
+    const double p2 = ((c - 1.0) * (c * (c - 2.0) * t + a * a) * p1 - 2.0 * (k + a - 1.0) * (k - 1.0) * c * p0) /
+                      (2.0 * k * (k + a) * (c - 2.0));
+    p0 = p1;
+    p1 = p2;
+  }
+  return p1;
+}
+
+// Dubiner basis of degree <= p on the triangle, x = lambda_0, y = lambda_1:
+// P_i((2x + y - 1) / (1 - y)) (1 - y)^i P_j^(2i+1,0)(2y - 1), i outer, j inner
+static void dubiner(int p, double x, double y, double* out) {
+  int n = 0;
+  const double w = 1.0 - y, s = 2.0 * x + y - 1.0;
+  for (int i = 0; i <= p; i++) {
+    // P_i(s / w) w^i without dividing by w: homogeneous Legendre recurrence
+    double l0 = 1.0, l1 = s;
+    double li = (i == 0) ? 1.0 : s;
+    for (int k = 2; k <= i; k++) {
+      const double l2 = ((2.0 * k - 1.0) * s * l1 - (k - 1.0) * w * w * l0) / k;
+      l0 = l1;
+      l1 = l2;
+      li = l2;
+    }
+    for (int j = 0; j + i <= p; j++) out[n++] = li * jacobi(j, 2.0 * i + 1.0, 2.0 * y - 1.0);
+  }
+}
+
 static void eval_shape(const Mesh& m, const Space& s, int ev, int e, double xi, double eta, double* out) {
   if (s.kind == SP_P0) { out[0] = 1.0; return; }
+  if (s.kind == SP_L2_1 || s.kind == SP_L2_2) { dubiner(s.kind == SP_L2_1 ? 1 : 2, xi, eta, out); return; }
   if (s.kind == SP_RWG || s.kind == SP_NED) {
     // RT0 with unit flux: phi_l = sign_l / J (x - p_opp(l)), div = 2 sign_l / J; the
     // rotated trace of the Whitney function of an HCurl edge is -phi_l
@@ -376,17 +438,31 @@ static void eval_shape(const Mesh& m, const Space& s, int ev, int e, double xi, 
     return;
   }
   double lam[3] = {xi, eta, 1.0 - xi - eta};
-  if (ev == EV_ID) { for (int k = 0; k < 3; k++) out[k] = lam[k]; return; }
+  // H1: values and surface gradients of lambda_k (k < 3) and, for order 2, of
+  // the bubbles lambda_a lambda_b of the edges (a, b) = (l, l+1)
   double g[3][3];
   m.grads(e, g);
+  const int nl = s.nloc;
+  double val[6], grad[6][3];
+  for (int k = 0; k < 3; k++) {
+    val[k] = lam[k];
+    for (int c = 0; c < 3; c++) grad[k][c] = g[k][c];
+  }
+  for (int l = 3; l < nl; l++) {
+    const int a = l - 3, b = (l - 2) % 3;
+    val[l] = lam[a] * lam[b];
+    for (int c = 0; c < 3; c++) grad[l][c] = lam[a] * g[b][c] + lam[b] * g[a][c];
+  }
+  if (ev == EV_ID) { for (int k = 0; k < nl; k++) out[k] = val[k]; return; }
   const double* n = &m.nrm[3 * e];
   int dim = ev_dim(ev);
-  for (int k = 0; k < 3; k++) {
-    double c[3] = {n[1] * g[k][2] - n[2] * g[k][1], n[2] * g[k][0] - n[0] * g[k][2], n[0] * g[k][1] - n[1] * g[k][0]};
+  for (int k = 0; k < nl; k++) {
+    const double* gk = grad[k];
+    double c[3] = {n[1] * gk[2] - n[2] * gk[1], n[2] * gk[0] - n[0] * gk[2], n[0] * gk[1] - n[1] * gk[0]};
     out[k * dim + 0] = c[0];
     out[k * dim + 1] = c[1];
     out[k * dim + 2] = c[2];
-    if (ev == EV_HELM) out[k * dim + 3] = lam[k];
+    if (ev == EV_HELM) out[k * dim + 3] = val[k];
   }
 }
 
@@ -705,7 +781,7 @@ static void integrate4d(const Op& op, int ei, int ej, const double* xx, const do
   const Mesh& m = *op.mesh;
   int dti = ev_dim(op.K.ev_test), dtj = ev_dim(op.K.ev_trial);
   int ni = op.test.nloc, nj = op.trial.nloc;
-  double si[12], sj[12];
+  double si[24], sj[24];
   cplx kv[3];
   for (int q = 0; q < nq; q++) {
     double X[3], Y[3];
@@ -953,10 +1029,11 @@ static bool calc_aca(const Op& op, const std::vector<int>& trialdofs, const std:
   V2.assign((size_t)R * ncols, cplx(0.0));
   int ni = op.test.nloc, nj = op.trial.nloc;
   int dti = ev_dim(op.K.ev_test), dtj = ev_dim(op.K.ev_trial);
-  std::vector<double> shq((size_t)nt * 12);
+  const int SQ = 24;  // nloc * dim <= 6 * 4
+  std::vector<double> shq((size_t)nt * SQ);
   for (size_t a = 0; a < patchi.size(); a++) {
     int e = patchi[a];
-    for (int q = 0; q < nt; q++) eval_shape(m, op.test, op.K.ev_test, e, tr.xi[q], tr.eta[q], &shq[12 * q]);
+    for (int q = 0; q < nt; q++) eval_shape(m, op.test, op.K.ev_test, e, tr.xi[q], tr.eta[q], &shq[SQ * q]);
     for (size_t t = 0; t < op.K.terms.size(); t++) {
       const Term& tm = op.K.terms[t];
       int rk = ranks[tm.kc];
@@ -965,7 +1042,7 @@ static bool calc_aca(const Op& op, const std::vector<int>& trialdofs, const std:
         if (row < 0) continue;
         for (int l = 0; l < rk; l++) {
           cplx s = 0.0;
-          for (int q = 0; q < nt; q++) s += shq[12 * q + ii * dti + tm.tec] * Us[tm.kc][(size_t)l * np + a * nt + q];
+          for (int q = 0; q < nt; q++) s += shq[SQ * q + ii * dti + tm.tec] * Us[tm.kc][(size_t)l * np + a * nt + q];
           U2[(size_t)row * R + r0[t] + l] += tm.fac * s;
         }
       }
@@ -973,7 +1050,7 @@ static bool calc_aca(const Op& op, const std::vector<int>& trialdofs, const std:
   }
   for (size_t a = 0; a < patchj.size(); a++) {
     int e = patchj[a];
-    for (int q = 0; q < nt; q++) eval_shape(m, op.trial, op.K.ev_trial, e, tr.xi[q], tr.eta[q], &shq[12 * q]);
+    for (int q = 0; q < nt; q++) eval_shape(m, op.trial, op.K.ev_trial, e, tr.xi[q], tr.eta[q], &shq[SQ * q]);
     for (size_t t = 0; t < op.K.terms.size(); t++) {
       const Term& tm = op.K.terms[t];
       int rk = ranks[tm.kc];
@@ -982,7 +1059,7 @@ static bool calc_aca(const Op& op, const std::vector<int>& trialdofs, const std:
         if (col < 0) continue;
         for (int l = 0; l < rk; l++) {
           cplx s = 0.0;
-          for (int q = 0; q < nt; q++) s += shq[12 * q + jj * dtj + tm.trc] * Vs[tm.kc][(size_t)l * nq + a * nt + q];
+          for (int q = 0; q < nt; q++) s += shq[SQ * q + jj * dtj + tm.trc] * Vs[tm.kc][(size_t)l * nq + a * nt + q];
           V2[(size_t)(r0[t] + l) * ncols + col] += s;
         }
       }
@@ -1303,7 +1380,7 @@ int orc_potential(void* p, const double* coef, const double* pts, const double* 
   for (int e = 0; e < m.nt; e++) use[e] = op.trial.has(e) && (!elmask || elmask[e]);
   for (int e = 0; e < m.nt; e++) {
     if (!use[e]) continue;
-    std::vector<double> sh(12);
+    std::vector<double> sh(24);
     for (int q = 0; q < nq; q++) {
       m.point(e, tr.xi[q], tr.eta[q], &Y[3 * ((size_t)e * nq + q)]);
       W[(size_t)e * nq + q] = m.J[e] * tr.w[q];

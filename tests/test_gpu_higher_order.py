@@ -1,0 +1,134 @@
+"""SurfaceL2 orders 1-2 and H1 order 2 on the GPU, through the C-ABI, against
+the oracle (same meshes and parameters).
+
+Bars: near-field entries within 1e-12 of the oracle's (max over the block's
+largest entry); each low-rank block as close to the exact block as the
+oracle's cross approximation (twice its error plus 10 eps); the apply of
+identical blocks (forward and transpose) within 1e-13; the potential within
+1e-12.
+"""
+import numpy as np
+import pytest
+import torch
+
+import ngbem_amd as nb
+from oracle import oracle as O
+
+pytestmark = pytest.mark.gpu
+
+SP = {O.P0: (nb.SurfaceL2, 0), O.L2_1: (nb.SurfaceL2, 1), O.L2_2: (nb.SurfaceL2, 2), O.P1: (nb.H1, 1),
+      O.P2: (nb.H1, 2)}
+# (name, kernel, trial, test, kappa)
+CASES = [
+    ("LSL", O.LAPLACE_SL, O.L2_1, O.L2_1, 0.0),
+    ("LSL", O.LAPLACE_SL, O.L2_2, O.L2_2, 0.0),
+    ("LDL", O.LAPLACE_DL, O.P2, O.L2_1, 0.0),
+    ("LDL", O.LAPLACE_DL, O.P1, O.L2_1, 0.0),
+    ("LDL", O.LAPLACE_DL, O.P2, O.P0, 0.0),
+    ("LHS", O.LAPLACE_HS, O.P2, O.P2, 0.0),
+    ("HSL", O.HELMHOLTZ_SL, O.L2_1, O.L2_1, 3.0),
+    ("HCF", O.HELMHOLTZ_CF, O.L2_2, O.L2_1, 3.0),
+    ("HHS", O.HELMHOLTZ_HS, O.P2, O.P2, 3.0),
+]
+
+
+def _build(case, level=2, eps=1e-8):
+    name, kid, trk, tek, kappa = case
+    if O.P1 in (trk, tek) or O.P0 in (trk, tek):
+        level = 3  # a lowest-order side has too few dofs for admissible blocks at level 2
+    m = nb.icosphere(level)
+    tr = SP[trk][0](m, order=SP[trk][1])
+    te = SP[tek][0](m, order=SP[tek][1])
+    kw = dict(intorder=4, leafsize=24, eta=2.0, eps=eps)
+    print(f"higher-order case {case}: ndof {tr.ndof} x {te.ndof}", flush=True)
+    if name == "LSL":
+        op = nb.SingleLayerPotentialOperator(tr, **kw)
+    elif name == "LDL":
+        op = nb.DoubleLayerPotentialOperator(tr, te, **kw)
+    elif name == "LHS":
+        op = nb.HypersingularOperator(tr, **kw)
+    elif name == "HSL":
+        op = nb.HelmholtzSingleLayerPotentialOperator(tr, te, kappa=kappa, **kw)
+    elif name == "HCF":
+        op = nb.HelmholtzCombinedFieldOperator(tr, te, kappa=kappa, **kw)
+    else:
+        op = nb.HelmholtzHypersingularOperator(tr, kappa=kappa, **kw)
+    ref = O.OracleOp(m, kid, kappa=kappa, trial=trk, test=tek, **kw)
+    return op, ref
+
+
+@pytest.mark.parametrize("case", CASES, ids=[f"{c[0]}-{c[2]}{c[3]}" for c in CASES])
+def test_higher_order_blocks_and_apply_match_oracle(gpu, case):
+    eps = 1e-8
+    op, ref = _build(case, eps=eps)
+    assert (op.trial_space.ndof, op.test_space.ndof) == (ref.ntrial, ref.ntest)
+    ob = ref.blocks()
+    assert op.num_blocks() == len(ob)
+    nfar = 0
+    for b, rb in enumerate(ob):
+        info = op.block_info(b)
+        assert (info["row_cluster"], info["col_cluster"]) == (rb["row"], rb["col"])
+        rows, cols = op.block_dofs(b)
+        data = op.block_data(b)
+        D = ref.dense(cols, rows)
+        if info["rank"] < 0:
+            D = D if op.is_complex else D.real
+            assert np.abs(data - D).max() <= 1e-12 * max(np.abs(D).max(), 1e-300), f"block {b}"
+            ref.set_block(b, data.ravel(), -1)
+        else:
+            nfar += 1
+            U, V = ref.aca(cols, rows)
+            A = data[0] @ data[1]
+            err, err_ref = np.linalg.norm(A - D), np.linalg.norm(U @ V - D)
+            assert err <= 2 * err_ref + 10 * eps * np.linalg.norm(D), f"far block {b}: {err} vs {err_ref}"
+            ref.set_block(b, np.concatenate([data[0].ravel(), data[1].ravel()]), data[0].shape[1])
+    assert nfar > 0
+    rng = np.random.default_rng(2)
+    for trans in (False, True):
+        n_in = op.test_space.ndof if trans else op.trial_space.ndof
+        x = rng.standard_normal(n_in) + (1j * rng.standard_normal(n_in) if op.is_complex else 0)
+        y_ref = ref.apply(x.astype(np.complex128), trans=trans)
+        y = ((op.mat.T if trans else op.mat) * torch.from_numpy(x).to(gpu)).cpu().numpy()
+        assert np.linalg.norm(y - y_ref) <= 1e-13 * np.linalg.norm(y_ref)
+
+
+@pytest.mark.parametrize("kid,trk,tek", [(O.LAPLACE_SL, O.L2_2, O.L2_2), (O.HELMHOLTZ_DL, O.P2, O.L2_1),
+                                         (O.HELMHOLTZ_HS, O.P2, O.P2)])
+def test_higher_order_potential_matches_oracle(gpu, kid, trk, tek):
+    m = nb.icosphere(2)
+    tr = SP[trk][0](m, order=SP[trk][1])
+    te = SP[tek][0](m, order=SP[tek][1])
+    kappa = 2.5 if kid >= O.HELMHOLTZ_SL else 0.0
+    kw = dict(intorder=4, leafsize=32)
+    if kid == O.LAPLACE_SL:
+        op = nb.SingleLayerPotentialOperator(tr, **kw)
+    elif kid == O.HELMHOLTZ_DL:
+        op = nb.HelmholtzDoubleLayerPotentialOperator(tr, te, kappa=kappa, **kw)
+    else:
+        op = nb.HelmholtzHypersingularOperator(tr, kappa=kappa, **kw)
+    ref = O.OracleOp(m, kid, kappa, trial=trk, test=tek, **kw)
+    rng = np.random.default_rng(9)
+    c = rng.standard_normal(tr.ndof) + (1j * rng.standard_normal(tr.ndof) if op.is_complex else 0)
+    pts = rng.normal(size=(200, 3)) * 2.0
+    pts = pts[np.linalg.norm(pts, axis=1) > 1.3]
+    nrm = pts / np.linalg.norm(pts, axis=1)[:, None]
+    u = op.GetPotential(torch.from_numpy(c).to(gpu))(torch.from_numpy(pts).to(gpu),
+                                                     torch.from_numpy(nrm).to(gpu)).cpu().numpy()
+    u_ref = ref.potential(c + 0j, pts, nrm)
+    u_ref = u_ref[:, 0] if u.ndim == 1 else u_ref
+    if not op.is_complex:
+        u_ref = u_ref.real
+    assert np.abs(u - u_ref).max() <= 1e-12 * np.abs(u_ref).max()
+
+
+def test_mixed_and_block_apply_higher_order(gpu):
+    """FP32 copies and the many-right-hand-side product on an order-2 space."""
+    m = nb.icosphere(3)
+    h2 = nb.H1(m, order=2)
+    W = nb.HelmholtzHypersingularOperator(h2, kappa=2.0, intorder=3, leafsize=32, eps=1e-7, mixed=True)
+    X = torch.randn(5, h2.ndof, dtype=torch.complex128, device=gpu)
+    Y = W.mat * X
+    Y1 = torch.stack([W.mat * X[i] for i in range(5)])
+    assert (torch.linalg.norm(Y - Y1) / torch.linalg.norm(Y1)).item() < 1e-6
+    Ym = W.mat.mixed * X[0]
+    assert (torch.linalg.norm(Ym - Y1[0]) / torch.linalg.norm(Y1[0])).item() < 2e-6

@@ -1,0 +1,219 @@
+"""SurfaceL2 orders 1-2 and H1 order 2, on the CPU: the oracle's operators and
+the product's host side.
+
+Pins of the oracle:
+- inclusion: the constants of SurfaceL2 order p and the vertex hats of H1
+  order 2 are the order-0 / order-1 functions, so those blocks of the
+  higher-order operators are the lower-order operators, entry for entry;
+- the hypersingular operator annihilates the constant (vertex coefficients
+  1, edge bubbles 0) exactly, as for order 1;
+- entries between two far-apart triangles against an independent numpy
+  quadrature (tensor Gauss on each triangle, the shape functions of
+  ngbem_amd.spaces) of the kernels as the reference writes them;
+- the Python shapes span the polynomials of their degree: the L2 projection
+  of a degree-p polynomial (mass matrix + load vector) reproduces it.
+Then the product's host side: dof counts and element -> dof tables equal to
+the oracle's, cluster trees and block lists bit-identical to the oracle's.
+"""
+import ctypes
+
+import numpy as np
+import pytest
+
+import ngbem_amd as nb
+from ngbem_amd import _capi
+from oracle import oracle as O
+
+
+def _collapsed_rule(n):
+    g, w = np.polynomial.legendre.leggauss(n)
+    g, w = 0.5 * (g + 1.0), 0.5 * w
+    U, V = np.meshgrid(g, g, indexing="ij")
+    W = (w[:, None] * w[None, :]) * (1.0 - U)
+    x, y = U.ravel(), (V * (1.0 - U)).ravel()
+    return np.stack([x, y, 1.0 - x - y], 1), W.ravel()  # weights sum to 1/2
+
+
+@pytest.mark.parametrize("kern,lo,hi", [(O.LAPLACE_SL, (O.P0, O.P0), (O.L2_1, O.L2_1)),
+                                        (O.LAPLACE_SL, (O.P0, O.P0), (O.L2_2, O.L2_2)),
+                                        (O.LAPLACE_DL, (O.P1, O.P0), (O.P2, O.L2_1)),
+                                        (O.LAPLACE_HS, (O.P1, O.P1), (O.P2, O.P2)),
+                                        (O.HELMHOLTZ_HS, (O.P1, O.P1), (O.P2, O.P2)),
+                                        (O.HELMHOLTZ_CF, (O.P0, O.P0), (O.L2_1, O.L2_2))])
+def test_lower_order_block_is_the_lower_order_operator(kern, lo, hi):
+    m = nb.icosphere(1)
+    kap = 2.0 if kern >= O.HELMHOLTZ_SL else 0.0
+    D0 = O.OracleOp(m, kern, kap, trial=lo[0], test=lo[1], intorder=3).dense()
+    D1 = O.OracleOp(m, kern, kap, trial=hi[0], test=hi[1], intorder=3).dense()
+    n_test, n_trial = D0.shape
+    assert np.abs(D1[:n_test, :n_trial] - D0).max() <= 1e-14 * np.abs(D0).max()
+    assert np.abs(D1).max() > 0
+
+
+def test_hypersingular_order2_annihilates_constants():
+    m = nb.icosphere(2)
+    for kern, kap in ((O.LAPLACE_HS, 0.0), (O.HELMHOLTZ_HS, 2.0)):
+        ref = O.OracleOp(m, kern, kap, trial=O.P2, test=O.P2, intorder=3)
+        W = ref.dense()
+        one = np.zeros(ref.ntrial)
+        one[:m.nv] = 1.0
+        if kern == O.LAPLACE_HS:
+            assert np.abs(W @ one).max() <= 1e-12 * np.abs(W).max()
+        # a P2 space holds nv + ne dofs
+        assert ref.ntrial == m.nv + len(m.edges())
+
+
+def _far_pair(m):
+    c = m.xyz[m.tri].mean(1)
+    e = 0
+    f = int(np.argmax(np.linalg.norm(c - c[e], axis=1)))
+    return e, f
+
+
+@pytest.mark.parametrize("space", ["L2_1", "L2_2"])
+def test_far_entries_against_numpy_quadrature(space):
+    m = nb.icosphere(1)
+    k = {"L2_1": O.L2_1, "L2_2": O.L2_2}[space]
+    order = {"L2_1": 1, "L2_2": 2}[space]
+    sp = nb.SurfaceL2(m, order=order)
+    e, f = _far_pair(m)
+    bary, w = _collapsed_rule(8)
+    P = m.xyz[m.tri]
+    X, Y = bary @ P[e], bary @ P[f]
+    J = 2.0 * m.areas()
+    phi = sp.shapes(bary)  # q x nloc
+    nx, ny = m.normals()[e], m.normals()[f]
+    d = X[:, None, :] - Y[None, :, :]
+    r = np.linalg.norm(d, axis=-1)
+    kap = 1.7
+    kernels = {O.LAPLACE_SL: 1 / (4 * np.pi * r),
+               O.LAPLACE_DL: np.einsum("ijc,c->ij", d, ny) / (4 * np.pi * r ** 3),
+               O.HELMHOLTZ_SL: np.exp(1j * kap * r) / (4 * np.pi * r)}
+    for kern, G in kernels.items():
+        ref = O.OracleOp(m, kern, kap if kern >= O.HELMHOLTZ_SL else 0.0, trial=k, test=k, intorder=12)
+        rows = sp.elem_dofs[e]
+        cols = sp.elem_dofs[f]
+        Do = ref.dense(cols, rows)
+        Dn = J[e] * J[f] * np.einsum("i,j,ij,ia,jb->ab", w, w, G, phi, phi)
+        assert np.abs(Do - Dn).max() <= 1e-10 * np.abs(Dn).max(), kern
+
+
+def test_far_edge_bubble_entries_against_numpy_quadrature():
+    """H1 order 2: single layer between an edge bubble (two triangles) and a
+    far triangle's SurfaceL2 order-1 functions (double layer, P2 -> L2_1)."""
+    m = nb.icosphere(1)
+    h2, l1 = nb.H1(m, order=2), nb.SurfaceL2(m, order=1)
+    e, f = _far_pair(m)
+    dof = h2.elem_dofs[e, 3]  # bubble of edge (v0, v1) of triangle e
+    supp = [(t, l) for t in range(m.nt) for l in range(6) if h2.elem_dofs[t, l] == dof]
+    assert len(supp) == 2
+    bary, w = _collapsed_rule(8)
+    P = m.xyz[m.tri]
+    J = 2.0 * m.areas()
+    Y = bary @ P[f]
+    psi = l1.shapes(bary)
+    ny = m.normals()[f]
+    ref = O.OracleOp(m, O.LAPLACE_DL, 0.0, trial=O.P2, test=O.L2_1, intorder=12)
+    Do = ref.dense(np.array([dof]), l1.elem_dofs[f])[:, 0]
+    Dn = np.zeros(3)
+    for t, l in supp:
+        X = bary @ P[t]
+        phi = h2.shapes(bary)[:, l]
+        d = Y[:, None, :] - X[None, :, :]  # test point minus trial point
+        r = np.linalg.norm(d, axis=-1)
+        # double layer kernel of the reference: n_y . (x - y) / (4 pi |x - y|^3),
+        # y the trial point (src/ngbem.hpp:257-274)
+        G = np.einsum("ijc,c->ij", d, m.normals()[t]) / (4 * np.pi * r ** 3)
+        Dn += J[f] * J[t] * np.einsum("i,j,ij,ia,j->a", w, w, G, psi, phi)
+    assert np.abs(Do - Dn).max() <= 1e-10 * np.abs(Dn).max()
+
+
+@pytest.mark.parametrize("cls,order", [(nb.SurfaceL2, 1), (nb.SurfaceL2, 2), (nb.H1, 2)])
+def test_projection_reproduces_polynomials(cls, order):
+    """int (f - f_h) v = 0 with f of degree `order` in the element's own
+    barycentrics: f_h = f at every quadrature point (SurfaceL2: per element;
+    H1: f continuous, a global quadratic restricted to a flat triangle)."""
+    m = nb.ellipsoid(1, (1.0, 0.8, 0.6))
+    sp = cls(m, order=order)
+    r, c, v = nb.spaces.mass_matrix_coo(sp, sp)
+    M = np.zeros((sp.ndof, sp.ndof))
+    np.add.at(M, (r, c), v)
+    bary, w = _collapsed_rule(6)
+    P = m.xyz[m.tri]
+    J = 2.0 * m.areas()
+    phi = sp.shapes(bary)
+
+    def f(X):
+        return 0.3 + X[..., 0] - 0.7 * X[..., 1] + (order >= 2) * (X[..., 0] * X[..., 2] - 0.4 * X[..., 1] ** 2)
+
+    b = np.zeros(sp.ndof)
+    for t in range(m.nt):
+        X = bary @ P[t]
+        np.add.at(b, sp.elem_dofs[t], J[t] * np.einsum("q,q,qa->a", w, f(X), phi))
+    coef = np.linalg.solve(M, b)
+    err = 0.0
+    for t in range(m.nt):
+        X = bary @ P[t]
+        err = max(err, np.abs(phi @ coef[sp.elem_dofs[t]] - f(X)).max())
+    assert err < 1e-11
+
+
+@pytest.mark.parametrize("cls,order,ik", [(nb.SurfaceL2, 1, O.L2_1), (nb.SurfaceL2, 2, O.L2_2), (nb.H1, 2, O.P2)])
+def test_numbering_matches_oracle(cls, order, ik):
+    m = nb.icosphere(2)
+    sp = cls(m, order=order)
+    ref = O.OracleOp(m, O.LAPLACE_SL, 0.0, trial=ik, test=ik)
+    assert sp.ndof == ref.ntrial and sp.ikind == ik
+    assert np.array_equal(sp.elem_dofs, ref.elem_dofs())
+    if cls is nb.H1:
+        assert sp.ndof == m.nv + len(m.edges())
+        assert np.array_equal(sp.elem_dofs[:, :3], m.tri)
+    else:
+        assert sp.ndof == m.nt * ((order + 1) * (order + 2) // 2)
+        assert np.array_equal(sp.elem_dofs[:, 0], np.arange(m.nt))
+
+
+@pytest.mark.parametrize("trk,tek,kern", [(O.L2_1, O.L2_1, _capi.LAPLACE_SL), (O.P2, O.L2_1, _capi.LAPLACE_DL),
+                                          (O.P2, O.P2, _capi.LAPLACE_HS), (O.L2_2, O.P0, _capi.HELMHOLTZ_SL)])
+def test_tree_identical_to_oracle(trk, tek, kern):
+    m = nb.icosphere(3)
+    mk = {O.P0: lambda: nb.SurfaceL2(m), O.L2_1: lambda: nb.SurfaceL2(m, order=1),
+          O.L2_2: lambda: nb.SurfaceL2(m, order=2), O.P2: lambda: nb.H1(m, order=2)}
+    tr, te = mk[trk](), mk[tek]()
+    L = _capi.lib()
+    params = _capi.Params(3, 32, 2.0, 1e-6, 0, 0)
+    h = ctypes.c_void_p()
+    _capi.check(L.bem_op_create(kern, 1.0, tr.handle, te.handle, ctypes.byref(params), ctypes.byref(h)))
+    try:
+        ref = O.OracleOp(m, kern, kappa=1.0, trial=trk, test=tek, leafsize=32)
+        for which, n in ((0, tr.ndof), (1, te.ndof)):
+            perm = np.zeros(n, dtype=np.int32)
+            ncl = L.bem_op_tree(h, which, _capi.iptr(perm), None, None, 0)
+            cl, cen = np.zeros(7 * ncl), np.zeros(3 * ncl)
+            L.bem_op_tree(h, which, _capi.iptr(perm), _capi.dptr(cl), _capi.dptr(cen), ncl)
+            operm, ocl, ocen = ref.tree(which)
+            assert np.array_equal(perm, operm) and np.array_equal(cl.reshape(-1, 7), ocl)
+        assert L.bem_op_num_blocks(h) == ref.nblocks
+    finally:
+        L.bem_op_destroy(h)
+
+
+@pytest.mark.parametrize("kid,trk,tek", [(O.HELMHOLTZ_HS, O.P2, O.P2), (O.LAPLACE_HS, O.P2, O.P2),
+                                         (O.LAPLACE_DL, O.P2, O.L2_2), (O.HELMHOLTZ_SL, O.L2_2, O.L2_1)])
+def test_oracle_aca_of_higher_order_blocks(kid, trk, tek):
+    """The oracle's cross approximation projects onto up to 6 functions x 4
+    evaluator components per triangle; every far block against its dense form."""
+    m = nb.icosphere(2)
+    ref = O.OracleOp(m, kid, 3.0, trial=trk, test=tek, intorder=4, leafsize=24, eta=2.0, eps=1e-8)
+    nfar = 0
+    for b, bi in enumerate(ref.blocks()):
+        if bi["near"]:
+            continue
+        rows, cols = ref.block_dofs(b)
+        UV = ref.aca(cols, rows)
+        if UV is None:  # patches touch: the reference falls back to a dense block
+            continue
+        D = ref.dense(cols, rows)
+        assert np.linalg.norm(UV[0] @ UV[1] - D) <= 1e-6 * np.linalg.norm(D)
+        nfar += 1
+    assert nfar > 0
