@@ -67,6 +67,7 @@ def main():
     ap.add_argument("--kappa", type=float, default=10.0)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-solve", action="store_true")
+    ap.add_argument("--no-aca-gram", action="store_true", help="skip the extra assembly with NGBEM_ACA=gram")
     ap.add_argument("--calib-copy", action="store_true",
                     help="end with one elementwise kernel reading and writing 512 MiB (calibrates rocprof "
                          "FETCH_SIZE/WRITE_SIZE)")
@@ -134,6 +135,25 @@ def main():
         del C2
         log(f"second assembly {t_assembly2:.3f} s  near {st2['t_near']:.3f} s  far {st2['t_far']:.3f} s  "
             f"setup {st2['t_setup']:.3f} s")
+
+    # the same assembly with the single-pass (Gram) cross approximation,
+    # NGBEM_ACA=gram: faster far field, stops that may move by a step
+    aca_gram = None
+    if not args.no_aca_gram:
+        os.environ["NGBEM_ACA"] = "gram"
+        try:
+            torch.cuda.synchronize()
+            t0 = time.perf_counter()
+            C3 = nb.HelmholtzCombinedFieldOperator(fes, fes, kappa=kappa, mixed=True, **params)
+            torch.cuda.synchronize()
+            tg = time.perf_counter() - t0
+            st3 = C3.stats()
+            aca_gram = {"assembly_s": tg, "far_s": st3["t_far"], "far_factor_entries": st3["far_factor_entries"],
+                        "far_factor_entries_default": st["far_factor_entries"]}
+            del C3
+        finally:
+            del os.environ["NGBEM_ACA"]
+        log(f"assembly with NGBEM_ACA=gram {tg:.3f} s  far {st3['t_far']:.3f} s")
 
     x = torch.randn(fes.ndof, dtype=torch.complex128, device=dev)
     y = torch.zeros_like(x)
@@ -231,6 +251,7 @@ def main():
         "assembly_s": t_assembly,
         "assembly_breakdown_s": {"setup": st["t_setup"], "near": st["t_near"], "far_aca": st["t_far"]},
         "second_assembly_s": t_assembly2,
+        "assembly_aca_gram": aca_gram,
         "fp64_matvec_per_s": args.steps / t64,
         "mixed_vs_fp64_matvec_speedup": t64 / t32,
         "mixed_matvec_rel_err": mv_err,

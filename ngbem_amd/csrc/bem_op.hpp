@@ -169,6 +169,7 @@ struct FarBlk {
 // back by the kernel
 struct AcaTask {
   int64_t uoff, voff;  // workspace offsets (elements)
+  int64_t goff;        // Gram matrices (2 cap x cap) of k_aca_gram
   double norm2;        // resume state: squared Frobenius norm estimate
   int blk, comp;
   int np, nq;          // points (patch size * trig points)

@@ -218,3 +218,32 @@ def test_aca_resume_matches_oracle(gpu, case, monkeypatch):
         d = op.block_data(b)
         assert np.linalg.norm(d[0] @ d[1] - A_ref) <= 10 * eps * np.linalg.norm(A_ref)
     assert nfar > 0
+
+
+@pytest.mark.parametrize("case", [CASES[9], CASES[12], CASES[18]], ids=lambda c: f"{c[0]}-{c[1]}{c[2]}")
+def test_gram_aca_option(gpu, case, monkeypatch):
+    """NGBEM_ACA=gram (single-pass cross approximation with Gram matrices):
+    the same pivots, so every low-rank block is as close to the exact block as
+    the oracle's cross approximation (twice its error plus 10 eps) and the
+    ranks agree with the oracle's within one step in at least 90% of blocks."""
+    monkeypatch.setenv("NGBEM_ACA", "gram")
+    op, ref = _build(case)
+    eps = case[7]
+    nfar = close = 0
+    for b, rb in enumerate(ref.blocks()):
+        info = op.block_info(b)
+        if info["rank"] < 0:
+            continue
+        rows, cols = op.block_dofs(b)
+        UV = ref.aca(cols, rows)
+        if UV is None:
+            continue
+        nfar += 1
+        d = op.block_data(b)
+        D = ref.dense(cols, rows)
+        err, err_ref = np.linalg.norm(d[0] @ d[1] - D), np.linalg.norm(UV[0] @ UV[1] - D)
+        assert err <= 2 * err_ref + 10 * eps * np.linalg.norm(D), f"far block {b}: {err} vs {err_ref}"
+        # one step of the point-level ACA is one rank per kernel term
+        terms = {"HCF": 1, "LHS": 3, "MSL": 4}[case[0]]
+        close += abs(UV[0].shape[1] - info["rank"]) <= terms
+    assert nfar > 0 and close >= 0.9 * nfar
