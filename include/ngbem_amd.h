@@ -38,10 +38,11 @@ typedef struct bem_batch bem_batch;
 
 /* space kinds (src/python_bem.cpp callers pass NGSolve spaces of these types) */
 enum {
-  BEM_SPACE_P0 = 0,    /* SurfaceL2, order 0-2: order 0 one dof per triangle (shape 1); orders 1, 2 add
-                          the triangle's higher Dubiner functions (3 / 6 per triangle, constants first) */
-  BEM_SPACE_P1 = 1,    /* H1, order 1-2: one dof per vertex (barycentric hats); order 2 adds one edge
-                          bubble lambda_a lambda_b per edge (numbered after the vertices) */
+  BEM_SPACE_P0 = 0,    /* SurfaceL2, order 0-3: order 0 one dof per triangle (shape 1); orders 1-3 add
+                          the triangle's higher Dubiner functions (3 / 6 / 10 per triangle, constants first) */
+  BEM_SPACE_P1 = 1,    /* H1, order 1-3: one dof per vertex (barycentric hats); order 2 adds one edge
+                          bubble lambda_a lambda_b per edge, order 3 a second, oriented edge function per
+                          edge and a face bubble (numbered vertices, edges, faces) */
   BEM_SPACE_HDIV = 2,  /* HDivSurface, order 0: one dof per edge, unit-flux RT0 */
   BEM_SPACE_HCURL = 3  /* HCurl, order 0, on the surface: one dof per edge, Whitney */
 };
@@ -147,7 +148,7 @@ int bem_space_create(bem_mesh* m, int kind, int order, bem_space** out);
  * whole-mesh numbering gives them. */
 int bem_space_create_on(bem_mesh* m, int kind, int order, const unsigned char* elmask, bem_space** out);
 int bem_space_ndof(const bem_space* s);
-/* element -> dof table (nt x nloc, nloc <= 6; -1 on elements outside the
+/* element -> dof table (nt x nloc, nloc <= 10; -1 on elements outside the
  * space's region); returns nloc.  Replaces FESpace::GetDofNrs over the surface
  * elements. */
 int bem_space_dofs(const bem_space* s, int32_t* dofs);

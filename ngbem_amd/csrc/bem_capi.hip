@@ -64,25 +64,34 @@ double now() {
 
 // admissible blocks whose patches share a vertex are stored dense, like the
 // reference's exception path (src/ngbem.cpp:839-842, 143-153)
+// admissible blocks whose test and trial patches share a vertex are stored
+// dense (src/ngbem.cpp:839-842, 143-153); blocks split over host threads,
+// each with its own vertex stamps
 void mark_fallbacks(bem::Operator& op) {
   const bem::Mesh& m = *op.mesh;
-  std::vector<int> stamp(m.nv, -1);
   const bem::ClusterPatches& ci = *op.cp_test;
   const bem::ClusterPatches& cj = op.cp_trial;
-  for (size_t b = 0; b < op.blocks.size(); b++) {
-    bem::Block& bl = op.blocks[b];
-    if (bl.near) continue;
-    for (int k = ci.off[bl.row]; k < ci.off[bl.row + 1]; k++)
-      for (int c = 0; c < 3; c++) stamp[m.tri[3 * ci.list[k] + c]] = (int)b;
-    bool shared = false;
-    for (int k = cj.off[bl.col]; k < cj.off[bl.col + 1] && !shared; k++)
-      for (int c = 0; c < 3; c++)
-        if (stamp[m.tri[3 * cj.list[k] + c]] == (int)b) shared = true;
-    if (shared) {
-      bl.fallback = 1;
-      op.n_fallback++;
+  const size_t nb = op.blocks.size();
+  const int nth = (int)std::max<size_t>(1, std::min<size_t>(16, std::thread::hardware_concurrency()));
+  auto work = [&](int t) {
+    std::vector<int> stamp(m.nv, -1);
+    for (size_t b = t; b < nb; b += nth) {
+      bem::Block& bl = op.blocks[b];
+      if (bl.near) continue;
+      for (int k = ci.off[bl.row]; k < ci.off[bl.row + 1]; k++)
+        for (int c = 0; c < 3; c++) stamp[m.tri[3 * ci.list[k] + c]] = (int)b;
+      bool shared = false;
+      for (int k = cj.off[bl.col]; k < cj.off[bl.col + 1] && !shared; k++)
+        for (int c = 0; c < 3; c++)
+          if (stamp[m.tri[3 * cj.list[k] + c]] == (int)b) shared = true;
+      if (shared) bl.fallback = 1;
     }
-  }
+  };
+  std::vector<std::thread> th;
+  for (int t = 1; t < nth; t++) th.emplace_back(work, t);
+  work(0);
+  for (auto& x : th) x.join();
+  for (const bem::Block& bl : op.blocks) op.n_fallback += bl.fallback;
 }
 }  // namespace
 
@@ -120,13 +129,16 @@ int bem_mesh_destroy(bem_mesh* m) {
 
 int bem_space_create_on(bem_mesh* m, int kind, int order, const unsigned char* elmask, bem_space** out) {
   if (!m || !out) return fail(-1, "bem_space_create: bad arguments");
-  if (!((kind == BEM_SPACE_P0 && order >= 0 && order <= 2) || (kind == BEM_SPACE_P1 && (order == 1 || order == 2)) ||
+  if (!((kind == BEM_SPACE_P0 && order >= 0 && order <= 3) || (kind == BEM_SPACE_P1 && order >= 1 && order <= 3) ||
         (kind == BEM_SPACE_HDIV && order == 0) || (kind == BEM_SPACE_HCURL && order == 0)))
-    return fail(-1, "bem_space_create: supported spaces are SurfaceL2 orders 0-2, H1 orders 1-2, HDivSurface "
+    return fail(-1, "bem_space_create: supported spaces are SurfaceL2 orders 0-3, H1 orders 1-3, HDivSurface "
                     "order 0 and HCurl order 0");
   int ik = kind;  // internal kind
-  if (kind == BEM_SPACE_P0) ik = order == 0 ? bem::SP_P0 : (order == 1 ? bem::SP_L2_1 : bem::SP_L2_2);
-  if (kind == BEM_SPACE_P1) ik = order == 1 ? bem::SP_P1 : bem::SP_P2;
+  if (kind == BEM_SPACE_P0) {
+    static const int k0[4] = {bem::SP_P0, bem::SP_L2_1, bem::SP_L2_2, bem::SP_L2_3};
+    ik = k0[order];
+  }
+  if (kind == BEM_SPACE_P1) ik = order == 1 ? bem::SP_P1 : (order == 2 ? bem::SP_P2 : bem::SP_P3);
   if (elmask) {
     bool any = false;
     for (int e = 0; e < m->m->nt && !any; e++) any = elmask[e] != 0;

@@ -274,12 +274,12 @@ struct KMaxDL {  // e^{ikr}(ikr - 1)(x-y)/(4 pi r^3), cross-product terms (src/n
 // shape rows at one point, layout dof*DIM + comp (src/diffops.hpp:12-338).
 // Per-element shape data (stride SDS doubles): [n x grad lambda_k (9),
 // edge scale s_k = sign_k / J (3), vertex opposite edge k (9), test space
-// kind, trial space kind].
+// kind, trial space kind, orientation of edge k (+1 when v_k < v_k+1) (3)].
 // load(): copy what the evaluator needs into registers; PERM evaluators index
 // their dofs by Sauter-Schwab vertex (px) and take barycentrics in that order,
 // the others index by element-local dof and take local barycentrics.
 // MULTI: more than one dof per element (scatter through the dof tables).
-constexpr int SDS = 23;
+constexpr int SDS = 26;
 
 template <int SP, int EV>
 struct Ev;
@@ -350,72 +350,98 @@ struct EvEdge {
     }
   }
 };
-// Higher-order scalar spaces (SurfaceL2 orders 1-2, H1 order 2, and the
+// Jacobi polynomial P_n^(a,0)(t) by the three-term recurrence
+__device__ __forceinline__ double jacobi_a0(int n, double a, double t) {
+  double p0 = 1.0, p1 = 0.5 * ((a + 2.0) * t + a);
+  if (n == 0) return p0;
+  for (int k = 2; k <= n; k++) {
+    const double c = 2.0 * k + a;
+    const double p2 = ((c - 1.0) * (c * (c - 2.0) * t + a * a) * p1 - 2.0 * (k + a - 1.0) * (k - 1.0) * c * p0) /
+                      (2.0 * k * (k + a) * (c - 2.0));
+    p0 = p1;
+    p1 = p2;
+  }
+  return p1;
+}
+
+// Higher-order scalar spaces (SurfaceL2 orders 1-3, H1 orders 2-3, and the
 // lowest orders when paired with one of them): one evaluator whose space kind
 // is read at run time from the element data (slot 21 + SIDE; uniform across
-// the launch, so the switch does not diverge); NL = NLOC_MAX, unused rows
+// the launch, so the switch does not diverge); NL = 6 or 10 rows, unused rows
 // are zero.  EVK: EV_ID (values), EV_ROT (n x grad_Gamma, H1 only), EV_HELM
 // ([n x grad_Gamma, value], H1 only).
-//   SurfaceL2: Dubiner basis in x = lambda_0, y = lambda_1 (orders i + j <= p,
-//     i outer): 1 | 3y-1, 2x+y-1 | P2^(1,0)(2y-1), (2x+y-1)(5y-1),
-//     (3(2x+y-1)^2 - (1-y)^2)/2, with P2^(1,0)(t) = (5t^2 + 2t - 1)/2;
-//   H1: lambda_0..2, then lambda_l lambda_(l+1) for the edges l = 0, 1, 2.
-template <int SIDE, int EVK>
+//   SurfaceL2: Dubiner basis in x = lambda_0, y = lambda_1, i + j <= p, i
+//     outer: P_i((2x+y-1)/(1-y)) (1-y)^i P_j^(2i+1,0)(2y-1);
+//   H1: lambda_0..2; per edge l = (a, b) = (l, l+1): lambda_a lambda_b and
+//     (order 3) s_l lambda_a lambda_b (lambda_b - lambda_a) with the edge's
+//     orientation s_l (slots 23-25); (order 3) lambda_0 lambda_1 lambda_2.
+template <int SIDE, int EVK, int NLR>
 struct EvGen {
-  static constexpr int NL = NLOC_MAX, DIM = EVK == EV_ID ? 1 : (EVK == EV_ROT ? 3 : 4), NB = 10;
+  static constexpr int NL = NLR, DIM = EVK == EV_ID ? 1 : (EVK == EV_ROT ? 3 : 4), NB = 13;
   static constexpr bool MULTI = true, PERM = false;
   __device__ __forceinline__ static void load(const double* sd, const int*, double* b) {
     for (int i = 0; i < 9; i++) b[i] = sd[i];
     b[9] = sd[21 + SIDE];
+    b[10] = sd[23];
+    b[11] = sd[24];
+    b[12] = sd[25];
+  }
+  __device__ __forceinline__ static void put(double* o, int k, double v, const double* g) {
+    if (k >= NL) return;
+    if (EVK == EV_ID) {
+      o[k] = v;
+    } else {
+      o[DIM * k] = g[0];
+      o[DIM * k + 1] = g[1];
+      o[DIM * k + 2] = g[2];
+      if (EVK == EV_HELM) o[DIM * k + 3] = v;
+    }
   }
   __device__ __forceinline__ static void shape(const double* lam, P3, const double* b, double* o) {
     const int kind = (int)b[9];
 #pragma unroll
     for (int i = 0; i < NL * DIM; i++) o[i] = 0.0;
-    if (EVK == EV_ID && (kind == SP_P0 || kind == SP_L2_1 || kind == SP_L2_2)) {
-      const double x = lam[0], y = lam[1];
-      o[0] = 1.0;
-      if (kind == SP_P0) return;
-      const double t = 2.0 * y - 1.0, sx = 2.0 * x + y - 1.0;
-      if (kind == SP_L2_1) {
-        o[1] = 3.0 * y - 1.0;
-        o[2] = sx;
-        return;
+    if (EVK == EV_ID && (kind == SP_P0 || kind == SP_L2_1 || kind == SP_L2_2 || kind == SP_L2_3)) {
+      const int p = kind == SP_P0 ? 0 : (kind == SP_L2_1 ? 1 : (kind == SP_L2_2 ? 2 : 3));
+      const double y = lam[1], w = 1.0 - y, sx = 2.0 * lam[0] + y - 1.0, t = 2.0 * y - 1.0;
+      double qm = 0.0, qc = 1.0;  // (1-y)^i P_i(sx / (1-y)), homogeneous Legendre recurrence
+      int n = 0;
+      for (int i = 0; i <= p; i++) {
+        if (i == 1) {
+          qm = qc;
+          qc = sx;
+        } else if (i > 1) {
+          const double qn = ((2.0 * i - 1.0) * sx * qc - (i - 1.0) * w * w * qm) / i;
+          qm = qc;
+          qc = qn;
+        }
+        for (int j = 0; j + i <= p; j++)
+          if (n < NL) o[n++] = qc * jacobi_a0(j, 2.0 * i + 1.0, t);
       }
-      const double w = 1.0 - y;
-      o[1] = 3.0 * y - 1.0;
-      o[2] = 0.5 * (5.0 * t * t + 2.0 * t - 1.0);
-      o[3] = sx;
-      o[4] = sx * (5.0 * y - 1.0);
-      o[5] = 0.5 * (3.0 * sx * sx - w * w);
       return;
     }
-    // H1 orders 1, 2 (vertex functions, then edge bubbles)
-    const int nl = kind == SP_P2 ? 6 : 3;
-#pragma unroll
-    for (int k = 0; k < 6; k++) {
-      if (k >= nl) break;
-      double v, g[3];
-      if (k < 3) {
-        v = lam[k];
-        g[0] = b[3 * k];
-        g[1] = b[3 * k + 1];
-        g[2] = b[3 * k + 2];
-      } else {
-        const int a = k - 3, c = (k - 2) % 3;  // edge (a, a + 1)
-        v = lam[a] * lam[c];
-        g[0] = lam[a] * b[3 * c] + lam[c] * b[3 * a];
-        g[1] = lam[a] * b[3 * c + 1] + lam[c] * b[3 * a + 1];
-        g[2] = lam[a] * b[3 * c + 2] + lam[c] * b[3 * a + 2];
+    // H1 orders 1-3
+    const int pe = kind == SP_P1 ? 0 : (kind == SP_P2 ? 1 : 2);
+    const double* gv[3] = {b, b + 3, b + 6};  // n x grad lambda_k
+    for (int k = 0; k < 3; k++) put(o, k, lam[k], gv[k]);
+    for (int l = 0; l < 3 && pe > 0; l++) {
+      const int a = l, c = (l + 1) % 3;
+      const double v2 = lam[a] * lam[c];
+      double g2[3];
+      for (int d = 0; d < 3; d++) g2[d] = lam[a] * gv[c][d] + lam[c] * gv[a][d];
+      put(o, 3 + pe * l, v2, g2);
+      if (pe == 2) {
+        const double sl = b[10 + l], df = lam[c] - lam[a];
+        double g3[3];
+        for (int d = 0; d < 3; d++) g3[d] = sl * (g2[d] * df + v2 * (gv[c][d] - gv[a][d]));
+        put(o, 4 + pe * l, sl * v2 * df, g3);
       }
-      if (EVK == EV_ID) {
-        o[k] = v;
-      } else {
-        o[DIM * k] = g[0];
-        o[DIM * k + 1] = g[1];
-        o[DIM * k + 2] = g[2];
-        if (EVK == EV_HELM) o[DIM * k + 3] = v;
-      }
+    }
+    if (pe == 2) {
+      double gf[3];
+      for (int d = 0; d < 3; d++)
+        gf[d] = lam[1] * lam[2] * gv[0][d] + lam[0] * lam[2] * gv[1][d] + lam[0] * lam[1] * gv[2][d];
+      put(o, 9, lam[0] * lam[1] * lam[2], gf);
     }
   }
 };
@@ -1813,19 +1839,24 @@ static void dispatch_full(const Operator& op, F&& f) {
   dispatch_kernel(op.kernel, op.kappa, [&](auto ker) {
     using KER = decltype(ker);
     const bool hi = is_higher_kind(st) || is_higher_kind(sr);
+    const bool big = nloc_of_kind(st) > 6 || nloc_of_kind(sr) > 6;  // order 3: 10 rows
     if constexpr (std::is_same<KER, KLapHS>::value) {
-      if (hi) f(ker, EvGen<0, EV_ROT>{}, EvGen<1, EV_ROT>{});
+      if (big) f(ker, EvGen<0, EV_ROT, 10>{}, EvGen<1, EV_ROT, 10>{});
+      else if (hi) f(ker, EvGen<0, EV_ROT, 6>{}, EvGen<1, EV_ROT, 6>{});
       else f(ker, Ev<SP_P1, EV_ROT>{}, Ev<SP_P1, EV_ROT>{});
     } else if constexpr (std::is_same<KER, KHelHS>::value) {
-      if (hi) f(ker, EvGen<0, EV_HELM>{}, EvGen<1, EV_HELM>{});
+      if (big) f(ker, EvGen<0, EV_HELM, 10>{}, EvGen<1, EV_HELM, 10>{});
+      else if (hi) f(ker, EvGen<0, EV_HELM, 6>{}, EvGen<1, EV_HELM, 6>{});
       else f(ker, Ev<SP_P1, EV_HELM>{}, Ev<SP_P1, EV_HELM>{});
     } else if constexpr (std::is_same<KER, KMaxSL>::value) {
       if (st == SP_RWG) f(ker, Ev<SP_RWG, EV_MAXW>{}, Ev<SP_RWG, EV_MAXW>{});
       else f(ker, Ev<SP_NED, EV_MAXW>{}, Ev<SP_NED, EV_MAXW>{});
     } else if constexpr (std::is_same<KER, KMaxDL>::value) {
       f(ker, Ev<SP_RWG, EV_TRACE3>{}, Ev<SP_NED, EV_TRACE3>{});  // test: HDivSurface trace, trial: rotated HCurl trace
+    } else if (big) {
+      f(ker, EvGen<0, EV_ID, 10>{}, EvGen<1, EV_ID, 10>{});
     } else if (hi) {
-      f(ker, EvGen<0, EV_ID>{}, EvGen<1, EV_ID>{});
+      f(ker, EvGen<0, EV_ID, 6>{}, EvGen<1, EV_ID, 6>{});
     } else {
       if (st == SP_P0 && sr == SP_P0) f(ker, Ev<SP_P0, EV_ID>{}, Ev<SP_P0, EV_ID>{});
       else if (st == SP_P0 && sr == SP_P1) f(ker, Ev<SP_P0, EV_ID>{}, Ev<SP_P1, EV_ID>{});
@@ -1873,6 +1904,7 @@ void setup_device(Operator& op, hipStream_t s) {
     }
     sd[21] = op.test->kind;
     sd[22] = op.trial->kind;
+    for (int k = 0; k < 3; k++) sd[23 + k] = m.tri[3 * e + k] < m.tri[3 * e + (k + 1) % 3] ? 1.0 : -1.0;
   }
   // unused slots (l >= nloc; the generic evaluators' zero rows) hold -1
   auto dof_table = [&](const Space& sp) {

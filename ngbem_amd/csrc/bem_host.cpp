@@ -8,6 +8,7 @@
 #include "bem_host.hpp"
 
 #include <algorithm>
+#include <thread>
 #include <cmath>
 #include <cstring>
 
@@ -84,9 +85,9 @@ void Space::setup(const Mesh* m, int k, const unsigned char* elmask) {
     nloc = 1;
     dofs.resize(m->nt);
     for (int e = 0; e < m->nt; e++) dofs[e] = e;
-  } else if (k == SP_L2_1 || k == SP_L2_2) {
+  } else if (k == SP_L2_1 || k == SP_L2_2 || k == SP_L2_3) {
     // constants first (dof e), then each element's higher functions in a block
-    nloc = k == SP_L2_1 ? 3 : 6;
+    nloc = nloc_of_kind(k);
     ndof = m->nt * nloc;
     dofs.resize((size_t)m->nt * nloc);
     for (int e = 0; e < m->nt; e++) {
@@ -97,10 +98,11 @@ void Space::setup(const Mesh* m, int k, const unsigned char* elmask) {
     ndof = m->nv;
     nloc = 3;
     dofs.assign(m->tri.begin(), m->tri.end());
-  } else if (k == SP_P2) {
-    // vertex dofs, then one dof per edge (v_l, v_l+1), edges numbered in
-    // sorted (min, max) order after the vertices
-    nloc = 6;
+  } else if (k == SP_P2 || k == SP_P3) {
+    // vertex dofs, then p - 1 dofs per edge (v_l, v_l+1), edges in sorted
+    // (min, max) order after the vertices, then (order 3) one per face
+    const int pe = k == SP_P2 ? 1 : 2;
+    nloc = nloc_of_kind(k);
     auto key = [m](int e, int l) {
       const int a = m->tri[3 * e + l], b = m->tri[3 * e + (l + 1) % 3];
       return (long long)std::min(a, b) * m->nv + std::max(a, b);
@@ -111,14 +113,20 @@ void Space::setup(const Mesh* m, int k, const unsigned char* elmask) {
       for (int l = 0; l < 3; l++) uk.push_back(key(e, l));
     std::sort(uk.begin(), uk.end());
     uk.erase(std::unique(uk.begin(), uk.end()), uk.end());
-    ndof = m->nv + (int)uk.size();
-    dofs.resize(6 * (size_t)m->nt);
-    for (int e = 0; e < m->nt; e++)
+    const int ne = (int)uk.size();
+    ndof = m->nv + pe * ne + (k == SP_P3 ? m->nt : 0);
+    dofs.resize((size_t)nloc * m->nt);
+    sign.resize(3 * (size_t)m->nt);
+    for (int e = 0; e < m->nt; e++) {
+      int* d = &dofs[(size_t)nloc * e];
       for (int l = 0; l < 3; l++) {
-        dofs[6 * (size_t)e + l] = m->tri[3 * e + l];
-        dofs[6 * (size_t)e + 3 + l] =
-            m->nv + (int)(std::lower_bound(uk.begin(), uk.end(), key(e, l)) - uk.begin());
+        d[l] = m->tri[3 * e + l];
+        const int ed = (int)(std::lower_bound(uk.begin(), uk.end(), key(e, l)) - uk.begin());
+        for (int q = 0; q < pe; q++) d[3 + pe * l + q] = m->nv + pe * ed + q;
+        sign[3 * e + l] = m->tri[3 * e + l] < m->tri[3 * e + (l + 1) % 3] ? 1 : -1;
       }
+      if (k == SP_P3) d[9] = m->nv + pe * ne + e;
+    }
   } else {
     // edges sorted by (min vertex, max vertex); local dof k of a triangle is
     // its edge (v_k, v_{k+1}), signed +1 when v_k < v_{k+1}
@@ -417,15 +425,21 @@ void Tree::build(const Space& s, int leafsize, const std::vector<int>* inst) {
     if (!s.has(e)) continue;
     double mip[7][3];
     for (int j = 0; j < 7; j++) m.point(e, xh[j][0], xh[j][1], mip[j]);
-    int vc = 0, ec = 0;
+    // edge dofs (two supporting elements) come in groups of nce / 3 per edge
+    // and go to the midpoint of their group's edge (src/hmat.cpp:120-140;
+    // fewer than three such dofs, as at the rim of an open surface: one per
+    // midpoint, the last midpoint repeated)
+    int vc = 0, ec = 0, nce = 0;
+    for (int l = 0; l < s.nloc; l++) nce += cnt[s.dofs[e * s.nloc + l]] == 2;
+    const int grp = std::max(1, nce / 3);
     for (int l = 0; l < s.nloc; l++) {
       int d = s.dofs[e * s.nloc + l];
       if (cnt[d] > 2) {
-        for (int k = 0; k < 3; k++) X[3 * d + k] = mip[vc][k];
+        for (int k = 0; k < 3; k++) X[3 * d + k] = mip[std::min(vc, 2)][k];
         vc++;
       }
-      if (cnt[d] == 2) {  // edge dof: midpoint of the element's ec-th edge (src/hmat.cpp:120-140)
-        for (int k = 0; k < 3; k++) X[3 * d + k] = mip[3 + ec][k];
+      if (cnt[d] == 2) {
+        for (int k = 0; k < 3; k++) X[3 * d + k] = mip[3 + std::min(ec / grp, 2)][k];
         ec++;
       }
       if (cnt[d] == 1)
@@ -438,7 +452,16 @@ void Tree::build(const Space& s, int leafsize, const std::vector<int>* inst) {
   for (int i = 0; i < s.ndof; i++) perm[i] = i;
   cl.clear();
   cl.push_back(Cluster{0, -1, -1, -1, s.ndof, 0, 0.0, {0, 0, 0}});
-  for (size_t ic = 0; ic < cl.size(); ic++) {  // breadth-first, src/hmat.cpp:379-407
+  // breadth-first (src/hmat.cpp:379-407), one level at a time: the clusters of
+  // a level own disjoint ranges of perm, so host threads split them
+  // independently (each cluster's arithmetic as in a sequential sweep, so the
+  // tree is the same); the children are then appended in cluster order
+  struct Split {
+    int kind;  // 0 leaf, 1 split at n1 (PCA), 2 split between instances at n1
+    long n1;
+    int lo, hi;
+  };
+  auto process = [&](size_t ic) -> Split {
     const long nclu = cl[ic].number, b = cl[ic].permupos, e = b + nclu - 1;
     double mom[10] = {0};
     for (long i = b; i <= e; i++) {
@@ -477,30 +500,25 @@ void Tree::build(const Space& s, int leafsize, const std::vector<int>* inst) {
     cl[ic].radius = radius;
     for (int k = 0; k < 3; k++) cl[ic].centre[k] = ctr[k];
     cl[ic].child1 = cl[ic].child2 = -1;
+    Split r{0, 0, 0, 0};
     if (inst) {  // instance range; a cluster over several instances splits between them
       int lo = (*inst)[perm[b]], hi = lo;
       for (long i = b; i <= e; i++) {
         lo = std::min(lo, (*inst)[perm[i]]);
         hi = std::max(hi, (*inst)[perm[i]]);
       }
-      ilo.resize(cl.size());
-      ihi.resize(cl.size());
-      ilo[ic] = lo;
-      ihi[ic] = hi + 1;
+      r.lo = lo;
+      r.hi = hi + 1;
       if (hi > lo) {  // positions are still in dof order here, i.e. sorted by instance
         const int mid = (lo + hi + 1) / 2;
         long split = b;
         while (split <= e && (*inst)[perm[split]] < mid) split++;
-        const long lvl = cl[ic].level;
-        const int n0 = (int)cl.size();
-        cl[ic].child1 = n0;
-        cl[ic].child2 = n0 + 1;
-        cl.push_back(Cluster{lvl + 1, (long)ic, -1, -1, split - b, b, 0.0, {0, 0, 0}});
-        cl.push_back(Cluster{lvl + 1, (long)ic, -1, -1, e + 1 - split, split, 0.0, {0, 0, 0}});
-        continue;
+        r.kind = 2;
+        r.n1 = split - b;
+        return r;
       }
     }
-    if (nclu <= leafsize) continue;
+    if (nclu <= leafsize) return r;
     long num1 = 0, num2 = 0, icheck = b, iend = e;
     const double crit = ctr[0] * ax[0] + ctr[1] * ax[1] + ctr[2] * ax[2];
     for (long i = 0; i < nclu; i++) {
@@ -515,13 +533,43 @@ void Tree::build(const Space& s, int leafsize, const std::vector<int>* inst) {
         iend--;
       }
     }
-    if (num1 == 0 || num2 == 0) continue;  // non-separable: leaf
-    const long lvl = cl[ic].level;
-    const int n0 = (int)cl.size();
-    cl[ic].child1 = n0;
-    cl[ic].child2 = n0 + 1;
-    cl.push_back(Cluster{lvl + 1, (long)ic, -1, -1, num1, b, 0.0, {0, 0, 0}});
-    cl.push_back(Cluster{lvl + 1, (long)ic, -1, -1, num2, b + num1, 0.0, {0, 0, 0}});
+    if (num1 == 0 || num2 == 0) return r;  // non-separable: leaf
+    r.kind = 1;
+    r.n1 = num1;
+    return r;
+  };
+  const int nth = (int)std::max(1u, std::min(16u, std::thread::hardware_concurrency()));
+  for (size_t lo = 0; lo < cl.size();) {
+    const size_t hi = cl.size();
+    std::vector<Split> res(hi - lo);
+    auto work = [&](int t) {
+      for (size_t ic = lo + t; ic < hi; ic += nth) res[ic - lo] = process(ic);
+    };
+    if (hi - lo == 1 || nth == 1) {
+      for (size_t ic = lo; ic < hi; ic++) res[ic - lo] = process(ic);
+    } else {
+      std::vector<std::thread> th;
+      for (int t = 1; t < nth; t++) th.emplace_back(work, t);
+      work(0);
+      for (auto& x : th) x.join();
+    }
+    for (size_t ic = lo; ic < hi; ic++) {
+      const Split& r = res[ic - lo];
+      if (inst) {
+        ilo.resize(cl.size());
+        ihi.resize(cl.size());
+        ilo[ic] = r.lo;
+        ihi[ic] = r.hi;
+      }
+      if (r.kind == 0) continue;
+      const long lvl = cl[ic].level, b = cl[ic].permupos, n = cl[ic].number;
+      const int n0 = (int)cl.size();
+      cl[ic].child1 = n0;
+      cl[ic].child2 = n0 + 1;
+      cl.push_back(Cluster{lvl + 1, (long)ic, -1, -1, r.n1, b, 0.0, {0, 0, 0}});
+      cl.push_back(Cluster{lvl + 1, (long)ic, -1, -1, n - r.n1, b + r.n1, 0.0, {0, 0, 0}});
+    }
+    lo = hi;
   }
   if (inst) {
     ilo.resize(cl.size());

@@ -12,19 +12,27 @@ namespace bem {
 
 static const double kPi = 3.14159265358979323846;
 
-// SP_P0 / SP_L2_1 / SP_L2_2: SurfaceL2 order 0 / 1 / 2 (Dubiner basis in the
-// element's barycentrics x = lambda_0, y = lambda_1; dof (e, 0) = e, the higher
-// ones nt + e (nloc - 1) + k - 1); SP_P1 / SP_P2: H1 order 1 / 2 (vertex hats,
-// then the edge bubbles lambda_a lambda_b of edges (v_l, v_l+1); dofs: vertices,
-// then nv + edge number in sorted (min, max) order);
+// SP_P0 / SP_L2_1 / SP_L2_2 / SP_L2_3: SurfaceL2 order 0-3 (Dubiner basis in
+// the element's barycentrics x = lambda_0, y = lambda_1; dof (e, 0) = e, the
+// higher ones nt + e (nloc - 1) + k - 1); SP_P1 / SP_P2 / SP_P3: H1 order 1-3
+// (vertex hats; per edge (a, b) = (v_l, v_l+1) the bubble lambda_a lambda_b
+// and for order 3 s_l lambda_a lambda_b (lambda_b - lambda_a), s_l = +1 when
+// v_l < v_l+1; for order 3 the face bubble lambda_0 lambda_1 lambda_2; dofs:
+// vertices, then the edges' dofs (p - 1 each, edges in sorted (min, max)
+// order), then the faces');
 // SP_RWG: HDivSurface order 0 (unit-flux Raviart-Thomas on edges);
 // SP_NED: HCurl order 0 on the surface (Whitney/Nedelec edge functions)
-enum SpaceKind { SP_P0 = 0, SP_P1 = 1, SP_RWG = 2, SP_NED = 3, SP_L2_1 = 4, SP_L2_2 = 5, SP_P2 = 6 };
+enum SpaceKind {
+  SP_P0 = 0, SP_P1 = 1, SP_RWG = 2, SP_NED = 3, SP_L2_1 = 4, SP_L2_2 = 5, SP_P2 = 6, SP_L2_3 = 7, SP_P3 = 8
+};
 inline bool is_edge_kind(int k) { return k == SP_RWG || k == SP_NED; }
-inline bool is_h1_kind(int k) { return k == SP_P1 || k == SP_P2; }
+inline bool is_h1_kind(int k) { return k == SP_P1 || k == SP_P2 || k == SP_P3; }
 inline bool is_higher_kind(int k) { return k >= SP_L2_1; }  // evaluated by the generic evaluators
-constexpr int NLOC_MAX = 6;  // local dofs per element, at most (H1 / SurfaceL2 order 2)
-constexpr int DOF_STRIDE = 8;  // element stride of the device dof tables
+inline int nloc_of_kind(int k) {
+  return (k == SP_P0) ? 1 : (k == SP_L2_2 || k == SP_P2) ? 6 : (k == SP_L2_3 || k == SP_P3) ? 10 : 3;
+}
+constexpr int NLOC_MAX = 10;   // local dofs per element, at most (H1 / SurfaceL2 order 3)
+constexpr int DOF_STRIDE = 16;  // element stride of the device dof tables
 // EV_MAXW: [phi, div phi] (DiffOpMaxwellNew on HDivSurface, DiffOpMaxwell on
 // HCurl); EV_TRACE3: vector trace (HDivSurface BND, RotatedTrace on HCurl)
 enum EvalKind { EV_ID = 0, EV_ROT = 1, EV_HELM = 2, EV_MAXW = 3, EV_TRACE3 = 4 };

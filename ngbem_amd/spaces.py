@@ -79,7 +79,7 @@ class FESpace:
                     "bem_space_create")
         self._h = _Handle(out, L.bem_space_destroy)
         self.ndof = int(L.bem_space_ndof(out))
-        tab = np.zeros(6 * mesh.nt, dtype=np.int32)
+        tab = np.zeros(10 * mesh.nt, dtype=np.int32)
         nloc = int(L.bem_space_dofs(out, _capi.iptr(tab)))
         if nloc < 1:
             _capi.check(nloc, "bem_space_dofs")
@@ -125,15 +125,19 @@ def _dubiner(p, x, y):
 
 
 class SurfaceL2(FESpace):
-    """SurfaceL2, orders 0-2: constants (dof = triangle), plus for orders 1, 2
+    """SurfaceL2, orders 0-3: constants (dof = triangle), plus for orders 1-3
     the triangle's higher Dubiner functions (in the element's barycentrics
     x = lambda_0, y = lambda_1; dofs nt + e (nloc - 1) + k - 1)."""
     kind = _capi.SPACE_P0
-    orders = (0, 1, 2)
+    orders = (0, 1, 2, 3)
 
     def __init__(self, mesh, order=0, complex=False, definedon=None, dirichlet=None, dual_mapping=False):
         super().__init__(mesh, order, complex, definedon, dirichlet, dual_mapping)
-        self.ikind = (0, 4, 5)[self.order]
+        self.ikind = (0, 4, 5, 7)[self.order]
+
+    def elem_signs(self):
+        """Per element and local dof: +1 (SurfaceL2 functions are element-local)."""
+        return np.ones(self.elem_dofs.shape)
 
     def shapes(self, bary):
         """Shape values at barycentric points (q x 3): q x nloc."""
@@ -141,21 +145,42 @@ class SurfaceL2(FESpace):
 
 
 class H1(FESpace):
-    """H1, orders 1-2 on the surface: vertex hats lambda_k, plus for order 2
-    the edge bubbles lambda_l lambda_(l+1) (dofs nv + edge number, edges in
-    sorted (min, max) order)."""
+    """H1, orders 1-3 on the surface: vertex hats lambda_k; per edge
+    (a, b) = (v_l, v_l+1) the bubble lambda_a lambda_b and, for order 3,
+    s_l lambda_a lambda_b (lambda_b - lambda_a) with s_l = +1 when v_l < v_l+1;
+    for order 3 the face bubble lambda_0 lambda_1 lambda_2.  Dofs: vertices,
+    then p - 1 per edge (edges in sorted (min, max) order), then the faces."""
     kind = _capi.SPACE_P1
-    orders = (1, 2)
+    orders = (1, 2, 3)
 
     def __init__(self, mesh, order=1, complex=False, definedon=None, dirichlet=None):
         super().__init__(mesh, order, complex, definedon, dirichlet)
-        self.ikind = (None, 1, 6)[self.order]
+        self.ikind = (None, 1, 6, 8)[self.order]
 
     def shapes(self, bary):
-        v = [bary[:, 0], bary[:, 1], bary[:, 2]]
-        if self.order == 2:
-            v += [bary[:, 0] * bary[:, 1], bary[:, 1] * bary[:, 2], bary[:, 2] * bary[:, 0]]
+        """Shape values with every edge oriented +1 (multiply by elem_signs())."""
+        l0, l1, l2 = bary[:, 0], bary[:, 1], bary[:, 2]
+        v = [l0, l1, l2]
+        lam = [l0, l1, l2]
+        for e in range(3):
+            a, b = lam[e], lam[(e + 1) % 3]
+            if self.order >= 2:
+                v.append(a * b)
+            if self.order == 3:
+                v.append(a * b * (b - a))
+        if self.order == 3:
+            v.append(l0 * l1 * l2)
         return np.stack(v, axis=-1)
+
+    def elem_signs(self):
+        """Per element and local dof: the orientation factor of the order-3 edge
+        functions, +1 elsewhere."""
+        sg = np.ones(self.elem_dofs.shape)
+        if self.order == 3:
+            t = self.mesh.tri
+            o = np.where(t < np.roll(t, -1, axis=1), 1.0, -1.0)
+            sg[:, [4, 6, 8]] = o
+        return sg
 
 
 class _EdgeSpace(FESpace):
@@ -229,6 +254,7 @@ def mass_matrix_coo(trial: FESpace, test: FESpace):
         bary = np.stack([x, y, 1.0 - x - y], axis=1)
         G = np.einsum("q,qi,qj->ij", W.ravel(), test.shapes(bary), trial.shapes(bary))
         loc = 2.0 * m.areas()[:, None, None] * G[None]
+        loc = loc * test.elem_signs()[:, :, None] * trial.elem_signs()[:, None, :]
     else:
         area = m.areas()
         ni, nj = test.elem_dofs.shape[1], trial.elem_dofs.shape[1]

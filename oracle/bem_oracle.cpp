@@ -292,7 +292,8 @@ struct Mesh {
 // SP_RWG: HDivSurface order 0 (unit-flux RT0 per edge); SP_NED: HCurl order 0;
 // SP_L2_1 / SP_L2_2: SurfaceL2 order 1 / 2 (Dubiner basis); SP_P2: H1 order 2
 // (vertex hats, then the edge bubbles lambda_a lambda_b)
-enum { SP_P0 = 0, SP_P1 = 1, SP_RWG = 2, SP_NED = 3, SP_L2_1 = 4, SP_L2_2 = 5, SP_P2 = 6 };
+enum { SP_P0 = 0, SP_P1 = 1, SP_RWG = 2, SP_NED = 3, SP_L2_1 = 4, SP_L2_2 = 5, SP_P2 = 6, SP_L2_3 = 7, SP_P3 = 8 };
+static int l2_order(int k) { return k == SP_L2_1 ? 1 : (k == SP_L2_2 ? 2 : (k == SP_L2_3 ? 3 : 0)); }
 // EV_MAXW: [phi, div] (DiffOpMaxwellNew, src/diffops.hpp:263-338; DiffOpMaxwell :170-255)
 // EV_TR3: vector trace (HDivSurface BND; RotatedTrace :62-107 on HCurl)
 enum { EV_ID = 0, EV_ROT = 1, EV_HELM = 2, EV_MAXW = 3, EV_TR3 = 4 };
@@ -315,16 +316,18 @@ struct Space {
     } else if (k == SP_P1) {
       ndof = m.nv; nloc = 3;
       dofs.assign(m.tri.begin(), m.tri.end());
-    } else if (k == SP_L2_1 || k == SP_L2_2) {  // constants e, then the higher functions element by element
-      nloc = (k == SP_L2_1) ? 3 : 6;
+    } else if (k == SP_L2_1 || k == SP_L2_2 || k == SP_L2_3) {  // constants e, then the higher functions
+      const int p = l2_order(k);
+      nloc = (p + 1) * (p + 2) / 2;
       ndof = m.nt * nloc;
       dofs.resize((size_t)m.nt * nloc);
       for (int e = 0; e < m.nt; e++) {
         dofs[(size_t)e * nloc] = e;
         for (int l = 1; l < nloc; l++) dofs[(size_t)e * nloc + l] = m.nt + e * (nloc - 1) + (l - 1);
       }
-    } else if (k == SP_P2) {  // vertices, then the edges (v_l, v_l+1) in (min, max) order
-      nloc = 6;
+    } else if (k == SP_P2 || k == SP_P3) {  // vertices, then the edges (v_l, v_l+1) in (min, max) order, faces
+      const int pe = (k == SP_P2) ? 1 : 2;
+      nloc = 3 + 3 * pe + (pe == 2 ? 1 : 0);
       std::vector<std::pair<int, int>> ed;
       for (int e = 0; e < m.nt; e++)
         for (int l = 0; l < 3; l++) {
@@ -334,13 +337,19 @@ struct Space {
       std::vector<std::pair<int, int>> u(ed);
       std::sort(u.begin(), u.end());
       u.erase(std::unique(u.begin(), u.end()), u.end());
-      ndof = m.nv + (int)u.size();
-      dofs.resize(6 * (size_t)m.nt);
-      for (int e = 0; e < m.nt; e++)
+      const int ne = (int)u.size();
+      ndof = m.nv + pe * ne + (pe == 2 ? m.nt : 0);
+      dofs.resize((size_t)nloc * m.nt);
+      sign.resize(3 * (size_t)m.nt);
+      for (int e = 0; e < m.nt; e++) {
         for (int l = 0; l < 3; l++) {
-          dofs[6 * (size_t)e + l] = m.tri[3 * e + l];
-          dofs[6 * (size_t)e + 3 + l] = m.nv + (int)(std::lower_bound(u.begin(), u.end(), ed[3 * e + l]) - u.begin());
+          dofs[(size_t)nloc * e + l] = m.tri[3 * e + l];
+          const int id = (int)(std::lower_bound(u.begin(), u.end(), ed[3 * e + l]) - u.begin());
+          for (int q = 0; q < pe; q++) dofs[(size_t)nloc * e + 3 + pe * l + q] = m.nv + pe * id + q;
+          sign[3 * e + l] = m.tri[3 * e + l] < m.tri[3 * e + (l + 1) % 3] ? 1 : -1;
         }
+        if (pe == 2) dofs[(size_t)nloc * e + 9] = m.nv + pe * ne + e;
+      }
     } else {  // edges numbered in (min vertex, max vertex) order; local edge l = (v_l, v_{l+1})
       nloc = 3;
       std::vector<std::pair<int, int>> ed;
@@ -421,7 +430,7 @@ static void dubiner(int p, double x, double y, double* out) {
 
 static void eval_shape(const Mesh& m, const Space& s, int ev, int e, double xi, double eta, double* out) {
   if (s.kind == SP_P0) { out[0] = 1.0; return; }
-  if (s.kind == SP_L2_1 || s.kind == SP_L2_2) { dubiner(s.kind == SP_L2_1 ? 1 : 2, xi, eta, out); return; }
+  if (l2_order(s.kind) > 0) { dubiner(l2_order(s.kind), xi, eta, out); return; }
   if (s.kind == SP_RWG || s.kind == SP_NED) {
     // RT0 with unit flux: phi_l = sign_l / J (x - p_opp(l)), div = 2 sign_l / J; the
     // rotated trace of the Whitney function of an HCurl edge is -phi_l
@@ -442,16 +451,31 @@ static void eval_shape(const Mesh& m, const Space& s, int ev, int e, double xi, 
   // the bubbles lambda_a lambda_b of the edges (a, b) = (l, l+1)
   double g[3][3];
   m.grads(e, g);
-  const int nl = s.nloc;
-  double val[6], grad[6][3];
+  // order 3 adds, per edge (a, b), s lambda_a lambda_b (lambda_b - lambda_a) with
+  // s = +1 when the edge runs from the lower to the higher vertex number, and
+  // the face bubble lambda_0 lambda_1 lambda_2
+  const int nl = s.nloc, pe = (nl == 3) ? 0 : (nl == 6 ? 1 : 2);
+  double val[10], grad[10][3];
   for (int k = 0; k < 3; k++) {
     val[k] = lam[k];
     for (int c = 0; c < 3; c++) grad[k][c] = g[k][c];
   }
-  for (int l = 3; l < nl; l++) {
-    const int a = l - 3, b = (l - 2) % 3;
-    val[l] = lam[a] * lam[b];
-    for (int c = 0; c < 3; c++) grad[l][c] = lam[a] * g[b][c] + lam[b] * g[a][c];
+  for (int l = 0; l < 3 && pe > 0; l++) {
+    const int a = l, b = (l + 1) % 3, i2 = 3 + pe * l;
+    val[i2] = lam[a] * lam[b];
+    for (int c = 0; c < 3; c++) grad[i2][c] = lam[a] * g[b][c] + lam[b] * g[a][c];
+    if (pe == 2) {
+      const double sg = s.sign[3 * e + l];
+      val[i2 + 1] = sg * lam[a] * lam[b] * (lam[b] - lam[a]);
+      for (int c = 0; c < 3; c++)
+        grad[i2 + 1][c] = sg * (lam[b] * lam[b] * g[a][c] + 2.0 * lam[a] * lam[b] * g[b][c] -
+                                2.0 * lam[a] * lam[b] * g[a][c] - lam[a] * lam[a] * g[b][c]);
+    }
+  }
+  if (pe == 2) {
+    val[9] = lam[0] * lam[1] * lam[2];
+    for (int c = 0; c < 3; c++)
+      grad[9][c] = lam[1] * lam[2] * g[0][c] + lam[0] * lam[2] * g[1][c] + lam[0] * lam[1] * g[2][c];
   }
   if (ev == EV_ID) { for (int k = 0; k < nl; k++) out[k] = val[k]; return; }
   const double* n = &m.nrm[3 * e];
@@ -624,11 +648,16 @@ static void cluster_data(const Mesh& m, const Space& s, std::vector<double>& G, 
     if (!s.has(e)) continue;
     double mip[7][3];
     for (int j = 0; j < 7; j++) m.point(e, xh[j][0], xh[j][1], mip[j]);
-    int vc = 0, ec = 0;
+    // src/hmat.cpp:96-140: vertex dofs (support > 2) at the vertices in order,
+    // edge dofs (support 2) in groups of (count / 3) at their edge's midpoint,
+    // bubbles at the centroid
+    int vc = 0, ec = 0, nce = 0;
+    for (int l = 0; l < s.nloc; l++) nce += cnt[s.dofs[e * s.nloc + l]] == 2;
+    const int grp = std::max(1, nce / 3);
     for (int l = 0; l < s.nloc; l++) {
       int d = s.dofs[e * s.nloc + l];
-      if (cnt[d] > 2) { for (int k = 0; k < 3; k++) X[3 * d + k] = mip[vc][k]; vc++; }
-      if (cnt[d] == 2) { for (int k = 0; k < 3; k++) X[3 * d + k] = mip[3 + ec][k]; ec++; }  // edge midpoints
+      if (cnt[d] > 2) { for (int k = 0; k < 3; k++) X[3 * d + k] = mip[std::min(vc, 2)][k]; vc++; }
+      if (cnt[d] == 2) { for (int k = 0; k < 3; k++) X[3 * d + k] = mip[3 + std::min(ec / grp, 2)][k]; ec++; }
       if (cnt[d] == 1) for (int k = 0; k < 3; k++) X[3 * d + k] = mip[6][k];
     }
   }
@@ -781,7 +810,7 @@ static void integrate4d(const Op& op, int ei, int ej, const double* xx, const do
   const Mesh& m = *op.mesh;
   int dti = ev_dim(op.K.ev_test), dtj = ev_dim(op.K.ev_trial);
   int ni = op.test.nloc, nj = op.trial.nloc;
-  double si[24], sj[24];
+  double si[40], sj[40];
   cplx kv[3];
   for (int q = 0; q < nq; q++) {
     double X[3], Y[3];
@@ -1029,7 +1058,7 @@ static bool calc_aca(const Op& op, const std::vector<int>& trialdofs, const std:
   V2.assign((size_t)R * ncols, cplx(0.0));
   int ni = op.test.nloc, nj = op.trial.nloc;
   int dti = ev_dim(op.K.ev_test), dtj = ev_dim(op.K.ev_trial);
-  const int SQ = 24;  // nloc * dim <= 6 * 4
+  const int SQ = 40;  // nloc * dim <= 10 * 4
   std::vector<double> shq((size_t)nt * SQ);
   for (size_t a = 0; a < patchi.size(); a++) {
     int e = patchi[a];
@@ -1380,7 +1409,7 @@ int orc_potential(void* p, const double* coef, const double* pts, const double* 
   for (int e = 0; e < m.nt; e++) use[e] = op.trial.has(e) && (!elmask || elmask[e]);
   for (int e = 0; e < m.nt; e++) {
     if (!use[e]) continue;
-    std::vector<double> sh(24);
+    std::vector<double> sh(40);
     for (int q = 0; q < nq; q++) {
       m.point(e, tr.xi[q], tr.eta[q], &Y[3 * ((size_t)e * nq + q)]);
       W[(size_t)e * nq + q] = m.J[e] * tr.w[q];

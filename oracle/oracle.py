@@ -18,7 +18,7 @@ _lib = None
 # kernel ids (same numbering as the product's C-ABI)
 (LAPLACE_SL, LAPLACE_DL, LAPLACE_HS, HELMHOLTZ_SL, HELMHOLTZ_DL, HELMHOLTZ_CF, HELMHOLTZ_HS, MAXWELL_SL,
  MAXWELL_DL, HELMHOLTZ_SL_DKAPPA) = range(10)
-P0, P1, HDIV, HCURL, L2_1, L2_2, P2 = 0, 1, 2, 3, 4, 5, 6  # SurfaceL2 orders 1, 2; H1 order 2
+P0, P1, HDIV, HCURL, L2_1, L2_2, P2, L2_3, P3 = range(9)  # SurfaceL2 orders 1-3; H1 orders 2-3
 
 
 def build():
@@ -157,7 +157,7 @@ class OracleOp:
 
     def elem_dofs(self, test=False):
         """Element -> dof table (nt x nloc), -1 outside the space's region."""
-        out = np.zeros(6 * len(self.mesh.tri), dtype=np.int32)
+        out = np.zeros(10 * len(self.mesh.tri), dtype=np.int32)
         nloc = lib().orc_op_dofs(self.h, int(test), _ip(out))
         return out[: nloc * len(self.mesh.tri)].reshape(len(self.mesh.tri), nloc)
 

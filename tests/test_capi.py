@@ -47,9 +47,9 @@ def test_bad_arguments_are_reported():
         _capi.check(L.bem_op_create(_capi.LAPLACE_HS, 0.0, p0.handle, p0.handle, ctypes.byref(params),
                                     ctypes.byref(out)))
     with pytest.raises(NotImplementedError):
-        nb.H1(m, order=3)
+        nb.H1(m, order=4)
     with pytest.raises(NotImplementedError):
-        nb.SurfaceL2(m, order=3)
+        nb.SurfaceL2(m, order=4)
     with pytest.raises(RuntimeError, match="hypersingular operators need H1"):
         l2 = nb.SurfaceL2(m, order=1)
         _capi.check(L.bem_op_create(_capi.LAPLACE_HS, 0.0, l2.handle, l2.handle, ctypes.byref(params),

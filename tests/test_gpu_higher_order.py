@@ -1,5 +1,5 @@
-"""SurfaceL2 orders 1-2 and H1 order 2 on the GPU, through the C-ABI, against
-the oracle (same meshes and parameters).
+"""SurfaceL2 orders 1-3 and H1 orders 2-3 on the GPU, through the C-ABI,
+against the oracle (same meshes and parameters).
 
 Bars: near-field entries within 1e-12 of the oracle's (max over the block's
 largest entry); each low-rank block as close to the exact block as the
@@ -16,8 +16,8 @@ from oracle import oracle as O
 
 pytestmark = pytest.mark.gpu
 
-SP = {O.P0: (nb.SurfaceL2, 0), O.L2_1: (nb.SurfaceL2, 1), O.L2_2: (nb.SurfaceL2, 2), O.P1: (nb.H1, 1),
-      O.P2: (nb.H1, 2)}
+SP = {O.P0: (nb.SurfaceL2, 0), O.L2_1: (nb.SurfaceL2, 1), O.L2_2: (nb.SurfaceL2, 2), O.L2_3: (nb.SurfaceL2, 3),
+      O.P1: (nb.H1, 1), O.P2: (nb.H1, 2), O.P3: (nb.H1, 3)}
 # (name, kernel, trial, test, kappa)
 CASES = [
     ("LSL", O.LAPLACE_SL, O.L2_1, O.L2_1, 0.0),
@@ -29,6 +29,11 @@ CASES = [
     ("HSL", O.HELMHOLTZ_SL, O.L2_1, O.L2_1, 3.0),
     ("HCF", O.HELMHOLTZ_CF, O.L2_2, O.L2_1, 3.0),
     ("HHS", O.HELMHOLTZ_HS, O.P2, O.P2, 3.0),
+    # order 3: the reference's tests/test_Laplace.py pairs H1 order 3 with SurfaceL2 order 2
+    ("LSL", O.LAPLACE_SL, O.L2_3, O.L2_3, 0.0),
+    ("LDL", O.LAPLACE_DL, O.P3, O.L2_2, 0.0),
+    ("LHS", O.LAPLACE_HS, O.P3, O.P3, 0.0),
+    ("HHS", O.HELMHOLTZ_HS, O.P3, O.P3, 3.0),
 ]
 
 
@@ -93,7 +98,7 @@ def test_higher_order_blocks_and_apply_match_oracle(gpu, case):
 
 
 @pytest.mark.parametrize("kid,trk,tek", [(O.LAPLACE_SL, O.L2_2, O.L2_2), (O.HELMHOLTZ_DL, O.P2, O.L2_1),
-                                         (O.HELMHOLTZ_HS, O.P2, O.P2)])
+                                         (O.HELMHOLTZ_HS, O.P2, O.P2), (O.HELMHOLTZ_DL, O.P3, O.L2_2)])
 def test_higher_order_potential_matches_oracle(gpu, kid, trk, tek):
     m = nb.icosphere(2)
     tr = SP[trk][0](m, order=SP[trk][1])

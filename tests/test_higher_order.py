@@ -1,5 +1,5 @@
-"""SurfaceL2 orders 1-2 and H1 order 2, on the CPU: the oracle's operators and
-the product's host side.
+"""SurfaceL2 orders 1-3 and H1 orders 2-3, on the CPU: the oracle's operators
+and the product's host side.
 
 Pins of the oracle:
 - inclusion: the constants of SurfaceL2 order p and the vertex hats of H1
@@ -50,17 +50,57 @@ def test_lower_order_block_is_the_lower_order_operator(kern, lo, hi):
     assert np.abs(D1).max() > 0
 
 
-def test_hypersingular_order2_annihilates_constants():
+# local index of each order-(p-1) Dubiner function inside the order-p list
+# (i outer, j inner): (i, j) keeps its function, only its position moves
+def _dubiner_embed(p):
+    pos = {}
+    n = 0
+    for i in range(p + 1):
+        for j in range(p - i + 1):
+            pos[(i, j)] = n
+            n += 1
+    lower = [(i, j) for i in range(p) for j in range(p - 1 - i + 1)]
+    return [pos[ij] for ij in lower]
+
+
+def _embed_dofs(lo_tab, hi_tab, local):
+    """global dof of the lower space -> global dof of the higher space"""
+    mp = {}
+    for e in range(lo_tab.shape[0]):
+        for k, kk in enumerate(local):
+            mp[lo_tab[e, k]] = hi_tab[e, kk]
+    return np.array([mp[d] for d in range(len(mp))])
+
+
+@pytest.mark.parametrize("kern,sp", [(O.LAPLACE_SL, "L2"), (O.HELMHOLTZ_SL, "L2"), (O.LAPLACE_HS, "H1"),
+                                     (O.HELMHOLTZ_HS, "H1")])
+def test_order3_contains_order2(kern, sp):
+    """The order-3 spaces contain the order-2 functions unchanged, so the
+    order-2 operator is a sub-matrix of the order-3 one."""
+    m = nb.icosphere(1)
+    kap = 2.0 if kern >= O.HELMHOLTZ_SL else 0.0
+    lo, hi = (O.L2_2, O.L2_3) if sp == "L2" else (O.P2, O.P3)
+    r2 = O.OracleOp(m, kern, kap, trial=lo, test=lo, intorder=3)
+    r3 = O.OracleOp(m, kern, kap, trial=hi, test=hi, intorder=3)
+    local = _dubiner_embed(3) if sp == "L2" else [0, 1, 2, 3, 5, 7]
+    idx = _embed_dofs(r2.elem_dofs(), r3.elem_dofs(), local)
+    D2, D3 = r2.dense(), r3.dense()
+    assert np.abs(D3[np.ix_(idx, idx)] - D2).max() <= 1e-14 * np.abs(D2).max()
+
+
+@pytest.mark.parametrize("order", [2, 3])
+def test_hypersingular_annihilates_constants(order):
     m = nb.icosphere(2)
+    k = {2: O.P2, 3: O.P3}[order]
     for kern, kap in ((O.LAPLACE_HS, 0.0), (O.HELMHOLTZ_HS, 2.0)):
-        ref = O.OracleOp(m, kern, kap, trial=O.P2, test=O.P2, intorder=3)
+        ref = O.OracleOp(m, kern, kap, trial=k, test=k, intorder=3)
         W = ref.dense()
         one = np.zeros(ref.ntrial)
         one[:m.nv] = 1.0
         if kern == O.LAPLACE_HS:
             assert np.abs(W @ one).max() <= 1e-12 * np.abs(W).max()
-        # a P2 space holds nv + ne dofs
-        assert ref.ntrial == m.nv + len(m.edges())
+        ne = len(m.edges())
+        assert ref.ntrial == m.nv + (order - 1) * ne + (m.nt if order == 3 else 0)
 
 
 def _far_pair(m):
@@ -70,11 +110,11 @@ def _far_pair(m):
     return e, f
 
 
-@pytest.mark.parametrize("space", ["L2_1", "L2_2"])
+@pytest.mark.parametrize("space", ["L2_1", "L2_2", "L2_3"])
 def test_far_entries_against_numpy_quadrature(space):
     m = nb.icosphere(1)
-    k = {"L2_1": O.L2_1, "L2_2": O.L2_2}[space]
-    order = {"L2_1": 1, "L2_2": 2}[space]
+    k = {"L2_1": O.L2_1, "L2_2": O.L2_2, "L2_3": O.L2_3}[space]
+    order = {"L2_1": 1, "L2_2": 2, "L2_3": 3}[space]
     sp = nb.SurfaceL2(m, order=order)
     e, f = _far_pair(m)
     bary, w = _collapsed_rule(8)
@@ -98,27 +138,32 @@ def test_far_entries_against_numpy_quadrature(space):
         assert np.abs(Do - Dn).max() <= 1e-10 * np.abs(Dn).max(), kern
 
 
-def test_far_edge_bubble_entries_against_numpy_quadrature():
-    """H1 order 2: single layer between an edge bubble (two triangles) and a
-    far triangle's SurfaceL2 order-1 functions (double layer, P2 -> L2_1)."""
+@pytest.mark.parametrize("order,local", [(2, 3), (3, 3), (3, 4), (3, 9)])
+def test_far_edge_bubble_entries_against_numpy_quadrature(order, local):
+    """H1 orders 2-3: double layer (H1 -> SurfaceL2 order 1) between an edge
+    or face function of triangle e (order 3: the oriented cubic edge function
+    on two triangles, the face bubble on one) and a far triangle."""
     m = nb.icosphere(1)
-    h2, l1 = nb.H1(m, order=2), nb.SurfaceL2(m, order=1)
+    h2, l1 = nb.H1(m, order=order), nb.SurfaceL2(m, order=1)
     e, f = _far_pair(m)
-    dof = h2.elem_dofs[e, 3]  # bubble of edge (v0, v1) of triangle e
-    supp = [(t, l) for t in range(m.nt) for l in range(6) if h2.elem_dofs[t, l] == dof]
-    assert len(supp) == 2
-    bary, w = _collapsed_rule(8)
+    dof = h2.elem_dofs[e, local]
+    nl = h2.elem_dofs.shape[1]
+    supp = [(t, l) for t in range(m.nt) for l in range(nl) if h2.elem_dofs[t, l] == dof]
+    assert len(supp) == (1 if local == 9 else 2)
+    sg = h2.elem_signs()
+    bary, w = _collapsed_rule(12)
     P = m.xyz[m.tri]
     J = 2.0 * m.areas()
     Y = bary @ P[f]
     psi = l1.shapes(bary)
     ny = m.normals()[f]
-    ref = O.OracleOp(m, O.LAPLACE_DL, 0.0, trial=O.P2, test=O.L2_1, intorder=12)
+    # the cubic edge function integrates to nearly zero: a high order on both sides
+    ref = O.OracleOp(m, O.LAPLACE_DL, 0.0, trial=h2.ikind, test=O.L2_1, intorder=20)
     Do = ref.dense(np.array([dof]), l1.elem_dofs[f])[:, 0]
     Dn = np.zeros(3)
     for t, l in supp:
         X = bary @ P[t]
-        phi = h2.shapes(bary)[:, l]
+        phi = h2.shapes(bary)[:, l] * sg[t, l]
         d = Y[:, None, :] - X[None, :, :]  # test point minus trial point
         r = np.linalg.norm(d, axis=-1)
         # double layer kernel of the reference: n_y . (x - y) / (4 pi |x - y|^3),
@@ -128,7 +173,8 @@ def test_far_edge_bubble_entries_against_numpy_quadrature():
     assert np.abs(Do - Dn).max() <= 1e-10 * np.abs(Dn).max()
 
 
-@pytest.mark.parametrize("cls,order", [(nb.SurfaceL2, 1), (nb.SurfaceL2, 2), (nb.H1, 2)])
+@pytest.mark.parametrize("cls,order", [(nb.SurfaceL2, 1), (nb.SurfaceL2, 2), (nb.SurfaceL2, 3), (nb.H1, 2),
+                                       (nb.H1, 3)])
 def test_projection_reproduces_polynomials(cls, order):
     """int (f - f_h) v = 0 with f of degree `order` in the element's own
     barycentrics: f_h = f at every quadrature point (SurfaceL2: per element;
@@ -141,24 +187,27 @@ def test_projection_reproduces_polynomials(cls, order):
     bary, w = _collapsed_rule(6)
     P = m.xyz[m.tri]
     J = 2.0 * m.areas()
-    phi = sp.shapes(bary)
+    phi0 = sp.shapes(bary)
+    sg = sp.elem_signs()
 
     def f(X):
-        return 0.3 + X[..., 0] - 0.7 * X[..., 1] + (order >= 2) * (X[..., 0] * X[..., 2] - 0.4 * X[..., 1] ** 2)
+        return (0.3 + X[..., 0] - 0.7 * X[..., 1] + (order >= 2) * (X[..., 0] * X[..., 2] - 0.4 * X[..., 1] ** 2)
+                + (order >= 3) * (0.5 * X[..., 0] ** 3 - X[..., 0] * X[..., 1] * X[..., 2]))
 
     b = np.zeros(sp.ndof)
     for t in range(m.nt):
         X = bary @ P[t]
-        np.add.at(b, sp.elem_dofs[t], J[t] * np.einsum("q,q,qa->a", w, f(X), phi))
+        np.add.at(b, sp.elem_dofs[t], J[t] * np.einsum("q,q,qa->a", w, f(X), phi0 * sg[t]))
     coef = np.linalg.solve(M, b)
     err = 0.0
     for t in range(m.nt):
         X = bary @ P[t]
-        err = max(err, np.abs(phi @ coef[sp.elem_dofs[t]] - f(X)).max())
-    assert err < 1e-11
+        err = max(err, np.abs((phi0 * sg[t]) @ coef[sp.elem_dofs[t]] - f(X)).max())
+    assert err < 1e-10
 
 
-@pytest.mark.parametrize("cls,order,ik", [(nb.SurfaceL2, 1, O.L2_1), (nb.SurfaceL2, 2, O.L2_2), (nb.H1, 2, O.P2)])
+@pytest.mark.parametrize("cls,order,ik", [(nb.SurfaceL2, 1, O.L2_1), (nb.SurfaceL2, 2, O.L2_2),
+                                          (nb.SurfaceL2, 3, O.L2_3), (nb.H1, 2, O.P2), (nb.H1, 3, O.P3)])
 def test_numbering_matches_oracle(cls, order, ik):
     m = nb.icosphere(2)
     sp = cls(m, order=order)
@@ -166,7 +215,7 @@ def test_numbering_matches_oracle(cls, order, ik):
     assert sp.ndof == ref.ntrial and sp.ikind == ik
     assert np.array_equal(sp.elem_dofs, ref.elem_dofs())
     if cls is nb.H1:
-        assert sp.ndof == m.nv + len(m.edges())
+        assert sp.ndof == m.nv + (order - 1) * len(m.edges()) + (m.nt if order == 3 else 0)
         assert np.array_equal(sp.elem_dofs[:, :3], m.tri)
     else:
         assert sp.ndof == m.nt * ((order + 1) * (order + 2) // 2)
@@ -174,11 +223,13 @@ def test_numbering_matches_oracle(cls, order, ik):
 
 
 @pytest.mark.parametrize("trk,tek,kern", [(O.L2_1, O.L2_1, _capi.LAPLACE_SL), (O.P2, O.L2_1, _capi.LAPLACE_DL),
-                                          (O.P2, O.P2, _capi.LAPLACE_HS), (O.L2_2, O.P0, _capi.HELMHOLTZ_SL)])
+                                          (O.P2, O.P2, _capi.LAPLACE_HS), (O.L2_2, O.P0, _capi.HELMHOLTZ_SL),
+                                          (O.P3, O.L2_2, _capi.LAPLACE_DL), (O.L2_3, O.L2_3, _capi.LAPLACE_SL)])
 def test_tree_identical_to_oracle(trk, tek, kern):
     m = nb.icosphere(3)
     mk = {O.P0: lambda: nb.SurfaceL2(m), O.L2_1: lambda: nb.SurfaceL2(m, order=1),
-          O.L2_2: lambda: nb.SurfaceL2(m, order=2), O.P2: lambda: nb.H1(m, order=2)}
+          O.L2_2: lambda: nb.SurfaceL2(m, order=2), O.P2: lambda: nb.H1(m, order=2),
+          O.L2_3: lambda: nb.SurfaceL2(m, order=3), O.P3: lambda: nb.H1(m, order=3)}
     tr, te = mk[trk](), mk[tek]()
     L = _capi.lib()
     params = _capi.Params(3, 32, 2.0, 1e-6, 0, 0)
@@ -199,7 +250,8 @@ def test_tree_identical_to_oracle(trk, tek, kern):
 
 
 @pytest.mark.parametrize("kid,trk,tek", [(O.HELMHOLTZ_HS, O.P2, O.P2), (O.LAPLACE_HS, O.P2, O.P2),
-                                         (O.LAPLACE_DL, O.P2, O.L2_2), (O.HELMHOLTZ_SL, O.L2_2, O.L2_1)])
+                                         (O.LAPLACE_DL, O.P2, O.L2_2), (O.HELMHOLTZ_SL, O.L2_2, O.L2_1),
+                                         (O.HELMHOLTZ_HS, O.P3, O.P3), (O.LAPLACE_DL, O.P3, O.L2_3)])
 def test_oracle_aca_of_higher_order_blocks(kid, trk, tek):
     """The oracle's cross approximation projects onto up to 6 functions x 4
     evaluator components per triangle; every far block against its dense form."""
