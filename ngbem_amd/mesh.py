@@ -168,6 +168,50 @@ def ellipsoid(level: int, axes=(1.0, 1.0, 1.0)) -> Mesh:
     return Mesh(m.xyz * np.asarray(axes, dtype=np.float64)[None, :], m.tri)
 
 
+def box(n: int, size=(1.0, 1.0, 1.0), centre=(0.0, 0.0, 0.0)) -> Mesh:
+    """Surface of an axis-parallel box, each face a structured n x n grid cut
+    into two triangles per cell, oriented outward; faces are regions 1-6
+    named "left", "right", "front", "back", "bottom", "top" (-x, +x, -y, +y,
+    -z, +z).  Flat triangles represent the box exactly."""
+    n = int(n)
+    if n < 1:
+        raise ValueError("box needs n >= 1")
+    h = np.asarray(size, dtype=np.float64)
+    c = np.asarray(centre, dtype=np.float64)
+    t = np.linspace(-0.5, 0.5, n + 1)
+    pts, tris, regions = [], [], []
+    index = {}
+
+    def vid(p):
+        key = tuple(np.round(p, 12))
+        if key not in index:
+            index[key] = len(pts)
+            pts.append(p)
+        return index[key]
+
+    # (axis, side): the face's outward normal is side * e_axis
+    for r, (ax, side) in enumerate(((0, -1), (0, 1), (1, -1), (1, 1), (2, -1), (2, 1))):
+        u, v = [a for a in range(3) if a != ax]
+        ids = np.zeros((n + 1, n + 1), dtype=np.int64)
+        for i in range(n + 1):
+            for j in range(n + 1):
+                p = np.zeros(3)
+                p[ax] = 0.5 * side
+                p[u], p[v] = t[i], t[j]
+                ids[i, j] = vid(p * h + c)
+        for i in range(n):
+            for j in range(n):
+                a, b, cc, d = ids[i, j], ids[i + 1, j], ids[i + 1, j + 1], ids[i, j + 1]
+                # (p0 - p2) x (p1 - p2) = e_u x e_v = sign * e_ax; flip for outward
+                if np.cross(np.eye(3)[u], np.eye(3)[v])[ax] * side > 0:
+                    tris += [[a, b, cc], [a, cc, d]]
+                else:
+                    tris += [[a, cc, b], [a, d, cc]]
+                regions += [r + 1, r + 1]
+    names = {1: "left", 2: "right", 3: "front", 4: "back", 5: "bottom", 6: "top"}
+    return Mesh(np.array(pts), np.array(tris), regions, names)
+
+
 # ------------------------------------------------------------ file formats --
 # Surface triangle meshes from files.  The reference reads geometry through
 # NGSolve/Netgen (e.g. the ship STL of demos/Maxwell_Ship.ipynb); here the

@@ -117,3 +117,18 @@ def test_ship_surface():
     dl = O.OracleOp(ship, O.LAPLACE_DL, 0.0, O.P0, O.P0, intorder=8, leafsize=10**6)
     u = dl.potential(np.ones(ship.nt), np.array([centroid, far]))[:, 0].real
     assert abs(u[0] + 1.0) < 1e-3 and abs(u[1]) < 1e-6
+
+
+def test_box_mesh():
+    m = nb.box(3, size=(2.0, 1.0, 0.5), centre=(0.1, 0.0, -0.2))
+    assert m.nt == 6 * 2 * 9 and m.nv == 6 * 16 - 12 * 4 + 8
+    p0, p1, p2 = (m.xyz[m.tri[:, k]] for k in range(3))
+    assert np.isclose(np.einsum("ij,ij->i", p0, np.cross(p1, p2)).sum() / 6.0, 1.0)
+    assert np.isclose(m.areas().sum(), 2 * (2.0 * 1.0 + 2.0 * 0.5 + 1.0 * 0.5))
+    c = m.xyz[m.tri].mean(1) - np.array([0.1, 0.0, -0.2])
+    assert np.all(np.einsum("ij,ij->i", c, m.normals()) > 0)
+    e = np.sort(np.concatenate([m.tri[:, [0, 1]], m.tri[:, [1, 2]], m.tri[:, [2, 0]]]), axis=1)
+    _, cnt = np.unique(e, axis=0, return_counts=True)
+    assert np.all(cnt == 2)
+    assert m.GetBoundaries() == ("left", "right", "front", "back", "bottom", "top")
+    assert m.Boundaries("top").mask.sum() == 18
