@@ -84,7 +84,7 @@ def lib():
     L.bem_op_create_multi.argtypes = [ctypes.c_int, dp, ctypes.c_int, ip, P, P, ctypes.POINTER(Params),
                                       ctypes.POINTER(P)]
     L.bem_op_assemble.argtypes = [P, P]
-    L.bem_op_apply.argtypes = [P, ctypes.c_int, dp, P, dp, P, ctypes.c_int, ctypes.c_int, ctypes.c_int, P]
+    L.bem_op_apply.argtypes = [P, ctypes.c_int, P, P, P, P, ctypes.c_int, ctypes.c_int, ctypes.c_int, P]
     L.bem_op_stats.argtypes = [P, ctypes.POINTER(Stats)]
     L.bem_op_profile_apply.argtypes = [P, ctypes.c_int, ctypes.c_int, dp, P]
     L.bem_op_destroy.argtypes = [P]

@@ -35,6 +35,20 @@ def device() -> torch.device:
     return torch.device("cuda", torch.cuda.current_device())
 
 
+_SCALARS = {}
+
+
+def _scalar(v):
+    """(re, im) of a scalar as a float64 array; the common ones are cached."""
+    v = complex(v)
+    a = _SCALARS.get(v)
+    if a is None:
+        a = np.array([v.real, v.imag], dtype=np.float64)
+        if len(_SCALARS) < 64:
+            _SCALARS[v] = a
+    return a
+
+
 def _as_vec(x, dtype, dev):
     if isinstance(x, np.ndarray):
         x = torch.from_numpy(np.ascontiguousarray(x))
@@ -283,6 +297,15 @@ class HMatrix(BaseMatrix):
     def MultAdd(self, s, x, y):
         self._apply(0, s, x, y)
 
+    def Mult(self, x, y):
+        # y = A x in one call (beta = 0): no separate clearing of y
+        if (self.is_complex and torch.is_complex(x) and x.dtype == y.dtype == torch.complex128 and x.is_cuda
+                and y.is_cuda and x.dim() == 1 and x.shape[0] == self.width and y.shape[0] == self.height
+                and x.is_contiguous() and y.is_contiguous()):
+            self._op._call_apply(_capi.APPLY_MIXED if self._mixed else 0, 1.0, x, 0.0, y)
+            return
+        BaseMatrix.Mult(self, x, y)
+
     def MultTransAdd(self, s, x, y):
         self._apply(_capi.APPLY_TRANS, s, x, y)
 
@@ -440,8 +463,8 @@ class IntegralOperator:
     def _call_apply(self, flags, alpha, x, beta, y):
         """x, y: device vectors, or batches (k, n) of k row vectors (the forward
         product of a batch runs as one block product on the matrix cores)."""
-        a = np.array([complex(alpha).real, complex(alpha).imag], dtype=np.float64)
-        b = np.array([complex(beta).real, complex(beta).imag], dtype=np.float64)
+        a = _scalar(alpha)
+        b = _scalar(beta)
         dtype = _capi.C128 if self.is_complex else _capi.F64
         nrhs, ld = 1, 0
         xb, yb = x, y
@@ -454,12 +477,17 @@ class IntegralOperator:
                 yb = torch.zeros(nrhs, ld, dtype=y.dtype, device=y.device)
                 if b.any():
                     yb[:, :nout] = y
-        with torch.cuda.device(y.device):
+        L = _capi.lib()
+        if y.device.index == torch.cuda.current_device():  # no device switch needed
             stream = torch.cuda.current_stream().cuda_stream
-            _capi.check(_capi.lib().bem_op_apply(self._h, flags | _capi.APPLY_ON_DEVICE, _capi.dptr(a),
-                                                 ctypes.c_void_p(xb.data_ptr()), _capi.dptr(b),
-                                                 ctypes.c_void_p(yb.data_ptr()), nrhs, ld, dtype,
-                                                 ctypes.c_void_p(stream)), "bem_op_apply")
+            rc = L.bem_op_apply(self._h, flags | _capi.APPLY_ON_DEVICE, a.ctypes.data, xb.data_ptr(), b.ctypes.data,
+                                yb.data_ptr(), nrhs, ld, dtype, stream)
+        else:
+            with torch.cuda.device(y.device):
+                stream = torch.cuda.current_stream().cuda_stream
+                rc = L.bem_op_apply(self._h, flags | _capi.APPLY_ON_DEVICE, a.ctypes.data, xb.data_ptr(),
+                                    b.ctypes.data, yb.data_ptr(), nrhs, ld, dtype, stream)
+        _capi.check(rc, "bem_op_apply")
         if yb is not y:
             y.copy_(yb[:, : y.shape[1]])
 
