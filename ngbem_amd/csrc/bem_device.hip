@@ -26,6 +26,7 @@
 #include <cmath>
 #include <cstring>
 #include <numeric>
+#include <type_traits>
 
 #include "bem_op.hpp"
 
@@ -1289,16 +1290,92 @@ __global__ void k_scatter(const int* __restrict__ perm, int n, const VT* __restr
 // cross-lane reduction is needed.
 
 // t_b = V_b x[c0 : c0+n] (forward)  or  t_b = Ut_b x[r0 : r0+m] (transpose).
-// Work items are waves: a long row (>= 256 entries) gets a whole wave, short
-// rows go 64 / lanes to a wave (lanes per row: 16 by default); an item holds
-// all a wave needs (first row's element offset in the factor array, row
-// length, x offset, t index of the first row, rows, lanes), so one load
-// precedes the data loads.  Each lane keeps eight loads in flight, the last
-// ones predicated off past the row end.
+// A work item is up to NR rows of one block's factor (row-major, consecutive
+// rows), run by one wave: lane l takes the columns j = l + 64 c, loads x[j]
+// once and multiplies it into all NR rows, KC column chunks per round, so a
+// lane has NR * KC factor loads in flight (16 entries of 8 bytes or less, 8 of
+// complex128).  The NR row sums are then reduced across the wave by a
+// butterfly that halves the values per lane at each of the first log2(NR)
+// steps (row r ends in lane r * 64 / NR).  (NR, KC) is picked per item by the
+// host from the row length and stored in `lanes`; the item holds all a wave
+// needs (first row's entry offset in the factor array, row length, x offset,
+// t index of the first row, rows), read through the scalar cache.
 struct RowItem {
   int64_t off;
   int len, xoff, toff, nrows, lanes, pad;
 };
+
+template <int NR, int KC, class ST, class VT, bool CONJ>
+__device__ __forceinline__ VT far_rows(const RowItem& it, int lane, const ST* __restrict__ F,
+                                       const VT* __restrict__ xp) {
+  VT acc[NR];
+#pragma unroll
+  for (int r = 0; r < NR; r++) acc[r] = zero_<VT>();
+  const int len = it.len, nrows = it.nrows;
+  const ST* base = F + it.off;
+  const VT* xx = xp + it.xoff;
+  for (int c0 = 0; c0 < len; c0 += 64 * KC) {
+    VT xv[KC];
+    ST v[NR][KC];
+#pragma unroll
+    for (int c = 0; c < KC; c++) {
+      const int j = c0 + lane + 64 * c;
+      xv[c] = j < len ? xx[j] : zero_<VT>();
+#pragma unroll
+      for (int r = 0; r < NR; r++) v[r][c] = (r < nrows && j < len) ? base[(int64_t)r * len + j] : zero_<ST>();
+    }
+#pragma unroll
+    for (int r = 0; r < NR; r++)
+#pragma unroll
+      for (int c = 0; c < KC; c++) {
+        VT u;
+        cvt(u, v[r][c]);
+        if (CONJ) u = cconj(u);
+        acc[r] += u * xv[c];
+      }
+  }
+  int h = NR;
+#pragma unroll
+  for (int m = 32; m >= 64 / NR; m >>= 1) {
+    h >>= 1;
+    const bool up = (lane & m) != 0;
+#pragma unroll
+    for (int i = 0; i < NR / 2; i++) {
+      if (i < h) {
+        const VT keep = up ? acc[i + h] : acc[i];
+        const VT send = up ? acc[i] : acc[i + h];
+        acc[i] = keep + shfl_xor(send, m);
+      }
+    }
+  }
+#pragma unroll
+  for (int m = 32 / NR; m >= 1; m >>= 1) acc[0] += shfl_xor(acc[0], m);
+  return acc[0];
+}
+
+template <class ST, class VT, bool CONJ>
+__global__ __launch_bounds__(256) void k_mv_far_t(const RowItem* __restrict__ items, int nitems,
+                                                  const ST* __restrict__ F, const VT* __restrict__ xp,
+                                                  VT* __restrict__ tbuf) {
+  constexpr bool WIDE = sizeof(ST) >= 16;  // complex128: 8 loads in flight, else 16
+  const int stride = gridDim.x * 4;
+  int w = __builtin_amdgcn_readfirstlane(blockIdx.x * 4 + (threadIdx.x >> 6));
+  const int lane = threadIdx.x & 63;
+  for (; w < nitems; w += stride) {
+    const RowItem it = items[w];
+    VT r;
+    int grp;
+    if (WIDE) {
+      if (it.lanes == 1) { r = far_rows<8, 1, ST, VT, CONJ>(it, lane, F, xp); grp = 8; }
+      else { r = far_rows<4, 2, ST, VT, CONJ>(it, lane, F, xp); grp = 16; }
+    } else {
+      if (it.lanes == 2) { r = far_rows<8, 2, ST, VT, CONJ>(it, lane, F, xp); grp = 8; }
+      else { r = far_rows<4, 4, ST, VT, CONJ>(it, lane, F, xp); grp = 16; }
+    }
+    const int row = lane / grp;
+    if (lane == row * grp && row < it.nrows) tbuf[it.toff + row] = r;
+  }
+}
 
 // Workgroup b of a grid of G runs on XCD b % 8 (round robin); this maps it to
 // the logical index that gives every XCD one contiguous range of G / 8 work
@@ -1310,47 +1387,6 @@ __device__ __forceinline__ int xcd_major(int b, int G, int remap) {
   if (!remap) return b;
   const int q = G >> 3, rem = G & 7, x = b & 7, k = b >> 3;
   return x * q + min(x, rem) + k;
-}
-
-template <class ST, class VT, bool CONJ>
-__global__ __launch_bounds__(256) void k_mv_far_t(const RowItem* __restrict__ items, int nitems,
-                                                  const ST* __restrict__ F, const VT* __restrict__ xp,
-                                                  VT* __restrict__ tbuf) {
-  const int w = blockIdx.x * 4 + (threadIdx.x >> 6);
-  if (w >= nitems) return;
-  const RowItem it = items[w];
-  const int lane = threadIdx.x & 63;
-  const int lanes = it.lanes;
-  const int g = lane / lanes, li = lane - g * lanes;
-  const bool act = g < it.nrows;
-  const int len = it.len;
-  const ST* row = F + it.off + (int64_t)(act ? g : 0) * len;
-  const VT* xx = xp + it.xoff;
-  VT a[8];
-#pragma unroll
-  for (int k = 0; k < 8; k++) a[k] = zero_<VT>();
-  for (int j0 = li; j0 < len; j0 += 8 * lanes) {
-    VT u[8], xv[8];
-#pragma unroll
-    for (int k = 0; k < 8; k++) {
-      const int j = j0 + k * lanes;
-      u[k] = zero_<VT>();
-      xv[k] = zero_<VT>();
-      if (j < len) {
-        cvt(u[k], row[j]);
-        xv[k] = xx[j];
-      }
-    }
-#pragma unroll
-    for (int k = 0; k < 8; k++) a[k] += (CONJ ? cconj(u[k]) : u[k]) * xv[k];
-  }
-  VT r = ((a[0] + a[1]) + (a[2] + a[3])) + ((a[4] + a[5]) + (a[6] + a[7]));
-  if (lanes == 64) {
-    r = wave_sum(r);
-  } else {
-    for (int m = lanes >> 1; m >= 1; m >>= 1) r += shfl_xor(r, m);
-  }
-  if (li == 0 && act) tbuf[it.toff + g] = r;
 }
 
 // z_b = U_b t_b (forward) or V_b^T t_b (transpose, conjugated for the
@@ -2530,33 +2566,40 @@ static void plan(const Operator& op, bool trans, std::vector<Tile>& tiles, std::
 }
 
 // low-rank B^T x work items of every far block (forward: rows of V times
-// x[c0 ..], transpose: rows of Ut times x[r0 ..])
-static int far_lanes() {
-  static const int l = [] {
-    const char* e = std::getenv("NGBEM_FAR_LANES");
-    const int v = e ? std::atoi(e) : 16;
-    return (v == 4 || v == 8 || v == 16 || v == 32) ? v : 16;
-  }();
-  return l;
+// x[c0 ..], transpose: rows of Ut times x[r0 ..]), one list per storage
+// precision.  Rows per item and column chunks per round (NR, KC) of
+// k_mv_far_t: complex128 storage (8 loads in flight) (8, 1) for rows up to 64
+// entries, else (4, 2); narrower storage (16 loads) (8, 2) up to 128 entries,
+// else (4, 4).  Longer rows take several rounds.
+static void far_variant(int len, bool wide, int& kc, int& nr) {
+  if (wide) {
+    kc = len <= 64 ? 1 : 2;
+    nr = len <= 64 ? 8 : 4;
+  } else {
+    kc = len <= 128 ? 2 : 4;
+    nr = len <= 128 ? 8 : 4;
+  }
 }
 
 static void upload_row_items(Operator& op, hipStream_t s) {
-  const int L = far_lanes(), per = 64 / L;
-  for (int trans = 0; trans < 2; trans++) {
-    std::vector<RowItem> items;
-    for (size_t k = 0; k < op.far.size(); k++) {
-      const FarBlk& fb = op.far[k];
-      const int len = trans ? fb.m : fb.n;
-      const int64_t base = trans ? fb.uoff : fb.voff;
-      const int xoff = trans ? fb.r0 : fb.c0;
-      if (len >= 256)
-        for (int l = 0; l < fb.R; l++) items.push_back(RowItem{base + (int64_t)l * len, len, xoff, fb.toff + l, 1, 64, 0});
-      else
-        for (int l = 0; l < fb.R; l += per)
-          items.push_back(RowItem{base + (int64_t)l * len, len, xoff, fb.toff + l, std::min(per, fb.R - l), L, 0});
+  const int esz = op.complex_valued ? 16 : 8;
+  for (int prec = 0; prec < 2; prec++) {
+    const bool wide = (prec ? esz / 2 : esz) >= 16;
+    for (int trans = 0; trans < 2; trans++) {
+      std::vector<RowItem> items;
+      for (size_t k = 0; k < op.far.size(); k++) {
+        const FarBlk& fb = op.far[k];
+        const int len = trans ? fb.m : fb.n;
+        const int64_t base = trans ? fb.uoff : fb.voff;
+        const int xoff = trans ? fb.r0 : fb.c0;
+        int kc, nr;
+        far_variant(len, wide, kc, nr);
+        for (int l = 0; l < fb.R; l += nr)
+          items.push_back(RowItem{base + (int64_t)l * len, len, xoff, fb.toff + l, std::min(nr, fb.R - l), kc, 0});
+      }
+      op.d_items[prec][trans].upload(items, s);
+      op.n_items[prec][trans] = (int)items.size();
     }
-    (trans ? op.d_items_trn : op.d_items_fwd).upload(items, s);
-    (trans ? op.n_items_trn : op.n_items_fwd) = (int)items.size();
   }
 }
 
@@ -3154,6 +3197,17 @@ static bool rows_flat() {
   return flat;
 }
 
+// t = B^T x of every far block through k_mv_far_t, with the item list of the
+// storage precision of F
+template <class ST, class VT, bool CONJ>
+static void launch_far_t(const Operator& op, int trans, const ST* F, const VT* xp, VT* tb, hipStream_t s) {
+  constexpr int prec = (std::is_same<ST, fc>::value || std::is_same<ST, float>::value) ? 1 : 0;
+  const int n = op.n_items[prec][trans];
+  if (!n) return;
+  hipLaunchKernelGGL((k_mv_far_t<ST, VT, CONJ>), dim3((n + 3) / 4), dim3(256), 0, s,
+                     op.d_items[prec][trans].as<RowItem>(), n, F, xp, tb);
+}
+
 template <class ST, class VT, class XT>
 static void apply_t(Operator& op, int flags, const XT* x, XT* y, XT alpha, XT beta, bool beta_zero, const ST* nval,
                     const ST* fU, const ST* fV, hipStream_t s, hipEvent_t* ev) {
@@ -3179,8 +3233,7 @@ static void apply_t(Operator& op, int flags, const XT* x, XT* y, XT alpha, XT be
     const int pk = mv_plan();
     const bool zpath = pk == PLAN_Z;
     if (nfar) {
-      hipLaunchKernelGGL((k_mv_far_t<ST, VT, false>), dim3((op.n_items_fwd + 3) / 4), dim3(256), 0, s,
-                         op.d_items_fwd.as<RowItem>(), op.n_items_fwd, fV, xp, tb);
+      launch_far_t<ST, VT, false>(op, 0, fV, xp, tb, s);
       mark(2);
       if (zpath)
         hipLaunchKernelGGL((k_mv_far_z<ST, VT, false, false>), dim3((op.n_zitems_fwd + 3) / 4), dim3(256), 0, s,
@@ -3199,11 +3252,9 @@ static void apply_t(Operator& op, int flags, const XT* x, XT* y, XT alpha, XT be
   } else {
     if (nfar) {
       if (conj)
-        hipLaunchKernelGGL((k_mv_far_t<ST, VT, true>), dim3((op.n_items_trn + 3) / 4), dim3(256), 0, s,
-                           op.d_items_trn.as<RowItem>(), op.n_items_trn, fU, xp, tb);
+        launch_far_t<ST, VT, true>(op, 1, fU, xp, tb, s);
       else
-        hipLaunchKernelGGL((k_mv_far_t<ST, VT, false>), dim3((op.n_items_trn + 3) / 4), dim3(256), 0, s,
-                           op.d_items_trn.as<RowItem>(), op.n_items_trn, fU, xp, tb);
+        launch_far_t<ST, VT, false>(op, 1, fU, xp, tb, s);
       mark(2);
       if (conj)
         hipLaunchKernelGGL((k_mv_far_z<ST, VT, true, true>), dim3((op.n_zitems_trn + 3) / 4), dim3(256), 0, s,

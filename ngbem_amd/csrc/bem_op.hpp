@@ -248,8 +248,8 @@ struct Operator {
   int n_zitems_fwd = 0, n_zitems_trn = 0;
   int z_total = 0, zT_total = 0;    // scratch lengths for U t / V^T t
   DBuf d_z, d_zT;
-  DBuf d_items_fwd, d_items_trn;  // low-rank B^T x work items
-  int n_items_fwd = 0, n_items_trn = 0;
+  DBuf d_items[2][2];  // low-rank B^T x work items [FP64, FP32 copy][forward, transpose]
+  int n_items[2][2] = {{0, 0}, {0, 0}};
   int t_total = 0;         // scratch length for far B^T x
   DBuf d_xp, d_yp, d_t;    // scratch vectors (complex128 or double)
   // block apply (64 right-hand sides): work items built on first use, slabs
