@@ -271,6 +271,12 @@ def main():
                      "apply_frac_mixed": apply_bytes(8) / (kms["mixed"]["apply"] * 1e-3) / 1e9 / HBM_PEAK_GBS},
     }
 
+    # what a plain streaming kernel reaches on this GPU: torch's copy of 1 GiB
+    # (bytes read + written / time), for the frac's context
+    probe = hbm_probe(dev)
+    out["roofline"]["hbm_probe"] = probe
+    log(f"HBM probe: torch copy {probe['copy_gbs']:.0f} GB/s")
+
     if not args.no_solve:
         # Burton-Miller combined-field solve: (1/2 M + C) u = f; FP64 GMRES vs
         # mixed inner GMRES with FP64 iterative refinement
@@ -326,6 +332,26 @@ def main():
         torch.mul(a, 2.0, out=b)
         torch.cuda.synchronize()
     print(json.dumps(out), flush=True)
+
+
+def hbm_probe(dev, reps=10):
+    """Streaming bandwidth of torch's copy kernel: 1 GiB read and 1 GiB
+    written per call, (bytes read + written) / time over `reps` calls."""
+    import torch
+    a = torch.ones(256 << 20, dtype=torch.float32, device=dev)
+    b = torch.empty_like(a)
+    b.copy_(a)
+    torch.cuda.synchronize()
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    e0.record()
+    for _ in range(reps):
+        b.copy_(a)
+    e1.record()
+    torch.cuda.synchronize()
+    gbs = 2 * a.numel() * 4 * reps / (e0.elapsed_time(e1) * 1e-3) / 1e9
+    del a, b
+    torch.cuda.empty_cache()
+    return {"copy_gbs": gbs, "how": "torch copy_ of 1 GiB float32, 10 calls, HIP events"}
 
 
 def bench_maxwell_block(args):
