@@ -20,12 +20,15 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <atomic>
 #include <chrono>
 #include <cstdio>
 #include <cstdlib>
 #include <cmath>
 #include <cstring>
+#include <exception>
 #include <numeric>
+#include <thread>
 #include <type_traits>
 
 #include "bem_op.hpp"
@@ -1258,6 +1261,55 @@ __global__ __launch_bounds__(256) void k_project(const ProjTask* __restrict__ ta
   }
 }
 
+// Both sides SurfaceL2 order 0 (one element per dof, shape value 1): one
+// thread per patch element a of a block side, all of that side's columns in
+// a loop.  The element's point weights J w are loaded once, consecutive
+// threads read consecutive segments W[l][a ntq .. a ntq + ntq) of the
+// workspace, and the dof's row / column position comes from the dof table and
+// the inverse permutation.  The sums are those of k_project term by term
+// (shape 1.0 times J w), so the factors are bit-identical.
+constexpr int PROJ_P0_NTQ = 16;  // point rules up to 16 points
+
+template <class KER>
+__global__ __launch_bounds__(256) void k_project_p0(const ProjTask* __restrict__ tasks, const int4* __restrict__ items,
+                                                    const FarBlk* __restrict__ fb, const int* __restrict__ fpatch,
+                                                    const int* __restrict__ dofs_test, const int* __restrict__ dofs_trial,
+                                                    const int* __restrict__ iperm_test,
+                                                    const int* __restrict__ iperm_trial, const double* __restrict__ quad,
+                                                    int ntq, KER ker, const typename KER::T* __restrict__ wsU,
+                                                    const typename KER::T* __restrict__ wsV,
+                                                    typename KER::T* __restrict__ oU, typename KER::T* __restrict__ oV) {
+  typedef typename KER::T T;
+  const int4 it = items[blockIdx.x];
+  const ProjTask tk = tasks[it.x];
+  const FarBlk b = fb[tk.blk];
+  const bool isU = it.y == 0;
+  const int len = isU ? b.m : b.n;
+  const int a = it.z + (int)threadIdx.x;
+  if (a >= len) return;
+  const int e = fpatch[(isU ? b.pI : b.pJ) + a];
+  const int dof = (isU ? dofs_test : dofs_trial)[(size_t)e * DOF_STRIDE];
+  const int pos = (isU ? iperm_test : iperm_trial)[dof] - (isU ? b.r0 : b.c0);
+  if (pos < 0 || pos >= len) return;  // (a patch element's dof lies in the block's range)
+  double w[PROJ_P0_NTQ];
+#pragma unroll
+  for (int q = 0; q < PROJ_P0_NTQ; q++) w[q] = q < ntq ? 1.0 * quad[((size_t)e * ntq + q) * 4 + 3] : 0.0;
+  const int np = isU ? tk.np : tk.nq;
+  int col = 0;
+#pragma unroll
+  for (int t = 0; t < KER::NTERM; t++) {
+    const TermC tm = ker.term(t);
+    const int rk = tk.rank[tm.kc];
+    const T* W = (isU ? wsU + tk.uoff[tm.kc] : wsV + tk.voff[tm.kc]) + (size_t)a * ntq;
+    for (int l = 0; l < rk; l++, col++, W += np) {
+      T s = zero_<T>();
+      for (int q = 0; q < ntq; q++) s += rmul(W[q], w[q]);
+      if (isU) oU[tk.ou + (size_t)col * b.m + pos] = rmul(s, tm.fac);
+      else oV[tk.ov + (size_t)col * b.n + pos] = s;
+    }
+  }
+}
+
 // ---------------------------------------------------------------- matvec --
 
 template <class VT, class XT>
@@ -1905,6 +1957,7 @@ static void dispatch_full(const Operator& op, F&& f) {
 // ------------------------------------------------------------------ setup --
 
 void setup_device(Operator& op, hipStream_t s) {
+  PhaseTimer pt_("setup");
   const Mesh& m = *op.mesh;
   const int nt = m.nt, ntq = op.trig.size();
   std::vector<double> geo((size_t)nt * GEO, 0.0), quad((size_t)nt * ntq * 4), sdat((size_t)nt * SDS, 0.0);
@@ -1949,6 +2002,7 @@ void setup_device(Operator& op, hipStream_t s) {
       for (int l = 0; l < sp.nloc; l++) t[DOF_STRIDE * (size_t)e + l] = sp.dofs[(size_t)e * sp.nloc + l];
     return t;
   };
+  pt_.mark("host tables");
   op.d_dofs_test.upload(dof_table(*op.test), s);
   op.d_dofs_trial.upload(dof_table(*op.trial), s);
   std::vector<double> trig((size_t)ntq * 3);
@@ -1981,6 +2035,7 @@ void setup_device(Operator& op, hipStream_t s) {
   op.d_perm_test.upload(op.test_tree->perm, s);
   op.d_iperm_test.upload(op.test_tree->iperm, s);
   BEM_HIP(hipStreamSynchronize(s));
+  pt_.mark("uploads");
 }
 
 // ------------------------------------------------------------- near field --
@@ -2158,6 +2213,15 @@ void assemble_far(Operator& op, hipStream_t s) {
   PhaseTimer pt_("far");
   // patches live in d_patch (test list, then the trial list when the spaces differ)
   const int joff = op.same_space ? 0 : ci.off[tt.cl.size()];
+  {  // the row and column maps of every far block, built in parallel
+    std::vector<int> rows(nfar), cols(nfar);
+    for (int k = 0; k < nfar; k++) {
+      rows[k] = op.blocks[op.far_block[k]].row;
+      cols[k] = op.blocks[op.far_block[k]].col;
+    }
+    ci.map_many(te, tt, rows);
+    cj.map_many(tr, rt, cols);
+  }
   for (int k = 0; k < nfar; k++) {
     FarBlk& fb = op.far[k];
     const Block& bl = op.blocks[op.far_block[k]];
@@ -2183,7 +2247,10 @@ void assemble_far(Operator& op, hipStream_t s) {
   BEM_HIP(hipMemGetInfo(&free_b, &total_b));
   const size_t esz = op.complex_valued ? 16 : 8;
   // workspace budget: a third of the free memory, at most NGBEM_ACA_BUDGET_GB
-  // (default 16): blocks beyond it go in further batches that reuse the buffer
+  // (default 16): blocks beyond it go in further batches that reuse the
+  // buffer.  (The Helmholtz sphere at 80k triangles needs ~61 GB: one batch
+  // cut its far field from 0.222 to 0.205 s, but freeing 61 GB took from a few
+  // to over 200 ms.)
   size_t budget = std::max<size_t>(free_b / 3, (size_t)256 << 20);
   {
     const char* be = std::getenv("NGBEM_ACA_BUDGET_GB");
@@ -2447,9 +2514,36 @@ void assemble_far(Operator& op, hipStream_t s) {
         for (int st = 0; st < total; st += PROJ_CHUNK) items.push_back(make_int2((int)t, st));
       }
       DBuf d_items;
-      d_items.upload(items, s);
       DBuf& SU = bufU(w);
       DBuf& SV = bufV(w);
+      // both sides SurfaceL2 order 0 with one patch element per dof: k_project_p0
+      bool p0 = op.test->kind == SP_P0 && op.trial->kind == SP_P0 && ntq <= PROJ_P0_NTQ;
+      for (size_t t = 0; p0 && t < ptasks.size(); t++) {
+        const FarBlk& fb = op.far[ptasks[t].blk];
+        p0 = fb.pm == fb.m && fb.pn == fb.n;
+      }
+      if (p0) {
+        std::vector<int4> it4;
+        for (size_t t = 0; t < ptasks.size(); t++) {
+          const FarBlk& fb = op.far[ptasks[t].blk];
+          for (int a0 = 0; a0 < fb.m; a0 += 256) it4.push_back(make_int4((int)t, 0, a0, 0));
+          for (int a0 = 0; a0 < fb.n; a0 += 256) it4.push_back(make_int4((int)t, 1, a0, 0));
+        }
+        d_items.upload(it4, s);
+        if (!it4.empty())
+          dispatch_kernel(op.kernel, op.kappa, [&](auto ker) {
+            using KER = decltype(ker);
+            using T = typename KER::T;
+            hipLaunchKernelGGL((k_project_p0<KER>), dim3((unsigned)it4.size()), dim3(256), 0, s,
+                               d_ptasks.as<ProjTask>(), d_items.as<int4>(), op.d_far_blk.as<FarBlk>(),
+                               op.d_patch.as<int>(), op.d_dofs_test.as<int>(), op.d_dofs_trial.as<int>(),
+                               op.d_iperm_test.as<int>(), op.d_iperm_trial.as<int>(), op.d_quad.as<double>(), ntq,
+                               ker, SU.as<T>(), SV.as<T>(), OU.as<T>(), OV.as<T>());
+          });
+        items.clear();
+      } else {
+        d_items.upload(items, s);
+      }
       if (!items.empty())
         dispatch_full(op, [&](auto ker, auto ei, auto ej) {
           using KER = decltype(ker);
@@ -2581,12 +2675,13 @@ static void far_variant(int len, bool wide, int& kc, int& nr) {
   }
 }
 
-static void upload_row_items(Operator& op, hipStream_t s) {
+static void build_row_items(const Operator& op, std::vector<RowItem> (&items)[2][2]) {
   const int esz = op.complex_valued ? 16 : 8;
   for (int prec = 0; prec < 2; prec++) {
     const bool wide = (prec ? esz / 2 : esz) >= 16;
     for (int trans = 0; trans < 2; trans++) {
-      std::vector<RowItem> items;
+      std::vector<RowItem>& it = items[prec][trans];
+      it.clear();
       for (size_t k = 0; k < op.far.size(); k++) {
         const FarBlk& fb = op.far[k];
         const int len = trans ? fb.m : fb.n;
@@ -2595,12 +2690,25 @@ static void upload_row_items(Operator& op, hipStream_t s) {
         int kc, nr;
         far_variant(len, wide, kc, nr);
         for (int l = 0; l < fb.R; l += nr)
-          items.push_back(RowItem{base + (int64_t)l * len, len, xoff, fb.toff + l, std::min(nr, fb.R - l), kc, 0});
+          it.push_back(RowItem{base + (int64_t)l * len, len, xoff, fb.toff + l, std::min(nr, fb.R - l), kc, 0});
       }
-      op.d_items[prec][trans].upload(items, s);
-      op.n_items[prec][trans] = (int)items.size();
     }
   }
+}
+
+static void upload_row_items(Operator& op, const std::vector<RowItem> (&items)[2][2], hipStream_t s) {
+  for (int prec = 0; prec < 2; prec++)
+    for (int trans = 0; trans < 2; trans++) {
+      op.d_items[prec][trans].upload(items[prec][trans], s);
+      op.n_items[prec][trans] = (int)items[prec][trans].size();
+    }
+}
+
+static void upload_row_items(Operator& op, hipStream_t s) {
+  std::vector<RowItem> items[2][2];
+  build_row_items(op, items);
+  upload_row_items(op, items, s);
+  BEM_HIP(hipStreamSynchronize(s));
 }
 
 // z offsets of the low-rank blocks and the U t / V^T t work items
@@ -2627,58 +2735,108 @@ static void upload_z_items(Operator& op, hipStream_t s) {
 }
 
 // flat column lists of the forward Ut plan (k_mv_rows_flat); tiles of one
-// leaf share the leaf's list.  Left empty when an offset needs more than 31 bits.
-static void upload_flat_columns(Operator& op, const std::vector<Tile>& tiles, const std::vector<Seg>& segs,
-                                hipStream_t s) {
-  std::vector<int2> cols, range(tiles.size());
-  int ls0 = -1, ls1 = -1, c0 = 0;
-  for (size_t t = 0; t < tiles.size(); t++) {
-    const Tile& tl = tiles[t];
-    if (tl.s0 != ls0 || tl.s1 != ls1) {
-      c0 = (int)cols.size();
-      for (int k = tl.s0; k < tl.s1; k++) {
-        const Seg& sg = segs[k];
-        if (sg.far == 2) throw std::runtime_error("flat column plan: z segment in the Ut plan");
-        for (int j = 0; j < sg.cnt; j++) {
-          const int64_t off = sg.base + (int64_t)j * sg.ld;
-          const int64_t xi = (int64_t)sg.xoff + j;
-          if (off + sg.ld > (int64_t)INT32_MAX || xi >= COL_FAR) {
-            op.d_fcols.release();
-            op.d_frange.release();
-            return;
+// leaf share the leaf's list.  Lists are sized first and filled on host
+// threads.  False (no lists) when an offset needs more than 31 bits.
+static bool flat_columns(const std::vector<Tile>& tiles, const std::vector<Seg>& segs, std::vector<int2>& cols,
+                         std::vector<int2>& range) {
+  std::vector<int> first;  // first tile of each distinct list
+  for (size_t t = 0; t < tiles.size(); t++)
+    if (t == 0 || tiles[t].s0 != tiles[t - 1].s0 || tiles[t].s1 != tiles[t - 1].s1) first.push_back((int)t);
+  const int nl = (int)first.size();
+  std::vector<int64_t> start(nl + 1, 0);
+  for (int i = 0; i < nl; i++) {
+    const Tile& tl = tiles[first[i]];
+    int64_t cnt = 0;
+    for (int k = tl.s0; k < tl.s1; k++) {
+      if (segs[k].far == 2) throw std::runtime_error("flat column plan: z segment in the Ut plan");
+      cnt += segs[k].cnt;
+    }
+    start[i + 1] = start[i] + cnt;
+  }
+  if (start[nl] >= (int64_t)INT32_MAX) return false;
+  cols.assign(std::max<int64_t>(start[nl], 1), make_int2(0, 0));
+  std::atomic<bool> too_big(false);
+  const int nth = (int)std::max(1u, std::min(16u, std::thread::hardware_concurrency()));
+  std::vector<std::thread> th;
+  for (int w = 0; w < nth; w++)
+    th.emplace_back([&, w] {
+      for (int i = w; i < nl; i += nth) {
+        const Tile& tl = tiles[first[i]];
+        int64_t c = start[i];
+        for (int k = tl.s0; k < tl.s1; k++) {
+          const Seg& sg = segs[k];
+          for (int j = 0; j < sg.cnt; j++) {
+            const int64_t off = sg.base + (int64_t)j * sg.ld;
+            const int64_t xi = (int64_t)sg.xoff + j;
+            if (off + sg.ld > (int64_t)INT32_MAX || xi >= COL_FAR) {
+              too_big = true;
+              return;
+            }
+            cols[c++] = make_int2((int)off, (int)xi | (sg.far ? COL_FAR : 0));
           }
-          cols.push_back(make_int2((int)off, (int)xi | (sg.far ? COL_FAR : 0)));
         }
       }
-      ls0 = tl.s0;
-      ls1 = tl.s1;
-    }
-    range[t] = make_int2(c0, (int)cols.size());
+    });
+  for (auto& x : th) x.join();
+  if (too_big) return false;
+  range.assign(tiles.size(), make_int2(0, 0));
+  for (int i = 0; i < nl; i++) {
+    const int t1 = i + 1 < nl ? first[i + 1] : (int)tiles.size();
+    for (int t = first[i]; t < t1; t++) range[t] = make_int2((int)start[i], (int)start[i + 1]);
   }
-  if (cols.empty()) cols.push_back(make_int2(0, 0));
-  op.d_fcols.upload(cols, s);
-  op.d_frange.upload(range, s);
-  BEM_HIP(hipStreamSynchronize(s));
+  return true;
 }
 
+// the apply's tables: z items first (they set the blocks' z offsets), then
+// the Ut plan with its flat column lists, the z plan, the transposed plan and
+// the low-rank row items, built on host threads at once and uploaded after
 void build_matvec_plan(Operator& op, hipStream_t s) {
+  PhaseTimer pt_("plan");
   upload_z_items(op, s);
-  std::vector<Tile> tiles;
-  std::vector<Seg> segs;
-  plan(op, false, tiles, segs, PLAN_UT);
-  op.d_seg_fwd_mm.upload(segs, s);
-  BEM_HIP(hipStreamSynchronize(s));
-  upload_flat_columns(op, tiles, segs, s);
-  plan(op, false, tiles, segs);
-  op.d_tile_fwd.upload(tiles, s);
-  op.d_seg_fwd.upload(segs, s);
-  op.n_tiles_fwd = (int)tiles.size();
-  BEM_HIP(hipStreamSynchronize(s));  // the host vectors are reused below
-  plan(op, true, tiles, segs);
-  op.d_tile_trn.upload(tiles, s);
-  op.d_seg_trn.upload(segs, s);
-  op.n_tiles_trn = (int)tiles.size();
-  upload_row_items(op, s);
+  pt_.mark("z items");
+  std::vector<Tile> t_ut, t_fwd, t_trn;
+  std::vector<Seg> s_ut, s_fwd, s_trn;
+  std::vector<int2> cols, range;
+  bool flat_ok = false;
+  std::vector<RowItem> items[2][2];
+  std::exception_ptr err[4];
+  auto guarded = [&](int i, auto fn) {
+    return std::thread([&, i, fn] {
+      try {
+        fn();
+      } catch (...) {
+        err[i] = std::current_exception();
+      }
+    });
+  };
+  std::thread th[4] = {
+      guarded(0, [&] {
+        plan(op, false, t_ut, s_ut, PLAN_UT);
+        flat_ok = flat_columns(t_ut, s_ut, cols, range);
+      }),
+      guarded(1, [&] { plan(op, false, t_fwd, s_fwd); }),
+      guarded(2, [&] { plan(op, true, t_trn, s_trn); }),
+      guarded(3, [&] { build_row_items(op, items); })};
+  for (auto& t : th) t.join();
+  for (auto& e : err)
+    if (e) std::rethrow_exception(e);
+  pt_.mark("host tables");
+  op.d_seg_fwd_mm.upload(s_ut, s);
+  if (flat_ok) {
+    op.d_fcols.upload(cols, s);
+    op.d_frange.upload(range, s);
+  } else {
+    op.d_fcols.release();
+    op.d_frange.release();
+  }
+  op.d_tile_fwd.upload(t_fwd, s);
+  op.d_seg_fwd.upload(s_fwd, s);
+  op.n_tiles_fwd = (int)t_fwd.size();
+  op.d_tile_trn.upload(t_trn, s);
+  op.d_seg_trn.upload(s_trn, s);
+  op.n_tiles_trn = (int)t_trn.size();
+  upload_row_items(op, items, s);
+  pt_.mark("uploads");
   const size_t esz = op.complex_valued ? 16 : 8;
   const int nmax = std::max(op.ndof_trial(), op.ndof_test());
   op.d_xp.alloc((size_t)nmax * esz);
@@ -2687,6 +2845,7 @@ void build_matvec_plan(Operator& op, hipStream_t s) {
   op.d_z.alloc((size_t)std::max(op.z_total, 1) * esz);
   op.d_zT.alloc((size_t)std::max(op.zT_total, 1) * esz);
   BEM_HIP(hipStreamSynchronize(s));
+  pt_.mark("scratch + sync");
 }
 
 void make_mixed_copy(Operator& op, hipStream_t s) {

@@ -686,16 +686,17 @@ void ClusterPatches::build(const Space& s, const Tree& t) {
   maps.clear();
 }
 
-int ClusterPatches::map_of(const Space& s, const Tree& t, int c) {
-  if (moff[c] >= 0) return moff[c];
+// one cluster's map: number+1 offsets (relative to the entries), then the
+// entries a*DOF_STRIDE+k
+static void cluster_map(const ClusterPatches& cp, const Space& s, const Tree& t, int c, std::vector<int>& out) {
   const Cluster& cl = t.cl[c];
-  const int len = (int)cl.number, base = (int)maps.size();
-  const int* P = &list[off[c]];
-  const int np = size(c);
-  maps.resize(base + len + 1);
+  const int len = (int)cl.number;
+  const int* P = &cp.list[cp.off[c]];
+  const int np = cp.size(c);
+  out.assign(len + 1, 0);
   std::vector<int> ents;
   for (int ii = 0; ii < len; ii++) {
-    maps[base + ii] = (int)ents.size();
+    out[ii] = (int)ents.size();
     const int d = t.perm[cl.permupos + ii];
     for (int q = s.ed_off[d]; q < s.ed_off[d + 1]; q++) {
       const int e = s.ed_list[q];
@@ -706,10 +707,41 @@ int ClusterPatches::map_of(const Space& s, const Tree& t, int c) {
       ents.push_back(a * DOF_STRIDE + kl);
     }
   }
-  maps[base + len] = (int)ents.size();
-  maps.insert(maps.end(), ents.begin(), ents.end());
+  out[len] = (int)ents.size();
+  out.insert(out.end(), ents.begin(), ents.end());
+}
+
+int ClusterPatches::map_of(const Space& s, const Tree& t, int c) {
+  if (moff[c] >= 0) return moff[c];
+  std::vector<int> m;
+  cluster_map(*this, s, t, c, m);
+  const int base = (int)maps.size();
+  maps.insert(maps.end(), m.begin(), m.end());
   moff[c] = base;
   return base;
+}
+
+void ClusterPatches::map_many(const Space& s, const Tree& t, const std::vector<int>& cl) {
+  std::vector<int> todo;
+  std::vector<char> seen(moff.size(), 0);
+  for (int c : cl)
+    if (moff[c] < 0 && !seen[c]) {
+      seen[c] = 1;
+      todo.push_back(c);
+    }
+  if (todo.empty()) return;
+  std::vector<std::vector<int>> part(todo.size());
+  const int nth = (int)std::max(1u, std::min(16u, std::thread::hardware_concurrency()));
+  std::vector<std::thread> th;
+  for (int w = 0; w < nth; w++)
+    th.emplace_back([&, w] {
+      for (size_t i = w; i < todo.size(); i += nth) cluster_map(*this, s, t, todo[i], part[i]);
+    });
+  for (auto& x : th) x.join();
+  for (size_t i = 0; i < todo.size(); i++) {
+    moff[todo[i]] = (int)maps.size();
+    maps.insert(maps.end(), part[i].begin(), part[i].end());
+  }
 }
 
 void BlockLookup::build(const Tree& rows, const std::vector<Block>& blocks) {

@@ -118,13 +118,17 @@ void patch(const Space& s, const int* dofs, int ndofs, std::vector<int>& out);
 // Patches of every cluster of a tree (sorted unique elements touching the
 // cluster's dofs, src/ngbem.cpp:415-443), built bottom-up by merging the
 // children's lists, and on request the dof -> (patch index, local dof) maps the
-// far-field projection needs (layout: number+1 offsets, then entries a*4+k).
+// far-field projection needs (layout: number+1 offsets, then entries
+// a*DOF_STRIDE+k).
 struct ClusterPatches {
   std::vector<int> off, list;
   std::vector<int> moff;  // per cluster offset into maps, -1 if not built
   std::vector<int> maps;
   void build(const Space& s, const Tree& t);
   int map_of(const Space& s, const Tree& t, int c);
+  // the maps of clusters cl[0], cl[1], ... not built yet, on host threads;
+  // the same layout as map_of called in that order
+  void map_many(const Space& s, const Tree& t, const std::vector<int>& cl);
   int size(int c) const { return off[c + 1] - off[c]; }
 };
 
