@@ -165,8 +165,8 @@ def test_hmatrix_vs_dense(gpu):
 
 @pytest.mark.parametrize("kind", ["LSL-P0", "LDL-P1P0", "LHS-P1"])
 def test_mixed_real_matches_fp64(gpu, kind):
-    """FP32 factor copies (4-byte elements, 16-byte chunk loads in the
-    forward low-rank product, rows starting at every alignment)."""
+    """FP32 factor copies of real operators (4-byte elements in the
+    row-blocked low-rank product and the row tiles)."""
     mesh = nb.icosphere(3)
     kw = dict(intorder=3, leafsize=24, eps=1e-7, mixed=True)
     if kind == "LSL-P0":
@@ -247,3 +247,4 @@ def test_gram_aca_option(gpu, case, monkeypatch):
         terms = {"HCF": 1, "LHS": 3, "MSL": 4}[case[0]]
         close += abs(UV[0].shape[1] - info["rank"]) <= terms
     assert nfar > 0 and close >= 0.9 * nfar
+
