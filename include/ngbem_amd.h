@@ -43,8 +43,11 @@ enum {
   BEM_SPACE_P1 = 1,    /* H1, order 1-3: one dof per vertex (barycentric hats); order 2 adds one edge
                           bubble lambda_a lambda_b per edge, order 3 a second, oriented edge function per
                           edge and a face bubble (numbered vertices, edges, faces) */
-  BEM_SPACE_HDIV = 2,  /* HDivSurface, order 0: one dof per edge, unit-flux RT0 */
-  BEM_SPACE_HCURL = 3  /* HCurl, order 0, on the surface: one dof per edge, Whitney */
+  BEM_SPACE_HDIV = 2,  /* HDivSurface, order 0-1: order 0 one dof per edge, unit-flux RT0; order 1 adds
+                          per edge the surface curl n x grad(lambda_a lambda_b) of the edge bubble (dofs:
+                          the order-0 ones, then ne + edge; element-local 2l / 2l+1) */
+  BEM_SPACE_HCURL = 3  /* HCurl, order 0-1, on the surface: order 0 one dof per edge, Whitney; order 1
+                          adds per edge the gradient grad(lambda_a lambda_b) of the edge bubble */
 };
 
 /* kernel ids (src/ngbem.hpp:230-510); evaluators follow the factories:

@@ -130,15 +130,17 @@ int bem_mesh_destroy(bem_mesh* m) {
 int bem_space_create_on(bem_mesh* m, int kind, int order, const unsigned char* elmask, bem_space** out) {
   if (!m || !out) return fail(-1, "bem_space_create: bad arguments");
   if (!((kind == BEM_SPACE_P0 && order >= 0 && order <= 3) || (kind == BEM_SPACE_P1 && order >= 1 && order <= 3) ||
-        (kind == BEM_SPACE_HDIV && order == 0) || (kind == BEM_SPACE_HCURL && order == 0)))
+        (kind == BEM_SPACE_HDIV && order >= 0 && order <= 1) || (kind == BEM_SPACE_HCURL && order >= 0 && order <= 1)))
     return fail(-1, "bem_space_create: supported spaces are SurfaceL2 orders 0-3, H1 orders 1-3, HDivSurface "
-                    "order 0 and HCurl order 0");
+                    "orders 0-1 and HCurl orders 0-1");
   int ik = kind;  // internal kind
   if (kind == BEM_SPACE_P0) {
     static const int k0[4] = {bem::SP_P0, bem::SP_L2_1, bem::SP_L2_2, bem::SP_L2_3};
     ik = k0[order];
   }
   if (kind == BEM_SPACE_P1) ik = order == 1 ? bem::SP_P1 : (order == 2 ? bem::SP_P2 : bem::SP_P3);
+  if (kind == BEM_SPACE_HDIV) ik = order == 0 ? bem::SP_RWG : bem::SP_RWG1;
+  if (kind == BEM_SPACE_HCURL) ik = order == 0 ? bem::SP_NED : bem::SP_NED1;
   if (elmask) {
     bool any = false;
     for (int e = 0; e < m->m->nt && !any; e++) any = elmask[e] != 0;
@@ -186,9 +188,10 @@ static int create_op(int kernel, double kappa, bem_space* trial, bem_space* test
     const bool scalar = !bem::is_edge_kind(tk) && !bem::is_edge_kind(sk);
     if ((kernel <= bem::K_HHS || kernel == bem::K_HSLK) && !scalar)
       return fail(-1, "bem_op_create: Laplace/Helmholtz kernels need SurfaceL2 / H1 spaces");
-    if (kernel == bem::K_MSL && !((tk == bem::SP_RWG && sk == bem::SP_RWG) || (tk == bem::SP_NED && sk == bem::SP_NED)))
-      return fail(-1, "bem_op_create: Maxwell single layer needs HDivSurface (or HCurl for the curl form) on both sides");
-    if (kernel == bem::K_MDL && !(tk == bem::SP_NED && sk == bem::SP_RWG))
+    if (kernel == bem::K_MSL && !(tk == sk && bem::is_edge_kind(tk)))
+      return fail(-1, "bem_op_create: Maxwell single layer needs the same HDivSurface (or HCurl, for the curl form) "
+                      "space on both sides");
+    if (kernel == bem::K_MDL && !(bem::is_hcurl_kind(tk) && bem::is_hdiv_kind(sk)))
       return fail(-1, "bem_op_create: Maxwell double layer needs an HCurl trial and an HDivSurface test space");
   }
   return guarded([&] {

@@ -455,6 +455,44 @@ template <> struct Ev<SP_NED, EV_MAXW> : EvEdge<-1, true> {};
 template <> struct Ev<SP_RWG, EV_TRACE3> : EvEdge<1, false> {};
 template <> struct Ev<SP_NED, EV_TRACE3> : EvEdge<-1, false> {};
 
+// order-1 edge elements, local rows 2l / 2l+1 for edge l = (a, b) = (l, l+1):
+// the order-0 function of EvEdge, and psi_l = lambda_a c_b + lambda_b c_a with
+// c_k = n x grad(lambda_k) (element data slots 0-8): for HDivSurface psi_l is
+// the basis function itself (surface curl of lambda_a lambda_b, div 0), for
+// HCurl the function is grad(lambda_a lambda_b), whose rotated trace n x . is
+// psi_l and whose surface curl is 0.
+template <int SGN, bool DIV>
+struct EvEdge1 {
+  static constexpr int NL = 6, DIM = DIV ? 4 : 3, NB = 21;
+  static constexpr bool MULTI = true, PERM = false;
+  __device__ __forceinline__ static void load(const double* sd, const int*, double* b) {
+    for (int i = 0; i < 21; i++) b[i] = sd[i];
+  }
+  __device__ __forceinline__ static void shape(const double* lam, P3 X, const double* b, double* o) {
+#pragma unroll
+    for (int k = 0; k < 3; k++) {
+      const int a = k, c = (k + 1) % 3;
+      const double sk = SGN * b[9 + k];
+      double* lo = o + DIM * (2 * k);
+      double* hi = o + DIM * (2 * k + 1);
+      lo[0] = sk * (X.x - b[12 + 3 * k]);
+      lo[1] = sk * (X.y - b[13 + 3 * k]);
+      lo[2] = sk * (X.z - b[14 + 3 * k]);
+      hi[0] = lam[a] * b[3 * c] + lam[c] * b[3 * a];
+      hi[1] = lam[a] * b[3 * c + 1] + lam[c] * b[3 * a + 1];
+      hi[2] = lam[a] * b[3 * c + 2] + lam[c] * b[3 * a + 2];
+      if (DIV) {
+        lo[3] = 2.0 * b[9 + k];
+        hi[3] = 0.0;
+      }
+    }
+  }
+};
+template <> struct Ev<SP_RWG1, EV_MAXW> : EvEdge1<1, true> {};
+template <> struct Ev<SP_NED1, EV_MAXW> : EvEdge1<-1, true> {};
+template <> struct Ev<SP_RWG1, EV_TRACE3> : EvEdge1<1, false> {};
+template <> struct Ev<SP_NED1, EV_TRACE3> : EvEdge1<-1, false> {};
+
 // elmat[ii][jj] += sum_t fac_t * f * K_{kc_t} * phi_i[ii,tec_t] * phi_j[jj,trc_t]
 template <class KER, class EI, class EJ>
 __device__ __forceinline__ void accum(const KER& ker, typename KER::T (&el)[EI::NL][EJ::NL],
@@ -1936,11 +1974,17 @@ static void dispatch_full(const Operator& op, F&& f) {
       if (big) f(ker, EvGen<0, EV_HELM, 10>{}, EvGen<1, EV_HELM, 10>{});
       else if (hi) f(ker, EvGen<0, EV_HELM, 6>{}, EvGen<1, EV_HELM, 6>{});
       else f(ker, Ev<SP_P1, EV_HELM>{}, Ev<SP_P1, EV_HELM>{});
-    } else if constexpr (std::is_same<KER, KMaxSL>::value) {
+    } else if constexpr (std::is_same<KER, KMaxSL>::value) {  // the same space on both sides
       if (st == SP_RWG) f(ker, Ev<SP_RWG, EV_MAXW>{}, Ev<SP_RWG, EV_MAXW>{});
-      else f(ker, Ev<SP_NED, EV_MAXW>{}, Ev<SP_NED, EV_MAXW>{});
+      else if (st == SP_NED) f(ker, Ev<SP_NED, EV_MAXW>{}, Ev<SP_NED, EV_MAXW>{});
+      else if (st == SP_RWG1) f(ker, Ev<SP_RWG1, EV_MAXW>{}, Ev<SP_RWG1, EV_MAXW>{});
+      else f(ker, Ev<SP_NED1, EV_MAXW>{}, Ev<SP_NED1, EV_MAXW>{});
     } else if constexpr (std::is_same<KER, KMaxDL>::value) {
-      f(ker, Ev<SP_RWG, EV_TRACE3>{}, Ev<SP_NED, EV_TRACE3>{});  // test: HDivSurface trace, trial: rotated HCurl trace
+      // test: HDivSurface trace, trial: rotated HCurl trace
+      if (st == SP_RWG && sr == SP_NED) f(ker, Ev<SP_RWG, EV_TRACE3>{}, Ev<SP_NED, EV_TRACE3>{});
+      else if (st == SP_RWG) f(ker, Ev<SP_RWG, EV_TRACE3>{}, Ev<SP_NED1, EV_TRACE3>{});
+      else if (sr == SP_NED) f(ker, Ev<SP_RWG1, EV_TRACE3>{}, Ev<SP_NED, EV_TRACE3>{});
+      else f(ker, Ev<SP_RWG1, EV_TRACE3>{}, Ev<SP_NED1, EV_TRACE3>{});
     } else if (big) {
       f(ker, EvGen<0, EV_ID, 10>{}, EvGen<1, EV_ID, 10>{});
     } else if (hi) {

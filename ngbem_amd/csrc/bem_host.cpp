@@ -129,8 +129,10 @@ void Space::setup(const Mesh* m, int k, const unsigned char* elmask) {
     }
   } else {
     // edges sorted by (min vertex, max vertex); local dof k of a triangle is
-    // its edge (v_k, v_{k+1}), signed +1 when v_k < v_{k+1}
-    nloc = 3;
+    // its edge (v_k, v_{k+1}), signed +1 when v_k < v_{k+1}; order 1: local
+    // 2k / 2k+1 = the edge's order-0 / bubble function, global edge / ne + edge
+    const bool o1 = k == SP_RWG1 || k == SP_NED1;
+    nloc = o1 ? 6 : 3;
     std::vector<long long> keys;
     keys.reserve(3 * (size_t)m->nt);
     auto key = [m](int e, int l) {
@@ -142,17 +144,24 @@ void Space::setup(const Mesh* m, int k, const unsigned char* elmask) {
     std::vector<long long> uk(keys);
     std::sort(uk.begin(), uk.end());
     uk.erase(std::unique(uk.begin(), uk.end()), uk.end());
-    ndof = (int)uk.size();
+    const int ne = (int)uk.size();
+    ndof = o1 ? 2 * ne : ne;
     edges.resize(2 * (size_t)ndof);
     for (int i = 0; i < ndof; i++) {
-      edges[2 * i] = (int)(uk[i] / m->nv);
-      edges[2 * i + 1] = (int)(uk[i] % m->nv);
+      edges[2 * i] = (int)(uk[i % ne] / m->nv);
+      edges[2 * i + 1] = (int)(uk[i % ne] % m->nv);
     }
-    dofs.resize(3 * (size_t)m->nt);
+    dofs.resize((size_t)nloc * m->nt);
     sign.resize(3 * (size_t)m->nt);
     for (int e = 0; e < m->nt; e++)
       for (int l = 0; l < 3; l++) {
-        dofs[3 * e + l] = (int)(std::lower_bound(uk.begin(), uk.end(), key(e, l)) - uk.begin());
+        const int ed = (int)(std::lower_bound(uk.begin(), uk.end(), key(e, l)) - uk.begin());
+        if (o1) {
+          dofs[6 * e + 2 * l] = ed;
+          dofs[6 * e + 2 * l + 1] = ne + ed;
+        } else {
+          dofs[3 * e + l] = ed;
+        }
         sign[3 * e + l] = m->tri[3 * e + l] < m->tri[3 * e + (l + 1) % 3] ? 1 : -1;
       }
   }

@@ -21,15 +21,24 @@ static const double kPi = 3.14159265358979323846;
 // vertices, then the edges' dofs (p - 1 each, edges in sorted (min, max)
 // order), then the faces');
 // SP_RWG: HDivSurface order 0 (unit-flux Raviart-Thomas on edges);
-// SP_NED: HCurl order 0 on the surface (Whitney/Nedelec edge functions)
+// SP_NED: HCurl order 0 on the surface (Whitney/Nedelec edge functions);
+// SP_RWG1 / SP_NED1: order 1 (complete linear fields): per edge l = (v_l,
+// v_l+1) the order-0 function and, for HDivSurface, the surface curl
+// n x grad(lambda_a lambda_b) of the edge bubble, for HCurl its gradient
+// grad(lambda_a lambda_b); local dofs interleaved (2l: order 0, 2l+1: the
+// bubble's), global: the order-0 dofs (edge index), then ne + edge index
 enum SpaceKind {
-  SP_P0 = 0, SP_P1 = 1, SP_RWG = 2, SP_NED = 3, SP_L2_1 = 4, SP_L2_2 = 5, SP_P2 = 6, SP_L2_3 = 7, SP_P3 = 8
+  SP_P0 = 0, SP_P1 = 1, SP_RWG = 2, SP_NED = 3, SP_L2_1 = 4, SP_L2_2 = 5, SP_P2 = 6, SP_L2_3 = 7, SP_P3 = 8,
+  SP_RWG1 = 9, SP_NED1 = 10
 };
-inline bool is_edge_kind(int k) { return k == SP_RWG || k == SP_NED; }
+inline bool is_edge_kind(int k) { return k == SP_RWG || k == SP_NED || k == SP_RWG1 || k == SP_NED1; }
+inline bool is_hdiv_kind(int k) { return k == SP_RWG || k == SP_RWG1; }
+inline bool is_hcurl_kind(int k) { return k == SP_NED || k == SP_NED1; }
 inline bool is_h1_kind(int k) { return k == SP_P1 || k == SP_P2 || k == SP_P3; }
-inline bool is_higher_kind(int k) { return k >= SP_L2_1; }  // evaluated by the generic evaluators
+inline bool is_higher_kind(int k) { return k >= SP_L2_1 && !is_edge_kind(k); }  // the generic scalar evaluators
 inline int nloc_of_kind(int k) {
-  return (k == SP_P0) ? 1 : (k == SP_L2_2 || k == SP_P2) ? 6 : (k == SP_L2_3 || k == SP_P3) ? 10 : 3;
+  return (k == SP_P0) ? 1 : (k == SP_L2_2 || k == SP_P2 || k == SP_RWG1 || k == SP_NED1) ? 6
+                          : (k == SP_L2_3 || k == SP_P3) ? 10 : 3;
 }
 constexpr int NLOC_MAX = 10;   // local dofs per element, at most (H1 / SurfaceL2 order 3)
 constexpr int DOF_STRIDE = 16;  // element stride of the device dof tables

@@ -184,23 +184,27 @@ class H1(FESpace):
 
 
 class _EdgeSpace(FESpace):
-    """Lowest-order edge elements: dof = mesh edge (sorted (min, max) vertex
+    """Edge elements.  Order 0: dof = mesh edge (sorted (min, max) vertex
     pairs); local dof l of a triangle is its edge (v_l, v_{l+1}), signed +1 when
-    v_l < v_{l+1}."""
+    v_l < v_{l+1}.  Order 1: local dofs 2l (the order-0 function of edge l) and
+    2l+1 (the edge bubble's function, below); global dofs: the order-0 ones
+    (edge index), then ne + edge index."""
 
-    orders = (0,)
+    orders = (0, 1)
 
     def __init__(self, mesh, order=0, complex=False, definedon=None, dirichlet=None):
         super().__init__(mesh, order, complex, definedon, dirichlet)
-        self.ikind = self.kind
         tri = mesh.tri
         loc = np.stack([tri, np.roll(tri, -1, axis=1)], axis=2)  # nt x 3 x 2
+        if self.order == 1:
+            loc = np.repeat(loc, 2, axis=1)                      # nt x 6 x 2
         lo, hi = loc.min(2), loc.max(2)
         self.edges = np.zeros((self.ndof, 2), dtype=np.int32)  # dof -> sorted vertex pair
         on = self.elem_dofs >= 0
         self.edges[self.elem_dofs[on], 0] = lo[on]
         self.edges[self.elem_dofs[on], 1] = hi[on]
-        self.elem_sign = np.where(loc[:, :, 0] < loc[:, :, 1], 1.0, -1.0)
+        e3 = np.stack([tri, np.roll(tri, -1, axis=1)], axis=2)
+        self.elem_sign = np.where(e3[:, :, 0] < e3[:, :, 1], 1.0, -1.0)  # nt x 3, per local edge
 
     def rt_shapes(self, pts_bary):
         """phi_l at barycentric points (q x 3) of every triangle: nt x q x 3(l) x 3(xyz),
@@ -213,15 +217,65 @@ class _EdgeSpace(FESpace):
         s = self.elem_sign / J[:, None]
         return s[:, None, :, None] * (X[:, :, None, :] - popp[:, None, :, :])
 
+    def bubble_curls(self, pts_bary):
+        """psi_l = n x grad(lambda_a lambda_b) for the edges (a, b) = (l, l+1):
+        nt x q x 3(l) x 3(xyz)."""
+        m = self.mesh
+        P = m.xyz[m.tri]
+        e1, e2 = P[:, 0] - P[:, 2], P[:, 1] - P[:, 2]
+        G = np.stack([e1, e2], 1)                                  # nt x 2 x 3
+        Ginv = np.linalg.inv(np.einsum("tic,tjc->tij", G, G))     # metric inverse
+        g01 = np.einsum("tij,tjc->tic", Ginv, G)                   # grad lambda_0, lambda_1
+        g = np.stack([g01[:, 0], g01[:, 1], -g01[:, 0] - g01[:, 1]], 1)  # nt x 3 x 3
+        c = np.cross(m.normals()[:, None, :], g)                  # n x grad lambda_k
+        lam = pts_bary
+        out = []
+        for l in range(3):
+            a, b = l, (l + 1) % 3
+            out.append(lam[None, :, a, None] * c[:, None, b, :] + lam[None, :, b, None] * c[:, None, a, :])
+        return np.stack(out, axis=2)
+
+    def trace_shapes(self, pts_bary):
+        """The tangential fields the Maxwell operators see, nt x q x nloc x 3:
+        HDivSurface: the basis functions; HCurl: their rotated traces n x N."""
+        phi = self.rt_shapes(pts_bary)
+        if isinstance(self, HCurl):
+            phi = -phi                                    # n x (n x phi_l)
+        if self.order == 0:
+            return phi
+        psi = self.bubble_curls(pts_bary)
+        return np.stack([phi, psi], axis=3).reshape(phi.shape[0], phi.shape[1], 6, 3)
+
+    def vec_shapes(self, pts_bary):
+        """The basis functions themselves, nt x q x nloc x 3: HDivSurface phi_l
+        (and psi_l); HCurl N_l = n x phi_l (and grad(lambda_a lambda_b) =
+        -n x psi_l)."""
+        t = self.trace_shapes(pts_bary)
+        if isinstance(self, HCurl):
+            n = self.mesh.normals()[:, None, None, :]
+            t = -np.cross(n, t)                           # N = -n x (n x N)
+        return t
+
 
 class HDivSurface(_EdgeSpace):
-    """HDivSurface, order 0: unit-flux Raviart-Thomas (RWG-type) functions."""
+    """HDivSurface, orders 0-1: unit-flux Raviart-Thomas (RWG-type) functions,
+    for order 1 with the surface curls of the edge bubbles (all linear
+    tangential fields)."""
     kind = _capi.SPACE_HDIV
+
+    def __init__(self, mesh, order=0, complex=False, definedon=None, dirichlet=None):
+        super().__init__(mesh, order, complex, definedon, dirichlet)
+        self.ikind = (2, 9)[self.order]
 
 
 class HCurl(_EdgeSpace):
-    """HCurl, order 0, on the surface: Whitney edge functions N_l = n x phi_l."""
+    """HCurl, orders 0-1, on the surface: Whitney edge functions N_l = n x phi_l,
+    for order 1 with the gradients of the edge bubbles."""
     kind = _capi.SPACE_HCURL
+
+    def __init__(self, mesh, order=0, complex=False, definedon=None, dirichlet=None):
+        super().__init__(mesh, order, complex, definedon, dirichlet)
+        self.ikind = (3, 10)[self.order]
 
 
 def mass_matrix_coo(trial: FESpace, test: FESpace):
@@ -231,17 +285,12 @@ def mass_matrix_coo(trial: FESpace, test: FESpace):
     if isinstance(trial, _EdgeSpace) or isinstance(test, _EdgeSpace):
         if not (isinstance(trial, _EdgeSpace) and isinstance(test, _EdgeSpace)):
             raise NotImplementedError("mass matrix between an edge space and a scalar space")
-        # edge-midpoint rule, exact for the quadratic integrands
+        # edge-midpoint rule, exact for the quadratic integrands (orders 0-1)
         bary = np.array([[0.5, 0.5, 0.0], [0.0, 0.5, 0.5], [0.5, 0.0, 0.5]])
         w = m.areas()[:, None] / 3.0
-        n = m.normals()
-        phi_i = test.rt_shapes(bary)
-        phi_j = trial.rt_shapes(bary)
-        if isinstance(test, HCurl):
-            phi_i = np.cross(n[:, None, None, :], phi_i)
-        if isinstance(trial, HCurl):
-            phi_j = np.cross(n[:, None, None, :], phi_j)
-        loc = np.einsum("tq,tqic,tqjc->tij", w, phi_i, phi_j)  # nt x 3 x 3
+        phi_i = test.vec_shapes(bary)
+        phi_j = trial.vec_shapes(bary)
+        loc = np.einsum("tq,tqic,tqjc->tij", w, phi_i, phi_j)  # nt x nloc x nloc
     elif trial.order > 1 or test.order > 1 or isinstance(trial, SurfaceL2) and trial.order or \
             isinstance(test, SurfaceL2) and test.order:
         # higher orders: reference Gram matrix by a collapsed Gauss rule

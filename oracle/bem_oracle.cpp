@@ -292,7 +292,13 @@ struct Mesh {
 // SP_RWG: HDivSurface order 0 (unit-flux RT0 per edge); SP_NED: HCurl order 0;
 // SP_L2_1 / SP_L2_2: SurfaceL2 order 1 / 2 (Dubiner basis); SP_P2: H1 order 2
 // (vertex hats, then the edge bubbles lambda_a lambda_b)
-enum { SP_P0 = 0, SP_P1 = 1, SP_RWG = 2, SP_NED = 3, SP_L2_1 = 4, SP_L2_2 = 5, SP_P2 = 6, SP_L2_3 = 7, SP_P3 = 8 };
+// SP_RWG1 / SP_NED1: HDivSurface / HCurl order 1: per edge (a, b) = (v_l, v_l+1)
+// the order-0 function (local 2l) and the edge bubble's surface curl
+// n x grad(lambda_a lambda_b) (HDivSurface) or gradient grad(lambda_a lambda_b)
+// (HCurl) (local 2l+1); global dofs: the order-0 ones (edge index), then ne +
+// edge index
+enum { SP_P0 = 0, SP_P1 = 1, SP_RWG = 2, SP_NED = 3, SP_L2_1 = 4, SP_L2_2 = 5, SP_P2 = 6, SP_L2_3 = 7, SP_P3 = 8,
+       SP_RWG1 = 9, SP_NED1 = 10 };
 static int l2_order(int k) { return k == SP_L2_1 ? 1 : (k == SP_L2_2 ? 2 : (k == SP_L2_3 ? 3 : 0)); }
 // EV_MAXW: [phi, div] (DiffOpMaxwellNew, src/diffops.hpp:263-338; DiffOpMaxwell :170-255)
 // EV_TR3: vector trace (HDivSurface BND; RotatedTrace :62-107 on HCurl)
@@ -351,7 +357,8 @@ struct Space {
         if (pe == 2) dofs[(size_t)nloc * e + 9] = m.nv + pe * ne + e;
       }
     } else {  // edges numbered in (min vertex, max vertex) order; local edge l = (v_l, v_{l+1})
-      nloc = 3;
+      const bool o1 = (k == SP_RWG1 || k == SP_NED1);
+      nloc = o1 ? 6 : 3;
       std::vector<std::pair<int, int>> ed;
       for (int e = 0; e < m.nt; e++)
         for (int l = 0; l < 3; l++) {
@@ -361,12 +368,19 @@ struct Space {
       std::vector<std::pair<int, int>> u(ed);
       std::sort(u.begin(), u.end());
       u.erase(std::unique(u.begin(), u.end()), u.end());
-      ndof = (int)u.size();
-      dofs.resize(3 * (size_t)m.nt);
+      const int ne = (int)u.size();
+      ndof = o1 ? 2 * ne : ne;
+      dofs.resize((size_t)nloc * m.nt);
       sign.resize(3 * (size_t)m.nt);
       for (int e = 0; e < m.nt; e++)
         for (int l = 0; l < 3; l++) {
-          dofs[3 * e + l] = (int)(std::lower_bound(u.begin(), u.end(), ed[3 * e + l]) - u.begin());
+          const int id = (int)(std::lower_bound(u.begin(), u.end(), ed[3 * e + l]) - u.begin());
+          if (o1) {
+            dofs[6 * e + 2 * l] = id;
+            dofs[6 * e + 2 * l + 1] = ne + id;
+          } else {
+            dofs[3 * e + l] = id;
+          }
           sign[3 * e + l] = m.tri[3 * e + l] < m.tri[3 * e + (l + 1) % 3] ? 1 : -1;
         }
     }
@@ -431,18 +445,38 @@ static void dubiner(int p, double x, double y, double* out) {
 static void eval_shape(const Mesh& m, const Space& s, int ev, int e, double xi, double eta, double* out) {
   if (s.kind == SP_P0) { out[0] = 1.0; return; }
   if (l2_order(s.kind) > 0) { dubiner(l2_order(s.kind), xi, eta, out); return; }
-  if (s.kind == SP_RWG || s.kind == SP_NED) {
+  if (s.kind == SP_RWG || s.kind == SP_NED || s.kind == SP_RWG1 || s.kind == SP_NED1) {
     // RT0 with unit flux: phi_l = sign_l / J (x - p_opp(l)), div = 2 sign_l / J; the
-    // rotated trace of the Whitney function of an HCurl edge is -phi_l
+    // rotated trace of the Whitney function of an HCurl edge is -phi_l.  Order 1
+    // (rows 2l, 2l+1): phi_l, then psi_l = n x grad(lambda_a lambda_b) for the
+    // edge (a, b) = (l, l+1), div 0: the HDivSurface function itself, or the
+    // rotated trace n x grad(lambda_a lambda_b) of the HCurl one (curl 0)
     double X[3];
     m.point(e, xi, eta, X);
-    const double sg = (s.kind == SP_NED) ? -1.0 : 1.0;
-    const int dim = ev_dim(ev);
+    const bool o1 = (s.kind == SP_RWG1 || s.kind == SP_NED1);
+    const double sg = (s.kind == SP_NED || s.kind == SP_NED1) ? -1.0 : 1.0;
+    const int dim = ev_dim(ev), st = o1 ? 2 : 1;
     for (int l = 0; l < 3; l++) {
       const double sl = s.sign[3 * e + l] / m.J[e];
       const double* po = &m.xyz[3 * m.tri[3 * e + (l + 2) % 3]];
-      for (int c = 0; c < 3; c++) out[l * dim + c] = sg * sl * (X[c] - po[c]);
-      if (ev == EV_MAXW) out[l * dim + 3] = 2.0 * sl;
+      for (int c = 0; c < 3; c++) out[st * l * dim + c] = sg * sl * (X[c] - po[c]);
+      if (ev == EV_MAXW) out[st * l * dim + 3] = 2.0 * sl;
+    }
+    if (o1) {
+      const double lam[3] = {xi, eta, 1.0 - xi - eta};
+      double g[3][3];
+      m.grads(e, g);
+      const double* n = &m.nrm[3 * e];
+      for (int l = 0; l < 3; l++) {
+        const int a = l, b = (l + 1) % 3;
+        double gb[3];  // grad(lambda_a lambda_b)
+        for (int c = 0; c < 3; c++) gb[c] = lam[a] * g[b][c] + lam[b] * g[a][c];
+        double* o = out + (2 * l + 1) * dim;
+        o[0] = n[1] * gb[2] - n[2] * gb[1];
+        o[1] = n[2] * gb[0] - n[0] * gb[2];
+        o[2] = n[0] * gb[1] - n[1] * gb[0];
+        if (ev == EV_MAXW) o[3] = 0.0;
+      }
     }
     return;
   }

@@ -18,7 +18,7 @@ _lib = None
 # kernel ids (same numbering as the product's C-ABI)
 (LAPLACE_SL, LAPLACE_DL, LAPLACE_HS, HELMHOLTZ_SL, HELMHOLTZ_DL, HELMHOLTZ_CF, HELMHOLTZ_HS, MAXWELL_SL,
  MAXWELL_DL, HELMHOLTZ_SL_DKAPPA) = range(10)
-P0, P1, HDIV, HCURL, L2_1, L2_2, P2, L2_3, P3 = range(9)  # SurfaceL2 orders 1-3; H1 orders 2-3
+P0, P1, HDIV, HCURL, L2_1, L2_2, P2, L2_3, P3, HDIV1, HCURL1 = range(11)  # SurfaceL2 1-3; H1 2-3; edge order 1
 
 
 def build():

@@ -65,7 +65,16 @@ def test_bad_arguments_are_reported():
         with pytest.raises(RuntimeError, match=msg):
             _capi.check(L.bem_op_create(kern, 1.0, tr.handle, te.handle, ctypes.byref(params), ctypes.byref(out)))
     with pytest.raises(NotImplementedError):
-        nb.HDivSurface(m, order=1)
+        nb.HDivSurface(m, order=2)
+    with pytest.raises(NotImplementedError):
+        nb.HCurl(m, order=2)
+    rc = L.bem_space_create(m._bem_handle.ptr, _capi.SPACE_HDIV, 2, ctypes.byref(out))
+    assert rc == -1 and b"HDivSurface orders 0-1" in L.bem_last_error()
+    # the Maxwell single layer takes one space kind on both sides
+    hd1 = nb.HDivSurface(m, order=1)
+    with pytest.raises(RuntimeError, match="Maxwell single layer needs"):
+        _capi.check(L.bem_op_create(_capi.MAXWELL_SL, 1.0, hd1.handle, hd.handle, ctypes.byref(params),
+                                    ctypes.byref(out)))
 
 
 def op_create(space_trial, space_test, kernel, **p):
