@@ -2006,38 +2006,51 @@ void setup_device(Operator& op, hipStream_t s) {
   const int nt = m.nt, ntq = op.trig.size();
   std::vector<double> geo((size_t)nt * GEO, 0.0), quad((size_t)nt * ntq * 4), sdat((size_t)nt * SDS, 0.0);
   std::vector<int> vid((size_t)nt * 4, 0);
-  for (int e = 0; e < nt; e++) {
-    for (int k = 0; k < 3; k++) {
-      const int v = m.tri[3 * e + k];
-      vid[4 * e + k] = v;
-      for (int c = 0; c < 3; c++) geo[(size_t)e * GEO + 3 * k + c] = m.xyz[3 * v + c];
+  // per-element tables, elements split over host threads (disjoint slots)
+  const Space& es = is_edge_kind(op.test->kind) ? *op.test : *op.trial;
+  auto fill = [&](int e0, int e1) {
+    for (int e = e0; e < e1; e++) {
+      for (int k = 0; k < 3; k++) {
+        const int v = m.tri[3 * e + k];
+        vid[4 * e + k] = v;
+        for (int c = 0; c < 3; c++) geo[(size_t)e * GEO + 3 * k + c] = m.xyz[3 * v + c];
+      }
+      for (int c = 0; c < 3; c++) geo[(size_t)e * GEO + 9 + c] = m.nrm[3 * e + c];
+      geo[(size_t)e * GEO + 12] = m.J[e];
+      // wavenumber of the element's geometry (several geometries in one operator)
+      geo[(size_t)e * GEO + 13] = op.kappa_el.empty() ? op.kappa : op.kappa_el[e];
+      for (int q = 0; q < ntq; q++) {
+        double x[3];
+        m.point(e, op.trig.xi[q], op.trig.eta[q], x);
+        double* o = &quad[((size_t)e * ntq + q) * 4];
+        o[0] = x[0]; o[1] = x[1]; o[2] = x[2];
+        o[3] = m.J[e] * op.trig.w[q];
+      }
+      double c[3][3];
+      m.curls(e, c);
+      double* sd = &sdat[(size_t)e * SDS];
+      for (int k = 0; k < 3; k++)
+        for (int d = 0; d < 3; d++) sd[3 * k + d] = c[k][d];
+      // edge spaces: s_k = sign_k / J and the vertex opposite edge k (= vertex k+2)
+      for (int k = 0; k < 3; k++) {
+        sd[9 + k] = es.sign.empty() ? 0.0 : es.sign[3 * e + k] / m.J[e];
+        const int vo = m.tri[3 * e + (k + 2) % 3];
+        for (int d = 0; d < 3; d++) sd[12 + 3 * k + d] = m.xyz[3 * vo + d];
+      }
+      sd[21] = op.test->kind;
+      sd[22] = op.trial->kind;
+      for (int k = 0; k < 3; k++) sd[23 + k] = m.tri[3 * e + k] < m.tri[3 * e + (k + 1) % 3] ? 1.0 : -1.0;
     }
-    for (int c = 0; c < 3; c++) geo[(size_t)e * GEO + 9 + c] = m.nrm[3 * e + c];
-    geo[(size_t)e * GEO + 12] = m.J[e];
-    // wavenumber of the element's geometry (several geometries in one operator)
-    geo[(size_t)e * GEO + 13] = op.kappa_el.empty() ? op.kappa : op.kappa_el[e];
-    for (int q = 0; q < ntq; q++) {
-      double x[3];
-      m.point(e, op.trig.xi[q], op.trig.eta[q], x);
-      double* o = &quad[((size_t)e * ntq + q) * 4];
-      o[0] = x[0]; o[1] = x[1]; o[2] = x[2];
-      o[3] = m.J[e] * op.trig.w[q];
+  };
+  {
+    const int nth = (int)std::max(1u, std::min(16u, std::thread::hardware_concurrency()));
+    const int chunk = (nt + nth - 1) / nth;
+    std::vector<std::thread> th;
+    for (int w = 0; w < nth; w++) {
+      const int e0 = w * chunk, e1 = std::min(nt, e0 + chunk);
+      if (e0 < e1) th.emplace_back(fill, e0, e1);
     }
-    double c[3][3];
-    m.curls(e, c);
-    double* sd = &sdat[(size_t)e * SDS];
-    for (int k = 0; k < 3; k++)
-      for (int d = 0; d < 3; d++) sd[3 * k + d] = c[k][d];
-    // edge spaces: s_k = sign_k / J and the vertex opposite edge k (= vertex k+2)
-    const Space& es = is_edge_kind(op.test->kind) ? *op.test : *op.trial;
-    for (int k = 0; k < 3; k++) {
-      sd[9 + k] = es.sign.empty() ? 0.0 : es.sign[3 * e + k] / m.J[e];
-      const int vo = m.tri[3 * e + (k + 2) % 3];
-      for (int d = 0; d < 3; d++) sd[12 + 3 * k + d] = m.xyz[3 * vo + d];
-    }
-    sd[21] = op.test->kind;
-    sd[22] = op.trial->kind;
-    for (int k = 0; k < 3; k++) sd[23 + k] = m.tri[3 * e + k] < m.tri[3 * e + (k + 1) % 3] ? 1.0 : -1.0;
+    for (auto& t : th) t.join();
   }
   // unused slots (l >= nloc; the generic evaluators' zero rows) hold -1
   auto dof_table = [&](const Space& sp) {
