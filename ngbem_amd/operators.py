@@ -204,13 +204,15 @@ class AdjointMatrix(BaseMatrix):
 
 
 class MassMatrix(BaseMatrix):
-    """Surface mass matrix  int u v ds  (test x trial) for SurfaceL2 p0 / H1 p1,
-    the ``Id`` of the reference's formulations (e.g. demos/HelmholtzBurtonMiller.ipynb)."""
+    """Surface mass matrix  int u . v ds  (test x trial) of any two spaces of
+    the same family (scalar or edge), the ``Id`` of the reference's
+    formulations (e.g. demos/HelmholtzBurtonMiller.ipynb)."""
 
     def __init__(self, trial: FESpace, test: FESpace | None = None):
         test = test or trial
         r, c, v = mass_matrix_coo(trial, test)
         dev = device()
+        self.trial, self.test = trial, test
         self.height, self.width = test.ndof, trial.ndof
         self.is_complex = False
         self.rows = torch.from_numpy(r).to(dev)
@@ -224,6 +226,89 @@ class MassMatrix(BaseMatrix):
     def MultTransAdd(self, s, x, y):
         x = x.to(y.dtype)
         y.index_add_(-1, self.cols, (s * self.vals).to(y.dtype) * x[..., self.rows])
+
+    def Inverse(self, freedofs=None, tol=1e-14, maxsteps=2000):
+        """The inverse of a square mass matrix (NGSolve's ``mat.Inverse(freedofs)``,
+        as the reference's demos use it for preconditioners): on the free dofs
+        (all when ``freedofs`` is None; else a boolean mask) the solution of
+        M_ff z_f = x_f, zero elsewhere.  SurfaceL2 spaces (element-local dofs,
+        block-diagonal M) are inverted exactly block by block; the others by
+        Jacobi-preconditioned CG to the relative residual ``tol``."""
+        if self.trial is not self.test:
+            raise ValueError("Inverse needs a square mass matrix (one space)")
+        return _MassInverse(self, freedofs, tol, maxsteps)
+
+
+class _MassInverse(BaseMatrix):
+    def __init__(self, M: MassMatrix, freedofs, tol, maxsteps):
+        dev = M.vals.device
+        n = M.height
+        self.height = self.width = n
+        self.is_complex = False
+        self._M, self._tol, self._maxsteps = M, tol, maxsteps
+        free = np.ones(n, dtype=bool) if freedofs is None else np.asarray(
+            freedofs.cpu() if isinstance(freedofs, torch.Tensor) else freedofs, dtype=bool).reshape(-1)
+        if free.shape != (n,):
+            raise ValueError(f"freedofs must have {n} entries")
+        self._free = torch.from_numpy(free.astype(np.float64)).to(dev)
+        self._blocks = None
+        sp = M.trial
+        if isinstance(sp, SurfaceL2):
+            # element-local dofs: M is block-diagonal with one nloc x nloc block
+            # per element; restricted to the free dofs (identity on the others,
+            # masked out again after the solve)
+            ed = torch.from_numpy(sp.elem_dofs).to(dev)
+            on = (ed[:, 0] >= 0)
+            ed = ed[on]
+            ne, nl = ed.shape
+            pos = torch.full((n,), -1, dtype=torch.int64, device=dev)
+            pos[ed.reshape(-1)] = torch.arange(ne * nl, device=dev)
+            B = torch.zeros(ne * nl * nl, dtype=torch.float64, device=dev)
+            pr, pc = pos[M.rows], pos[M.cols]
+            B.index_add_(0, (pr // nl) * nl * nl + (pr % nl) * nl + (pc % nl), M.vals)
+            B = B.view(ne, nl, nl)
+            f = self._free[ed]                                   # ne x nl
+            B = B * f[:, :, None] * f[:, None, :] + torch.diag_embed(1.0 - f)
+            self._blocks = (ed, torch.linalg.inv(B))
+        else:
+            d = torch.zeros(n, dtype=torch.float64, device=dev)
+            d.index_add_(0, M.rows[M.rows == M.cols], M.vals[M.rows == M.cols])
+            self._dinv = torch.where(self._free > 0, 1.0 / d, torch.zeros_like(d))
+
+    def _solve(self, x):
+        f = self._free.to(x.dtype)
+        b = x * f
+        if self._blocks is not None:
+            ed, Binv = self._blocks
+            z = torch.zeros_like(b)
+            loc = b[..., ed]                                      # (..., ne, nl)
+            z[..., ed] = torch.einsum("eij,...ej->...ei", Binv.to(b.dtype), loc)
+            return z * f
+        M, dinv = self._M, self._dinv.to(x.dtype)
+        Mf = lambda v: M * (v * f) * f  # noqa: E731  (M_ff, zero on the fixed dofs)
+        z = torch.zeros_like(b)
+        r = b.clone()
+        p = r * dinv
+        rz = (r.conj() * p).sum(-1, keepdim=True)
+        nb = torch.linalg.norm(b, dim=-1, keepdim=True).clamp_min(1e-300)
+        for _ in range(self._maxsteps):
+            if bool((torch.linalg.norm(r, dim=-1, keepdim=True) <= self._tol * nb).all()):
+                break
+            q = Mf(p)
+            alpha = rz / (p.conj() * q).sum(-1, keepdim=True)
+            z = z + alpha * p
+            r = r - alpha * q
+            w = r * dinv
+            rz1 = (r.conj() * w).sum(-1, keepdim=True)
+            p = w + (rz1 / rz) * p
+            rz = rz1
+        return z * f
+
+    def MultAdd(self, s, x, y):
+        y += s * self._solve(x.to(y.dtype))
+
+    def MultTransAdd(self, s, x, y):  # symmetric
+        y += s * self._solve(x.to(y.dtype))
 
 
 # ---------------------------------------------------------------- HMatrix --
