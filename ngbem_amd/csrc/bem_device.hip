@@ -1897,6 +1897,23 @@ __global__ __launch_bounds__(256) void k_mm_rows(const Tile* __restrict__ tiles,
   }
 }
 
+// transpose, column tiles: y[c] = sum over the tile's segments (in block
+// order) of the dense blocks' column partials (far = 3) and the low-rank
+// blocks' z slices (far = 2); one wave per tile, lane c owns column roff + c
+template <class VT>
+__global__ __launch_bounds__(64) void k_mv_cols_sum(const Tile* __restrict__ tiles, const Seg* __restrict__ segs,
+                                                    const VT* __restrict__ part, const VT* __restrict__ zT,
+                                                    VT* __restrict__ yp) {
+  const Tile tl = tiles[blockIdx.x];
+  const int c = threadIdx.x;
+  VT acc = zero_<VT>();
+  for (int k = tl.s0; k < tl.s1; k++) {
+    const Seg sg = segs[k];
+    if (c < tl.len) acc += (sg.far == 2 ? zT : part)[sg.base + tl.roff + c];
+  }
+  if (c < tl.len) yp[tl.p0 + c] = acc;
+}
+
 // xm[p][r] = x[perm[p] + (r0 + r) ld] for r < nr, zero beyond
 template <class VT, class XT>
 __global__ void k_gather_mm(const int* __restrict__ perm, int n, const XT* __restrict__ x, int64_t ld, int r0, int nr,
@@ -2667,8 +2684,22 @@ void assemble_far(Operator& op, hipStream_t s) {
 
 // How the forward plan covers the low-rank blocks: PLAN_UT as columns of U
 // (Ut rows, stride m) times t inside the row tiles, PLAN_Z as slices of
-// z = U t computed per block beforehand.  The transpose always uses z.
-enum PlanKind { PLAN_Z = 0, PLAN_UT = 1 };
+// z = U t computed per block beforehand.  The transpose always uses z;
+// PLAN_TSUM (transpose only) lists, per column tile, the slices to add: the
+// dense blocks' column partials (far = 3, base = the block's first partial +
+// the leaf's offset) and the z slices (far = 2).
+enum PlanKind { PLAN_Z = 0, PLAN_UT = 1, PLAN_TSUM = 2 };
+
+// first (block, column) partial of every near block, in slot order
+static std::vector<int64_t> partial_slots(const Operator& op, int64_t& total) {
+  std::vector<int64_t> ps(op.near.size());
+  total = 0;
+  for (size_t k = 0; k < op.near.size(); k++) {
+    ps[k] = total;
+    total += op.near[k].n;
+  }
+  return ps;
+}
 
 static void plan(const Operator& op, bool trans, std::vector<Tile>& tiles, std::vector<Seg>& segs,
                  int kind = PLAN_Z) {
@@ -2677,6 +2708,8 @@ static void plan(const Operator& op, bool trans, std::vector<Tile>& tiles, std::
   std::vector<int> lp0(nleaf);
   for (int i = 0; i < nleaf; i++) lp0[i] = (int)tree.cl[tree.leaves[i]].permupos;
   std::vector<std::vector<Seg>> per(nleaf);
+  int64_t ptotal = 0;
+  const std::vector<int64_t> pslot = kind == PLAN_TSUM ? partial_slots(op, ptotal) : std::vector<int64_t>();
   // p0/len: the block's position range on the tiled side; mk(o, ls): segment
   // for a leaf of ls positions starting o positions into the block
   auto add = [&](int p0, int len, auto mk) {
@@ -2692,6 +2725,8 @@ static void plan(const Operator& op, bool trans, std::vector<Tile>& tiles, std::
       const NearBlk& nb = op.near[slot];
       if (!trans)
         add(nb.r0, nb.m, [&](int o, int) { return Seg{nb.off + o, nb.m, nb.n, nb.c0, 0}; });
+      else if (kind == PLAN_TSUM)
+        add(nb.c0, nb.n, [&](int o, int) { return Seg{pslot[slot] + o, 0, 1, 0, 3}; });
       else
         add(nb.c0, nb.n, [&](int o, int) { return Seg{nb.off + (int64_t)o * nb.m, nb.m, nb.m, nb.r0, 0}; });
     } else {
@@ -2749,6 +2784,28 @@ static void build_row_items(const Operator& op, std::vector<RowItem> (&items)[2]
         for (int l = 0; l < fb.R; l += nr)
           it.push_back(RowItem{base + (int64_t)l * len, len, xoff, fb.toff + l, std::min(nr, fb.R - l), kc, 0});
       }
+    }
+  }
+}
+
+// the transpose of every dense block as row items over its columns
+// (contiguous, length m, times x[r0 ..]); item t index = the column's partial
+static void build_col_items(const Operator& op, std::vector<RowItem> (&items)[2]) {
+  const int esz = op.complex_valued ? 16 : 8;
+  int64_t total = 0;
+  const std::vector<int64_t> ps = partial_slots(op, total);
+  if (total >= (int64_t)INT32_MAX) throw std::runtime_error("too many dense-block columns for the transpose items");
+  for (int prec = 0; prec < 2; prec++) {
+    const bool wide = (prec ? esz / 2 : esz) >= 16;
+    std::vector<RowItem>& it = items[prec];
+    it.clear();
+    for (size_t k = 0; k < op.near.size(); k++) {
+      const NearBlk& nb = op.near[k];
+      int kc, nr;
+      far_variant(nb.m, wide, kc, nr);
+      for (int c = 0; c < nb.n; c += nr)
+        it.push_back(RowItem{nb.off + (int64_t)c * nb.m, nb.m, nb.r0, (int)(ps[k] + c), std::min(nr, nb.n - c), kc,
+                             0});
     }
   }
 }
@@ -2855,8 +2912,10 @@ void build_matvec_plan(Operator& op, hipStream_t s) {
   std::vector<Seg> s_ut, s_fwd, s_trn;
   std::vector<int2> cols, range;
   bool flat_ok = false;
-  std::vector<RowItem> items[2][2];
-  std::exception_ptr err[4];
+  std::vector<RowItem> items[2][2], citems[2];
+  std::vector<Tile> t_tsum;
+  std::vector<Seg> s_tsum;
+  std::exception_ptr err[6];
   auto guarded = [&](int i, auto fn) {
     return std::thread([&, i, fn] {
       try {
@@ -2866,14 +2925,16 @@ void build_matvec_plan(Operator& op, hipStream_t s) {
       }
     });
   };
-  std::thread th[4] = {
+  std::thread th[6] = {
       guarded(0, [&] {
         plan(op, false, t_ut, s_ut, PLAN_UT);
         flat_ok = flat_columns(t_ut, s_ut, cols, range);
       }),
       guarded(1, [&] { plan(op, false, t_fwd, s_fwd); }),
       guarded(2, [&] { plan(op, true, t_trn, s_trn); }),
-      guarded(3, [&] { build_row_items(op, items); })};
+      guarded(3, [&] { build_row_items(op, items); }),
+      guarded(4, [&] { build_col_items(op, citems); }),
+      guarded(5, [&] { plan(op, true, t_tsum, s_tsum, PLAN_TSUM); })};
   for (auto& t : th) t.join();
   for (auto& e : err)
     if (e) std::rethrow_exception(e);
@@ -2893,6 +2954,17 @@ void build_matvec_plan(Operator& op, hipStream_t s) {
   op.d_seg_trn.upload(s_trn, s);
   op.n_tiles_trn = (int)t_trn.size();
   upload_row_items(op, items, s);
+  for (int prec = 0; prec < 2; prec++) {
+    op.d_titems[prec].upload(citems[prec], s);
+    op.n_titems[prec] = (int)citems[prec].size();
+  }
+  if (t_tsum.size() != t_trn.size()) throw std::runtime_error("transpose plans disagree on the column tiles");
+  op.d_tsum_seg.upload(s_tsum, s);
+  {
+    int64_t ptotal = 0;
+    partial_slots(op, ptotal);
+    op.d_part.alloc((size_t)std::max<int64_t>(ptotal, 1) * (op.complex_valued ? 16 : 8));
+  }
   pt_.mark("uploads");
   const size_t esz = op.complex_valued ? 16 : 8;
   const int nmax = std::max(op.ndof_trial(), op.ndof_test());
@@ -3415,13 +3487,42 @@ static bool rows_flat() {
 
 // t = B^T x of every far block through k_mv_far_t, with the item list of the
 // storage precision of F
+template <class ST>
+constexpr int st_prec() {
+  return (std::is_same<ST, fc>::value || std::is_same<ST, float>::value) ? 1 : 0;
+}
+
 template <class ST, class VT, bool CONJ>
 static void launch_far_t(const Operator& op, int trans, const ST* F, const VT* xp, VT* tb, hipStream_t s) {
-  constexpr int prec = (std::is_same<ST, fc>::value || std::is_same<ST, float>::value) ? 1 : 0;
+  constexpr int prec = st_prec<ST>();
   const int n = op.n_items[prec][trans];
   if (!n) return;
   hipLaunchKernelGGL((k_mv_far_t<ST, VT, CONJ>), dim3((n + 3) / 4), dim3(256), 0, s,
                      op.d_items[prec][trans].as<RowItem>(), n, F, xp, tb);
+}
+
+// transposed dense blocks: column partials part = A_b[:, c]^T x (conjugated
+// for the adjoint) through the same row kernel, then the column-tile sums
+template <class ST, class VT, bool CONJ>
+static void launch_cols_sum(const Operator& op, const ST* nval, const VT* xp, const VT* z, VT* yp, hipStream_t s) {
+  constexpr int prec = st_prec<ST>();
+  const int n = op.n_titems[prec];
+  VT* part = op.d_part.as<VT>();
+  if (n)
+    hipLaunchKernelGGL((k_mv_far_t<ST, VT, CONJ>), dim3((n + 3) / 4), dim3(256), 0, s,
+                       op.d_titems[prec].as<RowItem>(), n, nval, xp, part);
+  hipLaunchKernelGGL((k_mv_cols_sum<VT>), dim3(op.n_tiles_trn), dim3(64), 0, s, op.d_tile_trn.as<Tile>(),
+                     op.d_tsum_seg.as<Seg>(), part, z, yp);
+}
+
+// column tiles of the transpose from dense column partials (default) or, with
+// NGBEM_MV_COLS=segments, one wave sum per column and segment
+static bool cols_sum() {
+  static const bool on = [] {
+    const char* e = std::getenv("NGBEM_MV_COLS");
+    return !(e && std::string(e) == "segments");
+  }();
+  return on;
 }
 
 template <class ST, class VT, class XT>
@@ -3482,12 +3583,16 @@ static void apply_t(Operator& op, int flags, const XT* x, XT* y, XT alpha, XT be
       mark(2);
     }
     mark(3);
-    if (conj)
+    if (op.d_tsum_seg.p && cols_sum()) {
+      if (conj) launch_cols_sum<ST, VT, true>(op, nval, xp, z, yp, s);
+      else launch_cols_sum<ST, VT, false>(op, nval, xp, z, yp, s);
+    } else if (conj) {
       hipLaunchKernelGGL((k_mv_cols<ST, VT, true>), dim3(op.n_tiles_trn), dim3(256), 0, s, op.d_tile_trn.as<Tile>(),
                          op.d_seg_trn.as<Seg>(), nval, z, xp, yp);
-    else
+    } else {
       hipLaunchKernelGGL((k_mv_cols<ST, VT, false>), dim3(op.n_tiles_trn), dim3(256), 0, s, op.d_tile_trn.as<Tile>(),
                          op.d_seg_trn.as<Seg>(), nval, z, xp, yp);
+    }
   }
   mark(4);
   hipLaunchKernelGGL((k_scatter<XT, VT>), dim3((nout + 255) / 256), dim3(256), 0, s, pout, nout, yp, y, alpha, beta,

@@ -148,7 +148,7 @@ struct DBuf {
 
 // near block record (device)
 struct NearBlk {
-  int64_t off;     // offset of the m x n row-major values
+  int64_t off;     // offset of the m x n column-major values (entry (r, c) at off + c m + r)
   int r0, m, c0, n;
   int pI, pm;      // test patch: offset, length (P0: cluster slice of perm)
   int pJ, pn;      // trial patch
@@ -186,7 +186,8 @@ struct AcaTask {
 // columns start at base + c*ld and hold cnt rows (x at xoff).  far = 2 (low
 // rank, vector path): add z[base + r] (forward) / zT[base + c] (transpose), the
 // block's U t or V^T t computed beforehand.  far = 1 (low rank, block product):
-// Ut columns as in the dense forward case, multiplied by t[xoff + l].
+// Ut columns as in the dense forward case, multiplied by t[xoff + l].  far = 3
+// (transpose sums): add the dense block's column partials part[base + c].
 struct Seg {
   int64_t base;
   int ld, cnt, xoff, far;
@@ -250,6 +251,12 @@ struct Operator {
   DBuf d_z, d_zT;
   DBuf d_items[2][2];  // low-rank B^T x work items [FP64, FP32 copy][forward, transpose]
   int n_items[2][2] = {{0, 0}, {0, 0}};
+  // transpose of the dense blocks as row items over their columns [FP64, FP32
+  // copy] (one partial per (block, column) in d_part), and the column tiles'
+  // sums of those partials and of the z slices; empty for batch operators
+  DBuf d_titems[2];
+  int n_titems[2] = {0, 0};
+  DBuf d_tsum_seg, d_part;
   int t_total = 0;         // scratch length for far B^T x
   DBuf d_xp, d_yp, d_t;    // scratch vectors (complex128 or double)
   // block apply (64 right-hand sides): work items built on first use, slabs
