@@ -189,6 +189,10 @@ def main():
 
     prof32 = profile(_capi.APPLY_MIXED)
     prof64 = profile(0)
+    # the transposed apply (HMatrix::MultTrans; the adjoint costs the same)
+    tr32 = profile(_capi.APPLY_MIXED | _capi.APPLY_TRANS)
+    tr64 = profile(_capi.APPLY_TRANS)
+    log(f"transposed apply: mixed {tr32[5]:.3f} ms  fp64 {tr64[5]:.3f} ms")
     n = fes.ndof
     near = st["near_entries"]
     far_u = far_v = 0
@@ -261,6 +265,9 @@ def main():
                     "near_kernel_evals": st["near_kernel_evals"], "far_kernel_evals": st["far_kernel_evals"]},
         "near_field_evals_per_s": st["near_kernel_evals"] / st["t_near"] if st["t_near"] > 0 else None,
         "kernel_ms": kms,
+        "transposed_apply_ms": {"mixed": float(tr32[5]), "fp64": float(tr64[5]),
+                                "stages_mixed": dict(zip(["gather", "k_mv_far_t", "k_mv_far_z", "cols", "scatter",
+                                                          "apply"], map(float, tr32)))},
         "kernel_gbs": gbs,
         "roofline": {"kernel": f"{dom_name}<cx<float>,cx<float>>", "bound": "hbm", "achieved": gbs["mixed"][dom],
                      "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": gbs["mixed"][dom] / HBM_PEAK_GBS, "traffic": None,
