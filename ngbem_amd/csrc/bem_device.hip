@@ -3330,9 +3330,10 @@ void build_batch(const std::vector<Operator*>& ops, Operator& b, hipStream_t s, 
   }
   std::vector<int> perm_tr, perm_te;
   std::vector<Tile> tf, tt, tiles;
-  std::vector<Seg> sf, st, smm, segs;
+  std::vector<Seg> sf, st, smm, segs, stsum;
+  std::vector<RowItem> cit[2];
   b.far.clear();
-  int64_t NO = 0, UO = 0, VO = 0;
+  int64_t NO = 0, UO = 0, VO = 0, PO = 0;
   int TO = 0, XO = 0, YO = 0, ZO = 0, ZOT = 0;
   auto cp = [&](DBuf& dst, int64_t off, const DBuf& src, size_t n, size_t e) {
     if (n) BEM_HIP(hipMemcpyAsync((char*)dst.p + off * e, src.p, n * e, hipMemcpyDeviceToDevice, s));
@@ -3385,6 +3386,29 @@ void build_batch(const std::vector<Operator*>& ops, Operator& b, hipStream_t s, 
         S.push_back(g);
       }
     }
+    // the transpose's dense-column items and column-tile sums (the same
+    // segments per tile as the transposed plan above)
+    {
+      std::vector<RowItem> ci[2];
+      build_col_items(*op, ci);
+      for (int prec = 0; prec < 2; prec++)
+        for (RowItem it : ci[prec]) {
+          it.off += NO;
+          it.xoff += YO;
+          it.toff += (int)PO;
+          cit[prec].push_back(it);
+        }
+      plan(*op, true, tiles, segs, PLAN_TSUM);
+      for (Seg g : segs) {
+        g.base += g.far == 2 ? ZOT : PO;
+        stsum.push_back(g);
+      }
+      if (stsum.size() != st.size()) throw std::runtime_error("batch: transpose plans disagree");
+      int64_t pt = 0;
+      partial_slots(*op, pt);
+      PO += pt;
+      if (PO >= (int64_t)INT32_MAX) throw std::runtime_error("batch: too many dense-block columns");
+    }
     ZO += op->z_total;
     ZOT += op->zT_total;
     if (consume) {  // the operator's block storage moves into the batch
@@ -3424,6 +3448,12 @@ void build_batch(const std::vector<Operator*>& ops, Operator& b, hipStream_t s, 
   b.d_seg_trn.upload(st, s);
   b.n_tiles_trn = (int)tt.size();
   upload_row_items(b, s);
+  for (int prec = 0; prec < 2; prec++) {
+    b.d_titems[prec].upload(cit[prec], s);
+    b.n_titems[prec] = (int)cit[prec].size();
+  }
+  b.d_tsum_seg.upload(stsum, s);
+  b.d_part.alloc((size_t)std::max<int64_t>(PO, 1) * esz);
   const int nmax = std::max(XO, YO);
   b.d_xp.alloc((size_t)nmax * esz);
   b.d_yp.alloc((size_t)nmax * esz);
