@@ -26,7 +26,7 @@ import torch
 
 from . import _capi
 from .operators import (BaseMatrix, HelmholtzSingleLayerPotentialOperator,
-                        HelmholtzSingleLayerPotentialOperatorDKappa, device)
+                        HelmholtzSingleLayerPotentialOperatorDKappa, _materialized, _raw_view, device)
 
 __all__ = ["OperatorBatch", "assemble_many", "SingleLayerBatch"]
 
@@ -112,8 +112,8 @@ class OperatorBatch(BaseMatrix):
         dt = torch.complex128 if self.is_complex else torch.float64
         if not self.is_complex and (torch.is_complex(x) or torch.is_complex(y)):
             raise TypeError("real batch needs real vectors")
-        xc = x.to(device=y.device, dtype=dt).contiguous()
-        yc = y if y.is_contiguous() else y.contiguous()
+        xc = _materialized(x.to(device=y.device, dtype=dt)).contiguous()
+        yc = y if _raw_view(y) else _materialized(y).contiguous().clone()
         a = np.array([complex(s).real, complex(s).imag])
         one = np.array([1.0, 0.0])
         with torch.cuda.device(y.device):

@@ -54,7 +54,22 @@ def _as_vec(x, dtype, dev):
         x = torch.from_numpy(np.ascontiguousarray(x))
     if not isinstance(x, torch.Tensor):
         raise TypeError("vectors are torch tensors or numpy arrays")
-    return x.to(device=dev, dtype=dtype)
+    return _materialized(x.to(device=dev, dtype=dtype))
+
+
+def _materialized(t):
+    """t with torch's lazy conjugate / negative views resolved: the library
+    reads and writes the raw memory, which such a view does not hold."""
+    if t.is_conj():
+        t = t.resolve_conj()
+    if t.is_neg():
+        t = t.resolve_neg()
+    return t
+
+
+def _raw_view(t):
+    """True when t's memory holds its values as they read (no lazy bits)."""
+    return t.is_contiguous() and not t.is_conj() and not t.is_neg()
 
 
 # ------------------------------------------------------------- BaseMatrix --
@@ -367,15 +382,20 @@ class HMatrix(BaseMatrix):
             if s.imag != 0.0:
                 raise TypeError("complex scalar with a real vector")
             xr = _as_vec(x, torch.float64, y.device).contiguous()
-            op._call_apply(flags, s.real, xr, 1.0, y)
+            if _raw_view(y):
+                op._call_apply(flags, s.real, xr, 1.0, y)
+            else:
+                tmp = _materialized(y).contiguous().clone()
+                op._call_apply(flags, s.real, xr, 1.0, tmp)
+                y.copy_(tmp)
             return
         if not torch.is_complex(y):
             raise TypeError("complex operator needs a complex output vector")
         xc = _as_vec(x, torch.complex128, y.device).contiguous()
-        if y.is_contiguous():
+        if _raw_view(y):
             op._call_apply(flags, s, xc, 1.0, y)
         else:
-            tmp = y.contiguous()
+            tmp = _materialized(y).contiguous().clone()
             op._call_apply(flags, s, xc, 1.0, tmp)
             y.copy_(tmp)
 
@@ -386,7 +406,7 @@ class HMatrix(BaseMatrix):
         # y = A x in one call (beta = 0): no separate clearing of y
         if (self.is_complex and torch.is_complex(x) and x.dtype == y.dtype == torch.complex128 and x.is_cuda
                 and y.is_cuda and x.dim() == 1 and x.shape[0] == self.width and y.shape[0] == self.height
-                and x.is_contiguous() and y.is_contiguous()):
+                and _raw_view(x) and _raw_view(y)):
             self._op._call_apply(_capi.APPLY_MIXED if self._mixed else 0, 1.0, x, 0.0, y)
             return
         BaseMatrix.Mult(self, x, y)
