@@ -41,3 +41,20 @@ def test_conjugate_and_negative_views(gpu, ops):
     # non-contiguous input
     X = torch.stack([x, 2 * x], 1)
     assert torch.equal(C.mat * X[:, 1], C.mat * (2 * x))
+
+
+def test_matrix_algebra_adjoints(gpu, ops):
+    """Scaled, summed, transposed and adjoint combinations against their
+    definitions: (c A + B)^H x = conj(c) A^H x + B^H x, and (M^H)^T = conj(M)."""
+    C, V = ops
+    n = C.trial_space.ndof
+    g = torch.Generator(device="cpu").manual_seed(1)
+    x = torch.complex(torch.randn(n, generator=g, dtype=torch.float64),
+                      torch.randn(n, generator=g, dtype=torch.float64)).to(gpu)
+    c = 0.3 - 2.0j
+    M = c * C.mat + V.mat
+    ref = np.conj(c) * (C.mat.H * x) + V.mat.T * x
+    tol = 1e-13 * float(ref.abs().max())
+    assert torch.allclose(M.H * x, ref, rtol=0, atol=tol)
+    assert torch.allclose(M.T * x, c * (C.mat.T * x) + V.mat.T * x, rtol=0, atol=tol)
+    assert torch.allclose(M.H.T * x, (M * x.conj()).conj(), rtol=0, atol=tol)
