@@ -120,3 +120,32 @@ def test_efie_order1_vs_mie(gpu):
     # the oracle pins give 0.143 / 0.068 at level 2 (order 0 / 1)
     assert np.all(errs[0] < 0.08) and np.all(errs[1] < 0.035)
     assert np.all(errs[1] < 0.6 * errs[0])
+
+
+def test_order1_on_region_matches_oracle(gpu):
+    """HDivSurface / HCurl order 1 on part of the mesh (compressed dofs)."""
+    from test_gpu_regions import capped_sphere
+
+    m = capped_sphere(3, zcut=-0.3)
+    cap = m.Boundaries("cap")
+    hd, hc = nb.HDivSurface(m, order=1, definedon=cap), nb.HCurl(m, order=1, definedon=cap)
+    kw = dict(intorder=3, leafsize=24, eta=2.0, eps=1e-8)
+    for name, op, kid, tk, sk in (
+            ("MSL", nb.MaxwellSingleLayerPotentialOperator(hd, 3.0, **kw), O.MAXWELL_SL, O.HDIV1, O.HDIV1),
+            ("MDL", nb.MaxwellDoubleLayerPotentialOperator(hc, hd, 3.0, **kw), O.MAXWELL_DL, O.HCURL1, O.HDIV1)):
+        print(f"region order-1 {name}: ndof {op.trial_space.ndof}", flush=True)
+        ref = O.OracleOp(m, kid, kappa=3.0, trial=tk, test=sk, trial_mask=cap.mask, test_mask=cap.mask, **kw)
+        assert (op.trial_space.ndof, op.test_space.ndof) == (ref.ntrial, ref.ntest)
+        for b in range(op.num_blocks()):
+            d = op.block_data(b)
+            if isinstance(d, tuple):
+                ref.set_block(b, np.concatenate([d[0].ravel(), d[1].ravel()]), d[0].shape[1])
+            else:
+                rows, cols = op.block_dofs(b)
+                D = ref.dense(cols, rows)
+                assert np.abs(d - D).max() <= 1e-12 * max(np.abs(D).max(), 1e-300), f"{name} block {b}"
+                ref.set_block(b, d.ravel(), -1)
+        x = np.random.default_rng(4).standard_normal(op.trial_space.ndof) + 0j
+        y = (op.mat * torch.from_numpy(x).to(gpu)).cpu().numpy()
+        y_ref = ref.apply(x)
+        assert np.linalg.norm(y - y_ref) <= 1e-13 * np.linalg.norm(y_ref)
