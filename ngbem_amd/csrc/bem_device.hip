@@ -1299,6 +1299,96 @@ __global__ __launch_bounds__(256) void k_project(const ProjTask* __restrict__ ta
   }
 }
 
+constexpr int PROJ_P0_NTQ = 16;  // point rules up to 16 points (the per-dof projections)
+
+// Spaces whose dofs touch at most two elements (SurfaceL2 of any order: one;
+// HDivSurface / HCurl: the edge's two triangles): one thread per dof of a
+// block side, all of that side's columns in a loop.  Per kernel term the
+// dof's shape value times J w at every point of its (at most two) elements is
+// formed once and kept in registers, instead of once per column; the sums run
+// over the same terms in the same order as k_project (bit-identical factors).
+template <class KER, class EI, class EJ>
+__global__ __launch_bounds__(256) void k_project_dof(const ProjTask* __restrict__ tasks, const int4* __restrict__ items,
+                                                     const FarBlk* __restrict__ fb, const int* __restrict__ fpatch,
+                                                     const int* __restrict__ maps, const double* __restrict__ quad,
+                                                     const double* __restrict__ sdat, const double* __restrict__ trig,
+                                                     int ntq, KER ker, const typename KER::T* __restrict__ wsU,
+                                                     const typename KER::T* __restrict__ wsV,
+                                                     typename KER::T* __restrict__ oU, typename KER::T* __restrict__ oV) {
+  typedef typename KER::T T;
+  const int4 it = items[blockIdx.x];
+  const ProjTask tk = tasks[it.x];
+  const FarBlk b = fb[tk.blk];
+  const bool isU = it.y == 0;
+  const int len = isU ? b.m : b.n;
+  const int pos = it.z + (int)threadIdx.x;
+  if (pos >= len) return;
+  const int* mp = maps + (isU ? b.rmap : b.cmap);
+  const int* P = fpatch + (isU ? b.pI : b.pJ);
+  const int k0 = mp[pos], ne = min(mp[pos + 1] - k0, 2);
+  int a[2] = {0, 0}, kl[2] = {0, 0}, el[2] = {0, 0};
+  for (int k2 = 0; k2 < ne; k2++) {
+    const int ent = mp[len + 1 + k0 + k2];
+    a[k2] = ent / DOF_STRIDE;
+    kl[k2] = ent % DOF_STRIDE;
+    el[k2] = P[a[k2]];
+  }
+  const int idp[3] = {0, 1, 2};
+  const int np = isU ? tk.np : tk.nq;
+  int col = 0;
+#pragma unroll
+  for (int t = 0; t < KER::NTERM; t++) {
+    const TermC tm = ker.term(t);
+    const int comp = isU ? tm.tec : tm.trc;
+    double w[2][PROJ_P0_NTQ];  // fully unrolled (predicated) loops keep it in registers
+#pragma unroll
+    for (int k2 = 0; k2 < 2; k2++) {
+      double bb[EI::NB > EJ::NB ? EI::NB : EJ::NB];
+      if (k2 < ne) {
+        if (isU) EI::load(sdat + (size_t)el[k2] * SDS, idp, bb);
+        else EJ::load(sdat + (size_t)el[k2] * SDS, idp, bb);
+      }
+      const int idx = kl[k2] * (isU ? EI::DIM : EJ::DIM) + comp;
+#pragma unroll
+      for (int q = 0; q < PROJ_P0_NTQ; q++) {
+        w[k2][q] = 0.0;
+        if (k2 < ne && q < ntq) {
+          const double lam[3] = {trig[3 * q], trig[3 * q + 1], 1.0 - trig[3 * q] - trig[3 * q + 1]};
+          const double* qp = quad + ((size_t)el[k2] * ntq + q) * 4;
+          const P3 X = ld3(qp);
+          double sh = 0.0;
+          if (isU) {
+            double v[EI::NL * EI::DIM];
+            EI::shape(lam, X, bb, v);
+#pragma unroll
+            for (int i = 0; i < EI::NL * EI::DIM; i++)
+              if (i == idx) sh = v[i];
+          } else {
+            double v[EJ::NL * EJ::DIM];
+            EJ::shape(lam, X, bb, v);
+#pragma unroll
+            for (int i = 0; i < EJ::NL * EJ::DIM; i++)
+              if (i == idx) sh = v[i];
+          }
+          w[k2][q] = sh * qp[3];
+        }
+      }
+    }
+    const int rk = tk.rank[tm.kc];
+    const T* W = isU ? wsU + tk.uoff[tm.kc] : wsV + tk.voff[tm.kc];
+    for (int l = 0; l < rk; l++, col++, W += np) {
+      T s = zero_<T>();
+#pragma unroll
+      for (int k2 = 0; k2 < 2; k2++)
+#pragma unroll
+        for (int q = 0; q < PROJ_P0_NTQ; q++)
+          if (k2 < ne && q < ntq) s += rmul(W[a[k2] * ntq + q], w[k2][q]);
+      if (isU) oU[tk.ou + (size_t)col * b.m + pos] = rmul(s, tm.fac);
+      else oV[tk.ov + (size_t)col * b.n + pos] = s;
+    }
+  }
+}
+
 // Both sides SurfaceL2 order 0 (one element per dof, shape value 1): one
 // thread per patch element a of a block side, all of that side's columns in
 // a loop.  The element's point weights J w are loaded once, consecutive
@@ -1306,8 +1396,6 @@ __global__ __launch_bounds__(256) void k_project(const ProjTask* __restrict__ ta
 // workspace, and the dof's row / column position comes from the dof table and
 // the inverse permutation.  The sums are those of k_project term by term
 // (shape 1.0 times J w), so the factors are bit-identical.
-constexpr int PROJ_P0_NTQ = 16;  // point rules up to 16 points
-
 template <class KER>
 __global__ __launch_bounds__(256) void k_project_p0(const ProjTask* __restrict__ tasks, const int4* __restrict__ items,
                                                     const FarBlk* __restrict__ fb, const int* __restrict__ fpatch,
@@ -2266,6 +2354,16 @@ void assemble_near(Operator& op, hipStream_t s) {
 
 // -------------------------------------------------------------- far field --
 
+// the per-dof projection for SurfaceL2 / edge spaces (default); NGBEM_PROJ=generic
+// keeps k_project
+static bool proj_dof_on() {
+  static const bool on = [] {
+    const char* e = std::getenv("NGBEM_PROJ");
+    return !(e && std::string(e) == "generic");
+  }();
+  return on;
+}
+
 void assemble_far(Operator& op, hipStream_t s) {
   const double t0 = now();
   op.far_factor_entries = 0;
@@ -2596,6 +2694,10 @@ void assemble_far(Operator& op, hipStream_t s) {
         const FarBlk& fb = op.far[ptasks[t].blk];
         p0 = fb.pm == fb.m && fb.pn == fb.n;
       }
+      // dofs on at most two elements (SurfaceL2 of any order, edge spaces): k_project_dof
+      auto two_elems = [](int k) { return k == SP_P0 || k == SP_L2_1 || k == SP_L2_2 || k == SP_L2_3 || is_edge_kind(k); };
+      const bool dof_proj = !p0 && two_elems(op.test->kind) && two_elems(op.trial->kind) && ntq <= PROJ_P0_NTQ &&
+                            proj_dof_on();
       if (p0) {
         std::vector<int4> it4;
         for (size_t t = 0; t < ptasks.size(); t++) {
@@ -2613,6 +2715,25 @@ void assemble_far(Operator& op, hipStream_t s) {
                                op.d_patch.as<int>(), op.d_dofs_test.as<int>(), op.d_dofs_trial.as<int>(),
                                op.d_iperm_test.as<int>(), op.d_iperm_trial.as<int>(), op.d_quad.as<double>(), ntq,
                                ker, SU.as<T>(), SV.as<T>(), OU.as<T>(), OV.as<T>());
+          });
+        items.clear();
+      } else if (dof_proj) {
+        std::vector<int4> it4;
+        for (size_t t = 0; t < ptasks.size(); t++) {
+          const FarBlk& fb = op.far[ptasks[t].blk];
+          for (int a0 = 0; a0 < fb.m; a0 += 256) it4.push_back(make_int4((int)t, 0, a0, 0));
+          for (int a0 = 0; a0 < fb.n; a0 += 256) it4.push_back(make_int4((int)t, 1, a0, 0));
+        }
+        d_items.upload(it4, s);
+        if (!it4.empty())
+          dispatch_full(op, [&](auto ker, auto ei, auto ej) {
+            using KER = decltype(ker);
+            using T = typename KER::T;
+            hipLaunchKernelGGL((k_project_dof<KER, decltype(ei), decltype(ej)>), dim3((unsigned)it4.size()),
+                               dim3(256), 0, s, d_ptasks.as<ProjTask>(), d_items.as<int4>(), op.d_far_blk.as<FarBlk>(),
+                               op.d_patch.as<int>(), op.d_far_maps.as<int>(), op.d_quad.as<double>(),
+                               op.d_sdat.as<double>(), op.d_trig.as<double>(), ntq, ker, SU.as<T>(), SV.as<T>(),
+                               OU.as<T>(), OV.as<T>());
           });
         items.clear();
       } else {
