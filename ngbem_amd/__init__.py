@@ -9,7 +9,7 @@ C-ABI in include/ngbem_amd.h.
 """
 from .mesh import Mesh, Region, icosphere, ellipsoid, box, read_mesh, read_stl, read_gmsh, read_netgen_vol, orient_outward
 from .spaces import SurfaceL2, H1, HDivSurface, HCurl
-from .operators import (BaseMatrix, HMatrix, IntegralOperator, IntegralOperatorC, MassMatrix,
+from .operators import (BaseMatrix, BlockMatrix, BlockVector, HMatrix, IntegralOperator, IntegralOperatorC, MassMatrix,
                         SingleLayerPotentialOperator, DoubleLayerPotentialOperator, HypersingularOperator,
                         HelmholtzSingleLayerPotentialOperator, HelmholtzDoubleLayerPotentialOperator,
                         HelmholtzCombinedFieldOperator, HelmholtzHypersingularOperator,
@@ -29,7 +29,7 @@ def reserve(gib: float) -> None:
     _capi.check(_capi.lib().bem_reserve(float(gib) * 2 ** 30), "bem_reserve")
 
 __all__ = ["Mesh", "Region", "icosphere", "ellipsoid", "box", "read_mesh", "read_stl", "read_gmsh", "read_netgen_vol", "orient_outward", "SurfaceL2", "H1", "HDivSurface", "HCurl", "BaseMatrix", "HMatrix", "IntegralOperator",
-           "IntegralOperatorC", "MassMatrix", "SingleLayerPotentialOperator", "DoubleLayerPotentialOperator",
+           "IntegralOperatorC", "MassMatrix", "BlockMatrix", "BlockVector", "SingleLayerPotentialOperator", "DoubleLayerPotentialOperator",
            "HypersingularOperator", "HelmholtzSingleLayerPotentialOperator", "HelmholtzDoubleLayerPotentialOperator",
            "HelmholtzCombinedFieldOperator", "HelmholtzHypersingularOperator", "MaxwellSingleLayerPotentialOperator",
            "MaxwellSingleLayerPotentialOperatorCurl", "MaxwellDoubleLayerPotentialOperator",

@@ -21,7 +21,7 @@ from . import _capi
 from .mesh import Mesh, Region
 from .spaces import FESpace, H1, HCurl, HDivSurface, SurfaceL2, mass_matrix_coo
 
-__all__ = ["BaseMatrix", "HMatrix", "IntegralOperator", "IntegralOperatorC", "MassMatrix",
+__all__ = ["BaseMatrix", "HMatrix", "IntegralOperator", "IntegralOperatorC", "MassMatrix", "BlockMatrix", "BlockVector",
            "SingleLayerPotentialOperator", "DoubleLayerPotentialOperator", "HypersingularOperator",
            "HelmholtzSingleLayerPotentialOperator", "HelmholtzDoubleLayerPotentialOperator",
            "HelmholtzCombinedFieldOperator", "HelmholtzHypersingularOperator",
@@ -216,6 +216,55 @@ class AdjointMatrix(BaseMatrix):
         tmp = torch.zeros_like(y)
         self.A.MultAdd(1.0, x.conj() if torch.is_complex(x) else x, tmp)
         y.add_((s * tmp.conj()) if torch.is_complex(tmp) else s * tmp)
+
+
+class BlockMatrix(BaseMatrix):
+    """NGSolve's ``BlockMatrix([[A, B], [C, D]])`` (entries may be None) on
+    vectors that are the concatenation of the blocks' vectors
+    (:func:`BlockVector`); each block's height / width is read from any
+    entry of its block row / column."""
+
+    def __init__(self, blocks):
+        rows, cols = len(blocks), len(blocks[0])
+        if any(len(r) != cols for r in blocks):
+            raise ValueError("BlockMatrix: rows of different lengths")
+        hs, ws = [None] * rows, [None] * cols
+        for i, r in enumerate(blocks):
+            for j, b in enumerate(r):
+                if b is None:
+                    continue
+                if (hs[i] not in (None, b.height)) or (ws[j] not in (None, b.width)):
+                    raise ValueError("BlockMatrix: block sizes do not line up")
+                hs[i], ws[j] = b.height, b.width
+        if None in hs or None in ws:
+            raise ValueError("BlockMatrix: every block row and column needs one entry")
+        self.blocks = blocks
+        self.roff = np.concatenate([[0], np.cumsum(hs)]).astype(int)
+        self.coff = np.concatenate([[0], np.cumsum(ws)]).astype(int)
+        self.height, self.width = int(self.roff[-1]), int(self.coff[-1])
+        self.is_complex = any(b.is_complex for r in blocks for b in r if b is not None)
+
+    def MultAdd(self, s, x, y):
+        for i, r in enumerate(self.blocks):
+            for j, b in enumerate(r):
+                if b is not None:
+                    b.MultAdd(s, x[..., self.coff[j]:self.coff[j + 1]], y[..., self.roff[i]:self.roff[i + 1]])
+
+    def MultTransAdd(self, s, x, y):
+        for i, r in enumerate(self.blocks):
+            for j, b in enumerate(r):
+                if b is not None:
+                    b.MultTransAdd(s, x[..., self.roff[i]:self.roff[i + 1]], y[..., self.coff[j]:self.coff[j + 1]])
+
+
+def BlockVector(parts):
+    """NGSolve's ``BlockVector([a, b])``: the concatenated vector a BlockMatrix
+    acts on (slice it by the block sizes to get the parts back), on the device
+    of the first torch part (the GPU when all parts are numpy arrays)."""
+    dev = next((p.device for p in parts if isinstance(p, torch.Tensor)), None) or device()
+    parts = [p if isinstance(p, torch.Tensor) else torch.from_numpy(np.asarray(p)) for p in parts]
+    dt = torch.complex128 if any(torch.is_complex(p) for p in parts) else torch.float64
+    return torch.cat([p.to(device=dev, dtype=dt) for p in parts], dim=-1)
 
 
 class MassMatrix(BaseMatrix):

@@ -90,6 +90,17 @@ class FESpace:
         """Triangles the space lives on."""
         return np.flatnonzero(self.elem_dofs[:, 0] >= 0)
 
+    def FreeDofs(self):
+        """NGSolve's ``fes.FreeDofs()``: a boolean array over the dofs, False for
+        the dofs of the triangles in the ``dirichlet`` region (a regex over the
+        region names, or a Region), True elsewhere."""
+        free = np.ones(self.ndof, dtype=bool)
+        if self.dirichlet is not None:
+            reg = self.dirichlet if isinstance(self.dirichlet, Region) else self.mesh.Boundaries(self.dirichlet)
+            d = self.elem_dofs[reg.mask]
+            free[d[d >= 0]] = False
+        return free
+
     @property
     def handle(self):
         return self._h.ptr

@@ -51,3 +51,22 @@ def test_cg_and_refined_solve():
     A32 = Dense(A.to(torch.float32).to(torch.float64))
     x2, hist = K.RefinedSolve(Dense(A), A32, b, tol=1e-12, inner_tol=1e-4)
     assert hist[-1] <= 1e-12
+
+
+def test_gmres_left_preconditioner_drops_fixed_rows():
+    """A preconditioner that is zero on some dofs (a mass inverse on the free
+    dofs) makes GMRes solve the free rows for the free unknowns and leave the
+    fixed unknowns at zero, as ngsolve's left-preconditioned GMRes does; a
+    right-preconditioned iteration would instead minimise the residual of all
+    rows and miss the free-rows solution."""
+    A, rng = _system(n=50, seed=3)
+    b = torch.from_numpy(rng.standard_normal(50) + 1j * rng.standard_normal(50))
+    free = torch.from_numpy(rng.random(50) < 0.6)
+    fi = torch.nonzero(free).reshape(-1)
+    ref = torch.zeros_like(b)
+    ref[fi] = torch.linalg.solve(A[fi][:, fi], b[fi])
+    D = torch.diag(free.to(torch.complex128) / torch.from_numpy(1.0 + rng.random(50)))
+    for pre, kw in ((lambda r: D @ r, {}), (Dense(D), dict(restart=8))):
+        x = K.GMRes(Dense(A), b, pre=pre, tol=1e-13, maxsteps=300, printrates=False, **kw)
+        assert (torch.linalg.norm(x - ref) / torch.linalg.norm(ref)).item() < 1e-11
+        assert torch.all(x[~free] == 0)
