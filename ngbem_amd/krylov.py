@@ -154,26 +154,29 @@ def GMRes(A, b, pre=None, freedofs=None, x=None, maxsteps=100, tol=None, innerpr
 
 
 def GMResBatch(A, B, pre=None, maxsteps=100, tol=1e-7, restart=None, printrates=False):
-    """Right-preconditioned GMRES for the k right-hand sides in the rows of B
-    (k x n), each with its own Krylov space, all advanced together: every step
-    applies A to the k new basis vectors as one block product (the matrix-core
-    path of the H-matrix) and orthogonalises them with batched CGS2.  Stops when
-    every row has ||r_i|| <= tol ||b_i||; returns X (k x n)."""
+    """GMRES for the k right-hand sides in the rows of B (k x n), each with its
+    own Krylov space, all advanced together: every step applies A to the k new
+    basis vectors as one block product (the matrix-core path of the H-matrix)
+    and orthogonalises them with batched CGS2.  Left-preconditioned like
+    :func:`GMRes` (a ``pre`` that is zero on some dofs drops those rows); stops
+    when every row has ||pre r_i|| <= tol ||pre b_i||; returns X (k x n)."""
     dt = torch.complex128 if (torch.is_complex(B) or getattr(A, "is_complex", False)) else torch.float64
     B = B.to(dt)
     k, n = B.shape
     X = torch.zeros_like(B)
     m = min(restart or maxsteps, maxsteps)
-    bnorm = torch.linalg.norm(B, dim=1).cpu().numpy()
-    bnorm[bnorm == 0] = 1.0
+    P = (lambda v: _apply(pre, v)) if pre is not None else (lambda v: v)  # noqa: E731
     V = torch.empty(m + 1, k, n, dtype=dt, device=B.device)
-    Z = torch.empty(m, k, n, dtype=dt, device=B.device) if pre is not None else None
     total = 0
     res = np.full(k, np.inf)
+    bnorm = None  # per row: the preconditioned residual norm at the start
     while total < maxsteps:
-        R = B - _apply(A, X)
+        R = P(B - _apply(A, X)).to(dt)
         beta = torch.linalg.norm(R, dim=1)
         beta_h = beta.cpu().numpy()
+        if bnorm is None:
+            bnorm = beta_h.copy()
+            bnorm[bnorm == 0] = 1.0
         if np.all(beta_h <= tol * bnorm):
             res = beta_h / bnorm
             break
@@ -185,10 +188,7 @@ def GMResBatch(A, B, pre=None, maxsteps=100, tol=1e-7, restart=None, printrates=
         g[:, 0] = beta_h
         used = 0
         for j in range(m):
-            z = _apply(pre, V[j]) if pre is not None else V[j]
-            if Z is not None:
-                Z[j] = z
-            w = _apply(A, z)
+            w = P(_apply(A, V[j])).to(dt)
             Vk = V[: j + 1].transpose(0, 1).contiguous()  # k x (j+1) x n
             h = torch.bmm(Vk.conj(), w.unsqueeze(2)).squeeze(2)  # k x (j+1)
             w = w - torch.bmm(h.unsqueeze(1), Vk).squeeze(1)
@@ -230,7 +230,7 @@ def GMResBatch(A, B, pre=None, maxsteps=100, tol=1e-7, restart=None, printrates=
                 if q:
                     Y[i, :q] = np.linalg.solve(Hi[:q, :q], g[i, :q])
         Yt = torch.from_numpy(Y if torch.is_complex(X) else Y.real.copy()).to(device=X.device, dtype=dt)
-        basis = (Z if Z is not None else V)[:used].transpose(0, 1).contiguous()  # k x used x n
+        basis = V[:used].transpose(0, 1).contiguous()  # k x used x n
         X.add_(torch.bmm(Yt.unsqueeze(1), basis).squeeze(1))
         if np.all(res <= tol):
             break

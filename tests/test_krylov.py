@@ -66,7 +66,10 @@ def test_gmres_left_preconditioner_drops_fixed_rows():
     ref = torch.zeros_like(b)
     ref[fi] = torch.linalg.solve(A[fi][:, fi], b[fi])
     D = torch.diag(free.to(torch.complex128) / torch.from_numpy(1.0 + rng.random(50)))
-    for pre, kw in ((lambda r: D @ r, {}), (Dense(D), dict(restart=8))):
+    for pre, kw in ((lambda r: r @ D.T, {}), (Dense(D), dict(restart=8))):
         x = K.GMRes(Dense(A), b, pre=pre, tol=1e-13, maxsteps=300, printrates=False, **kw)
         assert (torch.linalg.norm(x - ref) / torch.linalg.norm(ref)).item() < 1e-11
         assert torch.all(x[~free] == 0)
+        X = K.GMResBatch(Dense(A), torch.stack([b, 2 * b]), pre=pre, tol=1e-13, maxsteps=300, **kw)
+        assert (torch.linalg.norm(X[1] - 2 * ref) / torch.linalg.norm(2 * ref)).item() < 1e-11
+        assert torch.all(X[:, ~free] == 0)
