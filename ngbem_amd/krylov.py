@@ -71,7 +71,7 @@ def GMRes(A, b, pre=None, freedofs=None, x=None, maxsteps=100, tol=None, innerpr
     that is zero on some dofs (a mass inverse on ``FreeDofs()``) therefore
     drops those rows of the system and leaves x unchanged there, which is how
     the reference's mixed Dirichlet / Neumann scripts impose the given data
-    (tests/Laplace_Convergence.py:52-55).  Without ``pre`` the criterion is
+    (tests/Laplace_Convergence.py:57-61).  Without ``pre`` the criterion is
     ||b - A x|| <= tol ||b - A x0||.
 
     The Krylov basis is one (m+1) x n tensor on the GPU; each step orthogonalises

@@ -1,7 +1,7 @@
 """Dirichlet dof flags and block systems on the CPU: ``FreeDofs`` against the
 mesh's region membership, ``BlockMatrix`` / ``BlockVector`` against the dense
 block matrix (NGSolve's classes of the same names, as the reference's
-tests/Laplace_Convergence.py:44-55 builds its Calderon system)."""
+tests/Laplace_Convergence.py:27-61 builds its Calderon system)."""
 import numpy as np
 import torch
 
