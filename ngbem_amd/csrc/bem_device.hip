@@ -2694,10 +2694,18 @@ void assemble_far(Operator& op, hipStream_t s) {
         const FarBlk& fb = op.far[ptasks[t].blk];
         p0 = fb.pm == fb.m && fb.pn == fb.n;
       }
-      // dofs on at most two elements (SurfaceL2 of any order, edge spaces): k_project_dof
-      auto two_elems = [](int k) { return k == SP_P0 || k == SP_L2_1 || k == SP_L2_2 || k == SP_L2_3 || is_edge_kind(k); };
-      const bool dof_proj = !p0 && two_elems(op.test->kind) && two_elems(op.trial->kind) && ntq <= PROJ_P0_NTQ &&
-                            proj_dof_on();
+      // dofs on at most two elements (SurfaceL2 of any order, edge spaces on a
+      // manifold mesh): k_project_dof.  An edge shared by three or more
+      // triangles (a non-manifold mesh) sends its space to the generic k_project,
+      // which sums over every element of the dof.
+      auto two_elems = [](const Space& sp) {
+        const int k = sp.kind;
+        if (!(k == SP_P0 || k == SP_L2_1 || k == SP_L2_2 || k == SP_L2_3 || is_edge_kind(k))) return false;
+        for (int d = 0; d < sp.ndof; d++)
+          if (sp.ed_off[d + 1] - sp.ed_off[d] > 2) return false;
+        return true;
+      };
+      const bool dof_proj = !p0 && ntq <= PROJ_P0_NTQ && proj_dof_on() && two_elems(*op.test) && two_elems(*op.trial);
       if (p0) {
         std::vector<int4> it4;
         for (size_t t = 0; t < ptasks.size(); t++) {

@@ -354,18 +354,24 @@ class _MassInverse(BaseMatrix):
         r = b.clone()
         p = r * dinv
         rz = (r.conj() * p).sum(-1, keepdim=True)
-        nb = torch.linalg.norm(b, dim=-1, keepdim=True).clamp_min(1e-300)
+        nb = torch.linalg.norm(b, dim=-1, keepdim=True)
+        one = torch.ones_like(rz)
         for _ in range(self._maxsteps):
-            if bool((torch.linalg.norm(r, dim=-1, keepdim=True) <= self._tol * nb).all()):
+            # every right-hand side stops on its own: a converged row (a zero
+            # row from the start included) keeps z and r from then on, so its
+            # zero rz never enters a division
+            done = torch.linalg.norm(r, dim=-1, keepdim=True) <= self._tol * nb
+            if bool(done.all()):
                 break
             q = Mf(p)
-            alpha = rz / (p.conj() * q).sum(-1, keepdim=True)
+            pq = (p.conj() * q).sum(-1, keepdim=True)
+            alpha = torch.where(done, torch.zeros_like(rz), rz / torch.where(done, one, pq))
             z = z + alpha * p
             r = r - alpha * q
             w = r * dinv
             rz1 = (r.conj() * w).sum(-1, keepdim=True)
-            p = w + (rz1 / rz) * p
-            rz = rz1
+            p = w + torch.where(done, torch.zeros_like(rz), rz1 / torch.where(done, one, rz)) * p
+            rz = torch.where(done, rz, rz1)
         return z * f
 
     def MultAdd(self, s, x, y):

@@ -54,6 +54,10 @@ CASES = [
     ("MSL", 2, 2, 3, 3, 32, 2.0, 1e-6, 6.0),
     ("MDL", 3, 2, 3, 3, 32, 2.0, 1e-6, 6.0),
     ("HSLK", 0, 0, 3, 3, 32, 2.0, 1e-8, 5.0),
+    # double layer with H1 on both sides (the Laplace DL + HS problem's operator)
+    ("LDL", 1, 1, 2, 4, 16, 2.0, 1e-8, 0.0),
+    ("LDL", 1, 1, 3, 4, 40, 2.0, 1e-6, 0.0),
+    ("HDL", 1, 1, 3, 3, 16, 2.0, 1e-8, 3.5),
 ]
 
 
@@ -122,7 +126,7 @@ def test_blocks_match_oracle(gpu, case):
         assert same_rank >= 0.9 * nfar
 
 
-@pytest.mark.parametrize("case", CASES[:1] + CASES[4:6] + CASES[9:10] + CASES[11:15] + CASES[18:20],
+@pytest.mark.parametrize("case", CASES[:1] + CASES[4:6] + CASES[9:10] + CASES[11:15] + CASES[18:20] + CASES[21:],
                          ids=lambda c: f"{c[0]}-{c[1]}{c[2]}")
 def test_matvec_matches_oracle(gpu, case):
     op, ref = _build(case)

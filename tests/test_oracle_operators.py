@@ -6,7 +6,9 @@ transposed operand fails:
   exactly (surface curl of a constant), and the spherical-harmonic eigenvalues
   V: 1/(2n+1), K: -1/(2(2n+1)), W: n(n+1)/(2n+1) through Rayleigh quotients.
 - Helmholtz kernels reduce to Laplace as kappa -> 0; combined field =
-  DL - i kappa SL entry by entry.
+  DL - i kappa SL entry by entry; at finite kappa the Y_1 eigenvalues of V, K
+  and W on the unit sphere (spherical Bessel / Hankel closed forms), which
+  fix the hypersingular operator's -kappa^2 <n_x, n_y> G term.
 - ACA: ||D - U V||_F <= 10 eps ||D||_F against the dense block, and the best
   rank-R approximation (SVD) is never worse than the ACA one.
 - Tree and partition: permutation is a bijection, children partition the
@@ -88,6 +90,28 @@ def test_double_layer_h1_trial(sphere3):
     assert np.abs(K.sum(1) + 0.5 * m.areas()).max() / m.areas().max() < 2e-2
 
 
+@pytest.mark.parametrize("kappa", [0.0, 3.5])
+def test_double_layer_h1_h1(kappa):
+    """DL with H1 order 1 on both sides (the operator of the Laplace DL + HS
+    problem): K 1 = -1/2 M 1 (Laplace) and the Y_1 quotient z^T K z / z^T M z
+    tends to the sphere eigenvalue (-1/6, or i kappa^2 j_1 h_1' + 1/2)."""
+    m = M.icosphere(3)
+    if kappa == 0.0:
+        K = O.OracleOp(m, O.LAPLACE_DL, trial=O.P1, test=O.P1, intorder=4).dense()
+        assert np.abs(K.imag).max() == 0.0
+        ex = -1.0 / 6.0
+        Mp1 = mass_p1(m)
+        assert np.abs(K.real.sum(1) + 0.5 * Mp1.sum(1)).max() / Mp1.sum(1).max() < 2e-2
+    else:
+        K = O.OracleOp(m, O.HELMHOLTZ_DL, kappa=kappa, trial=O.P1, test=O.P1, intorder=4).dense()
+        ex = _sphere_eigs(kappa)[1]
+        Mp1 = mass_p1(m)
+    assert K.shape == (m.nv, m.nv)
+    z = m.xyz[:, 2]
+    rq = (z @ K @ z) / (z @ Mp1 @ z)
+    assert abs(rq - ex) < 0.02 * abs(ex), (rq, ex)
+
+
 def test_hypersingular_sphere():
     m = M.icosphere(2)
     W = O.OracleOp(m, O.LAPLACE_HS, trial=O.P1, test=O.P1, intorder=4).dense().real
@@ -123,6 +147,75 @@ def test_helmholtz_reduces_to_laplace(hk, lk, trial):
     if hk == O.HELMHOLTZ_SL:
         w = np.outer(m.areas(), m.areas())
         assert np.allclose(H.imag / kappa, w / (4 * np.pi), rtol=1e-6, atol=0)
+
+
+def _sphere_lambda(l, kappa):
+    """Eigenvalue of the Helmholtz single layer on the unit sphere for degree-l
+    spherical harmonics: i kappa j_l(kappa) h_l(kappa) (addition theorem of
+    e^{i kappa r}/(4 pi r); Colton & Kress, Inverse Acoustic and EM Scattering,
+    2.3)."""
+    from scipy.special import spherical_jn, spherical_yn
+
+    return 1j * kappa * spherical_jn(l, kappa) * (spherical_jn(l, kappa) + 1j * spherical_yn(l, kappa))
+
+
+def _sphere_eigs(kappa):
+    """Y_1 eigenvalues of V, K (= d/dn_y, principal value) and W (= -d/dn_x K)
+    on the unit sphere, from j_1, h_1 and their derivatives; W also from the
+    Maue form the reference integrates (src/ngbem.hpp:371-397):
+    W = curl-curl part - kappa^2 <n_x, n_y> part, whose Y_n eigenvalue is
+    n(n+1) lambda_n - kappa^2 ((n+1) lambda_{n+1} + n lambda_{n-1}) / (2n+1)
+    (x.y couples degree n to n +- 1).  Returns (V, K, W, W with the kappa^2
+    term's sign flipped, W with <n_x, n_y> dropped)."""
+    from scipy.special import spherical_jn, spherical_yn
+
+    j, jd = spherical_jn(1, kappa), spherical_jn(1, kappa, True)
+    hd = jd + 1j * spherical_yn(1, kappa, True)
+    lam = [_sphere_lambda(l, kappa) for l in range(3)]
+    V = lam[1]
+    K = 1j * kappa ** 2 * j * hd + 0.5
+    W = -1j * kappa ** 3 * jd * hd
+    maue = 2 * lam[1] - kappa ** 2 * (2 * lam[2] + lam[0]) / 3
+    assert abs(maue - W) <= 1e-13 * abs(W)
+    flipped = 2 * lam[1] + kappa ** 2 * (2 * lam[2] + lam[0]) / 3
+    dropped = 2 * lam[1] - kappa ** 2 * lam[1]
+    return V, K, W, flipped, dropped
+
+
+def test_sphere_eigs_reduce_to_laplace():
+    """kappa -> 0: V, K, W eigenvalues of Y_1 tend to 1/3, -1/6, 2/3."""
+    V, K, W, _, _ = _sphere_eigs(1e-4)
+    assert abs(V - 1 / 3) < 1e-6 and abs(K + 1 / 6) < 1e-6 and abs(W - 2 / 3) < 1e-6
+
+
+@pytest.mark.parametrize("kappa", [1.5, 3.5])
+def test_helmholtz_sphere_eigenvalues(kappa):
+    """Rayleigh quotients z^T A z / z^T M z (z = Y_1) of the oracle's Helmholtz
+    single layer, double layer (SurfaceL2 0) and hypersingular operator (H1 1)
+    converge at O(h^2) to the sphere eigenvalues at finite kappa.  kappa avoids
+    the zeros of j_1 and j_1' (interior resonances), where the relative error
+    would blow up.  The hypersingular quotient is more than 100% away from the
+    value with the -kappa^2 <n_x, n_y> G term's sign flipped or with <n_x, n_y>
+    dropped, so either mistake fails the 2% bar."""
+    V_ex, K_ex, W_ex, W_flip, W_drop = _sphere_eigs(kappa)
+    assert abs(W_flip - W_ex) > 1.0 * abs(W_ex) and abs(W_drop - W_ex) > 1.0 * abs(W_ex)
+    errs = {}
+    for lvl in (2, 3):
+        m = M.icosphere(lvl)
+        kw = dict(kappa=kappa, intorder=4)
+        zc, area = m.xyz[m.tri].mean(1)[:, 2], m.areas()
+        z1 = m.xyz[:, 2]
+        V = O.OracleOp(m, O.HELMHOLTZ_SL, trial=O.P0, test=O.P0, **kw).dense()
+        K = O.OracleOp(m, O.HELMHOLTZ_DL, trial=O.P0, test=O.P0, **kw).dense()
+        W = O.OracleOp(m, O.HELMHOLTZ_HS, trial=O.P1, test=O.P1, **kw).dense()
+        q0 = zc @ (area * zc)
+        rq = dict(V=(zc @ V @ zc) / q0, K=(zc @ K @ zc) / q0, W=(z1 @ W @ z1) / (z1 @ mass_p1(m) @ z1))
+        errs[lvl] = {k: abs(rq[k] - ex) / abs(ex) for k, ex in (("V", V_ex), ("K", K_ex), ("W", W_ex))}
+        # complex symmetric (not Hermitian) up to quadrature error
+        assert np.abs(W - W.T).max() <= 1e-3 * np.abs(W).max()
+    for k in "VKW":
+        assert errs[3][k] < 0.02, (k, errs)
+        assert errs[3][k] < errs[2][k] / 2.5, (k, errs)  # O(h^2): about 4x per level
 
 
 def test_aca_vs_dense_and_svd():
