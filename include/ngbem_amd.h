@@ -25,6 +25,7 @@
 #ifndef NGBEM_AMD_H
 #define NGBEM_AMD_H
 
+#include <stddef.h>
 #include <stdint.h>
 
 #ifdef __cplusplus
@@ -207,6 +208,22 @@ int bem_op_destroy(bem_op* op);
 int bem_op_potential_dim(const bem_op* op);
 int bem_op_potential(bem_op* op, const void* coef, const double* points, const double* normals, int npoints,
                      const unsigned char* elmask, void* out, int dtype, void* stream);
+
+/* bem_op_potential without allocating device memory, for callers that cannot
+ * call hipMalloc (code under HIP graph capture, tight serving loops,
+ * harnesses that forbid allocation).  No counterpart in the reference.  Same
+ * arguments, values and stream synchronisation as bem_op_potential (the same
+ * kernels; the partial sums are split over fewer chunks when they would not
+ * fit).  The element list, the sources and the partial sums live in one
+ * static device array of BEM_POTENTIAL_SCRATCH_BYTES per GPU, held until the
+ * process ends.  The sources take (64 + D s) bytes each, one per trial
+ * element and quadrature point (D = bem_op_potential_dim, s = 8 real / 16
+ * complex), plus 4 bytes per element: a call that needs more returns an
+ * error and computes nothing.  Every call on one GPU uses that GPU's array:
+ * calls that overlap on one GPU (two streams or threads) race. */
+#define BEM_POTENTIAL_SCRATCH_BYTES ((size_t)64 << 20)
+int bem_op_potential_noalloc(bem_op* op, const void* coef, const double* points, const double* normals,
+                             int npoints, const unsigned char* elmask, void* out, int dtype, void* stream);
 
 /* introspection for parity tests */
 int bem_op_num_blocks(const bem_op* op);

@@ -587,6 +587,18 @@ int bem_op_potential(bem_op* h, const void* coef, const double* points, const do
       [&] { bem::potential_device(h->op, coef, points, normals, npoints, elmask, out, (hipStream_t)stream); });
 }
 
+int bem_op_potential_noalloc(bem_op* h, const void* coef, const double* points, const double* normals, int npoints,
+                             const unsigned char* elmask, void* out, int dtype, void* stream) {
+  if (!h || !coef || !points || !out || npoints < 0) return fail(-1, "bem_op_potential_noalloc: bad arguments");
+  if (h->op.complex_valued != (dtype == BEM_C128)) return fail(-1, "bem_op_potential_noalloc: dtype does not match");
+  if (!h->op.kappa_el.empty())
+    return fail(-1, "bem_op_potential_noalloc: not available for several geometries in one operator");
+  if (npoints == 0) return 0;
+  return guarded([&] {
+    bem::potential_device_noalloc(h->op, coef, points, normals, npoints, elmask, out, (hipStream_t)stream);
+  });
+}
+
 // ------------------------------------------------------------------ batch --
 
 int bem_batch_create(bem_op* const* ops, int n, int flags, bem_batch** out) {

@@ -27,6 +27,7 @@
 #include <cmath>
 #include <cstring>
 #include <exception>
+#include <mutex>
 #include <numeric>
 #include <thread>
 #include <type_traits>
@@ -3240,46 +3241,122 @@ int potential_dim(const Operator& op) {
   return d;
 }
 
-void potential_device(Operator& op, const void* coef, const double* pts, const double* nrm, int np,
-                      const unsigned char* elmask, void* out, hipStream_t s) {
+template <class T>
+__global__ void k_fill_zero(T* __restrict__ out, int64_t n) {
+  const int64_t j = (int64_t)blockIdx.x * 256 + threadIdx.x;
+  if (j < n) out[j] = zero_<T>();
+}
+
+// Scratch memory of bem_op_potential_noalloc: the element list, the sources
+// and the partial sums of one call.  Every GPU the library runs on holds its
+// own copy until the process ends; calls on one GPU share it (they must not
+// overlap).  Each call writes every byte it reads back, so nothing has to be
+// cleared between calls.
+static __device__ __align__(256) unsigned char scratch[POTENTIAL_SCRATCH_BYTES];
+
+// device address of `scratch` per device, looked up once per device with that
+// device current (the host-side name is not a device address)
+static void* scratch_ptr() {
+  static std::mutex mu;
+  static std::vector<void*> ptrs;
+  int dev = 0;
+  BEM_HIP(hipGetDevice(&dev));
+  std::lock_guard<std::mutex> lk(mu);
+  if ((int)ptrs.size() <= dev) ptrs.resize(dev + 1, nullptr);
+  if (!ptrs[dev]) {
+    void* p = nullptr;
+    const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    if (e != hipSuccess || !p)
+      throw std::runtime_error(std::string("potential scratch: hipGetSymbolAddress failed: ") + hipGetErrorString(e));
+    ptrs[dev] = p;
+  }
+  return ptrs[dev];
+}
+
+static size_t align256(size_t n) { return (n + 255) & ~(size_t)255; }
+
+// The potential: sources of the (masked) trial elements, then the N-body sum
+// over source chunks (nsplit partial sums per point, added in chunk order).
+// noalloc: the element list, sources and partials live in `scratch`; with
+// too little room for the partials the split count shrinks, and with one
+// chunk the sums go straight to `out` (the same values the reduction of a
+// single chunk copies there).
+static void potential_run(Operator& op, const void* coef, const double* pts, const double* nrm, int np,
+                          const unsigned char* elmask, void* out, hipStream_t s, bool noalloc) {
   if (!op.assembled) throw std::runtime_error("operator not assembled");
   const int nt = op.mesh->nt, ntq = op.trig.size();
   // source elements: those of the trial space, restricted to the mask
-  std::vector<int> elist;
+  static thread_local std::vector<int> elist;
+  elist.clear();
   elist.reserve(nt);
   for (int e = 0; e < nt; e++)
     if (op.trial->has(e) && (!elmask || elmask[e])) elist.push_back(e);
   const int ne = (int)elist.size(), nsrc = ne * ntq;
-  if (ne == 0) {
-    BEM_HIP(hipMemsetAsync(out, 0, (size_t)np * potential_dim(op) * (op.complex_valued ? 16 : 8), s));
-    BEM_HIP(hipStreamSynchronize(s));
-    return;
-  }
-  DBuf d_elist;
-  d_elist.upload(elist, s);
   dispatch_full(op, [&](auto ker, auto, auto ej) {
     using KER = decltype(ker);
     using EJ = decltype(ej);
     using T = typename KER::T;
     constexpr int D = EJ::DIM;
-    DBuf src, part;
-    src.alloc((size_t)nsrc * sizeof(PotSrc<T, D>));
-    hipLaunchKernelGGL((k_pot_src<KER, EJ>), dim3((nsrc + 255) / 256), dim3(256), 0, s, d_elist.as<int>(), ne, ntq,
-                       op.d_quad.as<double>(), op.d_geo.as<double>(), op.d_sdat.as<double>(), op.d_trig.as<double>(),
-                       op.d_dofs_trial.as<int>(), (const T*)coef, src.as<PotSrc<T, D>>());
+    const int64_t n = (int64_t)np * D;
+    if (ne == 0) {
+      hipLaunchKernelGGL((k_fill_zero<T>), dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, (T*)out, n);
+      BEM_HIP(hipGetLastError());
+      BEM_HIP(hipStreamSynchronize(s));
+      return;
+    }
     // enough workgroups to fill the chip: split the sources when targets are few
     const int gx = (np + 255) / 256;
-    const int nsplit = std::max(1, std::min(std::max(1, 2048 / gx), (nsrc + 1023) / 1024));
+    int nsplit = std::max(1, std::min(std::max(1, 2048 / gx), (nsrc + 1023) / 1024));
+    DBuf d_elist, src, part;
+    int* el = nullptr;
+    PotSrc<T, D>* sp = nullptr;
+    T* pp = nullptr;
+    if (noalloc) {
+      const size_t off_src = align256((size_t)ne * sizeof(int));
+      const size_t off_part = off_src + align256((size_t)nsrc * sizeof(PotSrc<T, D>));
+      if (off_part > POTENTIAL_SCRATCH_BYTES)
+        throw std::runtime_error("bem_op_potential_noalloc: " + std::to_string(ne) + " source elements x " +
+                                 std::to_string(ntq) + " points need " + std::to_string(off_part) +
+                                 " bytes of scratch memory, more than its " +
+                                 std::to_string((size_t)POTENTIAL_SCRATCH_BYTES) + " bytes");
+      const size_t per_split = (size_t)n * sizeof(T);
+      const size_t room = POTENTIAL_SCRATCH_BYTES - off_part;
+      if (nsplit > 1 && (size_t)nsplit * per_split > room) nsplit = (int)std::max<size_t>(1, room / per_split);
+      unsigned char* base = (unsigned char*)scratch_ptr();
+      el = (int*)base;
+      sp = (PotSrc<T, D>*)(base + off_src);
+      pp = nsplit > 1 ? (T*)(base + off_part) : (T*)out;
+      BEM_HIP(hipMemcpyAsync(el, elist.data(), (size_t)ne * sizeof(int), hipMemcpyHostToDevice, s));
+    } else {
+      d_elist.upload(elist, s);
+      src.alloc((size_t)nsrc * sizeof(PotSrc<T, D>));
+      part.alloc((size_t)nsplit * n * sizeof(T));
+      el = d_elist.as<int>();
+      sp = src.as<PotSrc<T, D>>();
+      pp = part.as<T>();
+    }
     const int chunk = (nsrc + nsplit - 1) / nsplit;
-    part.alloc((size_t)nsplit * np * D * sizeof(T));
-    hipLaunchKernelGGL((k_potential<KER, D>), dim3(gx, nsplit), dim3(256), 0, s, src.as<PotSrc<T, D>>(), nsrc,
-                       chunk, pts, nrm, np, ker, part.as<T>());
-    const int64_t n = (int64_t)np * D;
-    hipLaunchKernelGGL((k_pot_reduce<T>), dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, part.as<T>(), nsplit, n,
-                       (T*)out);
+    hipLaunchKernelGGL((k_pot_src<KER, EJ>), dim3((nsrc + 255) / 256), dim3(256), 0, s, el, ne, ntq,
+                       op.d_quad.as<double>(), op.d_geo.as<double>(), op.d_sdat.as<double>(), op.d_trig.as<double>(),
+                       op.d_dofs_trial.as<int>(), (const T*)coef, sp);
+    hipLaunchKernelGGL((k_potential<KER, D>), dim3(gx, nsplit), dim3(256), 0, s, sp, nsrc, chunk, pts, nrm, np, ker,
+                       pp);
+    if (pp != (T*)out)
+      hipLaunchKernelGGL((k_pot_reduce<T>), dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, pp, nsplit, n,
+                         (T*)out);
     BEM_HIP(hipGetLastError());
     BEM_HIP(hipStreamSynchronize(s));
   });
+}
+
+void potential_device(Operator& op, const void* coef, const double* pts, const double* nrm, int np,
+                      const unsigned char* elmask, void* out, hipStream_t s) {
+  potential_run(op, coef, pts, nrm, np, elmask, out, s, false);
+}
+
+void potential_device_noalloc(Operator& op, const void* coef, const double* pts, const double* nrm, int np,
+                              const unsigned char* elmask, void* out, hipStream_t s) {
+  potential_run(op, coef, pts, nrm, np, elmask, out, s, true);
 }
 
 // ------------------------------------------------------ compression test --

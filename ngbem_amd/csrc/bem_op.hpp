@@ -293,6 +293,13 @@ void apply_device(Operator& op, int flags, const double* alpha, const void* x, c
 int potential_dim(const Operator& op);
 void potential_device(Operator& op, const void* coef, const double* pts, const double* nrm, int np,
                       const unsigned char* elmask, void* out, hipStream_t s);
+// the same values without allocating device memory: element list, sources and
+// partial sums in a static device array of POTENTIAL_SCRATCH_BYTES per GPU
+// (calls on one GPU share it and must not overlap); throws when the sources
+// do not fit
+constexpr size_t POTENTIAL_SCRATCH_BYTES = (size_t)64 << 20;
+void potential_device_noalloc(Operator& op, const void* coef, const double* pts, const double* nrm, int np,
+                              const unsigned char* elmask, void* out, hipStream_t s);
 // ACA of 1/|x_i - y_j| between two point clouds (TestCompression, method aca)
 void test_compression_aca(const double* X, int nx, const double* Y, int ny, double eps, int* rank, double* err,
                           double* seconds);
