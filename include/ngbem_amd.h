@@ -85,8 +85,8 @@ enum {
   BEM_BATCH_CONSUME = 1       /* bem_batch_create: move the operators' storage into the batch */
 };
 
-/* vector dtypes */
-enum { BEM_F64 = 0, BEM_C128 = 1 };
+/* vector dtypes; BEM_F32 / BEM_C64 only for the Krylov basis of bem_arnoldi_step */
+enum { BEM_F64 = 0, BEM_C128 = 1, BEM_F32 = 2, BEM_C64 = 3 };
 
 typedef struct {
   int intorder;  /* quadrature order (BEMParameters.intorder) */
@@ -224,6 +224,23 @@ int bem_op_potential(bem_op* op, const void* coef, const double* points, const d
 #define BEM_POTENTIAL_SCRATCH_BYTES ((size_t)64 << 20)
 int bem_op_potential_noalloc(bem_op* op, const void* coef, const double* points, const double* normals,
                              int npoints, const unsigned char* elmask, void* out, int dtype, void* stream);
+
+/* One Arnoldi step of GMRES on device vectors, for on-device Krylov solvers
+ * (the reference's scripts run ngsolve.krylovspace.GMRes, whose inner
+ * products are host reductions; no counterpart in ngbem itself).  V: basis
+ * rows of length n with stride ldv (device, k + 2 rows); w: A v_k (device,
+ * overwritten with the orthogonalised vector); h: device array of k + 2
+ * values receiving the Hessenberg column (the coefficients of both classical
+ * Gram-Schmidt passes added, then |w|); V row k + 1 receives w / |w| (zero
+ * when |w| = 0).  work: device buffer of bem_arnoldi_work_bytes(k, n, dtype)
+ * bytes.  dtype is the basis type: BEM_F64 / BEM_C128, or BEM_F32 / BEM_C64
+ * for the low-precision basis of a mixed-precision inner solve (values
+ * widened on load, products accumulated in FP64); w and h are double for a
+ * real basis and complex128 for a complex one.  k + 1 < 1024.  Asynchronous
+ * on stream; every inner product is a fixed-order reduction (deterministic). */
+int64_t bem_arnoldi_work_bytes(int k, int n, int dtype);
+int bem_arnoldi_step(void* V, int64_t ldv, int k, int n, void* w, void* h, void* work, int64_t work_bytes, int dtype,
+                     void* stream);
 
 /* introspection for parity tests */
 int bem_op_num_blocks(const bem_op* op);

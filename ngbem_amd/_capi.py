@@ -16,7 +16,7 @@ SPACE_P0, SPACE_P1, SPACE_HDIV, SPACE_HCURL = 0, 1, 2, 3
  MAXWELL_DL, HELMHOLTZ_SL_DKAPPA) = range(10)
 APPLY_TRANS, APPLY_CONJ, APPLY_MIXED, APPLY_ON_DEVICE, APPLY_PER_VECTOR, BATCH_EAGER = 1, 2, 4, 8, 16, 32
 BATCH_CONSUME = 1
-F64, C128 = 0, 1
+F64, C128, F32, C64 = 0, 1, 2, 3
 
 
 class Params(ctypes.Structure):
@@ -52,7 +52,7 @@ EXPORTS = ["bem_last_error", "bem_device_count", "bem_reserve", "bem_test_points
            "bem_space_dofs",
            "bem_space_ndof", "bem_space_destroy", "bem_op_create", "bem_op_create_multi", "bem_op_assemble", "bem_op_apply",
            "bem_op_stats", "bem_op_profile_apply", "bem_op_destroy", "bem_op_num_blocks", "bem_op_block_info", "bem_op_block_dofs",
-           "bem_op_block_data", "bem_op_tree", "bem_op_potential_dim", "bem_op_potential", "bem_op_potential_noalloc", "bem_batch_create", "bem_batch_sizes", "bem_batch_apply",
+           "bem_op_block_data", "bem_op_tree", "bem_op_potential_dim", "bem_op_potential", "bem_op_potential_noalloc", "bem_arnoldi_work_bytes", "bem_arnoldi_step", "bem_batch_create", "bem_batch_sizes", "bem_batch_apply",
            "bem_batch_destroy"]
 
 
@@ -100,6 +100,10 @@ def lib():
     L.bem_op_potential_dim.argtypes = [P]
     L.bem_op_potential.argtypes = [P, P, P, P, ctypes.c_int, P, P, ctypes.c_int, P]
     L.bem_op_potential_noalloc.argtypes = [P, P, P, P, ctypes.c_int, P, P, ctypes.c_int, P]
+    L.bem_arnoldi_work_bytes.restype = ctypes.c_int64
+    L.bem_arnoldi_work_bytes.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.c_int]
+    L.bem_arnoldi_step.argtypes = [P, ctypes.c_int64, ctypes.c_int, ctypes.c_int, P, P, P, ctypes.c_int64,
+                                   ctypes.c_int, P]
     L.bem_batch_create.argtypes = [ctypes.POINTER(P), ctypes.c_int, ctypes.c_int, ctypes.POINTER(P)]
     L.bem_batch_sizes.argtypes = [P, ctypes.POINTER(ctypes.c_int), ctypes.POINTER(ctypes.c_int),
                                   ctypes.POINTER(ctypes.c_int)]

@@ -47,8 +47,9 @@ def build(verbose: bool = False) -> str:
          os.path.join(CSRC, "bem_host.cpp"), "-o", os.path.join(bdir, "bem_host.o")],
         [hipcc] + HIP_FLAGS + ["-c", os.path.join(CSRC, "bem_device.hip"), "-o", os.path.join(bdir, "bem_device.o")],
         [hipcc] + HIP_FLAGS + ["-c", os.path.join(CSRC, "bem_capi.hip"), "-o", os.path.join(bdir, "bem_capi.o")],
+        [hipcc] + HIP_FLAGS + ["-c", os.path.join(CSRC, "bem_krylov.hip"), "-o", os.path.join(bdir, "bem_krylov.o")],
     ]
-    with cf.ThreadPoolExecutor(max_workers=3) as ex:
+    with cf.ThreadPoolExecutor(max_workers=len(jobs)) as ex:
         outs = list(ex.map(_run, jobs))
     if verbose:
         for o in outs:
@@ -56,7 +57,8 @@ def build(verbose: bool = False) -> str:
                 print(o)
     tmp = LIB + ".tmp"
     _run([hipcc, "-shared", "-fPIC", f"--offload-arch={ARCH}", "-o", tmp,
-          os.path.join(bdir, "bem_host.o"), os.path.join(bdir, "bem_device.o"), os.path.join(bdir, "bem_capi.o")])
+          os.path.join(bdir, "bem_host.o"), os.path.join(bdir, "bem_device.o"), os.path.join(bdir, "bem_capi.o"),
+          os.path.join(bdir, "bem_krylov.o")])
     os.replace(tmp, LIB)
     return LIB
 

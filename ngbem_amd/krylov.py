@@ -63,8 +63,85 @@ def CG(mat, rhs, pre=None, sol=None, tol=1e-12, maxsteps=100, printrates=True, i
     return x
 
 
+def _givens_column(H, cs, sn, g, k):
+    """Apply the previous rotations to column k of H, make the next one and
+    update g; returns |g[k+1]| (the residual norm after step k)."""
+    for i in range(k):
+        hi, hj = H[i, k], H[i + 1, k]
+        H[i, k] = np.conj(cs[i]) * hi + np.conj(sn[i]) * hj
+        H[i + 1, k] = -sn[i] * hi + cs[i] * hj
+    a, bb = H[k, k], H[k + 1, k]
+    t = math.hypot(abs(a), abs(bb))
+    cs[k], sn[k] = (a / t, bb / t) if t > 0 else (1.0, 0.0)
+    H[k, k] = t
+    H[k + 1, k] = 0.0
+    g[k + 1] = -sn[k] * g[k]
+    g[k] = np.conj(cs[k]) * g[k]
+    return abs(g[k + 1])
+
+
+_ARNOLDI_CACHE = {}
+
+
+def _arnoldi_for(m, n, dt, dev, basis_dt):
+    """The device workspace of a GMRES cycle of up to m steps, kept between
+    calls (the basis, the pinned Hessenberg buffer and the events are
+    allocated once per vector size and dtype, and again only when a longer
+    cycle is asked for)."""
+    key = (n, dt, basis_dt, str(dev))
+    a = _ARNOLDI_CACHE.get(key)
+    if a is None or a.m < m:
+        _ARNOLDI_CACHE.pop(key, None)
+        a = _ARNOLDI_CACHE[key] = _DeviceArnoldi(m, n, dt, dev, basis_dt)
+    return a
+
+
+class _DeviceArnoldi:
+    """Basis, Hessenberg columns and workspace of one GMRES cycle on the GPU;
+    step(k, w) orthogonalises w = P A v_k against v_0..v_k and writes v_{k+1}
+    (library call bem_arnoldi_step), then starts the copy of column k to
+    pinned host memory; column(k) waits for that copy only."""
+
+    def __init__(self, m, n, dt, dev, basis_dt=None):
+        from . import _capi
+
+        self._capi = _capi
+        self.cplx = dt == torch.complex128
+        basis_dt = dt if basis_dt is None else basis_dt
+        if basis_dt not in ((torch.complex128, torch.complex64) if self.cplx else (torch.float64, torch.float32)):
+            raise ValueError(f"basis dtype {basis_dt} does not fit vectors of dtype {dt}")
+        self.dtype_id = {torch.float64: _capi.F64, torch.complex128: _capi.C128, torch.float32: _capi.F32,
+                         torch.complex64: _capi.C64}[basis_dt]
+        self.V = torch.empty(m + 1, n, dtype=basis_dt, device=dev)
+        self.Hd = torch.zeros(m, m + 1, dtype=dt, device=dev)
+        self.Hh = torch.zeros(m, m + 1, dtype=dt, pin_memory=True)
+        nbytes = int(_capi.lib().bem_arnoldi_work_bytes(m, n, self.dtype_id))
+        if nbytes < 0:
+            raise RuntimeError(_capi.lib().bem_last_error().decode())
+        self.work = torch.empty(nbytes, dtype=torch.uint8, device=dev)
+        self.events = [torch.cuda.Event() for _ in range(m)]
+        self.n = n
+        self.m = m
+
+    def step(self, k, w):
+        import ctypes
+
+        w = w.contiguous()
+        stream = torch.cuda.current_stream(w.device).cuda_stream
+        self._capi.check(self._capi.lib().bem_arnoldi_step(
+            ctypes.c_void_p(self.V.data_ptr()), self.n, k, self.n, ctypes.c_void_p(w.data_ptr()),
+            ctypes.c_void_p(self.Hd[k].data_ptr()), ctypes.c_void_p(self.work.data_ptr()), self.work.numel(),
+            self.dtype_id, ctypes.c_void_p(stream)), "bem_arnoldi_step")
+        self.Hh[k, : k + 2].copy_(self.Hd[k, : k + 2], non_blocking=True)
+        self.events[k].record()
+
+    def column(self, k):
+        self.events[k].synchronize()
+        return self.Hh[k, : k + 2].numpy()
+
+
 def GMRes(A, b, pre=None, freedofs=None, x=None, maxsteps=100, tol=None, innerproduct=None, callback=None,
-          restart=None, startiteration=0, printrates=True, reltol=None):
+          restart=None, startiteration=0, printrates=True, reltol=None, basis=None):
     """Restarted GMRES, left-preconditioned like ngsolve.krylovspace.GMRes: it
     minimises ||pre (b - A x)|| over the Krylov space of pre A and stops when
     that norm has fallen to tol times its value at the start.  A preconditioner
@@ -74,9 +151,18 @@ def GMRes(A, b, pre=None, freedofs=None, x=None, maxsteps=100, tol=None, innerpr
     (tests/Laplace_Convergence.py:57-61).  Without ``pre`` the criterion is
     ||b - A x|| <= tol ||b - A x0||.
 
-    The Krylov basis is one (m+1) x n tensor on the GPU; each step orthogonalises
-    with classical Gram-Schmidt applied twice (two stacked matrix-vector
-    products), so the host reads back one small coefficient vector per step.
+    On GPU vectors the Krylov basis is one (m+1) x n tensor and each Arnoldi
+    step (classical Gram-Schmidt twice, the norm and the next basis vector)
+    runs on the device without a read back (bem_arnoldi_step); the host applies
+    the Givens rotations to each Hessenberg column one step later, when its
+    asynchronous copy has arrived, so the GPU never waits for the host.  When
+    step k's column shows convergence, step k + 1 has already been issued and
+    is discarded; to avoid that, once the observed convergence rate predicts
+    that the step just issued reaches the tolerance, the host waits for its
+    column before issuing the next one.  Restart cycles are at most 1022
+    steps long.  ``basis`` (an extension, GPU vectors only): the dtype of the
+    Krylov basis, e.g. torch.complex64 for the inner solve of a
+    mixed-precision refinement (products still accumulate in FP64).
     """
     if freedofs is not None or innerproduct is not None:
         raise NotImplementedError("freedofs / innerproduct are not supported")
@@ -85,67 +171,97 @@ def GMRes(A, b, pre=None, freedofs=None, x=None, maxsteps=100, tol=None, innerpr
         tol = reltol
     dt = torch.complex128 if (torch.is_complex(b) or getattr(A, "is_complex", False)) else torch.float64
     b = b.to(dt)
+    x_zero = x is None
     if x is None:
         x = torch.zeros_like(b)
-    m = min(restart or maxsteps, maxsteps)
+    m = min(restart or maxsteps, maxsteps, 1022)
     n = b.shape[0]
-    if float(torch.linalg.norm(b)) == 0.0:
-        x.zero_()
-        GMRes.iterations = 0
-        return x
     P = (lambda v: _apply(pre, v)) if pre is not None else (lambda v: v)  # noqa: E731
-    Vb = torch.empty(m + 1, n, dtype=dt, device=b.device)
+    dev_path = b.is_cuda
+    if basis is not None and not dev_path:
+        raise ValueError("a basis dtype needs GPU vectors")
+    arn = _arnoldi_for(m, n, dt, b.device, basis) if dev_path else None
+    Vb = arn.V if dev_path else torch.empty(m + 1, n, dtype=dt, device=b.device)
     cdt = np.complex128
     total = 0
     bnorm = None  # preconditioned residual norm at the start
+    if not x_zero:
+        x_zero = not bool(torch.any(x != 0))
+    # (b = 0 with x = 0: beta = 0 below ends the solve at x = 0)
     while total < maxsteps:
-        r = P(b - _apply(A, x)).to(dt)
+        r = P(b if x_zero else b - _apply(A, x)).to(dt)
+        x_zero = False
         beta = float(torch.linalg.norm(r))
         if bnorm is None:
             bnorm = beta
         if beta <= tol * bnorm:
             break
-        Vb[0] = r / beta
+        Vb[0].copy_(r / beta)
         H = np.zeros((m + 1, m), dtype=cdt)
         cs = np.zeros(m, dtype=cdt)
         sn = np.zeros(m, dtype=cdt)
         g = np.zeros(m + 1, dtype=cdt)
         g[0] = beta
         k_used = 0
-        for k in range(m):
-            w = P(_apply(A, Vb[k])).to(dt)
-            Vk = Vb[: k + 1]
-            h = Vk.conj() @ w
-            w = w - h @ Vk
-            h2 = Vk.conj() @ w  # second pass (CGS2)
-            w = w - h2 @ Vk
-            hn_t = torch.linalg.norm(w)
-            hcol = torch.cat([h + h2, hn_t.reshape(1).to(h.dtype)]).cpu().numpy()
-            hn = float(hcol[k + 1].real)
+        stop = False
+
+        hist = []
+
+        def take(k):
+            # column k has arrived: rotate, report, decide
+            nonlocal k_used, total, stop
+            hcol = arn.column(k) if dev_path else hcols[k]
             H[: k + 2, k] = hcol
-            for i in range(k):  # previous rotations
-                hi, hj = H[i, k], H[i + 1, k]
-                H[i, k] = np.conj(cs[i]) * hi + np.conj(sn[i]) * hj
-                H[i + 1, k] = -sn[i] * hi + cs[i] * hj
-            a, bb = H[k, k], H[k + 1, k]
-            t = math.hypot(abs(a), abs(bb))
-            cs[k], sn[k] = (a / t, bb / t) if t > 0 else (1.0, 0.0)
-            H[k, k] = t
-            H[k + 1, k] = 0.0
-            g[k + 1] = -sn[k] * g[k]
-            g[k] = np.conj(cs[k]) * g[k]
+            hn = float(hcol[k + 1].real)
+            res = _givens_column(H, cs, sn, g, k)
+            hist.append(res)
             k_used = k + 1
             total += 1
-            res = abs(g[k + 1])
             if printrates:
                 print(f"GMRes iteration {total}, residual = {res}")
             if callback is not None:
                 callback(res)
-            if res <= tol * bnorm or hn == 0.0 or total >= maxsteps:
-                break
-            Vb[k + 1] = w / hn
+            stop = res <= tol * bnorm or hn == 0.0 or total >= maxsteps
+
+        def likely_last():
+            # the residual after the step just issued, extrapolated at the last rate
+            if len(hist) < 2 or hist[-2] == 0.0:
+                return False
+            return hist[-1] * (hist[-1] / hist[-2]) <= 2.0 * tol * bnorm
+
+        hcols = {}
+        steps = min(m, maxsteps - total)
+        for k in range(steps):
+            v = Vb[k] if Vb.dtype == dt else Vb[k].to(dt)
+            w = P(_apply(A, v)).to(dt)
+            if dev_path:
+                arn.step(k, w)
+                if k >= 1 and k_used == k - 1:
+                    take(k - 1)
+                    if stop:
+                        break
+                if (likely_last() or total + 1 >= maxsteps) and k + 1 < steps:
+                    take(k)
+                    if stop:
+                        break
+            else:
+                Vk = Vb[: k + 1]
+                h = Vk.conj() @ w
+                w = w - h @ Vk
+                h2 = Vk.conj() @ w  # second pass (CGS2)
+                w = w - h2 @ Vk
+                hn_t = torch.linalg.norm(w)
+                hcols[k] = torch.cat([h + h2, hn_t.reshape(1).to(h.dtype)]).numpy()
+                hn = float(hcols[k][k + 1].real)
+                Vb[k + 1] = w / hn if hn > 0 else 0.0
+                take(k)
+                if stop:
+                    break
+        else:
+            if dev_path and steps > 0 and k_used < steps:
+                take(steps - 1)
         y = np.linalg.solve(np.triu(H[:k_used, :k_used]), g[:k_used])
-        yt = torch.from_numpy(y if torch.is_complex(x) else y.real.copy()).to(device=x.device, dtype=dt)
+        yt = torch.from_numpy(y if torch.is_complex(x) else y.real.copy()).to(device=x.device, dtype=Vb.dtype)
         x.add_(yt @ Vb[:k_used])
         if abs(g[k_used]) <= tol * bnorm:
             break
@@ -239,22 +355,27 @@ def GMResBatch(A, B, pre=None, maxsteps=100, tol=1e-7, restart=None, printrates=
     return X
 
 
-def RefinedSolve(A, A_inner, b, pre=None, tol=1e-10, inner_tol=1e-5, maxouter=20, maxsteps=500, printrates=False):
+def RefinedSolve(A, A_inner, b, pre=None, tol=1e-10, inner_tol=1e-5, maxouter=20, maxsteps=500, printrates=False,
+                 basis=None):
     """FP64 iterative refinement around an inner GMRES on A_inner (typically
-    ``A.mixed``): x += GMRes(A_inner, b - A x).  Returns (x, history)."""
+    ``A.mixed``): x += GMRes(A_inner, b - A x).  Returns (x, history).
+    ``basis``: Krylov basis dtype of the inner GMRES (default: complex64 /
+    float32 on GPU vectors, the precision of the mixed operator)."""
     x = torch.zeros_like(b)
     bnorm = float(torch.linalg.norm(b))
     hist = []
     inner_total = 0
+    if basis is None and b.is_cuda:
+        basis = torch.complex64 if (torch.is_complex(b) or getattr(A, "is_complex", False)) else torch.float32
     for outer in range(maxouter):
-        r = b - _apply(A, x)
+        r = b - _apply(A, x) if outer > 0 else b.clone()
         rn = float(torch.linalg.norm(r)) / bnorm
         hist.append(rn)
         if printrates:
             print(f"refinement step {outer}: relative residual {rn:.3e}")
         if rn <= tol:
             break
-        d = GMRes(A_inner, r, pre=pre, maxsteps=maxsteps, tol=inner_tol, printrates=False)
+        d = GMRes(A_inner, r, pre=pre, maxsteps=maxsteps, tol=inner_tol, printrates=False, basis=basis)
         inner_total += GMRes.iterations
         x.add_(d)
     RefinedSolve.inner_iterations = inner_total

@@ -282,13 +282,32 @@ class MassMatrix(BaseMatrix):
         self.rows = torch.from_numpy(r).to(dev)
         self.cols = torch.from_numpy(c).to(dev)
         self.vals = torch.from_numpy(v).to(dev)
+        # SurfaceL2 order 0 on one mesh: one entry per dof, on the diagonal
+        self._diag = None
+        if self.height == self.width and len(r) == self.height and np.array_equal(r, c) and np.array_equal(
+                np.sort(r), np.arange(self.height)):
+            d = np.zeros(self.height)
+            d[r] = v
+            self._diag = {torch.float64: torch.from_numpy(d).to(dev)}
+
+    def _diag_as(self, dtype):
+        d = self._diag.get(dtype)
+        if d is None:
+            d = self._diag[dtype] = self._diag[torch.float64].to(dtype)
+        return d
 
     def MultAdd(self, s, x, y):
         x = x.to(y.dtype)
+        if self._diag is not None:
+            y.addcmul_(x, self._diag_as(y.dtype), value=s)
+            return
         y.index_add_(-1, self.rows, (s * self.vals).to(y.dtype) * x[..., self.cols])
 
     def MultTransAdd(self, s, x, y):
         x = x.to(y.dtype)
+        if self._diag is not None:
+            y.addcmul_(x, self._diag_as(y.dtype), value=s)
+            return
         y.index_add_(-1, self.cols, (s * self.vals).to(y.dtype) * x[..., self.rows])
 
     def Inverse(self, freedofs=None, tol=1e-14, maxsteps=2000):
