@@ -71,6 +71,9 @@ def main():
     ap.add_argument("--calib-copy", action="store_true",
                     help="end with one elementwise kernel reading and writing 512 MiB (calibrates rocprof "
                          "FETCH_SIZE/WRITE_SIZE)")
+    ap.add_argument("--assembly-profile", type=str,
+                    default=os.path.join(ROOT, "profiles", "assembly_helmholtz_cf_L6_fp64_counters.md"),
+                    help="rocprof summary with the executed FP64 flops of the assembly kernels")
     ap.add_argument("--traffic", type=str,
                     default=os.path.join(ROOT, "profiles", "hmatvec_helmholtz_cf_L6_traffic.md"),
                     help="summary of the PMC passes with 'bytes_per_launch <kernel>:' lines (complex64 apply)")
@@ -359,6 +362,8 @@ def main():
             cb["gpu_speedup_assemble_plus_solve_mixed"] = cb["assemble_plus_solve_s"] / (t_assembly + t_s32)
         out["cpu_baseline"] = cb
 
+    out["roofline_assembly"] = assembly_roofline(C, st, args, kappa)
+
     if args.traffic and os.path.exists(args.traffic) and args.level == 6 and kappa == 10.0:
         for line in open(args.traffic):  # "bytes_per_launch <kernel>: <bytes>"
             key, _, val = line.partition(":")
@@ -572,6 +577,54 @@ def bench_batch(args):
 
 def RefinedSolve_iters(nb):
     return getattr(nb.RefinedSolve, "inner_iterations", -1)
+
+
+FP64_PEAK_TFLOPS = 78.6
+
+
+def assembly_roofline(C, st, args, kappa):
+    """Assembly kernels against their bounds: executed FP64 flops (from the
+    committed rocprof counter summary of this workload) over the kernels'
+    device seconds measured now with HIP events (bem_stats), and for the cross
+    approximation its HBM bytes (measured traffic and the algorithmic
+    workspace reads) over its seconds.  Algorithmic ACA bytes: step k of a
+    block reads the k earlier rows of V and columns of U once (the residual
+    row and column) and writes one of each: 16 (m' + n') (r (r - 1) / 2 + r)
+    bytes per block of rank r on m' x n' quadrature points."""
+    ntq = 6  # points of the intorder-4 triangle rule
+    acab = 0.0
+    for b in range(C.num_blocks()):
+        bi = C.block_info(b)
+        r = bi["rank"]
+        if r > 0:
+            acab += 16.0 * ntq * (bi["m"] + bi["n"]) * (r * (r - 1) / 2 + r)
+    t_nr, t_ns, t_aca = st["t_kernel_near_regular"], st["t_kernel_near_singular"], st["t_kernel_aca"]
+    out = {"kernel_seconds": {"k_near_regular": t_nr, "k_near_singular": t_ns, "k_aca": t_aca},
+           "near_evals_per_s": st["near_kernel_evals"] / (t_nr + t_ns) if t_nr + t_ns > 0 else None,
+           "aca_algorithmic_bytes": acab,
+           "aca_algorithmic_gbs": acab / t_aca / 1e9 if t_aca > 0 else None}
+    flops, hbm = {}, {}
+    if args.assembly_profile and os.path.exists(args.assembly_profile) and args.level == 6 and kappa == 10.0:
+        for line in open(args.assembly_profile):  # "fp64_flops <kernel>: <flops>" / "hbm_bytes <kernel>: <bytes>"
+            key, _, val = line.partition(":")
+            if key.startswith("fp64_flops "):
+                flops[key.split()[1]] = float(val)
+            elif key.startswith("hbm_bytes "):
+                hbm[key.split()[1]] = float(val)
+        out["source"] = os.path.relpath(args.assembly_profile, ROOT)
+    for k, t in (("k_near_regular", t_nr), ("k_near_singular", t_ns), ("k_aca", t_aca)):
+        if k in flops and t > 0:
+            a = flops[k] / t / 1e12
+            out[k] = {"bound": "fp64" if k != "k_aca" else "hbm", "fp64_flops": flops[k], "fp64_achieved": a,
+                      "fp64_peak": FP64_PEAK_TFLOPS, "unit": "TFLOP/s", "fp64_frac": a / FP64_PEAK_TFLOPS}
+    if "k_aca" in out and "k_aca" in hbm and t_aca > 0:
+        out["k_aca"].update(traffic=hbm["k_aca"], achieved_gbs=hbm["k_aca"] / t_aca / 1e9,
+                            peak_gbs=HBM_PEAK_GBS, frac=hbm["k_aca"] / t_aca / 1e9 / HBM_PEAK_GBS)
+    log("assembly kernels: " + ", ".join(f"{k} {t * 1e3:.1f} ms" for k, t in out["kernel_seconds"].items())
+        + "".join(f"; {k} {out[k]['fp64_achieved']:.2f} TFLOP/s FP64 ({out[k]['fp64_frac']:.3f})"
+                  for k in ("k_near_regular", "k_near_singular", "k_aca") if k in out)
+        + (f"; k_aca {out['k_aca']['achieved_gbs']:.0f} GB/s HBM" if "achieved_gbs" in out.get("k_aca", {}) else ""))
+    return out
 
 
 def cpu_model():

@@ -108,6 +108,10 @@ typedef struct {
   int32_t n_singular_pairs[3]; /* identical, common edge, common vertex */
   double t_setup, t_near, t_far, t_total; /* seconds (host clock, device synchronised) */
   double near_kernel_evals, far_kernel_evals;
+  /* device seconds from HIP events: the regular and the singular near-field
+   * launches, and the cross approximation launches (summed over workspace
+   * batches; they overlap the near field, which runs on its own stream) */
+  double t_kernel_near_regular, t_kernel_near_singular, t_kernel_aca;
 } bem_stats;
 
 typedef struct {

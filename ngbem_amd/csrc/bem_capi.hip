@@ -463,6 +463,9 @@ int bem_op_stats(const bem_op* h, bem_stats* st) {
   st->t_total = op.t_setup + op.t_near + op.t_far;
   st->near_kernel_evals = op.near_evals;
   st->far_kernel_evals = op.far_evals;
+  st->t_kernel_near_regular = op.t_ev_near_regular;
+  st->t_kernel_near_singular = op.t_ev_near_singular;
+  st->t_kernel_aca = op.t_ev_aca;
   return 0;
 }
 

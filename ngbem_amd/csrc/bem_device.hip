@@ -2319,6 +2319,8 @@ void assemble_near(Operator& op, hipStream_t s) {
                        : (te.region.empty() ? op.d_perm_test.as<int>() : elem_of_pos(te, tt, d_pe_test));
   const int* pJb = p1j ? op.d_patch.as<int>()
                        : (tr.region.empty() ? op.d_perm_trial.as<int>() : elem_of_pos(tr, rt, d_pe_trial));
+  hipEvent_t tev[3];
+  for (auto& e : tev) BEM_HIP(hipEventCreate(&e));
   dispatch_full(op, [&](auto ker, auto ei, auto ej) {
     using KER = decltype(ker);
     using EI = decltype(ei);
@@ -2327,12 +2329,14 @@ void assemble_near(Operator& op, hipStream_t s) {
     T* val = op.d_near_val.as<T>();
     auto launch = [&](auto dflag) {
       constexpr bool D = decltype(dflag)::value;
+      BEM_HIP(hipEventRecord(tev[0], s));
       if (!chunks.empty())
         hipLaunchKernelGGL((k_near_regular<KER, EI, EJ, D>), dim3((unsigned)chunks.size()), dim3(256), 0, s,
                            op.d_near_blk.as<NearBlk>(), op.d_near_chunks.as<int2>(), pIb, pJb, op.d_geo.as<double>(),
                            op.d_vid.as<int4>(), op.d_quad.as<double>(), op.d_sdat.as<double>(),
                            op.d_trig.as<double>(), ntq, op.d_dofs_test.as<int>(), op.d_dofs_trial.as<int>(),
                            op.d_iperm_test.as<int>(), op.d_iperm_trial.as<int>(), ker, val);
+      BEM_HIP(hipEventRecord(tev[1], s));
       for (int c = 0; c < 3; c++) {
         const int np = (int)pairs[c].size();
         if (!np) continue;
@@ -2346,9 +2350,18 @@ void assemble_near(Operator& op, hipStream_t s) {
     pt_.mark("uploads + alloc");
     if constexpr (!EI::MULTI && !EJ::MULTI) launch(std::true_type{});
     else launch(std::false_type{});
+    BEM_HIP(hipEventRecord(tev[2], s));
   });
   BEM_HIP(hipGetLastError());
   BEM_HIP(hipStreamSynchronize(s));
+  {
+    float a = 0.f, b = 0.f;
+    BEM_HIP(hipEventElapsedTime(&a, tev[0], tev[1]));
+    BEM_HIP(hipEventElapsedTime(&b, tev[1], tev[2]));
+    op.t_ev_near_regular = a * 1e-3;
+    op.t_ev_near_singular = b * 1e-3;
+  }
+  for (auto& e : tev) (void)hipEventDestroy(e);
   pt_.mark("kernels");
   op.t_near = now() - t0;
 }
@@ -2452,9 +2465,12 @@ void assemble_far(Operator& op, hipStream_t s) {
   op.far_rank.assign(nfar, 0);
   hipStream_t s2;
   BEM_HIP(hipStreamCreateWithFlags(&s2, hipStreamNonBlocking));
-  hipEvent_t ev_s, ev_s2;
+  hipEvent_t ev_s, ev_s2, ev_t0, ev_t1;
   BEM_HIP(hipEventCreateWithFlags(&ev_s, hipEventDisableTiming));
   BEM_HIP(hipEventCreateWithFlags(&ev_s2, hipEventDisableTiming));
+  BEM_HIP(hipEventCreate(&ev_t0));
+  BEM_HIP(hipEventCreate(&ev_t1));
+  op.t_ev_aca = 0;
   DBuf wsU, wsV, wsG, d_tasks, d_ptasks;
   // NGBEM_ACA=gram: first passes in k_aca_gram (one workspace pass per step,
   // 25% less far-field time on the Helmholtz sphere at 80k triangles, but the
@@ -2502,13 +2518,18 @@ void assemble_far(Operator& op, hipStream_t s) {
     if (G && (size_t)10 * maxcap * esz > (48u << 10)) G = nullptr;
     d_tasks.upload(tasks, s);
     BEM_HIP(hipEventRecord(ev_s, s));
+    BEM_HIP(hipEventRecord(ev_t0, s));
     BEM_HIP(hipStreamWaitEvent(s2, ev_s, 0));
     launch(d_tasks.as<AcaTask>(), nbig, true, maxcap, s2, U, V, G);
     launch(d_tasks.as<AcaTask>() + nbig, (int)tasks.size() - nbig, false, maxcap, s, U, V, G);
     BEM_HIP(hipEventRecord(ev_s2, s2));
     BEM_HIP(hipStreamWaitEvent(s, ev_s2, 0));
+    BEM_HIP(hipEventRecord(ev_t1, s));
     BEM_HIP(hipMemcpyAsync(tasks.data(), d_tasks.p, tasks.size() * sizeof(AcaTask), hipMemcpyDeviceToHost, s));
     BEM_HIP(hipStreamSynchronize(s));
+    float ms = 0.f;
+    BEM_HIP(hipEventElapsedTime(&ms, ev_t0, ev_t1));
+    op.t_ev_aca += ms * 1e-3;
   };
 
   size_t next = 0;
@@ -2766,6 +2787,8 @@ void assemble_far(Operator& op, hipStream_t s) {
   pt_.mark("projection");
   (void)hipEventDestroy(ev_s);
   (void)hipEventDestroy(ev_s2);
+  (void)hipEventDestroy(ev_t0);
+  (void)hipEventDestroy(ev_t1);
   (void)hipStreamDestroy(s2);
   wsU.release();
   wsV.release();

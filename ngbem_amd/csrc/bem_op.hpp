@@ -222,6 +222,10 @@ struct Operator {
   int64_t near_entries = 0, far_factor_entries = 0;
   int n_sing[3] = {0, 0, 0};
   double t_setup = 0, t_near = 0, t_far = 0;
+  // device time from HIP events: the regular and the singular near-field
+  // launches, and the ACA launches (both streams of each batch, summed over
+  // batches; they overlap the near field, which runs on its own stream)
+  double t_ev_near_regular = 0, t_ev_near_singular = 0, t_ev_aca = 0;
   double near_evals = 0, far_evals = 0;
   bool assembled = false;
 
