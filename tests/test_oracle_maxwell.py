@@ -277,7 +277,7 @@ def efie_plane_wave_rhs(hd, k, d, p, order=6):
     return out
 
 
-def efie_current_error(hd, J, k, d, p, order=6):
+def efie_current_error(hd, J, k, d, p, order=6, progress=False):
     """Relative L2 error of the RT currents (rows of J) against the Mie current."""
     import mie_maxwell as M
 
@@ -293,6 +293,8 @@ def efie_current_error(hd, J, k, d, p, order=6):
         jex = M.current_rotated(Xs, k, d[r], p[r]).reshape(X.shape)
         jh = np.einsum("tl,tqlc->tqc", J[r][hd.elem_dofs], phi)
         errs.append(np.sqrt(np.sum(w * (np.abs(jh - jex) ** 2).sum(-1)) / np.sum(w * (np.abs(jex) ** 2).sum(-1))))
+        if progress and (r + 1) % 8 == 0:
+            print(f"  Mie currents: {r + 1} of {len(d)} waves, max error so far {max(errs):.4f}", flush=True)
     return np.array(errs)
 
 
