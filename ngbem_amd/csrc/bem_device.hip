@@ -1693,15 +1693,25 @@ __device__ __forceinline__ void col_step(const int2 c, const ST* __restrict__ nv
   xv = (far ? tbuf : xp)[c.y & (COL_FAR - 1)];
 }
 
+// A tile's column list is cut into pieces of at most a set number of columns
+// (one workgroup each), so that the few tiles with twice the average work do
+// not run alone at the end of the launch; a piece of a cut tile writes its 64
+// partial sums to its slot, and k_mv_rows_psum adds a tile's slots in order.
+struct RowPiece {
+  int tile, c0, c1, slot;  // slot < 0: the tile's only piece, straight to y
+};
+
 template <class ST, class VT>
-__global__ __launch_bounds__(256) void k_mv_rows_flat(const Tile* __restrict__ tiles, const int2* __restrict__ frange,
+__global__ __launch_bounds__(256) void k_mv_rows_flat(const Tile* __restrict__ tiles,
+                                                      const RowPiece* __restrict__ pieces,
                                                       const int2* __restrict__ fcols, const ST* __restrict__ nval,
                                                       const ST* __restrict__ fUt, const VT* __restrict__ xp,
-                                                      const VT* __restrict__ tbuf, VT* __restrict__ yp, int remap) {
+                                                      const VT* __restrict__ tbuf, VT* __restrict__ yp,
+                                                      VT* __restrict__ part, int remap) {
   __shared__ VT red[4][64];
-  const int tb = xcd_major(blockIdx.x, gridDim.x, remap);
-  const Tile tl = tiles[tb];
-  const int2 rg = frange[tb];
+  const RowPiece pc = pieces[xcd_major(blockIdx.x, gridDim.x, remap)];
+  const Tile tl = tiles[pc.tile];
+  const int2 rg = make_int2(pc.c0, pc.c1);
   const int lane = threadIdx.x & 63;
   const int wv = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
   const bool act = lane < tl.len;
@@ -1734,7 +1744,25 @@ __global__ __launch_bounds__(256) void k_mv_rows_flat(const Tile* __restrict__ t
   }
   red[wv][lane] = ((a[0] + a[1]) + (a[2] + a[3])) + ((a[4] + a[5]) + (a[6] + a[7]));
   __syncthreads();
-  if (wv == 0 && act) yp[tl.p0 + lane] = red[0][lane] + red[1][lane] + red[2][lane] + red[3][lane];
+  if (wv == 0 && act) {
+    const VT v = red[0][lane] + red[1][lane] + red[2][lane] + red[3][lane];
+    if (pc.slot < 0) yp[tl.p0 + lane] = v;
+    else part[(int64_t)pc.slot * 64 + lane] = v;
+  }
+}
+
+// y rows of the cut tiles: the pieces' partial sums in piece order
+// (tile, first slot, slot count)
+template <class VT>
+__global__ __launch_bounds__(64) void k_mv_rows_psum(const Tile* __restrict__ tiles, const int4* __restrict__ sums,
+                                                     const VT* __restrict__ part, VT* __restrict__ yp) {
+  const int4 sm = sums[blockIdx.x];
+  const Tile tl = tiles[sm.x];
+  const int lane = threadIdx.x;
+  if (lane >= tl.len) return;
+  VT a = part[(int64_t)sm.y * 64 + lane];
+  for (int k = 1; k < sm.z; k++) a += part[(int64_t)(sm.y + k) * 64 + lane];
+  yp[tl.p0 + lane] = a;
 }
 
 // transpose: one workgroup per (leaf column cluster, 64-column chunk); dense
@@ -3001,6 +3029,16 @@ static void upload_z_items(Operator& op, hipStream_t s) {
   }
 }
 
+// most columns of one row-tile workgroup (NGBEM_ROWS_CHUNK, default 768; 0:
+// one workgroup per tile)
+static int rows_chunk() {
+  static const int c = [] {
+    const char* e = std::getenv("NGBEM_ROWS_CHUNK");
+    return e ? std::max(0, std::atoi(e)) : 768;
+  }();
+  return c;
+}
+
 // flat column lists of the forward Ut plan (k_mv_rows_flat); tiles of one
 // leaf share the leaf's list.  Lists are sized first and filled on host
 // threads.  False (no lists) when an offset needs more than 31 bits.
@@ -3095,10 +3133,36 @@ void build_matvec_plan(Operator& op, hipStream_t s) {
   op.d_seg_fwd_mm.upload(s_ut, s);
   if (flat_ok) {
     op.d_fcols.upload(cols, s);
-    op.d_frange.upload(range, s);
+    // cut the tiles' column lists into pieces of at most rows_chunk() columns
+    std::vector<RowPiece> pieces;
+    std::vector<int4> sums;
+    const int ch = rows_chunk();
+    int slot = 0;
+    for (size_t t = 0; t < range.size(); t++) {
+      const int c0 = range[t].x, len = range[t].y - range[t].x;
+      const int np = ch > 0 ? std::max(1, (len + ch - 1) / ch) : 1;
+      if (np == 1) {
+        pieces.push_back(RowPiece{(int)t, c0, c0 + len, -1});
+        continue;
+      }
+      const int step = ((len + np - 1) / np + 31) / 32 * 32;  // whole 32-column steps of the workgroup
+      int k = 0;
+      for (int c = c0; c < c0 + len; c += step, k++) pieces.push_back(RowPiece{(int)t, c, std::min(c0 + len, c + step),
+                                                                                 slot + k});
+      sums.push_back(make_int4((int)t, slot, k, 0));
+      slot += k;
+    }
+    op.d_fpieces.upload(pieces, s);
+    op.n_fpieces = (int)pieces.size();
+    op.d_fsums.upload(sums, s);
+    op.n_fsums = (int)sums.size();
+    op.d_fpart.alloc((size_t)std::max(slot, 1) * 64 * (op.complex_valued ? 16 : 8));
   } else {
     op.d_fcols.release();
-    op.d_frange.release();
+    op.d_fpieces.release();
+    op.d_fsums.release();
+    op.d_fpart.release();
+    op.n_fpieces = op.n_fsums = 0;
   }
   op.d_tile_fwd.upload(t_fwd, s);
   op.d_seg_fwd.upload(s_fwd, s);
@@ -3819,10 +3883,14 @@ static void apply_t(Operator& op, int flags, const XT* x, XT* y, XT alpha, XT be
     }
     mark(3);
     const DBuf& segs = pk == PLAN_Z ? op.d_seg_fwd : op.d_seg_fwd_mm;
-    if (!zpath && op.d_frange.p && rows_flat())
-      hipLaunchKernelGGL((k_mv_rows_flat<ST, VT>), dim3(op.n_tiles_fwd), dim3(256), 0, s, op.d_tile_fwd.as<Tile>(),
-                         op.d_frange.as<int2>(), op.d_fcols.as<int2>(), nval, fU, xp, tb, yp, xcd_remap());
-    else
+    if (!zpath && op.d_fpieces.p && rows_flat()) {
+      hipLaunchKernelGGL((k_mv_rows_flat<ST, VT>), dim3(op.n_fpieces), dim3(256), 0, s, op.d_tile_fwd.as<Tile>(),
+                         op.d_fpieces.as<RowPiece>(), op.d_fcols.as<int2>(), nval, fU, xp, tb, yp,
+                         op.d_fpart.as<VT>(), xcd_remap());
+      if (op.n_fsums)
+        hipLaunchKernelGGL((k_mv_rows_psum<VT>), dim3(op.n_fsums), dim3(64), 0, s, op.d_tile_fwd.as<Tile>(),
+                           op.d_fsums.as<int4>(), (const VT*)op.d_fpart.as<VT>(), yp);
+    } else
       hipLaunchKernelGGL((k_mv_rows<ST, VT>), dim3(op.n_tiles_fwd), dim3(256), 0, s, op.d_tile_fwd.as<Tile>(),
                          segs.as<Seg>(), nval, fU, z, xp, tb, yp);
   } else {

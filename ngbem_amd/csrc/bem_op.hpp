@@ -247,7 +247,8 @@ struct Operator {
   // the same forward plan as one flat column list per row tile: (element
   // offset of the column, x / t index | COL_FAR); empty when the offsets
   // overflow 31 bits
-  DBuf d_fcols, d_frange;
+  DBuf d_fcols, d_fpieces, d_fsums, d_fpart;  // flat column lists, their pieces, sums of cut tiles, partials
+  int n_fpieces = 0, n_fsums = 0;
   int n_tiles_fwd = 0, n_tiles_trn = 0;
   DBuf d_zitems_fwd, d_zitems_trn;  // U t / V^T t work items
   int n_zitems_fwd = 0, n_zitems_trn = 0;
