@@ -67,6 +67,7 @@ def main():
     ap.add_argument("--kappa", type=float, default=10.0)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-solve", action="store_true")
+    ap.add_argument("--no-autotune", action="store_true", help="keep the built-in apply work shapes")
     ap.add_argument("--no-aca-gram", action="store_true", help="skip the extra assembly with NGBEM_ACA=gram")
     ap.add_argument("--calib-copy", action="store_true",
                     help="end with one elementwise kernel reading and writing 512 MiB (calibrates rocprof "
@@ -159,6 +160,18 @@ def main():
             del os.environ["NGBEM_ACA"]
         log(f"assembly with NGBEM_ACA=gram {tg:.3f} s  far {st3['t_far']:.3f} s")
 
+    # the apply's autotuner on the assembled operator (both precisions)
+    tune = None
+    if not args.no_autotune:
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        tune = C.Autotune(iters=7)
+        torch.cuda.synchronize()
+        tune["seconds"] = time.perf_counter() - t0
+        log("autotune ({:.2f} s): ".format(tune["seconds"]) + "; ".join(
+            f"{k}: far_t rows8 up to {r.get('far_t_choice')}, row pieces of {r.get('rows_choice')} columns, apply "
+            f"{r['apply_ms_before']:.3f} -> {r['apply_ms_after']:.3f} ms" for k, r in tune.items() if k != "seconds"))
+
     x = torch.randn(fes.ndof, dtype=torch.complex128, device=dev)
     y = torch.zeros_like(x)
     M64, M32 = C.mat, C.mat.mixed
@@ -177,9 +190,13 @@ def main():
 
     t64 = time_matvecs(M64, args.steps, args.warmup)
     t32 = time_matvecs(M32, args.steps, args.warmup)
+    # the same applies replayed from hipGraphs (BEM_APPLY_GRAPH)
+    t64g = time_matvecs(M64.graph, args.steps, args.warmup)
+    t32g = time_matvecs(M32.graph, args.steps, args.warmup)
     y64 = M64 * x
     y32 = M32 * x
     mv_err = (torch.linalg.norm(y64 - y32) / torch.linalg.norm(y64)).item()
+    log(f"graph replay: fp64 {t64g / args.steps * 1e3:.3f} ms  mixed {t32g / args.steps * 1e3:.3f} ms")
     log(f"H-matvec fp64 {t64 / args.steps * 1e3:.3f} ms  mixed {t32 / args.steps * 1e3:.3f} ms  "
         f"mixed rel err {mv_err:.2e}")
 
@@ -260,6 +277,8 @@ def main():
         "assembly_breakdown_s": {"setup": st["t_setup"], "near": st["t_near"], "far_aca": st["t_far"]},
         "second_assembly_s": t_assembly2,
         "assembly_aca_gram": aca_gram,
+        "autotune": tune,
+        "graph_replay_ms_per_matvec": {"mixed": t32g / args.steps * 1e3, "fp64": t64g / args.steps * 1e3},
         "fp64_matvec_per_s": args.steps / t64,
         "mixed_vs_fp64_matvec_speedup": t64 / t32,
         "mixed_matvec_rel_err": mv_err,

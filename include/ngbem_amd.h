@@ -79,7 +79,10 @@ enum {
   BEM_APPLY_MIXED = 4,     /* use the FP32 / complex64 copy of the operator */
   BEM_APPLY_ON_DEVICE = 8, /* x and y are device pointers (else host pointers) */
   BEM_APPLY_PER_VECTOR = 16, /* nrhs > 1: one vector at a time instead of the block product */
-  BEM_BATCH_EAGER = 32        /* bem_batch_apply: launch the kernels directly, no graph replay */
+  BEM_BATCH_EAGER = 32,       /* bem_batch_apply: launch the kernels directly, no graph replay */
+  BEM_APPLY_GRAPH = 64        /* bem_op_apply on device vectors, one right-hand side: replay the apply
+                                 captured in a hipGraph (once per flag combination, on the operator's own
+                                 vectors; x is copied in, y = alpha A x + beta y applied after) */
 };
 enum {
   BEM_BATCH_CONSUME = 1       /* bem_batch_create: move the operators' storage into the batch */
@@ -188,6 +191,17 @@ int bem_op_assemble(bem_op* op, void* stream);
 int bem_op_apply(bem_op* op, int flags, const double* alpha, const void* x, const double* beta, void* y,
                  int nrhs, int ld, int dtype, void* stream);
 int bem_op_stats(const bem_op* op, bem_stats* st);
+/* Autotuner of the forward apply (no counterpart in the reference, whose
+ * block products are fixed, src/hmat.cpp:477-488 and :527-557): on the
+ * assembled operator, for the FP64 storage or (flags & BEM_APPLY_MIXED) the
+ * FP32 copy, times with HIP events (median of iters) the low-rank stage for
+ * candidate row lengths below which its work items take 8 factor rows per
+ * wave instead of 4, then the row-tile stage for candidate column counts of
+ * its workgroup pieces, and keeps the fastest of each.  report receives a JSON
+ * object with every candidate's milliseconds and the choices (truncated to
+ * report_len); returns the full report length + 1, or a negative status.
+ * Graphs captured with BEM_APPLY_GRAPH are dropped (recaptured on next use). */
+int bem_op_autotune(bem_op* op, int flags, int iters, char* report, int report_len, void* stream);
 /* measurement: run `iters` applies on device buffers of ones and return in
  * ms[6] the average milliseconds of [gather, low-rank t = V x (Ut x),
  * low-rank z = U t (V^T t), row/column kernel, scatter, whole apply] from HIP

@@ -14,7 +14,7 @@ _lib = None
 SPACE_P0, SPACE_P1, SPACE_HDIV, SPACE_HCURL = 0, 1, 2, 3
 (LAPLACE_SL, LAPLACE_DL, LAPLACE_HS, HELMHOLTZ_SL, HELMHOLTZ_DL, HELMHOLTZ_CF, HELMHOLTZ_HS, MAXWELL_SL,
  MAXWELL_DL, HELMHOLTZ_SL_DKAPPA) = range(10)
-APPLY_TRANS, APPLY_CONJ, APPLY_MIXED, APPLY_ON_DEVICE, APPLY_PER_VECTOR, BATCH_EAGER = 1, 2, 4, 8, 16, 32
+APPLY_TRANS, APPLY_CONJ, APPLY_MIXED, APPLY_ON_DEVICE, APPLY_PER_VECTOR, BATCH_EAGER, APPLY_GRAPH = 1, 2, 4, 8, 16, 32, 64
 BATCH_CONSUME = 1
 F64, C128, F32, C64 = 0, 1, 2, 3
 
@@ -52,7 +52,7 @@ class BlockInfo(ctypes.Structure):
 EXPORTS = ["bem_last_error", "bem_device_count", "bem_reserve", "bem_test_points", "bem_test_compression_aca", "bem_mesh_create", "bem_mesh_destroy", "bem_space_create", "bem_space_create_on",
            "bem_space_dofs",
            "bem_space_ndof", "bem_space_destroy", "bem_op_create", "bem_op_create_multi", "bem_op_assemble", "bem_op_apply",
-           "bem_op_stats", "bem_op_profile_apply", "bem_op_destroy", "bem_op_num_blocks", "bem_op_block_info", "bem_op_block_dofs",
+           "bem_op_stats", "bem_op_autotune", "bem_op_profile_apply", "bem_op_destroy", "bem_op_num_blocks", "bem_op_block_info", "bem_op_block_dofs",
            "bem_op_block_data", "bem_op_tree", "bem_op_potential_dim", "bem_op_potential", "bem_op_potential_noalloc", "bem_arnoldi_work_bytes", "bem_arnoldi_step", "bem_batch_create", "bem_batch_sizes", "bem_batch_apply",
            "bem_batch_destroy"]
 
@@ -101,6 +101,7 @@ def lib():
     L.bem_op_potential_dim.argtypes = [P]
     L.bem_op_potential.argtypes = [P, P, P, P, ctypes.c_int, P, P, ctypes.c_int, P]
     L.bem_op_potential_noalloc.argtypes = [P, P, P, P, ctypes.c_int, P, P, ctypes.c_int, P]
+    L.bem_op_autotune.argtypes = [P, ctypes.c_int, ctypes.c_int, ctypes.c_char_p, ctypes.c_int, P]
     L.bem_arnoldi_work_bytes.restype = ctypes.c_int64
     L.bem_arnoldi_work_bytes.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.c_int]
     L.bem_arnoldi_step.argtypes = [P, ctypes.c_int64, ctypes.c_int, ctypes.c_int, P, P, P, ctypes.c_int64,

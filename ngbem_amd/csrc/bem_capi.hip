@@ -23,6 +23,22 @@ struct bem_space {
 };
 struct bem_op {
   bem::Operator op;
+  // BEM_APPLY_GRAPH: the apply captured once per flag combination into a
+  // hipGraph on its own input / output vectors, replayed after
+  bem::DBuf gx, gy;
+  hipStream_t cs = nullptr;
+  hipGraphExec_t exec[8] = {};
+  void drop_graphs() {
+    for (auto& e : exec)
+      if (e) {
+        (void)hipGraphExecDestroy(e);
+        e = nullptr;
+      }
+  }
+  ~bem_op() {
+    drop_graphs();
+    if (cs) (void)hipStreamDestroy(cs);
+  }
 };
 struct bem_batch {
   bem::Operator op;                 // the merged operator
@@ -365,12 +381,47 @@ int bem_op_apply(bem_op* h, int flags, const double* alpha, const void* x, const
   if (nrhs > 1 && ld < std::max(nin, nout)) return fail(-1, "bem_op_apply: ld smaller than the vector length");
   if (op.complex_valued && dtype != BEM_C128) return fail(-1, "bem_op_apply: complex operator needs complex128 vectors");
   if (!op.complex_valued && dtype != BEM_F64) return fail(-1, "bem_op_apply: real operator needs float64 vectors");
+  if ((flags & BEM_APPLY_GRAPH) && (!(flags & BEM_APPLY_ON_DEVICE) || nrhs != 1))
+    return fail(-1, "bem_op_apply: BEM_APPLY_GRAPH needs device vectors and one right-hand side");
   return guarded([&] {
     hipStream_t s = (hipStream_t)stream;
     const size_t esz = op.complex_valued ? 16 : 8;
     const int dflags = flags & (BEM_APPLY_TRANS | BEM_APPLY_CONJ | BEM_APPLY_MIXED);
     // several right-hand sides of the forward product go through the matrix cores
     const bool block = nrhs > 1 && !trans && !(flags & BEM_APPLY_PER_VECTOR);
+    if (flags & BEM_APPLY_GRAPH) {
+      if (!op.assembled) throw std::runtime_error("operator not assembled");
+      if ((dflags & BEM_APPLY_MIXED) && !op.mixed)
+        throw std::runtime_error("operator was created without the mixed-precision copy");
+      hipGraphExec_t& ex = h->exec[dflags];
+      if (!ex) {  // capture gather .. scatter once per flag combination
+        const int n = std::max(op.ndof_trial(), op.ndof_test());
+        if (!h->gx.p) h->gx.alloc((size_t)n * esz);
+        if (!h->gy.p) h->gy.alloc((size_t)n * esz);
+        if (!h->cs) BEM_HIP(hipStreamCreateWithFlags(&h->cs, hipStreamNonBlocking));
+        const double one[2] = {1.0, 0.0}, zero[2] = {0.0, 0.0};
+        hipGraph_t g = nullptr;
+        BEM_HIP(hipStreamBeginCapture(h->cs, hipStreamCaptureModeThreadLocal));
+        try {
+          bem::apply_device(op, dflags, one, h->gx.p, zero, h->gy.p, h->cs);
+        } catch (...) {
+          (void)hipStreamEndCapture(h->cs, &g);
+          if (g) (void)hipGraphDestroy(g);
+          throw;
+        }
+        BEM_HIP(hipStreamEndCapture(h->cs, &g));
+        const hipError_t e = hipGraphInstantiate(&ex, g, nullptr, nullptr, 0);
+        (void)hipGraphDestroy(g);
+        if (e != hipSuccess) {
+          ex = nullptr;
+          throw bem::HipError(std::string("hipGraphInstantiate: ") + hipGetErrorString(e));
+        }
+      }
+      BEM_HIP(hipMemcpyAsync(h->gx.p, x, (size_t)nin * esz, hipMemcpyDeviceToDevice, s));
+      BEM_HIP(hipGraphLaunch(ex, s));
+      bem::axpby_device(op.complex_valued, y, h->gy.p, nout, alpha, beta, s);
+      return;
+    }
     if (flags & BEM_APPLY_ON_DEVICE) {
       if (block) {
         bem::apply_device_mm(op, dflags, alpha, x, beta, y, nrhs, ld, s);
@@ -437,6 +488,21 @@ int bem_op_profile_apply(bem_op* h, int flags, int iters, double* ms, void* stre
     }
     for (auto& e : ev) (void)hipEventDestroy(e);
   });
+}
+
+int bem_op_autotune(bem_op* h, int flags, int iters, char* report, int report_len, void* stream) {
+  if (!h || iters < 1 || report_len < 0 || (report_len > 0 && !report)) return fail(-1, "bem_op_autotune: bad arguments");
+  std::string rep;
+  const int rc = guarded([&] {
+    h->drop_graphs();  // the tuner replaces tables the captured graphs read
+    rep = bem::autotune(h->op, (flags & BEM_APPLY_MIXED) ? 1 : 0, iters, (hipStream_t)stream);
+  });
+  if (rc) return rc;
+  if (report_len > 0) {
+    std::strncpy(report, rep.c_str(), (size_t)report_len - 1);
+    report[report_len - 1] = 0;
+  }
+  return (int)rep.size() + 1;
 }
 
 int bem_op_stats(const bem_op* h, bem_stats* st) {

@@ -2938,35 +2938,40 @@ static void plan(const Operator& op, bool trans, std::vector<Tile>& tiles, std::
 // k_mv_far_t: complex128 storage (8 loads in flight) (8, 1) for rows up to 64
 // entries, else (4, 2); narrower storage (16 loads) (8, 2) up to 128 entries,
 // else (4, 4).  Longer rows take several rounds.
-static void far_variant(int len, bool wide, int& kc, int& nr) {
+// item shape of a factor row of length len: 8 rows per wave (complex128
+// storage: one column per lane step, else two) up to a row length of lim
+// entries (lim < 0: 64 for complex128 storage, 128 otherwise), else 4 rows
+// (two / four columns)
+static void far_variant(int len, bool wide, int& kc, int& nr, int lim = -1) {
+  if (lim < 0) lim = wide ? 64 : 128;
   if (wide) {
-    kc = len <= 64 ? 1 : 2;
-    nr = len <= 64 ? 8 : 4;
+    kc = len <= lim ? 1 : 2;
+    nr = len <= lim ? 8 : 4;
   } else {
-    kc = len <= 128 ? 2 : 4;
-    nr = len <= 128 ? 8 : 4;
+    kc = len <= lim ? 2 : 4;
+    nr = len <= lim ? 8 : 4;
+  }
+}
+
+static void build_row_items_one(const Operator& op, int prec, int trans, std::vector<RowItem>& it) {
+  const int esz = op.complex_valued ? 16 : 8;
+  const bool wide = (prec ? esz / 2 : esz) >= 16;
+  it.clear();
+  for (size_t k = 0; k < op.far.size(); k++) {
+    const FarBlk& fb = op.far[k];
+    const int len = trans ? fb.m : fb.n;
+    const int64_t base = trans ? fb.uoff : fb.voff;
+    const int xoff = trans ? fb.r0 : fb.c0;
+    int kc, nr;
+    far_variant(len, wide, kc, nr, trans ? -1 : op.tune_far_len[prec]);
+    for (int l = 0; l < fb.R; l += nr)
+      it.push_back(RowItem{base + (int64_t)l * len, len, xoff, fb.toff + l, std::min(nr, fb.R - l), kc, 0});
   }
 }
 
 static void build_row_items(const Operator& op, std::vector<RowItem> (&items)[2][2]) {
-  const int esz = op.complex_valued ? 16 : 8;
-  for (int prec = 0; prec < 2; prec++) {
-    const bool wide = (prec ? esz / 2 : esz) >= 16;
-    for (int trans = 0; trans < 2; trans++) {
-      std::vector<RowItem>& it = items[prec][trans];
-      it.clear();
-      for (size_t k = 0; k < op.far.size(); k++) {
-        const FarBlk& fb = op.far[k];
-        const int len = trans ? fb.m : fb.n;
-        const int64_t base = trans ? fb.uoff : fb.voff;
-        const int xoff = trans ? fb.r0 : fb.c0;
-        int kc, nr;
-        far_variant(len, wide, kc, nr);
-        for (int l = 0; l < fb.R; l += nr)
-          it.push_back(RowItem{base + (int64_t)l * len, len, xoff, fb.toff + l, std::min(nr, fb.R - l), kc, 0});
-      }
-    }
-  }
+  for (int prec = 0; prec < 2; prec++)
+    for (int trans = 0; trans < 2; trans++) build_row_items_one(op, prec, trans, items[prec][trans]);
 }
 
 // the transpose of every dense block as row items over its columns
@@ -3037,6 +3042,45 @@ static int rows_chunk() {
     return e ? std::max(0, std::atoi(e)) : 768;
   }();
   return c;
+}
+
+// the tiles' column lists cut into pieces of at most ch columns (ch <= 0:
+// one piece per tile); a cut tile's pieces get consecutive partial-sum slots
+static int build_pieces(const std::vector<int2>& range, int ch, std::vector<RowPiece>& pieces,
+                        std::vector<int4>& sums) {
+  pieces.clear();
+  sums.clear();
+  int slot = 0;
+  for (size_t t = 0; t < range.size(); t++) {
+    const int c0 = range[t].x, len = range[t].y - range[t].x;
+    const int np = ch > 0 ? std::max(1, (len + ch - 1) / ch) : 1;
+    if (np == 1) {
+      pieces.push_back(RowPiece{(int)t, c0, c0 + len, -1});
+      continue;
+    }
+    const int step = ((len + np - 1) / np + 31) / 32 * 32;  // whole 32-column steps of the workgroup
+    int k = 0;
+    for (int c = c0; c < c0 + len; c += step, k++)
+      pieces.push_back(RowPiece{(int)t, c, std::min(c0 + len, c + step), slot + k});
+    sums.push_back(make_int4((int)t, slot, k, 0));
+    slot += k;
+  }
+  return slot;
+}
+
+// pieces of one precision's row tiles, with the chunk the tuner chose (or the
+// default); the partial-sum buffer grows to the larger need of the two
+static void upload_pieces(Operator& op, int prec, hipStream_t s) {
+  std::vector<RowPiece> pieces;
+  std::vector<int4> sums;
+  const int ch = op.tune_rows_chunk[prec] >= 0 ? op.tune_rows_chunk[prec] : rows_chunk();
+  const int slots = build_pieces(op.h_frange, ch, pieces, sums);
+  op.d_fpieces[prec].upload(pieces, s);
+  op.n_fpieces[prec] = (int)pieces.size();
+  op.d_fsums[prec].upload(sums, s);
+  op.n_fsums[prec] = (int)sums.size();
+  const size_t need = (size_t)std::max(slots, 1) * 64 * (op.complex_valued ? 16 : 8);
+  if (op.d_fpart.bytes < need) op.d_fpart.alloc(need);
 }
 
 // flat column lists of the forward Ut plan (k_mv_rows_flat); tiles of one
@@ -3133,36 +3177,17 @@ void build_matvec_plan(Operator& op, hipStream_t s) {
   op.d_seg_fwd_mm.upload(s_ut, s);
   if (flat_ok) {
     op.d_fcols.upload(cols, s);
-    // cut the tiles' column lists into pieces of at most rows_chunk() columns
-    std::vector<RowPiece> pieces;
-    std::vector<int4> sums;
-    const int ch = rows_chunk();
-    int slot = 0;
-    for (size_t t = 0; t < range.size(); t++) {
-      const int c0 = range[t].x, len = range[t].y - range[t].x;
-      const int np = ch > 0 ? std::max(1, (len + ch - 1) / ch) : 1;
-      if (np == 1) {
-        pieces.push_back(RowPiece{(int)t, c0, c0 + len, -1});
-        continue;
-      }
-      const int step = ((len + np - 1) / np + 31) / 32 * 32;  // whole 32-column steps of the workgroup
-      int k = 0;
-      for (int c = c0; c < c0 + len; c += step, k++) pieces.push_back(RowPiece{(int)t, c, std::min(c0 + len, c + step),
-                                                                                 slot + k});
-      sums.push_back(make_int4((int)t, slot, k, 0));
-      slot += k;
-    }
-    op.d_fpieces.upload(pieces, s);
-    op.n_fpieces = (int)pieces.size();
-    op.d_fsums.upload(sums, s);
-    op.n_fsums = (int)sums.size();
-    op.d_fpart.alloc((size_t)std::max(slot, 1) * 64 * (op.complex_valued ? 16 : 8));
+    op.h_frange = range;
+    for (int prec = 0; prec < 2; prec++) upload_pieces(op, prec, s);
   } else {
     op.d_fcols.release();
-    op.d_fpieces.release();
-    op.d_fsums.release();
+    op.h_frange.clear();
+    for (int prec = 0; prec < 2; prec++) {
+      op.d_fpieces[prec].release();
+      op.d_fsums[prec].release();
+      op.n_fpieces[prec] = op.n_fsums[prec] = 0;
+    }
     op.d_fpart.release();
-    op.n_fpieces = op.n_fsums = 0;
   }
   op.d_tile_fwd.upload(t_fwd, s);
   op.d_seg_fwd.upload(s_fwd, s);
@@ -3883,13 +3908,14 @@ static void apply_t(Operator& op, int flags, const XT* x, XT* y, XT alpha, XT be
     }
     mark(3);
     const DBuf& segs = pk == PLAN_Z ? op.d_seg_fwd : op.d_seg_fwd_mm;
-    if (!zpath && op.d_fpieces.p && rows_flat()) {
-      hipLaunchKernelGGL((k_mv_rows_flat<ST, VT>), dim3(op.n_fpieces), dim3(256), 0, s, op.d_tile_fwd.as<Tile>(),
-                         op.d_fpieces.as<RowPiece>(), op.d_fcols.as<int2>(), nval, fU, xp, tb, yp,
-                         op.d_fpart.as<VT>(), xcd_remap());
-      if (op.n_fsums)
-        hipLaunchKernelGGL((k_mv_rows_psum<VT>), dim3(op.n_fsums), dim3(64), 0, s, op.d_tile_fwd.as<Tile>(),
-                           op.d_fsums.as<int4>(), (const VT*)op.d_fpart.as<VT>(), yp);
+    constexpr int prec = st_prec<ST>();
+    if (!zpath && op.d_fpieces[prec].p && rows_flat()) {
+      hipLaunchKernelGGL((k_mv_rows_flat<ST, VT>), dim3(op.n_fpieces[prec]), dim3(256), 0, s,
+                         op.d_tile_fwd.as<Tile>(), op.d_fpieces[prec].as<RowPiece>(), op.d_fcols.as<int2>(), nval, fU,
+                         xp, tb, yp, op.d_fpart.as<VT>(), xcd_remap());
+      if (op.n_fsums[prec])
+        hipLaunchKernelGGL((k_mv_rows_psum<VT>), dim3(op.n_fsums[prec]), dim3(64), 0, s, op.d_tile_fwd.as<Tile>(),
+                           op.d_fsums[prec].as<int4>(), (const VT*)op.d_fpart.as<VT>(), yp);
     } else
       hipLaunchKernelGGL((k_mv_rows<ST, VT>), dim3(op.n_tiles_fwd), dim3(256), 0, s, op.d_tile_fwd.as<Tile>(),
                          segs.as<Seg>(), nval, fU, z, xp, tb, yp);
@@ -4018,6 +4044,103 @@ void apply_device(Operator& op, int flags, const double* alpha, const void* x, c
                                     ev);
   }
   BEM_HIP(hipGetLastError());
+}
+
+// ---------------------------------------------------------------- autotune --
+
+std::string autotune(Operator& op, int prec, int iters, hipStream_t s) {
+  if (!op.assembled) throw std::runtime_error("operator not assembled");
+  if (prec == 1 && !op.mixed) throw std::runtime_error("operator was created without the mixed-precision copy");
+  iters = std::max(3, iters);
+  const size_t esz = op.complex_valued ? 16 : 8;
+  const int n = std::max(op.ndof_trial(), op.ndof_test());
+  DBuf dx, dy;
+  dx.alloc((size_t)n * esz);
+  dy.alloc((size_t)n * esz);
+  {
+    std::vector<double> ones((size_t)n * esz / 8, 1.0);
+    BEM_HIP(hipMemcpyAsync(dx.p, ones.data(), (size_t)n * esz, hipMemcpyHostToDevice, s));
+    BEM_HIP(hipStreamSynchronize(s));
+  }
+  hipEvent_t ev[6];
+  for (auto& e : ev) BEM_HIP(hipEventCreate(&e));
+  const double one[2] = {1.0, 0.0}, zero[2] = {0.0, 0.0};
+  const int flags = prec ? 4 : 0;
+  // median milliseconds of the stages [k0, k1) of the forward apply
+  auto timed = [&](int k0, int k1) {
+    std::vector<float> t;
+    for (int it = 0; it < iters + 1; it++) {
+      apply_device(op, flags, one, dx.p, zero, dy.p, s, ev);
+      BEM_HIP(hipEventSynchronize(ev[5]));
+      float ms = 0.f;
+      BEM_HIP(hipEventElapsedTime(&ms, ev[k0], ev[k1]));
+      if (it) t.push_back(ms);  // the first run warms the new tables
+    }
+    std::sort(t.begin(), t.end());
+    return (double)t[t.size() / 2];
+  };
+  char buf[128];
+  std::string rep = std::string("{\"precision\": \"") + (prec ? "fp32" : "fp64") + "\"";
+  std::snprintf(buf, sizeof buf, ", \"apply_ms_before\": %.4f", timed(0, 5));
+  rep += buf;
+  // low-rank stage: the row length below which items take 8 rows
+  if (!op.far.empty()) {
+    const int cand[] = {0, 32, 64, 96, 128, 192, 256, 512, 1 << 30};
+    int best = op.tune_far_len[prec];
+    double tbest = 1e30;
+    rep += ", \"far_t\": [";
+    std::vector<RowItem> it;
+    for (size_t c = 0; c < sizeof cand / sizeof cand[0]; c++) {
+      op.tune_far_len[prec] = cand[c];
+      build_row_items_one(op, prec, 0, it);
+      op.d_items[prec][0].upload(it, s);
+      op.n_items[prec][0] = (int)it.size();
+      BEM_HIP(hipStreamSynchronize(s));
+      const double t = timed(1, 2);
+      std::snprintf(buf, sizeof buf, "%s{\"rows8_up_to_len\": %d, \"ms\": %.4f}", c ? ", " : "", cand[c], t);
+      rep += buf;
+      if (t < tbest) {
+        tbest = t;
+        best = cand[c];
+      }
+    }
+    op.tune_far_len[prec] = best;
+    build_row_items_one(op, prec, 0, it);
+    op.d_items[prec][0].upload(it, s);
+    op.n_items[prec][0] = (int)it.size();
+    BEM_HIP(hipStreamSynchronize(s));
+    std::snprintf(buf, sizeof buf, "], \"far_t_choice\": %d", best);
+    rep += buf;
+  }
+  // row tiles: the columns per workgroup piece
+  if (!op.h_frange.empty() && mv_plan() != PLAN_Z && rows_flat()) {
+    const int cand[] = {0, 256, 384, 512, 768, 1024, 1536, 2048};
+    int best = op.tune_rows_chunk[prec];
+    double tbest = 1e30;
+    rep += ", \"rows\": [";
+    for (size_t c = 0; c < sizeof cand / sizeof cand[0]; c++) {
+      op.tune_rows_chunk[prec] = cand[c];
+      upload_pieces(op, prec, s);
+      BEM_HIP(hipStreamSynchronize(s));
+      const double t = timed(3, 4);
+      std::snprintf(buf, sizeof buf, "%s{\"columns_per_piece\": %d, \"pieces\": %d, \"ms\": %.4f}", c ? ", " : "",
+                    cand[c], op.n_fpieces[prec], t);
+      rep += buf;
+      if (t < tbest) {
+        tbest = t;
+        best = cand[c];
+      }
+    }
+    op.tune_rows_chunk[prec] = best;
+    upload_pieces(op, prec, s);
+    BEM_HIP(hipStreamSynchronize(s));
+    std::snprintf(buf, sizeof buf, "], \"rows_choice\": %d", best);
+    rep += buf;
+  }
+  std::snprintf(buf, sizeof buf, ", \"apply_ms_after\": %.4f}", timed(0, 5));
+  rep += buf;
+  for (auto& e : ev) (void)hipEventDestroy(e);
+  return rep;
 }
 
 }  // namespace bem

@@ -247,8 +247,18 @@ struct Operator {
   // the same forward plan as one flat column list per row tile: (element
   // offset of the column, x / t index | COL_FAR); empty when the offsets
   // overflow 31 bits
-  DBuf d_fcols, d_fpieces, d_fsums, d_fpart;  // flat column lists, their pieces, sums of cut tiles, partials
-  int n_fpieces = 0, n_fsums = 0;
+  // flat column lists of the forward row tiles, their ranges per tile (host),
+  // and per precision (0 the FP64 / complex128 storage, 1 the FP32 copy) the
+  // pieces the lists are cut into and the sums of the cut tiles; partial sums
+  DBuf d_fcols, d_fpieces[2], d_fsums[2], d_fpart;
+  int n_fpieces[2] = {0, 0}, n_fsums[2] = {0, 0};
+  std::vector<int2> h_frange;
+  // apply tunables per precision, set by autotune(); -1: the built-in rules
+  // (low-rank row items of 8 rows up to a row length of 64 (complex128) / 128
+  // entries, else 4 rows; row tiles cut into pieces of NGBEM_ROWS_CHUNK or 768
+  // columns)
+  int tune_far_len[2] = {-1, -1};
+  int tune_rows_chunk[2] = {-1, -1};
   int n_tiles_fwd = 0, n_tiles_trn = 0;
   DBuf d_zitems_fwd, d_zitems_trn;  // U t / V^T t work items
   int n_zitems_fwd = 0, n_zitems_trn = 0;
@@ -305,6 +315,12 @@ void potential_device(Operator& op, const void* coef, const double* pts, const d
 constexpr size_t POTENTIAL_SCRATCH_BYTES = (size_t)64 << 20;
 void potential_device_noalloc(Operator& op, const void* coef, const double* pts, const double* nrm, int np,
                               const unsigned char* elmask, void* out, hipStream_t s);
+// times candidate shapes of the forward apply's two main launches on the
+// assembled operator (prec 0: FP64 / complex128 storage, 1: the FP32 copy):
+// the row length below which the low-rank items take 8 rows instead of 4,
+// then the column count of the row-tile pieces; keeps the fastest of each and
+// returns a JSON report of every candidate's median milliseconds
+std::string autotune(Operator& op, int prec, int iters, hipStream_t s);
 // GMRES Arnoldi step on device vectors (bem_krylov.hip): w orthogonalised
 // against the rows V[0..k] by classical Gram-Schmidt twice, h[0..k] = the
 // coefficients of both passes, h[k+1] = |w|, V[k+1] = w / |w|; work holds

@@ -405,11 +405,12 @@ class _MassInverse(BaseMatrix):
 class HMatrix(BaseMatrix):
     """The assembled H-matrix of one operator (src/hmat.hpp:163-201)."""
 
-    def __init__(self, op, mixed=False):
+    def __init__(self, op, mixed=False, graph=False):
         self._op = op
         self.height, self.width = op.test_space.ndof, op.trial_space.ndof
         self.is_complex = op.is_complex
         self._mixed = mixed
+        self._graph = graph
 
     @property
     def nze(self):
@@ -420,7 +421,14 @@ class HMatrix(BaseMatrix):
         """View of this matrix applied with the FP32 / complex64 copy."""
         if not self._op.has_mixed:
             raise RuntimeError("operator built without mixed=True")
-        return HMatrix(self._op, mixed=True)
+        return HMatrix(self._op, mixed=True, graph=self._graph)
+
+    @property
+    def graph(self):
+        """View of this matrix whose single-vector applies replay the apply
+        captured once in a hipGraph (BEM_APPLY_GRAPH); no counterpart in the
+        reference."""
+        return HMatrix(self._op, mixed=self._mixed, graph=True)
 
     def IsComplex(self):
         return self.is_complex
@@ -437,6 +445,8 @@ class HMatrix(BaseMatrix):
             raise ValueError("output vector must live on the GPU")
         if self._mixed:
             flags |= _capi.APPLY_MIXED
+        if self._graph and x.dim() == 1:
+            flags |= _capi.APPLY_GRAPH
         s = complex(s)
         if not self.is_complex:
             # real operator: real and imaginary parts separately
@@ -481,7 +491,8 @@ class HMatrix(BaseMatrix):
         if (self.is_complex and torch.is_complex(x) and x.dtype == y.dtype == torch.complex128 and x.is_cuda
                 and y.is_cuda and x.dim() == 1 and x.shape[0] == self.width and y.shape[0] == self.height
                 and _raw_view(x) and _raw_view(y)):
-            self._op._call_apply(_capi.APPLY_MIXED if self._mixed else 0, 1.0, x, 0.0, y)
+            self._op._call_apply((_capi.APPLY_MIXED if self._mixed else 0) | (_capi.APPLY_GRAPH if self._graph else 0),
+                                 1.0, x, 0.0, y)
             return
         BaseMatrix.Mult(self, x, y)
 
@@ -633,6 +644,28 @@ class IntegralOperator:
         vector) as a callable ``pot(points, normals=None)`` evaluated on the GPU
         (GenericIntegralOperator::GetPotential, src/ngbem.cpp:1166-1235)."""
         return PotentialCF(self, gf)
+
+    def Autotune(self, mixed=None, iters=10):
+        """Time candidate work shapes of the forward apply on this operator and
+        keep the fastest (bem_op_autotune): the FP64 storage, and the FP32 copy
+        too when the operator has one (``mixed=None``) or only it
+        (``mixed=True``).  Returns {"fp64": report, "fp32": report} with each
+        candidate's median milliseconds.  No counterpart in the reference."""
+        import json
+
+        precs = [False, True] if mixed is None else [bool(mixed)]
+        if mixed is None and not self.has_mixed:
+            precs = [False]
+        out = {}
+        L = _capi.lib()
+        buf = ctypes.create_string_buffer(1 << 14)
+        for m in precs:
+            stream = torch.cuda.current_stream().cuda_stream
+            rc = L.bem_op_autotune(self._h, _capi.APPLY_MIXED if m else 0, int(iters), buf, len(buf), stream)
+            if rc < 0:
+                _capi.check(rc, "bem_op_autotune")
+            out["fp32" if m else "fp64"] = json.loads(buf.value.decode())
+        return out
 
     def stats(self):
         st = _capi.Stats()
