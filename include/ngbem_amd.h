@@ -158,6 +158,14 @@ int bem_space_create(bem_mesh* m, int kind, int order, bem_space** out);
  * are numbered compactly over the region's elements, in the order the
  * whole-mesh numbering gives them. */
 int bem_space_create_on(bem_mesh* m, int kind, int order, const unsigned char* elmask, bem_space** out);
+/* NGSolve's SurfaceL2 flag dual_mapping (the reference's tests/test_aca.py:9,
+ * tests/test_SLIntegration.py:29): every shape function of element e is
+ * divided by its Jacobian J_e = 2 |e| (flat triangles), so an operator
+ * between such spaces is D_test^-1 A D_trial^-1 with D = diag(J) per dof and
+ * the same cluster trees and cross approximation; the potential of a
+ * coefficient vector c is that of D^-1 c.  Set before bem_op_create; only
+ * SurfaceL2 spaces (BEM_SPACE_P0, any order). */
+int bem_space_set_dual_mapping(bem_space* s, int on);
 int bem_space_ndof(const bem_space* s);
 /* element -> dof table (nt x nloc, nloc <= 10; -1 on elements outside the
  * space's region); returns nloc.  Replaces FESpace::GetDofNrs over the surface

@@ -173,6 +173,27 @@ int bem_space_create(bem_mesh* m, int kind, int order, bem_space** out) {
   return bem_space_create_on(m, kind, order, nullptr, out);
 }
 
+int bem_space_set_dual_mapping(bem_space* s, int on) {
+  if (!s) return fail(-1, "bem_space_set_dual_mapping: bad arguments");
+  bem::Space& sp = *s->s;
+  const int k = sp.kind;
+  if (!(k == bem::SP_P0 || k == bem::SP_L2_1 || k == bem::SP_L2_2 || k == bem::SP_L2_3))
+    return fail(-1, "bem_space_set_dual_mapping: only SurfaceL2 spaces have a dual mapping");
+  return guarded([&] {
+    if (!on) {
+      sp.dof_scale.clear();
+      return;
+    }
+    const bem::Mesh& m = *sp.mesh;
+    sp.dof_scale.assign(sp.ndof, 1.0);
+    for (int e = 0; e < m.nt; e++)
+      for (int l = 0; l < sp.nloc; l++) {
+        const int d = sp.dofs[(size_t)e * sp.nloc + l];
+        if (d >= 0) sp.dof_scale[d] = 1.0 / m.J[e];
+      }
+  });
+}
+
 int bem_space_ndof(const bem_space* s) { return s ? s->s->ndof : -1; }
 
 int bem_space_dofs(const bem_space* s, int32_t* dofs) {
@@ -362,6 +383,7 @@ int bem_op_assemble(bem_op* h, void* stream) {
     if (far_err) std::rethrow_exception(far_err);
     if (near_err) std::rethrow_exception(near_err);
     pt_.mark("near + far field");
+    bem::apply_dof_scale(op, s);
     bem::build_matvec_plan(op, s);
     pt_.mark("matvec plan");
     if (op.mixed) bem::make_mixed_copy(op, s);

@@ -2196,6 +2196,8 @@ void setup_device(Operator& op, hipStream_t s) {
   pt_.mark("host tables");
   op.d_dofs_test.upload(dof_table(*op.test), s);
   op.d_dofs_trial.upload(dof_table(*op.trial), s);
+  if (!op.trial->dof_scale.empty()) op.d_dscale_trial.upload(op.trial->dof_scale, s);
+  else op.d_dscale_trial.release();
   std::vector<double> trig((size_t)ntq * 3);
   for (int q = 0; q < ntq; q++) {
     trig[3 * q] = op.trig.xi[q];
@@ -3268,6 +3270,7 @@ __global__ __launch_bounds__(256) void k_pot_src(const int* __restrict__ elist, 
                                                  const double* __restrict__ quad, const double* __restrict__ geo,
                                                  const double* __restrict__ sdat, const double* __restrict__ trig,
                                                  const int* __restrict__ dofs, const typename KER::T* __restrict__ coef,
+                                                 const double* __restrict__ dscale,
                                                  PotSrc<typename KER::T, EJ::DIM>* __restrict__ src) {
   typedef typename KER::T T;
   const int idx = blockIdx.x * 256 + threadIdx.x;
@@ -3294,7 +3297,7 @@ __global__ __launch_bounds__(256) void k_pot_src(const int* __restrict__ elist, 
 #pragma unroll
     for (int l = 0; l < EJ::NL; l++) {
       const int d = dofs[DOF_STRIDE * (size_t)e + l];
-      if (d >= 0) a += rmul(coef[d], v[l * EJ::DIM + c]);
+      if (d >= 0) a += rmul(coef[d], dscale ? v[l * EJ::DIM + c] * dscale[d] : v[l * EJ::DIM + c]);
     }
     o.v[c] = a;
   }
@@ -3450,7 +3453,7 @@ static void potential_run(Operator& op, const void* coef, const double* pts, con
     const int chunk = (nsrc + nsplit - 1) / nsplit;
     hipLaunchKernelGGL((k_pot_src<KER, EJ>), dim3((nsrc + 255) / 256), dim3(256), 0, s, el, ne, ntq,
                        op.d_quad.as<double>(), op.d_geo.as<double>(), op.d_sdat.as<double>(), op.d_trig.as<double>(),
-                       op.d_dofs_trial.as<int>(), (const T*)coef, sp);
+                       op.d_dofs_trial.as<int>(), (const T*)coef, op.d_dscale_trial.as<double>(), sp);
     hipLaunchKernelGGL((k_potential<KER, D>), dim3(gx, nsplit), dim3(256), 0, s, sp, nsrc, chunk, pts, nrm, np, ker,
                        pp);
     if (pp != (T*)out)
@@ -4044,6 +4047,68 @@ void apply_device(Operator& op, int flags, const double* alpha, const void* x, c
                                     ev);
   }
   BEM_HIP(hipGetLastError());
+}
+
+// --------------------------------------------------------- dual mapping --
+
+template <class T>
+__global__ __launch_bounds__(256) void k_scale_near(const NearBlk* __restrict__ blks, int nblk,
+                                                    const double* __restrict__ st, const double* __restrict__ sr,
+                                                    T* __restrict__ val) {
+  for (int b = blockIdx.x; b < nblk; b += gridDim.x) {
+    const NearBlk nb = blks[b];
+    const int64_t cnt = (int64_t)nb.m * nb.n;
+    for (int64_t i = threadIdx.x; i < cnt; i += 256) {
+      const int c = (int)(i / nb.m), r = (int)(i - (int64_t)c * nb.m);
+      val[nb.off + i] = rmul(val[nb.off + i], (st ? st[nb.r0 + r] : 1.0) * (sr ? sr[nb.c0 + c] : 1.0));
+    }
+  }
+}
+
+// U stored as Ut (R x m, rank rows): entry (l, i) at uoff + l m + i; V (R x n)
+template <class T>
+__global__ __launch_bounds__(256) void k_scale_far(const FarBlk* __restrict__ blks, int nblk,
+                                                   const double* __restrict__ st, const double* __restrict__ sr,
+                                                   T* __restrict__ U, T* __restrict__ V) {
+  for (int b = blockIdx.x; b < nblk; b += gridDim.x) {
+    const FarBlk fb = blks[b];
+    if (st)
+      for (int64_t i = threadIdx.x; i < (int64_t)fb.R * fb.m; i += 256)
+        U[fb.uoff + i] = rmul(U[fb.uoff + i], st[fb.r0 + (int)(i % fb.m)]);
+    if (sr)
+      for (int64_t i = threadIdx.x; i < (int64_t)fb.R * fb.n; i += 256)
+        V[fb.voff + i] = rmul(V[fb.voff + i], sr[fb.c0 + (int)(i % fb.n)]);
+  }
+}
+
+void apply_dof_scale(Operator& op, hipStream_t s) {
+  const bool ts = !op.test->dof_scale.empty(), rs = !op.trial->dof_scale.empty();
+  if (!ts && !rs) return;
+  // factors in cluster order of each side
+  auto by_pos = [](const Space& sp, const Tree& t) {
+    std::vector<double> v(t.perm.size());
+    for (size_t i = 0; i < v.size(); i++) v[i] = sp.dof_scale[t.perm[i]];
+    return v;
+  };
+  DBuf dst, dsr;
+  if (ts) dst.upload(by_pos(*op.test, *op.test_tree), s);
+  if (rs) dsr.upload(by_pos(*op.trial, op.trial_tree), s);
+  const double* st = ts ? dst.as<double>() : nullptr;
+  const double* sr = rs ? dsr.as<double>() : nullptr;
+  const int nn = (int)op.near.size(), nf = (int)op.far.size();
+  auto run = [&](auto tag) {
+    using T = decltype(tag);
+    if (nn)
+      hipLaunchKernelGGL((k_scale_near<T>), dim3((unsigned)std::min(nn, 65536)), dim3(256), 0, s,
+                         op.d_near_blk.as<NearBlk>(), nn, st, sr, op.d_near_val.as<T>());
+    if (nf)
+      hipLaunchKernelGGL((k_scale_far<T>), dim3((unsigned)std::min(nf, 65536)), dim3(256), 0, s,
+                         op.d_far_blk.as<FarBlk>(), nf, st, sr, op.d_fU.as<T>(), op.d_fV.as<T>());
+  };
+  if (op.complex_valued) run(zc{});
+  else run(0.0);
+  BEM_HIP(hipGetLastError());
+  BEM_HIP(hipStreamSynchronize(s));
 }
 
 // ---------------------------------------------------------------- autotune --

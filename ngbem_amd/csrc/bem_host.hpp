@@ -67,6 +67,10 @@ struct Space {
   std::vector<int> edges;                  // 2*nedges (RWG/NED): sorted vertex pairs
   std::vector<int> ed_off, ed_list;        // dof -> elements (CSR, ascending)
   std::vector<unsigned char> region;       // per element, 1 = in the space (empty: every element)
+  // SurfaceL2 with NGSolve's dual_mapping: every shape of element e is
+  // divided by the Jacobian J_e = 2 |e|, i.e. dof d is scaled by dof_scale[d]
+  // (empty: no scaling)
+  std::vector<double> dof_scale;
   // elmask: the space lives on the elements with elmask[e] != 0 (null: all);
   // dofs are numbered compactly over those elements, in the order of the
   // numbering on the whole mesh

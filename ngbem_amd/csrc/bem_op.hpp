@@ -236,6 +236,7 @@ struct Operator {
   // device data
   DBuf d_geo, d_vid, d_sdat, d_quad, d_trig, d_ss[3];
   DBuf d_dofs_test, d_dofs_trial;  // int4 per element: the element's dofs
+  DBuf d_dscale_trial;             // dual-mapped trial side: factor per dof (else empty)
   DBuf d_perm_test, d_iperm_test, d_perm_trial, d_iperm_trial;
   DBuf d_patch;            // near patches of P1 sides (test then trial)
   DBuf d_near_blk, d_near_chunks, d_near_val, d_near_val32;
@@ -297,6 +298,10 @@ void assemble_near(Operator& op, hipStream_t s);
 void assemble_far(Operator& op, hipStream_t s);
 void build_matvec_plan(Operator& op, hipStream_t s);
 void make_mixed_copy(Operator& op, hipStream_t s);
+// dual-mapped SurfaceL2 sides: scale the stored blocks (dense entries, U rows,
+// V columns) by the test and trial dofs' factors; nothing when neither side
+// has one
+void apply_dof_scale(Operator& op, hipStream_t s);
 // y = alpha * op(A) x + beta * y on device vectors (complex128 or double)
 // ev (optional, 6 events): recorded before the gather and after the gather,
 // the low-rank t stage, the low-rank z stage, the row/column kernel and the

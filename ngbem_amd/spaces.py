@@ -52,8 +52,8 @@ class FESpace:
 
     def __init__(self, mesh: Mesh, order: int, complex: bool = False, definedon=None, dirichlet=None,
                  dual_mapping: bool = False):
-        if dual_mapping:
-            raise NotImplementedError("dual_mapping=True is not supported")
+        if dual_mapping and not isinstance(self, SurfaceL2):
+            raise NotImplementedError("dual_mapping is a SurfaceL2 option")
         if order not in self.orders:
             raise NotImplementedError(f"{type(self).__name__} supports orders {list(self.orders)}")
         self.order = int(order)
@@ -84,6 +84,18 @@ class FESpace:
         if nloc < 1:
             _capi.check(nloc, "bem_space_dofs")
         self.elem_dofs = tab[: nloc * mesh.nt].reshape(mesh.nt, nloc).astype(np.int64)
+        # NGSolve's dual_mapping: shapes divided by the Jacobian 2 |e| of their
+        # triangle; dof_scale[d] = 1 / J of dof d's triangle (None: no scaling)
+        self.dual_mapping = bool(dual_mapping)
+        self.dof_scale = None
+        if self.dual_mapping:
+            _capi.check(L.bem_space_set_dual_mapping(out, 1), "bem_space_set_dual_mapping")
+            sc = np.ones(self.ndof)
+            on = self.elem_dofs[:, 0] >= 0
+            J = 2.0 * mesh.areas()
+            for l in range(nloc):
+                sc[self.elem_dofs[on, l]] = 1.0 / J[on]
+            self.dof_scale = sc
 
     @property
     def elements(self):
@@ -138,7 +150,10 @@ def _dubiner(p, x, y):
 class SurfaceL2(FESpace):
     """SurfaceL2, orders 0-3: constants (dof = triangle), plus for orders 1-3
     the triangle's higher Dubiner functions (in the element's barycentrics
-    x = lambda_0, y = lambda_1; dofs nt + e (nloc - 1) + k - 1)."""
+    x = lambda_0, y = lambda_1; dofs nt + e (nloc - 1) + k - 1).
+    ``dual_mapping=True`` (NGSolve's flag): every shape divided by its
+    triangle's Jacobian 2 |e| (``dof_scale``); ``shapes`` stays the reference
+    values."""
     kind = _capi.SPACE_P0
     orders = (0, 1, 2, 3)
 
@@ -329,4 +344,10 @@ def mass_matrix_coo(trial: FESpace, test: FESpace):
     cols = np.tile(trial.elem_dofs, (1, ni)).reshape(-1)
     vals = np.ascontiguousarray(loc).reshape(-1)
     keep = (rows >= 0) & (cols >= 0)  # elements of both spaces' regions
-    return rows[keep], cols[keep], vals[keep]
+    rows, cols, vals = rows[keep], cols[keep], vals[keep]
+    # dual-mapped SurfaceL2: u = u_hat / J per triangle on either side
+    if getattr(test, "dof_scale", None) is not None:
+        vals = vals * test.dof_scale[rows]
+    if getattr(trial, "dof_scale", None) is not None:
+        vals = vals * trial.dof_scale[cols]
+    return rows, cols, vals
