@@ -147,6 +147,17 @@ int bem_test_compression_aca(const double* X, int nx, const double* Y, int ny, d
 int bem_reserve(double bytes);
 
 int bem_mesh_create(const double* xyz, int nv, const int32_t* tri, int nt, bem_mesh** out);
+/* Curved (isoparametric) triangles of order 2-4, NGSolve's mesh.Curve(order)
+ * as every test of the reference meshes (e.g. tests/test_aca.py:7): nodes =
+ * nt x (order+1)(order+2)/2 x 3 positions of each element's Lagrange nodes at
+ * reference coordinates (xi, eta) = (i / order, j / order), i + j <= order,
+ * listed j outer, i inner (xi = 1 is vertex 0, eta = 1 vertex 1, the origin
+ * vertex 2); the geometry of each element is the polynomial through them
+ * (src/ngbem.cpp:685-692 and :1086-1091 evaluate NGSolve's curved
+ * ElementTransformation at the quadrature points).  Before any space is
+ * created on the mesh.  Scalar spaces (SurfaceL2 without the dual mapping, H1)
+ * only; operators on edge spaces of a curved mesh are refused. */
+int bem_mesh_set_curved(bem_mesh* m, int order, const double* nodes);
 int bem_mesh_destroy(bem_mesh* m);
 
 int bem_space_create(bem_mesh* m, int kind, int order, bem_space** out);

@@ -49,7 +49,7 @@ class BlockInfo(ctypes.Structure):
 
 
 # every symbol include/ngbem_amd.h declares
-EXPORTS = ["bem_last_error", "bem_device_count", "bem_reserve", "bem_test_points", "bem_test_compression_aca", "bem_mesh_create", "bem_mesh_destroy", "bem_space_create", "bem_space_create_on", "bem_space_set_dual_mapping",
+EXPORTS = ["bem_last_error", "bem_device_count", "bem_reserve", "bem_test_points", "bem_test_compression_aca", "bem_mesh_create", "bem_mesh_set_curved", "bem_mesh_destroy", "bem_space_create", "bem_space_create_on", "bem_space_set_dual_mapping",
            "bem_space_dofs",
            "bem_space_ndof", "bem_space_destroy", "bem_op_create", "bem_op_create_multi", "bem_op_assemble", "bem_op_apply",
            "bem_op_stats", "bem_op_autotune", "bem_op_profile_apply", "bem_op_destroy", "bem_op_num_blocks", "bem_op_block_info", "bem_op_block_dofs",
@@ -79,6 +79,7 @@ def lib():
     L.bem_space_create.argtypes = [P, ctypes.c_int, ctypes.c_int, ctypes.POINTER(P)]
     L.bem_space_create_on.argtypes = [P, ctypes.c_int, ctypes.c_int, P, ctypes.POINTER(P)]
     L.bem_space_set_dual_mapping.argtypes = [P, ctypes.c_int]
+    L.bem_mesh_set_curved.argtypes = [P, ctypes.c_int, dp]
     L.bem_space_dofs.argtypes = [P, ip]
     L.bem_space_ndof.argtypes = [P]
     L.bem_space_destroy.argtypes = [P]

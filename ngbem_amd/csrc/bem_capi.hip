@@ -138,6 +138,13 @@ int bem_mesh_create(const double* xyz, int nv, const int32_t* tri, int nt, bem_m
   });
 }
 
+int bem_mesh_set_curved(bem_mesh* m, int order, const double* nodes) {
+  if (!m || !nodes || order < 1 || order > bem::GEO_ORDER_MAX) return fail(-1, "bem_mesh_set_curved: bad arguments");
+  if (m->m.use_count() > 1)
+    return fail(-1, "bem_mesh_set_curved: curve the mesh before creating spaces on it");
+  return guarded([&] { m->m->set_curved(order, nodes); });
+}
+
 int bem_mesh_destroy(bem_mesh* m) {
   delete m;
   return 0;
@@ -179,6 +186,8 @@ int bem_space_set_dual_mapping(bem_space* s, int on) {
   const int k = sp.kind;
   if (!(k == bem::SP_P0 || k == bem::SP_L2_1 || k == bem::SP_L2_2 || k == bem::SP_L2_3))
     return fail(-1, "bem_space_set_dual_mapping: only SurfaceL2 spaces have a dual mapping");
+  if (on && sp.mesh->gorder > 1)  // the mapping's 1 / J varies over a curved element
+    return fail(-1, "bem_space_set_dual_mapping: not available on curved meshes");
   return guarded([&] {
     if (!on) {
       sp.dof_scale.clear();
@@ -230,6 +239,8 @@ static int create_op(int kernel, double kappa, bem_space* trial, bem_space* test
                       "space on both sides");
     if (kernel == bem::K_MDL && !(bem::is_hcurl_kind(tk) && bem::is_hdiv_kind(sk)))
       return fail(-1, "bem_op_create: Maxwell double layer needs an HCurl trial and an HDivSurface test space");
+    if (trial->s->mesh->gorder > 1 && !scalar)
+      return fail(-1, "bem_op_create: edge spaces (HDivSurface, HCurl) on curved meshes are not supported");
   }
   return guarded([&] {
     const double t0 = now();

@@ -285,20 +285,23 @@ struct KMaxDL {  // e^{ikr}(ikr - 1)(x-y)/(4 pi r^3), cross-product terms (src/n
 // the others index by element-local dof and take local barycentrics.
 // MULTI: more than one dof per element (scatter through the dof tables).
 constexpr int SDS = 26;
+// curved elements, per quadrature point of the triangle rule: unit normal,
+// then n x grad_Gamma lambda_0, lambda_1, lambda_2
+constexpr int QG = 12;
 
 template <int SP, int EV>
 struct Ev;
 template <>
 struct Ev<SP_P0, EV_ID> {
   static constexpr int NL = 1, DIM = 1, NB = 1;
-  static constexpr bool MULTI = false, PERM = false;
+  static constexpr bool MULTI = false, PERM = false, CURLS = false;
   __device__ __forceinline__ static void load(const double*, const int*, double*) {}
   __device__ __forceinline__ static void shape(const double*, P3, const double*, double* o) { o[0] = 1.0; }
 };
 template <>
 struct Ev<SP_P1, EV_ID> {
   static constexpr int NL = 3, DIM = 1, NB = 1;
-  static constexpr bool MULTI = true, PERM = true;
+  static constexpr bool MULTI = true, PERM = true, CURLS = false;
   __device__ __forceinline__ static void load(const double*, const int*, double*) {}
   __device__ __forceinline__ static void shape(const double* lam, P3, const double*, double* o) {
     o[0] = lam[0]; o[1] = lam[1]; o[2] = lam[2];
@@ -307,7 +310,7 @@ struct Ev<SP_P1, EV_ID> {
 template <>
 struct Ev<SP_P1, EV_ROT> {
   static constexpr int NL = 3, DIM = 3, NB = 9;
-  static constexpr bool MULTI = true, PERM = true;
+  static constexpr bool MULTI = true, PERM = true, CURLS = true;
   __device__ __forceinline__ static void load(const double* sd, const int* px, double* b) {
     for (int k = 0; k < 3; k++)
       for (int c = 0; c < 3; c++) b[3 * k + c] = sd[3 * px[k] + c];
@@ -320,7 +323,7 @@ struct Ev<SP_P1, EV_ROT> {
 template <>
 struct Ev<SP_P1, EV_HELM> {
   static constexpr int NL = 3, DIM = 4, NB = 9;
-  static constexpr bool MULTI = true, PERM = true;
+  static constexpr bool MULTI = true, PERM = true, CURLS = true;
   __device__ __forceinline__ static void load(const double* sd, const int* px, double* b) {
     Ev<SP_P1, EV_ROT>::load(sd, px, b);
   }
@@ -340,7 +343,7 @@ struct Ev<SP_P1, EV_HELM> {
 template <int SGN, bool DIV>
 struct EvEdge {
   static constexpr int NL = 3, DIM = DIV ? 4 : 3, NB = 12;
-  static constexpr bool MULTI = true, PERM = false;
+  static constexpr bool MULTI = true, PERM = false, CURLS = false;
   __device__ __forceinline__ static void load(const double* sd, const int*, double* b) {
     for (int i = 0; i < 12; i++) b[i] = sd[9 + i];
   }
@@ -383,7 +386,7 @@ __device__ __forceinline__ double jacobi_a0(int n, double a, double t) {
 template <int SIDE, int EVK, int NLR>
 struct EvGen {
   static constexpr int NL = NLR, DIM = EVK == EV_ID ? 1 : (EVK == EV_ROT ? 3 : 4), NB = 13;
-  static constexpr bool MULTI = true, PERM = false;
+  static constexpr bool MULTI = true, PERM = false, CURLS = true;
   __device__ __forceinline__ static void load(const double* sd, const int*, double* b) {
     for (int i = 0; i < 9; i++) b[i] = sd[i];
     b[9] = sd[21 + SIDE];
@@ -465,7 +468,7 @@ template <> struct Ev<SP_NED, EV_TRACE3> : EvEdge<-1, false> {};
 template <int SGN, bool DIV>
 struct EvEdge1 {
   static constexpr int NL = 6, DIM = DIV ? 4 : 3, NB = 21;
-  static constexpr bool MULTI = true, PERM = false;
+  static constexpr bool MULTI = true, PERM = false, CURLS = false;
   __device__ __forceinline__ static void load(const double* sd, const int*, double* b) {
     for (int i = 0; i < 21; i++) b[i] = sd[i];
   }
@@ -493,6 +496,65 @@ template <> struct Ev<SP_RWG1, EV_MAXW> : EvEdge1<1, true> {};
 template <> struct Ev<SP_NED1, EV_MAXW> : EvEdge1<-1, true> {};
 template <> struct Ev<SP_RWG1, EV_TRACE3> : EvEdge1<1, false> {};
 template <> struct Ev<SP_NED1, EV_TRACE3> : EvEdge1<-1, false> {};
+
+// Curved elements: an evaluator's element curls (b[0..8], n x grad lambda_k)
+// replaced by the ones at the current point (c9 in element vertex order;
+// PERM evaluators hold them in the Sauter-Schwab order perm).
+template <class EV>
+__device__ __forceinline__ void point_curls(const double* c9, const int* perm, double* b) {
+  if constexpr (EV::CURLS) {
+#pragma unroll
+    for (int k = 0; k < 3; k++) {
+      const int v = EV::PERM ? perm[k] : k;
+      b[3 * k] = c9[3 * v];
+      b[3 * k + 1] = c9[3 * v + 1];
+      b[3 * k + 2] = c9[3 * v + 2];
+    }
+  }
+}
+
+// Curved element map x(xi, eta) = sum A_ab xi^a eta^b (a + b <= k, degree
+// outer, b inner; A: 3 doubles per monomial): position, unit normal,
+// Jacobian and n x grad_Gamma lambda_0..2 = (t2, -t1, t1 - t2) / J with the
+// tangents t1 = dx/dxi, t2 = dx/deta.
+__device__ __forceinline__ void curved_point(const double* __restrict__ A, int k, double xi, double eta, P3& X,
+                                             P3& n, double& Jq, double* c9) {
+  double pxi[GEO_ORDER_MAX + 1], peta[GEO_ORDER_MAX + 1];
+  pxi[0] = peta[0] = 1.0;
+#pragma unroll
+  for (int i = 1; i <= GEO_ORDER_MAX; i++) {
+    pxi[i] = pxi[i - 1] * xi;
+    peta[i] = peta[i - 1] * eta;
+  }
+  double x[3] = {0, 0, 0}, t1[3] = {0, 0, 0}, t2[3] = {0, 0, 0};
+  int m = 0;
+  for (int d = 0; d <= k; d++)
+    for (int b = 0; b <= d; b++, m++) {
+      const int a = d - b;
+      const double v = pxi[a] * peta[b];
+      const double dx = a ? a * pxi[a - 1] * peta[b] : 0.0;
+      const double dy = b ? b * pxi[a] * peta[b - 1] : 0.0;
+#pragma unroll
+      for (int c = 0; c < 3; c++) {
+        const double ac = A[3 * m + c];
+        x[c] += ac * v;
+        t1[c] += ac * dx;
+        t2[c] += ac * dy;
+      }
+    }
+  const double cr0 = t1[1] * t2[2] - t1[2] * t2[1], cr1 = t1[2] * t2[0] - t1[0] * t2[2],
+               cr2 = t1[0] * t2[1] - t1[1] * t2[0];
+  Jq = sqrt(cr0 * cr0 + cr1 * cr1 + cr2 * cr2);
+  const double iJ = 1.0 / Jq;
+  X = P3{x[0], x[1], x[2]};
+  n = P3{cr0 * iJ, cr1 * iJ, cr2 * iJ};
+#pragma unroll
+  for (int c = 0; c < 3; c++) {
+    c9[c] = t2[c] * iJ;
+    c9[3 + c] = -t1[c] * iJ;
+    c9[6 + c] = (t1[c] - t2[c]) * iJ;
+  }
+}
 
 // elmat[ii][jj] += sum_t fac_t * f * K_{kc_t} * phi_i[ii,tec_t] * phi_j[jj,trc_t]
 template <class KER, class EI, class EJ>
@@ -526,6 +588,7 @@ __global__ __launch_bounds__(256) void k_near_regular(const NearBlk* __restrict_
                                                       const double* __restrict__ trig, int ntq,
                                                       const int* __restrict__ dofs_t, const int* __restrict__ dofs_r,
                                                       const int* __restrict__ ipt, const int* __restrict__ ipr, KER ker,
+                                                      const double* __restrict__ qg,
                                                       typename KER::T* __restrict__ val) {
   typedef typename KER::T T;
   const int2 ch = chunks[blockIdx.x];
@@ -551,20 +614,33 @@ __global__ __launch_bounds__(256) void k_near_regular(const NearBlk* __restrict_
     for (int b = 0; b < EJ::NL; b++) el[a][b] = zero_<T>();
   const double* qe = quad + (size_t)e * ntq * 4;
   const double* qf = quad + (size_t)f * ntq * 4;
+  // curved elements: normal and curls per point (qg), else per element
+  const double* ge = qg ? qg + (size_t)e * ntq * QG : nullptr;
+  const double* gf = qg ? qg + (size_t)f * ntq * QG : nullptr;
   for (int ix = 0; ix < ntq; ix++) {
     const P3 x = ld3(qe + 4 * ix);
     const double wx = qe[4 * ix + 3];
     double lx[3] = {trig[3 * ix], trig[3 * ix + 1], 1.0 - trig[3 * ix] - trig[3 * ix + 1]};
+    P3 nx = ne;
+    if (ge) {
+      nx = ld3(ge + QG * ix);
+      point_curls<EI>(ge + QG * ix + 3, idp, be);
+    }
     double si[EI::NL * EI::DIM];
     EI::shape(lx, x, be, si);
     for (int iy = 0; iy < ntq; iy++) {
       const P3 y = ld3(qf + 4 * iy);
       const double wy = qf[4 * iy + 3];
       double ly[3] = {trig[3 * iy], trig[3 * iy + 1], 1.0 - trig[3 * iy] - trig[3 * iy + 1]};
+      P3 ny = nf;
+      if (gf) {
+        ny = ld3(gf + QG * iy);
+        point_curls<EJ>(gf + QG * iy + 3, idp, bf);
+      }
       double sj[EJ::NL * EJ::DIM];
       EJ::shape(ly, y, bf, sj);
       T kv[KER::NC];
-      ker.eval(x, y, ne, nf, kv);
+      ker.eval(x, y, nx, ny, kv);
       accum<KER, EI, EJ>(ker, el, kv, wx * wy, si, sj);
     }
   }
@@ -601,6 +677,7 @@ __global__ __launch_bounds__(256) void k_near_singular(const int4* __restrict__ 
                                                        const double* __restrict__ sdat, const int* __restrict__ dofs_t,
                                                        const int* __restrict__ dofs_r, const int* __restrict__ ipt,
                                                        const int* __restrict__ ipr, KER ker,
+                                                       const double* __restrict__ gp, int gk,
                                                        typename KER::T* __restrict__ val) {
   typedef typename KER::T T;
   const int p = blockIdx.x * 256 + threadIdx.x;
@@ -634,10 +711,23 @@ __global__ __launch_bounds__(256) void k_near_singular(const int4* __restrict__ 
     const double x0 = r[0], x1 = r[1], y0 = r[2], y1 = r[3], w = r[4];
     const double lx[3] = {1.0 - x0 - x1, x0, x1};
     const double ly[3] = {1.0 - y0 - y1, y0, y1};
-    const P3 X = {lx[0] * E0.x + lx[1] * E1.x + lx[2] * E2.x, lx[0] * E0.y + lx[1] * E1.y + lx[2] * E2.y,
-                  lx[0] * E0.z + lx[1] * E1.z + lx[2] * E2.z};
-    const P3 Y = {ly[0] * F0.x + ly[1] * F1.x + ly[2] * F2.x, ly[0] * F0.y + ly[1] * F1.y + ly[2] * F2.y,
-                  ly[0] * F0.z + ly[1] * F1.z + ly[2] * F2.z};
+    P3 X = {lx[0] * E0.x + lx[1] * E1.x + lx[2] * E2.x, lx[0] * E0.y + lx[1] * E1.y + lx[2] * E2.y,
+            lx[0] * E0.z + lx[1] * E1.z + lx[2] * E2.z};
+    P3 Y = {ly[0] * F0.x + ly[1] * F1.x + ly[2] * F2.x, ly[0] * F0.y + ly[1] * F1.y + ly[2] * F2.y,
+            ly[0] * F0.z + ly[1] * F1.z + ly[2] * F2.z};
+    P3 nx = ne, ny = nf;
+    double wq = JJ * w;
+    if (gp) {  // curved: the map at the element-local barycentrics of the point
+      const int nm = (gk + 1) * (gk + 2) / 2;
+      double le[3], lf[3], cx[9], cy[9], jx, jy;
+      le[px[0]] = lx[0]; le[px[1]] = lx[1]; le[px[2]] = lx[2];
+      lf[py[0]] = ly[0]; lf[py[1]] = ly[1]; lf[py[2]] = ly[2];
+      curved_point(gp + (size_t)e * nm * 3, gk, le[0], le[1], X, nx, jx, cx);
+      curved_point(gp + (size_t)f * nm * 3, gk, lf[0], lf[1], Y, ny, jy, cy);
+      point_curls<EI>(cx, px, be);
+      point_curls<EJ>(cy, py, bf);
+      wq = jx * jy * w;
+    }
     double si[EI::NL * EI::DIM], sj[EJ::NL * EJ::DIM];
     if (EI::PERM) {
       EI::shape(lx, X, be, si);
@@ -654,8 +744,8 @@ __global__ __launch_bounds__(256) void k_near_singular(const int4* __restrict__ 
       EJ::shape(l, Y, bf, sj);
     }
     T kv[KER::NC];
-    ker.eval(X, Y, ne, nf, kv);
-    accum<KER, EI, EJ>(ker, el, kv, JJ * w, si, sj);
+    ker.eval(X, Y, nx, ny, kv);
+    accum<KER, EI, EJ>(ker, el, kv, wq, si, sj);
   }
   if (DIRECT) {
     const int row = ipt[dofs_t[DOF_STRIDE * e]] - nb.r0, col = ipr[dofs_r[DOF_STRIDE * f]] - nb.c0;
@@ -734,11 +824,12 @@ struct PointSet {
   const double* quad;
   const double* geo;
   int ntq;
+  const double* qg;  // curved elements: the normal per quadrature point
   __device__ __forceinline__ void get(int i, P3& x, P3& n) const {
     const int a = i / ntq, q = i - a * ntq;
     const int e = patch[a];
     x = ld3(quad + ((size_t)e * ntq + q) * 4);
-    n = ld3(geo + (size_t)e * GEO + 9);
+    n = qg ? ld3(qg + ((size_t)e * ntq + q) * QG) : ld3(geo + (size_t)e * GEO + 9);
   }
 };
 
@@ -756,8 +847,9 @@ __device__ __forceinline__ typename KER::T kcomp(const KER& ker, P3 x, P3 y, P3 
 template <class KER, int NTH>
 __global__ __launch_bounds__(NTH) void k_aca(AcaTask* __restrict__ tasks, const FarBlk* __restrict__ fb,
                                              const int* __restrict__ fpatch, const double* __restrict__ quad,
-                                             const double* __restrict__ geo, int ntq, KER ker, double eps,
-                                             typename KER::T* __restrict__ wsU, typename KER::T* __restrict__ wsV) {
+                                             const double* __restrict__ geo, int ntq, const double* __restrict__ qg,
+                                             KER ker, double eps, typename KER::T* __restrict__ wsU,
+                                             typename KER::T* __restrict__ wsV) {
   typedef typename KER::T T;
   constexpr int NW = NTH / 64;
   extern __shared__ unsigned char smem_raw[];
@@ -770,7 +862,7 @@ __global__ __launch_bounds__(NTH) void k_aca(AcaTask* __restrict__ tasks, const 
   AcaTask tk = tasks[blockIdx.x];
   const FarBlk b = fb[tk.blk];
   ker.kappa = geo[(size_t)fpatch[b.pI] * GEO + 13];  // one geometry per block
-  const PointSet X{fpatch + b.pI, quad, geo, ntq}, Y{fpatch + b.pJ, quad, geo, ntq};
+  const PointSet X{fpatch + b.pI, quad, geo, ntq, qg}, Y{fpatch + b.pJ, quad, geo, ntq, qg};
   const int np = tk.np, nq = tk.nq, comp = tk.comp;
   T* U = wsU + tk.uoff;
   T* V = wsV + tk.voff;
@@ -1002,7 +1094,8 @@ __global__ __launch_bounds__(NTH) void k_aca(AcaTask* __restrict__ tasks, const 
 template <class KER, int NTH>
 __global__ __launch_bounds__(NTH) void k_aca_gram(AcaTask* __restrict__ tasks, const FarBlk* __restrict__ fb,
                                                   const int* __restrict__ fpatch, const double* __restrict__ quad,
-                                                  const double* __restrict__ geo, int ntq, KER ker, double eps,
+                                                  const double* __restrict__ geo, int ntq,
+                                                  const double* __restrict__ qg, KER ker, double eps,
                                                   typename KER::T* __restrict__ wsU, typename KER::T* __restrict__ wsV,
                                                   typename KER::T* __restrict__ wsG, int capmax) {
   typedef typename KER::T T;
@@ -1019,7 +1112,7 @@ __global__ __launch_bounds__(NTH) void k_aca_gram(AcaTask* __restrict__ tasks, c
   AcaTask tk = tasks[blockIdx.x];
   const FarBlk b = fb[tk.blk];
   ker.kappa = geo[(size_t)fpatch[b.pI] * GEO + 13];  // one geometry per block
-  const PointSet X{fpatch + b.pI, quad, geo, ntq}, Y{fpatch + b.pJ, quad, geo, ntq};
+  const PointSet X{fpatch + b.pI, quad, geo, ntq, qg}, Y{fpatch + b.pJ, quad, geo, ntq, qg};
   const int np = tk.np, nq = tk.nq, comp = tk.comp, cap = tk.cap;
   T* U = wsU + tk.uoff;
   T* V = wsV + tk.voff;
@@ -1239,7 +1332,8 @@ __global__ __launch_bounds__(256) void k_project(const ProjTask* __restrict__ ta
                                                  const FarBlk* __restrict__ fb, const int* __restrict__ fpatch,
                                                  const int* __restrict__ maps, const double* __restrict__ quad,
                                                  const double* __restrict__ sdat, const double* __restrict__ trig,
-                                                 int ntq, KER ker, const typename KER::T* __restrict__ wsU,
+                                                 int ntq, const double* __restrict__ qg, KER ker,
+                                                 const typename KER::T* __restrict__ wsU,
                                                  const typename KER::T* __restrict__ wsV,
                                                  typename KER::T* __restrict__ oU, typename KER::T* __restrict__ oV) {
   typedef typename KER::T T;
@@ -1282,6 +1376,10 @@ __global__ __launch_bounds__(256) void k_project(const ProjTask* __restrict__ ta
         const double lam[3] = {trig[3 * q], trig[3 * q + 1], 1.0 - trig[3 * q] - trig[3 * q + 1]};
         const double* qp = quad + ((size_t)e * ntq + q) * 4;
         const P3 X = ld3(qp);
+        if (qg) {  // curved: the curls at the point
+          if (isU) point_curls<EI>(qg + ((size_t)e * ntq + q) * QG + 3, idp, bb);
+          else point_curls<EJ>(qg + ((size_t)e * ntq + q) * QG + 3, idp, bb);
+        }
         double sh;
         if (isU) {
           double v[EI::NL * EI::DIM];
@@ -1313,7 +1411,8 @@ __global__ __launch_bounds__(256) void k_project_dof(const ProjTask* __restrict_
                                                      const FarBlk* __restrict__ fb, const int* __restrict__ fpatch,
                                                      const int* __restrict__ maps, const double* __restrict__ quad,
                                                      const double* __restrict__ sdat, const double* __restrict__ trig,
-                                                     int ntq, KER ker, const typename KER::T* __restrict__ wsU,
+                                                     int ntq, const double* __restrict__ qg, KER ker,
+                                                     const typename KER::T* __restrict__ wsU,
                                                      const typename KER::T* __restrict__ wsV,
                                                      typename KER::T* __restrict__ oU, typename KER::T* __restrict__ oV) {
   typedef typename KER::T T;
@@ -1357,6 +1456,10 @@ __global__ __launch_bounds__(256) void k_project_dof(const ProjTask* __restrict_
           const double lam[3] = {trig[3 * q], trig[3 * q + 1], 1.0 - trig[3 * q] - trig[3 * q + 1]};
           const double* qp = quad + ((size_t)el[k2] * ntq + q) * 4;
           const P3 X = ld3(qp);
+          if (qg) {
+            if (isU) point_curls<EI>(qg + ((size_t)el[k2] * ntq + q) * QG + 3, idp, bb);
+            else point_curls<EJ>(qg + ((size_t)el[k2] * ntq + q) * QG + 3, idp, bb);
+          }
           double sh = 0.0;
           if (isU) {
             double v[EI::NL * EI::DIM];
@@ -2140,6 +2243,9 @@ void setup_device(Operator& op, hipStream_t s) {
   const int nt = m.nt, ntq = op.trig.size();
   std::vector<double> geo((size_t)nt * GEO, 0.0), quad((size_t)nt * ntq * 4), sdat((size_t)nt * SDS, 0.0);
   std::vector<int> vid((size_t)nt * 4, 0);
+  // curved elements: per quadrature point the unit normal and n x grad lambda_0..2
+  const bool curved = m.gorder > 1;
+  std::vector<double> qgeo(curved ? (size_t)nt * ntq * QG : 0);
   // per-element tables, elements split over host threads (disjoint slots)
   const Space& es = is_edge_kind(op.test->kind) ? *op.test : *op.trial;
   auto fill = [&](int e0, int e1) {
@@ -2155,10 +2261,18 @@ void setup_device(Operator& op, hipStream_t s) {
       geo[(size_t)e * GEO + 13] = op.kappa_el.empty() ? op.kappa : op.kappa_el[e];
       for (int q = 0; q < ntq; q++) {
         double x[3];
-        m.point(e, op.trig.xi[q], op.trig.eta[q], x);
         double* o = &quad[((size_t)e * ntq + q) * 4];
+        if (curved) {
+          double n[3], Jq;
+          double* g = &qgeo[((size_t)e * ntq + q) * QG];
+          m.geom(e, op.trig.xi[q], op.trig.eta[q], x, n, Jq, g + 3);
+          g[0] = n[0]; g[1] = n[1]; g[2] = n[2];
+          o[3] = Jq * op.trig.w[q];
+        } else {
+          m.point(e, op.trig.xi[q], op.trig.eta[q], x);
+          o[3] = m.J[e] * op.trig.w[q];
+        }
         o[0] = x[0]; o[1] = x[1]; o[2] = x[2];
-        o[3] = m.J[e] * op.trig.w[q];
       }
       double c[3][3];
       m.curls(e, c);
@@ -2209,6 +2323,14 @@ void setup_device(Operator& op, hipStream_t s) {
   op.d_sdat.upload(sdat, s);
   op.d_quad.upload(quad, s);
   op.d_trig.upload(trig, s);
+  if (curved) {
+    op.d_qgeo.upload(qgeo, s);
+    op.d_gpoly.upload(m.gpoly, s);
+  } else {
+    op.d_qgeo.release();
+    op.d_gpoly.release();
+  }
+  op.geo_order = m.gorder;
   for (int k = 0; k < 3; k++) {
     const Rule4& r = op.ss[k];
     std::vector<double> rr((size_t)r.size() * 5);
@@ -2365,7 +2487,7 @@ void assemble_near(Operator& op, hipStream_t s) {
                            op.d_near_blk.as<NearBlk>(), op.d_near_chunks.as<int2>(), pIb, pJb, op.d_geo.as<double>(),
                            op.d_vid.as<int4>(), op.d_quad.as<double>(), op.d_sdat.as<double>(),
                            op.d_trig.as<double>(), ntq, op.d_dofs_test.as<int>(), op.d_dofs_trial.as<int>(),
-                           op.d_iperm_test.as<int>(), op.d_iperm_trial.as<int>(), ker, val);
+                           op.d_iperm_test.as<int>(), op.d_iperm_trial.as<int>(), ker, op.d_qgeo.as<double>(), val);
       BEM_HIP(hipEventRecord(tev[1], s));
       for (int c = 0; c < 3; c++) {
         const int np = (int)pairs[c].size();
@@ -2374,7 +2496,7 @@ void assemble_near(Operator& op, hipStream_t s) {
                            op.d_spairs[c].as<int4>(), np, op.d_near_blk.as<NearBlk>(), op.d_ss[c].as<double>(),
                            op.ss[c].size(), op.d_geo.as<double>(), op.d_sdat.as<double>(),
                            op.d_dofs_test.as<int>(), op.d_dofs_trial.as<int>(), op.d_iperm_test.as<int>(),
-                           op.d_iperm_trial.as<int>(), ker, val);
+                           op.d_iperm_trial.as<int>(), ker, op.d_gpoly.as<double>(), op.geo_order, val);
       }
     };
     pt_.mark("uploads + alloc");
@@ -2524,19 +2646,19 @@ void assemble_far(Operator& op, hipStream_t s) {
         if (big)
           hipLaunchKernelGGL((k_aca_gram<KER, 512>), dim3(count), dim3(512), sm, st, dtasks,
                              op.d_far_blk.as<FarBlk>(), op.d_patch.as<int>(), op.d_quad.as<double>(),
-                             op.d_geo.as<double>(), ntq, ker, op.eps, U.as<T>(), V.as<T>(), G->as<T>(), maxcap);
+                             op.d_geo.as<double>(), ntq, op.d_qgeo.as<double>(), ker, op.eps, U.as<T>(), V.as<T>(), G->as<T>(), maxcap);
         else
           hipLaunchKernelGGL((k_aca_gram<KER, 256>), dim3(count), dim3(256), sm, st, dtasks,
                              op.d_far_blk.as<FarBlk>(), op.d_patch.as<int>(), op.d_quad.as<double>(),
-                             op.d_geo.as<double>(), ntq, ker, op.eps, U.as<T>(), V.as<T>(), G->as<T>(), maxcap);
+                             op.d_geo.as<double>(), ntq, op.d_qgeo.as<double>(), ker, op.eps, U.as<T>(), V.as<T>(), G->as<T>(), maxcap);
       } else if (big)
         hipLaunchKernelGGL((k_aca<KER, 512>), dim3(count), dim3(512), maxcap * sizeof(T), st, dtasks,
                            op.d_far_blk.as<FarBlk>(), op.d_patch.as<int>(), op.d_quad.as<double>(),
-                           op.d_geo.as<double>(), ntq, ker, op.eps, U.as<T>(), V.as<T>());
+                           op.d_geo.as<double>(), ntq, op.d_qgeo.as<double>(), ker, op.eps, U.as<T>(), V.as<T>());
       else
         hipLaunchKernelGGL((k_aca<KER, 256>), dim3(count), dim3(256), maxcap * sizeof(T), st, dtasks,
                            op.d_far_blk.as<FarBlk>(), op.d_patch.as<int>(), op.d_quad.as<double>(),
-                           op.d_geo.as<double>(), ntq, ker, op.eps, U.as<T>(), V.as<T>());
+                           op.d_geo.as<double>(), ntq, op.d_qgeo.as<double>(), ker, op.eps, U.as<T>(), V.as<T>());
     });
     BEM_HIP(hipGetLastError());
   };
@@ -2792,7 +2914,7 @@ void assemble_far(Operator& op, hipStream_t s) {
             hipLaunchKernelGGL((k_project_dof<KER, decltype(ei), decltype(ej)>), dim3((unsigned)it4.size()),
                                dim3(256), 0, s, d_ptasks.as<ProjTask>(), d_items.as<int4>(), op.d_far_blk.as<FarBlk>(),
                                op.d_patch.as<int>(), op.d_far_maps.as<int>(), op.d_quad.as<double>(),
-                               op.d_sdat.as<double>(), op.d_trig.as<double>(), ntq, ker, SU.as<T>(), SV.as<T>(),
+                               op.d_sdat.as<double>(), op.d_trig.as<double>(), ntq, op.d_qgeo.as<double>(), ker, SU.as<T>(), SV.as<T>(),
                                OU.as<T>(), OV.as<T>());
           });
         items.clear();
@@ -2806,7 +2928,7 @@ void assemble_far(Operator& op, hipStream_t s) {
           hipLaunchKernelGGL((k_project<KER, decltype(ei), decltype(ej)>), dim3((unsigned)items.size()), dim3(256),
                              0, s, d_ptasks.as<ProjTask>(), d_items.as<int2>(), op.d_far_blk.as<FarBlk>(),
                              op.d_patch.as<int>(), op.d_far_maps.as<int>(), op.d_quad.as<double>(),
-                             op.d_sdat.as<double>(), op.d_trig.as<double>(), ntq, ker, SU.as<T>(), SV.as<T>(),
+                             op.d_sdat.as<double>(), op.d_trig.as<double>(), ntq, op.d_qgeo.as<double>(), ker, SU.as<T>(), SV.as<T>(),
                              OU.as<T>(), OV.as<T>());
         });
       BEM_HIP(hipGetLastError());
@@ -3270,7 +3392,7 @@ __global__ __launch_bounds__(256) void k_pot_src(const int* __restrict__ elist, 
                                                  const double* __restrict__ quad, const double* __restrict__ geo,
                                                  const double* __restrict__ sdat, const double* __restrict__ trig,
                                                  const int* __restrict__ dofs, const typename KER::T* __restrict__ coef,
-                                                 const double* __restrict__ dscale,
+                                                 const double* __restrict__ dscale, const double* __restrict__ qg,
                                                  PotSrc<typename KER::T, EJ::DIM>* __restrict__ src) {
   typedef typename KER::T T;
   const int idx = blockIdx.x * 256 + threadIdx.x;
@@ -3282,12 +3404,14 @@ __global__ __launch_bounds__(256) void k_pot_src(const int* __restrict__ elist, 
   const int idp[3] = {0, 1, 2};
   double bb[EJ::NB > 0 ? EJ::NB : 1];
   EJ::load(sdat + (size_t)e * SDS, idp, bb);
+  const double* gq = qg ? qg + ((size_t)e * ntq + q) * QG : nullptr;  // curved: per-point geometry
+  if (gq) point_curls<EJ>(gq + 3, idp, bb);
   double v[EJ::NL * EJ::DIM];
   EJ::shape(lam, ld3(qp), bb, v);
   PotSrc<T, EJ::DIM> o;
   for (int k = 0; k < 3; k++) {
     o.y[k] = qp[k];
-    o.ny[k] = geo[(size_t)e * GEO + 9 + k];
+    o.ny[k] = gq ? gq[k] : geo[(size_t)e * GEO + 9 + k];
   }
   o.w = qp[3];
   o.pad = 0.0;
@@ -3453,7 +3577,8 @@ static void potential_run(Operator& op, const void* coef, const double* pts, con
     const int chunk = (nsrc + nsplit - 1) / nsplit;
     hipLaunchKernelGGL((k_pot_src<KER, EJ>), dim3((nsrc + 255) / 256), dim3(256), 0, s, el, ne, ntq,
                        op.d_quad.as<double>(), op.d_geo.as<double>(), op.d_sdat.as<double>(), op.d_trig.as<double>(),
-                       op.d_dofs_trial.as<int>(), (const T*)coef, op.d_dscale_trial.as<double>(), sp);
+                       op.d_dofs_trial.as<int>(), (const T*)coef, op.d_dscale_trial.as<double>(),
+                       op.d_qgeo.as<double>(), sp);
     hipLaunchKernelGGL((k_potential<KER, D>), dim3(gx, nsplit), dim3(256), 0, s, sp, nsrc, chunk, pts, nrm, np, ker,
                        pp);
     if (pp != (T*)out)

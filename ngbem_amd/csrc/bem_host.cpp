@@ -11,6 +11,7 @@
 #include <thread>
 #include <cmath>
 #include <cstring>
+#include <stdexcept>
 
 namespace bem {
 
@@ -40,7 +41,129 @@ void Mesh::setup() {
     for (int k = 0; k < 3; k++) ve_list[fill[tri[3 * e + k]]++] = e;
 }
 
+// polynomial map of a curved element: position and the tangents d/dxi, d/deta
+static void poly_map(const double* A, int k, double xi, double eta, double* x, double* t1, double* t2) {
+  double pxi[GEO_ORDER_MAX + 1], peta[GEO_ORDER_MAX + 1];
+  pxi[0] = peta[0] = 1.0;
+  for (int i = 1; i <= k; i++) {
+    pxi[i] = pxi[i - 1] * xi;
+    peta[i] = peta[i - 1] * eta;
+  }
+  for (int c = 0; c < 3; c++) x[c] = t1[c] = t2[c] = 0.0;
+  int m = 0;
+  for (int d = 0; d <= k; d++)
+    for (int b = 0; b <= d; b++, m++) {
+      const int a = d - b;
+      const double v = pxi[a] * peta[b];
+      const double dx = a ? a * pxi[a - 1] * peta[b] : 0.0;
+      const double dy = b ? b * pxi[a] * peta[b - 1] : 0.0;
+      for (int c = 0; c < 3; c++) {
+        x[c] += A[3 * m + c] * v;
+        t1[c] += A[3 * m + c] * dx;
+        t2[c] += A[3 * m + c] * dy;
+      }
+    }
+}
+
+void Mesh::set_curved(int k, const double* nodes) {
+  if (k < 1 || k > GEO_ORDER_MAX) throw std::runtime_error("geometry order must be 1 to 4");
+  const int nm = geo_monomials(k);
+  // Vandermonde matrix of the nodes (i / k, j / k), inverted once
+  std::vector<double> V((size_t)nm * nm), Vi((size_t)nm * nm, 0.0);
+  {
+    int r = 0;
+    for (int j = 0; j <= k; j++)
+      for (int i = 0; i + j <= k; i++, r++) {
+        const double xi = (double)i / k, eta = (double)j / k;
+        int m = 0;
+        for (int d = 0; d <= k; d++)
+          for (int b = 0; b <= d; b++, m++) V[(size_t)r * nm + m] = std::pow(xi, d - b) * std::pow(eta, b);
+      }
+    for (int i = 0; i < nm; i++) Vi[(size_t)i * nm + i] = 1.0;
+    // Gauss-Jordan with partial pivoting on [V | I]
+    for (int c = 0; c < nm; c++) {
+      int piv = c;
+      for (int r2 = c + 1; r2 < nm; r2++)
+        if (std::fabs(V[(size_t)r2 * nm + c]) > std::fabs(V[(size_t)piv * nm + c])) piv = r2;
+      for (int q = 0; q < nm; q++) {
+        std::swap(V[(size_t)c * nm + q], V[(size_t)piv * nm + q]);
+        std::swap(Vi[(size_t)c * nm + q], Vi[(size_t)piv * nm + q]);
+      }
+      const double d = V[(size_t)c * nm + c];
+      for (int q = 0; q < nm; q++) {
+        V[(size_t)c * nm + q] /= d;
+        Vi[(size_t)c * nm + q] /= d;
+      }
+      for (int r2 = 0; r2 < nm; r2++) {
+        if (r2 == c) continue;
+        const double f = V[(size_t)r2 * nm + c];
+        if (f == 0.0) continue;
+        for (int q = 0; q < nm; q++) {
+          V[(size_t)r2 * nm + q] -= f * V[(size_t)c * nm + q];
+          Vi[(size_t)r2 * nm + q] -= f * Vi[(size_t)c * nm + q];
+        }
+      }
+    }
+  }
+  gorder = k;
+  gpoly.assign((size_t)nt * nm * 3, 0.0);
+  for (int e = 0; e < nt; e++)
+    for (int m = 0; m < nm; m++)
+      for (int r = 0; r < nm; r++)
+        for (int c = 0; c < 3; c++)
+          gpoly[((size_t)e * nm + m) * 3 + c] += Vi[(size_t)m * nm + r] * nodes[((size_t)e * nm + r) * 3 + c];
+  // element measure (twice the area, by a collapsed Gauss rule exact for the
+  // polynomial part) and the normal at the centroid
+  std::vector<double> g, gw;
+  gauss01(k + 3, g, gw);
+  for (int e = 0; e < nt; e++) {
+    double area = 0.0, x[3], n[3], Jq;
+    for (size_t a = 0; a < g.size(); a++)
+      for (size_t b = 0; b < g.size(); b++) {
+        const double xi = g[a], eta = g[b] * (1.0 - g[a]);
+        geom(e, xi, eta, x, n, Jq, nullptr);
+        area += gw[a] * gw[b] * (1.0 - g[a]) * Jq;
+      }
+    J[e] = 2.0 * area;
+    geom(e, 1.0 / 3.0, 1.0 / 3.0, x, n, Jq, nullptr);
+    for (int c = 0; c < 3; c++) nrm[3 * e + c] = n[c];
+  }
+}
+
+void Mesh::geom(int e, double xi, double eta, double* x, double* n, double& Jq, double* c9) const {
+  if (gorder <= 1) {
+    point(e, xi, eta, x);
+    for (int c = 0; c < 3; c++) n[c] = nrm[3 * e + c];
+    Jq = J[e];
+    if (c9) {
+      double c[3][3];
+      curls(e, c);
+      for (int k = 0; k < 3; k++)
+        for (int d = 0; d < 3; d++) c9[3 * k + d] = c[k][d];
+    }
+    return;
+  }
+  const int nm = geo_monomials(gorder);
+  double t1[3], t2[3];
+  poly_map(&gpoly[(size_t)e * nm * 3], gorder, xi, eta, x, t1, t2);
+  const double cr[3] = {t1[1] * t2[2] - t1[2] * t2[1], t1[2] * t2[0] - t1[0] * t2[2], t1[0] * t2[1] - t1[1] * t2[0]};
+  Jq = std::sqrt(cr[0] * cr[0] + cr[1] * cr[1] + cr[2] * cr[2]);
+  for (int c = 0; c < 3; c++) n[c] = cr[c] / Jq;
+  if (c9)  // n x grad_Gamma f = (f_xi t2 - f_eta t1) / J for f = lambda_0 = xi, lambda_1 = eta, lambda_2
+    for (int c = 0; c < 3; c++) {
+      c9[c] = t2[c] / Jq;
+      c9[3 + c] = -t1[c] / Jq;
+      c9[6 + c] = (t1[c] - t2[c]) / Jq;
+    }
+}
+
 void Mesh::point(int e, double xi, double eta, double* x) const {
+  if (gorder > 1) {
+    double t1[3], t2[3];
+    const int nm = geo_monomials(gorder);
+    poly_map(&gpoly[(size_t)e * nm * 3], gorder, xi, eta, x, t1, t2);
+    return;
+  }
   const double* p0 = &xyz[3 * tri[3 * e + 0]];
   const double* p1 = &xyz[3 * tri[3 * e + 1]];
   const double* p2 = &xyz[3 * tri[3 * e + 2]];

@@ -48,15 +48,32 @@ enum EvalKind { EV_ID = 0, EV_ROT = 1, EV_HELM = 2, EV_MAXW = 3, EV_TRACE3 = 4 }
 // K_HSLK: d/dkappa of the Helmholtz single layer kernel (no reference counterpart)
 enum KernelId { K_LSL = 0, K_LDL, K_LHS, K_HSL, K_HDL, K_HCF, K_HHS, K_MSL, K_MDL, K_HSLK, K_COUNT };
 
+// Curved (isoparametric) triangles of order k: the map of element e is the
+// polynomial x(xi, eta) = sum over a + b <= k of A_ab xi^a eta^b (monomials in
+// order (0,0), (1,0), (0,1), (2,0), (1,1), (0,2), ...), interpolating the
+// element's Lagrange nodes at (xi, eta) = (i / k, j / k), i + j <= k (listed j
+// outer, i inner); lambda_0 = xi, lambda_1 = eta (vertex 0 at xi = 1, vertex 1
+// at eta = 1, vertex 2 at the origin), as the flat map xi p0 + eta p1 +
+// (1 - xi - eta) p2, which is the case k = 1.
+inline int geo_monomials(int k) { return (k + 1) * (k + 2) / 2; }
+constexpr int GEO_ORDER_MAX = 4;
+
 struct Mesh {
   int nv = 0, nt = 0;
   std::vector<double> xyz;                 // 3*nv
   std::vector<int> tri;                    // 3*nt
-  std::vector<double> nrm, J;              // 3*nt, nt
+  std::vector<double> nrm, J;              // 3*nt, nt (curved: normal at the centroid, twice the area)
   std::vector<int> ve_off, ve_list;        // vertex -> elements (CSR)
+  int gorder = 1;                          // geometry order (1: flat triangles)
+  std::vector<double> gpoly;               // curved: nt x geo_monomials(gorder) x 3 map coefficients
   void setup();
+  // curved geometry of order k from the elements' Lagrange nodes (nt x (k+1)(k+2)/2 x 3)
+  void set_curved(int k, const double* nodes);
   void point(int e, double xi, double eta, double* x) const;
-  void curls(int e, double c[3][3]) const; // n x grad(lambda_k)
+  void curls(int e, double c[3][3]) const; // n x grad(lambda_k), flat element
+  // position, unit normal, Jacobian and (if c9) n x grad_Gamma lambda_0..2 at
+  // (xi, eta) of element e, flat or curved
+  void geom(int e, double xi, double eta, double* x, double* n, double& Jq, double* c9) const;
 };
 
 struct Space {

@@ -237,6 +237,10 @@ struct Operator {
   DBuf d_geo, d_vid, d_sdat, d_quad, d_trig, d_ss[3];
   DBuf d_dofs_test, d_dofs_trial;  // int4 per element: the element's dofs
   DBuf d_dscale_trial;             // dual-mapped trial side: factor per dof (else empty)
+  // curved elements: per quadrature point normal and curls (QG doubles), and
+  // the elements' map coefficients; empty for flat meshes
+  DBuf d_qgeo, d_gpoly;
+  int geo_order = 1;
   DBuf d_perm_test, d_iperm_test, d_perm_trial, d_iperm_trial;
   DBuf d_patch;            // near patches of P1 sides (test then trial)
   DBuf d_near_blk, d_near_chunks, d_near_val, d_near_val32;

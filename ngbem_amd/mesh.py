@@ -72,6 +72,64 @@ class Mesh:
         if self.regions.shape != (self.nt,):
             raise ValueError("one region number per triangle")
         self.region_names = dict(region_names or {})
+        self.geo_order = 1      # 1: flat triangles; 2-4 after Curve(order)
+        self.nodes = None       # curved: nt x (k+1)(k+2)/2 x 3 Lagrange nodes
+        self.project = None     # point -> nearest surface point (set by the generators)
+        self._gpoly = None
+
+    def Curve(self, order, project=None):
+        """Curved (isoparametric) triangles of ``order`` (NGSolve's
+        mesh.Curve, as the reference's tests mesh): each element's Lagrange
+        nodes at (xi, eta) = (i / k, j / k) are the points of the flat triangle
+        moved onto the surface by ``project`` (default: the generator's, e.g.
+        the sphere for ``icosphere``; without one the nodes stay on the flat
+        triangles).  Returns the mesh; call before creating spaces on it."""
+        order = int(order)
+        if getattr(self, "_bem_handle", None) is not None:
+            raise RuntimeError("Curve must come before spaces are created on the mesh")
+        if not 1 <= order <= 4:
+            raise ValueError("geometry order 1 to 4")
+        if order == 1:
+            self.geo_order, self.nodes, self._gpoly = 1, None, None
+            return self
+        proj = project or self.project
+        ref = lagrange_nodes(order)
+        p0, p1, p2 = (self.xyz[self.tri[:, k]] for k in range(3))
+        X = (ref[None, :, 0, None] * p0[:, None] + ref[None, :, 1, None] * p1[:, None]
+             + (1.0 - ref[None, :, 0, None] - ref[None, :, 1, None]) * p2[:, None])
+        if proj is not None:
+            X = np.asarray(proj(X.reshape(-1, 3)), dtype=np.float64).reshape(X.shape)
+        # the element's vertices are its corner nodes exactly
+        k = order
+        corner = {(k, 0): 0, (0, k): 1, (0, 0): 2}
+        idx = 0
+        for j in range(k + 1):
+            for i in range(k + 1 - j):
+                if (i, j) in corner:
+                    X[:, idx] = self.xyz[self.tri[:, corner[(i, j)]]]
+                idx += 1
+        self.geo_order, self.nodes = order, np.ascontiguousarray(X)
+        self._gpoly = np.einsum("mr,erc->emc", np.linalg.inv(_vandermonde(order, ref)), X)
+        return self
+
+    def geom(self, xi, eta):
+        """Positions, unit normals and Jacobians at the reference points (xi,
+        eta) of every element: (nt, q, 3), (nt, q, 3), (nt, q)."""
+        xi, eta = np.atleast_1d(np.asarray(xi, float)), np.atleast_1d(np.asarray(eta, float))
+        if self.geo_order == 1:
+            p0, p1, p2 = (self.xyz[self.tri[:, k]] for k in range(3))
+            t1, t2 = (p0 - p2)[:, None, :], (p1 - p2)[:, None, :]
+            X = p2[:, None, :] + xi[None, :, None] * t1 + eta[None, :, None] * t2
+            t1 = np.broadcast_to(t1, X.shape)
+            t2 = np.broadcast_to(t2, X.shape)
+        else:
+            v, dx, dy = _monomials(self.geo_order, xi, eta)
+            X = np.einsum("qm,emc->eqc", v, self._gpoly)
+            t1 = np.einsum("qm,emc->eqc", dx, self._gpoly)
+            t2 = np.einsum("qm,emc->eqc", dy, self._gpoly)
+        c = np.cross(t1, t2)
+        J = np.linalg.norm(c, axis=-1)
+        return X, c / J[..., None], J
 
     def region_name(self, r) -> str:
         return self.region_names.get(int(r), "default")
@@ -102,13 +160,50 @@ class Mesh:
         return np.unique(e, axis=0)
 
     def areas(self):
+        if self.geo_order > 1:  # collapsed Gauss rule, exact for the polynomial part
+            xi, eta, w = _collapsed_rule(self.geo_order + 3)
+            return self.geom(xi, eta)[2] @ w
         p0, p1, p2 = (self.xyz[self.tri[:, k]] for k in range(3))
         return 0.5 * np.linalg.norm(np.cross(p0 - p2, p1 - p2), axis=1)
 
     def normals(self):
+        """Unit normals (curved elements: at the centroid)."""
+        if self.geo_order > 1:
+            return self.geom(1.0 / 3.0, 1.0 / 3.0)[1][:, 0]
         p0, p1, p2 = (self.xyz[self.tri[:, k]] for k in range(3))
         c = np.cross(p0 - p2, p1 - p2)
         return c / np.linalg.norm(c, axis=1)[:, None]
+
+
+def lagrange_nodes(k):
+    """Reference coordinates (xi, eta) = (i / k, j / k), i + j <= k, j outer."""
+    return np.array([(i / k, j / k) for j in range(k + 1) for i in range(k + 1 - j)], dtype=np.float64)
+
+
+def _monomials(k, xi, eta):
+    """xi^a eta^b (a + b <= k, degree outer, b inner) and their xi / eta
+    derivatives at the points: (q, m) each."""
+    v, dx, dy = [], [], []
+    for d in range(k + 1):
+        for b in range(d + 1):
+            a = d - b
+            v.append(xi ** a * eta ** b)
+            dx.append(a * xi ** max(a - 1, 0) * eta ** b if a else np.zeros_like(xi))
+            dy.append(b * xi ** a * eta ** max(b - 1, 0) if b else np.zeros_like(xi))
+    return np.stack(v, 1), np.stack(dx, 1), np.stack(dy, 1)
+
+
+def _vandermonde(k, ref):
+    return _monomials(k, ref[:, 0], ref[:, 1])[0]
+
+
+def _collapsed_rule(n):
+    """Collapsed Gauss rule of n x n points on the reference triangle (weights sum to 1/2)."""
+    g, gw = np.polynomial.legendre.leggauss(n)
+    g, gw = 0.5 * (g + 1.0), 0.5 * gw
+    U, V = np.meshgrid(g, g, indexing="ij")
+    W = (gw[:, None] * gw[None, :]) * (1.0 - U)
+    return U.ravel(), (V * (1.0 - U)).ravel(), W.ravel()
 
 
 def _icosahedron():
@@ -159,13 +254,19 @@ def icosphere(level: int, radius: float = 1.0) -> Mesh:
     s = np.einsum("ij,ij->i", np.cross(p0 - p2, p1 - p2), p0 + p1 + p2)
     flip = s < 0
     tri[flip] = tri[flip][:, [1, 0, 2]]
-    return Mesh(xyz, tri)
+    m = Mesh(xyz, tri)
+    m.project = lambda P: radius * P / np.linalg.norm(P, axis=-1, keepdims=True)
+    return m
 
 
 def ellipsoid(level: int, axes=(1.0, 1.0, 1.0)) -> Mesh:
-    """Icosphere of ``level`` scaled by the semi-axes (orientation preserved)."""
+    """Icosphere of ``level`` scaled by the semi-axes (orientation preserved);
+    Curve projects radially in the sphere's coordinates."""
     m = icosphere(level)
-    return Mesh(m.xyz * np.asarray(axes, dtype=np.float64)[None, :], m.tri)
+    ax = np.asarray(axes, dtype=np.float64)
+    e = Mesh(m.xyz * ax[None, :], m.tri)
+    e.project = lambda P: ax * (P / ax) / np.linalg.norm(P / ax, axis=-1, keepdims=True)
+    return e
 
 
 def box(n: int, size=(1.0, 1.0, 1.0), centre=(0.0, 0.0, 0.0)) -> Mesh:

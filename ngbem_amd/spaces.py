@@ -20,7 +20,7 @@ import ctypes
 import numpy as np
 
 from . import _capi
-from .mesh import Mesh, Region
+from .mesh import Mesh, Region, _collapsed_rule
 
 
 def _mesh_handle(mesh: Mesh):
@@ -31,6 +31,8 @@ def _mesh_handle(mesh: Mesh):
         _capi.check(L.bem_mesh_create(_capi.dptr(mesh.xyz), mesh.nv, _capi.iptr(mesh.tri), mesh.nt,
                                       ctypes.byref(out)), "bem_mesh_create")
         h = _Handle(out, L.bem_mesh_destroy)
+        if getattr(mesh, "geo_order", 1) > 1:
+            _capi.check(L.bem_mesh_set_curved(out, mesh.geo_order, _capi.dptr(mesh.nodes)), "bem_mesh_set_curved")
         mesh._bem_handle = h
     return h
 
@@ -54,6 +56,10 @@ class FESpace:
                  dual_mapping: bool = False):
         if dual_mapping and not isinstance(self, SurfaceL2):
             raise NotImplementedError("dual_mapping is a SurfaceL2 option")
+        if dual_mapping and getattr(mesh, "geo_order", 1) > 1:
+            raise NotImplementedError("dual_mapping on curved meshes (its 1 / J varies over the element)")
+        if isinstance(self, _EdgeSpace) and getattr(mesh, "geo_order", 1) > 1:
+            raise NotImplementedError(f"{type(self).__name__} on curved meshes")
         if order not in self.orders:
             raise NotImplementedError(f"{type(self).__name__} supports orders {list(self.orders)}")
         self.order = int(order)
@@ -308,7 +314,15 @@ def mass_matrix_coo(trial: FESpace, test: FESpace):
     """Entries of the surface mass matrix  int u . v ds  (test x trial), exact for
     P0/P1 and the lowest-order edge spaces on flat triangles: rows, cols, values."""
     m = trial.mesh
-    if isinstance(trial, _EdgeSpace) or isinstance(test, _EdgeSpace):
+    if getattr(m, "geo_order", 1) > 1:
+        # curved elements: the Jacobian at each point of a collapsed Gauss rule
+        # of 8 x 8 points (exact to degree 15 for the polynomial part)
+        x, y, W = _collapsed_rule(8)
+        bary = np.stack([x, y, 1.0 - x - y], axis=1)
+        J = m.geom(x, y)[2]
+        loc = np.einsum("q,tq,qi,qj->tij", W, J, test.shapes(bary), trial.shapes(bary))
+        loc = loc * test.elem_signs()[:, :, None] * trial.elem_signs()[:, None, :]
+    elif isinstance(trial, _EdgeSpace) or isinstance(test, _EdgeSpace):
         if not (isinstance(trial, _EdgeSpace) and isinstance(test, _EdgeSpace)):
             raise NotImplementedError("mass matrix between an edge space and a scalar space")
         # edge-midpoint rule, exact for the quadratic integrands (orders 0-1)

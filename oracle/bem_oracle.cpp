@@ -31,7 +31,10 @@
 //
 // Element geometry follows NGSolve's flat triangle map:
 //   F(xi,eta) = xi*p0 + eta*p1 + (1-xi-eta)*p2,  n = (p0-p2)x(p1-p2)/|.|,
-//   measure J = |(p0-p2)x(p1-p2)| (= 2*area).
+//   measure J = |(p0-p2)x(p1-p2)| (= 2*area);
+// curved elements (mesh.Curve(k), k = 2..4): the Lagrange interpolant of the
+// element's nodes, with the normal, measure and surface gradients taken at
+// each quadrature point (Mesh::at).
 // Compile with -ffp-contract=off: the cluster tree of a symmetric mesh is
 // decided by rounding noise, so the product and the oracle must round alike.
 
@@ -263,13 +266,108 @@ struct Mesh {
     }
   }
   void point(int e, double xi, double eta, double* x) const {
+    if (gk > 1) {
+      double a[3], b[3];
+      curved_map(e, xi, eta, x, a, b);
+      return;
+    }
     const double* p0 = &xyz[3 * tri[3 * e + 0]];
     const double* p1 = &xyz[3 * tri[3 * e + 1]];
     const double* p2 = &xyz[3 * tri[3 * e + 2]];
     double l2 = 1.0 - xi - eta;
     for (int k = 0; k < 3; k++) x[k] = xi * p0[k] + eta * p1[k] + l2 * p2[k];
   }
+  // Curved elements (NGSolve's mesh.Curve(k); the reference's quadrature runs
+  // through the curved ElementTransformation, src/ngbem.cpp:685-692,
+  // :1086-1091): the map of element e is the Lagrange interpolant of its
+  // nodes at (xi, eta) = (i / k, j / k), i + j <= k (j outer, i inner),
+  //   F(xi, eta) = sum_r phi_r(xi, eta) node_r,
+  //   phi_(i,j) = f_i(lam0) f_j(lam1) f_l(lam2), l = k - i - j,
+  //   f_i(t) = prod_{a < i} (k t - a) / (a + 1),
+  // lam0 = xi, lam1 = eta, lam2 = 1 - xi - eta; normal (F_xi x F_eta) / |.|,
+  // measure |F_xi x F_eta|, surface gradients from the metric of the two
+  // tangents.  k = 1 is the flat map above.
+  int gk = 1;
+  std::vector<double> gnodes;  // nt x (gk+1)(gk+2)/2 x 3
+  static void lag_factor(int i, int k, double t, double& f, double& df) {
+    f = 1.0;
+    df = 0.0;
+    for (int a = 0; a < i; a++) {
+      const double c = (k * t - a) / (a + 1.0), dc = k / (a + 1.0);
+      df = df * c + f * dc;
+      f *= c;
+    }
+  }
+  void curved_map(int e, double xi, double eta, double X[3], double a[3], double b[3]) const {
+    const int k = gk, nm = (k + 1) * (k + 2) / 2;
+    const double* nd = &gnodes[(size_t)e * nm * 3];
+    const double l0 = xi, l1 = eta, l2 = 1.0 - xi - eta;
+    for (int c = 0; c < 3; c++) X[c] = a[c] = b[c] = 0.0;
+    int r = 0;
+    for (int j = 0; j <= k; j++)
+      for (int i = 0; i + j <= k; i++, r++) {
+        double fi, dfi, fj, dfj, fl, dfl;
+        lag_factor(i, k, l0, fi, dfi);
+        lag_factor(j, k, l1, fj, dfj);
+        lag_factor(k - i - j, k, l2, fl, dfl);
+        const double v = fi * fj * fl, dxi = dfi * fj * fl - fi * fj * dfl, deta = fi * dfj * fl - fi * fj * dfl;
+        for (int c = 0; c < 3; c++) {
+          X[c] += v * nd[3 * r + c];
+          a[c] += dxi * nd[3 * r + c];
+          b[c] += deta * nd[3 * r + c];
+        }
+      }
+  }
+  void set_curved(int k, const double* nodes) {
+    if (k < 2 || k > 4) throw std::runtime_error("oracle: geometry order 2 to 4");
+    gk = k;
+    gnodes.assign(nodes, nodes + (size_t)nt * (k + 1) * (k + 2) / 2 * 3);
+    // twice the area by the collapsed Gauss-Legendre rule of 2k + 4 points a side
+    std::vector<double> g, gw;
+    gauss01(2 * k + 4, g, gw);
+    for (int e = 0; e < nt; e++) {
+      double area = 0.0, X[3], n[3], Jq, gr[3][3];
+      for (size_t p = 0; p < g.size(); p++)
+        for (size_t q = 0; q < g.size(); q++) {
+          at(e, g[p], g[q] * (1.0 - g[p]), X, n, Jq, gr);
+          area += gw[p] * gw[q] * (1.0 - g[p]) * Jq;
+        }
+      J[e] = 2.0 * area;
+      at(e, 1.0 / 3.0, 1.0 / 3.0, X, n, Jq, gr);
+      for (int c = 0; c < 3; c++) nrm[3 * e + c] = n[c];
+    }
+  }
+  // metric-inverse surface gradients of lam0, lam1, lam2 for the tangents a, b
+  static void metric_grads(const double* a, const double* b, double g[3][3]) {
+    double aa = a[0] * a[0] + a[1] * a[1] + a[2] * a[2];
+    double ab = a[0] * b[0] + a[1] * b[1] + a[2] * b[2];
+    double bb = b[0] * b[0] + b[1] * b[1] + b[2] * b[2];
+    double det = aa * bb - ab * ab;
+    double i00 = bb / det, i01 = -ab / det, i11 = aa / det;
+    for (int k = 0; k < 3; k++) {
+      g[0][k] = i00 * a[k] + i01 * b[k];
+      g[1][k] = i01 * a[k] + i11 * b[k];
+      g[2][k] = -g[0][k] - g[1][k];
+    }
+  }
+  // position, unit normal, measure and surface gradients of lam0..2 at (xi, eta)
+  void at(int e, double xi, double eta, double X[3], double n[3], double& Jq, double g[3][3]) const {
+    if (gk == 1) {
+      point(e, xi, eta, X);
+      for (int c = 0; c < 3; c++) n[c] = nrm[3 * e + c];
+      Jq = J[e];
+      grads(e, g);
+      return;
+    }
+    double a[3], b[3];
+    curved_map(e, xi, eta, X, a, b);
+    const double c[3] = {a[1] * b[2] - a[2] * b[1], a[2] * b[0] - a[0] * b[2], a[0] * b[1] - a[1] * b[0]};
+    Jq = std::sqrt(c[0] * c[0] + c[1] * c[1] + c[2] * c[2]);
+    for (int k = 0; k < 3; k++) n[k] = c[k] / Jq;
+    metric_grads(a, b, g);
+  }
   // surface gradients of the barycentric coordinates lam0=xi, lam1=eta, lam2
+  // (flat element)
   void grads(int e, double g[3][3]) const {
     const double* p0 = &xyz[3 * tri[3 * e + 0]];
     const double* p1 = &xyz[3 * tri[3 * e + 1]];
@@ -451,6 +549,7 @@ static void eval_shape(const Mesh& m, const Space& s, int ev, int e, double xi, 
     // (rows 2l, 2l+1): phi_l, then psi_l = n x grad(lambda_a lambda_b) for the
     // edge (a, b) = (l, l+1), div 0: the HDivSurface function itself, or the
     // rotated trace n x grad(lambda_a lambda_b) of the HCurl one (curl 0)
+    if (m.gk > 1) throw std::runtime_error("oracle: edge spaces on curved meshes are not restated");
     double X[3];
     m.point(e, xi, eta, X);
     const bool o1 = (s.kind == SP_RWG1 || s.kind == SP_NED1);
@@ -482,9 +581,10 @@ static void eval_shape(const Mesh& m, const Space& s, int ev, int e, double xi, 
   }
   double lam[3] = {xi, eta, 1.0 - xi - eta};
   // H1: values and surface gradients of lambda_k (k < 3) and, for order 2, of
-  // the bubbles lambda_a lambda_b of the edges (a, b) = (l, l+1)
-  double g[3][3];
-  m.grads(e, g);
+  // the bubbles lambda_a lambda_b of the edges (a, b) = (l, l+1); the normal
+  // and the gradients at the point (curved elements)
+  double g[3][3], Xp[3], n[3], Jp;
+  m.at(e, xi, eta, Xp, n, Jp, g);
   // order 3 adds, per edge (a, b), s lambda_a lambda_b (lambda_b - lambda_a) with
   // s = +1 when the edge runs from the lower to the higher vertex number, and
   // the face bubble lambda_0 lambda_1 lambda_2
@@ -512,7 +612,6 @@ static void eval_shape(const Mesh& m, const Space& s, int ev, int e, double xi, 
       grad[9][c] = lam[1] * lam[2] * g[0][c] + lam[0] * lam[2] * g[1][c] + lam[0] * lam[1] * g[2][c];
   }
   if (ev == EV_ID) { for (int k = 0; k < nl; k++) out[k] = val[k]; return; }
-  const double* n = &m.nrm[3 * e];
   int dim = ev_dim(ev);
   for (int k = 0; k < nl; k++) {
     const double* gk = grad[k];
@@ -847,11 +946,11 @@ static void integrate4d(const Op& op, int ei, int ej, const double* xx, const do
   double si[40], sj[40];
   cplx kv[3];
   for (int q = 0; q < nq; q++) {
-    double X[3], Y[3];
-    m.point(ei, xx[q], xe[q], X);
-    m.point(ej, yx[q], ye[q], Y);
-    op.K.eval(X, Y, &m.nrm[3 * ei], &m.nrm[3 * ej], kv);
-    double f = m.J[ei] * m.J[ej] * w[q];
+    double X[3], Y[3], nx[3], ny[3], jx, jy, g[3][3];
+    m.at(ei, xx[q], xe[q], X, nx, jx, g);
+    m.at(ej, yx[q], ye[q], Y, ny, jy, g);
+    op.K.eval(X, Y, nx, ny, kv);
+    double f = jx * jy * w[q];
     eval_shape(m, op.test, op.K.ev_test, ei, xx[q], xe[q], si);
     eval_shape(m, op.trial, op.K.ev_trial, ej, yx[q], ye[q], sj);
     for (const Term& t : op.K.terms) {
@@ -938,19 +1037,20 @@ static void calc_block(const Op& op, const std::vector<int>& trialdofs, const st
           eval_shape(m, op.test, op.K.ev_test, ei, tr.xi[q], tr.eta[q], &shi[q * ni * dti]);
           eval_shape(m, op.trial, op.K.ev_trial, ej, tr.xi[q], tr.eta[q], &shj[q * nj * dtj]);
         }
-        std::vector<double> X(3 * nt), Y(3 * nt);
+        std::vector<double> X(3 * nt), Y(3 * nt), NX(3 * nt), NY(3 * nt), JX(nt), JY(nt);
         for (int q = 0; q < nt; q++) {
-          m.point(ei, tr.xi[q], tr.eta[q], &X[3 * q]);
-          m.point(ej, tr.xi[q], tr.eta[q], &Y[3 * q]);
+          double g[3][3];
+          m.at(ei, tr.xi[q], tr.eta[q], &X[3 * q], &NX[3 * q], JX[q], g);
+          m.at(ej, tr.xi[q], tr.eta[q], &Y[3 * q], &NY[3 * q], JY[q], g);
         }
         std::vector<cplx> kv((size_t)nt * nt * 3);
         for (int ix = 0; ix < nt; ix++)
           for (int iy = 0; iy < nt; iy++)
-            op.K.eval(&X[3 * ix], &Y[3 * iy], &m.nrm[3 * ei], &m.nrm[3 * ej], &kv[3 * (ix * nt + iy)]);
+            op.K.eval(&X[3 * ix], &Y[3 * iy], &NX[3 * ix], &NY[3 * iy], &kv[3 * (ix * nt + iy)]);
         for (const Term& t : op.K.terms) {
           for (int ix = 0; ix < nt; ix++)
             for (int iy = 0; iy < nt; iy++) {
-              double fac = (m.J[ei] * tr.w[ix]) * (m.J[ej] * tr.w[iy]);
+              double fac = (JX[ix] * tr.w[ix]) * (JY[iy] * tr.w[iy]);
               kij[ix * nt + iy] = t.fac * fac * kv[3 * (ix * nt + iy) + t.kc];
             }
           // kernel_shapesj = kij * shapesj1^T  (nt x nj)
@@ -1006,16 +1106,16 @@ static bool calc_aca(const Op& op, const std::vector<int>& trialdofs, const std:
   for (size_t a = 0; a < patchi.size(); a++)
     for (int q = 0; q < nt; q++) {
       int e = patchi[a], p = (int)a * nt + q;
-      m.point(e, tr.xi[q], tr.eta[q], &xi[3 * p]);
-      for (int k = 0; k < 3; k++) nxi[3 * p + k] = m.nrm[3 * e + k];
-      wxi[p] = m.J[e] * tr.w[q];
+      double g[3][3], jq;
+      m.at(e, tr.xi[q], tr.eta[q], &xi[3 * p], &nxi[3 * p], jq, g);
+      wxi[p] = jq * tr.w[q];
     }
   for (size_t a = 0; a < patchj.size(); a++)
     for (int q = 0; q < nt; q++) {
       int e = patchj[a], p = (int)a * nt + q;
-      m.point(e, tr.xi[q], tr.eta[q], &yj[3 * p]);
-      for (int k = 0; k < 3; k++) nyj[3 * p + k] = m.nrm[3 * e + k];
-      wyj[p] = m.J[e] * tr.w[q];
+      double g[3][3], jq;
+      m.at(e, tr.xi[q], tr.eta[q], &yj[3 * p], &nyj[3 * p], jq, g);
+      wyj[p] = jq * tr.w[q];
     }
   int p = std::min(np, nq);
   std::vector<std::vector<cplx>> Us(op.K.ncomp), Vs(op.K.ncomp);  // np x r col-major, r x nq row-major
@@ -1238,6 +1338,25 @@ void* orc_mesh_new(const double* xyz, int nv, const int* tri, int nt) {
 }
 void orc_mesh_free(void* m) { delete (Mesh*)m; }
 
+// geometry of element e at (xi, eta): out = position (3), unit normal (3),
+// measure (1), surface gradients of lambda_0..2 (9)
+void orc_mesh_at(void* m, int e, double xi, double eta, double* out) {
+  double g[3][3];
+  ((Mesh*)m)->at(e, xi, eta, out, out + 3, out[6], g);
+  for (int k = 0; k < 3; k++)
+    for (int c = 0; c < 3; c++) out[7 + 3 * k + c] = g[k][c];
+}
+
+// curved elements of order k from their Lagrange nodes (nt x (k+1)(k+2)/2 x 3)
+int orc_mesh_set_curved(void* m, int k, const double* nodes) {
+  try {
+    ((Mesh*)m)->set_curved(k, nodes);
+  } catch (const std::exception&) {
+    return -1;
+  }
+  return 0;
+}
+
 // kind 0 identical, 1 common edge, 2 common vertex; out = 5*Q (x0,x1,y0,y1,w)
 int orc_rule4(int kind, int order, double* out, int cap) {
   Rule4 r = ss_rule(kind, order);
@@ -1266,6 +1385,8 @@ void orc_gauss01(int n, double* x, double* w) {
 // with mask[e] != 0 (null: the whole mesh)
 void* orc_op_new_on(void* mesh, int kernel, double kappa, int trial_kind, int test_kind, const unsigned char* trial_mask,
                     const unsigned char* test_mask, int intorder, int leafsize, double eta, double eps) {
+  auto edge = [](int k) { return k == SP_RWG || k == SP_NED || k == SP_RWG1 || k == SP_NED1; };
+  if (((const Mesh*)mesh)->gk > 1 && (edge(trial_kind) || edge(test_kind))) return nullptr;
   Op* op = new Op;
   op->mesh = (const Mesh*)mesh;
   op->K.setup(kernel, kappa);
@@ -1437,7 +1558,7 @@ int orc_potential(void* p, const double* coef, const double* pts, const double* 
   const int nq = tr.size();
   const int dim = ev_dim(op.K.ev_trial), nl = op.trial.nloc;
   const cplx* c = (const cplx*)coef;
-  std::vector<double> Y((size_t)m.nt * nq * 3), W((size_t)m.nt * nq);
+  std::vector<double> Y((size_t)m.nt * nq * 3), NY((size_t)m.nt * nq * 3), W((size_t)m.nt * nq);
   std::vector<cplx> vals((size_t)m.nt * nq * dim);
   std::vector<char> use(m.nt);
   for (int e = 0; e < m.nt; e++) use[e] = op.trial.has(e) && (!elmask || elmask[e]);
@@ -1445,8 +1566,9 @@ int orc_potential(void* p, const double* coef, const double* pts, const double* 
     if (!use[e]) continue;
     std::vector<double> sh(40);
     for (int q = 0; q < nq; q++) {
-      m.point(e, tr.xi[q], tr.eta[q], &Y[3 * ((size_t)e * nq + q)]);
-      W[(size_t)e * nq + q] = m.J[e] * tr.w[q];
+      double g[3][3], jq;
+      m.at(e, tr.xi[q], tr.eta[q], &Y[3 * ((size_t)e * nq + q)], &NY[3 * ((size_t)e * nq + q)], jq, g);
+      W[(size_t)e * nq + q] = jq * tr.w[q];
       eval_shape(m, op.trial, op.K.ev_trial, e, tr.xi[q], tr.eta[q], sh.data());
       for (int d = 0; d < dim; d++) {
         cplx v = 0.0;
@@ -1466,7 +1588,7 @@ int orc_potential(void* p, const double* coef, const double* pts, const double* 
     for (int e = 0; e < m.nt; e++)
       for (int q = 0; q < nq && use[e]; q++) {
         const size_t sq = (size_t)e * nq + q;
-        op.K.eval(x, &Y[3 * sq], nx, &m.nrm[3 * e], kv);
+        op.K.eval(x, &Y[3 * sq], nx, &NY[3 * sq], kv);
         for (const Term& t : op.K.terms) acc[t.tec] += W[sq] * kv[t.kc] * vals[sq * dim + t.trc];
       }
     for (int d = 0; d < dim; d++) o[(size_t)i * dim + d] = acc[d];

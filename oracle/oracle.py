@@ -37,6 +37,8 @@ def lib():
         L.orc_mesh_new.restype = P
         L.orc_mesh_new.argtypes = [dp, ctypes.c_int, ip, ctypes.c_int]
         L.orc_mesh_free.argtypes = [P]
+        L.orc_mesh_set_curved.argtypes = [P, ctypes.c_int, dp]
+        L.orc_mesh_at.argtypes = [P, ctypes.c_int, ctypes.c_double, ctypes.c_double, dp]
         L.orc_rule4.argtypes = [ctypes.c_int, ctypes.c_int, dp, ctypes.c_int]
         L.orc_trig_rule.argtypes = [ctypes.c_int, dp, ctypes.c_int]
         L.orc_gauss01.argtypes = [ctypes.c_int, dp, dp]
@@ -121,10 +123,25 @@ def test_compression_aca(nx, ny, eta, eps, seed=1, factors=False):
 
 
 class OracleMesh:
-    def __init__(self, xyz, tri):
+    def __init__(self, xyz, tri, geo_order=1, nodes=None):
+        """geo_order > 1: curved elements through the Lagrange nodes (nt x
+        (k+1)(k+2)/2 x 3, (xi, eta) = (i / k, j / k), j outer, i inner)."""
         self.xyz = np.ascontiguousarray(xyz, dtype=np.float64)
         self.tri = np.ascontiguousarray(tri, dtype=np.int32)
         self.h = lib().orc_mesh_new(_dp(self.xyz), len(self.xyz), _ip(self.tri), len(self.tri))
+        self.geo_order = int(geo_order)
+        if self.geo_order > 1:
+            self.nodes = np.ascontiguousarray(nodes, dtype=np.float64)
+            if self.nodes.shape != (len(self.tri), (geo_order + 1) * (geo_order + 2) // 2, 3):
+                raise ValueError("nodes: nt x (k+1)(k+2)/2 x 3")
+            if lib().orc_mesh_set_curved(self.h, self.geo_order, _dp(self.nodes)) != 0:
+                raise ValueError("geometry order 2 to 4")
+
+    def at(self, e, xi, eta):
+        """(position, unit normal, measure, surface gradients of lambda_0..2 (3 x 3))."""
+        out = np.zeros(16)
+        lib().orc_mesh_at(self.h, int(e), float(xi), float(eta), _dp(out))
+        return out[:3], out[3:6], out[6], out[7:].reshape(3, 3)
 
     def __del__(self):
         if getattr(self, "h", None) and _lib is not None:
@@ -141,7 +158,7 @@ class OracleOp:
         """trial_mask / test_mask: boolean per element: the space lives on
         those elements only (dofs numbered compactly), None: the whole mesh."""
         if not isinstance(mesh, OracleMesh):
-            mesh = OracleMesh(mesh.xyz, mesh.tri)
+            mesh = OracleMesh(mesh.xyz, mesh.tri, getattr(mesh, "geo_order", 1), getattr(mesh, "nodes", None))
         self.mesh = mesh
         self.kernel = kernel
         self._masks = [None if k is None else np.ascontiguousarray(k, dtype=np.uint8) for k in (trial_mask, test_mask)]
@@ -151,6 +168,8 @@ class OracleOp:
         mp = [None if k is None else k.ctypes.data_as(ctypes.c_void_p) for k in self._masks]
         self.h = lib().orc_op_new_on(mesh.h, kernel, float(kappa), trial, test, mp[0], mp[1], intorder, leafsize,
                                      float(eta), float(eps))
+        if not self.h:
+            raise ValueError("oracle: edge spaces on curved meshes are not restated")
         a, b = ctypes.c_int(), ctypes.c_int()
         lib().orc_op_sizes(self.h, ctypes.byref(a), ctypes.byref(b))
         self.ntrial, self.ntest = a.value, b.value

@@ -1,0 +1,161 @@
+"""Pins for curved (isoparametric) elements, CPU only.
+
+The oracle restates the element map in Lagrange form (products of the 1-D
+factors prod (k t - a) / (a + 1) in the barycentrics) with metric-inverse
+surface gradients; the product's host code uses monomial coefficients from a
+Vandermonde solve and the cross-product identity for n x grad lambda.  What
+fixes them:
+- interpolation: the map reproduces every Lagrange node;
+- the two forms agree pointwise (position, normal, measure);
+- surface gradients: grad lambda_k . dX/dxi and . dX/deta are the reference
+  derivatives of lambda_k (finite differences of the map) and grad lambda_k
+  . n = 0;
+- straight curves (the nodes left on the flat triangles) give the flat
+  operators to rounding, for SL, DL, HS and H1 order 2;
+- on the unit sphere: capacitance 4 pi and V 1 = 1 converge at O(h^4) with
+  order 2 (the flat mesh: O(h^2)) and are 50x closer at level 2; K 1 = -1/2
+  (Gauss's law, on any closed surface, with the normal at each point); W 1 = 0
+  to rounding (the curls of lambda_0..2 sum to zero at
+  every point).
+"""
+import numpy as np
+import pytest
+
+import ngbem_amd as nb
+import ngbem_amd.mesh as M
+from oracle import oracle as O
+
+
+def _curved(level, k):
+    m = M.icosphere(level)
+    return m.Curve(k) if k > 1 else m
+
+
+@pytest.mark.parametrize("k", [2, 3, 4])
+def test_geometry_two_forms_and_nodes(k):
+    m = _curved(1, k)
+    om = O.OracleMesh(m.xyz, m.tri, m.geo_order, m.nodes)
+    ref = M.lagrange_nodes(k)
+    for e in (0, 7, 33, 79):
+        for r, (xi, eta) in enumerate(ref):  # interpolation
+            X, _, _, _ = om.at(e, xi, eta)
+            assert np.abs(X - m.nodes[e, r]).max() <= 1e-14
+    rng = np.random.default_rng(5)
+    u = rng.random((12, 2))
+    xi, eta = u[:, 0] * (1 - u[:, 1]), u[:, 1]
+    Xp, Np, Jp = m.geom(xi, eta)
+    h = 1e-5
+    for e in (0, 19, 61):
+        for q in range(len(xi)):
+            X, n, J, g = om.at(e, xi[q], eta[q])
+            assert np.abs(X - Xp[e, q]).max() <= 1e-13
+            assert np.abs(n - Np[e, q]).max() <= 1e-12
+            assert abs(J - Jp[e, q]) <= 1e-13 * J
+            ta = (om.at(e, xi[q] + h, eta[q])[0] - om.at(e, xi[q] - h, eta[q])[0]) / (2 * h)
+            tb = (om.at(e, xi[q], eta[q] + h)[0] - om.at(e, xi[q], eta[q] - h)[0]) / (2 * h)
+            # lambda_0 = xi, lambda_1 = eta, lambda_2 = 1 - xi - eta
+            D = np.array([[1.0, 0.0], [0.0, 1.0], [-1.0, -1.0]])
+            G = np.stack([g @ ta, g @ tb], axis=1)
+            assert np.abs(G - D).max() <= 1e-8, (G, D)
+            assert np.abs(g @ n).max() <= 1e-12 * np.abs(g).max()
+            # the product's curls (t2, -t1, t1 - t2) / J are n x grad lambda_k
+            c = np.stack([tb, -ta, ta - tb]) / np.linalg.norm(np.cross(ta, tb))
+            assert np.abs(np.cross(n, g) - c).max() <= 1e-8 * np.abs(c).max()
+
+
+@pytest.mark.parametrize("kernel,trial,test,kappa", [
+    (O.LAPLACE_SL, O.P0, O.P0, 0.0), (O.LAPLACE_DL, O.P1, O.P0, 0.0), (O.LAPLACE_HS, O.P1, O.P1, 0.0),
+    (O.HELMHOLTZ_HS, O.P2, O.P2, 2.0), (O.HELMHOLTZ_DL, O.P2, O.L2_1, 2.0)])
+def test_straight_curve_is_flat(kernel, trial, test, kappa):
+    flat = M.icosphere(1)
+    straight = M.Mesh(flat.xyz, flat.tri).Curve(3)  # no projection: the nodes stay on the triangles
+    kw = dict(kappa=kappa, trial=trial, test=test, intorder=4)
+    A = O.OracleOp(flat, kernel, **kw).dense()
+    B = O.OracleOp(straight, kernel, **kw).dense()
+    assert np.abs(A - B).max() <= 1e-12 * np.abs(A).max()
+
+
+def test_straight_curve_potential():
+    flat = M.icosphere(1)
+    straight = M.Mesh(flat.xyz, flat.tri).Curve(2)
+    P = np.random.default_rng(1).standard_normal((20, 3)) * 2.0
+    c = np.random.default_rng(2).standard_normal(flat.nv) + 0j
+    a = O.OracleOp(flat, O.HELMHOLTZ_DL, kappa=1.5, trial=O.P1, test=O.P1).potential(c, P)
+    b = O.OracleOp(straight, O.HELMHOLTZ_DL, kappa=1.5, trial=O.P1, test=O.P1).potential(c, P)
+    assert np.abs(a - b).max() <= 1e-12 * np.abs(a).max()
+
+
+def _cap_errors(level, k):
+    m = _curved(level, k)
+    V = O.OracleOp(m, O.LAPLACE_SL, intorder=6).dense().real
+    a = m.areas()
+    cap = a @ np.linalg.solve(V, a)
+    return abs(cap - 4 * np.pi) / (4 * np.pi), np.abs(V.sum(1) - a).max() / a.max()
+
+
+def test_capacitance_converges_at_order_four():
+    """P0 holds the exact density (1 on the unit sphere), so the capacitance
+    error is the geometry's: O(h^2) flat, O(h^4) with quadratic elements."""
+    e1, e2 = _cap_errors(1, 2), _cap_errors(2, 2)
+    f2 = _cap_errors(2, 1)
+    for i in range(2):
+        assert e1[i] / e2[i] > 10.0, (e1, e2)  # h halves: 16x for O(h^4)
+        assert e2[i] < f2[i] / 50.0, (e2, f2)
+    assert e2[0] < 1e-4
+
+
+def test_double_layer_and_hypersingular_curved():
+    m = _curved(2, 2)
+    flat = M.icosphere(2)
+    errs = []
+    for mesh in (flat, m):
+        K = O.OracleOp(mesh, O.LAPLACE_DL, trial=O.P0, test=O.P0, intorder=6).dense().real
+        a = mesh.areas()
+        errs.append(np.abs(K.sum(1) + 0.5 * a).max() / a.max())
+    # Gauss's law: K 1 = -1/2 on any closed surface, with the normal of the
+    # surface at each point (a per-element normal on the curved elements
+    # leaves an O(h) flux error)
+    assert max(errs) < 1e-5, errs
+    W = O.OracleOp(m, O.LAPLACE_HS, trial=O.P1, test=O.P1, intorder=4).dense().real
+    assert np.abs(W.sum(1)).max() <= 1e-13 * np.abs(W).max() * m.nv
+    assert np.abs(W - W.T).max() / np.abs(W).max() < 1e-3
+
+
+def test_curve_rules():
+    m = M.icosphere(1)
+    with pytest.raises(ValueError):
+        m.Curve(5)
+    nb.SurfaceL2(m, order=0)
+    with pytest.raises(RuntimeError):
+        m.Curve(2)  # spaces exist: the geometry is fixed
+    c = M.icosphere(1).Curve(2)
+    with pytest.raises(NotImplementedError):
+        nb.HDivSurface(c)
+    with pytest.raises(NotImplementedError):
+        nb.SurfaceL2(c, order=0, dual_mapping=True)
+    with pytest.raises(ValueError):
+        O.OracleOp(c, O.MAXWELL_SL, kappa=1.0, trial=O.HDIV, test=O.HDIV)
+
+
+def test_curved_mass_matrix():
+    """Surface mass matrices on curved elements: P0 diagonal = the element
+    areas; 1 (the vertex hats of H1 order 2) has the total area as its squared
+    norm; the P1
+    row sums are the integrals of the hats, which add up to the area too."""
+    m = _curved(2, 3)
+    a = m.areas()
+    assert abs(a.sum() - 4 * np.pi) < 1e-3
+    r, c, v = nb.spaces.mass_matrix_coo(nb.SurfaceL2(m, order=0), nb.SurfaceL2(m, order=0))
+    Mp0 = np.zeros((m.nt, m.nt))
+    np.add.at(Mp0, (r, c), v)
+    assert np.abs(np.diag(Mp0) - a).max() <= 1e-13 * a.max()
+    h2 = nb.H1(m, order=2)
+    r, c, v = nb.spaces.mass_matrix_coo(h2, h2)
+    one = (np.arange(h2.ndof) < m.nv).astype(float)  # hierarchical: 1 = the vertex hats
+    assert abs(np.sum(v * one[r] * one[c]) - a.sum()) <= 1e-12 * a.sum()
+    h1 = nb.H1(m, order=1)
+    r, c, v = nb.spaces.mass_matrix_coo(h1, h1)
+    Mp1 = np.zeros((h1.ndof, h1.ndof))
+    np.add.at(Mp1, (r, c), v)
+    assert np.abs(Mp1 - Mp1.T).max() <= 1e-15 * np.abs(Mp1).max()
+    assert abs(Mp1.sum() - a.sum()) <= 1e-12 * a.sum()
