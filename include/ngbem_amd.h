@@ -41,14 +41,18 @@ typedef struct bem_batch bem_batch;
 enum {
   BEM_SPACE_P0 = 0,    /* SurfaceL2, order 0-3: order 0 one dof per triangle (shape 1); orders 1-3 add
                           the triangle's higher Dubiner functions (3 / 6 / 10 per triangle, constants first) */
-  BEM_SPACE_P1 = 1,    /* H1, order 1-3: one dof per vertex (barycentric hats); order 2 adds one edge
-                          bubble lambda_a lambda_b per edge, order 3 a second, oriented edge function per
-                          edge and a face bubble (numbered vertices, edges, faces) */
-  BEM_SPACE_HDIV = 2,  /* HDivSurface, order 0-1: order 0 one dof per edge, unit-flux RT0; order 1 adds
-                          per edge the surface curl n x grad(lambda_a lambda_b) of the edge bubble (dofs:
-                          the order-0 ones, then ne + edge; element-local 2l / 2l+1) */
-  BEM_SPACE_HCURL = 3  /* HCurl, order 0-1, on the surface: order 0 one dof per edge, Whitney; order 1
-                          adds per edge the gradient grad(lambda_a lambda_b) of the edge bubble */
+  BEM_SPACE_P1 = 1,    /* H1, order 1-4: one dof per vertex (barycentric hats); order p adds p - 1
+                          functions per edge lambda_a lambda_b t^q s^q (t = lambda_b - lambda_a, s the
+                          edge's orientation, q < p - 1) and (p-1)(p-2)/2 face bubbles B, B lambda_0,
+                          B lambda_1 (B = lambda_0 lambda_1 lambda_2) (numbered vertices, edges, faces) */
+  BEM_SPACE_HDIV = 2,  /* HDivSurface, order 0-4, all tangential fields of degree p: per edge the
+                          unit-flux RT0 function and the surface curls n x grad(s^q lambda_a lambda_b t^q),
+                          q < p; (p+1)(p-1) interior functions per element (dofs: function i of every
+                          edge at i ne + edge, then the interior ones element by element; element-local
+                          l (p+1) + i, then 3 (p+1) + j) */
+  BEM_SPACE_HCURL = 3  /* HCurl, order 0-4, on the surface: the same layout; per edge the Whitney
+                          function and the gradients grad(s^q lambda_a lambda_b t^q), and interior
+                          functions whose rotated traces are those of HDivSurface */
 };
 
 /* kernel ids (src/ngbem.hpp:230-510); evaluators follow the factories:

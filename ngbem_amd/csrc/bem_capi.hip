@@ -152,18 +152,27 @@ int bem_mesh_destroy(bem_mesh* m) {
 
 int bem_space_create_on(bem_mesh* m, int kind, int order, const unsigned char* elmask, bem_space** out) {
   if (!m || !out) return fail(-1, "bem_space_create: bad arguments");
-  if (!((kind == BEM_SPACE_P0 && order >= 0 && order <= 3) || (kind == BEM_SPACE_P1 && order >= 1 && order <= 3) ||
-        (kind == BEM_SPACE_HDIV && order >= 0 && order <= 1) || (kind == BEM_SPACE_HCURL && order >= 0 && order <= 1)))
-    return fail(-1, "bem_space_create: supported spaces are SurfaceL2 orders 0-3, H1 orders 1-3, HDivSurface "
-                    "orders 0-1 and HCurl orders 0-1");
+  if (!((kind == BEM_SPACE_P0 && order >= 0 && order <= 3) || (kind == BEM_SPACE_P1 && order >= 1 && order <= 4) ||
+        (kind == BEM_SPACE_HDIV && order >= 0 && order <= 4) || (kind == BEM_SPACE_HCURL && order >= 0 && order <= 4)))
+    return fail(-1, "bem_space_create: supported spaces are SurfaceL2 orders 0-3, H1 orders 1-4, HDivSurface "
+                    "orders 0-4 and HCurl orders 0-4");
   int ik = kind;  // internal kind
   if (kind == BEM_SPACE_P0) {
     static const int k0[4] = {bem::SP_P0, bem::SP_L2_1, bem::SP_L2_2, bem::SP_L2_3};
     ik = k0[order];
   }
-  if (kind == BEM_SPACE_P1) ik = order == 1 ? bem::SP_P1 : (order == 2 ? bem::SP_P2 : bem::SP_P3);
-  if (kind == BEM_SPACE_HDIV) ik = order == 0 ? bem::SP_RWG : bem::SP_RWG1;
-  if (kind == BEM_SPACE_HCURL) ik = order == 0 ? bem::SP_NED : bem::SP_NED1;
+  if (kind == BEM_SPACE_P1) {
+    static const int k1[5] = {-1, bem::SP_P1, bem::SP_P2, bem::SP_P3, bem::SP_P4};
+    ik = k1[order];
+  }
+  if (kind == BEM_SPACE_HDIV) {
+    static const int kd[5] = {bem::SP_RWG, bem::SP_RWG1, bem::SP_RWG2, bem::SP_RWG3, bem::SP_RWG4};
+    ik = kd[order];
+  }
+  if (kind == BEM_SPACE_HCURL) {
+    static const int kc[5] = {bem::SP_NED, bem::SP_NED1, bem::SP_NED2, bem::SP_NED3, bem::SP_NED4};
+    ik = kc[order];
+  }
   if (elmask) {
     bool any = false;
     for (int e = 0; e < m->m->nt && !any; e++) any = elmask[e] != 0;
@@ -239,8 +248,6 @@ static int create_op(int kernel, double kappa, bem_space* trial, bem_space* test
                       "space on both sides");
     if (kernel == bem::K_MDL && !(bem::is_hcurl_kind(tk) && bem::is_hdiv_kind(sk)))
       return fail(-1, "bem_op_create: Maxwell double layer needs an HCurl trial and an HDivSurface test space");
-    if (trial->s->mesh->gorder > 1 && !scalar)
-      return fail(-1, "bem_op_create: edge spaces (HDivSurface, HCurl) on curved meshes are not supported");
   }
   return guarded([&] {
     const double t0 = now();

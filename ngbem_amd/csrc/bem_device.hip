@@ -279,29 +279,30 @@ struct KMaxDL {  // e^{ikr}(ikr - 1)(x-y)/(4 pi r^3), cross-product terms (src/n
 // shape rows at one point, layout dof*DIM + comp (src/diffops.hpp:12-338).
 // Per-element shape data (stride SDS doubles): [n x grad lambda_k (9),
 // edge scale s_k = sign_k / J (3), vertex opposite edge k (9), test space
-// kind, trial space kind, orientation of edge k (+1 when v_k < v_k+1) (3)].
+// kind, trial space kind, orientation of edge k (+1 when v_k < v_k+1) (3),
+// Jacobian J, pad].
 // load(): copy what the evaluator needs into registers; PERM evaluators index
 // their dofs by Sauter-Schwab vertex (px) and take barycentrics in that order,
 // the others index by element-local dof and take local barycentrics.
 // MULTI: more than one dof per element (scatter through the dof tables).
-constexpr int SDS = 26;
+constexpr int SDS = 28;
 // curved elements, per quadrature point of the triangle rule: unit normal,
-// then n x grad_Gamma lambda_0, lambda_1, lambda_2
-constexpr int QG = 12;
+// n x grad_Gamma lambda_0, lambda_1, lambda_2, the Jacobian, pad
+constexpr int QG = 16;
 
 template <int SP, int EV>
 struct Ev;
 template <>
 struct Ev<SP_P0, EV_ID> {
   static constexpr int NL = 1, DIM = 1, NB = 1;
-  static constexpr bool MULTI = false, PERM = false, CURLS = false;
+  static constexpr bool MULTI = false, PERM = false, CURLS = false, WIDE = false;
   __device__ __forceinline__ static void load(const double*, const int*, double*) {}
   __device__ __forceinline__ static void shape(const double*, P3, const double*, double* o) { o[0] = 1.0; }
 };
 template <>
 struct Ev<SP_P1, EV_ID> {
   static constexpr int NL = 3, DIM = 1, NB = 1;
-  static constexpr bool MULTI = true, PERM = true, CURLS = false;
+  static constexpr bool MULTI = true, PERM = true, CURLS = false, WIDE = false;
   __device__ __forceinline__ static void load(const double*, const int*, double*) {}
   __device__ __forceinline__ static void shape(const double* lam, P3, const double*, double* o) {
     o[0] = lam[0]; o[1] = lam[1]; o[2] = lam[2];
@@ -310,7 +311,7 @@ struct Ev<SP_P1, EV_ID> {
 template <>
 struct Ev<SP_P1, EV_ROT> {
   static constexpr int NL = 3, DIM = 3, NB = 9;
-  static constexpr bool MULTI = true, PERM = true, CURLS = true;
+  static constexpr bool MULTI = true, PERM = true, CURLS = true, WIDE = false;
   __device__ __forceinline__ static void load(const double* sd, const int* px, double* b) {
     for (int k = 0; k < 3; k++)
       for (int c = 0; c < 3; c++) b[3 * k + c] = sd[3 * px[k] + c];
@@ -323,7 +324,7 @@ struct Ev<SP_P1, EV_ROT> {
 template <>
 struct Ev<SP_P1, EV_HELM> {
   static constexpr int NL = 3, DIM = 4, NB = 9;
-  static constexpr bool MULTI = true, PERM = true, CURLS = true;
+  static constexpr bool MULTI = true, PERM = true, CURLS = true, WIDE = false;
   __device__ __forceinline__ static void load(const double* sd, const int* px, double* b) {
     Ev<SP_P1, EV_ROT>::load(sd, px, b);
   }
@@ -343,7 +344,7 @@ struct Ev<SP_P1, EV_HELM> {
 template <int SGN, bool DIV>
 struct EvEdge {
   static constexpr int NL = 3, DIM = DIV ? 4 : 3, NB = 12;
-  static constexpr bool MULTI = true, PERM = false, CURLS = false;
+  static constexpr bool MULTI = true, PERM = false, CURLS = false, WIDE = false;
   __device__ __forceinline__ static void load(const double* sd, const int*, double* b) {
     for (int i = 0; i < 12; i++) b[i] = sd[9 + i];
   }
@@ -386,7 +387,7 @@ __device__ __forceinline__ double jacobi_a0(int n, double a, double t) {
 template <int SIDE, int EVK, int NLR>
 struct EvGen {
   static constexpr int NL = NLR, DIM = EVK == EV_ID ? 1 : (EVK == EV_ROT ? 3 : 4), NB = 13;
-  static constexpr bool MULTI = true, PERM = false, CURLS = true;
+  static constexpr bool MULTI = true, PERM = false, CURLS = true, WIDE = false;
   __device__ __forceinline__ static void load(const double* sd, const int*, double* b) {
     for (int i = 0; i < 9; i++) b[i] = sd[i];
     b[9] = sd[21 + SIDE];
@@ -468,7 +469,7 @@ template <> struct Ev<SP_NED, EV_TRACE3> : EvEdge<-1, false> {};
 template <int SGN, bool DIV>
 struct EvEdge1 {
   static constexpr int NL = 6, DIM = DIV ? 4 : 3, NB = 21;
-  static constexpr bool MULTI = true, PERM = false, CURLS = false;
+  static constexpr bool MULTI = true, PERM = false, CURLS = false, WIDE = false;
   __device__ __forceinline__ static void load(const double* sd, const int*, double* b) {
     for (int i = 0; i < 21; i++) b[i] = sd[i];
   }
@@ -497,12 +498,217 @@ template <> struct Ev<SP_NED1, EV_MAXW> : EvEdge1<-1, true> {};
 template <> struct Ev<SP_RWG1, EV_TRACE3> : EvEdge1<1, false> {};
 template <> struct Ev<SP_NED1, EV_TRACE3> : EvEdge1<-1, false> {};
 
+// ------------------------------------------------------ wide evaluators --
+// H1 order 4 and HDivSurface / HCurl orders 2-4, and the edge spaces of any
+// order on curved meshes (whatever space they are paired with is evaluated
+// the same way): up to 30 functions per element, evaluated one at a time by
+// one(l, ...) (k_near_wide spreads the element matrix over a wavefront).
+// Element data b: n x grad_Gamma lambda_0..2 (9, per point on curved meshes),
+// the space kind (slot 21 + SIDE), the edge orientations s_l (3), the
+// Jacobian (per point on curved meshes).  With c_k = n x grad_Gamma lambda_k:
+//   H1 order p: lambda_k; per edge l = (a, b) = (l, l+1), t = lambda_b -
+//     lambda_a: lambda_a lambda_b t^q s_l^q, q < p - 1 (local 3 + (p-1) l + q);
+//     faces: B, B lambda_0, B lambda_1 with B = lambda_0 lambda_1 lambda_2,
+//     the first (p-1)(p-2)/2 of them; rotated gradients sum_k df/dlambda_k c_k.
+//   SurfaceL2: the Dubiner functions of EvGen.
+//   Edge spaces of order p, HDivSurface [phi, div phi]: per edge l (local
+//     l (p+1) + i) i = 0: s_l (lambda_b c_a - lambda_a c_b), div 2 s_l / J;
+//     i >= 1: the surface curl of s_l^(i-1) lambda_a lambda_b t^(i-1), div 0;
+//     interior (local 3 (p+1) + j): g c_c, g = lambda_(c+1) lambda_(c+2) q
+//     with q = lambda_0^u lambda_1^v (u + v <= p - 2, u outer) for c = 0 and
+//     c = 1, q = lambda_0^u lambda_1^(p-2-u) for c = 2; div (g c_c) =
+//     (dg/dlambda_(c+1) - dg/dlambda_(c+2)) / J (n . (grad lambda_k x grad
+//     lambda_(k+1)) = 1 / J).  HCurl [n x N, curl N]: n x N = -phi for i = 0
+//     and phi otherwise, curl N = div phi for i = 0 and -div phi otherwise.
+__device__ __forceinline__ double ipow_(double x, int n) {
+  double r = 1.0;
+  for (int i = 0; i < n; i++) r *= x;
+  return r;
+}
+__device__ __forceinline__ int wide_nloc(int k) {
+  switch (k) {
+    case SP_P0: return 1;
+    case SP_P1: case SP_L2_1: case SP_RWG: case SP_NED: return 3;
+    case SP_P2: case SP_L2_2: case SP_RWG1: case SP_NED1: return 6;
+    case SP_P3: case SP_L2_3: return 10;
+    case SP_RWG2: case SP_NED2: return 12;
+    case SP_P4: return 15;
+    case SP_RWG3: case SP_NED3: return 20;
+    case SP_RWG4: case SP_NED4: return 30;
+  }
+  return 0;
+}
+__device__ __forceinline__ int wide_edge_order(int k) {
+  if (k == SP_RWG1 || k == SP_NED1) return 1;
+  if (k >= SP_RWG2 && k <= SP_RWG4) return 2 + k - SP_RWG2;
+  if (k >= SP_NED2 && k <= SP_NED4) return 2 + k - SP_NED2;
+  return 0;
+}
+__device__ __forceinline__ bool wide_hcurl(int k) { return k == SP_NED || k == SP_NED1 || (k >= SP_NED2 && k <= SP_NED4); }
+
+// H1 function l of order p: value and df/dlambda_k (k = 0, 1, 2)
+__device__ __forceinline__ void h1_fn(int p, int l, const double* lam, const double* sgn, double& v, double* d) {
+  d[0] = d[1] = d[2] = 0.0;
+  if (l < 3) {
+    v = lam[l];
+    d[l] = 1.0;
+    return;
+  }
+  const int pe = p - 1;
+  int j = l - 3;
+  if (j < 3 * pe) {
+    const int e = j / pe, q = j - (j / pe) * pe, a = e, b = (e + 1) % 3;
+    const double s = (q & 1) ? sgn[e] : 1.0;
+    const double t = lam[b] - lam[a], tq = ipow_(t, q), tq1 = q ? ipow_(t, q - 1) : 0.0, ab = lam[a] * lam[b];
+    v = s * ab * tq;
+    d[a] = s * (lam[b] * tq - q * ab * tq1);
+    d[b] = s * (lam[a] * tq + q * ab * tq1);
+    return;
+  }
+  j -= 3 * pe;
+  const double B = lam[0] * lam[1] * lam[2];
+  const double mlt = j == 0 ? 1.0 : lam[j - 1];
+  v = B * mlt;
+  d[0] = lam[1] * lam[2] * mlt;
+  d[1] = lam[0] * lam[2] * mlt;
+  d[2] = lam[0] * lam[1] * mlt;
+  if (j > 0) d[j - 1] += B;
+}
+
+// edge-space function l of order p: phi = sum_k w[k] c_k, div phi = dj / J;
+// low: the order-0 function of its edge
+__device__ __forceinline__ void edge_fn(int p, int l, const double* lam, const double* sgn, double* w, double& dj,
+                                        bool& low) {
+  w[0] = w[1] = w[2] = 0.0;
+  dj = 0.0;
+  low = false;
+  const int nedge = 3 * (p + 1);
+  if (l < nedge) {
+    const int e = l / (p + 1), i = l - e * (p + 1), a = e, b = (e + 1) % 3;
+    const double s = sgn[e];
+    if (i == 0) {
+      w[a] = s * lam[b];
+      w[b] = -s * lam[a];
+      dj = 2.0 * s;
+      low = true;
+      return;
+    }
+    const int q = i - 1;
+    const double sq = (q & 1) ? s : 1.0;
+    const double t = lam[b] - lam[a], tq = ipow_(t, q), tq1 = q ? ipow_(t, q - 1) : 0.0, ab = lam[a] * lam[b];
+    w[a] = sq * (lam[b] * tq - q * ab * tq1);
+    w[b] = sq * (lam[a] * tq + q * ab * tq1);
+    return;
+  }
+  const int j = l - nedge, P = p * (p - 1) / 2;
+  int c, u, v;
+  if (j < 2 * P) {
+    c = j / P;
+    int r = j - c * P;
+    u = 0;
+    while (r > p - 2 - u) {
+      r -= p - 1 - u;
+      u++;
+    }
+    v = r;
+  } else {
+    c = 2;
+    u = j - 2 * P;
+    v = p - 2 - u;
+  }
+  const int a = (c + 1) % 3, b = (c + 2) % 3;
+  const double q = ipow_(lam[0], u) * ipow_(lam[1], v);
+  const double dq0 = u ? u * ipow_(lam[0], u - 1) * ipow_(lam[1], v) : 0.0;
+  const double dq1 = v ? v * ipow_(lam[0], u) * ipow_(lam[1], v - 1) : 0.0;
+  const double ab = lam[a] * lam[b];
+  w[c] = ab * q;
+  double dg[3] = {ab * dq0, ab * dq1, 0.0};
+  dg[a] += lam[b] * q;
+  dg[b] += lam[a] * q;
+  dj = dg[a] - dg[b];
+}
+
+// SurfaceL2 (Dubiner) function l of order p in the element's barycentrics
+__device__ __forceinline__ double l2_fn(int p, int l, const double* lam) {
+  const double y = lam[1], w = 1.0 - y, sx = 2.0 * lam[0] + y - 1.0, t = 2.0 * y - 1.0;
+  double qm = 0.0, qc = 1.0;
+  int n = 0;
+  for (int i = 0; i <= p; i++) {
+    if (i == 1) {
+      qm = qc;
+      qc = sx;
+    } else if (i > 1) {
+      const double qn = ((2.0 * i - 1.0) * sx * qc - (i - 1.0) * w * w * qm) / i;
+      qm = qc;
+      qc = qn;
+    }
+    for (int j = 0; j + i <= p; j++, n++)
+      if (n == l) return qc * jacobi_a0(j, 2.0 * i + 1.0, t);
+  }
+  return 0.0;
+}
+
+template <int SIDE, int EVK>
+struct EvWide {
+  static constexpr int NL = NLOC_MAX, NB = 14;
+  static constexpr int DIM = EVK == EV_ID ? 1 : ((EVK == EV_ROT || EVK == EV_TRACE3) ? 3 : 4);
+  static constexpr bool MULTI = true, PERM = false, CURLS = true, WIDE = true;
+  __device__ __forceinline__ static void load(const double* sd, const int*, double* b) {
+    for (int i = 0; i < 9; i++) b[i] = sd[i];
+    b[9] = sd[21 + SIDE];
+    b[10] = sd[23];
+    b[11] = sd[24];
+    b[12] = sd[25];
+    b[13] = sd[26];
+  }
+  __device__ __forceinline__ static int nl(const double* b) { return wide_nloc((int)b[9]); }
+  // the DIM values of function l at the element-local barycentrics lam
+  __device__ __forceinline__ static void one(int l, const double* lam, const double* b, double* o) {
+    const int kind = (int)b[9];
+    if constexpr (EVK == EV_MAXW || EVK == EV_TRACE3) {
+      double w[3], dj;
+      bool low;
+      edge_fn(wide_edge_order(kind), l, lam, b + 10, w, dj, low);
+      const bool curl = wide_hcurl(kind);
+      const double sv = (curl && low) ? -1.0 : 1.0;
+#pragma unroll
+      for (int d = 0; d < 3; d++) o[d] = sv * (w[0] * b[d] + w[1] * b[3 + d] + w[2] * b[6 + d]);
+      if constexpr (EVK == EV_MAXW) o[3] = ((curl && !low) ? -dj : dj) / b[13];
+    } else {
+      if (kind == SP_P0 || kind == SP_L2_1 || kind == SP_L2_2 || kind == SP_L2_3) {
+        const int p = kind == SP_P0 ? 0 : (kind == SP_L2_1 ? 1 : (kind == SP_L2_2 ? 2 : 3));
+        o[0] = l2_fn(p, l, lam);  // values only: SurfaceL2 spaces meet EV_ID
+        return;
+      }
+      const int p = kind == SP_P1 ? 1 : (kind == SP_P2 ? 2 : (kind == SP_P3 ? 3 : 4));
+      double v, d[3];
+      h1_fn(p, l, lam, b + 10, v, d);
+      if constexpr (EVK == EV_ID) {
+        o[0] = v;
+      } else {
+#pragma unroll
+        for (int c = 0; c < 3; c++) o[c] = d[0] * b[c] + d[1] * b[3 + c] + d[2] * b[6 + c];
+        if constexpr (EVK == EV_HELM) o[3] = v;
+      }
+    }
+  }
+  // all functions at once (the narrow interface; used by no kernel with WIDE)
+  __device__ __forceinline__ static void shape(const double* lam, P3, const double* b, double* o) {
+    for (int l = 0; l < NL; l++) {
+      if (l < nl(b)) one(l, lam, b, o + l * DIM);
+      else for (int d = 0; d < DIM; d++) o[l * DIM + d] = 0.0;
+    }
+  }
+};
+
 // Curved elements: an evaluator's element curls (b[0..8], n x grad lambda_k)
-// replaced by the ones at the current point (c9 in element vertex order;
-// PERM evaluators hold them in the Sauter-Schwab order perm).
+// replaced by the ones at the current point (g: a per-point record, normal,
+// curls in element vertex order, Jacobian; PERM evaluators hold the curls in
+// the Sauter-Schwab order perm; wide evaluators also take the Jacobian).
 template <class EV>
-__device__ __forceinline__ void point_curls(const double* c9, const int* perm, double* b) {
+__device__ __forceinline__ void point_curls(const double* g, const int* perm, double* b) {
   if constexpr (EV::CURLS) {
+    const double* c9 = g + 3;
 #pragma unroll
     for (int k = 0; k < 3; k++) {
       const int v = EV::PERM ? perm[k] : k;
@@ -511,14 +717,15 @@ __device__ __forceinline__ void point_curls(const double* c9, const int* perm, d
       b[3 * k + 2] = c9[3 * v + 2];
     }
   }
+  if constexpr (EV::WIDE) b[13] = g[12];
 }
 
 // Curved element map x(xi, eta) = sum A_ab xi^a eta^b (a + b <= k, degree
-// outer, b inner; A: 3 doubles per monomial): position, unit normal,
-// Jacobian and n x grad_Gamma lambda_0..2 = (t2, -t1, t1 - t2) / J with the
-// tangents t1 = dx/dxi, t2 = dx/deta.
+// outer, b inner; A: 3 doubles per monomial): position and the per-point
+// record g = unit normal, n x grad_Gamma lambda_0..2 = (t2, -t1, t1 - t2) / J
+// with the tangents t1 = dx/dxi, t2 = dx/deta, and the Jacobian J.
 __device__ __forceinline__ void curved_point(const double* __restrict__ A, int k, double xi, double eta, P3& X,
-                                             P3& n, double& Jq, double* c9) {
+                                             double* g) {
   double pxi[GEO_ORDER_MAX + 1], peta[GEO_ORDER_MAX + 1];
   pxi[0] = peta[0] = 1.0;
 #pragma unroll
@@ -544,16 +751,19 @@ __device__ __forceinline__ void curved_point(const double* __restrict__ A, int k
     }
   const double cr0 = t1[1] * t2[2] - t1[2] * t2[1], cr1 = t1[2] * t2[0] - t1[0] * t2[2],
                cr2 = t1[0] * t2[1] - t1[1] * t2[0];
-  Jq = sqrt(cr0 * cr0 + cr1 * cr1 + cr2 * cr2);
+  const double Jq = sqrt(cr0 * cr0 + cr1 * cr1 + cr2 * cr2);
   const double iJ = 1.0 / Jq;
   X = P3{x[0], x[1], x[2]};
-  n = P3{cr0 * iJ, cr1 * iJ, cr2 * iJ};
+  g[0] = cr0 * iJ;
+  g[1] = cr1 * iJ;
+  g[2] = cr2 * iJ;
 #pragma unroll
   for (int c = 0; c < 3; c++) {
-    c9[c] = t2[c] * iJ;
-    c9[3 + c] = -t1[c] * iJ;
-    c9[6 + c] = (t1[c] - t2[c]) * iJ;
+    g[3 + c] = t2[c] * iJ;
+    g[6 + c] = -t1[c] * iJ;
+    g[9 + c] = (t1[c] - t2[c]) * iJ;
   }
+  g[12] = Jq;
 }
 
 // elmat[ii][jj] += sum_t fac_t * f * K_{kc_t} * phi_i[ii,tec_t] * phi_j[jj,trc_t]
@@ -624,7 +834,7 @@ __global__ __launch_bounds__(256) void k_near_regular(const NearBlk* __restrict_
     P3 nx = ne;
     if (ge) {
       nx = ld3(ge + QG * ix);
-      point_curls<EI>(ge + QG * ix + 3, idp, be);
+      point_curls<EI>(ge + QG * ix, idp, be);
     }
     double si[EI::NL * EI::DIM];
     EI::shape(lx, x, be, si);
@@ -635,7 +845,7 @@ __global__ __launch_bounds__(256) void k_near_regular(const NearBlk* __restrict_
       P3 ny = nf;
       if (gf) {
         ny = ld3(gf + QG * iy);
-        point_curls<EJ>(gf + QG * iy + 3, idp, bf);
+        point_curls<EJ>(gf + QG * iy, idp, bf);
       }
       double sj[EJ::NL * EJ::DIM];
       EJ::shape(ly, y, bf, sj);
@@ -719,14 +929,16 @@ __global__ __launch_bounds__(256) void k_near_singular(const int4* __restrict__ 
     double wq = JJ * w;
     if (gp) {  // curved: the map at the element-local barycentrics of the point
       const int nm = (gk + 1) * (gk + 2) / 2;
-      double le[3], lf[3], cx[9], cy[9], jx, jy;
+      double le[3], lf[3], gx[QG], gy[QG];
       le[px[0]] = lx[0]; le[px[1]] = lx[1]; le[px[2]] = lx[2];
       lf[py[0]] = ly[0]; lf[py[1]] = ly[1]; lf[py[2]] = ly[2];
-      curved_point(gp + (size_t)e * nm * 3, gk, le[0], le[1], X, nx, jx, cx);
-      curved_point(gp + (size_t)f * nm * 3, gk, lf[0], lf[1], Y, ny, jy, cy);
-      point_curls<EI>(cx, px, be);
-      point_curls<EJ>(cy, py, bf);
-      wq = jx * jy * w;
+      curved_point(gp + (size_t)e * nm * 3, gk, le[0], le[1], X, gx);
+      curved_point(gp + (size_t)f * nm * 3, gk, lf[0], lf[1], Y, gy);
+      nx = ld3(gx);
+      ny = ld3(gy);
+      point_curls<EI>(gx, px, be);
+      point_curls<EJ>(gy, py, bf);
+      wq = gx[12] * gy[12] * w;
     }
     double si[EI::NL * EI::DIM], sj[EJ::NL * EJ::DIM];
     if (EI::PERM) {
@@ -768,6 +980,208 @@ __global__ __launch_bounds__(256) void k_near_singular(const int4* __restrict__ 
         atomic_add(&val[nb.off + (int64_t)col * nb.m + row], el[kx][ky]);
       }
     }
+  }
+}
+
+// ------------------------------------------------------- wide near field --
+// The wide evaluators' element matrices (up to 30 x 30) do not fit one
+// lane's registers: one wavefront per element pair, the entries spread over
+// the lanes (WENT each), the 4-D points taken WQ at a time.  Per step, lanes
+// 0 .. WQ-1 place a point pair, evaluate the kernel and store its per-term
+// weights (and the points' curls and Jacobians) in LDS; then every lane
+// evaluates (point, function) shape values into LDS; then every lane adds
+// its entries' products.  SING false: pair p of a near block's patch pairs
+// (256 per chunk; touching pairs skipped), the tensor rule ntq x ntq; SING
+// true: the touching pairs of one Sauter-Schwab class (src/ngbem.cpp:569-642,
+// as k_near_singular).  The block entries are added with atomics.
+constexpr int WQ = 8;
+constexpr int WENT = (NLOC_MAX * NLOC_MAX + 63) / 64;
+
+template <class KER, class EI, class EJ, bool SING>
+__global__ __launch_bounds__(64) void k_near_wide(const NearBlk* __restrict__ blks, const int2* __restrict__ chunks,
+                                                  const int* __restrict__ pIb, const int* __restrict__ pJb,
+                                                  const int4* __restrict__ pairs, const double* __restrict__ rule,
+                                                  int Q, const double* __restrict__ geo, const int4* __restrict__ vid,
+                                                  const double* __restrict__ quad, const double* __restrict__ sdat,
+                                                  const double* __restrict__ trig, int ntq,
+                                                  const int* __restrict__ dofs_t, const int* __restrict__ dofs_r,
+                                                  const int* __restrict__ ipt, const int* __restrict__ ipr, KER ker,
+                                                  const double* __restrict__ qg, const double* __restrict__ gp,
+                                                  int gk, typename KER::T* __restrict__ val) {
+  typedef typename KER::T T;
+  constexpr int NT = KER::NTERM, DI = EI::DIM, DJ = EJ::DIM;
+  __shared__ T s_kw[WQ][NT];
+  __shared__ double s_lx[WQ][3], s_ly[WQ][3], s_gx[WQ][10], s_gy[WQ][10];
+  __shared__ double s_si[WQ][NLOC_MAX * DI], s_sj[WQ][NLOC_MAX * DJ];
+  const int lane = threadIdx.x;
+  int bi, e, f, px[3] = {0, 1, 2}, py[3] = {0, 1, 2};
+  if constexpr (SING) {
+    const int4 pr = pairs[blockIdx.x];
+    bi = pr.x;
+    e = pr.y;
+    f = pr.z;
+#pragma unroll
+    for (int k = 0; k < 3; k++) {
+      px[k] = (pr.w >> (2 * k)) & 3;
+      py[k] = (pr.w >> (6 + 2 * k)) & 3;
+    }
+  } else {
+    const int2 ch = chunks[blockIdx.x >> 8];
+    bi = ch.x;
+    const NearBlk b0 = blks[bi];
+    const int p = ch.y + (int)(blockIdx.x & 255);
+    if (p >= b0.pm * b0.pn) return;  // the whole workgroup
+    const int ja = p / b0.pm, ia = p - ja * b0.pm;
+    e = pIb[b0.pI + ia];
+    f = pJb[b0.pJ + ja];
+    const int4 ve = vid[e], vf = vid[f];
+    if (ve.x == vf.x || ve.x == vf.y || ve.x == vf.z || ve.y == vf.x || ve.y == vf.y || ve.y == vf.z ||
+        ve.z == vf.x || ve.z == vf.y || ve.z == vf.z)
+      return;  // touching: the SING launches
+  }
+  const NearBlk nb = blks[bi];
+  const double* ge = geo + (size_t)e * GEO;
+  const double* gf = geo + (size_t)f * GEO;
+  ker.kappa = ge[13];
+  double be[EI::NB], bf[EJ::NB];
+  EI::load(sdat + (size_t)e * SDS, px, be);
+  EJ::load(sdat + (size_t)f * SDS, py, bf);
+  const int ni = EI::nl(be), nj = EJ::nl(bf), nent = ni * nj;
+  const int nq = SING ? Q : ntq * ntq;
+  T el[WENT];
+#pragma unroll
+  for (int r = 0; r < WENT; r++) el[r] = zero_<T>();
+  for (int q0 = 0; q0 < nq; q0 += WQ) {
+    const int nqc = min(WQ, nq - q0);
+    __syncthreads();
+    if (lane < nqc) {
+      const int q = q0 + lane;
+      P3 X, Y, nx, ny;
+      double wq, gx[QG], gy[QG];
+      if constexpr (SING) {
+        const double* r = rule + 5 * q;
+        const double lxs[3] = {1.0 - r[0] - r[1], r[0], r[1]}, lys[3] = {1.0 - r[2] - r[3], r[2], r[3]};
+        double le[3], lf[3];
+#pragma unroll
+        for (int k = 0; k < 3; k++) {
+          le[px[k]] = lxs[k];
+          lf[py[k]] = lys[k];
+        }
+        if (gp) {
+          const int nm = (gk + 1) * (gk + 2) / 2;
+          curved_point(gp + (size_t)e * nm * 3, gk, le[0], le[1], X, gx);
+          curved_point(gp + (size_t)f * nm * 3, gk, lf[0], lf[1], Y, gy);
+          wq = gx[12] * gy[12] * r[4];
+        } else {
+          X = P3{le[0] * ge[0] + le[1] * ge[3] + le[2] * ge[6], le[0] * ge[1] + le[1] * ge[4] + le[2] * ge[7],
+                 le[0] * ge[2] + le[1] * ge[5] + le[2] * ge[8]};
+          Y = P3{lf[0] * gf[0] + lf[1] * gf[3] + lf[2] * gf[6], lf[0] * gf[1] + lf[1] * gf[4] + lf[2] * gf[7],
+                 lf[0] * gf[2] + lf[1] * gf[5] + lf[2] * gf[8]};
+          wq = ge[12] * gf[12] * r[4];
+        }
+#pragma unroll
+        for (int k = 0; k < 3; k++) {
+          s_lx[lane][k] = le[k];
+          s_ly[lane][k] = lf[k];
+        }
+      } else {
+        const int ix = q / ntq, iy = q - (q / ntq) * ntq;
+        const double* qx = quad + ((size_t)e * ntq + ix) * 4;
+        const double* qy = quad + ((size_t)f * ntq + iy) * 4;
+        X = ld3(qx);
+        Y = ld3(qy);
+        wq = qx[3] * qy[3];
+        if (qg) {
+          for (int k = 0; k < QG; k++) {
+            gx[k] = qg[((size_t)e * ntq + ix) * QG + k];
+            gy[k] = qg[((size_t)f * ntq + iy) * QG + k];
+          }
+        }
+        const double lxs[3] = {trig[3 * ix], trig[3 * ix + 1], 1.0 - trig[3 * ix] - trig[3 * ix + 1]};
+        const double lys[3] = {trig[3 * iy], trig[3 * iy + 1], 1.0 - trig[3 * iy] - trig[3 * iy + 1]};
+#pragma unroll
+        for (int k = 0; k < 3; k++) {
+          s_lx[lane][k] = lxs[k];
+          s_ly[lane][k] = lys[k];
+        }
+      }
+      const bool cur = SING ? gp != nullptr : qg != nullptr;
+      if (cur) {
+        nx = ld3(gx);
+        ny = ld3(gy);
+        for (int k = 0; k < 9; k++) {
+          s_gx[lane][k] = gx[3 + k];
+          s_gy[lane][k] = gy[3 + k];
+        }
+        s_gx[lane][9] = gx[12];
+        s_gy[lane][9] = gy[12];
+      } else {
+        nx = ld3(ge + 9);
+        ny = ld3(gf + 9);
+        for (int k = 0; k < 9; k++) {
+          s_gx[lane][k] = be[k];
+          s_gy[lane][k] = bf[k];
+        }
+        s_gx[lane][9] = be[13];
+        s_gy[lane][9] = bf[13];
+      }
+      T kv[KER::NC];
+      ker.eval(X, Y, nx, ny, kv);
+#pragma unroll
+      for (int t = 0; t < NT; t++) {
+        const TermC tm = ker.term(t);
+        s_kw[lane][t] = rmul(kv[tm.kc], tm.fac * wq);
+      }
+    }
+    __syncthreads();
+    // shape values of the (point, function) items
+    for (int it = lane; it < nqc * (ni + nj); it += 64) {
+      const bool side_i = it < nqc * ni;
+      const int loc = side_i ? it : it - nqc * ni;
+      const int n = side_i ? ni : nj;
+      const int q = loc / n, l = loc - (loc / n) * n;
+      double b[14];
+      const double* src = side_i ? be : bf;
+      const double* g = side_i ? s_gx[q] : s_gy[q];
+      for (int k = 0; k < 9; k++) b[k] = g[k];
+      b[9] = src[9];
+      b[10] = src[10];
+      b[11] = src[11];
+      b[12] = src[12];
+      b[13] = g[9];
+      if (side_i) EI::one(l, s_lx[q], b, &s_si[q][l * DI]);
+      else EJ::one(l, s_ly[q], b, &s_sj[q][l * DJ]);
+    }
+    __syncthreads();
+#pragma unroll
+    for (int r = 0; r < WENT; r++) {
+      const int E = lane + 64 * r;
+      if (E >= nent) break;
+      const int ii = E % ni, jj = E / ni;
+      T acc = el[r];
+      for (int q = 0; q < nqc; q++)
+#pragma unroll
+        for (int t = 0; t < NT; t++) {
+          const TermC tm = ker.term(t);
+          acc += rmul(s_kw[q][t], s_si[q][ii * DI + tm.tec] * s_sj[q][jj * DJ + tm.trc]);
+        }
+      el[r] = acc;
+    }
+  }
+#pragma unroll
+  for (int r = 0; r < WENT; r++) {
+    const int E = lane + 64 * r;
+    if (E >= nent) break;
+    const int ii = E % ni, jj = E / ni;
+    const int di = dofs_t[DOF_STRIDE * e + ii];
+    if (di < 0) continue;
+    const int row = ipt[di] - nb.r0;
+    if (row < 0 || row >= nb.m) continue;
+    const int dj = dofs_r[DOF_STRIDE * f + jj];
+    if (dj < 0) continue;
+    const int col = ipr[dj] - nb.c0;
+    if (col < 0 || col >= nb.n) continue;
+    atomic_add(&val[nb.off + (int64_t)col * nb.m + row], el[r]);
   }
 }
 
@@ -1325,6 +1739,28 @@ struct ProjTask {
 
 constexpr int PROJ_CHUNK = 2048;  // outputs per workgroup
 
+// component comp of local function kl at a point
+template <class EV>
+__device__ __forceinline__ double shape_comp(int kl, int comp, const double* lam, P3 X, const double* bb) {
+  if constexpr (EV::WIDE) {
+    double o[EV::DIM];
+    EV::one(kl, lam, bb, o);
+    double r = o[0];
+#pragma unroll
+    for (int d = 1; d < EV::DIM; d++)
+      if (d == comp) r = o[d];
+    return r;
+  } else {
+    double v[EV::NL * EV::DIM];
+    EV::shape(lam, X, bb, v);
+    double r = 0.0;
+#pragma unroll
+    for (int i = 0; i < EV::NL * EV::DIM; i++)
+      if (i == kl * EV::DIM + comp) r = v[i];
+    return r;
+  }
+}
+
 // one workgroup per (task, chunk of PROJ_CHUNK outputs); outputs are Ut
 // (R x m, ii fastest) followed by V (R x n, jj fastest)
 template <class KER, class EI, class EJ>
@@ -1377,19 +1813,10 @@ __global__ __launch_bounds__(256) void k_project(const ProjTask* __restrict__ ta
         const double* qp = quad + ((size_t)e * ntq + q) * 4;
         const P3 X = ld3(qp);
         if (qg) {  // curved: the curls at the point
-          if (isU) point_curls<EI>(qg + ((size_t)e * ntq + q) * QG + 3, idp, bb);
-          else point_curls<EJ>(qg + ((size_t)e * ntq + q) * QG + 3, idp, bb);
+          if (isU) point_curls<EI>(qg + ((size_t)e * ntq + q) * QG, idp, bb);
+          else point_curls<EJ>(qg + ((size_t)e * ntq + q) * QG, idp, bb);
         }
-        double sh;
-        if (isU) {
-          double v[EI::NL * EI::DIM];
-          EI::shape(lam, X, bb, v);
-          sh = v[kl * EI::DIM + comp];
-        } else {
-          double v[EJ::NL * EJ::DIM];
-          EJ::shape(lam, X, bb, v);
-          sh = v[kl * EJ::DIM + comp];
-        }
+        const double sh = isU ? shape_comp<EI>(kl, comp, lam, X, bb) : shape_comp<EJ>(kl, comp, lam, X, bb);
         s += rmul(W[a * ntq + q], sh * qp[3]);
       }
     }
@@ -1448,7 +1875,6 @@ __global__ __launch_bounds__(256) void k_project_dof(const ProjTask* __restrict_
         if (isU) EI::load(sdat + (size_t)el[k2] * SDS, idp, bb);
         else EJ::load(sdat + (size_t)el[k2] * SDS, idp, bb);
       }
-      const int idx = kl[k2] * (isU ? EI::DIM : EJ::DIM) + comp;
 #pragma unroll
       for (int q = 0; q < PROJ_P0_NTQ; q++) {
         w[k2][q] = 0.0;
@@ -1457,23 +1883,10 @@ __global__ __launch_bounds__(256) void k_project_dof(const ProjTask* __restrict_
           const double* qp = quad + ((size_t)el[k2] * ntq + q) * 4;
           const P3 X = ld3(qp);
           if (qg) {
-            if (isU) point_curls<EI>(qg + ((size_t)el[k2] * ntq + q) * QG + 3, idp, bb);
-            else point_curls<EJ>(qg + ((size_t)el[k2] * ntq + q) * QG + 3, idp, bb);
+            if (isU) point_curls<EI>(qg + ((size_t)el[k2] * ntq + q) * QG, idp, bb);
+            else point_curls<EJ>(qg + ((size_t)el[k2] * ntq + q) * QG, idp, bb);
           }
-          double sh = 0.0;
-          if (isU) {
-            double v[EI::NL * EI::DIM];
-            EI::shape(lam, X, bb, v);
-#pragma unroll
-            for (int i = 0; i < EI::NL * EI::DIM; i++)
-              if (i == idx) sh = v[i];
-          } else {
-            double v[EJ::NL * EJ::DIM];
-            EJ::shape(lam, X, bb, v);
-#pragma unroll
-            for (int i = 0; i < EJ::NL * EJ::DIM; i++)
-              if (i == idx) sh = v[i];
-          }
+          const double sh = isU ? shape_comp<EI>(kl[k2], comp, lam, X, bb) : shape_comp<EJ>(kl[k2], comp, lam, X, bb);
           w[k2][q] = sh * qp[3];
         }
       }
@@ -2199,8 +2612,20 @@ static void dispatch_kernel(int id, double kappa, F&& f) {
 template <class F>
 static void dispatch_full(const Operator& op, F&& f) {
   const int st = op.test->kind, sr = op.trial->kind;
+  // the wide evaluators: orders above the narrow ones, and edge spaces on
+  // curved meshes (their functions are written in the per-point curls)
+  const bool wide = is_wide_kind(st) || is_wide_kind(sr) ||
+                    (op.geo_order > 1 && (is_edge_kind(st) || is_edge_kind(sr)));
   dispatch_kernel(op.kernel, op.kappa, [&](auto ker) {
     using KER = decltype(ker);
+    if (wide) {
+      if constexpr (std::is_same<KER, KLapHS>::value) f(ker, EvWide<0, EV_ROT>{}, EvWide<1, EV_ROT>{});
+      else if constexpr (std::is_same<KER, KHelHS>::value) f(ker, EvWide<0, EV_HELM>{}, EvWide<1, EV_HELM>{});
+      else if constexpr (std::is_same<KER, KMaxSL>::value) f(ker, EvWide<0, EV_MAXW>{}, EvWide<1, EV_MAXW>{});
+      else if constexpr (std::is_same<KER, KMaxDL>::value) f(ker, EvWide<0, EV_TRACE3>{}, EvWide<1, EV_TRACE3>{});
+      else f(ker, EvWide<0, EV_ID>{}, EvWide<1, EV_ID>{});
+      return;
+    }
     const bool hi = is_higher_kind(st) || is_higher_kind(sr);
     const bool big = nloc_of_kind(st) > 6 || nloc_of_kind(sr) > 6;  // order 3: 10 rows
     if constexpr (std::is_same<KER, KLapHS>::value) {
@@ -2267,6 +2692,7 @@ void setup_device(Operator& op, hipStream_t s) {
           double* g = &qgeo[((size_t)e * ntq + q) * QG];
           m.geom(e, op.trig.xi[q], op.trig.eta[q], x, n, Jq, g + 3);
           g[0] = n[0]; g[1] = n[1]; g[2] = n[2];
+          g[12] = Jq;
           o[3] = Jq * op.trig.w[q];
         } else {
           m.point(e, op.trig.xi[q], op.trig.eta[q], x);
@@ -2288,6 +2714,7 @@ void setup_device(Operator& op, hipStream_t s) {
       sd[21] = op.test->kind;
       sd[22] = op.trial->kind;
       for (int k = 0; k < 3; k++) sd[23 + k] = m.tri[3 * e + k] < m.tri[3 * e + (k + 1) % 3] ? 1.0 : -1.0;
+      sd[26] = m.J[e];
     }
   };
   {
@@ -2500,8 +2927,34 @@ void assemble_near(Operator& op, hipStream_t s) {
       }
     };
     pt_.mark("uploads + alloc");
-    if constexpr (!EI::MULTI && !EJ::MULTI) launch(std::true_type{});
-    else launch(std::false_type{});
+    if constexpr (EI::WIDE) {
+      BEM_HIP(hipEventRecord(tev[0], s));
+      if (chunks.size() >= ((size_t)1 << 23)) throw std::runtime_error("near field too large for one launch grid");
+      if (!chunks.empty())
+        hipLaunchKernelGGL((k_near_wide<KER, EI, EJ, false>), dim3((unsigned)(chunks.size() * 256)), dim3(64), 0, s,
+                           op.d_near_blk.as<NearBlk>(), op.d_near_chunks.as<int2>(), pIb, pJb, (const int4*)nullptr,
+                           (const double*)nullptr, 0, op.d_geo.as<double>(), op.d_vid.as<int4>(),
+                           op.d_quad.as<double>(), op.d_sdat.as<double>(), op.d_trig.as<double>(), ntq,
+                           op.d_dofs_test.as<int>(), op.d_dofs_trial.as<int>(), op.d_iperm_test.as<int>(),
+                           op.d_iperm_trial.as<int>(), ker, op.d_qgeo.as<double>(), op.d_gpoly.as<double>(),
+                           op.geo_order, val);
+      BEM_HIP(hipEventRecord(tev[1], s));
+      for (int c = 0; c < 3; c++) {
+        const int np = (int)pairs[c].size();
+        if (!np) continue;
+        hipLaunchKernelGGL((k_near_wide<KER, EI, EJ, true>), dim3((unsigned)np), dim3(64), 0, s,
+                           op.d_near_blk.as<NearBlk>(), (const int2*)nullptr, pIb, pJb, op.d_spairs[c].as<int4>(),
+                           op.d_ss[c].as<double>(), op.ss[c].size(), op.d_geo.as<double>(), op.d_vid.as<int4>(),
+                           op.d_quad.as<double>(), op.d_sdat.as<double>(), op.d_trig.as<double>(), ntq,
+                           op.d_dofs_test.as<int>(), op.d_dofs_trial.as<int>(), op.d_iperm_test.as<int>(),
+                           op.d_iperm_trial.as<int>(), ker, op.d_qgeo.as<double>(), op.d_gpoly.as<double>(),
+                           op.geo_order, val);
+      }
+    } else if constexpr (!EI::MULTI && !EJ::MULTI) {
+      launch(std::true_type{});
+    } else {
+      launch(std::false_type{});
+    }
     BEM_HIP(hipEventRecord(tev[2], s));
   });
   BEM_HIP(hipGetLastError());
@@ -3405,9 +3858,7 @@ __global__ __launch_bounds__(256) void k_pot_src(const int* __restrict__ elist, 
   double bb[EJ::NB > 0 ? EJ::NB : 1];
   EJ::load(sdat + (size_t)e * SDS, idp, bb);
   const double* gq = qg ? qg + ((size_t)e * ntq + q) * QG : nullptr;  // curved: per-point geometry
-  if (gq) point_curls<EJ>(gq + 3, idp, bb);
-  double v[EJ::NL * EJ::DIM];
-  EJ::shape(lam, ld3(qp), bb, v);
+  if (gq) point_curls<EJ>(gq, idp, bb);
   PotSrc<T, EJ::DIM> o;
   for (int k = 0; k < 3; k++) {
     o.y[k] = qp[k];
@@ -3415,15 +3866,31 @@ __global__ __launch_bounds__(256) void k_pot_src(const int* __restrict__ elist, 
   }
   o.w = qp[3];
   o.pad = 0.0;
+  if constexpr (EJ::WIDE) {
 #pragma unroll
-  for (int c = 0; c < EJ::DIM; c++) {
-    T a = zero_<T>();
-#pragma unroll
-    for (int l = 0; l < EJ::NL; l++) {
+    for (int c = 0; c < EJ::DIM; c++) o.v[c] = zero_<T>();
+    const int nl = EJ::nl(bb);
+    for (int l = 0; l < nl; l++) {
       const int d = dofs[DOF_STRIDE * (size_t)e + l];
-      if (d >= 0) a += rmul(coef[d], dscale ? v[l * EJ::DIM + c] * dscale[d] : v[l * EJ::DIM + c]);
+      if (d < 0) continue;
+      double v[EJ::DIM];
+      EJ::one(l, lam, bb, v);
+#pragma unroll
+      for (int c = 0; c < EJ::DIM; c++) o.v[c] += rmul(coef[d], dscale ? v[c] * dscale[d] : v[c]);
     }
-    o.v[c] = a;
+  } else {
+    double v[EJ::NL * EJ::DIM];
+    EJ::shape(lam, ld3(qp), bb, v);
+#pragma unroll
+    for (int c = 0; c < EJ::DIM; c++) {
+      T a = zero_<T>();
+#pragma unroll
+      for (int l = 0; l < EJ::NL; l++) {
+        const int d = dofs[DOF_STRIDE * (size_t)e + l];
+        if (d >= 0) a += rmul(coef[d], dscale ? v[l * EJ::DIM + c] * dscale[d] : v[l * EJ::DIM + c]);
+      }
+      o.v[c] = a;
+    }
   }
   src[idx] = o;
 }

@@ -221,10 +221,10 @@ void Space::setup(const Mesh* m, int k, const unsigned char* elmask) {
     ndof = m->nv;
     nloc = 3;
     dofs.assign(m->tri.begin(), m->tri.end());
-  } else if (k == SP_P2 || k == SP_P3) {
+  } else if (k == SP_P2 || k == SP_P3 || k == SP_P4) {
     // vertex dofs, then p - 1 dofs per edge (v_l, v_l+1), edges in sorted
-    // (min, max) order after the vertices, then (order 3) one per face
-    const int pe = k == SP_P2 ? 1 : 2;
+    // (min, max) order after the vertices, then (p - 1)(p - 2) / 2 per face
+    const int pe = k == SP_P2 ? 1 : (k == SP_P3 ? 2 : 3), nf = (pe * (pe - 1)) / 2;
     nloc = nloc_of_kind(k);
     auto key = [m](int e, int l) {
       const int a = m->tri[3 * e + l], b = m->tri[3 * e + (l + 1) % 3];
@@ -237,7 +237,7 @@ void Space::setup(const Mesh* m, int k, const unsigned char* elmask) {
     std::sort(uk.begin(), uk.end());
     uk.erase(std::unique(uk.begin(), uk.end()), uk.end());
     const int ne = (int)uk.size();
-    ndof = m->nv + pe * ne + (k == SP_P3 ? m->nt : 0);
+    ndof = m->nv + pe * ne + nf * m->nt;
     dofs.resize((size_t)nloc * m->nt);
     sign.resize(3 * (size_t)m->nt);
     for (int e = 0; e < m->nt; e++) {
@@ -248,14 +248,15 @@ void Space::setup(const Mesh* m, int k, const unsigned char* elmask) {
         for (int q = 0; q < pe; q++) d[3 + pe * l + q] = m->nv + pe * ed + q;
         sign[3 * e + l] = m->tri[3 * e + l] < m->tri[3 * e + (l + 1) % 3] ? 1 : -1;
       }
-      if (k == SP_P3) d[9] = m->nv + pe * ne + e;
+      for (int j = 0; j < nf; j++) d[3 + 3 * pe + j] = m->nv + pe * ne + nf * e + j;
     }
   } else {
     // edges sorted by (min vertex, max vertex); local dof k of a triangle is
-    // its edge (v_k, v_{k+1}), signed +1 when v_k < v_{k+1}; order 1: local
-    // 2k / 2k+1 = the edge's order-0 / bubble function, global edge / ne + edge
-    const bool o1 = k == SP_RWG1 || k == SP_NED1;
-    nloc = o1 ? 6 : 3;
+    // its edge (v_k, v_{k+1}), signed +1 when v_k < v_{k+1}; order p: local
+    // k (p+1) + i = the edge's function i (0: the order-0 one), global
+    // i ne + edge, then the element's (p+1)(p-1) interior functions
+    const int po = edge_order_of_kind(k), npe = po + 1, nint = po > 1 ? (po + 1) * (po - 1) : 0;
+    nloc = nloc_of_kind(k);
     std::vector<long long> keys;
     keys.reserve(3 * (size_t)m->nt);
     auto key = [m](int e, int l) {
@@ -268,25 +269,22 @@ void Space::setup(const Mesh* m, int k, const unsigned char* elmask) {
     std::sort(uk.begin(), uk.end());
     uk.erase(std::unique(uk.begin(), uk.end()), uk.end());
     const int ne = (int)uk.size();
-    ndof = o1 ? 2 * ne : ne;
-    edges.resize(2 * (size_t)ndof);
-    for (int i = 0; i < ndof; i++) {
+    ndof = npe * ne + nint * m->nt;
+    edges.assign(2 * (size_t)ndof, -1);  // interior dofs: no edge
+    for (int i = 0; i < npe * ne; i++) {
       edges[2 * i] = (int)(uk[i % ne] / m->nv);
       edges[2 * i + 1] = (int)(uk[i % ne] % m->nv);
     }
     dofs.resize((size_t)nloc * m->nt);
     sign.resize(3 * (size_t)m->nt);
-    for (int e = 0; e < m->nt; e++)
+    for (int e = 0; e < m->nt; e++) {
       for (int l = 0; l < 3; l++) {
         const int ed = (int)(std::lower_bound(uk.begin(), uk.end(), key(e, l)) - uk.begin());
-        if (o1) {
-          dofs[6 * e + 2 * l] = ed;
-          dofs[6 * e + 2 * l + 1] = ne + ed;
-        } else {
-          dofs[3 * e + l] = ed;
-        }
+        for (int i = 0; i < npe; i++) dofs[(size_t)nloc * e + npe * l + i] = i * ne + ed;
         sign[3 * e + l] = m->tri[3 * e + l] < m->tri[3 * e + (l + 1) % 3] ? 1 : -1;
       }
+      for (int j = 0; j < nint; j++) dofs[(size_t)nloc * e + 3 * npe + j] = npe * ne + nint * e + j;
+    }
   }
   if (!region.empty()) {
     // a space on part of the mesh (NGSolve's definedon, compressed): keep the

@@ -27,21 +27,40 @@ static const double kPi = 3.14159265358979323846;
 // n x grad(lambda_a lambda_b) of the edge bubble, for HCurl its gradient
 // grad(lambda_a lambda_b); local dofs interleaved (2l: order 0, 2l+1: the
 // bubble's), global: the order-0 dofs (edge index), then ne + edge index
+// SP_RWG2..4 / SP_NED2..4: orders 2-4 (complete polynomial tangential
+// fields of degree p, (p+1)(p+2) per element): per edge p + 1 functions (local
+// l (p+1) + i, global i ne + edge), then (p+1)(p-1) interior functions per
+// element (local 3 (p+1) + j, global (p+1) ne + e (p+1)(p-1) + j); SP_P4: H1
+// order 4 (three functions per edge, three face bubbles)
 enum SpaceKind {
   SP_P0 = 0, SP_P1 = 1, SP_RWG = 2, SP_NED = 3, SP_L2_1 = 4, SP_L2_2 = 5, SP_P2 = 6, SP_L2_3 = 7, SP_P3 = 8,
-  SP_RWG1 = 9, SP_NED1 = 10
+  SP_RWG1 = 9, SP_NED1 = 10, SP_RWG2 = 11, SP_RWG3 = 12, SP_RWG4 = 13, SP_NED2 = 14, SP_NED3 = 15, SP_NED4 = 16,
+  SP_P4 = 17
 };
-inline bool is_edge_kind(int k) { return k == SP_RWG || k == SP_NED || k == SP_RWG1 || k == SP_NED1; }
-inline bool is_hdiv_kind(int k) { return k == SP_RWG || k == SP_RWG1; }
-inline bool is_hcurl_kind(int k) { return k == SP_NED || k == SP_NED1; }
-inline bool is_h1_kind(int k) { return k == SP_P1 || k == SP_P2 || k == SP_P3; }
+inline bool is_hdiv_kind(int k) { return k == SP_RWG || k == SP_RWG1 || (k >= SP_RWG2 && k <= SP_RWG4); }
+inline bool is_hcurl_kind(int k) { return k == SP_NED || k == SP_NED1 || (k >= SP_NED2 && k <= SP_NED4); }
+inline bool is_edge_kind(int k) { return is_hdiv_kind(k) || is_hcurl_kind(k); }
+inline bool is_h1_kind(int k) { return k == SP_P1 || k == SP_P2 || k == SP_P3 || k == SP_P4; }
 inline bool is_higher_kind(int k) { return k >= SP_L2_1 && !is_edge_kind(k); }  // the generic scalar evaluators
-inline int nloc_of_kind(int k) {
-  return (k == SP_P0) ? 1 : (k == SP_L2_2 || k == SP_P2 || k == SP_RWG1 || k == SP_NED1) ? 6
-                          : (k == SP_L2_3 || k == SP_P3) ? 10 : 3;
+// polynomial order of an edge space kind (0 for order 0)
+inline int edge_order_of_kind(int k) {
+  if (k == SP_RWG1 || k == SP_NED1) return 1;
+  if (k >= SP_RWG2 && k <= SP_RWG4) return 2 + k - SP_RWG2;
+  if (k >= SP_NED2 && k <= SP_NED4) return 2 + k - SP_NED2;
+  return 0;
 }
-constexpr int NLOC_MAX = 10;   // local dofs per element, at most (H1 / SurfaceL2 order 3)
-constexpr int DOF_STRIDE = 16;  // element stride of the device dof tables
+// the kinds that only the wide (one function at a time) evaluators handle
+inline bool is_wide_kind(int k) { return k >= SP_RWG2; }
+inline int nloc_of_kind(int k) {
+  if (is_edge_kind(k)) {
+    const int p = edge_order_of_kind(k);
+    return p == 0 ? 3 : (p + 1) * (p + 2);
+  }
+  if (k == SP_P4) return 15;
+  return (k == SP_P0) ? 1 : (k == SP_L2_2 || k == SP_P2) ? 6 : (k == SP_L2_3 || k == SP_P3) ? 10 : 3;
+}
+constexpr int NLOC_MAX = 30;   // local dofs per element, at most (edge spaces of order 4)
+constexpr int DOF_STRIDE = 32;  // element stride of the device dof tables
 // EV_MAXW: [phi, div phi] (DiffOpMaxwellNew on HDivSurface, DiffOpMaxwell on
 // HCurl); EV_TRACE3: vector trace (HDivSurface BND, RotatedTrace on HCurl)
 enum EvalKind { EV_ID = 0, EV_ROT = 1, EV_HELM = 2, EV_MAXW = 3, EV_TRACE3 = 4 };

@@ -115,6 +115,10 @@ class Mesh:
     def geom(self, xi, eta):
         """Positions, unit normals and Jacobians at the reference points (xi,
         eta) of every element: (nt, q, 3), (nt, q, 3), (nt, q)."""
+        return self.frames(xi, eta)[:3]
+
+    def frames(self, xi, eta):
+        """geom plus the tangents t1 = dX/dxi, t2 = dX/deta, (nt, q, 3) each."""
         xi, eta = np.atleast_1d(np.asarray(xi, float)), np.atleast_1d(np.asarray(eta, float))
         if self.geo_order == 1:
             p0, p1, p2 = (self.xyz[self.tri[:, k]] for k in range(3))
@@ -129,7 +133,7 @@ class Mesh:
             t2 = np.einsum("qm,emc->eqc", dy, self._gpoly)
         c = np.cross(t1, t2)
         J = np.linalg.norm(c, axis=-1)
-        return X, c / J[..., None], J
+        return X, c / J[..., None], J, t1, t2
 
     def region_name(self, r) -> str:
         return self.region_names.get(int(r), "default")

@@ -58,8 +58,6 @@ class FESpace:
             raise NotImplementedError("dual_mapping is a SurfaceL2 option")
         if dual_mapping and getattr(mesh, "geo_order", 1) > 1:
             raise NotImplementedError("dual_mapping on curved meshes (its 1 / J varies over the element)")
-        if isinstance(self, _EdgeSpace) and getattr(mesh, "geo_order", 1) > 1:
-            raise NotImplementedError(f"{type(self).__name__} on curved meshes")
         if order not in self.orders:
             raise NotImplementedError(f"{type(self).__name__} supports orders {list(self.orders)}")
         self.order = int(order)
@@ -85,7 +83,7 @@ class FESpace:
                     "bem_space_create")
         self._h = _Handle(out, L.bem_space_destroy)
         self.ndof = int(L.bem_space_ndof(out))
-        tab = np.zeros(10 * mesh.nt, dtype=np.int32)
+        tab = np.zeros(30 * mesh.nt, dtype=np.int32)  # at most 30 local dofs (edge spaces of order 4)
         nloc = int(L.bem_space_dofs(out, _capi.iptr(tab)))
         if nloc < 1:
             _capi.check(nloc, "bem_space_dofs")
@@ -183,64 +181,123 @@ class H1(FESpace):
     for order 3 the face bubble lambda_0 lambda_1 lambda_2.  Dofs: vertices,
     then p - 1 per edge (edges in sorted (min, max) order), then the faces."""
     kind = _capi.SPACE_P1
-    orders = (1, 2, 3)
+    orders = (1, 2, 3, 4)
 
     def __init__(self, mesh, order=1, complex=False, definedon=None, dirichlet=None):
         super().__init__(mesh, order, complex, definedon, dirichlet)
-        self.ikind = (None, 1, 6, 8)[self.order]
+        self.ikind = (None, 1, 6, 8, 17)[self.order]
 
     def shapes(self, bary):
-        """Shape values with every edge oriented +1 (multiply by elem_signs())."""
+        """Shape values with every edge oriented +1 (multiply by elem_signs()):
+        lambda_k; per edge (a, b) lambda_a lambda_b (lambda_b - lambda_a)^q,
+        q < p - 1; faces B, B lambda_0, B lambda_1 (B = lambda_0 lambda_1
+        lambda_2), the first (p-1)(p-2)/2."""
         l0, l1, l2 = bary[:, 0], bary[:, 1], bary[:, 2]
         v = [l0, l1, l2]
         lam = [l0, l1, l2]
         for e in range(3):
             a, b = lam[e], lam[(e + 1) % 3]
-            if self.order >= 2:
-                v.append(a * b)
-            if self.order == 3:
-                v.append(a * b * (b - a))
-        if self.order == 3:
-            v.append(l0 * l1 * l2)
+            for q in range(self.order - 1):
+                v.append(a * b * (b - a) ** q)
+        B = l0 * l1 * l2
+        v += [B, B * l0, B * l1][: (self.order - 1) * (self.order - 2) // 2]
         return np.stack(v, axis=-1)
 
     def elem_signs(self):
-        """Per element and local dof: the orientation factor of the order-3 edge
-        functions, +1 elsewhere."""
+        """Per element and local dof: the orientation factor of the odd edge
+        functions (q = 1), +1 elsewhere."""
         sg = np.ones(self.elem_dofs.shape)
-        if self.order == 3:
+        if self.order >= 3:
             t = self.mesh.tri
             o = np.where(t < np.roll(t, -1, axis=1), 1.0, -1.0)
-            sg[:, [4, 6, 8]] = o
+            pe = self.order - 1
+            sg[:, [3 + pe * e + 1 for e in range(3)]] = o
         return sg
 
 
-class _EdgeSpace(FESpace):
-    """Edge elements.  Order 0: dof = mesh edge (sorted (min, max) vertex
-    pairs); local dof l of a triangle is its edge (v_l, v_{l+1}), signed +1 when
-    v_l < v_{l+1}.  Order 1: local dofs 2l (the order-0 function of edge l) and
-    2l+1 (the edge bubble's function, below); global dofs: the order-0 ones
-    (edge index), then ne + edge index."""
+def _interior_index(p):
+    """(c, u, v) of the interior functions lambda_(c+1) lambda_(c+2) q c_c of
+    the order-p edge spaces: q = lambda_0^u lambda_1^v, u + v <= p - 2 (u
+    outer) for c = 0, 1; q = lambda_0^u lambda_1^(p-2-u) for c = 2."""
+    out = [(c, u, v) for c in (0, 1) for u in range(p - 1) for v in range(p - 1 - u)]
+    return out + [(2, u, p - 2 - u) for u in range(p - 1)]
 
-    orders = (0, 1)
+
+def _edge_fns(p, lam, sgn):
+    """The order-p edge functions in the curls c_k = n x grad lambda_k at the
+    barycentric points lam (q x 3), orientations sgn (nt x 3): weights W (nt,
+    q, nloc, 3) with phi = sum_k W[..., k] c_k, D = J div phi (nt, q, nloc),
+    and the flags of the order-0 functions (nloc,).  Per edge l = (a, b) =
+    (l, l+1), t = lambda_b - lambda_a: function 0 = s (lambda_b c_a - lambda_a
+    c_b) (J div = 2 s), function i >= 1 = n x grad(s^(i-1) lambda_a lambda_b
+    t^(i-1)) (div 0); then the interior functions g c_c (_interior_index) with
+    J div = dg/dlambda_(c+1) - dg/dlambda_(c+2)."""
+    nt, Q, npe = sgn.shape[0], lam.shape[0], p + 1
+    nloc = 3 if p == 0 else (p + 1) * (p + 2)
+    W = np.zeros((nt, Q, nloc, 3))
+    D = np.zeros((nt, Q, nloc))
+    low = np.zeros(nloc, dtype=bool)
+    for e in range(3):
+        a, b = e, (e + 1) % 3
+        s = sgn[:, e][:, None]
+        t, ab = lam[:, b] - lam[:, a], lam[:, a] * lam[:, b]
+        for i in range(npe):
+            l = e * npe + i
+            if i == 0:
+                W[:, :, l, a] = s * lam[None, :, b]
+                W[:, :, l, b] = -s * lam[None, :, a]
+                D[:, :, l] = 2.0 * s
+                low[l] = True
+                continue
+            q = i - 1
+            sq = s if q % 2 else np.ones_like(s)
+            tq, tq1 = t ** q, (t ** (q - 1) if q else np.zeros_like(t))
+            W[:, :, l, a] = sq * (lam[:, b] * tq - q * ab * tq1)[None]
+            W[:, :, l, b] = sq * (lam[:, a] * tq + q * ab * tq1)[None]
+    for j, (c, u, v) in enumerate(_interior_index(p)):
+        l = 3 * npe + j
+        a, b = (c + 1) % 3, (c + 2) % 3
+        qv = lam[:, 0] ** u * lam[:, 1] ** v
+        dq = [u * lam[:, 0] ** max(u - 1, 0) * lam[:, 1] ** v if u else np.zeros(Q),
+              v * lam[:, 0] ** u * lam[:, 1] ** max(v - 1, 0) if v else np.zeros(Q), np.zeros(Q)]
+        ab = lam[:, a] * lam[:, b]
+        W[:, :, l, c] = (ab * qv)[None]
+        dg = [ab * dq[0], ab * dq[1], ab * dq[2]]
+        dg[a] = dg[a] + lam[:, b] * qv
+        dg[b] = dg[b] + lam[:, a] * qv
+        D[:, :, l] = (dg[a] - dg[b])[None]
+    return W, D, low
+
+
+class _EdgeSpace(FESpace):
+    """Edge elements of order p = 0-4 (all tangential fields of degree p).
+    Order 0: dof = mesh edge (sorted (min, max) vertex pairs); local dof l of
+    a triangle is its edge (v_l, v_{l+1}), signed +1 when v_l < v_{l+1}.
+    Order p: per edge p + 1 functions (local l (p+1) + i, global i ne +
+    edge), then (p+1)(p-1) interior functions (local 3 (p+1) + j, global
+    (p+1) ne + e (p+1)(p-1) + j); the functions are those of _edge_fns."""
+
+    orders = (0, 1, 2, 3, 4)
 
     def __init__(self, mesh, order=0, complex=False, definedon=None, dirichlet=None):
         super().__init__(mesh, order, complex, definedon, dirichlet)
         tri = mesh.tri
+        npe = self.order + 1
         loc = np.stack([tri, np.roll(tri, -1, axis=1)], axis=2)  # nt x 3 x 2
-        if self.order == 1:
-            loc = np.repeat(loc, 2, axis=1)                      # nt x 6 x 2
+        loc = np.repeat(loc, npe, axis=1)                        # nt x 3 (p+1) x 2
         lo, hi = loc.min(2), loc.max(2)
-        self.edges = np.zeros((self.ndof, 2), dtype=np.int32)  # dof -> sorted vertex pair
-        on = self.elem_dofs >= 0
-        self.edges[self.elem_dofs[on], 0] = lo[on]
-        self.edges[self.elem_dofs[on], 1] = hi[on]
+        self.edges = np.full((self.ndof, 2), -1, dtype=np.int32)  # dof -> sorted vertex pair (-1: interior)
+        d = self.elem_dofs[:, : 3 * npe]
+        on = d >= 0
+        self.edges[d[on], 0] = lo[on]
+        self.edges[d[on], 1] = hi[on]
         e3 = np.stack([tri, np.roll(tri, -1, axis=1)], axis=2)
         self.elem_sign = np.where(e3[:, :, 0] < e3[:, :, 1], 1.0, -1.0)  # nt x 3, per local edge
 
     def rt_shapes(self, pts_bary):
-        """phi_l at barycentric points (q x 3) of every triangle: nt x q x 3(l) x 3(xyz),
-        phi_l = sign_l / J (x - p_opp(l)) (unit-flux Raviart-Thomas)."""
+        """phi_l at barycentric points (q x 3) of every flat triangle: nt x q x
+        3(l) x 3(xyz), phi_l = sign_l / J (x - p_opp(l)) (unit-flux
+        Raviart-Thomas, the order-0 functions)."""
         m = self.mesh
         P = m.xyz[m.tri]  # nt x 3 x 3
         J = 2.0 * m.areas()
@@ -249,72 +306,84 @@ class _EdgeSpace(FESpace):
         s = self.elem_sign / J[:, None]
         return s[:, None, :, None] * (X[:, :, None, :] - popp[:, None, :, :])
 
-    def bubble_curls(self, pts_bary):
-        """psi_l = n x grad(lambda_a lambda_b) for the edges (a, b) = (l, l+1):
-        nt x q x 3(l) x 3(xyz)."""
-        m = self.mesh
-        P = m.xyz[m.tri]
-        e1, e2 = P[:, 0] - P[:, 2], P[:, 1] - P[:, 2]
-        G = np.stack([e1, e2], 1)                                  # nt x 2 x 3
-        Ginv = np.linalg.inv(np.einsum("tic,tjc->tij", G, G))     # metric inverse
-        g01 = np.einsum("tij,tjc->tic", Ginv, G)                   # grad lambda_0, lambda_1
-        g = np.stack([g01[:, 0], g01[:, 1], -g01[:, 0] - g01[:, 1]], 1)  # nt x 3 x 3
-        c = np.cross(m.normals()[:, None, :], g)                  # n x grad lambda_k
-        lam = pts_bary
-        out = []
-        for l in range(3):
-            a, b = l, (l + 1) % 3
-            out.append(lam[None, :, a, None] * c[:, None, b, :] + lam[None, :, b, None] * c[:, None, a, :])
-        return np.stack(out, axis=2)
+    def frames(self, pts_bary):
+        """Positions, unit normals, Jacobians and the curls c_k = n x
+        grad_Gamma lambda_k = (t2, -t1, t1 - t2) / J at the points of every
+        element (flat or curved): (nt, q, 3), (nt, q, 3), (nt, q), (nt, q, 3, 3)."""
+        X, n, J, t1, t2 = self.mesh.frames(pts_bary[:, 0], pts_bary[:, 1])
+        C = np.stack([t2, -t1, t1 - t2], axis=2) / J[..., None, None]
+        return X, n, J, C
 
     def trace_shapes(self, pts_bary):
         """The tangential fields the Maxwell operators see, nt x q x nloc x 3:
-        HDivSurface: the basis functions; HCurl: their rotated traces n x N."""
-        phi = self.rt_shapes(pts_bary)
+        HDivSurface: the basis functions; HCurl: their rotated traces n x N
+        (-phi for the order-0 functions, phi otherwise)."""
+        _, _, _, C = self.frames(pts_bary)
+        W, _, low = _edge_fns(self.order, pts_bary, self.elem_sign)
+        t = np.einsum("tqlk,tqkc->tqlc", W, C)
         if isinstance(self, HCurl):
-            phi = -phi                                    # n x (n x phi_l)
-        if self.order == 0:
-            return phi
-        psi = self.bubble_curls(pts_bary)
-        return np.stack([phi, psi], axis=3).reshape(phi.shape[0], phi.shape[1], 6, 3)
+            t[:, :, low] *= -1.0
+        return t
+
+    def div_shapes(self, pts_bary):
+        """The scalars of the Maxwell evaluators, nt x q x nloc: HDivSurface
+        div phi, HCurl curl N (div phi for the order-0 functions, -div phi
+        otherwise)."""
+        _, _, J, _ = self.frames(pts_bary)
+        _, D, low = _edge_fns(self.order, pts_bary, self.elem_sign)
+        d = D / J[..., None]
+        if isinstance(self, HCurl):
+            d[:, :, ~low] *= -1.0
+        return d
 
     def vec_shapes(self, pts_bary):
-        """The basis functions themselves, nt x q x nloc x 3: HDivSurface phi_l
-        (and psi_l); HCurl N_l = n x phi_l (and grad(lambda_a lambda_b) =
-        -n x psi_l)."""
+        """The basis functions themselves, nt x q x nloc x 3: HDivSurface phi;
+        HCurl N = -n x (n x N)."""
         t = self.trace_shapes(pts_bary)
         if isinstance(self, HCurl):
-            n = self.mesh.normals()[:, None, None, :]
-            t = -np.cross(n, t)                           # N = -n x (n x N)
+            n = self.frames(pts_bary)[1][:, :, None, :]
+            t = -np.cross(n, t)
         return t
 
 
 class HDivSurface(_EdgeSpace):
-    """HDivSurface, orders 0-1: unit-flux Raviart-Thomas (RWG-type) functions,
-    for order 1 with the surface curls of the edge bubbles (all linear
-    tangential fields)."""
+    """HDivSurface, orders 0-4: unit-flux Raviart-Thomas (RWG-type) functions,
+    the surface curls of the oriented edge functions lambda_a lambda_b t^q and
+    the interior functions (all tangential fields of degree p)."""
     kind = _capi.SPACE_HDIV
 
     def __init__(self, mesh, order=0, complex=False, definedon=None, dirichlet=None):
         super().__init__(mesh, order, complex, definedon, dirichlet)
-        self.ikind = (2, 9)[self.order]
+        self.ikind = (2, 9, 11, 12, 13)[self.order]
 
 
 class HCurl(_EdgeSpace):
-    """HCurl, orders 0-1, on the surface: Whitney edge functions N_l = n x phi_l,
-    for order 1 with the gradients of the edge bubbles."""
+    """HCurl, orders 0-4, on the surface: Whitney edge functions N_l = n x
+    phi_l, the gradients of the oriented edge functions, and interior
+    functions whose rotated traces are those of HDivSurface."""
     kind = _capi.SPACE_HCURL
 
     def __init__(self, mesh, order=0, complex=False, definedon=None, dirichlet=None):
         super().__init__(mesh, order, complex, definedon, dirichlet)
-        self.ikind = (3, 10)[self.order]
+        self.ikind = (3, 10, 14, 15, 16)[self.order]
 
 
 def mass_matrix_coo(trial: FESpace, test: FESpace):
     """Entries of the surface mass matrix  int u . v ds  (test x trial), exact for
-    P0/P1 and the lowest-order edge spaces on flat triangles: rows, cols, values."""
+    the polynomial integrands on flat triangles (curved ones: Gauss rules of
+    a few more points): rows, cols, values."""
     m = trial.mesh
-    if getattr(m, "geo_order", 1) > 1:
+    if isinstance(trial, _EdgeSpace) or isinstance(test, _EdgeSpace):
+        if not (isinstance(trial, _EdgeSpace) and isinstance(test, _EdgeSpace)):
+            raise NotImplementedError("mass matrix between an edge space and a scalar space")
+        # collapsed Gauss rule exact for the polynomial integrands of degree 2p
+        # (flat), with 3 more points a side for the curved elements' 1 / J
+        n = max(trial.order, test.order) + 2 + (3 if getattr(m, "geo_order", 1) > 1 else 0)
+        x, y, W = _collapsed_rule(n)
+        bary = np.stack([x, y, 1.0 - x - y], axis=1)
+        J = m.geom(x, y)[2]
+        loc = np.einsum("q,tq,tqic,tqjc->tij", W, J, test.vec_shapes(bary), trial.vec_shapes(bary))
+    elif getattr(m, "geo_order", 1) > 1:
         # curved elements: the Jacobian at each point of a collapsed Gauss rule
         # of 8 x 8 points (exact to degree 15 for the polynomial part)
         x, y, W = _collapsed_rule(8)
@@ -322,19 +391,10 @@ def mass_matrix_coo(trial: FESpace, test: FESpace):
         J = m.geom(x, y)[2]
         loc = np.einsum("q,tq,qi,qj->tij", W, J, test.shapes(bary), trial.shapes(bary))
         loc = loc * test.elem_signs()[:, :, None] * trial.elem_signs()[:, None, :]
-    elif isinstance(trial, _EdgeSpace) or isinstance(test, _EdgeSpace):
-        if not (isinstance(trial, _EdgeSpace) and isinstance(test, _EdgeSpace)):
-            raise NotImplementedError("mass matrix between an edge space and a scalar space")
-        # edge-midpoint rule, exact for the quadratic integrands (orders 0-1)
-        bary = np.array([[0.5, 0.5, 0.0], [0.0, 0.5, 0.5], [0.5, 0.0, 0.5]])
-        w = m.areas()[:, None] / 3.0
-        phi_i = test.vec_shapes(bary)
-        phi_j = trial.vec_shapes(bary)
-        loc = np.einsum("tq,tqic,tqjc->tij", w, phi_i, phi_j)  # nt x nloc x nloc
     elif trial.order > 1 or test.order > 1 or isinstance(trial, SurfaceL2) and trial.order or \
             isinstance(test, SurfaceL2) and test.order:
         # higher orders: reference Gram matrix by a collapsed Gauss rule
-        # (exact to degree 7), scaled by 2 * area
+        # (exact to degree 8), scaled by 2 * area
         g, gw = np.polynomial.legendre.leggauss(5)
         g, gw = 0.5 * (g + 1.0), 0.5 * gw
         U, Vv = np.meshgrid(g, g, indexing="ij")

@@ -366,6 +366,21 @@ struct Mesh {
     for (int k = 0; k < 3; k++) n[k] = c[k] / Jq;
     metric_grads(a, b, g);
   }
+  // tangents t1 = dF/dxi, t2 = dF/deta at (xi, eta)
+  void tangents(int e, double xi, double eta, double t1[3], double t2[3]) const {
+    if (gk > 1) {
+      double X[3];
+      curved_map(e, xi, eta, X, t1, t2);
+      return;
+    }
+    const double* p0 = &xyz[3 * tri[3 * e + 0]];
+    const double* p1 = &xyz[3 * tri[3 * e + 1]];
+    const double* p2 = &xyz[3 * tri[3 * e + 2]];
+    for (int k = 0; k < 3; k++) {
+      t1[k] = p0[k] - p2[k];
+      t2[k] = p1[k] - p2[k];
+    }
+  }
   // surface gradients of the barycentric coordinates lam0=xi, lam1=eta, lam2
   // (flat element)
   void grads(int e, double g[3][3]) const {
@@ -395,9 +410,22 @@ struct Mesh {
 // n x grad(lambda_a lambda_b) (HDivSurface) or gradient grad(lambda_a lambda_b)
 // (HCurl) (local 2l+1); global dofs: the order-0 ones (edge index), then ne +
 // edge index
+// SP_RWG2..4 / SP_NED2..4: order 2-4 edge spaces (all tangential fields of
+// degree p): per edge p + 1 functions (local l (p+1) + i, global i ne + edge),
+// then (p+1)(p-1) interior functions per element (local 3 (p+1) + j, global
+// (p+1) ne + e (p+1)(p-1) + j); SP_P4: H1 order 4
 enum { SP_P0 = 0, SP_P1 = 1, SP_RWG = 2, SP_NED = 3, SP_L2_1 = 4, SP_L2_2 = 5, SP_P2 = 6, SP_L2_3 = 7, SP_P3 = 8,
-       SP_RWG1 = 9, SP_NED1 = 10 };
+       SP_RWG1 = 9, SP_NED1 = 10, SP_RWG2 = 11, SP_RWG3 = 12, SP_RWG4 = 13, SP_NED2 = 14, SP_NED3 = 15,
+       SP_NED4 = 16, SP_P4 = 17 };
 static int l2_order(int k) { return k == SP_L2_1 ? 1 : (k == SP_L2_2 ? 2 : (k == SP_L2_3 ? 3 : 0)); }
+static bool is_edge(int k) { return k == SP_RWG || k == SP_NED || (k >= SP_RWG1 && k <= SP_NED4 && k != SP_P4); }
+static bool is_curl(int k) { return k == SP_NED || k == SP_NED1 || (k >= SP_NED2 && k <= SP_NED4); }
+static int edge_order(int k) {
+  if (k == SP_RWG1 || k == SP_NED1) return 1;
+  if (k >= SP_RWG2 && k <= SP_RWG4) return 2 + k - SP_RWG2;
+  if (k >= SP_NED2 && k <= SP_NED4) return 2 + k - SP_NED2;
+  return 0;
+}
 // EV_MAXW: [phi, div] (DiffOpMaxwellNew, src/diffops.hpp:263-338; DiffOpMaxwell :170-255)
 // EV_TR3: vector trace (HDivSurface BND; RotatedTrace :62-107 on HCurl)
 enum { EV_ID = 0, EV_ROT = 1, EV_HELM = 2, EV_MAXW = 3, EV_TR3 = 4 };
@@ -429,9 +457,9 @@ struct Space {
         dofs[(size_t)e * nloc] = e;
         for (int l = 1; l < nloc; l++) dofs[(size_t)e * nloc + l] = m.nt + e * (nloc - 1) + (l - 1);
       }
-    } else if (k == SP_P2 || k == SP_P3) {  // vertices, then the edges (v_l, v_l+1) in (min, max) order, faces
-      const int pe = (k == SP_P2) ? 1 : 2;
-      nloc = 3 + 3 * pe + (pe == 2 ? 1 : 0);
+    } else if (k == SP_P2 || k == SP_P3 || k == SP_P4) {  // vertices, the edges (v_l, v_l+1) in (min, max) order, faces
+      const int pe = (k == SP_P2) ? 1 : (k == SP_P3 ? 2 : 3), nf = pe * (pe - 1) / 2;
+      nloc = 3 + 3 * pe + nf;
       std::vector<std::pair<int, int>> ed;
       for (int e = 0; e < m.nt; e++)
         for (int l = 0; l < 3; l++) {
@@ -442,7 +470,7 @@ struct Space {
       std::sort(u.begin(), u.end());
       u.erase(std::unique(u.begin(), u.end()), u.end());
       const int ne = (int)u.size();
-      ndof = m.nv + pe * ne + (pe == 2 ? m.nt : 0);
+      ndof = m.nv + pe * ne + nf * m.nt;
       dofs.resize((size_t)nloc * m.nt);
       sign.resize(3 * (size_t)m.nt);
       for (int e = 0; e < m.nt; e++) {
@@ -452,11 +480,11 @@ struct Space {
           for (int q = 0; q < pe; q++) dofs[(size_t)nloc * e + 3 + pe * l + q] = m.nv + pe * id + q;
           sign[3 * e + l] = m.tri[3 * e + l] < m.tri[3 * e + (l + 1) % 3] ? 1 : -1;
         }
-        if (pe == 2) dofs[(size_t)nloc * e + 9] = m.nv + pe * ne + e;
+        for (int j = 0; j < nf; j++) dofs[(size_t)nloc * e + 3 + 3 * pe + j] = m.nv + pe * ne + nf * e + j;
       }
     } else {  // edges numbered in (min vertex, max vertex) order; local edge l = (v_l, v_{l+1})
-      const bool o1 = (k == SP_RWG1 || k == SP_NED1);
-      nloc = o1 ? 6 : 3;
+      const int po = edge_order(k), npe = po + 1, nint = po > 1 ? (po + 1) * (po - 1) : 0;
+      nloc = 3 * npe + nint;
       std::vector<std::pair<int, int>> ed;
       for (int e = 0; e < m.nt; e++)
         for (int l = 0; l < 3; l++) {
@@ -467,20 +495,17 @@ struct Space {
       std::sort(u.begin(), u.end());
       u.erase(std::unique(u.begin(), u.end()), u.end());
       const int ne = (int)u.size();
-      ndof = o1 ? 2 * ne : ne;
+      ndof = npe * ne + nint * m.nt;
       dofs.resize((size_t)nloc * m.nt);
       sign.resize(3 * (size_t)m.nt);
-      for (int e = 0; e < m.nt; e++)
+      for (int e = 0; e < m.nt; e++) {
         for (int l = 0; l < 3; l++) {
           const int id = (int)(std::lower_bound(u.begin(), u.end(), ed[3 * e + l]) - u.begin());
-          if (o1) {
-            dofs[6 * e + 2 * l] = id;
-            dofs[6 * e + 2 * l + 1] = ne + id;
-          } else {
-            dofs[3 * e + l] = id;
-          }
+          for (int i = 0; i < npe; i++) dofs[(size_t)nloc * e + npe * l + i] = i * ne + id;
           sign[3 * e + l] = m.tri[3 * e + l] < m.tri[3 * e + (l + 1) % 3] ? 1 : -1;
         }
+        for (int j = 0; j < nint; j++) dofs[(size_t)nloc * e + 3 * npe + j] = npe * ne + nint * e + j;
+      }
     }
     if (mask) {
       // the dofs of the region's elements only (src/ngbem.cpp:44-55: bnddofs
@@ -540,16 +565,109 @@ static void dubiner(int p, double x, double y, double* out) {
   }
 }
 
-static void eval_shape(const Mesh& m, const Space& s, int ev, int e, double xi, double eta, double* out) {
+// Scalar jets on the reference triangle: value and d/dxi, d/deta.
+struct Jet {
+  double v, dx, dy;
+};
+static Jet operator-(Jet a, Jet b) { return {a.v - b.v, a.dx - b.dx, a.dy - b.dy}; }
+static Jet operator*(Jet a, Jet b) { return {a.v * b.v, a.dx * b.v + a.v * b.dx, a.dy * b.v + a.v * b.dy}; }
+static Jet operator*(double c, Jet a) { return {c * a.v, c * a.dx, c * a.dy}; }
+static Jet jpow(Jet a, int q) {
+  Jet r{1.0, 0.0, 0.0};
+  for (int i = 0; i < q; i++) r = r * a;
+  return r;
+}
+
+// Edge spaces through the reference element, as NGSolve's DiffOpMaxwellNew
+// (HDivSurface: phi = F' phi^ / J, div phi = div^ phi^ / J) and DiffOpMaxwell
+// (HCurl: n x N with N = F'^-T N^, curl N = curl^ N^ / J), src/diffops.hpp:
+// 170-338, on flat and curved elements.  Reference fields of order p in
+// lambda_0 = xi, lambda_1 = eta, lambda_2 = 1 - xi - eta (grad^ lambda_k =
+// (1,0), (0,1), (-1,-1)); per edge l = (a, b) = (l, l+1) with orientation s
+// and t = lambda_b - lambda_a:
+//   i = 0: HDivSurface phi^ = s ((xi, eta) - p^_c) (c = l + 2, p^_0 = (1, 0),
+//     p^_1 = (0, 1), p^_2 = 0, div^ 2 s); HCurl N^ = s (lambda_a grad^ lambda_b
+//     - lambda_b grad^ lambda_a) (curl^ 2 s);
+//   i >= 1: N^ = grad^ (s^(i-1) lambda_a lambda_b t^(i-1)) (curl 0);
+//   interior j: N^ = lambda_(c+1) lambda_(c+2) m grad^ lambda_c with m =
+//     lambda_0^u lambda_1^v (u + v <= p - 2, u outer) for c = 0, 1 and
+//     lambda_0^u lambda_1^(p-2-u) for c = 2;
+// and HDivSurface phi^ = (-N^_2, N^_1) (n x on the reference triangle) for
+// i >= 1 and the interior ones, div^ phi^ = -curl^ N^.
+static void piola_edge_shape(const Mesh& m, const Space& s, int ev, int e, double xi, double eta, double* out) {
+  const int p = edge_order(s.kind), dim = ev_dim(ev), npe = p + 1;
+  const bool curl = is_curl(s.kind);
+  double X[3], n[3], J, g[3][3], t1[3], t2[3];
+  m.at(e, xi, eta, X, n, J, g);
+  m.tangents(e, xi, eta, t1, t2);
+  const Jet lam[3] = {{xi, 1.0, 0.0}, {eta, 0.0, 1.0}, {1.0 - xi - eta, -1.0, -1.0}};
+  const double gh[3][2] = {{1.0, 0.0}, {0.0, 1.0}, {-1.0, -1.0}};
+  auto put = [&](int l, const double ph[2], double sc, const double Nh[2], bool use_n) {
+    double* o = out + l * dim;
+    if (!curl || !use_n) {  // contravariant: F' phi^ / J
+      for (int c = 0; c < 3; c++) o[c] = (ph[0] * t1[c] + ph[1] * t2[c]) / J;
+    } else {  // n x (F'^-T N^) with F'^-T N^ = N^_1 grad lambda_0 + N^_2 grad lambda_1
+      double N[3];
+      for (int c = 0; c < 3; c++) N[c] = Nh[0] * g[0][c] + Nh[1] * g[1][c];
+      o[0] = n[1] * N[2] - n[2] * N[1];
+      o[1] = n[2] * N[0] - n[0] * N[2];
+      o[2] = n[0] * N[1] - n[1] * N[0];
+    }
+    if (ev == EV_MAXW) o[3] = sc / J;
+  };
+  for (int l = 0; l < 3; l++) {
+    const int a = l, b = (l + 1) % 3, c = (l + 2) % 3;
+    const double sg = s.sign[3 * e + l];
+    for (int i = 0; i < npe; i++) {
+      double ph[2], Nh[2], sc;
+      if (i == 0) {
+        const double pc[3][2] = {{1.0, 0.0}, {0.0, 1.0}, {0.0, 0.0}};
+        ph[0] = sg * (xi - pc[c][0]);
+        ph[1] = sg * (eta - pc[c][1]);
+        Nh[0] = sg * (lam[a].v * gh[b][0] - lam[b].v * gh[a][0]);
+        Nh[1] = sg * (lam[a].v * gh[b][1] - lam[b].v * gh[a][1]);
+        sc = 2.0 * sg;  // div^ phi^ (HDivSurface) and curl^ N^ (HCurl)
+      } else {
+        const int q = i - 1;
+        const Jet f = ((q & 1) ? sg : 1.0) * (lam[a] * lam[b] * jpow(lam[b] - lam[a], q));
+        Nh[0] = f.dx;
+        Nh[1] = f.dy;
+        ph[0] = -Nh[1];
+        ph[1] = Nh[0];
+        sc = 0.0;
+      }
+      put(l * npe + i, ph, sc, Nh, true);
+    }
+  }
+  int j = 3 * npe;
+  for (int cc = 0; cc < 3 && p >= 2; cc++) {
+    const int a = (cc + 1) % 3, b = (cc + 2) % 3;
+    for (int u = 0; u <= p - 2; u++)
+      for (int v = 0; v + u <= p - 2; v++) {
+        if (cc == 2 && v != p - 2 - u) continue;
+        const Jet gq = lam[a] * lam[b] * jpow(lam[0], u) * jpow(lam[1], v);
+        const double Nh[2] = {gq.v * gh[cc][0], gq.v * gh[cc][1]};
+        const double cu = gq.dx * gh[cc][1] - gq.dy * gh[cc][0];  // curl^ N^
+        const double ph[2] = {-Nh[1], Nh[0]};
+        put(j++, ph, curl ? cu : -cu, Nh, true);
+      }
+  }
+}
+
+static void eval_shape(const Mesh& m, const Space& s, int ev, int e, double xi, double eta, double* out,
+                       bool piola = false) {
   if (s.kind == SP_P0) { out[0] = 1.0; return; }
   if (l2_order(s.kind) > 0) { dubiner(l2_order(s.kind), xi, eta, out); return; }
+  if (is_edge(s.kind) && (piola || m.gk > 1 || edge_order(s.kind) > 1)) {
+    piola_edge_shape(m, s, ev, e, xi, eta, out);
+    return;
+  }
   if (s.kind == SP_RWG || s.kind == SP_NED || s.kind == SP_RWG1 || s.kind == SP_NED1) {
     // RT0 with unit flux: phi_l = sign_l / J (x - p_opp(l)), div = 2 sign_l / J; the
     // rotated trace of the Whitney function of an HCurl edge is -phi_l.  Order 1
     // (rows 2l, 2l+1): phi_l, then psi_l = n x grad(lambda_a lambda_b) for the
     // edge (a, b) = (l, l+1), div 0: the HDivSurface function itself, or the
     // rotated trace n x grad(lambda_a lambda_b) of the HCurl one (curl 0)
-    if (m.gk > 1) throw std::runtime_error("oracle: edge spaces on curved meshes are not restated");
     double X[3];
     m.point(e, xi, eta, X);
     const bool o1 = (s.kind == SP_RWG1 || s.kind == SP_NED1);
@@ -587,9 +705,10 @@ static void eval_shape(const Mesh& m, const Space& s, int ev, int e, double xi, 
   m.at(e, xi, eta, Xp, n, Jp, g);
   // order 3 adds, per edge (a, b), s lambda_a lambda_b (lambda_b - lambda_a) with
   // s = +1 when the edge runs from the lower to the higher vertex number, and
-  // the face bubble lambda_0 lambda_1 lambda_2
-  const int nl = s.nloc, pe = (nl == 3) ? 0 : (nl == 6 ? 1 : 2);
-  double val[10], grad[10][3];
+  // the face bubble lambda_0 lambda_1 lambda_2; order 4 per edge lambda_a
+  // lambda_b (lambda_b - lambda_a)^2 and the face bubbles times lambda_0, lambda_1
+  const int nl = s.nloc, pe = (nl == 3) ? 0 : (nl == 6 ? 1 : (nl == 10 ? 2 : 3));
+  double val[15], grad[15][3];
   for (int k = 0; k < 3; k++) {
     val[k] = lam[k];
     for (int c = 0; c < 3; c++) grad[k][c] = g[k][c];
@@ -598,18 +717,32 @@ static void eval_shape(const Mesh& m, const Space& s, int ev, int e, double xi, 
     const int a = l, b = (l + 1) % 3, i2 = 3 + pe * l;
     val[i2] = lam[a] * lam[b];
     for (int c = 0; c < 3; c++) grad[i2][c] = lam[a] * g[b][c] + lam[b] * g[a][c];
-    if (pe == 2) {
+    if (pe >= 2) {
       const double sg = s.sign[3 * e + l];
       val[i2 + 1] = sg * lam[a] * lam[b] * (lam[b] - lam[a]);
       for (int c = 0; c < 3; c++)
         grad[i2 + 1][c] = sg * (lam[b] * lam[b] * g[a][c] + 2.0 * lam[a] * lam[b] * g[b][c] -
                                 2.0 * lam[a] * lam[b] * g[a][c] - lam[a] * lam[a] * g[b][c]);
     }
+    if (pe == 3) {  // lambda_a lambda_b t^2: t^2 grad(lambda_a lambda_b) + 2 lambda_a lambda_b t grad t
+      const double t = lam[b] - lam[a], ab = lam[a] * lam[b];
+      val[i2 + 2] = ab * t * t;
+      for (int c = 0; c < 3; c++)
+        grad[i2 + 2][c] = t * t * (lam[a] * g[b][c] + lam[b] * g[a][c]) + 2.0 * ab * t * (g[b][c] - g[a][c]);
+    }
   }
-  if (pe == 2) {
-    val[9] = lam[0] * lam[1] * lam[2];
+  if (pe >= 2) {
+    const int f0 = 3 + 3 * pe;
+    const double B = lam[0] * lam[1] * lam[2];
+    double gB[3];
     for (int c = 0; c < 3; c++)
-      grad[9][c] = lam[1] * lam[2] * g[0][c] + lam[0] * lam[2] * g[1][c] + lam[0] * lam[1] * g[2][c];
+      gB[c] = lam[1] * lam[2] * g[0][c] + lam[0] * lam[2] * g[1][c] + lam[0] * lam[1] * g[2][c];
+    val[f0] = B;
+    for (int c = 0; c < 3; c++) grad[f0][c] = gB[c];
+    for (int k = 0; k < 2 && pe == 3; k++) {  // B lambda_0, B lambda_1
+      val[f0 + 1 + k] = B * lam[k];
+      for (int c = 0; c < 3; c++) grad[f0 + 1 + k][c] = lam[k] * gB[c] + B * g[k][c];
+    }
   }
   if (ev == EV_ID) { for (int k = 0; k < nl; k++) out[k] = val[k]; return; }
   int dim = ev_dim(ev);
@@ -943,7 +1076,7 @@ static void integrate4d(const Op& op, int ei, int ej, const double* xx, const do
   const Mesh& m = *op.mesh;
   int dti = ev_dim(op.K.ev_test), dtj = ev_dim(op.K.ev_trial);
   int ni = op.test.nloc, nj = op.trial.nloc;
-  double si[40], sj[40];
+  double si[120], sj[120];
   cplx kv[3];
   for (int q = 0; q < nq; q++) {
     double X[3], Y[3], nx[3], ny[3], jx, jy, g[3][3];
@@ -1192,7 +1325,7 @@ static bool calc_aca(const Op& op, const std::vector<int>& trialdofs, const std:
   V2.assign((size_t)R * ncols, cplx(0.0));
   int ni = op.test.nloc, nj = op.trial.nloc;
   int dti = ev_dim(op.K.ev_test), dtj = ev_dim(op.K.ev_trial);
-  const int SQ = 40;  // nloc * dim <= 10 * 4
+  const int SQ = 120;  // nloc * dim <= 30 * 4
   std::vector<double> shq((size_t)nt * SQ);
   for (size_t a = 0; a < patchi.size(); a++) {
     int e = patchi[a];
@@ -1338,6 +1471,27 @@ void* orc_mesh_new(const double* xyz, int nv, const int* tri, int nt) {
 }
 void orc_mesh_free(void* m) { delete (Mesh*)m; }
 
+// a space on a mesh, for evaluating its shape functions (tests of the basis)
+void* orc_space_new(void* m, int kind) {
+  Space* s = new Space;
+  s->setup(*(const Mesh*)m, kind);
+  return s;
+}
+void orc_space_free(void* s) { delete (Space*)s; }
+int orc_space_ndof(void* s) { return ((Space*)s)->ndof; }
+int orc_space_dofs(void* s, int* out) {
+  const Space& sp = *(Space*)s;
+  std::copy(sp.dofs.begin(), sp.dofs.end(), out);
+  return sp.nloc;
+}
+// nloc x dim values of the evaluator ev (EV_*) at (xi, eta) of element e;
+// piola: the edge spaces through the reference element at every order
+int orc_space_eval(void* s, void* m, int ev, int e, double xi, double eta, int piola, double* out) {
+  const Space& sp = *(Space*)s;
+  eval_shape(*(const Mesh*)m, sp, ev, e, xi, eta, out, piola != 0);
+  return sp.nloc * ev_dim(ev);
+}
+
 // geometry of element e at (xi, eta): out = position (3), unit normal (3),
 // measure (1), surface gradients of lambda_0..2 (9)
 void orc_mesh_at(void* m, int e, double xi, double eta, double* out) {
@@ -1385,8 +1539,6 @@ void orc_gauss01(int n, double* x, double* w) {
 // with mask[e] != 0 (null: the whole mesh)
 void* orc_op_new_on(void* mesh, int kernel, double kappa, int trial_kind, int test_kind, const unsigned char* trial_mask,
                     const unsigned char* test_mask, int intorder, int leafsize, double eta, double eps) {
-  auto edge = [](int k) { return k == SP_RWG || k == SP_NED || k == SP_RWG1 || k == SP_NED1; };
-  if (((const Mesh*)mesh)->gk > 1 && (edge(trial_kind) || edge(test_kind))) return nullptr;
   Op* op = new Op;
   op->mesh = (const Mesh*)mesh;
   op->K.setup(kernel, kappa);
@@ -1564,7 +1716,7 @@ int orc_potential(void* p, const double* coef, const double* pts, const double* 
   for (int e = 0; e < m.nt; e++) use[e] = op.trial.has(e) && (!elmask || elmask[e]);
   for (int e = 0; e < m.nt; e++) {
     if (!use[e]) continue;
-    std::vector<double> sh(40);
+    std::vector<double> sh(120);
     for (int q = 0; q < nq; q++) {
       double g[3][3], jq;
       m.at(e, tr.xi[q], tr.eta[q], &Y[3 * ((size_t)e * nq + q)], &NY[3 * ((size_t)e * nq + q)], jq, g);

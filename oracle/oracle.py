@@ -18,7 +18,10 @@ _lib = None
 # kernel ids (same numbering as the product's C-ABI)
 (LAPLACE_SL, LAPLACE_DL, LAPLACE_HS, HELMHOLTZ_SL, HELMHOLTZ_DL, HELMHOLTZ_CF, HELMHOLTZ_HS, MAXWELL_SL,
  MAXWELL_DL, HELMHOLTZ_SL_DKAPPA) = range(10)
-P0, P1, HDIV, HCURL, L2_1, L2_2, P2, L2_3, P3, HDIV1, HCURL1 = range(11)  # SurfaceL2 1-3; H1 2-3; edge order 1
+# SurfaceL2 1-3; H1 2-4; edge orders 1-4
+P0, P1, HDIV, HCURL, L2_1, L2_2, P2, L2_3, P3, HDIV1, HCURL1, HDIV2, HDIV3, HDIV4, HCURL2, HCURL3, HCURL4, P4 = \
+    range(18)
+EV_ID, EV_ROT, EV_HELM, EV_MAXW, EV_TR3 = range(5)
 
 
 def build():
@@ -39,6 +42,13 @@ def lib():
         L.orc_mesh_free.argtypes = [P]
         L.orc_mesh_set_curved.argtypes = [P, ctypes.c_int, dp]
         L.orc_mesh_at.argtypes = [P, ctypes.c_int, ctypes.c_double, ctypes.c_double, dp]
+        L.orc_space_new.restype = P
+        L.orc_space_new.argtypes = [P, ctypes.c_int]
+        L.orc_space_free.argtypes = [P]
+        L.orc_space_ndof.argtypes = [P]
+        L.orc_space_dofs.argtypes = [P, ip]
+        L.orc_space_eval.argtypes = [P, P, ctypes.c_int, ctypes.c_int, ctypes.c_double, ctypes.c_double, ctypes.c_int,
+                                     dp]
         L.orc_rule4.argtypes = [ctypes.c_int, ctypes.c_int, dp, ctypes.c_int]
         L.orc_trig_rule.argtypes = [ctypes.c_int, dp, ctypes.c_int]
         L.orc_gauss01.argtypes = [ctypes.c_int, dp, dp]
@@ -149,6 +159,31 @@ class OracleMesh:
             self.h = None
 
 
+class OracleSpace:
+    """A space of the oracle on a mesh: its dofs and shape functions."""
+
+    def __init__(self, mesh, kind):
+        if not isinstance(mesh, OracleMesh):
+            mesh = OracleMesh(mesh.xyz, mesh.tri, getattr(mesh, "geo_order", 1), getattr(mesh, "nodes", None))
+        self.mesh, self.kind = mesh, kind
+        self.h = lib().orc_space_new(mesh.h, kind)
+        self.ndof = lib().orc_space_ndof(self.h)
+        out = np.zeros(30 * len(mesh.tri), dtype=np.int32)
+        nloc = lib().orc_space_dofs(self.h, _ip(out))
+        self.elem_dofs = out[: nloc * len(mesh.tri)].reshape(len(mesh.tri), nloc)
+
+    def eval(self, ev, e, xi, eta, piola=False):
+        """nloc x dim values of evaluator ev at (xi, eta) of element e."""
+        out = np.zeros(120)
+        n = lib().orc_space_eval(self.h, self.mesh.h, ev, int(e), float(xi), float(eta), int(piola), _dp(out))
+        return out[:n].reshape(self.elem_dofs.shape[1], -1)
+
+    def __del__(self):
+        if getattr(self, "h", None) and _lib is not None:
+            _lib.orc_space_free(self.h)
+            self.h = None
+
+
 class OracleOp:
     """Operator restated from the reference: tree + partition at construction;
     blocks are assembled on request (``assemble``)."""
@@ -169,14 +204,14 @@ class OracleOp:
         self.h = lib().orc_op_new_on(mesh.h, kernel, float(kappa), trial, test, mp[0], mp[1], intorder, leafsize,
                                      float(eta), float(eps))
         if not self.h:
-            raise ValueError("oracle: edge spaces on curved meshes are not restated")
+            raise ValueError("oracle: operator not created")
         a, b = ctypes.c_int(), ctypes.c_int()
         lib().orc_op_sizes(self.h, ctypes.byref(a), ctypes.byref(b))
         self.ntrial, self.ntest = a.value, b.value
 
     def elem_dofs(self, test=False):
         """Element -> dof table (nt x nloc), -1 outside the space's region."""
-        out = np.zeros(10 * len(self.mesh.tri), dtype=np.int32)
+        out = np.zeros(30 * len(self.mesh.tri), dtype=np.int32)
         nloc = lib().orc_op_dofs(self.h, int(test), _ip(out))
         return out[: nloc * len(self.mesh.tri)].reshape(len(self.mesh.tri), nloc)
 

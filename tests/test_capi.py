@@ -47,7 +47,9 @@ def test_bad_arguments_are_reported():
         _capi.check(L.bem_op_create(_capi.LAPLACE_HS, 0.0, p0.handle, p0.handle, ctypes.byref(params),
                                     ctypes.byref(out)))
     with pytest.raises(NotImplementedError):
-        nb.H1(m, order=4)
+        nb.H1(m, order=5)
+    with pytest.raises(NotImplementedError):
+        nb.HDivSurface(m, order=5)
     with pytest.raises(NotImplementedError):
         nb.SurfaceL2(m, order=4)
     with pytest.raises(RuntimeError, match="hypersingular operators need H1"):
@@ -65,11 +67,11 @@ def test_bad_arguments_are_reported():
         with pytest.raises(RuntimeError, match=msg):
             _capi.check(L.bem_op_create(kern, 1.0, tr.handle, te.handle, ctypes.byref(params), ctypes.byref(out)))
     with pytest.raises(NotImplementedError):
-        nb.HDivSurface(m, order=2)
-    with pytest.raises(NotImplementedError):
-        nb.HCurl(m, order=2)
-    rc = L.bem_space_create(m._bem_handle.ptr, _capi.SPACE_HDIV, 2, ctypes.byref(out))
-    assert rc == -1 and b"HDivSurface orders 0-1" in L.bem_last_error()
+        nb.HCurl(m, order=5)
+    rc = L.bem_space_create(m._bem_handle.ptr, _capi.SPACE_HDIV, 5, ctypes.byref(out))
+    assert rc == -1 and b"HDivSurface orders 0-4" in L.bem_last_error()
+    rc = L.bem_space_create(m._bem_handle.ptr, _capi.SPACE_P1, 5, ctypes.byref(out))
+    assert rc == -1 and b"H1 orders 1-4" in L.bem_last_error()
     # the Maxwell single layer takes one space kind on both sides
     hd1 = nb.HDivSurface(m, order=1)
     with pytest.raises(RuntimeError, match="Maxwell single layer needs"):

@@ -130,11 +130,7 @@ def test_curve_rules():
         m.Curve(2)  # spaces exist: the geometry is fixed
     c = M.icosphere(1).Curve(2)
     with pytest.raises(NotImplementedError):
-        nb.HDivSurface(c)
-    with pytest.raises(NotImplementedError):
         nb.SurfaceL2(c, order=0, dual_mapping=True)
-    with pytest.raises(ValueError):
-        O.OracleOp(c, O.MAXWELL_SL, kappa=1.0, trial=O.HDIV, test=O.HDIV)
 
 
 def test_curved_mass_matrix():
