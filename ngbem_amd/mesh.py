@@ -273,17 +273,18 @@ def ellipsoid(level: int, axes=(1.0, 1.0, 1.0)) -> Mesh:
     return e
 
 
-def box(n: int, size=(1.0, 1.0, 1.0), centre=(0.0, 0.0, 0.0)) -> Mesh:
-    """Surface of an axis-parallel box, each face a structured n x n grid cut
-    into two triangles per cell, oriented outward; faces are regions 1-6
-    named "left", "right", "front", "back", "bottom", "top" (-x, +x, -y, +y,
-    -z, +z).  Flat triangles represent the box exactly."""
-    n = int(n)
-    if n < 1:
-        raise ValueError("box needs n >= 1")
+def box(n, size=(1.0, 1.0, 1.0), centre=(0.0, 0.0, 0.0)) -> Mesh:
+    """Surface of an axis-parallel box, each face a structured grid (n cells
+    along every axis, or n = (nx, ny, nz) per axis) cut into two triangles per
+    cell, oriented outward; faces are regions 1-6 named "left", "right",
+    "front", "back", "bottom", "top" (-x, +x, -y, +y, -z, +z).  Flat triangles
+    represent the box exactly."""
+    ns = [int(n)] * 3 if np.isscalar(n) else [int(x) for x in n]
+    if min(ns) < 1 or len(ns) != 3:
+        raise ValueError("box needs n >= 1 (or three of them)")
     h = np.asarray(size, dtype=np.float64)
     c = np.asarray(centre, dtype=np.float64)
-    t = np.linspace(-0.5, 0.5, n + 1)
+    ts = [np.linspace(-0.5, 0.5, k + 1) for k in ns]
     pts, tris, regions = [], [], []
     index = {}
 
@@ -297,15 +298,15 @@ def box(n: int, size=(1.0, 1.0, 1.0), centre=(0.0, 0.0, 0.0)) -> Mesh:
     # (axis, side): the face's outward normal is side * e_axis
     for r, (ax, side) in enumerate(((0, -1), (0, 1), (1, -1), (1, 1), (2, -1), (2, 1))):
         u, v = [a for a in range(3) if a != ax]
-        ids = np.zeros((n + 1, n + 1), dtype=np.int64)
-        for i in range(n + 1):
-            for j in range(n + 1):
+        ids = np.zeros((ns[u] + 1, ns[v] + 1), dtype=np.int64)
+        for i in range(ns[u] + 1):
+            for j in range(ns[v] + 1):
                 p = np.zeros(3)
                 p[ax] = 0.5 * side
-                p[u], p[v] = t[i], t[j]
+                p[u], p[v] = ts[u][i], ts[v][j]
                 ids[i, j] = vid(p * h + c)
-        for i in range(n):
-            for j in range(n):
+        for i in range(ns[u]):
+            for j in range(ns[v]):
                 a, b, cc, d = ids[i, j], ids[i + 1, j], ids[i + 1, j + 1], ids[i, j + 1]
                 # (p0 - p2) x (p1 - p2) = e_u x e_v = sign * e_ax; flip for outward
                 if np.cross(np.eye(3)[u], np.eye(3)[v])[ax] * side > 0:
