@@ -803,9 +803,9 @@ __global__ __launch_bounds__(256) void k_near_regular(const NearBlk* __restrict_
   typedef typename KER::T T;
   const int2 ch = chunks[blockIdx.x];
   const NearBlk nb = blks[ch.x];
-  const int p = ch.y + (int)threadIdx.x;
-  if (p >= nb.pm * nb.pn) return;
-  const int ja = p / nb.pm, ia = p - ja * nb.pm;  // consecutive lanes: consecutive test rows
+  const int64_t p = (int64_t)ch.y * 256 + threadIdx.x;  // chunk ch.y of 256 pairs of the block
+  if (p >= (int64_t)nb.pm * nb.pn) return;
+  const int ja = (int)(p / nb.pm), ia = (int)(p - (int64_t)ja * nb.pm);  // consecutive lanes: consecutive test rows
   const int e = pIb[nb.pI + ia], f = pJb[nb.pJ + ja];
   const int4 ve = vid[e], vf = vid[f];
   const bool touch = ve.x == vf.x || ve.x == vf.y || ve.x == vf.z || ve.y == vf.x || ve.y == vf.y || ve.y == vf.z ||
@@ -1029,9 +1029,9 @@ __global__ __launch_bounds__(64) void k_near_wide(const NearBlk* __restrict__ bl
     const int2 ch = chunks[blockIdx.x >> 8];
     bi = ch.x;
     const NearBlk b0 = blks[bi];
-    const int p = ch.y + (int)(blockIdx.x & 255);
-    if (p >= b0.pm * b0.pn) return;  // the whole workgroup
-    const int ja = p / b0.pm, ia = p - ja * b0.pm;
+    const int64_t p = (int64_t)ch.y * 256 + (blockIdx.x & 255);
+    if (p >= (int64_t)b0.pm * b0.pn) return;  // the whole workgroup
+    const int ja = (int)(p / b0.pm), ia = (int)(p - (int64_t)ja * b0.pm);
     e = pIb[b0.pI + ia];
     f = pJb[b0.pJ + ja];
     const int4 ve = vid[e], vf = vid[f];
@@ -2822,9 +2822,11 @@ void assemble_near(Operator& op, hipStream_t s) {
       nb.pJ = nb.c0;
       nb.pn = nb.n;
     }
-    const int64_t np = (int64_t)nb.pm * nb.pn;
-    if (np >= (int64_t)1 << 31) throw std::runtime_error("near block too large for one launch grid");
-    for (int64_t st = 0; st < np; st += 256) chunks.push_back(make_int2((int)k, (int)st));
+    // chunks of 256 element pairs, numbered per block (a block of the dense
+    // method at 81,920 elements holds 6.7e9 pairs)
+    const int64_t nch = ((int64_t)nb.pm * nb.pn + 255) / 256;
+    if (nch >= (int64_t)INT32_MAX) throw std::runtime_error("near block too large");
+    for (int64_t c = 0; c < nch; c++) chunks.push_back(make_int2((int)k, (int)c));
   }
   // touching element pairs: for each test element, its vertex neighbours; a
   // pair belongs to every near block holding one of its dof combinations
@@ -2909,12 +2911,15 @@ void assemble_near(Operator& op, hipStream_t s) {
     auto launch = [&](auto dflag) {
       constexpr bool D = decltype(dflag)::value;
       BEM_HIP(hipEventRecord(tev[0], s));
-      if (!chunks.empty())
-        hipLaunchKernelGGL((k_near_regular<KER, EI, EJ, D>), dim3((unsigned)chunks.size()), dim3(256), 0, s,
-                           op.d_near_blk.as<NearBlk>(), op.d_near_chunks.as<int2>(), pIb, pJb, op.d_geo.as<double>(),
-                           op.d_vid.as<int4>(), op.d_quad.as<double>(), op.d_sdat.as<double>(),
+      // launches of at most 2^22 chunks (2^30 threads)
+      for (size_t c0 = 0; c0 < chunks.size(); c0 += (size_t)1 << 22) {
+        const size_t nc = std::min(chunks.size() - c0, (size_t)1 << 22);
+        hipLaunchKernelGGL((k_near_regular<KER, EI, EJ, D>), dim3((unsigned)nc), dim3(256), 0, s,
+                           op.d_near_blk.as<NearBlk>(), op.d_near_chunks.as<int2>() + c0, pIb, pJb,
+                           op.d_geo.as<double>(), op.d_vid.as<int4>(), op.d_quad.as<double>(), op.d_sdat.as<double>(),
                            op.d_trig.as<double>(), ntq, op.d_dofs_test.as<int>(), op.d_dofs_trial.as<int>(),
                            op.d_iperm_test.as<int>(), op.d_iperm_trial.as<int>(), ker, op.d_qgeo.as<double>(), val);
+      }
       BEM_HIP(hipEventRecord(tev[1], s));
       for (int c = 0; c < 3; c++) {
         const int np = (int)pairs[c].size();
@@ -2929,10 +2934,12 @@ void assemble_near(Operator& op, hipStream_t s) {
     pt_.mark("uploads + alloc");
     if constexpr (EI::WIDE) {
       BEM_HIP(hipEventRecord(tev[0], s));
-      if (chunks.size() >= ((size_t)1 << 23)) throw std::runtime_error("near field too large for one launch grid");
-      if (!chunks.empty())
-        hipLaunchKernelGGL((k_near_wide<KER, EI, EJ, false>), dim3((unsigned)(chunks.size() * 256)), dim3(64), 0, s,
-                           op.d_near_blk.as<NearBlk>(), op.d_near_chunks.as<int2>(), pIb, pJb, (const int4*)nullptr,
+      // launches of at most 2^16 chunks (2^24 workgroups of one wave)
+      for (size_t c0 = 0; c0 < chunks.size(); c0 += (size_t)1 << 16)
+        hipLaunchKernelGGL((k_near_wide<KER, EI, EJ, false>),
+                           dim3((unsigned)(std::min(chunks.size() - c0, (size_t)1 << 16) * 256)), dim3(64), 0, s,
+                           op.d_near_blk.as<NearBlk>(), op.d_near_chunks.as<int2>() + c0, pIb, pJb,
+                           (const int4*)nullptr,
                            (const double*)nullptr, 0, op.d_geo.as<double>(), op.d_vid.as<int4>(),
                            op.d_quad.as<double>(), op.d_sdat.as<double>(), op.d_trig.as<double>(), ntq,
                            op.d_dofs_test.as<int>(), op.d_dofs_trial.as<int>(), op.d_iperm_test.as<int>(),
