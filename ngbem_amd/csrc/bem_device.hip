@@ -2217,15 +2217,38 @@ struct RowPiece {
   int tile, c0, c1, slot;  // slot < 0: the tile's only piece, straight to y
 };
 
+// Piece j (j = 0, 1, ...) of workgroup b when a grid of G workgroups loops
+// over n pieces: with remap, XCD x = b % 8 (round robin) takes the pieces of
+// the x-th of eight contiguous ranges, its G / 8 workgroups in turn (G a
+// multiple of 8); without, b + j G.  -1 past the end.
+__device__ __forceinline__ int persistent_piece(int b, int G, int n, int j, int remap) {
+  if (!remap) {
+    const int64_t p = (int64_t)b + (int64_t)j * G;
+    return p < n ? (int)p : -1;
+  }
+  const int W = G >> 3, x = b & 7, k = b >> 3;
+  const int q = n >> 3, rem = n & 7;
+  const int start = x * q + min(x, rem), cnt = q + (x < rem ? 1 : 0);
+  const int64_t o = (int64_t)k + (int64_t)j * W;
+  return o < cnt ? start + (int)o : -1;
+}
+
 template <class ST, class VT>
 __global__ __launch_bounds__(256) void k_mv_rows_flat(const Tile* __restrict__ tiles,
                                                       const RowPiece* __restrict__ pieces,
                                                       const int2* __restrict__ fcols, const ST* __restrict__ nval,
                                                       const ST* __restrict__ fUt, const VT* __restrict__ xp,
                                                       const VT* __restrict__ tbuf, VT* __restrict__ yp,
-                                                      VT* __restrict__ part, int remap) {
+                                                      VT* __restrict__ part, int remap, int npieces) {
   __shared__ VT red[4][64];
-  const RowPiece pc = pieces[xcd_major(blockIdx.x, gridDim.x, remap)];
+  // npieces == 0: one workgroup per piece; else a persistent grid looping
+  // over npieces pieces
+  for (int j = 0;; j++) {
+  const int pi = npieces ? persistent_piece(blockIdx.x, gridDim.x, npieces, j, remap)
+                         : (j ? -1 : xcd_major(blockIdx.x, gridDim.x, remap));
+  if (pi < 0) break;
+  if (j) __syncthreads();  // red[] of the previous piece has been read
+  const RowPiece pc = pieces[pi];
   const Tile tl = tiles[pc.tile];
   const int2 rg = make_int2(pc.c0, pc.c1);
   const int lane = threadIdx.x & 63;
@@ -2264,6 +2287,7 @@ __global__ __launch_bounds__(256) void k_mv_rows_flat(const Tile* __restrict__ t
     const VT v = red[0][lane] + red[1][lane] + red[2][lane] + red[3][lane];
     if (pc.slot < 0) yp[tl.p0 + lane] = v;
     else part[(int64_t)pc.slot * 64 + lane] = v;
+  }
   }
 }
 
@@ -2575,6 +2599,19 @@ __global__ void k_scatter_mm(const int* __restrict__ perm, int n, const VT* __re
 template <class S, class D>
 __global__ void k_convert(const S* __restrict__ a, D* __restrict__ b, int64_t n) {
   for (int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x; i < n; i += (int64_t)gridDim.x * 256) cvt(b[i], a[i]);
+}
+
+// U panels of the FP32 copy: one workgroup per panel, converted from the
+// FP64 factors (the same rounding as k_convert)
+template <class S, class D>
+__global__ __launch_bounds__(256) void k_pack_panels(const Panel* __restrict__ panels, const S* __restrict__ a,
+                                                     D* __restrict__ b) {
+  const Panel p = panels[blockIdx.x];
+  const int n = p.R * p.len;
+  for (int i = threadIdx.x; i < n; i += 256) {
+    const int k = i / p.len, r = i - k * p.len;
+    cvt(b[p.dst + i], a[p.src + (int64_t)k * p.ld + r]);
+  }
 }
 
 // --------------------------------------------------------------- dispatch --
@@ -3658,7 +3695,7 @@ static void upload_pieces(Operator& op, int prec, hipStream_t s) {
   std::vector<RowPiece> pieces;
   std::vector<int4> sums;
   const int ch = op.tune_rows_chunk[prec] >= 0 ? op.tune_rows_chunk[prec] : rows_chunk();
-  const int slots = build_pieces(op.h_frange, ch, pieces, sums);
+  const int slots = build_pieces(prec == 1 && op.panels ? op.h_frange_p : op.h_frange, ch, pieces, sums);
   op.d_fpieces[prec].upload(pieces, s);
   op.n_fpieces[prec] = (int)pieces.size();
   op.d_fsums[prec].upload(sums, s);
@@ -3720,6 +3757,87 @@ static bool flat_columns(const std::vector<Tile>& tiles, const std::vector<Seg>&
   return true;
 }
 
+// U panels of the FP32 copy (NGBEM_PANELS=0: the Ut layout for both
+// precisions)
+static bool panels_on() {
+  static const bool on = [] {
+    const char* e = std::getenv("NGBEM_PANELS");
+    return !(e && std::string(e) == "0");
+  }();
+  return on;
+}
+
+// per-tile flat column lists over U panels: dense columns as in
+// flat_columns, and for each low-rank segment of the tile's leaf a panel of
+// the segment's R columns restricted to the tile's rows, laid out column
+// after column from offset 64 on (so that panel offset - roff >= 0; the
+// kernel adds roff + lane).  False when an offset needs more than 31 bits.
+static bool panel_columns(const std::vector<Tile>& tiles, const std::vector<Seg>& segs, std::vector<int2>& cols,
+                          std::vector<int2>& range, std::vector<Panel>& panels, int64_t& total) {
+  const int nt = (int)tiles.size();
+  std::vector<int64_t> start(nt + 1, 0), pstart(nt + 1, 0), estart(nt + 1, 0);
+  for (int t = 0; t < nt; t++) {
+    const Tile& tl = tiles[t];
+    int64_t cnt = 0, np = 0, ne = 0;
+    for (int k = tl.s0; k < tl.s1; k++) {
+      cnt += segs[k].cnt;
+      if (segs[k].far == 1) {
+        np++;
+        ne += (int64_t)segs[k].cnt * tl.len;
+      } else if (segs[k].far != 0) {
+        throw std::runtime_error("panel plan: z segment in the Ut plan");
+      }
+    }
+    start[t + 1] = start[t] + cnt;
+    pstart[t + 1] = pstart[t] + np;
+    estart[t + 1] = estart[t] + ne;
+  }
+  total = 64 + estart[nt];
+  if (start[nt] >= (int64_t)INT32_MAX || total >= (int64_t)INT32_MAX) return false;
+  cols.assign(std::max<int64_t>(start[nt], 1), make_int2(0, 0));
+  panels.assign(pstart[nt], Panel{0, 0, 0, 0, 0, 0});
+  std::atomic<bool> too_big(false);
+  const int nth = (int)std::max(1u, std::min(16u, std::thread::hardware_concurrency()));
+  std::vector<std::thread> th;
+  for (int w = 0; w < nth; w++)
+    th.emplace_back([&, w] {
+      for (int t = w; t < nt; t += nth) {
+        const Tile& tl = tiles[t];
+        int64_t c = start[t], p = pstart[t], dst = 64 + estart[t];
+        for (int k = tl.s0; k < tl.s1; k++) {
+          const Seg& sg = segs[k];
+          if (sg.far == 1) {
+            panels[p++] = Panel{sg.base + tl.roff, dst, sg.ld, sg.cnt, tl.len, 0};
+            for (int j = 0; j < sg.cnt; j++) {
+              const int64_t xi = (int64_t)sg.xoff + j;
+              if (xi >= COL_FAR) {
+                too_big = true;
+                return;
+              }
+              cols[c++] = make_int2((int)(dst + (int64_t)j * tl.len - tl.roff), (int)xi | COL_FAR);
+            }
+            dst += (int64_t)sg.cnt * tl.len;
+          } else {
+            for (int j = 0; j < sg.cnt; j++) {
+              const int64_t off = sg.base + (int64_t)j * sg.ld;
+              const int64_t xi = (int64_t)sg.xoff + j;
+              if (off + sg.ld > (int64_t)INT32_MAX || xi >= COL_FAR) {
+                too_big = true;
+                return;
+              }
+              cols[c++] = make_int2((int)off, (int)xi);
+            }
+          }
+        }
+      }
+    });
+  for (auto& x : th) x.join();
+  if (too_big) return false;
+  range.assign(nt, make_int2(0, 0));
+  for (int t = 0; t < nt; t++) range[t] = make_int2((int)start[t], (int)start[t + 1]);
+  return true;
+}
+
 // the apply's tables: z items first (they set the blocks' z offsets), then
 // the Ut plan with its flat column lists, the z plan, the transposed plan and
 // the low-rank row items, built on host threads at once and uploaded after
@@ -3729,8 +3847,10 @@ void build_matvec_plan(Operator& op, hipStream_t s) {
   pt_.mark("z items");
   std::vector<Tile> t_ut, t_fwd, t_trn;
   std::vector<Seg> s_ut, s_fwd, s_trn;
-  std::vector<int2> cols, range;
-  bool flat_ok = false;
+  std::vector<int2> cols, range, pcols, prange;
+  std::vector<Panel> pans;
+  int64_t ptot = 0;
+  bool flat_ok = false, pan_ok = false;
   std::vector<RowItem> items[2][2], citems[2];
   std::vector<Tile> t_tsum;
   std::vector<Seg> s_tsum;
@@ -3748,6 +3868,7 @@ void build_matvec_plan(Operator& op, hipStream_t s) {
       guarded(0, [&] {
         plan(op, false, t_ut, s_ut, PLAN_UT);
         flat_ok = flat_columns(t_ut, s_ut, cols, range);
+        if (flat_ok && op.mixed && panels_on()) pan_ok = panel_columns(t_ut, s_ut, pcols, prange, pans, ptot);
       }),
       guarded(1, [&] { plan(op, false, t_fwd, s_fwd); }),
       guarded(2, [&] { plan(op, true, t_trn, s_trn); }),
@@ -3759,6 +3880,18 @@ void build_matvec_plan(Operator& op, hipStream_t s) {
     if (e) std::rethrow_exception(e);
   pt_.mark("host tables");
   op.d_seg_fwd_mm.upload(s_ut, s);
+  op.panels = flat_ok && pan_ok;
+  if (op.panels) {
+    op.d_fcols_p.upload(pcols, s);
+    op.h_frange_p = prange;
+    op.h_panels = std::move(pans);
+    op.panel_total = ptot;
+  } else {
+    op.d_fcols_p.release();
+    op.h_frange_p.clear();
+    op.h_panels.clear();
+    op.panel_total = 0;
+  }
   if (flat_ok) {
     op.d_fcols.upload(cols, s);
     op.h_frange = range;
@@ -3827,6 +3960,27 @@ void make_mixed_copy(Operator& op, hipStream_t s) {
   }
   conv(op.d_fU, op.d_fU32, tu);
   conv(op.d_fV, op.d_fV32, tv);
+  if (op.panels) {
+    const size_t esz32 = op.complex_valued ? sizeof(fc) : sizeof(float);
+    op.d_fUp.alloc((size_t)op.panel_total * esz32);
+    BEM_HIP(hipMemsetAsync(op.d_fUp.p, 0, (size_t)op.panel_total * esz32, s));
+    DBuf d_pan;
+    d_pan.upload(op.h_panels, s);
+    const unsigned np = (unsigned)op.h_panels.size();
+    if (np) {
+      if (op.complex_valued)
+        hipLaunchKernelGGL((k_pack_panels<zc, fc>), dim3(np), dim3(256), 0, s, d_pan.as<Panel>(), op.d_fU.as<zc>(),
+                           op.d_fUp.as<fc>());
+      else
+        hipLaunchKernelGGL((k_pack_panels<double, float>), dim3(np), dim3(256), 0, s, d_pan.as<Panel>(),
+                           op.d_fU.as<double>(), op.d_fUp.as<float>());
+    }
+    BEM_HIP(hipGetLastError());
+    BEM_HIP(hipStreamSynchronize(s));
+    op.h_panels.clear();
+    op.h_panels.shrink_to_fit();
+    return;
+  }
   BEM_HIP(hipGetLastError());
   BEM_HIP(hipStreamSynchronize(s));
 }
@@ -4425,6 +4579,27 @@ static int xcd_remap() {
   return r;
 }
 
+// persistent grid of the forward row kernel: workgroups per CU x CUs
+// (multiple of 8); 0 (default): one workgroup per piece.  The tuned value
+// per precision, else NGBEM_ROWS_GRID
+static int rows_grid(const Operator& op, int prec) {
+  static const int g = [] {
+    const char* e = std::getenv("NGBEM_ROWS_GRID");
+    return e ? std::max(0, std::atoi(e)) / 8 * 8 : 0;
+  }();
+  return op.tune_rows_grid[prec] >= 0 ? op.tune_rows_grid[prec] : g;
+}
+
+// dynamic LDS bytes of the forward row kernel's workgroups, unused by the
+// kernel: it caps the workgroups per CU (NGBEM_ROWS_LDS, default 0)
+static int rows_lds() {
+  static const int b = [] {
+    const char* e = std::getenv("NGBEM_ROWS_LDS");
+    return e ? std::min(std::max(0, std::atoi(e)), 60 * 1024) : 0;
+  }();
+  return b;
+}
+
 // row tiles of the forward apply from the flat column lists (default) or,
 // with NGBEM_MV_ROWS=segments, segment by segment
 static bool rows_flat() {
@@ -4512,9 +4687,13 @@ static void apply_t(Operator& op, int flags, const XT* x, XT* y, XT alpha, XT be
     const DBuf& segs = pk == PLAN_Z ? op.d_seg_fwd : op.d_seg_fwd_mm;
     constexpr int prec = st_prec<ST>();
     if (!zpath && op.d_fpieces[prec].p && rows_flat()) {
-      hipLaunchKernelGGL((k_mv_rows_flat<ST, VT>), dim3(op.n_fpieces[prec]), dim3(256), 0, s,
-                         op.d_tile_fwd.as<Tile>(), op.d_fpieces[prec].as<RowPiece>(), op.d_fcols.as<int2>(), nval, fU,
-                         xp, tb, yp, op.d_fpart.as<VT>(), xcd_remap());
+      const bool pan = prec == 1 && op.panels;
+      const int np = op.n_fpieces[prec], pg = rows_grid(op, prec);
+      const bool pers = pg > 0 && pg < np;
+      hipLaunchKernelGGL((k_mv_rows_flat<ST, VT>), dim3(pers ? pg : np), dim3(256), rows_lds(), s,
+                         op.d_tile_fwd.as<Tile>(), op.d_fpieces[prec].as<RowPiece>(),
+                         (pan ? op.d_fcols_p : op.d_fcols).as<int2>(), nval, pan ? op.d_fUp.as<ST>() : fU, xp, tb,
+                         yp, op.d_fpart.as<VT>(), xcd_remap(), pers ? np : 0);
       if (op.n_fsums[prec])
         hipLaunchKernelGGL((k_mv_rows_psum<VT>), dim3(op.n_fsums[prec]), dim3(64), 0, s, op.d_tile_fwd.as<Tile>(),
                            op.d_fsums[prec].as<int4>(), (const VT*)op.d_fpart.as<VT>(), yp);
@@ -4799,6 +4978,38 @@ std::string autotune(Operator& op, int prec, int iters, hipStream_t s) {
     upload_pieces(op, prec, s);
     BEM_HIP(hipStreamSynchronize(s));
     std::snprintf(buf, sizeof buf, "], \"rows_choice\": %d", best);
+    rep += buf;
+    // the same pieces from a persistent grid of 64 workgroups or k CUs'
+    // worth (0: one workgroup per piece; a grid of at least as many
+    // workgroups as pieces is the same)
+    int ncu = 0, dev = 0;
+    BEM_HIP(hipGetDevice(&dev));
+    BEM_HIP(hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev));
+    ncu = std::max(8, ncu / 8 * 8);
+    const int gcand[] = {0, 64, 2 * ncu, 3 * ncu, 4 * ncu, 6 * ncu, 8 * ncu};
+    int gbest = 0;
+    double gt = 1e30;
+    // every grid computes the same pieces: the outputs must be bit-identical
+    std::vector<char> y0((size_t)op.ndof_test() * esz), y1(y0.size());
+    bool same = true;
+    rep += ", \"rows_grid\": [";
+    for (size_t c = 0; c < sizeof gcand / sizeof gcand[0]; c++) {
+      const int g = gcand[c];
+      op.tune_rows_grid[prec] = g;
+      const double t = timed(3, 4);
+      BEM_HIP(hipMemcpyAsync(c ? y1.data() : y0.data(), dy.p, y0.size(), hipMemcpyDeviceToHost, s));
+      BEM_HIP(hipStreamSynchronize(s));
+      if (c && std::memcmp(y0.data(), y1.data(), y0.size()) != 0) same = false;
+      std::snprintf(buf, sizeof buf, "%s{\"workgroups\": %d, \"ms\": %.4f}", c ? ", " : "", g, t);
+      rep += buf;
+      if (t < gt) {
+        gt = t;
+        gbest = g;
+      }
+    }
+    op.tune_rows_grid[prec] = gbest;
+    std::snprintf(buf, sizeof buf, "], \"rows_grid_choice\": %d, \"rows_grid_identical\": %s", gbest,
+                  same ? "true" : "false");
     rep += buf;
   }
   std::snprintf(buf, sizeof buf, ", \"apply_ms_after\": %.4f}", timed(0, 5));

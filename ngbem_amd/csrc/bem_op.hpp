@@ -193,6 +193,13 @@ struct Seg {
   int ld, cnt, xoff, far;
 };
 
+// one U panel: R columns of len entries gathered from src (entry (k, i) at
+// src + k ld + i) to dst + k len + i
+struct Panel {
+  int64_t src, dst;
+  int ld, R, len, pad;
+};
+
 struct Operator {
   int kernel = K_LSL;
   double kappa = 0.0;
@@ -258,12 +265,24 @@ struct Operator {
   DBuf d_fcols, d_fpieces[2], d_fsums[2], d_fpart;
   int n_fpieces[2] = {0, 0}, n_fsums[2] = {0, 0};
   std::vector<int2> h_frange;
+  // the FP32 copy's forward lists over U panels: per (row tile, low-rank
+  // block over its rows) the tile's rows of the block's R columns of U,
+  // column after column, so that a tile reads each block's part of U as one
+  // contiguous run (in the Ut layout it reads R runs of the tile's height,
+  // the block's height apart).  panels: on when the FP32 copy uses them;
+  // h_panels: the copy's gather table (source and destination of each panel)
+  bool panels = false;
+  DBuf d_fcols_p, d_fUp;
+  std::vector<int2> h_frange_p;
+  std::vector<Panel> h_panels;
+  int64_t panel_total = 0;
   // apply tunables per precision, set by autotune(); -1: the built-in rules
   // (low-rank row items of 8 rows up to a row length of 64 (complex128) / 128
   // entries, else 4 rows; row tiles cut into pieces of NGBEM_ROWS_CHUNK or 768
   // columns)
   int tune_far_len[2] = {-1, -1};
   int tune_rows_chunk[2] = {-1, -1};
+  int tune_rows_grid[2] = {-1, -1};  // persistent grid of the row kernel (0: a workgroup per piece)
   int n_tiles_fwd = 0, n_tiles_trn = 0;
   DBuf d_zitems_fwd, d_zitems_trn;  // U t / V^T t work items
   int n_zitems_fwd = 0, n_zitems_trn = 0;

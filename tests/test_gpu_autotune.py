@@ -37,6 +37,9 @@ def test_autotune_keeps_the_apply(gpu, cf_op):
         best = min(c["ms"] for c in r["rows"])
         assert [c["ms"] for c in r["rows"] if c["columns_per_piece"] == r["rows_choice"]][0] == best
         assert r["apply_ms_after"] > 0
+        # the persistent grids run the same pieces: bit-identical outputs
+        assert len(r["rows_grid"]) >= 4 and r["rows_grid_identical"] is True
+        assert r["rows_grid_choice"] in [c["workgroups"] for c in r["rows_grid"]]
     rel = lambda a, b: (torch.linalg.norm(a - b) / torch.linalg.norm(b)).item()  # noqa: E731
     assert rel(C.mat * x, y64) <= 1e-13
     assert rel(C.mat.mixed * x, y32) <= 1e-6
