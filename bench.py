@@ -68,7 +68,10 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-solve", action="store_true")
     ap.add_argument("--no-autotune", action="store_true", help="keep the built-in apply work shapes")
-    ap.add_argument("--no-aca-gram", action="store_true", help="skip the extra assembly with NGBEM_ACA=gram")
+    ap.add_argument("--aca-gram", action="store_true", help="also assemble with NGBEM_ACA=gram (opt-in variant)")
+    ap.add_argument("--no-aca-gram", action="store_true", help=argparse.SUPPRESS)  # the default now
+    ap.add_argument("--no-block-product", action="store_true",
+                    help="skip the Maxwell EFIE 64-right-hand-side block product after the headline workload")
     ap.add_argument("--calib-copy", action="store_true",
                     help="end with one elementwise kernel reading and writing 512 MiB (calibrates rocprof "
                          "FETCH_SIZE/WRITE_SIZE)")
@@ -144,7 +147,7 @@ def main():
     # the same assembly with the single-pass (Gram) cross approximation,
     # NGBEM_ACA=gram: faster far field, stops that may move by a step
     aca_gram = None
-    if not args.no_aca_gram:
+    if args.aca_gram and not args.no_aca_gram:
         os.environ["NGBEM_ACA"] = "gram"
         try:
             torch.cuda.synchronize()
@@ -397,6 +400,24 @@ def main():
         torch.cuda.synchronize()
         torch.mul(a, 2.0, out=b)
         torch.cuda.synchronize()
+    if not args.no_block_product:
+        # the many-right-hand-side configuration (Maxwell EFIE, 64 plane-wave
+        # loads per block product on the matrix cores), after the headline
+        # operator's memory is released
+        C = M64 = M32 = A64 = A32 = None  # noqa: F841  (drop every reference to the headline operator)
+        import gc
+
+        gc.collect()
+        torch.cuda.synchronize()
+        torch.cuda.empty_cache()
+        bp = bench_maxwell_block(argparse.Namespace(level=args.level, steps=10, warmup=2), emit=False)
+        out["block_product_maxwell_efie_64rhs"] = {
+            "config": bp["config"]["workload"], "ms_fp32": bp["ms_per_step"],
+            "ms_fp64": 1e3 / bp["fp64_block_per_s"], "tflops_fp32": bp["roofline"]["achieved"],
+            "frac_fp32_mfma": bp["roofline"]["frac"], "tflops_fp64": bp["roofline"]["fp64_achieved"],
+            "frac_fp64_mfma": bp["roofline"]["fp64_achieved"] / bp["roofline"]["fp64_peak"],
+            "block_vs_vector_speedup": bp["block_vs_vector_speedup"],
+            "fp32_vs_fp64_rel_err": bp["fp32_vs_fp64_rel_err"], "assembly_s": bp["assembly_s"]}
     print(json.dumps(out), flush=True)
 
 
@@ -420,7 +441,7 @@ def hbm_probe(dev, reps=10):
     return {"copy_gbs": gbs, "how": "torch copy_ of 1 GiB float32, 10 calls, HIP events"}
 
 
-def bench_maxwell_block(args):
+def bench_maxwell_block(args, emit=True):
     """Maxwell EFIE single layer, HDivSurface p0, kappa 5, intorder 4,
     leafsize 40, eta 2, ACA eps 1e-5; one step = one block product of 64
     right-hand sides (MFMA path), FP64 and FP32 copies, against 64 vector
@@ -501,7 +522,9 @@ def bench_maxwell_block(args):
         "hmatrix": {"near_entries": st["near_entries"], "far_factor_entries": st["far_factor_entries"],
                     "n_near": st["n_near"], "n_far": st["n_far"], "max_rank": st["max_rank"]},
     }
-    print(json.dumps(out), flush=True)
+    if emit:
+        print(json.dumps(out), flush=True)
+    return out
 
 
 def bench_batch(args):

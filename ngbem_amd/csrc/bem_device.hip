@@ -2596,6 +2596,248 @@ __global__ void k_scatter_mm(const int* __restrict__ perm, int n, const VT* __re
   *dst = beta_zero ? alpha * v : alpha * v + beta * *dst;
 }
 
+// ---- block product, slab form --------------------------------------------
+// The 64 right-hand sides of a row k of X (and of the low-rank stage's T) are
+// kept as NP planes of 64 values: plane p at (k NP + p) 64, and inside a
+// plane right-hand side j = 16 t + i at 4 i + t, so that lane i of an MFMA
+// reads the values of its four 16-column tiles as one 16- or 32-byte load.
+// Complex values take three planes (re, im, re + im) and three real MFMAs per
+// product (Gauss): T1 = Ar Xr, T2 = Ai Xi, T3 = (Ar + Ai)(Xr + Xi), then
+// Re Y = T1 - T2 and Im Y = T3 - T1 - T2; real values one plane.
+template <class VT>
+struct SlabT;
+template <>
+struct SlabT<fc> {
+  static constexpr int NP = 3;
+  typedef float R;
+};
+template <>
+struct SlabT<zc> {
+  static constexpr int NP = 3;
+  typedef double R;
+};
+template <>
+struct SlabT<float> {
+  static constexpr int NP = 1;
+  typedef float R;
+};
+template <>
+struct SlabT<double> {
+  static constexpr int NP = 1;
+  typedef double R;
+};
+
+template <class R>
+__device__ __forceinline__ void split_planes(const cx<R>& v, R (&a)[3]) {
+  a[0] = v.x;
+  a[1] = v.y;
+  a[2] = v.x + v.y;
+}
+template <class R>
+__device__ __forceinline__ void split_planes(const R& v, R (&a)[1]) {
+  a[0] = v;
+}
+template <class R>
+__device__ __forceinline__ cx<R> join_planes(const R (&y)[3]) {
+  return {y[0] - y[1], y[2] - y[0] - y[1]};
+}
+template <class R>
+__device__ __forceinline__ R join_planes(const R (&y)[1]) {
+  return y[0];
+}
+
+// one 16-column group of one wave: lane (i, g) holds, for the four MFMA
+// k-steps s, its row's entry of column kk + 4 g + s (split into planes) and
+// that X row's four tile values per plane
+template <class VT>
+struct MMBuf {
+  typedef typename SlabT<VT>::R R;
+  static constexpr int NP = SlabT<VT>::NP;
+  typedef typename MF<R>::V4 V4;
+  R a[4][NP];
+  V4 b[4][NP];
+  template <class ST>
+  __device__ __forceinline__ void load(const ST* __restrict__ A, int64_t lda, bool rok, const R* __restrict__ X,
+                                       int kk, int cnt, int g, int i) {
+#pragma unroll
+    for (int s = 0; s < 4; s++) {
+      const int k = kk + 4 * g + s;
+      VT av = zero_<VT>();
+      if (rok && k < cnt) cvt(av, A[(int64_t)k * lda]);
+      split_planes(av, a[s]);
+#pragma unroll
+      for (int p = 0; p < NP; p++) {
+        if (k < cnt) b[s][p] = *(const V4*)(X + ((int64_t)k * NP + p) * MM_W + 4 * i);
+        else b[s][p] = V4{R(0), R(0), R(0), R(0)};
+      }
+    }
+  }
+};
+
+template <class VT>
+struct MMAcc2 {
+  typedef typename SlabT<VT>::R R;
+  static constexpr int NP = SlabT<VT>::NP;
+  typename MF<R>::V4 c[NP][4];
+  __device__ __forceinline__ void zero() {
+#pragma unroll
+    for (int p = 0; p < NP; p++)
+#pragma unroll
+      for (int t = 0; t < 4; t++) c[p][t] = typename MF<R>::V4{R(0), R(0), R(0), R(0)};
+  }
+  __device__ __forceinline__ void run(const MMBuf<VT>& m) {
+#pragma unroll
+    for (int s = 0; s < 4; s++)
+#pragma unroll
+      for (int p = 0; p < NP; p++)
+#pragma unroll
+        for (int t = 0; t < 4; t++) c[p][t] = MF<R>::mma(m.a[s][p], m.b[s][p][t], c[p][t]);
+  }
+  // result of tile t, MFMA output register v
+  __device__ __forceinline__ VT get(int t, int v) const {
+    R y[NP];
+#pragma unroll
+    for (int p = 0; p < NP; p++) y[p] = c[p][t][v];
+    return join_planes(y);
+  }
+};
+
+// X[k][j] (k < n, j < 64) of right-hand sides r0 .. r0 + nr of x (column
+// stride ld), rows in cluster order, into slab planes; zero beyond nr
+template <class VT, class XT>
+__global__ void k_gather_slab(const int* __restrict__ perm, int n, const XT* __restrict__ x, int64_t ld, int r0,
+                              int nr, typename SlabT<VT>::R* __restrict__ xs) {
+  typedef typename SlabT<VT>::R R;
+  constexpr int NP = SlabT<VT>::NP;
+  const int64_t idx = (int64_t)blockIdx.x * 256 + threadIdx.x;
+  if (idx >= (int64_t)n * MM_W) return;
+  const int k = (int)(idx / MM_W), j = (int)(idx % MM_W);
+  VT v = zero_<VT>();
+  if (j < nr) cvt(v, x[(int64_t)perm[k] + (int64_t)(r0 + j) * ld]);
+  R a[NP];
+  split_planes(v, a);
+  const int pos = 4 * (j & 15) + (j >> 4);
+#pragma unroll
+  for (int p = 0; p < NP; p++) xs[((int64_t)k * NP + p) * MM_W + pos] = a[p];
+}
+
+// T[toff + l0 + r] = V[l0 + r] X[c0 ..] for 16 rank rows of a low-rank block
+// (one workgroup per item): the four waves take every fourth 16-column group,
+// each loading the next group while its MFMAs run, and are summed through
+// LDS; T is written in slab planes
+template <class ST, class VT>
+__global__ __launch_bounds__(256) void k_mm2_far_t(const MMItem* __restrict__ items, const FarBlk* __restrict__ fb,
+                                                   const ST* __restrict__ fV,
+                                                   const typename SlabT<VT>::R* __restrict__ xs,
+                                                   typename SlabT<VT>::R* __restrict__ ts) {
+  typedef typename SlabT<VT>::R R;
+  constexpr int NP = SlabT<VT>::NP;
+  __shared__ VT red[3][16][MM_W];
+  const MMItem it = items[blockIdx.x];
+  const FarBlk b = fb[it.blk];
+  const int lane = threadIdx.x & 63;
+  const int wv = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+  const int i = lane & 15, g = lane >> 4;
+  const bool rok = it.l0 + i < b.R;
+  const ST* A = fV + b.voff + (int64_t)(rok ? it.l0 + i : 0) * b.n;  // row-major: column k at A[k]
+  const R* X = xs + (int64_t)b.c0 * NP * MM_W;
+  MMAcc2<VT> acc;
+  acc.zero();
+  MMBuf<VT> cur, nxt;
+  int kk = 16 * wv;
+  if (kk < b.n) cur.load(A, 1, rok, X, kk, b.n, g, i);
+  for (; kk < b.n; kk += 64) {
+    if (kk + 64 < b.n) nxt.load(A, 1, rok, X, kk + 64, b.n, g, i);
+    acc.run(cur);
+    cur = nxt;
+  }
+  if (wv > 0)
+#pragma unroll
+    for (int t = 0; t < 4; t++)
+#pragma unroll
+      for (int v = 0; v < 4; v++) red[wv - 1][MF<R>::row(g, v)][16 * t + i] = acc.get(t, v);
+  __syncthreads();
+  if (wv == 0) {
+#pragma unroll
+    for (int t = 0; t < 4; t++)
+#pragma unroll
+      for (int v = 0; v < 4; v++) {
+        const int r = MF<R>::row(g, v), c = 16 * t + i;
+        if (it.l0 + r >= b.R) continue;
+        const VT y = acc.get(t, v) + red[0][r][c] + red[1][r][c] + red[2][r][c];
+        R a[NP];
+        split_planes(y, a);
+        const int64_t row = b.toff + it.l0 + r;
+#pragma unroll
+        for (int p = 0; p < NP; p++) ts[(row * NP + p) * MM_W + 4 * i + t] = a[p];
+      }
+  }
+}
+
+// forward block product over the row tiles: one wave per (tile, 16-row
+// strip), all of the tile's segments and columns, the next 16-column group
+// loaded while the current one's MFMAs run; four strips per workgroup
+struct MMStrip {
+  int tile, strip;
+};
+
+template <class ST, class VT>
+__global__ __launch_bounds__(256) void k_mm2_rows(const MMStrip* __restrict__ strips, int nstrips,
+                                                  const Tile* __restrict__ tiles, const Seg* __restrict__ segs,
+                                                  const ST* __restrict__ nval, const ST* __restrict__ fUt,
+                                                  const typename SlabT<VT>::R* __restrict__ xs,
+                                                  const typename SlabT<VT>::R* __restrict__ ts, VT* __restrict__ ym) {
+  typedef typename SlabT<VT>::R R;
+  constexpr int NP = SlabT<VT>::NP;
+  const int w = __builtin_amdgcn_readfirstlane(blockIdx.x * 4 + (threadIdx.x >> 6));
+  if (w >= nstrips) return;
+  const MMStrip st = strips[w];
+  const Tile tl = tiles[st.tile];
+  const int lane = threadIdx.x & 63;
+  const int i = lane & 15, g = lane >> 4;
+  const bool rok = st.strip * 16 + i < tl.len;
+  const int r = tl.roff + st.strip * 16 + (rok ? i : 0);
+  MMAcc2<VT> acc;
+  acc.zero();
+  MMBuf<VT> cur, nxt;
+  // (segment, column group) cursor over the tile's segments with columns
+  int sk = tl.s0;
+  while (sk < tl.s1 && segs[sk].cnt <= 0) sk++;
+  if (sk < tl.s1) {
+    Seg sg = segs[sk];
+    int kk = 0;
+    cur.load((sg.far ? fUt : nval) + sg.base + r, sg.ld, rok, (sg.far ? ts : xs) + (int64_t)sg.xoff * NP * MM_W, kk,
+             sg.cnt, g, i);
+    for (;;) {
+      int nk = sk, nkk = kk + 16;
+      Seg ng = sg;
+      if (nkk >= sg.cnt) {
+        nkk = 0;
+        nk++;
+        while (nk < tl.s1 && segs[nk].cnt <= 0) nk++;
+        if (nk < tl.s1) ng = segs[nk];
+      }
+      const bool more = nk < tl.s1;
+      if (more)
+        nxt.load((ng.far ? fUt : nval) + ng.base + r, ng.ld, rok, (ng.far ? ts : xs) + (int64_t)ng.xoff * NP * MM_W,
+                 nkk, ng.cnt, g, i);
+      acc.run(cur);
+      if (!more) break;
+      cur = nxt;
+      sk = nk;
+      kk = nkk;
+      sg = ng;
+    }
+  }
+#pragma unroll
+  for (int t = 0; t < 4; t++)
+#pragma unroll
+    for (int v = 0; v < 4; v++) {
+      const int rr = st.strip * 16 + MF<R>::row(g, v);
+      if (rr < tl.len) ym[(int64_t)(tl.p0 + rr) * MM_W + 16 * t + i] = acc.get(t, v);
+    }
+}
+
 template <class S, class D>
 __global__ void k_convert(const S* __restrict__ a, D* __restrict__ b, int64_t n) {
   for (int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x; i < n; i += (int64_t)gridDim.x * 256) cvt(b[i], a[i]);
@@ -3909,6 +4151,13 @@ void build_matvec_plan(Operator& op, hipStream_t s) {
   op.d_tile_fwd.upload(t_fwd, s);
   op.d_seg_fwd.upload(s_fwd, s);
   op.n_tiles_fwd = (int)t_fwd.size();
+  {  // block product: one wave per 16-row strip of a row tile
+    std::vector<MMStrip> strips;
+    for (size_t t = 0; t < t_fwd.size(); t++)
+      for (int r = 0; r < t_fwd[t].len; r += 16) strips.push_back(MMStrip{(int)t, r / 16});
+    op.d_mm_strips.upload(strips, s);
+    op.n_mm_strips = (int)strips.size();
+  }
   op.d_tile_trn.upload(t_trn, s);
   op.d_seg_trn.upload(s_trn, s);
   op.n_tiles_trn = (int)t_trn.size();
@@ -4772,8 +5021,80 @@ static void apply_mm_t(Operator& op, const XT* x, XT* y, int nrhs, int64_t ld, X
   }
 }
 
+// the same in slab form (k_mm2_*), the default for FP32 storage (its double
+// buffers of FP64 values cost more occupancy than they gain: 28.3 against
+// 22.0 ms at the Maxwell EFIE workload); NGBEM_MM=v1 keeps the four-product
+// kernels above for both
+static bool mm_slab() {
+  static const bool on = [] {
+    const char* e = std::getenv("NGBEM_MM");
+    return !(e && std::string(e) == "v1");
+  }();
+  return on;
+}
+
+template <class ST, class VT, class XT>
+static void apply_mm2_t(Operator& op, const XT* x, XT* y, int nrhs, int64_t ld, XT alpha, XT beta, bool beta_zero,
+                        const ST* nval, const ST* fU, const ST* fV, hipStream_t s) {
+  typedef typename SlabT<VT>::R R;
+  constexpr int NP = SlabT<VT>::NP;
+  const int nin = op.ndof_trial(), nout = op.ndof_test();
+  if (op.n_mm_items < 0) {
+    std::vector<MMItem> items;
+    for (size_t k = 0; k < op.far.size(); k++)
+      for (int l = 0; l < op.far[k].R; l += 16) items.push_back(MMItem{(int)k, l});
+    op.d_mm_items.upload(items, s);
+    BEM_HIP(hipStreamSynchronize(s));
+    op.n_mm_items = (int)items.size();
+  }
+  const size_t need_x = (size_t)std::max(nin, 1) * MM_W * NP * sizeof(R);
+  const size_t need_y = (size_t)std::max(nout, 1) * MM_W * sizeof(VT);
+  const size_t need_t = (size_t)std::max(op.t_total, 1) * MM_W * NP * sizeof(R);
+  if (op.d_xm.bytes < need_x || op.d_ym.bytes < need_y || op.d_tm.bytes < need_t)
+    BEM_HIP(hipDeviceSynchronize());  // earlier block products may still use the slabs
+  if (op.d_xm.bytes < need_x) op.d_xm.alloc(need_x);
+  if (op.d_ym.bytes < need_y) op.d_ym.alloc(need_y);
+  if (op.d_tm.bytes < need_t) op.d_tm.alloc(need_t);
+  R* xs = op.d_xm.as<R>();
+  R* ts = op.d_tm.as<R>();
+  VT* ym = op.d_ym.as<VT>();
+  for (int r0 = 0; r0 < nrhs; r0 += MM_W) {
+    const int nr = std::min(MM_W, nrhs - r0);
+    hipLaunchKernelGGL((k_gather_slab<VT, XT>), dim3((unsigned)(((int64_t)nin * MM_W + 255) / 256)), dim3(256), 0, s,
+                       op.d_perm_trial.as<int>(), nin, x, ld, r0, nr, xs);
+    if (op.n_mm_items > 0)
+      hipLaunchKernelGGL((k_mm2_far_t<ST, VT>), dim3(op.n_mm_items), dim3(256), 0, s, op.d_mm_items.as<MMItem>(),
+                         op.d_far_blk.as<FarBlk>(), fV, (const R*)xs, ts);
+    if (op.n_mm_strips > 0)
+      hipLaunchKernelGGL((k_mm2_rows<ST, VT>), dim3((unsigned)((op.n_mm_strips + 3) / 4)), dim3(256), 0, s,
+                         op.d_mm_strips.as<MMStrip>(), op.n_mm_strips, op.d_tile_fwd.as<Tile>(),
+                         op.d_seg_fwd_mm.as<Seg>(), nval, fU, (const R*)xs, (const R*)ts, ym);
+    hipLaunchKernelGGL((k_scatter_mm<XT, VT>), dim3((unsigned)(((int64_t)nout * MM_W + 255) / 256)), dim3(256), 0, s,
+                       op.d_perm_test.as<int>(), nout, ym, y, ld, r0, nr, alpha, beta, (int)beta_zero);
+  }
+}
+
 void apply_device_mm(Operator& op, int flags, const double* alpha, const void* x, const double* beta, void* y,
                      int nrhs, int64_t ld, hipStream_t s) {
+  if (mm_slab() && (flags & 4)) {
+    if (!op.assembled) throw std::runtime_error("operator not assembled");
+    if (flags & 3) throw std::runtime_error("block apply is forward only");
+    const bool mixed = (flags & 4) != 0;
+    if (mixed && !op.mixed) throw std::runtime_error("operator was created without the mixed-precision copy");
+    const bool beta_zero = beta[0] == 0.0 && beta[1] == 0.0;
+    if (op.complex_valued) {
+      const zc a{alpha[0], alpha[1]}, b{beta[0], beta[1]};
+      apply_mm2_t<fc, fc, zc>(op, (const zc*)x, (zc*)y, nrhs, ld, a, b, beta_zero, op.d_near_val32.as<fc>(),
+                              op.d_fU32.as<fc>(), op.d_fV32.as<fc>(), s);
+    } else {
+      if (alpha[1] != 0.0 || beta[1] != 0.0) throw std::runtime_error("complex scalars on a real operator");
+      apply_mm2_t<float, float, double>(op, (const double*)x, (double*)y, nrhs, ld, alpha[0], beta[0], beta_zero,
+                                        op.d_near_val32.as<float>(), op.d_fU32.as<float>(), op.d_fV32.as<float>(),
+                                        s);
+    }
+    BEM_HIP(hipGetLastError());
+    return;
+  }
   if (!op.assembled) throw std::runtime_error("operator not assembled");
   if (flags & 3) throw std::runtime_error("block apply is forward only");
   const bool mixed = (flags & 4) != 0;

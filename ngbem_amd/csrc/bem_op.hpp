@@ -301,6 +301,8 @@ struct Operator {
   // block apply (64 right-hand sides): work items built on first use, slabs
   DBuf d_mm_items, d_xm, d_ym, d_tm;
   int n_mm_items = -1;
+  DBuf d_mm_strips;  // (row tile, 16-row strip) of the slab-form block product
+  int n_mm_strips = 0;
 
   int ndof_trial() const { return trial->ndof; }
   int ndof_test() const { return test->ndof; }
