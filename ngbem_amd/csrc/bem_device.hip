@@ -1966,11 +1966,19 @@ __global__ void k_gather(const int* __restrict__ perm, int n, const XT* __restri
 
 template <class YT, class VT>
 __global__ void k_scatter(const int* __restrict__ perm, int n, const VT* __restrict__ yp, YT* __restrict__ y,
-                          YT alpha, YT beta, int beta_zero, int conj) {
+                          YT alpha, YT beta, int beta_zero, int conj, const int2* __restrict__ cut,
+                          const VT* __restrict__ part) {
   const int i = blockIdx.x * 256 + threadIdx.x;
   if (i >= n) return;
   YT v;
-  cvt(v, yp[i]);
+  const int2 ct = cut ? cut[i] : make_int2(-1, 0);
+  if (ct.x < 0) {
+    cvt(v, yp[i]);
+  } else {  // a row of a cut tile: its pieces' partial sums in piece order
+    VT a = part[ct.x];
+    for (int k = 1; k < ct.y; k++) a += part[ct.x + 64 * k];
+    cvt(v, a);
+  }
   if (conj) v = cconj(v);
   const int d = perm[i];
   YT r = alpha * v;
@@ -2212,7 +2220,7 @@ __device__ __forceinline__ void col_step(const int2 c, const ST* __restrict__ nv
 // A tile's column list is cut into pieces of at most a set number of columns
 // (one workgroup each), so that the few tiles with twice the average work do
 // not run alone at the end of the launch; a piece of a cut tile writes its 64
-// partial sums to its slot, and k_mv_rows_psum adds a tile's slots in order.
+// partial sums to its slot, and the scatter adds a tile's slots in order.
 struct RowPiece {
   int tile, c0, c1, slot;  // slot < 0: the tile's only piece, straight to y
 };
@@ -2289,20 +2297,6 @@ __global__ __launch_bounds__(256) void k_mv_rows_flat(const Tile* __restrict__ t
     else part[(int64_t)pc.slot * 64 + lane] = v;
   }
   }
-}
-
-// y rows of the cut tiles: the pieces' partial sums in piece order
-// (tile, first slot, slot count)
-template <class VT>
-__global__ __launch_bounds__(64) void k_mv_rows_psum(const Tile* __restrict__ tiles, const int4* __restrict__ sums,
-                                                     const VT* __restrict__ part, VT* __restrict__ yp) {
-  const int4 sm = sums[blockIdx.x];
-  const Tile tl = tiles[sm.x];
-  const int lane = threadIdx.x;
-  if (lane >= tl.len) return;
-  VT a = part[(int64_t)sm.y * 64 + lane];
-  for (int k = 1; k < sm.z; k++) a += part[(int64_t)(sm.y + k) * 64 + lane];
-  yp[tl.p0 + lane] = a;
 }
 
 // transpose: one workgroup per (leaf column cluster, 64-column chunk); dense
@@ -3942,6 +3936,15 @@ static void upload_pieces(Operator& op, int prec, hipStream_t s) {
   op.n_fpieces[prec] = (int)pieces.size();
   op.d_fsums[prec].upload(sums, s);
   op.n_fsums[prec] = (int)sums.size();
+  if (!sums.empty()) {  // per output position: (first partial, pieces) of a cut tile's row, else -1
+    std::vector<int2> cut(std::max(op.ndof_test(), 1), make_int2(-1, 0));
+    for (const int4& sm : sums) {
+      const int2 tp = op.h_tile_pos[sm.x];
+      for (int r = 0; r < tp.y; r++) cut[tp.x + r] = make_int2(sm.y * 64 + r, sm.z);
+    }
+    op.d_fcut[prec].upload(cut, s);
+    BEM_HIP(hipStreamSynchronize(s));
+  }
   const size_t need = (size_t)std::max(slots, 1) * 64 * (op.complex_valued ? 16 : 8);
   if (op.d_fpart.bytes < need) op.d_fpart.alloc(need);
 }
@@ -4122,6 +4125,8 @@ void build_matvec_plan(Operator& op, hipStream_t s) {
     if (e) std::rethrow_exception(e);
   pt_.mark("host tables");
   op.d_seg_fwd_mm.upload(s_ut, s);
+  op.h_tile_pos.resize(t_fwd.size());
+  for (size_t t = 0; t < t_fwd.size(); t++) op.h_tile_pos[t] = make_int2(t_fwd[t].p0, t_fwd[t].len);
   op.panels = flat_ok && pan_ok;
   if (op.panels) {
     op.d_fcols_p.upload(pcols, s);
@@ -4151,6 +4156,7 @@ void build_matvec_plan(Operator& op, hipStream_t s) {
   op.d_tile_fwd.upload(t_fwd, s);
   op.d_seg_fwd.upload(s_fwd, s);
   op.n_tiles_fwd = (int)t_fwd.size();
+
   {  // block product: one wave per 16-row strip of a row tile
     std::vector<MMStrip> strips;
     for (size_t t = 0; t < t_fwd.size(); t++)
@@ -4915,6 +4921,7 @@ static void apply_t(Operator& op, int flags, const XT* x, XT* y, XT alpha, XT be
   const int* pout = trans ? op.d_perm_trial.as<int>() : op.d_perm_test.as<int>();
   const int nfar = (int)op.far.size();
   VT* z = trans ? op.d_zT.as<VT>() : op.d_z.as<VT>();
+  const int2* cut = nullptr;  // forward row tiles cut into pieces: per position (first partial, count)
   mark(0);
   hipLaunchKernelGGL((k_gather<VT, XT>), dim3((nin + 255) / 256), dim3(256), 0, s, pin, nin, x, xp, 0);
   mark(1);
@@ -4943,9 +4950,9 @@ static void apply_t(Operator& op, int flags, const XT* x, XT* y, XT alpha, XT be
                          op.d_tile_fwd.as<Tile>(), op.d_fpieces[prec].as<RowPiece>(),
                          (pan ? op.d_fcols_p : op.d_fcols).as<int2>(), nval, pan ? op.d_fUp.as<ST>() : fU, xp, tb,
                          yp, op.d_fpart.as<VT>(), xcd_remap(), pers ? np : 0);
-      if (op.n_fsums[prec])
-        hipLaunchKernelGGL((k_mv_rows_psum<VT>), dim3(op.n_fsums[prec]), dim3(64), 0, s, op.d_tile_fwd.as<Tile>(),
-                           op.d_fsums[prec].as<int4>(), (const VT*)op.d_fpart.as<VT>(), yp);
+      if (op.n_fsums[prec]) {  // the cut tiles' rows: summed in the scatter
+        cut = op.d_fcut[prec].as<int2>();
+      }
     } else
       hipLaunchKernelGGL((k_mv_rows<ST, VT>), dim3(op.n_tiles_fwd), dim3(256), 0, s, op.d_tile_fwd.as<Tile>(),
                          segs.as<Seg>(), nval, fU, z, xp, tb, yp);
@@ -4979,7 +4986,7 @@ static void apply_t(Operator& op, int flags, const XT* x, XT* y, XT alpha, XT be
   }
   mark(4);
   hipLaunchKernelGGL((k_scatter<XT, VT>), dim3((nout + 255) / 256), dim3(256), 0, s, pout, nout, yp, y, alpha, beta,
-                     (int)beta_zero, 0);
+                     (int)beta_zero, 0, cut, (const VT*)op.d_fpart.as<VT>());
   mark(5);
 }
 

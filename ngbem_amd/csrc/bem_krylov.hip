@@ -4,32 +4,42 @@
 // reference's scripts (ngsolve.krylovspace.GMRes, as tests/test_Maxwell_Mie.py
 // and demos/HelmholtzBurtonMiller.ipynb call it) orthogonalise with host-side
 // inner products; here every inner product is a fixed-order reduction on the
-// GPU, so one step is five launches and the host only reads the Hessenberg
+// GPU, so one step is four launches and the host only reads the Hessenberg
 // column when it wants to (asynchronously, for the Givens rotations).
 //
-// Launches per step, for nrow = k+1 basis rows of length n:
-//   k_kr_mdot   h_j = sum_i conj(V[j][i]) w[i], partial sums per chunk of KR_CH
-//               elements and per group of KR_RB rows (one pass over V and w)
-//   k_kr_axpy   each workgroup adds the chunk partials of every h_j (in chunk
-//               order), block 0 stores h, then w[i] -= sum_j V[j][i] h_j
-//   (again: the second Gram-Schmidt pass, h accumulated into the output)
-//   k_kr_scale  |w| from the per-workgroup partial sums of |w_i|^2 (fixed
-//               order), V[k+1] = w / |w| (zero when |w| = 0)
-// HBM traffic per step about 4 nrow n values: HBM-bound, like the H-matvec.
+// Launches per step, for nrow = k+1 basis rows of length n, in chunks of 256
+// elements (one workgroup each):
+//   k_kr_mdot   h_j = sum_i conj(V[j][i]) w[i], partial sums per chunk and
+//               per group of KR_RB rows (one pass over V and w)
+//   k_kr_axpy   each workgroup adds the chunk partials of every h_j (fixed
+//               order: lane l sums chunks l, l + 64, ..., then a butterfly;
+//               one wave per row), block 0 stores h, w[i] -= sum_j V[j][i] h_j
+//               on its chunk, then the second pass's partial inner products
+//               of its chunk (its rows of V are still in the cache)
+//   k_kr_axpy   the second pass: h accumulated into the output, w updated, the
+//               per-workgroup sums of |w_i|^2
+//   k_kr_scale  |w| from those sums (fixed order), V[k+1] = w / |w| (zero
+//               when |w| = 0)
+// HBM traffic per step about 4 nrow n values.  The steps are short (tens of
+// microseconds at n = 81,920): the launches and the reductions' latency, not
+// the bytes, set their time.
 // The basis may be stored in FP32 / complex64 (the inner solve of a mixed-
 // precision refinement): its values are widened on load and every product is
 // accumulated in FP64 against the FP64 / complex128 vector w.
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <cstdint>
+#include <cstdlib>
 #include <stdexcept>
+#include <string>
 
 #include "bem_op.hpp"
 
 namespace bem {
 namespace {
 
-constexpr int KR_CH = 2048;  // elements per multi-dot chunk (8 per thread)
+constexpr int KR_CH = 256;   // elements per multi-dot chunk (one per thread): the axpy's workgroups
 constexpr int KR_RB = 8;     // basis rows per multi-dot workgroup
 
 // value helpers for double and complex128 (double2: x real, y imaginary)
@@ -112,37 +122,75 @@ __global__ __launch_bounds__(256) void k_kr_mdot(const TV* __restrict__ V, int64
   }
 }
 
-// w -= V^T h with h_j = sum_c part[j * nch + c]; block 0 writes h (or adds it
-// to hout); with npart, the per-workgroup sums of |w_i|^2 of the result
+// sum over the nch chunk partials of one row in a fixed order: lane l takes
+// chunks l, l + 64, ..., then a butterfly (the same in every workgroup)
+template <class T>
+__device__ __forceinline__ T kr_sum_row(const T* __restrict__ in, int nch, int lane) {
+  T s = kr_zero<T>();
+  for (int c = lane; c < nch; c += 64) kr_add(s, in[c]);
+  return kr_wave_sum(s);
+}
+
+// w -= V^T h with h_j = sum_c part[j * nch + c] (one wave per row j); block 0
+// writes h (or adds it to hout); with npart, the per-workgroup sums of
+// |w_i|^2 of the result; with next, this chunk's partial inner products of
+// the result with every basis row (the next Gram-Schmidt pass)
 template <class T, class TV>
 __global__ __launch_bounds__(256) void k_kr_axpy(const TV* __restrict__ V, int64_t ldv, int nrow, T* __restrict__ w,
                                                  int n, const T* __restrict__ part, int nch, T* __restrict__ hout,
-                                                 int accumulate, double* __restrict__ npart) {
+                                                 int accumulate, double* __restrict__ npart, T* __restrict__ next) {
   __shared__ T s_h[KRYLOV_MAX_BASIS];
+  __shared__ T s_red[4][KR_RB];
   __shared__ double s_n[4];
-  for (int j = threadIdx.x; j < nrow; j += 256) {
-    T s = part[(int64_t)j * nch];
-    for (int c = 1; c < nch; c++) kr_add(s, part[(int64_t)j * nch + c]);
-    s_h[j] = s;
-    if (blockIdx.x == 0) {
-      if (accumulate) kr_add(hout[j], s);
-      else hout[j] = s;
+  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+  for (int j = wv; j < nrow; j += 4) {
+    const T sj = kr_sum_row(part + (int64_t)j * nch, nch, lane);
+    if (lane == 0) {
+      s_h[j] = sj;
+      if (blockIdx.x == 0) {
+        if (accumulate) kr_add(hout[j], sj);
+        else hout[j] = sj;
+      }
     }
   }
   __syncthreads();
   const int i = blockIdx.x * 256 + threadIdx.x;
   double a2 = 0.0;
+  T wi = kr_zero<T>();
   if (i < n) {
-    T wi = w[i];
+    wi = w[i];
     for (int j = 0; j < nrow; j++) kr_sub(wi, kr_mul(kr_wide(V[(int64_t)j * ldv + i]), s_h[j]));
     w[i] = wi;
     a2 = kr_abs2(wi);
   }
   if (npart) {
     a2 = kr_wave_sum(a2);
-    if ((threadIdx.x & 63) == 0) s_n[threadIdx.x >> 6] = a2;
+    if (lane == 0) s_n[wv] = a2;
     __syncthreads();
     if (threadIdx.x == 0) npart[blockIdx.x] = s_n[0] + s_n[1] + s_n[2] + s_n[3];
+  }
+  if (next) {
+    for (int j0 = 0; j0 < nrow; j0 += KR_RB) {
+      const int nr = min(KR_RB, nrow - j0);
+      T acc[KR_RB];
+#pragma unroll
+      for (int r = 0; r < KR_RB; r++) {
+        acc[r] = kr_zero<T>();
+        if (r < nr && i < n) acc[r] = kr_conj_mul(kr_wide(V[(int64_t)(j0 + r) * ldv + i]), wi);
+        const T sr = kr_wave_sum(acc[r]);
+        if (lane == 0) s_red[wv][r] = sr;
+      }
+      __syncthreads();
+      if ((int)threadIdx.x < nr) {
+        const int r = threadIdx.x;
+        T sr = s_red[0][r];
+        kr_add(sr, s_red[1][r]);
+        kr_add(sr, s_red[2][r]);
+        kr_add(sr, s_red[3][r]);
+        next[(int64_t)(j0 + r) * nch + blockIdx.x] = sr;
+      }
+      __syncthreads();
+    }
   }
 }
 
@@ -170,14 +218,16 @@ template <class T, class TV>
 void arnoldi_t(TV* V, int64_t ldv, int k, int n, T* w, T* h, unsigned char* work, hipStream_t s) {
   const int nrow = k + 1;
   const int nch = (int)nchunks(n), nblk = (int)nblocks(n);
+  const int64_t pb = arnoldi_part_bytes(nrow, n, sizeof(T));
   T* part = (T*)work;
-  double* npart = (double*)(work + arnoldi_part_bytes(nrow, n, sizeof(T)));
+  T* part2 = (T*)(work + pb);
+  double* npart = (double*)(work + 2 * pb);
   const dim3 gd((unsigned)nch, (unsigned)((nrow + KR_RB - 1) / KR_RB));
-  for (int pass = 0; pass < 2; pass++) {
-    hipLaunchKernelGGL((k_kr_mdot<T, TV>), gd, dim3(256), 0, s, (const TV*)V, ldv, nrow, (const T*)w, n, part, nch);
-    hipLaunchKernelGGL((k_kr_axpy<T, TV>), dim3((unsigned)nblk), dim3(256), 0, s, (const TV*)V, ldv, nrow, w, n,
-                       (const T*)part, nch, h, pass, pass == 1 ? npart : (double*)nullptr);
-  }
+  hipLaunchKernelGGL((k_kr_mdot<T, TV>), gd, dim3(256), 0, s, (const TV*)V, ldv, nrow, (const T*)w, n, part, nch);
+  hipLaunchKernelGGL((k_kr_axpy<T, TV>), dim3((unsigned)nblk), dim3(256), 0, s, (const TV*)V, ldv, nrow, w, n,
+                     (const T*)part, nch, h, 0, (double*)nullptr, part2);
+  hipLaunchKernelGGL((k_kr_axpy<T, TV>), dim3((unsigned)nblk), dim3(256), 0, s, (const TV*)V, ldv, nrow, w, n,
+                     (const T*)part2, nch, h, 1, npart, (T*)nullptr);
   hipLaunchKernelGGL((k_kr_scale<T, TV>), dim3((unsigned)nblk), dim3(256), 0, s, (const T*)w, n,
                      (const double*)npart, nblk, V + (int64_t)nrow * ldv, h + nrow);
   BEM_HIP(hipGetLastError());
@@ -190,7 +240,7 @@ int64_t arnoldi_part_bytes(int nrow, int n, size_t esz) {
 }
 
 int64_t arnoldi_work_bytes(int nrow, int n, bool cplx) {
-  return arnoldi_part_bytes(nrow, n, cplx ? 16 : 8) + nblocks(n) * 8;
+  return 2 * arnoldi_part_bytes(nrow, n, cplx ? 16 : 8) + nblocks(n) * 8;
 }
 
 void arnoldi_step(bool cplx, bool low, void* V, int64_t ldv, int k, int n, void* w, void* h, void* work,

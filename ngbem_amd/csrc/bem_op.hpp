@@ -263,6 +263,8 @@ struct Operator {
   // and per precision (0 the FP64 / complex128 storage, 1 the FP32 copy) the
   // pieces the lists are cut into and the sums of the cut tiles; partial sums
   DBuf d_fcols, d_fpieces[2], d_fsums[2], d_fpart;
+  DBuf d_fcut[2];                    // per output position of a cut tile: (first partial, pieces) for the scatter
+  std::vector<int2> h_tile_pos;      // forward row tiles: (first position, rows)
   int n_fpieces[2] = {0, 0}, n_fsums[2] = {0, 0};
   std::vector<int2> h_frange;
   // the FP32 copy's forward lists over U panels: per (row tile, low-rank

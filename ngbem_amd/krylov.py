@@ -23,10 +23,10 @@ def _dot(a, b):
 
 def _apply(A, x):
     if isinstance(A, BaseMatrix):
-        y = torch.zeros(x.shape[:-1] + (A.height,),
+        y = torch.empty(x.shape[:-1] + (A.height,),
                         dtype=torch.complex128 if (A.is_complex or torch.is_complex(x)) else torch.float64,
                         device=x.device)
-        A.MultAdd(1.0, x, y)
+        A.MultScale(1.0, x, y)
         return y
     return A(x)
 

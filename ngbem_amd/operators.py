@@ -102,6 +102,15 @@ class BaseMatrix:
         y.zero_()
         self.MultAdd(1.0, x, y)
 
+    # y = s * A x, whatever y held (an extension used by the Krylov solvers:
+    # an H-matrix writes it in its own scatter, without a separate clearing of
+    # y, and a sum of terms starts with the term that can do that)
+    _native_scale = False
+
+    def MultScale(self, s, x, y):
+        y.zero_()
+        self.MultAdd(s, x, y)
+
     def MultTrans(self, x, y):
         y.zero_()
         self.MultTransAdd(1.0, x, y)
@@ -168,6 +177,13 @@ class ScaledMatrix(BaseMatrix):
     def MultTransAdd(self, s, x, y):
         self.A.MultTransAdd(s * self.s, x, y)
 
+    @property
+    def _native_scale(self):
+        return self.A._native_scale
+
+    def MultScale(self, s, x, y):
+        self.A.MultScale(s * self.s, x, y)
+
 
 class SumMatrix(BaseMatrix):
     def __init__(self, A, B, sign):
@@ -180,6 +196,18 @@ class SumMatrix(BaseMatrix):
     def MultAdd(self, s, x, y):
         self.A.MultAdd(s, x, y)
         self.B.MultAdd(s * self.sign, x, y)
+
+    @property
+    def _native_scale(self):
+        return self.A._native_scale or self.B._native_scale
+
+    def MultScale(self, s, x, y):
+        if self.B._native_scale and not self.A._native_scale:
+            self.B.MultScale(s * self.sign, x, y)
+            self.A.MultAdd(s, x, y)
+        else:
+            self.A.MultScale(s, x, y)
+            self.B.MultAdd(s * self.sign, x, y)
 
     def MultTransAdd(self, s, x, y):
         self.A.MultTransAdd(s, x, y)
@@ -432,6 +460,26 @@ class HMatrix(BaseMatrix):
 
     def IsComplex(self):
         return self.is_complex
+
+    _native_scale = True
+
+    def MultScale(self, s, x, y):
+        """y = s A x in one apply (beta = 0 in the scatter)."""
+        s = complex(s)
+        if (isinstance(x, torch.Tensor) and x.is_cuda and y.is_cuda and x.dim() == y.dim() and x.dim() <= 2
+                and x.shape[-1] == self.width and y.shape[-1] == self.height
+                and (x.dim() == 1 or x.shape[0] == y.shape[0]) and _raw_view(x) and _raw_view(y)
+                and x.is_contiguous() and y.is_contiguous()):
+            flags = (_capi.APPLY_MIXED if self._mixed else 0) | (
+                _capi.APPLY_GRAPH if self._graph and x.dim() == 1 else 0)
+            if self.is_complex and x.dtype == y.dtype == torch.complex128:
+                self._op._call_apply(flags, s, x, 0.0, y)
+                return
+            if not self.is_complex and x.dtype == y.dtype == torch.float64 and s.imag == 0.0:
+                self._op._call_apply(flags, s.real, x, 0.0, y)
+                return
+        y.zero_()
+        self.MultAdd(s, x, y)
 
     def _apply(self, flags, s, x, y):
         op = self._op
