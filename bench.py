@@ -316,7 +316,10 @@ def main():
         Mass = nb.MassMatrix(fes)
         f = torch.from_numpy(burton_miller_rhs(mesh, kappa)).to(dev)
         area = torch.from_numpy(mesh.areas()).to(dev)
-        pre = lambda r: r / area  # noqa: E731  (diagonal mass preconditioner)
+        # the inverse of the diagonal mass matrix, as the reference's demo
+        # (demos/HelmholtzBurtonMiller.ipynb: BilinearForm(u*v*ds,
+        # diagonal=True) ... Inverse()); GMRES applies it inside its Arnoldi step
+        pre = nb.MassMatrix(fes).Inverse()
         A64 = 0.5 * Mass + M64
         A32 = 0.5 * Mass + M32
         nb.GMRes(A64, f, pre=pre, tol=1e-4, maxsteps=5, printrates=False)  # warm up torch's kernels

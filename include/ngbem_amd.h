@@ -270,18 +270,24 @@ int bem_op_potential_noalloc(bem_op* op, const void* coef, const double* points,
  * (the reference's scripts run ngsolve.krylovspace.GMRes, whose inner
  * products are host reductions; no counterpart in ngbem itself).  V: basis
  * rows of length n with stride ldv (device, k + 2 rows); w: A v_k (device,
- * overwritten with the orthogonalised vector); h: device array of k + 2
- * values receiving the Hessenberg column (the coefficients of both classical
- * Gram-Schmidt passes added, then |w|); V row k + 1 receives w / |w| (zero
- * when |w| = 0).  work: device buffer of bem_arnoldi_work_bytes(k, n, dtype)
- * bytes.  dtype is the basis type: BEM_F64 / BEM_C128, or BEM_F32 / BEM_C64
- * for the low-precision basis of a mixed-precision inner solve (values
- * widened on load, products accumulated in FP64); w and h are double for a
- * real basis and complex128 for a complex one.  k + 1 < 1024.  Asynchronous
- * on stream; every inner product is a fixed-order reduction (deterministic). */
+ * overwritten with the orthogonalised vector); dscale: NULL, or a device
+ * array of n doubles, the diagonal of a left preconditioner D (the step then
+ * orthogonalises D w, e.g. the inverse of a diagonal mass matrix, without a
+ * launch of its own); h: k + 2 values receiving the Hessenberg column (the
+ * coefficients of both classical Gram-Schmidt passes added, then |w|), in
+ * device memory or in pinned host memory (hipHostMalloc), written by the
+ * step's last launch only, so a host reader needs only an event recorded
+ * after the call; V row k + 1 receives w / |w| (zero when |w| = 0).  work:
+ * device buffer of bem_arnoldi_work_bytes(k, n, dtype) bytes.  dtype is the
+ * basis type: BEM_F64 / BEM_C128, or BEM_F32 / BEM_C64 for the low-precision
+ * basis of a mixed-precision inner solve (values widened on load, products
+ * accumulated in FP64); w and h are double for a real basis and complex128
+ * for a complex one.  k + 1 < 1024.  k = -1 starts a cycle: V row 0 receives
+ * D w / |D w| and h[0] = |D w| (one value).  Asynchronous on stream; every
+ * inner product is a fixed-order reduction (deterministic). */
 int64_t bem_arnoldi_work_bytes(int k, int n, int dtype);
-int bem_arnoldi_step(void* V, int64_t ldv, int k, int n, void* w, void* h, void* work, int64_t work_bytes, int dtype,
-                     void* stream);
+int bem_arnoldi_step(void* V, int64_t ldv, int k, int n, void* w, const double* dscale, void* h, void* work,
+                     int64_t work_bytes, int dtype, void* stream);
 
 /* introspection for parity tests */
 int bem_op_num_blocks(const bem_op* op);

@@ -106,7 +106,7 @@ def lib():
     L.bem_op_autotune.argtypes = [P, ctypes.c_int, ctypes.c_int, ctypes.c_char_p, ctypes.c_int, P]
     L.bem_arnoldi_work_bytes.restype = ctypes.c_int64
     L.bem_arnoldi_work_bytes.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.c_int]
-    L.bem_arnoldi_step.argtypes = [P, ctypes.c_int64, ctypes.c_int, ctypes.c_int, P, P, P, ctypes.c_int64,
+    L.bem_arnoldi_step.argtypes = [P, ctypes.c_int64, ctypes.c_int, ctypes.c_int, P, P, P, P, ctypes.c_int64,
                                    ctypes.c_int, P]
     L.bem_batch_create.argtypes = [ctypes.POINTER(P), ctypes.c_int, ctypes.c_int, ctypes.POINTER(P)]
     L.bem_batch_sizes.argtypes = [P, ctypes.POINTER(ctypes.c_int), ctypes.POINTER(ctypes.c_int),

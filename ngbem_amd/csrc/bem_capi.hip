@@ -718,16 +718,16 @@ int64_t bem_arnoldi_work_bytes(int k, int n, int dtype) {
   return bem::arnoldi_work_bytes(k + 1, n, basis_complex(dtype));
 }
 
-int bem_arnoldi_step(void* V, int64_t ldv, int k, int n, void* w, void* h, void* work, int64_t work_bytes, int dtype,
-                     void* stream) {
-  if (!V || !w || !h || !work || k < 0 || n < 1 || ldv < n || !basis_dtype_ok(dtype))
+int bem_arnoldi_step(void* V, int64_t ldv, int k, int n, void* w, const double* dscale, void* h, void* work,
+                     int64_t work_bytes, int dtype, void* stream) {
+  if (!V || !w || !h || !work || k < -1 || n < 1 || ldv < n || !basis_dtype_ok(dtype))
     return fail(-1, "bem_arnoldi_step: bad arguments");
   if (k + 1 >= bem::KRYLOV_MAX_BASIS)
     return fail(-1, "bem_arnoldi_step: more than " + std::to_string(bem::KRYLOV_MAX_BASIS - 1) + " basis vectors");
   if (work_bytes < bem::arnoldi_work_bytes(k + 1, n, basis_complex(dtype)))
     return fail(-1, "bem_arnoldi_step: workspace smaller than bem_arnoldi_work_bytes");
   return guarded([&] {
-    bem::arnoldi_step(basis_complex(dtype), dtype == BEM_F32 || dtype == BEM_C64, V, ldv, k, n, w, h, work,
+    bem::arnoldi_step(basis_complex(dtype), dtype == BEM_F32 || dtype == BEM_C64, V, ldv, k, n, w, dscale, h, work,
                       (hipStream_t)stream);
   });
 }

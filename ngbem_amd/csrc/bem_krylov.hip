@@ -19,7 +19,14 @@
 //   k_kr_axpy   the second pass: h accumulated into the output, w updated, the
 //               per-workgroup sums of |w_i|^2
 //   k_kr_scale  |w| from those sums (fixed order), V[k+1] = w / |w| (zero
-//               when |w| = 0)
+//               when |w| = 0); block 0 writes the Hessenberg column
+// k = -1 starts a cycle: two launches (the second and the last of the list)
+// give V[0] = w / |w| and h[0] = |w|.
+// With a diagonal left preconditioner d (dscale), the first two launches
+// read d_i w_i in place of w_i (the second stores the scaled, updated w), so
+// the preconditioner costs no launch of its own.  The column is accumulated
+// in the workspace and written once by the last launch, so it may go straight
+// to pinned host memory: no copy is queued behind the step.
 // HBM traffic per step about 4 nrow n values.  The steps are short (tens of
 // microseconds at n = 81,920): the launches and the reductions' latency, not
 // the bytes, set their time.
@@ -91,7 +98,8 @@ __device__ __forceinline__ T kr_wave_sum(T v) {
 // partial inner products: part[j * nch + c] = sum over chunk c of conj(V[j][i]) w[i]
 template <class T, class TV>
 __global__ __launch_bounds__(256) void k_kr_mdot(const TV* __restrict__ V, int64_t ldv, int nrow,
-                                                 const T* __restrict__ w, int n, T* __restrict__ part, int nch) {
+                                                 const T* __restrict__ w, const double* __restrict__ dsc, int n,
+                                                 T* __restrict__ part, int nch) {
   __shared__ T s_red[4][KR_RB];
   const int c = blockIdx.x, j0 = blockIdx.y * KR_RB;
   const int nr = min(KR_RB, nrow - j0);
@@ -100,7 +108,7 @@ __global__ __launch_bounds__(256) void k_kr_mdot(const TV* __restrict__ V, int64
 #pragma unroll
   for (int r = 0; r < KR_RB; r++) acc[r] = kr_zero<T>();
   for (int i = i0 + (int)threadIdx.x; i < i1; i += 256) {
-    const T wi = w[i];
+    const T wi = dsc ? kr_scale(w[i], dsc[i]) : w[i];
 #pragma unroll
     for (int r = 0; r < KR_RB; r++)
       if (r < nr) kr_add(acc[r], kr_conj_mul(kr_wide(V[(int64_t)(j0 + r) * ldv + i]), wi));
@@ -137,8 +145,9 @@ __device__ __forceinline__ T kr_sum_row(const T* __restrict__ in, int nch, int l
 // the result with every basis row (the next Gram-Schmidt pass)
 template <class T, class TV>
 __global__ __launch_bounds__(256) void k_kr_axpy(const TV* __restrict__ V, int64_t ldv, int nrow, T* __restrict__ w,
-                                                 int n, const T* __restrict__ part, int nch, T* __restrict__ hout,
-                                                 int accumulate, double* __restrict__ npart, T* __restrict__ next) {
+                                                 const double* __restrict__ dsc, int n, const T* __restrict__ part,
+                                                 int nch, T* __restrict__ hout, int accumulate,
+                                                 double* __restrict__ npart, T* __restrict__ next) {
   __shared__ T s_h[KRYLOV_MAX_BASIS];
   __shared__ T s_red[4][KR_RB];
   __shared__ double s_n[4];
@@ -158,7 +167,7 @@ __global__ __launch_bounds__(256) void k_kr_axpy(const TV* __restrict__ V, int64
   double a2 = 0.0;
   T wi = kr_zero<T>();
   if (i < n) {
-    wi = w[i];
+    wi = dsc ? kr_scale(w[i], dsc[i]) : w[i];
     for (int j = 0; j < nrow; j++) kr_sub(wi, kr_mul(kr_wide(V[(int64_t)j * ldv + i]), s_h[j]));
     w[i] = wi;
     a2 = kr_abs2(wi);
@@ -198,7 +207,8 @@ __global__ __launch_bounds__(256) void k_kr_axpy(const TV* __restrict__ V, int64
 // block 0 stores |w| at hn
 template <class T, class TV>
 __global__ __launch_bounds__(256) void k_kr_scale(const T* __restrict__ w, int n, const double* __restrict__ npart,
-                                                  int nblk, TV* __restrict__ v, T* __restrict__ hn) {
+                                                  int nblk, TV* __restrict__ v, const T* __restrict__ hwork, int nrow,
+                                                  T* __restrict__ h) {
   __shared__ double s_n[4];
   double a = 0.0;
   for (int b = threadIdx.x; b < nblk; b += 256) a += npart[b];
@@ -206,7 +216,10 @@ __global__ __launch_bounds__(256) void k_kr_scale(const T* __restrict__ w, int n
   if ((threadIdx.x & 63) == 0) s_n[threadIdx.x >> 6] = a;
   __syncthreads();
   const double nrm = sqrt(s_n[0] + s_n[1] + s_n[2] + s_n[3]);
-  if (blockIdx.x == 0 && threadIdx.x == 0) hn[0] = kr_real<T>(nrm);
+  if (blockIdx.x == 0) {  // the column: both passes' coefficients, then |w|
+    for (int j = threadIdx.x; j < nrow; j += 256) h[j] = hwork[j];
+    if (threadIdx.x == 0) h[nrow] = kr_real<T>(nrm);
+  }
   const int i = blockIdx.x * 256 + threadIdx.x;
   if (i < n) kr_store(v[i], nrm > 0.0 ? kr_scale(w[i], 1.0 / nrm) : kr_zero<T>());
 }
@@ -214,22 +227,35 @@ __global__ __launch_bounds__(256) void k_kr_scale(const T* __restrict__ w, int n
 int64_t nchunks(int n) { return ((int64_t)n + KR_CH - 1) / KR_CH; }
 int64_t nblocks(int n) { return ((int64_t)n + 255) / 256; }
 
+int64_t align256(int64_t b) { return (b + 255) / 256 * 256; }
+
 template <class T, class TV>
-void arnoldi_t(TV* V, int64_t ldv, int k, int n, T* w, T* h, unsigned char* work, hipStream_t s) {
+void arnoldi_t(TV* V, int64_t ldv, int k, int n, T* w, const double* dsc, T* h, unsigned char* work,
+               hipStream_t s) {
   const int nrow = k + 1;
   const int nch = (int)nchunks(n), nblk = (int)nblocks(n);
   const int64_t pb = arnoldi_part_bytes(nrow, n, sizeof(T));
   T* part = (T*)work;
   T* part2 = (T*)(work + pb);
   double* npart = (double*)(work + 2 * pb);
+  T* hwork = (T*)(work + 2 * pb + align256(nblocks(n) * 8));
+  if (nrow == 0) {  // the start of a cycle: V[0] = D w / |D w|, h[0] = |D w|
+    hipLaunchKernelGGL((k_kr_axpy<T, TV>), dim3((unsigned)nblk), dim3(256), 0, s, (const TV*)V, ldv, 0, w, dsc, n,
+                       (const T*)nullptr, nch, hwork, 0, npart, (T*)nullptr);
+    hipLaunchKernelGGL((k_kr_scale<T, TV>), dim3((unsigned)nblk), dim3(256), 0, s, (const T*)w, n,
+                       (const double*)npart, nblk, V, (const T*)hwork, 0, h);
+    BEM_HIP(hipGetLastError());
+    return;
+  }
   const dim3 gd((unsigned)nch, (unsigned)((nrow + KR_RB - 1) / KR_RB));
-  hipLaunchKernelGGL((k_kr_mdot<T, TV>), gd, dim3(256), 0, s, (const TV*)V, ldv, nrow, (const T*)w, n, part, nch);
-  hipLaunchKernelGGL((k_kr_axpy<T, TV>), dim3((unsigned)nblk), dim3(256), 0, s, (const TV*)V, ldv, nrow, w, n,
-                     (const T*)part, nch, h, 0, (double*)nullptr, part2);
-  hipLaunchKernelGGL((k_kr_axpy<T, TV>), dim3((unsigned)nblk), dim3(256), 0, s, (const TV*)V, ldv, nrow, w, n,
-                     (const T*)part2, nch, h, 1, npart, (T*)nullptr);
+  hipLaunchKernelGGL((k_kr_mdot<T, TV>), gd, dim3(256), 0, s, (const TV*)V, ldv, nrow, (const T*)w, dsc, n, part,
+                     nch);
+  hipLaunchKernelGGL((k_kr_axpy<T, TV>), dim3((unsigned)nblk), dim3(256), 0, s, (const TV*)V, ldv, nrow, w, dsc, n,
+                     (const T*)part, nch, hwork, 0, (double*)nullptr, part2);
+  hipLaunchKernelGGL((k_kr_axpy<T, TV>), dim3((unsigned)nblk), dim3(256), 0, s, (const TV*)V, ldv, nrow, w,
+                     (const double*)nullptr, n, (const T*)part2, nch, hwork, 1, npart, (T*)nullptr);
   hipLaunchKernelGGL((k_kr_scale<T, TV>), dim3((unsigned)nblk), dim3(256), 0, s, (const T*)w, n,
-                     (const double*)npart, nblk, V + (int64_t)nrow * ldv, h + nrow);
+                     (const double*)npart, nblk, V + (int64_t)nrow * ldv, (const T*)hwork, nrow, h);
   BEM_HIP(hipGetLastError());
 }
 
@@ -240,16 +266,18 @@ int64_t arnoldi_part_bytes(int nrow, int n, size_t esz) {
 }
 
 int64_t arnoldi_work_bytes(int nrow, int n, bool cplx) {
-  return 2 * arnoldi_part_bytes(nrow, n, cplx ? 16 : 8) + nblocks(n) * 8;
+  // two chunk-partial arrays, the per-workgroup norm sums, the column
+  return 2 * arnoldi_part_bytes(nrow, n, cplx ? 16 : 8) + align256(nblocks(n) * 8) +
+         align256((int64_t)(nrow + 1) * (cplx ? 16 : 8));
 }
 
-void arnoldi_step(bool cplx, bool low, void* V, int64_t ldv, int k, int n, void* w, void* h, void* work,
-                  hipStream_t s) {
+void arnoldi_step(bool cplx, bool low, void* V, int64_t ldv, int k, int n, void* w, const double* dscale, void* h,
+                  void* work, hipStream_t s) {
   unsigned char* wk = (unsigned char*)work;
-  if (cplx && low) arnoldi_t<double2, float2>((float2*)V, ldv, k, n, (double2*)w, (double2*)h, wk, s);
-  else if (cplx) arnoldi_t<double2, double2>((double2*)V, ldv, k, n, (double2*)w, (double2*)h, wk, s);
-  else if (low) arnoldi_t<double, float>((float*)V, ldv, k, n, (double*)w, (double*)h, wk, s);
-  else arnoldi_t<double, double>((double*)V, ldv, k, n, (double*)w, (double*)h, wk, s);
+  if (cplx && low) arnoldi_t<double2, float2>((float2*)V, ldv, k, n, (double2*)w, dscale, (double2*)h, wk, s);
+  else if (cplx) arnoldi_t<double2, double2>((double2*)V, ldv, k, n, (double2*)w, dscale, (double2*)h, wk, s);
+  else if (low) arnoldi_t<double, float>((float*)V, ldv, k, n, (double*)w, dscale, (double*)h, wk, s);
+  else arnoldi_t<double, double>((double*)V, ldv, k, n, (double*)w, dscale, (double*)h, wk, s);
 }
 
 }  // namespace bem

@@ -353,16 +353,18 @@ void potential_device_noalloc(Operator& op, const void* coef, const double* pts,
 // then the column count of the row-tile pieces; keeps the fastest of each and
 // returns a JSON report of every candidate's median milliseconds
 std::string autotune(Operator& op, int prec, int iters, hipStream_t s);
-// GMRES Arnoldi step on device vectors (bem_krylov.hip): w orthogonalised
-// against the rows V[0..k] by classical Gram-Schmidt twice, h[0..k] = the
-// coefficients of both passes, h[k+1] = |w|, V[k+1] = w / |w|; work holds
+// GMRES Arnoldi step on device vectors (bem_krylov.hip): w (times dscale
+// elementwise when dscale is not null) orthogonalised against the rows
+// V[0..k] by classical Gram-Schmidt twice, h[0..k] = the coefficients of both
+// passes, h[k+1] = |w|, V[k+1] = w / |w|; h (device or pinned host memory) is
+// written by the step's last launch only; work holds
 // arnoldi_work_bytes(k + 1, n, cplx) bytes.  At most KRYLOV_MAX_BASIS rows.
 constexpr int KRYLOV_MAX_BASIS = 1024;
 int64_t arnoldi_part_bytes(int nrow, int n, size_t esz);
 int64_t arnoldi_work_bytes(int nrow, int n, bool cplx);
 // low: the basis V is FP32 / complex64 (w and h stay FP64 / complex128)
-void arnoldi_step(bool cplx, bool low, void* V, int64_t ldv, int k, int n, void* w, void* h, void* work,
-                  hipStream_t s);
+void arnoldi_step(bool cplx, bool low, void* V, int64_t ldv, int k, int n, void* w, const double* dscale, void* h,
+                  void* work, hipStream_t s);
 // ACA of 1/|x_i - y_j| between two point clouds (TestCompression, method aca)
 void test_compression_aca(const double* X, int nx, const double* Y, int ny, double eps, int* rank, double* err,
                           double* seconds);

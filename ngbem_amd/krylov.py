@@ -99,8 +99,9 @@ def _arnoldi_for(m, n, dt, dev, basis_dt):
 class _DeviceArnoldi:
     """Basis, Hessenberg columns and workspace of one GMRES cycle on the GPU;
     step(k, w) orthogonalises w = P A v_k against v_0..v_k and writes v_{k+1}
-    (library call bem_arnoldi_step), then starts the copy of column k to
-    pinned host memory; column(k) waits for that copy only."""
+    (library call bem_arnoldi_step), whose last launch writes column k
+    straight into pinned host memory; column(k) waits for that launch only
+    (an event recorded after it)."""
 
     def __init__(self, m, n, dt, dev, basis_dt=None):
         from . import _capi
@@ -113,27 +114,40 @@ class _DeviceArnoldi:
         self.dtype_id = {torch.float64: _capi.F64, torch.complex128: _capi.C128, torch.float32: _capi.F32,
                          torch.complex64: _capi.C64}[basis_dt]
         self.V = torch.empty(m + 1, n, dtype=basis_dt, device=dev)
-        self.Hd = torch.zeros(m, m + 1, dtype=dt, device=dev)
         self.Hh = torch.zeros(m, m + 1, dtype=dt, pin_memory=True)
         nbytes = int(_capi.lib().bem_arnoldi_work_bytes(m, n, self.dtype_id))
         if nbytes < 0:
             raise RuntimeError(_capi.lib().bem_last_error().decode())
         self.work = torch.empty(nbytes, dtype=torch.uint8, device=dev)
         self.events = [torch.cuda.Event() for _ in range(m)]
+        self.beta_h = torch.zeros(1, dtype=dt, pin_memory=True)  # |r| at the start of a cycle
+        self.beta_ev = torch.cuda.Event()
+        self.y_h = torch.zeros(m, dtype=dt, pin_memory=True)  # the cycle's coefficients, on their way to the device
         self.n = n
         self.m = m
 
-    def step(self, k, w):
+    def start(self, r, dscale=None):
+        """V[0] = D r / |D r| (r overwritten with D r) and |D r| on its way
+        to beta_h (event beta_ev); D = diag(dscale) or the identity."""
+        self.step(-1, r, dscale)
+
+    def step(self, k, w, dscale=None):
+        """dscale: None, or a float64 device vector d (w is replaced by d * w
+        before the orthogonalisation: a diagonal left preconditioner)."""
         import ctypes
 
         w = w.contiguous()
+        if dscale is not None and (dscale.dtype != torch.float64 or dscale.shape != (self.n,)
+                                   or dscale.device != w.device or not dscale.is_contiguous()):
+            raise ValueError("dscale must be a contiguous float64 vector of n entries on the device of w")
         stream = torch.cuda.current_stream(w.device).cuda_stream
         self._capi.check(self._capi.lib().bem_arnoldi_step(
             ctypes.c_void_p(self.V.data_ptr()), self.n, k, self.n, ctypes.c_void_p(w.data_ptr()),
-            ctypes.c_void_p(self.Hd[k].data_ptr()), ctypes.c_void_p(self.work.data_ptr()), self.work.numel(),
-            self.dtype_id, ctypes.c_void_p(stream)), "bem_arnoldi_step")
-        self.Hh[k, : k + 2].copy_(self.Hd[k, : k + 2], non_blocking=True)
-        self.events[k].record()
+            ctypes.c_void_p(dscale.data_ptr() if dscale is not None else None),
+            ctypes.c_void_p((self.Hh[k] if k >= 0 else self.beta_h).data_ptr()),
+            ctypes.c_void_p(self.work.data_ptr()), self.work.numel(), self.dtype_id, ctypes.c_void_p(stream)),
+            "bem_arnoldi_step")
+        (self.events[k] if k >= 0 else self.beta_ev).record()
 
     def column(self, k):
         self.events[k].synchronize()
@@ -178,6 +192,11 @@ def GMRes(A, b, pre=None, freedofs=None, x=None, maxsteps=100, tol=None, innerpr
     n = b.shape[0]
     P = (lambda v: _apply(pre, v)) if pre is not None else (lambda v: v)  # noqa: E731
     dev_path = b.is_cuda
+    # a diagonal preconditioner (the inverse of a diagonal mass matrix) is
+    # applied inside the device Arnoldi step instead of by a launch of its own
+    dscale = getattr(pre, "_diagonal", None) if dev_path else None
+    if dscale is not None and (dscale.device != b.device or dscale.shape != (b.shape[0],)):
+        dscale = None
     if basis is not None and not dev_path:
         raise ValueError("a basis dtype needs GPU vectors")
     arn = _arnoldi_for(m, n, dt, b.device, basis) if dev_path else None
@@ -189,27 +208,47 @@ def GMRes(A, b, pre=None, freedofs=None, x=None, maxsteps=100, tol=None, innerpr
         x_zero = not bool(torch.any(x != 0))
     # (b = 0 with x = 0: beta = 0 below ends the solve at x = 0)
     while total < maxsteps:
-        r = P(b if x_zero else b - _apply(A, x)).to(dt)
+        r = b if x_zero else b - _apply(A, x)
+        if not (dev_path and dscale is not None):
+            r = P(r)  # else applied by the cycle's first Arnoldi launch
+        r = r.to(dt)
         x_zero = False
-        beta = float(torch.linalg.norm(r))
-        if bnorm is None:
-            bnorm = beta
-        if beta <= tol * bnorm:
-            break
-        Vb[0].copy_(r / beta)
         H = np.zeros((m + 1, m), dtype=cdt)
         cs = np.zeros(m, dtype=cdt)
         sn = np.zeros(m, dtype=cdt)
         g = np.zeros(m + 1, dtype=cdt)
-        g[0] = beta
+        if dev_path:
+            # |r| stays on the device: the first step is issued before the
+            # host reads it (written to pinned memory, read at the first column)
+            arn.start(r if r is not b else r.clone(), dscale)
+            beta = None
+        else:
+            beta = float(torch.linalg.norm(r))
+            if bnorm is None:
+                bnorm = beta
+            if beta <= tol * bnorm:
+                break
+            Vb[0].copy_(r / beta)
+            g[0] = beta
         k_used = 0
         stop = False
+        done_at_start = False
 
         hist = []
 
         def take(k):
             # column k has arrived: rotate, report, decide
-            nonlocal k_used, total, stop
+            nonlocal k_used, total, stop, beta, bnorm, done_at_start
+            if beta is None:
+                arn.beta_ev.synchronize()
+                beta = float(arn.beta_h[0].real)
+                g[0] = beta
+                if bnorm is None:
+                    bnorm = beta
+                if beta <= tol * bnorm:
+                    # converged before this cycle (or b = 0): the issued steps are dropped
+                    done_at_start = stop = True
+                    return
             hcol = arn.column(k) if dev_path else hcols[k]
             H[: k + 2, k] = hcol
             hn = float(hcol[k + 1].real)
@@ -233,9 +272,12 @@ def GMRes(A, b, pre=None, freedofs=None, x=None, maxsteps=100, tol=None, innerpr
         steps = min(m, maxsteps - total)
         for k in range(steps):
             v = Vb[k] if Vb.dtype == dt else Vb[k].to(dt)
-            w = P(_apply(A, v)).to(dt)
+            if dscale is not None:
+                w = _apply(A, v).to(dt)
+            else:
+                w = P(_apply(A, v)).to(dt)
             if dev_path:
-                arn.step(k, w)
+                arn.step(k, w, dscale)
                 if k >= 1 and k_used == k - 1:
                     take(k - 1)
                     if stop:
@@ -260,8 +302,17 @@ def GMRes(A, b, pre=None, freedofs=None, x=None, maxsteps=100, tol=None, innerpr
         else:
             if dev_path and steps > 0 and k_used < steps:
                 take(steps - 1)
+        if done_at_start:
+            break
         y = np.linalg.solve(np.triu(H[:k_used, :k_used]), g[:k_used])
-        yt = torch.from_numpy(y if torch.is_complex(x) else y.real.copy()).to(device=x.device, dtype=Vb.dtype)
+        y = y if torch.is_complex(x) else y.real.copy()
+        if dev_path:
+            # through pinned memory: an asynchronous copy (the next write to
+            # y_h follows a column read, which waits for later launches)
+            arn.y_h[:k_used].copy_(torch.from_numpy(y))
+            yt = arn.y_h[:k_used].to(device=x.device, non_blocking=True).to(Vb.dtype)
+        else:
+            yt = torch.from_numpy(y).to(device=x.device, dtype=Vb.dtype)
         x.add_(yt @ Vb[:k_used])
         if abs(g[k_used]) <= tol * bnorm:
             break

@@ -363,6 +363,8 @@ class _MassInverse(BaseMatrix):
             raise ValueError(f"freedofs must have {n} entries")
         self._free = torch.from_numpy(free.astype(np.float64)).to(dev)
         self._blocks = None
+        self._diagonal = None  # float64 vector when the inverse is diagonal
+        self._diag_cache = {}
         sp = M.trial
         if isinstance(sp, SurfaceL2):
             # element-local dofs: M is block-diagonal with one nloc x nloc block
@@ -381,12 +383,23 @@ class _MassInverse(BaseMatrix):
             f = self._free[ed]                                   # ne x nl
             B = B * f[:, :, None] * f[:, None, :] + torch.diag_embed(1.0 - f)
             self._blocks = (ed, torch.linalg.inv(B))
+            if nl == 1:
+                # order 0: one dof per element, the inverse is the diagonal 1 / |T| on the free dofs
+                d = torch.zeros(n, dtype=torch.float64, device=dev)
+                d[ed[:, 0]] = self._blocks[1][:, 0, 0]
+                self._diagonal = (d * self._free).contiguous()
         else:
             d = torch.zeros(n, dtype=torch.float64, device=dev)
             d.index_add_(0, M.rows[M.rows == M.cols], M.vals[M.rows == M.cols])
             self._dinv = torch.where(self._free > 0, 1.0 / d, torch.zeros_like(d))
 
     def _solve(self, x):
+        if self._diagonal is not None:
+            rdt = x.real.dtype if torch.is_complex(x) else x.dtype
+            d = self._diag_cache.get(rdt)
+            if d is None:
+                d = self._diag_cache[rdt] = self._diagonal.to(rdt)
+            return x * d
         f = self._free.to(x.dtype)
         b = x * f
         if self._blocks is not None:

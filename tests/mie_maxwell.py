@@ -44,9 +44,10 @@ def _radial(kind, n, rho):
     return z, (z + rho * dz) / rho
 
 
-def _mn(kind, n, rho, cos_p, sin_p, sin_t, pi_n, tau_n):
-    """Spherical components (r, theta, phi) of M_o1n, M_e1n, N_o1n, N_e1n."""
-    z, dz = _radial(kind, n, rho)
+def _mn(kind, n, rho, cos_p, sin_p, sin_t, pi_n, tau_n, rho_rad=None):
+    """Spherical components (r, theta, phi) of M_o1n, M_e1n, N_o1n, N_e1n;
+    the radial functions at rho_rad (a scalar when every point has one radius)."""
+    z, dz = _radial(kind, n, rho if rho_rad is None else rho_rad)
     zeros = np.zeros_like(rho, dtype=complex)
     M_o = np.stack([zeros, cos_p * pi_n * z, -sin_p * tau_n * z])
     M_e = np.stack([zeros, -sin_p * pi_n * z, -cos_p * tau_n * z])
@@ -76,13 +77,15 @@ def fields(X, k, a=1.0, nmax=None, parts=("inc", "sca")):
     r, ct, st, cp, sp, er, et, ep = _frame(X)
     pi, tau = _pi_tau(ct, nmax)
     rho = k * r
+    # points on one sphere: the radial functions once per order, not per point
+    rho_rad = rho.flat[0] if rho.size and np.ptp(rho) <= 1e-13 * abs(rho.flat[0]) else None
     E = np.zeros((3,) + r.shape, dtype=complex)
     C = np.zeros_like(E)
     x = k * a
     for n in range(1, nmax + 1):
         En = (1j ** n) * (2 * n + 1) / (n * (n + 1))
         if "inc" in parts:
-            M_o, M_e, N_o, N_e = _mn(1, n, rho, cp, sp, st, pi[n], tau[n])
+            M_o, M_e, N_o, N_e = _mn(1, n, rho, cp, sp, st, pi[n], tau[n], rho_rad)
             E += En * (M_o - 1j * N_e)
             C += En * k * (N_o - 1j * M_e)
         if "sca" in parts:
@@ -91,7 +94,7 @@ def fields(X, k, a=1.0, nmax=None, parts=("inc", "sca")):
             dh = spherical_jn(n, x, derivative=True) + 1j * spherical_yn(n, x, derivative=True)
             xi, dxi = x * h, h + x * dh
             an, bn = dpsi / dxi, psi / xi
-            M_o, M_e, N_o, N_e = _mn(3, n, rho, cp, sp, st, pi[n], tau[n])
+            M_o, M_e, N_o, N_e = _mn(3, n, rho, cp, sp, st, pi[n], tau[n], rho_rad)
             E += En * (1j * an * N_e - bn * M_o)
             C += En * k * (1j * an * M_e - bn * N_o)
     to_cart = lambda F: (F[0][..., None] * er + F[1][..., None] * et + F[2][..., None] * ep)  # noqa: E731

@@ -67,3 +67,27 @@ def test_gmres_batch_with_mass_preconditioner_and_zero_rhs(cpu_device):
     ref = torch.linalg.solve(A, b)
     assert (torch.linalg.norm(X[0] - ref) / torch.linalg.norm(ref)).item() < 1e-8
     assert np.all(np.isfinite(K.GMResBatch.residuals))
+
+
+@pytest.mark.parametrize("free", [None, "some"])
+def test_order0_inverse_is_the_diagonal(cpu_device, free):
+    """SurfaceL2 order 0: the inverse is the diagonal 1 / |T| (zero on fixed
+    dofs), exposed as ``_diagonal`` for GMRES's fused preconditioner and
+    applied as one product; it equals the dense inverse on the free dofs."""
+    sp = nb.SurfaceL2(nb.icosphere(2), order=0)
+    fd = None
+    mask = np.ones(sp.ndof, dtype=bool)
+    if free == "some":
+        mask[::7] = False
+        fd = mask
+    Minv = nb.MassMatrix(sp).Inverse(freedofs=fd)
+    assert Minv._diagonal is not None and Minv._diagonal.dtype == torch.float64
+    area = sp.mesh.areas()
+    np.testing.assert_array_equal(Minv._diagonal.numpy(), np.where(mask, 1.0 / area, 0.0))
+    x = np.random.default_rng(3).standard_normal(sp.ndof) + 1j * np.random.default_rng(4).standard_normal(sp.ndof)
+    z = (Minv * torch.from_numpy(x)).numpy()
+    D = _dense_mass(sp)
+    ref = np.zeros_like(x)
+    ref[mask] = np.linalg.solve(D[np.ix_(mask, mask)], x[mask])
+    assert np.abs(z - ref).max() <= 1e-13 * np.abs(ref).max()
+    assert nb.MassMatrix(nb.SurfaceL2(nb.icosphere(1), order=1)).Inverse()._diagonal is None
