@@ -119,6 +119,9 @@ typedef struct {
    * launches, and the cross approximation launches (summed over workspace
    * batches; they overlap the near field, which runs on its own stream) */
   double t_kernel_near_regular, t_kernel_near_singular, t_kernel_aca;
+  /* the near field's gather of element matrices into the blocks (spaces
+   * with several functions per element; 0 when the pairs store directly) */
+  double t_kernel_near_gather;
 } bem_stats;
 
 typedef struct {

@@ -32,7 +32,8 @@ class Stats(ctypes.Structure):
                 ("t_setup", ctypes.c_double), ("t_near", ctypes.c_double), ("t_far", ctypes.c_double),
                 ("t_total", ctypes.c_double), ("near_kernel_evals", ctypes.c_double),
                 ("far_kernel_evals", ctypes.c_double), ("t_kernel_near_regular", ctypes.c_double),
-                ("t_kernel_near_singular", ctypes.c_double), ("t_kernel_aca", ctypes.c_double)]
+                ("t_kernel_near_singular", ctypes.c_double), ("t_kernel_aca", ctypes.c_double),
+                ("t_kernel_near_gather", ctypes.c_double)]
 
     def as_dict(self):
         d = {}

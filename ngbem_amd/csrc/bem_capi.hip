@@ -571,6 +571,7 @@ int bem_op_stats(const bem_op* h, bem_stats* st) {
   st->far_kernel_evals = op.far_evals;
   st->t_kernel_near_regular = op.t_ev_near_regular;
   st->t_kernel_near_singular = op.t_ev_near_singular;
+  st->t_kernel_near_gather = op.t_ev_near_gather;
   st->t_kernel_aca = op.t_ev_aca;
   return 0;
 }

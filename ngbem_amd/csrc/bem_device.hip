@@ -20,6 +20,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <array>
 #include <atomic>
 #include <chrono>
 #include <cstdio>
@@ -789,6 +790,29 @@ constexpr int GEO = 16;  // p0 p1 p2 n J kappa pad
 __device__ __forceinline__ P3 ld3(const double* p) { return {p[0], p[1], p[2]}; }
 
 // ------------------------------------------------------ regular near field --
+// Blocks whose sides have several functions per element (H1, edge spaces,
+// SurfaceL2 of order >= 1) do not add into the blocks: each element pair of a
+// block's patch product stores its element matrix in its slot of an
+// element-matrix buffer em (slot stride emI x emJ, entry (ii, jj) at
+// ii emJ + jj, element-local function order), and k_near_gather sums, per
+// block entry and in a fixed order, the slots of the pairs whose functions
+// carry the entry's two dofs.  The result does not depend on the order in
+// which the pairs finish.
+
+// patch-local index of element e on one side of a near block: its position in
+// the sorted patch list (several functions per element), or the position of
+// its dof in the cluster (one dof per element: the patch is the cluster's slice)
+__device__ __forceinline__ int patch_pos(bool multi, const int* __restrict__ patch, int p0, int len, int e,
+                                         const int* __restrict__ dofs, const int* __restrict__ ip, int r0) {
+  if (!multi) return ip[dofs[DOF_STRIDE * e]] - r0;
+  int lo = 0, hi = len;
+  while (lo < hi) {
+    const int mid = (lo + hi) >> 1;
+    if (patch[p0 + mid] < e) lo = mid + 1;
+    else hi = mid;
+  }
+  return lo;
+}
 
 template <class KER, class EI, class EJ, bool DIRECT>
 __global__ __launch_bounds__(256) void k_near_regular(const NearBlk* __restrict__ blks, const int2* __restrict__ chunks,
@@ -799,7 +823,9 @@ __global__ __launch_bounds__(256) void k_near_regular(const NearBlk* __restrict_
                                                       const int* __restrict__ dofs_t, const int* __restrict__ dofs_r,
                                                       const int* __restrict__ ipt, const int* __restrict__ ipr, KER ker,
                                                       const double* __restrict__ qg,
-                                                      typename KER::T* __restrict__ val) {
+                                                      typename KER::T* __restrict__ val,
+                                                      typename KER::T* __restrict__ em, int64_t em_base, int emI,
+                                                      int emJ) {
   typedef typename KER::T T;
   const int2 ch = chunks[blockIdx.x];
   const NearBlk nb = blks[ch.x];
@@ -856,6 +882,12 @@ __global__ __launch_bounds__(256) void k_near_regular(const NearBlk* __restrict_
   }
   if (DIRECT) {
     val[nb.off + (int64_t)ja * nb.m + ia] = el[0][0];
+  } else if (em) {
+    T* o = em + ((nb.emoff - em_base) + (int64_t)ja * nb.pm + ia) * (int64_t)(emI * emJ);
+#pragma unroll
+    for (int ii = 0; ii < EI::NL; ii++)
+#pragma unroll
+      for (int jj = 0; jj < EJ::NL; jj++) o[ii * emJ + jj] = el[ii][jj];
   } else {
 #pragma unroll
     for (int ii = 0; ii < EI::NL; ii++) {
@@ -888,7 +920,10 @@ __global__ __launch_bounds__(256) void k_near_singular(const int4* __restrict__ 
                                                        const int* __restrict__ dofs_r, const int* __restrict__ ipt,
                                                        const int* __restrict__ ipr, KER ker,
                                                        const double* __restrict__ gp, int gk,
-                                                       typename KER::T* __restrict__ val) {
+                                                       typename KER::T* __restrict__ val,
+                                                       const int* __restrict__ pIb, const int* __restrict__ pJb,
+                                                       typename KER::T* __restrict__ em, int64_t em_base, int emI,
+                                                       int emJ, int multi_i, int multi_j) {
   typedef typename KER::T T;
   const int p = blockIdx.x * 256 + threadIdx.x;
   if (p >= npairs) return;
@@ -962,6 +997,19 @@ __global__ __launch_bounds__(256) void k_near_singular(const int4* __restrict__ 
   if (DIRECT) {
     const int row = ipt[dofs_t[DOF_STRIDE * e]] - nb.r0, col = ipr[dofs_r[DOF_STRIDE * f]] - nb.c0;
     val[nb.off + (int64_t)col * nb.m + row] = el[0][0];
+  } else if (em) {  // the pair's slot in the block's patch product, element-local order
+    const int ia = patch_pos(multi_i, pIb, nb.pI, nb.pm, e, dofs_t, ipt, nb.r0);
+    const int ja = patch_pos(multi_j, pJb, nb.pJ, nb.pn, f, dofs_r, ipr, nb.c0);
+    T* o = em + ((nb.emoff - em_base) + (int64_t)ja * nb.pm + ia) * (int64_t)(emI * emJ);
+#pragma unroll
+    for (int kx = 0; kx < EI::NL; kx++) {
+      const int dx = EI::PERM ? px[kx] : kx;
+#pragma unroll
+      for (int ky = 0; ky < EJ::NL; ky++) {
+        const int dy = EJ::PERM ? py[ky] : ky;
+        o[dx * emJ + dy] = el[kx][ky];
+      }
+    }
   } else {
 #pragma unroll
     for (int kx = 0; kx < EI::NL; kx++) {
@@ -1007,14 +1055,16 @@ __global__ __launch_bounds__(64) void k_near_wide(const NearBlk* __restrict__ bl
                                                   const int* __restrict__ dofs_t, const int* __restrict__ dofs_r,
                                                   const int* __restrict__ ipt, const int* __restrict__ ipr, KER ker,
                                                   const double* __restrict__ qg, const double* __restrict__ gp,
-                                                  int gk, typename KER::T* __restrict__ val) {
+                                                  int gk, typename KER::T* __restrict__ val,
+                                                  typename KER::T* __restrict__ em, int64_t em_base, int emI,
+                                                  int emJ, int multi_i, int multi_j) {
   typedef typename KER::T T;
   constexpr int NT = KER::NTERM, DI = EI::DIM, DJ = EJ::DIM;
   __shared__ T s_kw[WQ][NT];
   __shared__ double s_lx[WQ][3], s_ly[WQ][3], s_gx[WQ][10], s_gy[WQ][10];
   __shared__ double s_si[WQ][NLOC_MAX * DI], s_sj[WQ][NLOC_MAX * DJ];
   const int lane = threadIdx.x;
-  int bi, e, f, px[3] = {0, 1, 2}, py[3] = {0, 1, 2};
+  int bi, e, f, px[3] = {0, 1, 2}, py[3] = {0, 1, 2}, ia = 0, ja = 0;
   if constexpr (SING) {
     const int4 pr = pairs[blockIdx.x];
     bi = pr.x;
@@ -1031,7 +1081,8 @@ __global__ __launch_bounds__(64) void k_near_wide(const NearBlk* __restrict__ bl
     const NearBlk b0 = blks[bi];
     const int64_t p = (int64_t)ch.y * 256 + (blockIdx.x & 255);
     if (p >= (int64_t)b0.pm * b0.pn) return;  // the whole workgroup
-    const int ja = (int)(p / b0.pm), ia = (int)(p - (int64_t)ja * b0.pm);
+    ja = (int)(p / b0.pm);
+    ia = (int)(p - (int64_t)ja * b0.pm);
     e = pIb[b0.pI + ia];
     f = pJb[b0.pJ + ja];
     const int4 ve = vid[e], vf = vid[f];
@@ -1168,6 +1219,21 @@ __global__ __launch_bounds__(64) void k_near_wide(const NearBlk* __restrict__ bl
       el[r] = acc;
     }
   }
+  if (em) {  // the pair's slot in the block's patch product
+    if constexpr (SING) {
+      ia = patch_pos(multi_i, pIb, nb.pI, nb.pm, e, dofs_t, ipt, nb.r0);
+      ja = patch_pos(multi_j, pJb, nb.pJ, nb.pn, f, dofs_r, ipr, nb.c0);
+    }
+    T* o = em + ((nb.emoff - em_base) + (int64_t)ja * nb.pm + ia) * (int64_t)(emI * emJ);
+#pragma unroll
+    for (int r = 0; r < WENT; r++) {
+      const int E = lane + 64 * r;
+      if (E >= nent) break;
+      const int ii = E % ni, jj = E / ni;
+      if (ii < emI && jj < emJ) o[ii * emJ + jj] = el[r];
+    }
+    return;
+  }
 #pragma unroll
   for (int r = 0; r < WENT; r++) {
     const int E = lane + 64 * r;
@@ -1183,6 +1249,41 @@ __global__ __launch_bounds__(64) void k_near_wide(const NearBlk* __restrict__ bl
     if (col < 0 || col >= nb.n) continue;
     atomic_add(&val[nb.off + (int64_t)col * nb.m + row], el[r]);
   }
+}
+
+// entry (i, j) of a near block = sum over the row dof's (patch element a,
+// function kl) and the column dof's (element c, function kc) of the element
+// matrix entry (kl, kc) of pair (a, c), in map order; one thread per entry,
+// consecutive threads down a column.  A side with one dof per element has the
+// single term (a, kl) = (i, 0).
+template <class T>
+__global__ __launch_bounds__(256) void k_near_gather(const NearBlk* __restrict__ blks, const int2* __restrict__ chunks,
+                                                     const int* __restrict__ maps, int multi_i, int multi_j,
+                                                     const T* __restrict__ em, int64_t em_base, int emI, int emJ,
+                                                     T* __restrict__ val) {
+  const int2 ch = chunks[blockIdx.x];
+  const NearBlk nb = blks[ch.x];
+  const int64_t p = (int64_t)ch.y * 256 + threadIdx.x;
+  if (p >= (int64_t)nb.m * nb.n) return;
+  const int j = (int)(p / nb.m), i = (int)(p - (int64_t)j * nb.m);
+  const int* mr = maps + nb.rmap;
+  const int* mc = maps + nb.cmap;
+  int a0 = i, a1 = i + 1, c0 = j, c1 = j + 1;
+  if (multi_i) { a0 = mr[i]; a1 = mr[i + 1]; }
+  if (multi_j) { c0 = mc[j]; c1 = mc[j + 1]; }
+  const int64_t st = (int64_t)emI * emJ;
+  const T* E = em + (nb.emoff - em_base) * st;
+  T s = zero_<T>();
+  for (int ka = a0; ka < a1; ka++) {
+    const int ea = multi_i ? mr[nb.m + 1 + ka] : ka * DOF_STRIDE;
+    const int a = ea / DOF_STRIDE, kl = ea - (ea / DOF_STRIDE) * DOF_STRIDE;
+    for (int kc = c0; kc < c1; kc++) {
+      const int ec = multi_j ? mc[nb.n + 1 + kc] : kc * DOF_STRIDE;
+      const int c = ec / DOF_STRIDE, kk = ec - (ec / DOF_STRIDE) * DOF_STRIDE;
+      s += E[((int64_t)c * nb.pm + a) * st + kl * emJ + kk];
+    }
+  }
+  val[nb.off + (int64_t)j * nb.m + i] = s;
 }
 
 // ------------------------------------------------------------------- ACA --
@@ -3055,6 +3156,26 @@ void setup_device(Operator& op, hipStream_t s) {
 
 // ------------------------------------------------------------- near field --
 
+// the element-matrix form of the near field for sides with several functions
+// per element (deterministic sums); NGBEM_NEAR=atomic keeps the per-pair
+// atomic adds into the blocks
+static bool near_em_on() {
+  static const bool on = [] {
+    const char* e = std::getenv("NGBEM_NEAR");
+    return !(e && std::string(e) == "atomic");
+  }();
+  return on;
+}
+
+// device bytes for the element matrices of one batch of near blocks (at most
+// a sixth of the free memory; NGBEM_NEAR_EM_GB, read at each assembly,
+// overrides the 4 GiB default)
+static size_t near_em_budget() {
+  const char* e = std::getenv("NGBEM_NEAR_EM_GB");
+  const double gb = e ? std::atof(e) : 4.0;
+  return (size_t)(std::max(gb, 1e-4) * (double)(1ull << 30));
+}
+
 static int perm_code(const int px[3], const int py[3]) {
   return px[0] | (px[1] << 2) | (px[2] << 4) | (py[0] << 6) | (py[1] << 8) | (py[2] << 10);
 }
@@ -3071,8 +3192,9 @@ void assemble_near(Operator& op, hipStream_t s) {
   PhaseTimer pt_("near");
   // near tables; P1 sides use the per-cluster patches, P0 sides the permutation
   std::vector<int2> chunks;
+  std::vector<size_t> chunk0(op.near.size() + 1, 0);  // first chunk of each block
   std::vector<int4> pairs[3];
-  int64_t off = 0;
+  int64_t off = 0, emslots = 0;
   const ClusterPatches& cpi = *op.cp_test;
   const ClusterPatches& cpj = op.cp_trial;
   const int ncl_test = (int)tt.cl.size();
@@ -3081,6 +3203,7 @@ void assemble_near(Operator& op, hipStream_t s) {
     const Block& bl = op.blocks[op.near_block[k]];
     nb.off = off;
     off += (int64_t)nb.m * nb.n;
+    chunk0[k] = chunks.size();
     if (p1i) {
       nb.pI = cpi.off[bl.row];
       nb.pm = cpi.size(bl.row);
@@ -3100,47 +3223,68 @@ void assemble_near(Operator& op, hipStream_t s) {
     const int64_t nch = ((int64_t)nb.pm * nb.pn + 255) / 256;
     if (nch >= (int64_t)INT32_MAX) throw std::runtime_error("near block too large");
     for (int64_t c = 0; c < nch; c++) chunks.push_back(make_int2((int)k, (int)c));
+    nb.emoff = emslots;
+    emslots += (int64_t)nb.pm * nb.pn;
+    nb.rmap = nb.cmap = 0;
   }
+  chunk0[op.near.size()] = chunks.size();
   // touching element pairs: for each test element, its vertex neighbours; a
   // pair belongs to every near block holding one of its dof combinations
-  // (the reference recomputes it in each such block, src/ngbem.cpp:501-737)
+  // (the reference recomputes it in each such block, src/ngbem.cpp:501-737).
+  // Host threads take contiguous ranges of test elements; their lists are
+  // joined in range order, so the pair lists are the same for any thread count.
   {
-    std::vector<char> mark(m.nt, 0);
-    std::vector<int> neigh, got;
-    for (int e = 0; e < m.nt; e++) {
-      if (!te.has(e)) continue;
-      neigh.clear();
-      for (int c = 0; c < 3; c++) {
-        const int v = m.tri[3 * e + c];
-        for (int q = m.ve_off[v]; q < m.ve_off[v + 1]; q++) {
-          const int f = m.ve_list[q];
-          if (!mark[f]) { mark[f] = 1; neigh.push_back(f); }
-        }
-      }
-      for (int f : neigh) {
-        mark[f] = 0;
-        if (!tr.has(f)) continue;
-        int px[3], py[3];
-        const int nc = classify_pair(m, e, f, px, py);
-        int cls;
-        if (nc == 3) {  // identical: reference point (x0,x1) is (xi,eta): SS vertex 0 -> local 2
-          px[0] = 2; px[1] = 0; px[2] = 1;
-          py[0] = 2; py[1] = 0; py[2] = 1;
-          cls = 0;
-        } else if (nc == 2) cls = 1;
-        else if (nc == 1) cls = 2;
-        else continue;
-        got.clear();
-        for (int i = 0; i < te.nloc; i++)
-          for (int j = 0; j < tr.nloc; j++) {
-            const int rp = tt.iperm[te.dofs[e * te.nloc + i]], cp = rt.iperm[tr.dofs[f * tr.nloc + j]];
-            const int b = op.lookup.find(rp, cp);
-            if (b < 0 || !(op.blocks[b].near || op.blocks[b].fallback)) continue;
-            const int slot = op.blk_slot[b];
-            if (std::find(got.begin(), got.end(), slot) == got.end()) got.push_back(slot);
+    const int nth = (int)std::max(1u, std::min(16u, std::thread::hardware_concurrency()));
+    const int nw = std::max(1, std::min(nth, m.nt / 512));
+    std::vector<std::array<std::vector<int4>, 3>> part(nw);
+    std::vector<std::thread> th;
+    for (int w = 0; w < nw; w++)
+      th.emplace_back([&, w] {
+        std::vector<char> mark(m.nt, 0);
+        std::vector<int> neigh, got;
+        const int e0 = (int)((int64_t)m.nt * w / nw), e1 = (int)((int64_t)m.nt * (w + 1) / nw);
+        for (int e = e0; e < e1; e++) {
+          if (!te.has(e)) continue;
+          neigh.clear();
+          for (int c = 0; c < 3; c++) {
+            const int v = m.tri[3 * e + c];
+            for (int q = m.ve_off[v]; q < m.ve_off[v + 1]; q++) {
+              const int f = m.ve_list[q];
+              if (!mark[f]) { mark[f] = 1; neigh.push_back(f); }
+            }
           }
-        for (int slot : got) pairs[cls].push_back(make_int4(slot, e, f, perm_code(px, py)));
-      }
+          for (int f : neigh) {
+            mark[f] = 0;
+            if (!tr.has(f)) continue;
+            int px[3], py[3];
+            const int nc = classify_pair(m, e, f, px, py);
+            int cls;
+            if (nc == 3) {  // identical: reference point (x0,x1) is (xi,eta): SS vertex 0 -> local 2
+              px[0] = 2; px[1] = 0; px[2] = 1;
+              py[0] = 2; py[1] = 0; py[2] = 1;
+              cls = 0;
+            } else if (nc == 2) cls = 1;
+            else if (nc == 1) cls = 2;
+            else continue;
+            got.clear();
+            for (int i = 0; i < te.nloc; i++)
+              for (int j = 0; j < tr.nloc; j++) {
+                const int rp = tt.iperm[te.dofs[e * te.nloc + i]], cp = rt.iperm[tr.dofs[f * tr.nloc + j]];
+                const int b = op.lookup.find(rp, cp);
+                if (b < 0 || !(op.blocks[b].near || op.blocks[b].fallback)) continue;
+                const int slot = op.blk_slot[b];
+                if (std::find(got.begin(), got.end(), slot) == got.end()) got.push_back(slot);
+              }
+            for (int slot : got) part[w][cls].push_back(make_int4(slot, e, f, perm_code(px, py)));
+          }
+        }
+      });
+    for (auto& x : th) x.join();
+    for (int c = 0; c < 3; c++) {
+      size_t tot = 0;
+      for (auto& p : part) tot += p[c].size();
+      pairs[c].reserve(tot);
+      for (auto& p : part) pairs[c].insert(pairs[c].end(), p[c].begin(), p[c].end());
     }
   }
   pt_.mark("host work lists");
@@ -3151,14 +3295,49 @@ void assemble_near(Operator& op, hipStream_t s) {
   for (const NearBlk& nb : op.near) op.near_evals += (double)nb.pm * nb.pn * ntq * ntq;
   for (int c = 0; c < 3; c++) op.near_evals += (double)pairs[c].size() * (op.ss[c].size() - (double)ntq * ntq);
 
+  // element-matrix mode (sides with several functions per element): the
+  // dof -> (patch element, function) maps of the blocks' row and column
+  // clusters (copies: the far field builds its own cached maps concurrently)
+  // and the gather's chunks of 256 block entries
+  const bool em_mode = !direct && near_em_on();
+  std::vector<int> nmaps;
+  std::vector<int2> gchunks;
+  std::vector<size_t> gchunk0(op.near.size() + 1, 0);
+  if (em_mode) {
+    std::vector<int> moff_i(tt.cl.size(), -1), moff_j(rt.cl.size(), -1), tmp;
+    for (size_t k = 0; k < op.near.size(); k++) {
+      NearBlk& nb = op.near[k];
+      const Block& bl = op.blocks[op.near_block[k]];
+      if (p1i) {
+        if (moff_i[bl.row] < 0) {
+          cpi.map_copy(te, tt, bl.row, tmp);
+          moff_i[bl.row] = (int)nmaps.size();
+          nmaps.insert(nmaps.end(), tmp.begin(), tmp.end());
+        }
+        nb.rmap = moff_i[bl.row];
+      }
+      if (p1j) {
+        if (moff_j[bl.col] < 0) {
+          cpj.map_copy(tr, rt, bl.col, tmp);
+          moff_j[bl.col] = (int)nmaps.size();
+          nmaps.insert(nmaps.end(), tmp.begin(), tmp.end());
+        }
+        nb.cmap = moff_j[bl.col];
+      }
+      gchunk0[k] = gchunks.size();
+      const int64_t ng = ((int64_t)nb.m * nb.n + 255) / 256;
+      for (int64_t c = 0; c < ng; c++) gchunks.push_back(make_int2((int)k, (int)c));
+    }
+    gchunk0[op.near.size()] = gchunks.size();
+  }
+  pt_.mark("element-pair maps");
+
   op.d_near_blk.upload(op.near, s);
   pt_.mark("stats");
   op.d_near_chunks.upload(chunks, s);
 
-  for (int c = 0; c < 3; c++) op.d_spairs[c].upload(pairs[c], s);
   const size_t esz = op.complex_valued ? 16 : 8;
   op.d_near_val.alloc((size_t)off * esz);
-  if (!direct && off) BEM_HIP(hipMemsetAsync(op.d_near_val.p, 0, (size_t)off * esz, s));
   // P0 sides: cluster position -> element; on part of the mesh the dof is not
   // the element number
   DBuf d_pe_test, d_pe_trial;
@@ -3173,78 +3352,183 @@ void assemble_near(Operator& op, hipStream_t s) {
                        : (te.region.empty() ? op.d_perm_test.as<int>() : elem_of_pos(te, tt, d_pe_test));
   const int* pJb = p1j ? op.d_patch.as<int>()
                        : (tr.region.empty() ? op.d_perm_trial.as<int>() : elem_of_pos(tr, rt, d_pe_trial));
-  hipEvent_t tev[3];
-  for (auto& e : tev) BEM_HIP(hipEventCreate(&e));
+  std::vector<hipEvent_t> tev;
+  auto mark_ev = [&]() {
+    hipEvent_t e;
+    BEM_HIP(hipEventCreate(&e));
+    BEM_HIP(hipEventRecord(e, s));
+    tev.push_back(e);
+  };
+  // per launch group: (first event, last event) pairs of the regular, singular
+  // and gather launches
+  std::vector<std::array<int, 2>> tr_reg, tr_sing, tr_gat;
+  DBuf d_em, d_maps, d_gchunks;
+  std::vector<int4> sorted_pairs[3];  // host sources of asynchronous uploads: alive until the stream is synchronised
+  if (nmaps.empty()) nmaps.push_back(0);
   dispatch_full(op, [&](auto ker, auto ei, auto ej) {
     using KER = decltype(ker);
     using EI = decltype(ei);
     using EJ = decltype(ej);
     using T = typename KER::T;
     T* val = op.d_near_val.as<T>();
-    auto launch = [&](auto dflag) {
-      constexpr bool D = decltype(dflag)::value;
-      BEM_HIP(hipEventRecord(tev[0], s));
-      // launches of at most 2^22 chunks (2^30 threads)
-      for (size_t c0 = 0; c0 < chunks.size(); c0 += (size_t)1 << 22) {
-        const size_t nc = std::min(chunks.size() - c0, (size_t)1 << 22);
-        hipLaunchKernelGGL((k_near_regular<KER, EI, EJ, D>), dim3((unsigned)nc), dim3(256), 0, s,
-                           op.d_near_blk.as<NearBlk>(), op.d_near_chunks.as<int2>() + c0, pIb, pJb,
-                           op.d_geo.as<double>(), op.d_vid.as<int4>(), op.d_quad.as<double>(), op.d_sdat.as<double>(),
-                           op.d_trig.as<double>(), ntq, op.d_dofs_test.as<int>(), op.d_dofs_trial.as<int>(),
-                           op.d_iperm_test.as<int>(), op.d_iperm_trial.as<int>(), ker, op.d_qgeo.as<double>(), val);
-      }
-      BEM_HIP(hipEventRecord(tev[1], s));
-      for (int c = 0; c < 3; c++) {
-        const int np = (int)pairs[c].size();
-        if (!np) continue;
-        hipLaunchKernelGGL((k_near_singular<KER, EI, EJ, D>), dim3((np + 255) / 256), dim3(256), 0, s,
-                           op.d_spairs[c].as<int4>(), np, op.d_near_blk.as<NearBlk>(), op.d_ss[c].as<double>(),
-                           op.ss[c].size(), op.d_geo.as<double>(), op.d_sdat.as<double>(),
-                           op.d_dofs_test.as<int>(), op.d_dofs_trial.as<int>(), op.d_iperm_test.as<int>(),
-                           op.d_iperm_trial.as<int>(), ker, op.d_gpoly.as<double>(), op.geo_order, val);
-      }
+    // slot stride of the element matrices
+    const int emI = EI::WIDE ? te.nloc : std::max(EI::NL, te.nloc);
+    const int emJ = EJ::WIDE ? tr.nloc : std::max(EJ::NL, tr.nloc);
+    if (EI::WIDE && (te.nloc > NLOC_MAX || tr.nloc > NLOC_MAX))
+      throw std::runtime_error("near field: more local functions than NLOC_MAX");
+    const int64_t slot_bytes = (int64_t)emI * emJ * (int64_t)sizeof(T);
+    // batches of blocks whose element matrices fit the budget
+    size_t free_b = 0, total_b = 0;
+    BEM_HIP(hipMemGetInfo(&free_b, &total_b));
+    const int64_t budget = (int64_t)std::min<size_t>(near_em_budget(), free_b / 6);
+    bool use_em = em_mode;
+    for (const NearBlk& nb : op.near)
+      if ((int64_t)nb.pm * nb.pn * slot_bytes > budget) use_em = false;  // one block alone too large
+    struct Batch {
+      size_t k0, k1;
+      int64_t em0, slots;
     };
-    pt_.mark("uploads + alloc");
-    if constexpr (EI::WIDE) {
-      BEM_HIP(hipEventRecord(tev[0], s));
-      // launches of at most 2^16 chunks (2^24 workgroups of one wave)
-      for (size_t c0 = 0; c0 < chunks.size(); c0 += (size_t)1 << 16)
-        hipLaunchKernelGGL((k_near_wide<KER, EI, EJ, false>),
-                           dim3((unsigned)(std::min(chunks.size() - c0, (size_t)1 << 16) * 256)), dim3(64), 0, s,
-                           op.d_near_blk.as<NearBlk>(), op.d_near_chunks.as<int2>() + c0, pIb, pJb,
-                           (const int4*)nullptr,
-                           (const double*)nullptr, 0, op.d_geo.as<double>(), op.d_vid.as<int4>(),
-                           op.d_quad.as<double>(), op.d_sdat.as<double>(), op.d_trig.as<double>(), ntq,
-                           op.d_dofs_test.as<int>(), op.d_dofs_trial.as<int>(), op.d_iperm_test.as<int>(),
-                           op.d_iperm_trial.as<int>(), ker, op.d_qgeo.as<double>(), op.d_gpoly.as<double>(),
-                           op.geo_order, val);
-      BEM_HIP(hipEventRecord(tev[1], s));
-      for (int c = 0; c < 3; c++) {
-        const int np = (int)pairs[c].size();
-        if (!np) continue;
-        hipLaunchKernelGGL((k_near_wide<KER, EI, EJ, true>), dim3((unsigned)np), dim3(64), 0, s,
-                           op.d_near_blk.as<NearBlk>(), (const int2*)nullptr, pIb, pJb, op.d_spairs[c].as<int4>(),
-                           op.d_ss[c].as<double>(), op.ss[c].size(), op.d_geo.as<double>(), op.d_vid.as<int4>(),
-                           op.d_quad.as<double>(), op.d_sdat.as<double>(), op.d_trig.as<double>(), ntq,
-                           op.d_dofs_test.as<int>(), op.d_dofs_trial.as<int>(), op.d_iperm_test.as<int>(),
-                           op.d_iperm_trial.as<int>(), ker, op.d_qgeo.as<double>(), op.d_gpoly.as<double>(),
-                           op.geo_order, val);
+    std::vector<Batch> batches;
+    if (use_em) {
+      size_t k0 = 0;
+      int64_t cur = 0;
+      for (size_t k = 0; k < op.near.size(); k++) {
+        const int64_t sl = (int64_t)op.near[k].pm * op.near[k].pn;
+        if (cur > 0 && (cur + sl) * slot_bytes > budget) {
+          batches.push_back(Batch{k0, k, op.near[k0].emoff, cur});
+          k0 = k;
+          cur = 0;
+        }
+        cur += sl;
       }
-    } else if constexpr (!EI::MULTI && !EJ::MULTI) {
-      launch(std::true_type{});
+      if (k0 < op.near.size()) batches.push_back(Batch{k0, op.near.size(), op.near[k0].emoff, cur});
     } else {
-      launch(std::false_type{});
+      batches.push_back(Batch{0, op.near.size(), 0, 0});
     }
-    BEM_HIP(hipEventRecord(tev[2], s));
+    // touching pairs grouped by batch (stable: the order inside a batch is kept)
+    std::vector<size_t> sp0[3];
+    {
+      std::vector<int> bat(op.near.size(), 0);
+      for (size_t b = 0; b < batches.size(); b++)
+        for (size_t k = batches[b].k0; k < batches[b].k1; k++) bat[k] = (int)b;
+      for (int c = 0; c < 3; c++) {
+        sp0[c].assign(batches.size() + 1, 0);
+        for (const int4& pr : pairs[c]) sp0[c][bat[pr.x] + 1]++;
+        for (size_t b = 0; b < batches.size(); b++) sp0[c][b + 1] += sp0[c][b];
+        std::vector<int4>& sorted = sorted_pairs[c];
+        sorted.resize(pairs[c].size());
+        std::vector<size_t> pos(sp0[c].begin(), sp0[c].end() - 1);
+        for (const int4& pr : pairs[c]) sorted[pos[bat[pr.x]]++] = pr;
+        op.d_spairs[c].upload(sorted, s);
+      }
+    }
+    int64_t max_slots = 0;
+    for (const Batch& b : batches) max_slots = std::max(max_slots, b.slots);
+    T* em = nullptr;
+    if (use_em) {
+      d_em.alloc((size_t)std::max<int64_t>(max_slots, 1) * slot_bytes);
+      em = d_em.as<T>();
+      d_maps.upload(nmaps, s);
+      d_gchunks.upload(gchunks, s);
+    } else if (!direct && off) {
+      BEM_HIP(hipMemsetAsync(op.d_near_val.p, 0, (size_t)off * esz, s));
+    }
+    pt_.mark("uploads + alloc");
+    for (const Batch& bt : batches) {
+      const size_t c0 = chunk0[bt.k0], c1 = chunk0[bt.k1];
+      const int64_t em0 = bt.em0;
+      const int t0 = (int)tev.size();
+      mark_ev();
+      if constexpr (EI::WIDE) {
+        // launches of at most 2^16 chunks (2^24 workgroups of one wave)
+        for (size_t c = c0; c < c1; c += (size_t)1 << 16)
+          hipLaunchKernelGGL((k_near_wide<KER, EI, EJ, false>),
+                             dim3((unsigned)(std::min(c1 - c, (size_t)1 << 16) * 256)), dim3(64), 0, s,
+                             op.d_near_blk.as<NearBlk>(), op.d_near_chunks.as<int2>() + c, pIb, pJb,
+                             (const int4*)nullptr, (const double*)nullptr, 0, op.d_geo.as<double>(),
+                             op.d_vid.as<int4>(), op.d_quad.as<double>(), op.d_sdat.as<double>(),
+                             op.d_trig.as<double>(), ntq, op.d_dofs_test.as<int>(), op.d_dofs_trial.as<int>(),
+                             op.d_iperm_test.as<int>(), op.d_iperm_trial.as<int>(), ker, op.d_qgeo.as<double>(),
+                             op.d_gpoly.as<double>(), op.geo_order, val, em, em0, emI, emJ, (int)p1i, (int)p1j);
+      } else {
+        // launches of at most 2^22 chunks (2^30 threads)
+        for (size_t c = c0; c < c1; c += (size_t)1 << 22) {
+          const size_t nc = std::min(c1 - c, (size_t)1 << 22);
+          if (direct)
+            hipLaunchKernelGGL((k_near_regular<KER, EI, EJ, true>), dim3((unsigned)nc), dim3(256), 0, s,
+                               op.d_near_blk.as<NearBlk>(), op.d_near_chunks.as<int2>() + c, pIb, pJb,
+                               op.d_geo.as<double>(), op.d_vid.as<int4>(), op.d_quad.as<double>(),
+                               op.d_sdat.as<double>(), op.d_trig.as<double>(), ntq, op.d_dofs_test.as<int>(),
+                               op.d_dofs_trial.as<int>(), op.d_iperm_test.as<int>(), op.d_iperm_trial.as<int>(), ker,
+                               op.d_qgeo.as<double>(), val, (T*)nullptr, (int64_t)0, 1, 1);
+          else
+            hipLaunchKernelGGL((k_near_regular<KER, EI, EJ, false>), dim3((unsigned)nc), dim3(256), 0, s,
+                               op.d_near_blk.as<NearBlk>(), op.d_near_chunks.as<int2>() + c, pIb, pJb,
+                               op.d_geo.as<double>(), op.d_vid.as<int4>(), op.d_quad.as<double>(),
+                               op.d_sdat.as<double>(), op.d_trig.as<double>(), ntq, op.d_dofs_test.as<int>(),
+                               op.d_dofs_trial.as<int>(), op.d_iperm_test.as<int>(), op.d_iperm_trial.as<int>(), ker,
+                               op.d_qgeo.as<double>(), val, em, em0, emI, emJ);
+        }
+      }
+      mark_ev();
+      for (int c = 0; c < 3; c++) {
+        const size_t p0 = sp0[c][&bt - batches.data()], p1 = sp0[c][&bt - batches.data() + 1];
+        const int np = (int)(p1 - p0);
+        if (!np) continue;
+        const int4* prs = op.d_spairs[c].as<int4>() + p0;
+        if constexpr (EI::WIDE) {
+          hipLaunchKernelGGL((k_near_wide<KER, EI, EJ, true>), dim3((unsigned)np), dim3(64), 0, s,
+                             op.d_near_blk.as<NearBlk>(), (const int2*)nullptr, pIb, pJb, prs,
+                             op.d_ss[c].as<double>(), op.ss[c].size(), op.d_geo.as<double>(), op.d_vid.as<int4>(),
+                             op.d_quad.as<double>(), op.d_sdat.as<double>(), op.d_trig.as<double>(), ntq,
+                             op.d_dofs_test.as<int>(), op.d_dofs_trial.as<int>(), op.d_iperm_test.as<int>(),
+                             op.d_iperm_trial.as<int>(), ker, op.d_qgeo.as<double>(), op.d_gpoly.as<double>(),
+                             op.geo_order, val, em, em0, emI, emJ, (int)p1i, (int)p1j);
+        } else if (direct) {
+          hipLaunchKernelGGL((k_near_singular<KER, EI, EJ, true>), dim3((np + 255) / 256), dim3(256), 0, s, prs, np,
+                             op.d_near_blk.as<NearBlk>(), op.d_ss[c].as<double>(), op.ss[c].size(),
+                             op.d_geo.as<double>(), op.d_sdat.as<double>(), op.d_dofs_test.as<int>(),
+                             op.d_dofs_trial.as<int>(), op.d_iperm_test.as<int>(), op.d_iperm_trial.as<int>(), ker,
+                             op.d_gpoly.as<double>(), op.geo_order, val, pIb, pJb, (T*)nullptr, (int64_t)0, 1, 1, 0,
+                             0);
+        } else {
+          hipLaunchKernelGGL((k_near_singular<KER, EI, EJ, false>), dim3((np + 255) / 256), dim3(256), 0, s, prs,
+                             np, op.d_near_blk.as<NearBlk>(), op.d_ss[c].as<double>(), op.ss[c].size(),
+                             op.d_geo.as<double>(), op.d_sdat.as<double>(), op.d_dofs_test.as<int>(),
+                             op.d_dofs_trial.as<int>(), op.d_iperm_test.as<int>(), op.d_iperm_trial.as<int>(), ker,
+                             op.d_gpoly.as<double>(), op.geo_order, val, pIb, pJb, em, em0, emI, emJ, (int)p1i,
+                             (int)p1j);
+        }
+      }
+      mark_ev();
+      if (use_em) {
+        const size_t g0 = gchunk0[bt.k0], g1 = gchunk0[bt.k1];
+        for (size_t g = g0; g < g1; g += (size_t)1 << 22)
+          hipLaunchKernelGGL((k_near_gather<T>), dim3((unsigned)std::min(g1 - g, (size_t)1 << 22)), dim3(256), 0, s,
+                             op.d_near_blk.as<NearBlk>(), d_gchunks.as<int2>() + g, d_maps.as<int>(), (int)p1i,
+                             (int)p1j, (const T*)em, em0, emI, emJ, val);
+      }
+      mark_ev();
+      tr_reg.push_back({t0, t0 + 1});
+      tr_sing.push_back({t0 + 1, t0 + 2});
+      tr_gat.push_back({t0 + 2, t0 + 3});
+    }
   });
   BEM_HIP(hipGetLastError());
   BEM_HIP(hipStreamSynchronize(s));
   {
-    float a = 0.f, b = 0.f;
-    BEM_HIP(hipEventElapsedTime(&a, tev[0], tev[1]));
-    BEM_HIP(hipEventElapsedTime(&b, tev[1], tev[2]));
-    op.t_ev_near_regular = a * 1e-3;
-    op.t_ev_near_singular = b * 1e-3;
+    auto total = [&](const std::vector<std::array<int, 2>>& v) {
+      double t = 0.0;
+      for (const auto& p : v) {
+        float a = 0.f;
+        BEM_HIP(hipEventElapsedTime(&a, tev[p[0]], tev[p[1]]));
+        t += a * 1e-3;
+      }
+      return t;
+    };
+    op.t_ev_near_regular = total(tr_reg);
+    op.t_ev_near_singular = total(tr_sing);
+    op.t_ev_near_gather = total(tr_gat);
   }
   for (auto& e : tev) (void)hipEventDestroy(e);
   pt_.mark("kernels");

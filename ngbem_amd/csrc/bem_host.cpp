@@ -841,6 +841,10 @@ static void cluster_map(const ClusterPatches& cp, const Space& s, const Tree& t,
   out.insert(out.end(), ents.begin(), ents.end());
 }
 
+void ClusterPatches::map_copy(const Space& s, const Tree& t, int c, std::vector<int>& out) const {
+  cluster_map(*this, s, t, c, out);
+}
+
 int ClusterPatches::map_of(const Space& s, const Tree& t, int c) {
   if (moff[c] >= 0) return moff[c];
   std::vector<int> m;

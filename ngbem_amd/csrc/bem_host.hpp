@@ -175,6 +175,8 @@ struct ClusterPatches {
   std::vector<int> maps;
   void build(const Space& s, const Tree& t);
   int map_of(const Space& s, const Tree& t, int c);
+  // cluster c's map (the layout of map_of) into out, without caching it
+  void map_copy(const Space& s, const Tree& t, int c, std::vector<int>& out) const;
   // the maps of clusters cl[0], cl[1], ... not built yet, on host threads;
   // the same layout as map_of called in that order
   void map_many(const Space& s, const Tree& t, const std::vector<int>& cl);

@@ -152,6 +152,8 @@ struct NearBlk {
   int r0, m, c0, n;
   int pI, pm;      // test patch: offset, length (P0: cluster slice of perm)
   int pJ, pn;      // trial patch
+  int64_t emoff;   // first element-pair slot (pm x pn slots, pair (ia, ja) at emoff + ja pm + ia)
+  int rmap, cmap;  // dof -> (patch element, local function) maps of the row / column cluster
 };
 
 // far block record (device)
@@ -232,7 +234,7 @@ struct Operator {
   // device time from HIP events: the regular and the singular near-field
   // launches, and the ACA launches (both streams of each batch, summed over
   // batches; they overlap the near field, which runs on its own stream)
-  double t_ev_near_regular = 0, t_ev_near_singular = 0, t_ev_aca = 0;
+  double t_ev_near_regular = 0, t_ev_near_singular = 0, t_ev_near_gather = 0, t_ev_aca = 0;
   double near_evals = 0, far_evals = 0;
   bool assembled = false;
 
