@@ -73,8 +73,8 @@ def main():
     ap.add_argument("--no-block-product", action="store_true",
                     help="skip the Maxwell EFIE 64-right-hand-side block product after the headline workload")
     ap.add_argument("--calib-copy", action="store_true",
-                    help="end with one elementwise kernel reading and writing 512 MiB (calibrates rocprof "
-                         "FETCH_SIZE/WRITE_SIZE)")
+                    help="end with one elementwise kernel reading and writing 512 MiB and two 512 MiB streaming "
+                         "reads with the apply kernels' load shape (calibrate rocprof FETCH_SIZE/WRITE_SIZE)")
     ap.add_argument("--assembly-profile", type=str,
                     default=os.path.join(ROOT, "profiles", "assembly_helmholtz_cf_L6_fp64_counters.md"),
                     help="rocprof summary with the executed FP64 flops of the assembly kernels")
@@ -402,6 +402,15 @@ def main():
         b = torch.empty_like(a)
         torch.cuda.synchronize()
         torch.mul(a, 2.0, out=b)
+        torch.cuda.synchronize()
+        # and the library's streaming read with the apply kernels' load shape
+        # (one 8- or 16-byte element per lane, 64 consecutive per wave load):
+        # 512 MiB each, 4096 workgroups (k_probe_read<cx<float>> / <cx<double>>)
+        part = torch.empty(4096, dtype=torch.complex128, device=dev)
+        stream = ctypes.c_void_p(torch.cuda.current_stream().cuda_stream)
+        for esz in (8, 16):
+            _capi.check(_capi.lib().bem_probe_read(ctypes.c_void_p(a.data_ptr()), (512 << 20) // esz, esz,
+                                                    ctypes.c_void_p(part.data_ptr()), 4096, stream), "bem_probe_read")
         torch.cuda.synchronize()
     if not args.no_block_product:
         # the many-right-hand-side configuration (Maxwell EFIE, 64 plane-wave

@@ -235,6 +235,13 @@ int bem_op_autotune(bem_op* op, int flags, int iters, char* report, int report_l
  * No reference counterpart (the reference times with NGSolve Timers,
  * src/hmat.cpp:650-651). */
 int bem_op_profile_apply(bem_op* op, int flags, int iters, double* ms, void* stream);
+/* measurement: read n elements of elem_bytes (8 or 16) bytes from the device
+ * buffer buf with the apply kernels' access shape (each wave load is 64
+ * consecutive elements, one per lane) and write one sum per workgroup to
+ * partial (npartial elements of elem_bytes, device; npartial workgroups).
+ * For calibrating HBM byte counters against a read of known size.  No
+ * reference counterpart. */
+int bem_probe_read(const void* buf, int64_t n, int elem_bytes, void* partial, int npartial, void* stream);
 int bem_op_destroy(bem_op* op);
 
 /* Potential of a trial-side density at arbitrary points: replaces

@@ -53,7 +53,7 @@ class BlockInfo(ctypes.Structure):
 EXPORTS = ["bem_last_error", "bem_device_count", "bem_reserve", "bem_test_points", "bem_test_compression_aca", "bem_mesh_create", "bem_mesh_set_curved", "bem_mesh_destroy", "bem_space_create", "bem_space_create_on", "bem_space_set_dual_mapping",
            "bem_space_dofs",
            "bem_space_ndof", "bem_space_destroy", "bem_op_create", "bem_op_create_multi", "bem_op_assemble", "bem_op_apply",
-           "bem_op_stats", "bem_op_autotune", "bem_op_profile_apply", "bem_op_destroy", "bem_op_num_blocks", "bem_op_block_info", "bem_op_block_dofs",
+           "bem_op_stats", "bem_op_autotune", "bem_op_profile_apply", "bem_probe_read", "bem_op_destroy", "bem_op_num_blocks", "bem_op_block_info", "bem_op_block_dofs",
            "bem_op_block_data", "bem_op_tree", "bem_op_potential_dim", "bem_op_potential", "bem_op_potential_noalloc", "bem_arnoldi_work_bytes", "bem_arnoldi_step", "bem_batch_create", "bem_batch_sizes", "bem_batch_apply",
            "bem_batch_destroy"]
 
@@ -91,6 +91,7 @@ def lib():
     L.bem_op_apply.argtypes = [P, ctypes.c_int, P, P, P, P, ctypes.c_int, ctypes.c_int, ctypes.c_int, P]
     L.bem_op_stats.argtypes = [P, ctypes.POINTER(Stats)]
     L.bem_op_profile_apply.argtypes = [P, ctypes.c_int, ctypes.c_int, dp, P]
+    L.bem_probe_read.argtypes = [P, ctypes.c_int64, ctypes.c_int, P, ctypes.c_int, P]
     L.bem_op_destroy.argtypes = [P]
     L.bem_op_num_blocks.argtypes = [P]
     L.bem_op_block_info.argtypes = [P, ctypes.c_int, ctypes.POINTER(BlockInfo)]

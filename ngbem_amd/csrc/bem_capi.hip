@@ -498,6 +498,12 @@ int bem_op_apply(bem_op* h, int flags, const double* alpha, const void* x, const
   });
 }
 
+int bem_probe_read(const void* buf, int64_t n, int elem_bytes, void* partial, int npartial, void* stream) {
+  if (!buf || !partial || n < 0 || npartial < 1 || (elem_bytes != 8 && elem_bytes != 16))
+    return fail(-1, "bem_probe_read: bad arguments");
+  return guarded([&] { bem::probe_read(buf, n, elem_bytes, partial, npartial, (hipStream_t)stream); });
+}
+
 int bem_op_profile_apply(bem_op* h, int flags, int iters, double* ms, void* stream) {
   if (!h || !ms || iters < 1) return fail(-1, "bem_op_profile_apply: bad arguments");
   return guarded([&] {

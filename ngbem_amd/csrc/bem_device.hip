@@ -4475,6 +4475,31 @@ void build_matvec_plan(Operator& op, hipStream_t s) {
   pt_.mark("scratch + sync");
 }
 
+// a streaming read with the load shape of the apply kernels: lane l of a wave
+// loads element 64 q + l of its group q (8 or 16 bytes), grid-stride; one sum
+// per workgroup keeps the loads
+template <class V>
+__global__ __launch_bounds__(256) void k_probe_read(const V* __restrict__ a, int64_t n, V* __restrict__ part) {
+  __shared__ V red[256];
+  V acc = zero_<V>();
+  for (int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x; i < n; i += (int64_t)gridDim.x * 256) acc += a[i];
+  red[threadIdx.x] = acc;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    V s = zero_<V>();
+    for (int k = 0; k < 256; k++) s += red[k];
+    part[blockIdx.x] = s;
+  }
+}
+
+void probe_read(const void* buf, int64_t n, int elem_bytes, void* partial, int npartial, hipStream_t s) {
+  if (elem_bytes == 8)
+    hipLaunchKernelGGL((k_probe_read<fc>), dim3((unsigned)npartial), dim3(256), 0, s, (const fc*)buf, n, (fc*)partial);
+  else
+    hipLaunchKernelGGL((k_probe_read<zc>), dim3((unsigned)npartial), dim3(256), 0, s, (const zc*)buf, n, (zc*)partial);
+  BEM_HIP(hipGetLastError());
+}
+
 void make_mixed_copy(Operator& op, hipStream_t s) {
   const size_t n = (size_t)op.near_entries, nu = 0;
   (void)nu;

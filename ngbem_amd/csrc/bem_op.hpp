@@ -327,6 +327,8 @@ void assemble_near(Operator& op, hipStream_t s);
 void assemble_far(Operator& op, hipStream_t s);
 void build_matvec_plan(Operator& op, hipStream_t s);
 void make_mixed_copy(Operator& op, hipStream_t s);
+// calibration read with the apply kernels' load shape (bem_probe_read)
+void probe_read(const void* buf, int64_t n, int elem_bytes, void* partial, int npartial, hipStream_t s);
 // dual-mapped SurfaceL2 sides: scale the stored blocks (dense entries, U rows,
 // V columns) by the test and trial dofs' factors; nothing when neither side
 // has one
