@@ -381,22 +381,29 @@ int bem_op_assemble(bem_op* h, void* stream) {
     BEM_HIP(hipGetDevice(&dev));
     hipStream_t sn;
     BEM_HIP(hipStreamCreateWithFlags(&sn, hipStreamNonBlocking));
+    // (NGBEM_ASSEMBLY_SERIAL=1: near field, then far field, on this thread:
+    // each one's kernel times without the other beside it)
+    const char* ser = std::getenv("NGBEM_ASSEMBLY_SERIAL");
+    const bool serial = ser && std::string(ser) == "1";
     std::exception_ptr near_err;
-    std::thread near_thread([&] {
+    auto near_job = [&] {
       try {
         BEM_HIP(hipSetDevice(dev));
         bem::assemble_near(op, sn);
       } catch (...) {
         near_err = std::current_exception();
       }
-    });
+    };
+    std::thread near_thread;
+    if (serial) near_job();
+    else near_thread = std::thread(near_job);
     std::exception_ptr far_err;
     try {
       bem::assemble_far(op, s);
     } catch (...) {
       far_err = std::current_exception();
     }
-    near_thread.join();
+    if (near_thread.joinable()) near_thread.join();
     (void)hipStreamDestroy(sn);
     if (far_err) std::rethrow_exception(far_err);
     if (near_err) std::rethrow_exception(near_err);

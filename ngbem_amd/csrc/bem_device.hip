@@ -814,6 +814,46 @@ __device__ __forceinline__ int patch_pos(bool multi, const int* __restrict__ pat
   return lo;
 }
 
+// An element pair lies in every near block holding one of its dof
+// combinations; inside one batch of blocks its element matrix is computed
+// once, in the first such block (the owner), and the other blocks' slots
+// point there.  The owner is the least near-block index in [k0, k] over the
+// row leaves of e's dofs (eleaf[e], -1 ends the list) and the column leaves of
+// f's dofs (fcol[f]), looked up in the leaf-pair table lp (row leaf x column
+// leaf -> near-table index, -1 for a low-rank block); x = -2 marks an element
+// with more than four leaves, whose pairs are computed in every block.
+__device__ __forceinline__ int pair_owner(int e, int f, int k, int k0, const int4* __restrict__ eleaf,
+                                          const int4* __restrict__ fcol, const int* __restrict__ lp, int ncl) {
+  const int4 L = eleaf[e], C = fcol[f];
+  if (L.x == -2 || C.x == -2) return k;
+  const int ls[4] = {L.x, L.y, L.z, L.w}, cs[4] = {C.x, C.y, C.z, C.w};
+  int owner = k;
+  for (int a = 0; a < 4 && ls[a] >= 0; a++)
+    for (int b = 0; b < 4 && cs[b] >= 0; b++) {
+      const int ns = lp[(int64_t)ls[a] * ncl + cs[b]];
+      if (ns >= k0 && ns < owner) owner = ns;
+    }
+  return owner;
+}
+
+// once-per-pair passes of k_near_regular.  mode 1: every slot of a chunk gets
+// its source slot (src), and the chunk's pairs that this block computes
+// (owner, not touching) are packed to the front of its 256-entry list (their
+// positions in the chunk) with their count; mode 2: the threads of a chunk
+// take the packed pairs, so the waves past the count end at once.
+struct PairOwners {
+  const int4* eleaf;
+  const int4* fcol;
+  const int* lp;
+  int ncl;
+  int* src;              // per slot of the batch: the slot holding its element matrix
+  unsigned char* list;   // per chunk of the batch: 256 positions
+  int* count;            // per chunk of the batch
+  int k0;                // first block of the batch
+  int cbase;             // this launch's first chunk, counted from the batch's first
+  int mode;              // 0: every pair, 1: sources and lists, 2: the listed pairs
+};
+
 template <class KER, class EI, class EJ, bool DIRECT>
 __global__ __launch_bounds__(256) void k_near_regular(const NearBlk* __restrict__ blks, const int2* __restrict__ chunks,
                                                       const int* __restrict__ pIb, const int* __restrict__ pJb,
@@ -825,11 +865,49 @@ __global__ __launch_bounds__(256) void k_near_regular(const NearBlk* __restrict_
                                                       const double* __restrict__ qg,
                                                       typename KER::T* __restrict__ val,
                                                       typename KER::T* __restrict__ em, int64_t em_base, int emI,
-                                                      int emJ) {
+                                                      int emJ, PairOwners po, int multi_i, int multi_j) {
   typedef typename KER::T T;
   const int2 ch = chunks[blockIdx.x];
   const NearBlk nb = blks[ch.x];
-  const int64_t p = (int64_t)ch.y * 256 + threadIdx.x;  // chunk ch.y of 256 pairs of the block
+  const int cb = po.cbase + (int)blockIdx.x;  // chunk index within the batch (once-per-pair passes)
+  int tp = threadIdx.x;
+  if (!DIRECT && po.mode == 2) {  // the packed pairs this block computes
+    if (tp >= po.count[cb]) return;
+    tp = po.list[(int64_t)cb * 256 + tp];
+  }
+  const int64_t p = (int64_t)ch.y * 256 + tp;  // chunk ch.y of 256 pairs of the block
+  if (!DIRECT && po.mode == 1) {  // sources of the chunk's slots and the packed list
+    bool need = false;
+    if (p < (int64_t)nb.pm * nb.pn) {
+      const int ja = (int)(p / nb.pm), ia = (int)(p - (int64_t)ja * nb.pm);
+      const int e = pIb[nb.pI + ia], f = pJb[nb.pJ + ja];
+      const int4 ve = vid[e], vf = vid[f];
+      const bool touch = ve.x == vf.x || ve.x == vf.y || ve.x == vf.z || ve.y == vf.x || ve.y == vf.y ||
+                         ve.y == vf.z || ve.z == vf.x || ve.z == vf.y || ve.z == vf.z;
+      const int sidx = (int)((nb.emoff - em_base) + (int64_t)ja * nb.pm + ia);
+      int src = sidx;
+      const int own = pair_owner(e, f, ch.x, po.k0, po.eleaf, po.fcol, po.lp, po.ncl);
+      if (own != ch.x) {
+        const NearBlk ob = blks[own];
+        const int oa = patch_pos(multi_i, pIb, ob.pI, ob.pm, e, dofs_t, ipt, ob.r0);
+        const int oc = patch_pos(multi_j, pJb, ob.pJ, ob.pn, f, dofs_r, ipr, ob.c0);
+        if (oa >= 0 && oa < ob.pm && oc >= 0 && oc < ob.pn && pIb[ob.pI + oa] == e && pJb[ob.pJ + oc] == f)
+          src = (int)((ob.emoff - em_base) + (int64_t)oc * ob.pm + oa);
+      }
+      po.src[sidx] = src;
+      need = src == sidx && !touch;
+    }
+    __shared__ int s_cnt[4];
+    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+    const unsigned long long bal = __ballot(need);
+    if (lane == 0) s_cnt[wv] = __popcll(bal);
+    __syncthreads();
+    int off = 0;
+    for (int w = 0; w < wv; w++) off += s_cnt[w];
+    if (need) po.list[(int64_t)cb * 256 + off + __popcll(bal & ((1ull << lane) - 1ull))] = (unsigned char)threadIdx.x;
+    if (threadIdx.x == 0) po.count[cb] = s_cnt[0] + s_cnt[1] + s_cnt[2] + s_cnt[3];
+    return;
+  }
   if (p >= (int64_t)nb.pm * nb.pn) return;
   const int ja = (int)(p / nb.pm), ia = (int)(p - (int64_t)ja * nb.pm);  // consecutive lanes: consecutive test rows
   const int e = pIb[nb.pI + ia], f = pJb[nb.pJ + ja];
@@ -1260,7 +1338,7 @@ template <class T>
 __global__ __launch_bounds__(256) void k_near_gather(const NearBlk* __restrict__ blks, const int2* __restrict__ chunks,
                                                      const int* __restrict__ maps, int multi_i, int multi_j,
                                                      const T* __restrict__ em, int64_t em_base, int emI, int emJ,
-                                                     T* __restrict__ val) {
+                                                     const int* __restrict__ src, T* __restrict__ val) {
   const int2 ch = chunks[blockIdx.x];
   const NearBlk nb = blks[ch.x];
   const int64_t p = (int64_t)ch.y * 256 + threadIdx.x;
@@ -1272,7 +1350,7 @@ __global__ __launch_bounds__(256) void k_near_gather(const NearBlk* __restrict__
   if (multi_i) { a0 = mr[i]; a1 = mr[i + 1]; }
   if (multi_j) { c0 = mc[j]; c1 = mc[j + 1]; }
   const int64_t st = (int64_t)emI * emJ;
-  const T* E = em + (nb.emoff - em_base) * st;
+  const int64_t base = nb.emoff - em_base;
   T s = zero_<T>();
   for (int ka = a0; ka < a1; ka++) {
     const int ea = multi_i ? mr[nb.m + 1 + ka] : ka * DOF_STRIDE;
@@ -1280,7 +1358,9 @@ __global__ __launch_bounds__(256) void k_near_gather(const NearBlk* __restrict__
     for (int kc = c0; kc < c1; kc++) {
       const int ec = multi_j ? mc[nb.n + 1 + kc] : kc * DOF_STRIDE;
       const int c = ec / DOF_STRIDE, kk = ec - (ec / DOF_STRIDE) * DOF_STRIDE;
-      s += E[((int64_t)c * nb.pm + a) * st + kl * emJ + kk];
+      int64_t slot = base + (int64_t)c * nb.pm + a;
+      if (src) slot = src[slot];
+      s += em[slot * st + kl * emJ + kk];
     }
   }
   val[nb.off + (int64_t)j * nb.m + i] = s;
@@ -3167,6 +3247,13 @@ static bool near_em_on() {
   return on;
 }
 
+// element pairs of the one-lane evaluators computed once per batch of blocks
+// (NGBEM_NEAR_DEDUP=0: once per block holding them)
+static bool near_dedup_on() {
+  const char* e = std::getenv("NGBEM_NEAR_DEDUP");
+  return !(e && std::string(e) == "0");
+}
+
 // device bytes for the element matrices of one batch of near blocks (at most
 // a sixth of the free memory; NGBEM_NEAR_EM_GB, read at each assembly,
 // overrides the 4 GiB default)
@@ -3330,6 +3417,52 @@ void assemble_near(Operator& op, hipStream_t s) {
     }
     gchunk0[op.near.size()] = gchunks.size();
   }
+  // once-per-pair tables: the leaf-pair table (row leaf x column leaf -> near
+  // index or -1), per test element the row leaves of its dofs, per trial
+  // element the column leaves of its dofs (at most four each; -2 marks more,
+  // and such an element's pairs are computed in every block)
+  std::vector<int> lp;
+  std::vector<int4> eleaf, fcol;
+  const int nlr = (int)tt.leaves.size(), nlc = (int)rt.leaves.size();
+  bool owners_ok = em_mode && (int64_t)nlr * nlc <= ((int64_t)1 << 26);
+  if (owners_ok) {
+    const BlockLookup& lk = op.lookup;
+    std::vector<int> cleaf(rt.perm.size(), 0);  // trial position -> column leaf
+    for (int i = 0; i < nlc; i++) {
+      const Cluster& c = rt.cl[rt.leaves[i]];
+      for (long q = c.permupos; q < c.permupos + c.number; q++) cleaf[q] = i;
+    }
+    lp.assign((size_t)nlr * nlc, -1);
+    for (size_t b = 0; b < op.blocks.size(); b++) {
+      const Block& bl = op.blocks[b];
+      if (!(bl.near || bl.fallback) || bl.m == 0 || bl.n == 0) continue;
+      const int la0 = lk.pos2leaf[bl.r0], la1 = lk.pos2leaf[bl.r0 + bl.m - 1];
+      const int lb0 = cleaf[bl.c0], lb1 = cleaf[bl.c0 + bl.n - 1];
+      for (int la = la0; la <= la1; la++)
+        for (int lb = lb0; lb <= lb1; lb++) lp[(size_t)la * nlc + lb] = op.blk_slot[b];
+    }
+    eleaf.assign(m.nt, make_int4(-1, -1, -1, -1));
+    fcol.assign(m.nt, make_int4(-1, -1, -1, -1));
+    auto leaves_of = [](const Space& sp, int e, auto leaf_of) {
+      int v[4] = {-1, -1, -1, -1}, n = 0;
+      for (int i = 0; i < sp.nloc && n <= 4; i++) {
+        const int d = sp.dofs[e * sp.nloc + i];
+        if (d < 0) continue;
+        const int l = leaf_of(d);
+        bool seen = false;
+        for (int q = 0; q < std::min(n, 4); q++) seen = seen || v[q] == l;
+        if (!seen) {
+          if (n < 4) v[n] = l;
+          n++;
+        }
+      }
+      return n > 4 ? make_int4(-2, -1, -1, -1) : make_int4(v[0], v[1], v[2], v[3]);
+    };
+    for (int e = 0; e < m.nt; e++) {
+      if (te.has(e)) eleaf[e] = leaves_of(te, e, [&](int d) { return lk.pos2leaf[tt.iperm[d]]; });
+      if (tr.has(e)) fcol[e] = leaves_of(tr, e, [&](int d) { return cleaf[rt.iperm[d]]; });
+    }
+  }
   pt_.mark("element-pair maps");
 
   op.d_near_blk.upload(op.near, s);
@@ -3362,7 +3495,7 @@ void assemble_near(Operator& op, hipStream_t s) {
   // per launch group: (first event, last event) pairs of the regular, singular
   // and gather launches
   std::vector<std::array<int, 2>> tr_reg, tr_sing, tr_gat;
-  DBuf d_em, d_maps, d_gchunks;
+  DBuf d_em, d_maps, d_gchunks, d_src, d_lp, d_eleaf, d_fcol, d_plist, d_pcount;
   std::vector<int4> sorted_pairs[3];  // host sources of asynchronous uploads: alive until the stream is synchronised
   if (nmaps.empty()) nmaps.push_back(0);
   dispatch_full(op, [&](auto ker, auto ei, auto ej) {
@@ -3376,7 +3509,11 @@ void assemble_near(Operator& op, hipStream_t s) {
     const int emJ = EJ::WIDE ? tr.nloc : std::max(EJ::NL, tr.nloc);
     if (EI::WIDE && (te.nloc > NLOC_MAX || tr.nloc > NLOC_MAX))
       throw std::runtime_error("near field: more local functions than NLOC_MAX");
-    const int64_t slot_bytes = (int64_t)emI * emJ * (int64_t)sizeof(T);
+    // once per pair for the one-lane evaluators (the wide ones keep a slot per block)
+    const bool dedup = owners_ok && !EI::WIDE && near_dedup_on();
+    // per slot: the element matrix, and with the once-per-pair passes its
+    // source index and list position
+    const int64_t slot_bytes = (int64_t)emI * emJ * (int64_t)sizeof(T) + (dedup ? 5 : 0);
     // batches of blocks whose element matrices fit the budget
     size_t free_b = 0, total_b = 0;
     BEM_HIP(hipMemGetInfo(&free_b, &total_b));
@@ -3412,13 +3549,38 @@ void assemble_near(Operator& op, hipStream_t s) {
       for (size_t b = 0; b < batches.size(); b++)
         for (size_t k = batches[b].k0; k < batches[b].k1; k++) bat[k] = (int)b;
       for (int c = 0; c < 3; c++) {
+        // once per pair: of the entries of one touching pair (consecutive in
+        // the list, one per block) keep, per batch, the one of the first
+        // block; pair_owner names the same block for the other slots
+        std::vector<int4> kept;
+        const std::vector<int4>* list = &pairs[c];
+        if (use_em && dedup) {
+          kept.reserve(pairs[c].size());
+          const std::vector<int4>& P = pairs[c];
+          for (size_t i = 0; i < P.size();) {
+            size_t j = i;
+            while (j < P.size() && P[j].y == P[i].y && P[j].z == P[i].z) j++;
+            if (eleaf[P[i].y].x == -2 || fcol[P[i].z].x == -2) {
+              for (size_t q = i; q < j; q++) kept.push_back(P[q]);
+            } else {
+              for (size_t q = i; q < j; q++) {
+                bool first = true;  // no entry of the same batch with a smaller block
+                for (size_t r = i; r < j && first; r++)
+                  if (r != q && bat[P[r].x] == bat[P[q].x] && P[r].x < P[q].x) first = false;
+                if (first) kept.push_back(P[q]);
+              }
+            }
+            i = j;
+          }
+          list = &kept;
+        }
         sp0[c].assign(batches.size() + 1, 0);
-        for (const int4& pr : pairs[c]) sp0[c][bat[pr.x] + 1]++;
+        for (const int4& pr : *list) sp0[c][bat[pr.x] + 1]++;
         for (size_t b = 0; b < batches.size(); b++) sp0[c][b + 1] += sp0[c][b];
         std::vector<int4>& sorted = sorted_pairs[c];
-        sorted.resize(pairs[c].size());
+        sorted.resize(list->size());
         std::vector<size_t> pos(sp0[c].begin(), sp0[c].end() - 1);
-        for (const int4& pr : pairs[c]) sorted[pos[bat[pr.x]]++] = pr;
+        for (const int4& pr : *list) sorted[pos[bat[pr.x]]++] = pr;
         op.d_spairs[c].upload(sorted, s);
       }
     }
@@ -3430,6 +3592,16 @@ void assemble_near(Operator& op, hipStream_t s) {
       em = d_em.as<T>();
       d_maps.upload(nmaps, s);
       d_gchunks.upload(gchunks, s);
+      if (dedup) {
+        size_t max_chunks = 1;
+        for (const Batch& bt : batches) max_chunks = std::max(max_chunks, chunk0[bt.k1] - chunk0[bt.k0]);
+        d_src.alloc((size_t)std::max<int64_t>(max_slots, 1) * 4);
+        d_plist.alloc(max_chunks * 256);
+        d_pcount.alloc(max_chunks * 4);
+        d_lp.upload(lp, s);
+        d_eleaf.upload(eleaf, s);
+        d_fcol.upload(fcol, s);
+      }
     } else if (!direct && off) {
       BEM_HIP(hipMemsetAsync(op.d_near_val.p, 0, (size_t)off * esz, s));
     }
@@ -3437,6 +3609,10 @@ void assemble_near(Operator& op, hipStream_t s) {
     for (const Batch& bt : batches) {
       const size_t c0 = chunk0[bt.k0], c1 = chunk0[bt.k1];
       const int64_t em0 = bt.em0;
+      PairOwners po{};
+      if (use_em && dedup)
+        po = PairOwners{d_eleaf.as<int4>(), d_fcol.as<int4>(), d_lp.as<int>(), nlc, d_src.as<int>(),
+                        d_plist.as<unsigned char>(), d_pcount.as<int>(), (int)bt.k0, 0, 1};
       const int t0 = (int)tev.size();
       mark_ev();
       if constexpr (EI::WIDE) {
@@ -3451,23 +3627,29 @@ void assemble_near(Operator& op, hipStream_t s) {
                              op.d_iperm_test.as<int>(), op.d_iperm_trial.as<int>(), ker, op.d_qgeo.as<double>(),
                              op.d_gpoly.as<double>(), op.geo_order, val, em, em0, emI, emJ, (int)p1i, (int)p1j);
       } else {
-        // launches of at most 2^22 chunks (2^30 threads)
+        // launches of at most 2^22 chunks (2^30 threads); with the
+        // once-per-pair passes, the sources and lists first, then the listed pairs
+        for (int pass = (po.mode ? 1 : 0); pass <= (po.mode ? 2 : 0); pass++)
         for (size_t c = c0; c < c1; c += (size_t)1 << 22) {
           const size_t nc = std::min(c1 - c, (size_t)1 << 22);
+          if (po.mode) {
+            po.mode = pass;
+            po.cbase = (int)(c - c0);
+          }
           if (direct)
             hipLaunchKernelGGL((k_near_regular<KER, EI, EJ, true>), dim3((unsigned)nc), dim3(256), 0, s,
                                op.d_near_blk.as<NearBlk>(), op.d_near_chunks.as<int2>() + c, pIb, pJb,
                                op.d_geo.as<double>(), op.d_vid.as<int4>(), op.d_quad.as<double>(),
                                op.d_sdat.as<double>(), op.d_trig.as<double>(), ntq, op.d_dofs_test.as<int>(),
                                op.d_dofs_trial.as<int>(), op.d_iperm_test.as<int>(), op.d_iperm_trial.as<int>(), ker,
-                               op.d_qgeo.as<double>(), val, (T*)nullptr, (int64_t)0, 1, 1);
+                               op.d_qgeo.as<double>(), val, (T*)nullptr, (int64_t)0, 1, 1, PairOwners{}, 0, 0);
           else
             hipLaunchKernelGGL((k_near_regular<KER, EI, EJ, false>), dim3((unsigned)nc), dim3(256), 0, s,
                                op.d_near_blk.as<NearBlk>(), op.d_near_chunks.as<int2>() + c, pIb, pJb,
                                op.d_geo.as<double>(), op.d_vid.as<int4>(), op.d_quad.as<double>(),
                                op.d_sdat.as<double>(), op.d_trig.as<double>(), ntq, op.d_dofs_test.as<int>(),
                                op.d_dofs_trial.as<int>(), op.d_iperm_test.as<int>(), op.d_iperm_trial.as<int>(), ker,
-                               op.d_qgeo.as<double>(), val, em, em0, emI, emJ);
+                               op.d_qgeo.as<double>(), val, em, em0, emI, emJ, po, (int)p1i, (int)p1j);
         }
       }
       mark_ev();
@@ -3506,7 +3688,7 @@ void assemble_near(Operator& op, hipStream_t s) {
         for (size_t g = g0; g < g1; g += (size_t)1 << 22)
           hipLaunchKernelGGL((k_near_gather<T>), dim3((unsigned)std::min(g1 - g, (size_t)1 << 22)), dim3(256), 0, s,
                              op.d_near_blk.as<NearBlk>(), d_gchunks.as<int2>() + g, d_maps.as<int>(), (int)p1i,
-                             (int)p1j, (const T*)em, em0, emI, emJ, val);
+                             (int)p1j, (const T*)em, em0, emI, emJ, (const int*)po.src, val);
       }
       mark_ev();
       tr_reg.push_back({t0, t0 + 1});

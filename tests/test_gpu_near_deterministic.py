@@ -62,3 +62,30 @@ def test_near_field_in_small_batches_is_identical(gpu, name, make, monkeypatch):
     monkeypatch.setenv("NGBEM_NEAR_EM_GB", "0.002")  # 2 MiB of element matrices per batch
     b = _near_blocks(make(mesh))
     _same(a, b, name)
+
+
+def test_atomic_fallback_for_blocks_beyond_the_budget(gpu, monkeypatch):
+    """A block whose element matrices alone exceed the budget sends the
+    operator to the per-pair atomic adds; the entries agree with the
+    element-matrix form to rounding."""
+    mesh = nb.icosphere(3)
+    make = CASES[0][1]
+    a = _near_blocks(make(mesh))
+    monkeypatch.setenv("NGBEM_NEAR_EM_GB", "0.0001")  # 100 KiB: below one block's element matrices
+    b = _near_blocks(make(mesh))
+    assert len(a) == len(b)
+    worst = max(float(np.abs(x - y).max() / np.abs(x).max()) for x, y in zip(a, b))
+    assert worst < 1e-13
+
+
+@pytest.mark.parametrize("name,make", CASES[:4], ids=[c[0].replace(" ", "_") for c in CASES[:4]])
+def test_once_per_pair_equals_once_per_block(gpu, name, make, monkeypatch):
+    """Each distinct element pair of a batch of blocks computed once (its first
+    block's slot, the default) against once per block holding it
+    (NGBEM_NEAR_DEDUP=0): the same element matrices summed in the same order,
+    so bit-identical blocks."""
+    mesh = nb.icosphere(3)
+    a = _near_blocks(make(mesh))
+    monkeypatch.setenv("NGBEM_NEAR_DEDUP", "0")
+    b = _near_blocks(make(mesh))
+    _same(a, b, name)
