@@ -327,7 +327,7 @@ def main():
         # the discrete solution to 1e-12 (untimed): what both forms are measured against
         u_ref = nb.GMRes(A64, f, pre=pre, tol=1e-12, maxsteps=400, printrates=False)
 
-        def timed(fn, reps=3):
+        def timed(fn, reps=5):
             ts, res = [], None
             for _ in range(reps):
                 torch.cuda.synchronize()
