@@ -4566,6 +4566,8 @@ void build_matvec_plan(Operator& op, hipStream_t s) {
   std::vector<Tile> t_tsum;
   std::vector<Seg> s_tsum;
   std::exception_ptr err[6];
+  double tdone[6] = {0, 0, 0, 0, 0, 0};
+  const double tstart = now();
   auto guarded = [&](int i, auto fn) {
     return std::thread([&, i, fn] {
       try {
@@ -4573,13 +4575,31 @@ void build_matvec_plan(Operator& op, hipStream_t s) {
       } catch (...) {
         err[i] = std::current_exception();
       }
+      tdone[i] = now() - tstart;
     });
   };
   std::thread th[6] = {
       guarded(0, [&] {
+        // the Ut plan, then its flat column lists for both storages side by side
         plan(op, false, t_ut, s_ut, PLAN_UT);
-        flat_ok = flat_columns(t_ut, s_ut, cols, range);
-        if (flat_ok && op.mixed && panels_on()) pan_ok = panel_columns(t_ut, s_ut, pcols, prange, pans, ptot);
+        std::exception_ptr perr;
+        std::thread pth;
+        if (op.mixed && panels_on())
+          pth = std::thread([&] {
+            try {
+              pan_ok = panel_columns(t_ut, s_ut, pcols, prange, pans, ptot);
+            } catch (...) {
+              perr = std::current_exception();
+            }
+          });
+        try {
+          flat_ok = flat_columns(t_ut, s_ut, cols, range);
+        } catch (...) {
+          if (pth.joinable()) pth.join();
+          throw;
+        }
+        if (pth.joinable()) pth.join();
+        if (perr) std::rethrow_exception(perr);
       }),
       guarded(1, [&] { plan(op, false, t_fwd, s_fwd); }),
       guarded(2, [&] { plan(op, true, t_trn, s_trn); }),
@@ -4589,6 +4609,10 @@ void build_matvec_plan(Operator& op, hipStream_t s) {
   for (auto& t : th) t.join();
   for (auto& e : err)
     if (e) std::rethrow_exception(e);
+  if (pt_.on)
+    std::fprintf(stderr, "[ngbem plan] threads done at (ms): Ut+columns %.1f fwd %.1f trn %.1f items %.1f "
+                 "col items %.1f tsum %.1f\n", tdone[0] * 1e3, tdone[1] * 1e3, tdone[2] * 1e3, tdone[3] * 1e3,
+                 tdone[4] * 1e3, tdone[5] * 1e3);
   pt_.mark("host tables");
   op.d_seg_fwd_mm.upload(s_ut, s);
   op.h_tile_pos.resize(t_fwd.size());
