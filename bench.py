@@ -308,7 +308,7 @@ def main():
     # (bytes read + written / time), for the frac's context
     probe = hbm_probe(dev)
     out["roofline"]["hbm_probe"] = probe
-    log(f"HBM probe: torch copy {probe['copy_gbs']:.0f} GB/s")
+    log(f"HBM probe: torch copy {probe['copy_gbs']:.0f} GB/s, streaming read {probe['read_gbs']:.0f} GB/s")
 
     if not args.no_solve:
         # Burton-Miller combined-field solve: (1/2 M + C) u = f; FP64 GMRES vs
@@ -448,9 +448,29 @@ def hbm_probe(dev, reps=10):
     e1.record()
     torch.cuda.synchronize()
     gbs = 2 * a.numel() * 4 * reps / (e0.elapsed_time(e1) * 1e-3) / 1e9
-    del a, b
+    # a read-only stream with the apply kernels' load shape (bem_probe_read:
+    # one 8-byte element per lane, 4096 workgroups looping over 1 GiB)
+    from ngbem_amd import _capi
+    part = torch.empty(4096, dtype=torch.complex128, device=dev)
+    stream = ctypes.c_void_p(torch.cuda.current_stream().cuda_stream)
+
+    def probe():
+        _capi.check(_capi.lib().bem_probe_read(ctypes.c_void_p(a.data_ptr()), a.numel() // 2, 8,
+                                                ctypes.c_void_p(part.data_ptr()), 4096, stream), "bem_probe_read")
+
+    probe()
+    torch.cuda.synchronize()
+    e0.record()
+    for _ in range(reps):
+        probe()
+    e1.record()
+    torch.cuda.synchronize()
+    rgbs = a.numel() * 4 * reps / (e0.elapsed_time(e1) * 1e-3) / 1e9
+    del a, b, part
     torch.cuda.empty_cache()
-    return {"copy_gbs": gbs, "how": "torch copy_ of 1 GiB float32, 10 calls, HIP events"}
+    return {"copy_gbs": gbs, "read_gbs": rgbs,
+            "how": "torch copy_ of 1 GiB float32 (read + written bytes), and bem_probe_read of 1 GiB "
+                   "(8-byte loads, 4096 workgroups); 10 calls each, HIP events"}
 
 
 def bench_maxwell_block(args, emit=True):
