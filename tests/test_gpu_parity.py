@@ -252,3 +252,21 @@ def test_gram_aca_option(gpu, case, monkeypatch):
         close += abs(UV[0].shape[1] - info["rank"]) <= terms
     assert nfar > 0 and close >= 0.9 * nfar
 
+
+
+def test_dense_fallback_blocks(gpu):
+    """Admissible blocks whose patches share a vertex on one mesh are stored
+    dense, as the reference's far-field exception does (src/ngbem.cpp:143-153,
+    :839-842): H1 at icosphere level 3 with leafsize 16 has some; the same
+    blocks as the oracle's, counted in the stats, entries equal to the
+    oracle's dense ones to 1e-12."""
+    op, ref = _build(("LHS", 1, 1, 3, 3, 16, 2.0, 1e-8, 0.0))
+    fb = [b for b in range(op.num_blocks()) if op.block_info(b)["fallback"]]
+    assert len(fb) > 0
+    assert op.stats()["n_fallback"] == len(fb)
+    assert fb == [b for b, rb in enumerate(ref.blocks()) if rb["fallback"]]
+    for b in fb:
+        assert op.block_info(b)["rank"] == -1
+        rows, cols = op.block_dofs(b)
+        D = ref.dense(cols, rows).real
+        assert np.abs(op.block_data(b) - D).max() <= 1e-12 * np.abs(D).max()
