@@ -264,9 +264,46 @@ def test_dense_fallback_blocks(gpu):
     fb = [b for b in range(op.num_blocks()) if op.block_info(b)["fallback"]]
     assert len(fb) > 0
     assert op.stats()["n_fallback"] == len(fb)
+    # the oracle marks its fallbacks when it assembles the admissible blocks
+    ref.assemble(subset=[b for b, rb in enumerate(ref.blocks()) if not rb["near"]])
     assert fb == [b for b, rb in enumerate(ref.blocks()) if rb["fallback"]]
     for b in fb:
         assert op.block_info(b)["rank"] == -1
         rows, cols = op.block_dofs(b)
         D = ref.dense(cols, rows).real
         assert np.abs(op.block_data(b) - D).max() <= 1e-12 * np.abs(D).max()
+
+
+@pytest.mark.parametrize("cap0", [None, "3"], ids=["plain", "resumed"])
+def test_aca_in_many_batches(gpu, cap0, monkeypatch):
+    """A workspace budget of 0.25 GiB splits the Helmholtz combined field at
+    icosphere level 4 into several batches of blocks (each projected into its
+    own piece, the pieces concatenated); with a rank capacity of 3 every block
+    also goes through the resume path.  The blocks equal those of a single
+    batch: the same factors, bit for bit, in the plain case; at the same ranks
+    and within the resume test's bar (10 eps) when resumed."""
+    if cap0:
+        monkeypatch.setenv("NGBEM_ACA_CAP0", cap0)
+    mesh = nb.icosphere(4)
+    fes = nb.SurfaceL2(mesh)
+    kw = dict(kappa=6.0, intorder=4, leafsize=40, eps=1e-6)
+    one = nb.HelmholtzCombinedFieldOperator(fes, fes, **kw)
+    monkeypatch.setenv("NGBEM_ACA_BUDGET_GB", "0.25")
+    many = nb.HelmholtzCombinedFieldOperator(fes, fes, **kw)
+    nfar = 0
+    for b in range(one.num_blocks()):
+        i1, i2 = one.block_info(b), many.block_info(b)
+        assert i1["rank"] == i2["rank"]
+        if i1["rank"] < 0:
+            continue
+        nfar += 1
+        (U1, V1), (U2, V2) = one.block_data(b), many.block_data(b)
+        if cap0 is None:
+            assert np.array_equal(U1, U2) and np.array_equal(V1, V2), f"far block {b}"
+        else:
+            A1 = U1 @ V1
+            assert np.linalg.norm(U2 @ V2 - A1) <= 10 * kw["eps"] * np.linalg.norm(A1)
+    assert nfar > 100
+    x = torch.randn(fes.ndof, dtype=torch.complex128, device=gpu)
+    y1, y2 = one.mat * x, many.mat * x
+    assert (torch.linalg.norm(y1 - y2) / torch.linalg.norm(y1)).item() < 1e-6
