@@ -2822,24 +2822,24 @@ __device__ __forceinline__ R join_planes(const R (&y)[1]) {
 }
 
 // one 16-column group of one wave: lane (i, g) holds, for the four MFMA
-// k-steps s, its row's entry of column kk + 4 g + s (split into planes) and
-// that X row's four tile values per plane
-template <class VT>
+// k-steps s, its row's entry of column kk + 4 g + s as stored (split into
+// planes only when the group runs, so a prefetched group's loads are waited
+// for after the current group's MFMAs have been issued) and that X row's
+// four tile values per plane
+template <class VT, class ST>
 struct MMBuf {
   typedef typename SlabT<VT>::R R;
   static constexpr int NP = SlabT<VT>::NP;
   typedef typename MF<R>::V4 V4;
-  R a[4][NP];
+  ST ra[4];
   V4 b[4][NP];
-  template <class ST>
   __device__ __forceinline__ void load(const ST* __restrict__ A, int64_t lda, bool rok, const R* __restrict__ X,
                                        int kk, int cnt, int g, int i) {
 #pragma unroll
     for (int s = 0; s < 4; s++) {
       const int k = kk + 4 * g + s;
-      VT av = zero_<VT>();
-      if (rok && k < cnt) cvt(av, A[(int64_t)k * lda]);
-      split_planes(av, a[s]);
+      ra[s] = zero_<ST>();
+      if (rok && k < cnt) ra[s] = A[(int64_t)k * lda];
 #pragma unroll
       for (int p = 0; p < NP; p++) {
         if (k < cnt) b[s][p] = *(const V4*)(X + ((int64_t)k * NP + p) * MM_W + 4 * i);
@@ -2860,13 +2860,21 @@ struct MMAcc2 {
 #pragma unroll
       for (int t = 0; t < 4; t++) c[p][t] = typename MF<R>::V4{R(0), R(0), R(0), R(0)};
   }
-  __device__ __forceinline__ void run(const MMBuf<VT>& m) {
+  template <class ST>
+  __device__ __forceinline__ void run(const MMBuf<VT, ST>& m) {
+    R a[4][NP];
+#pragma unroll
+    for (int s = 0; s < 4; s++) {
+      VT av;
+      cvt(av, m.ra[s]);
+      split_planes(av, a[s]);
+    }
 #pragma unroll
     for (int s = 0; s < 4; s++)
 #pragma unroll
       for (int p = 0; p < NP; p++)
 #pragma unroll
-        for (int t = 0; t < 4; t++) c[p][t] = MF<R>::mma(m.a[s][p], m.b[s][p][t], c[p][t]);
+        for (int t = 0; t < 4; t++) c[p][t] = MF<R>::mma(a[s][p], m.b[s][p][t], c[p][t]);
   }
   // result of tile t, MFMA output register v
   __device__ __forceinline__ VT get(int t, int v) const {
@@ -2918,7 +2926,7 @@ __global__ __launch_bounds__(256) void k_mm2_far_t(const MMItem* __restrict__ it
   const R* X = xs + (int64_t)b.c0 * NP * MM_W;
   MMAcc2<VT> acc;
   acc.zero();
-  MMBuf<VT> cur, nxt;
+  MMBuf<VT, ST> cur, nxt;
   int kk = 16 * wv;
   if (kk < b.n) cur.load(A, 1, rok, X, kk, b.n, g, i);
   for (; kk < b.n; kk += 64) {
@@ -2974,7 +2982,7 @@ __global__ __launch_bounds__(256) void k_mm2_rows(const MMStrip* __restrict__ st
   const int r = tl.roff + st.strip * 16 + (rok ? i : 0);
   MMAcc2<VT> acc;
   acc.zero();
-  MMBuf<VT> cur, nxt;
+  MMBuf<VT, ST> cur, nxt;
   // (segment, column group) cursor over the tile's segments with columns
   int sk = tl.s0;
   while (sk < tl.s1 && segs[sk].cnt <= 0) sk++;
