@@ -2957,6 +2957,49 @@ __global__ __launch_bounds__(256) void k_mm2_far_t(const MMItem* __restrict__ it
   }
 }
 
+// the same for a narrow block (at most mm_onewave() column groups): one wave
+// per item, four items per workgroup, every group of the item in one wave with
+// the next group loaded while this one's MFMAs run, so no LDS sum and no
+// barrier
+template <class ST, class VT>
+__global__ __launch_bounds__(256) void k_mm2_far_t1(const MMItem* __restrict__ items, int nitems,
+                                                    const FarBlk* __restrict__ fb, const ST* __restrict__ fV,
+                                                    const typename SlabT<VT>::R* __restrict__ xs,
+                                                    typename SlabT<VT>::R* __restrict__ ts) {
+  typedef typename SlabT<VT>::R R;
+  constexpr int NP = SlabT<VT>::NP;
+  const int w = __builtin_amdgcn_readfirstlane(blockIdx.x * 4 + (threadIdx.x >> 6));
+  if (w >= nitems) return;
+  const MMItem it = items[w];
+  const FarBlk b = fb[it.blk];
+  const int lane = threadIdx.x & 63;
+  const int i = lane & 15, g = lane >> 4;
+  const bool rok = it.l0 + i < b.R;
+  const ST* A = fV + b.voff + (int64_t)(rok ? it.l0 + i : 0) * b.n;  // row-major: column k at A[k]
+  const R* X = xs + (int64_t)b.c0 * NP * MM_W;
+  MMAcc2<VT> acc;
+  acc.zero();
+  MMBuf<VT, ST> cur, nxt;
+  if (b.n > 0) cur.load(A, 1, rok, X, 0, b.n, g, i);
+  for (int kk = 0; kk < b.n; kk += 16) {
+    if (kk + 16 < b.n) nxt.load(A, 1, rok, X, kk + 16, b.n, g, i);
+    acc.run(cur);
+    cur = nxt;
+  }
+#pragma unroll
+  for (int t = 0; t < 4; t++)
+#pragma unroll
+    for (int v = 0; v < 4; v++) {
+      const int r = MF<R>::row(g, v);
+      if (it.l0 + r >= b.R) continue;
+      R a[NP];
+      split_planes(acc.get(t, v), a);
+      const int64_t row = b.toff + it.l0 + r;
+#pragma unroll
+      for (int p = 0; p < NP; p++) ts[(row * NP + p) * MM_W + 4 * i + t] = a[p];
+    }
+}
+
 // forward block product over the row tiles: one wave per (tile, 16-row
 // strip), all of the tile's segments and columns, the next 16-column group
 // loaded while the current one's MFMAs run; four strips per workgroup
@@ -5555,6 +5598,19 @@ static void apply_mm_t(Operator& op, const XT* x, XT* y, int nrhs, int64_t ld, X
 // buffers of FP64 values cost more occupancy than they gain: 28.3 against
 // 22.0 ms at the Maxwell EFIE workload); NGBEM_MM=v1 keeps the four-product
 // kernels above for both
+// low-rank blocks of at most this many 16-column groups take one wave per
+// 16 rank rows in the slab block product, wider ones four (NGBEM_MM_ONEWAVE,
+// 0: four waves for every block).  At the Maxwell EFIE workload (icosphere
+// L6, leafsize 40, 64 right-hand sides) the product took 10.77 / 10.63 /
+// 10.45 / 10.35 / 10.35 / 10.51 / 10.80 ms with 0 / 4 / 8 / 16 / 32 / 64 /
+// all groups: most of its far blocks are 41-90 columns wide, one or two
+// groups per wave when four waves share them, so the LDS sum and the barrier
+// cost more than the split gains
+static int mm_onewave() {
+  const char* e = std::getenv("NGBEM_MM_ONEWAVE");
+  return e ? std::max(0, std::atoi(e)) : 16;
+}
+
 static bool mm_slab() {
   static const bool on = [] {
     const char* e = std::getenv("NGBEM_MM");
@@ -5569,13 +5625,18 @@ static void apply_mm2_t(Operator& op, const XT* x, XT* y, int nrhs, int64_t ld, 
   typedef typename SlabT<VT>::R R;
   constexpr int NP = SlabT<VT>::NP;
   const int nin = op.ndof_trial(), nout = op.ndof_test();
-  if (op.n_mm_items < 0) {
-    std::vector<MMItem> items;
+  if (op.n_mm2_wide < 0) {
+    // items of wide blocks (four waves each) first, then those of blocks of at
+    // most mm_onewave() column groups (one wave each)
+    const int64_t nmax = 16 * (int64_t)mm_onewave();
+    std::vector<MMItem> items, narrow;
     for (size_t k = 0; k < op.far.size(); k++)
-      for (int l = 0; l < op.far[k].R; l += 16) items.push_back(MMItem{(int)k, l});
-    op.d_mm_items.upload(items, s);
+      for (int l = 0; l < op.far[k].R; l += 16) (op.far[k].n <= nmax ? narrow : items).push_back(MMItem{(int)k, l});
+    op.n_mm2_wide = (int)items.size();
+    op.n_mm2_narrow = (int)narrow.size();
+    items.insert(items.end(), narrow.begin(), narrow.end());
+    op.d_mm2_items.upload(items, s);
     BEM_HIP(hipStreamSynchronize(s));
-    op.n_mm_items = (int)items.size();
   }
   const size_t need_x = (size_t)std::max(nin, 1) * MM_W * NP * sizeof(R);
   const size_t need_y = (size_t)std::max(nout, 1) * MM_W * sizeof(VT);
@@ -5592,9 +5653,13 @@ static void apply_mm2_t(Operator& op, const XT* x, XT* y, int nrhs, int64_t ld, 
     const int nr = std::min(MM_W, nrhs - r0);
     hipLaunchKernelGGL((k_gather_slab<VT, XT>), dim3((unsigned)(((int64_t)nin * MM_W + 255) / 256)), dim3(256), 0, s,
                        op.d_perm_trial.as<int>(), nin, x, ld, r0, nr, xs);
-    if (op.n_mm_items > 0)
-      hipLaunchKernelGGL((k_mm2_far_t<ST, VT>), dim3(op.n_mm_items), dim3(256), 0, s, op.d_mm_items.as<MMItem>(),
+    if (op.n_mm2_wide > 0)
+      hipLaunchKernelGGL((k_mm2_far_t<ST, VT>), dim3(op.n_mm2_wide), dim3(256), 0, s, op.d_mm2_items.as<MMItem>(),
                          op.d_far_blk.as<FarBlk>(), fV, (const R*)xs, ts);
+    if (op.n_mm2_narrow > 0)
+      hipLaunchKernelGGL((k_mm2_far_t1<ST, VT>), dim3((unsigned)((op.n_mm2_narrow + 3) / 4)), dim3(256), 0, s,
+                         op.d_mm2_items.as<MMItem>() + op.n_mm2_wide, op.n_mm2_narrow, op.d_far_blk.as<FarBlk>(), fV,
+                         (const R*)xs, ts);
     if (op.n_mm_strips > 0)
       hipLaunchKernelGGL((k_mm2_rows<ST, VT>), dim3((unsigned)((op.n_mm_strips + 3) / 4)), dim3(256), 0, s,
                          op.d_mm_strips.as<MMStrip>(), op.n_mm_strips, op.d_tile_fwd.as<Tile>(),

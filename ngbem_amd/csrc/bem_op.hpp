@@ -307,6 +307,8 @@ struct Operator {
   int n_mm_items = -1;
   DBuf d_mm_strips;  // (row tile, 16-row strip) of the slab-form block product
   int n_mm_strips = 0;
+  DBuf d_mm2_items;  // its low-rank items: those of wide blocks, then those of narrow ones
+  int n_mm2_wide = -1, n_mm2_narrow = 0;
 
   int ndof_trial() const { return trial->ndof; }
   int ndof_test() const { return test->ndof; }
