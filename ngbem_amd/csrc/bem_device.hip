@@ -2674,6 +2674,33 @@ __global__ __launch_bounds__(256) void k_mm_far_t(const MMItem* __restrict__ ite
   }
 }
 
+// the same for a narrow block: one wave per item, four items per workgroup,
+// no LDS sum and no barrier
+template <class ST, class VT>
+__global__ __launch_bounds__(256) void k_mm_far_t1(const MMItem* __restrict__ items, int nitems,
+                                                   const FarBlk* __restrict__ fb, const ST* __restrict__ fV,
+                                                   const VT* __restrict__ xm, VT* __restrict__ tm) {
+  const int w = __builtin_amdgcn_readfirstlane(blockIdx.x * 4 + (threadIdx.x >> 6));
+  if (w >= nitems) return;
+  const MMItem it = items[w];
+  const FarBlk b = fb[it.blk];
+  const int lane = threadIdx.x & 63;
+  const int i = lane & 15, g = lane >> 4;
+  const bool rok = it.l0 + i < b.R;
+  const ST* A = fV + b.voff + (int64_t)(rok ? it.l0 + i : 0) * b.n;  // row-major: column k at A[k]
+  const VT* X = xm + (int64_t)b.c0 * MM_W;
+  MAcc<VT> acc;
+  acc.zero();
+  for (int kk = 0; kk < b.n; kk += 16) mm_group<ST, VT>(acc, A, 1, rok, X, kk, b.n, g, i);
+#pragma unroll
+  for (int t = 0; t < 4; t++)
+#pragma unroll
+    for (int v = 0; v < 4; v++) {
+      const int r = MAcc<VT>::row(g, v);
+      if (it.l0 + r < b.R) tm[(int64_t)(b.toff + it.l0 + r) * MM_W + 16 * t + i] = acc.get(t, v);
+    }
+}
+
 // forward block apply over the row tiles of the single-vector plan: the tile's
 // rows go in 16-row strips to the waves; with fewer than three strips the
 // spare waves split the columns and are summed through LDS
@@ -5556,20 +5583,44 @@ static void apply_t(Operator& op, int flags, const XT* x, XT* y, XT alpha, XT be
   mark(5);
 }
 
+// low-rank blocks of at most this many 16-column groups take one wave per
+// 16 rank rows in the block products, wider ones four (NGBEM_MM_ONEWAVE,
+// 0: four waves for every block).  At the Maxwell EFIE workload (icosphere
+// L6, leafsize 40, 64 right-hand sides) the FP32 product took 10.77 / 10.63 /
+// 10.45 / 10.35 / 10.35 / 10.51 / 10.80 ms with 0 / 4 / 8 / 16 / 32 / 64 /
+// all groups, the FP64 one 22.14 / 21.33 / 21.01 / 20.88 ms with 0 / 8 / 16 /
+// 32: most of its far blocks are 41-90 columns wide, one or two groups per
+// wave when four waves share them, so the LDS sum and the barrier cost more
+// than the split gains
+static int mm_onewave() {
+  const char* e = std::getenv("NGBEM_MM_ONEWAVE");
+  return e ? std::max(0, std::atoi(e)) : 16;
+}
+
+// the low-rank work items of the block products, (block, first of 16 rank
+// rows): those of wide blocks (four waves each) first, then those of blocks of
+// at most mm_onewave() column groups (one wave each); built on first use
+static void mm_items(Operator& op, hipStream_t s) {
+  if (op.n_mm2_wide >= 0) return;
+  const int64_t nmax = 16 * (int64_t)mm_onewave();
+  std::vector<MMItem> items, narrow;
+  for (size_t k = 0; k < op.far.size(); k++)
+    for (int l = 0; l < op.far[k].R; l += 16) (op.far[k].n <= nmax ? narrow : items).push_back(MMItem{(int)k, l});
+  const int nw = (int)items.size();
+  items.insert(items.end(), narrow.begin(), narrow.end());
+  op.d_mm2_items.upload(items, s);
+  BEM_HIP(hipStreamSynchronize(s));
+  op.n_mm2_narrow = (int)narrow.size();
+  op.n_mm2_wide = nw;
+}
+
 // forward block apply, 64 right-hand sides per pass
 template <class ST, class VT, class XT>
 static void apply_mm_t(Operator& op, const XT* x, XT* y, int nrhs, int64_t ld, XT alpha, XT beta, bool beta_zero,
                        const ST* nval, const ST* fU, const ST* fV, hipStream_t s) {
   const int nin = op.ndof_trial(), nout = op.ndof_test();
   const size_t vsz = sizeof(VT);
-  if (op.n_mm_items < 0) {
-    std::vector<MMItem> items;
-    for (size_t k = 0; k < op.far.size(); k++)
-      for (int l = 0; l < op.far[k].R; l += 16) items.push_back(MMItem{(int)k, l});
-    op.d_mm_items.upload(items, s);
-    BEM_HIP(hipStreamSynchronize(s));
-    op.n_mm_items = (int)items.size();
-  }
+  mm_items(op, s);
   const size_t need_v = (size_t)std::max(nin, nout) * MM_W * vsz;
   const size_t need_t = (size_t)std::max(op.t_total, 1) * MM_W * vsz;
   if (op.d_xm.bytes < need_v || op.d_ym.bytes < need_v || op.d_tm.bytes < need_t)
@@ -5584,9 +5635,13 @@ static void apply_mm_t(Operator& op, const XT* x, XT* y, int nrhs, int64_t ld, X
     const int nr = std::min(MM_W, nrhs - r0);
     hipLaunchKernelGGL((k_gather_mm<VT, XT>), dim3((unsigned)(((int64_t)nin * MM_W + 255) / 256)), dim3(256), 0, s,
                        op.d_perm_trial.as<int>(), nin, x, ld, r0, nr, xm);
-    if (op.n_mm_items > 0)
-      hipLaunchKernelGGL((k_mm_far_t<ST, VT>), dim3(op.n_mm_items), dim3(256), 0, s, op.d_mm_items.as<MMItem>(),
+    if (op.n_mm2_wide > 0)
+      hipLaunchKernelGGL((k_mm_far_t<ST, VT>), dim3(op.n_mm2_wide), dim3(256), 0, s, op.d_mm2_items.as<MMItem>(),
                          op.d_far_blk.as<FarBlk>(), fV, xm, tm);
+    if (op.n_mm2_narrow > 0)
+      hipLaunchKernelGGL((k_mm_far_t1<ST, VT>), dim3((unsigned)((op.n_mm2_narrow + 3) / 4)), dim3(256), 0, s,
+                         op.d_mm2_items.as<MMItem>() + op.n_mm2_wide, op.n_mm2_narrow, op.d_far_blk.as<FarBlk>(), fV,
+                         xm, tm);
     hipLaunchKernelGGL((k_mm_rows<ST, VT>), dim3(op.n_tiles_fwd), dim3(256), 0, s, op.d_tile_fwd.as<Tile>(),
                        op.d_seg_fwd_mm.as<Seg>(), nval, fU, xm, tm, ym);
     hipLaunchKernelGGL((k_scatter_mm<XT, VT>), dim3((unsigned)(((int64_t)nout * MM_W + 255) / 256)), dim3(256), 0, s,
@@ -5598,19 +5653,6 @@ static void apply_mm_t(Operator& op, const XT* x, XT* y, int nrhs, int64_t ld, X
 // buffers of FP64 values cost more occupancy than they gain: 28.3 against
 // 22.0 ms at the Maxwell EFIE workload); NGBEM_MM=v1 keeps the four-product
 // kernels above for both
-// low-rank blocks of at most this many 16-column groups take one wave per
-// 16 rank rows in the slab block product, wider ones four (NGBEM_MM_ONEWAVE,
-// 0: four waves for every block).  At the Maxwell EFIE workload (icosphere
-// L6, leafsize 40, 64 right-hand sides) the product took 10.77 / 10.63 /
-// 10.45 / 10.35 / 10.35 / 10.51 / 10.80 ms with 0 / 4 / 8 / 16 / 32 / 64 /
-// all groups: most of its far blocks are 41-90 columns wide, one or two
-// groups per wave when four waves share them, so the LDS sum and the barrier
-// cost more than the split gains
-static int mm_onewave() {
-  const char* e = std::getenv("NGBEM_MM_ONEWAVE");
-  return e ? std::max(0, std::atoi(e)) : 16;
-}
-
 static bool mm_slab() {
   static const bool on = [] {
     const char* e = std::getenv("NGBEM_MM");
@@ -5625,19 +5667,7 @@ static void apply_mm2_t(Operator& op, const XT* x, XT* y, int nrhs, int64_t ld, 
   typedef typename SlabT<VT>::R R;
   constexpr int NP = SlabT<VT>::NP;
   const int nin = op.ndof_trial(), nout = op.ndof_test();
-  if (op.n_mm2_wide < 0) {
-    // items of wide blocks (four waves each) first, then those of blocks of at
-    // most mm_onewave() column groups (one wave each)
-    const int64_t nmax = 16 * (int64_t)mm_onewave();
-    std::vector<MMItem> items, narrow;
-    for (size_t k = 0; k < op.far.size(); k++)
-      for (int l = 0; l < op.far[k].R; l += 16) (op.far[k].n <= nmax ? narrow : items).push_back(MMItem{(int)k, l});
-    op.n_mm2_wide = (int)items.size();
-    op.n_mm2_narrow = (int)narrow.size();
-    items.insert(items.end(), narrow.begin(), narrow.end());
-    op.d_mm2_items.upload(items, s);
-    BEM_HIP(hipStreamSynchronize(s));
-  }
+  mm_items(op, s);
   const size_t need_x = (size_t)std::max(nin, 1) * MM_W * NP * sizeof(R);
   const size_t need_y = (size_t)std::max(nout, 1) * MM_W * sizeof(VT);
   const size_t need_t = (size_t)std::max(op.t_total, 1) * MM_W * NP * sizeof(R);

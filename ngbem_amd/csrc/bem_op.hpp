@@ -303,11 +303,10 @@ struct Operator {
   int t_total = 0;         // scratch length for far B^T x
   DBuf d_xp, d_yp, d_t;    // scratch vectors (complex128 or double)
   // block apply (64 right-hand sides): work items built on first use, slabs
-  DBuf d_mm_items, d_xm, d_ym, d_tm;
-  int n_mm_items = -1;
+  DBuf d_xm, d_ym, d_tm;
   DBuf d_mm_strips;  // (row tile, 16-row strip) of the slab-form block product
   int n_mm_strips = 0;
-  DBuf d_mm2_items;  // its low-rank items: those of wide blocks, then those of narrow ones
+  DBuf d_mm2_items;  // the block products' low-rank items: those of wide blocks, then narrow ones
   int n_mm2_wide = -1, n_mm2_narrow = 0;
 
   int ndof_trial() const { return trial->ndof; }
